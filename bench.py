@@ -1,0 +1,209 @@
+"""Gravity forward + adjoint throughput on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+
+Workload (pm_blobs_1m): 128x128x64 cells of 500 m (1,048,576 cells), 16,384
+stations on a 128x128 grid with 0-2,000 m topography, densities from 8 seeded
+Gaussian blobs + noise. One step = one FP32 forward g = G rho plus one FP32
+adjoint G^T r over the whole grid and station set (2 x 1.718e10 station-cell
+pairs), through libztm_gfm's plan API, inputs resident in HBM.
+
+Prints one JSON line: value = station x cell pairs per second (forward +
+adjoint), the FP64 GPU path's rate on the same step and the speedup over it,
+the FP32 path's max-normalised error against the FP64 GPU path, a roofline
+entry for the dominant kernel (timed with HIP events on its launch stream),
+and the CPU baseline: the FP64 restatement of the reference CPU gravity
+(TFM.cpp:95-99 arithmetic, pow(r^2,1.5)) timed on this host on a bounded
+sample, which also checks the GPU result on that sample.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "station×cell pairs/s fwd+adjoint, FP32/BF16 vs FP64 on GPU at ≤1e-5 rel err"
+FLOP_PER_PAIR = 13            # 3 sub, r^2 (1 mul + 2 fma), 2 mul r^-3, 1 mul rho*dz, 1 fma
+PEAK_FP32_TFLOPS = 157.3      # MI355X FP32 vector, published
+PEAK_HBM_GBS = 8000.0
+
+
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or "unknown"
+
+
+def relerr(a, b) -> float:
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    return float(np.max(np.abs(a - b)) / max(float(np.max(np.abs(b))), 1e-300))
+
+
+def cpu_baseline(w, r, g32, a32, n_st=256, n_cells=16384):
+    """Time the oracle's FP64 restatement of the reference CPU path on a sample:
+    the forward for n_st stations over all cells, the adjoint for n_cells cells
+    over all stations. Returns the baseline dict and the GPU errors on it."""
+    import oracle
+    threads = oracle.DEFAULT_THREADS
+    st_idx = np.unique((np.arange(n_st) * (w.n_stations / n_st)).astype(np.int64))
+    cell_idx = np.unique((np.arange(n_cells) * (w.grid.n_cells / n_cells)).astype(np.int64))
+    oracle.lib()
+    t0 = time.perf_counter()
+    g_cpu = oracle.pm_forward(w.grid, w.kappa, w.rho, w.stations[st_idx], threads=threads)
+    t1 = time.perf_counter()
+    a_cpu = oracle.pm_adjoint_cells(w.grid, w.kappa, w.stations, r, cell_idx, threads=threads)
+    t2 = time.perf_counter()
+    pairs = st_idx.size * w.grid.n_cells + cell_idx.size * w.n_stations
+    base = {
+        "value": pairs / (t2 - t0),
+        "unit": "pairs/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": (f"FP64 pow(r^2,1.5) point-mass sum restating TFM.cpp:95-99 on {threads} threads "
+                   f"({cpu_model()}, nproc={os.cpu_count()}): forward for {st_idx.size} of "
+                   f"{w.n_stations} stations over all {w.grid.n_cells} cells + adjoint for "
+                   f"{cell_idx.size} cells over all stations = {pairs:.3e} pairs in "
+                   f"{t2 - t0:.2f} s"),
+    }
+    errs = {"forward": relerr(g32[st_idx], g_cpu), "adjoint": relerr(a32[cell_idx], a_cpu)}
+    return base, errs
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--fp64-steps", type=int, default=2)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(sharding is described in DESIGN.md); run with --gpus 1", file=sys.stderr)
+        return 2
+
+    import torch
+    from ztm_amd import gfm, workloads as W
+
+    if not torch.cuda.is_available():
+        print("bench.py needs a GPU", file=sys.stderr)
+        return 1
+    dev = torch.device("cuda:0")
+    w = W.pm_blobs_1m()
+    r = W.normal(W.SEED0 + 2, 40, w.n_stations, 0.5)
+    pairs = w.pairs
+
+    plan = gfm.Plan(w.grid, w.stations, kappa=w.kappa)
+    rho32 = torch.from_numpy(w.rho.astype(np.float32)).to(dev)
+    r32 = torch.from_numpy(r.astype(np.float32)).to(dev)
+    rho64 = torch.from_numpy(w.rho).to(dev)
+    r64 = torch.from_numpy(r).to(dev)
+    g32 = torch.empty(w.n_stations, dtype=torch.float64, device=dev)
+    a32 = torch.empty(w.grid.n_cells, dtype=torch.float32, device=dev)
+    g64 = torch.empty(w.n_stations, dtype=torch.float64, device=dev)
+    a64 = torch.empty(w.grid.n_cells, dtype=torch.float64, device=dev)
+    stream = torch.cuda.current_stream()
+
+    def step32():
+        plan.forward(rho32, out=g32, stream=stream)
+        plan.adjoint(r32, out=a32, stream=stream)
+
+    def step64():
+        plan.forward(rho64, out=g64, stream=stream)
+        plan.adjoint(r64, out=a64, stream=stream)
+
+    for _ in range(args.warmup):
+        step32()
+    torch.cuda.synchronize()
+
+    gfm.profile_reset()
+    gfm.profile_enable(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step32()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    gfm.profile_enable(False)
+    fwd_ms, fwd_n = gfm.profile_read("pm_fwd")
+    adj_ms, adj_n = gfm.profile_read("pm_adj")
+    gfm.profile_reset()
+
+    ms_step = (t1 - t0) * 1e3 / args.steps
+    value = 2.0 * pairs * args.steps / (t1 - t0)
+
+    # FP64 GPU path, same step
+    step64()
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    for _ in range(args.fp64_steps):
+        step64()
+    torch.cuda.synchronize()
+    t3 = time.perf_counter()
+    value64 = 2.0 * pairs * args.fp64_steps / (t3 - t2)
+
+    g32h, a32h = g32.cpu().numpy(), a32.cpu().numpy()
+    g64h, a64h = g64.cpu().numpy(), a64.cpu().numpy()
+    err = {"forward": relerr(g32h, g64h), "adjoint": relerr(a32h, a64h)}
+
+    fwd_avg, adj_avg = fwd_ms / max(fwd_n, 1), adj_ms / max(adj_n, 1)
+    dom, dom_ms = ("pm_adj", adj_avg) if adj_avg >= fwd_avg else ("pm_fwd", fwd_avg)
+    achieved = FLOP_PER_PAIR * pairs / (dom_ms * 1e-3) / 1e12
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "pm_traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f).get(dom, {}).get("hbm_bytes_per_launch")
+
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "pairs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic (seeded Gaussian-blob densities, Gaussian-hill station heights)",
+        "config": {"workload": "pm_blobs_1m: FP32 forward + adjoint, 128x128x64 cells of 500 m "
+                               "(1,048,576) x 16,384 stations, point-mass cells",
+                   "cells": w.grid.n_cells, "stations": w.n_stations,
+                   "pairs_per_step": 2 * pairs},
+        "fp64_gpu": {"value": value64, "unit": "pairs/s", "steps": args.fp64_steps},
+        "speedup_vs_fp64_gpu": value / value64,
+        "rel_err_vs_fp64": err,
+        "kernels_ms": {"pm_fwd": fwd_avg, "pm_adj": adj_avg},
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
+                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS,
+                     "traffic": traffic, "kernel": dom,
+                     "flop_per_pair": FLOP_PER_PAIR, "pairs_per_launch": pairs},
+    }
+    if not args.no_cpu_baseline:
+        base, cerr = cpu_baseline(w, r, g32h, a32h)
+        line["cpu_baseline"] = base
+        line["rel_err_vs_oracle_sample"] = cerr
+        line["speedup_vs_cpu"] = value / base["value"]
+    plan.close()
+    print(json.dumps(line))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,167 @@
+/*
+ * ztm_gfm.h -- C ABI of the MI355X (gfx950) gravity forward/adjoint library.
+ *
+ * The library computes the vertical attraction g_z at every station as the sum,
+ * over all cells of a regular 3-D density-contrast grid, of each cell's
+ * attraction, and the transpose of that map (the adjoint G^T r).
+ *
+ * Reference interfaces replaced (abatchev/ZTM):
+ *   ztm_gf        <- kernel GF(int bfr,int di,int dj,int dk,int fh,int fv,
+ *                            float*g,float*rho,float*s)        GFM_GPU.cu:16-22,
+ *                    launched as GF<<<N/ii,ii>>>(...)          GFM_GPU.cu:31
+ *   ztm_gfm_host  <- the GFM program body (file buffers -> device -> kernel ->
+ *                    host buffers)                              GFM_GPU.cu:24-34
+ *   ztm_plan_*    <- the CPU gravity lambda GFM(j0) of FM::FM   TFM.cpp:95-99
+ *                    generalised to explicit geometry, SI units, FP32/FP64,
+ *                    the point-mass and the rectangular-prism cell formula,
+ *                    and its adjoint (no reference counterpart for the adjoint).
+ *
+ * Conventions
+ *   - All functions return ZTM_OK (0) or a positive ZTM_E* code; the detail of
+ *     the last failure on the calling thread is returned by ztm_last_error().
+ *   - Pointers named *_dev are device pointers (hipMalloc'd, or a torch CUDA
+ *     tensor's data_ptr()); pointers named *_host are host pointers.
+ *   - `stream` is a hipStream_t passed as void*; NULL means the null stream.
+ *     Every device entry point is stream-ordered and returns without waiting
+ *     for the GPU, like a kernel launch.
+ *   - Cell raster: n = i + nx*(j + ny*k), i fastest (GFM_GPU.cu:18 order).
+ *
+ * Cell formula (point mass, GFM_GPU.cu:22 / TFM.cpp:98):
+ *     g[m] = kappa * sum_n rho[n] * V * (z_m - z_n) / |x_m - x_n|^3
+ *   V = |dx*dy*dz| is the cell volume, lengths in the grid's length unit.
+ *   With SI lengths (m), rho in kg/m^3 and kappa = ZTM_KAPPA_SI the output is
+ *   in mGal. The reference's index-unit constant 66.7/(fh^2 fv) is kappa = 66.7
+ *   with V = 1/(fh^2 fv) (GFM_GPU.cu:17, TFM.cpp:190 h = 10 km).
+ * Cell formula (prism): V * (z_m - z_n)/r^3 is replaced by the exact volume
+ *   integral of (z_m - z)/r^3 over the cell (Plouff 1976; Nagy, Papp & Benedek
+ *   2000), evaluated with a station-local origin.
+ */
+#ifndef ZTM_GFM_H
+#define ZTM_GFM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ZTM_GFM_VERSION 100
+
+/* status codes */
+#define ZTM_OK      0
+#define ZTM_EINVAL  1   /* bad argument (sizes, null pointers, flags)        */
+#define ZTM_EHIP    2   /* a HIP runtime call or kernel launch failed          */
+#define ZTM_ENOMEM  3   /* device or host allocation failed                    */
+#define ZTM_ENODEV  4   /* no usable gfx950 device                             */
+#define ZTM_EIO     5   /* file missing, short or oversized (GFM program)      */
+
+/* precisions */
+#define ZTM_FP32    0   /* FP32 bulk, FP64 per-row/per-tile and final sums     */
+#define ZTM_FP64    1   /* FP64 everywhere                                     */
+
+/* cell formulas */
+#define ZTM_POINT_MASS 0
+#define ZTM_PRISM      1
+
+/* 1e5 (m/s^2 -> mGal) * G (6.674e-11 m^3 kg^-1 s^-2) */
+#define ZTM_KAPPA_SI   6.674e-6
+/* the reference's constant: G * 1e3 (g/cc -> kg/m^3) * 1e4 (h = 10 km) * 1e5 */
+#define ZTM_KAPPA_REF  66.7
+
+typedef struct ztm_grid {
+    int32_t nx, ny, nz;   /* cells along x (fastest), y, z                    */
+    double  x0, y0, z0;   /* centre of cell (0,0,0)                            */
+    double  dx, dy, dz;   /* signed centre-to-centre spacing (= cell edges)    */
+} ztm_grid;
+
+typedef struct ztm_problem {
+    ztm_grid grid;
+    int32_t  formula;     /* ZTM_POINT_MASS or ZTM_PRISM                       */
+    double   kappa;       /* output scale, e.g. ZTM_KAPPA_SI                   */
+    double   threshold;   /* forward skips cells with |rho| <= threshold       */
+} ztm_problem;
+
+typedef struct ztm_plan_s* ztm_plan;
+
+/* ---------------------------------------------------------------- library */
+int         ztm_version(void);
+const char* ztm_strerror(int code);
+const char* ztm_last_error(void);
+/* Initialise the current HIP device (checks it is gfx950). Optional: every
+ * entry point initialises lazily. */
+int ztm_init(void);
+
+/* ------------------------------------------- Level 1: reference kernel ABI */
+/* Replaces GF (GFM_GPU.cu:16-22) and its launch (GFM_GPU.cu:31).
+ * g_dev:   float[di*dj]  accumulated into (+=) at interior stations
+ *          bfr <= ni < di-bfr, bfr <= nj < dj-bfr, m = ni + nj*di; the others
+ *          are left untouched (the caller zeroes g, GFM_GPU.cu:27,29).
+ * rho_dev: float[N], N = ii*jj*kk with ii=(di-1)*fh+1, jj=(dj-1)*fh+1,
+ *          kk=(dk-1)*fv+1 (the producer's raster, TFM.cpp:83); cells with
+ *          |rho| <= .001f are skipped (GFM_GPU.cu:19).
+ * s_dev:   float[di*dj]  station heights k0 in k-index units (GFM_GPU.cu:21).
+ * Cell n sits at (i/fh, j/fh, k/fv) and station m at (ni, nj, s[m]) in index
+ * units; the constant is 66.7/(fh*fh*fv) (GFM_GPU.cu:17).
+ * Unlike the reference kernel, dk is used: it fixes N, so a rho buffer of the
+ * wrong size cannot be read past its end. ztm_gf runs on the null stream. */
+int ztm_gf(int bfr, int di, int dj, int dk, int fh, int fv,
+           float* g_dev, const float* rho_dev, const float* s_dev);
+int ztm_gf_async(int bfr, int di, int dj, int dk, int fh, int fv,
+                 float* g_dev, const float* rho_dev, const float* s_dev,
+                 void* stream);
+
+/* ---------------------------------------- Level 2: GFM program, host side */
+/* Replaces the body of the GFM program (GFM_GPU.cu:24-34): host float
+ * buffers in, g_host[di*dj] out (exterior stations 0, interior = the sum),
+ * synchronous. n_rho must equal ii*jj*kk for the given dk (checked). */
+int ztm_gfm_host(int bfr, int di, int dj, int dk, int fh, int fv,
+                 float* g_host, const float* rho_host, int64_t n_rho,
+                 const float* s_host);
+
+/* ------------------------------------------------ general-geometry plans */
+/* A plan holds one grid + one station set, prepared on the device once and
+ * reused by every forward/adjoint call (the geometry is fixed through an
+ * inversion, TFM.cpp:124-135). stations_host: double[n_stations][3] (x,y,z)
+ * in the grid's length unit. Plans are bound to the device current at create
+ * time; calls on one plan from several host threads must be serialised. */
+int ztm_plan_create(ztm_plan* plan, const ztm_problem* problem,
+                    int64_t n_stations, const double* stations_host);
+int ztm_plan_destroy(ztm_plan plan);
+int64_t ztm_plan_num_cells(ztm_plan plan);
+int64_t ztm_plan_num_stations(ztm_plan plan);
+
+/* Forward: g_dev[m] = kappa * sum_n rho[n] K(m,n)   (FP64 output)
+ * precision ZTM_FP32: rho_dev is float[N];  ZTM_FP64: rho_dev is double[N]. */
+int ztm_plan_forward(ztm_plan plan, int precision, const void* rho_dev,
+                     double* g_dev, void* stream);
+
+/* Adjoint: out_dev[n] = kappa * sum_m K(m,n) r[m]   (threshold not applied)
+ * precision ZTM_FP32: r_dev float[M], out_dev float[N];
+ *           ZTM_FP64: r_dev double[M], out_dev double[N]. */
+int ztm_plan_adjoint(ztm_plan plan, int precision, const void* r_dev,
+                     void* out_dev, void* stream);
+
+/* Tuning of the point-mass kernels (0 = keep the plan's current value).
+ * st_per_lane: stations held in registers per lane of the forward kernel
+ *              (1, 2, 4 or 8); rows_per_slice: grid rows (fixed j,k) staged in
+ *              LDS per forward workgroup; cells_per_lane: cells per lane of the
+ *              adjoint kernel (4, 8 or 16); adj_splits: station slices of the
+ *              adjoint (split-K over stations). */
+int ztm_plan_set_tuning(ztm_plan plan, int st_per_lane, int rows_per_slice,
+                        int cells_per_lane, int adj_splits);
+int ztm_plan_get_tuning(ztm_plan plan, int* st_per_lane, int* rows_per_slice,
+                        int* cells_per_lane, int* adj_splits);
+
+/* ------------------------------------------------------ kernel timing */
+/* When enabled, HIP events are recorded on the launch stream around the
+ * dominant kernel of every entry point; ztm_profile_read() waits for them and
+ * returns the summed duration (ms) and count for `kernel` ("pm_fwd", "pm_adj",
+ * ...) since the last reset. */
+int ztm_profile_enable(int on);
+int ztm_profile_read(const char* kernel, double* total_ms, int64_t* count);
+int ztm_profile_reset(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ZTM_GFM_H */

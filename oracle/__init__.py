@@ -1,0 +1,161 @@
+"""FP64 CPU oracle for the gravity forward/adjoint path -- TEST INFRASTRUCTURE.
+
+Only tests/, ``__graft_entry__.smoke()`` and bench.py's CPU-baseline leg import
+this package; it is the checker, never the thing measured or shipped. The
+product path (ztm_amd, libztm_gfm.so, GFM) does not import it and fails loudly
+when its HIP library is missing.
+
+The arithmetic lives in ``gfm_oracle.c`` (see its header for the reference
+file:line each function restates and how each is pinned); this module only
+loads ``liboracle.so`` (built by ``make -C oracle``) and wraps it for numpy.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+# nproc on the GPU host can report far more cores than the share a run gets.
+DEFAULT_THREADS = max(1, min(16, os.cpu_count() or 1))
+
+
+class OGrid(ctypes.Structure):
+    _fields_ = [("nx", ctypes.c_int32), ("ny", ctypes.c_int32), ("nz", ctypes.c_int32),
+                ("x0", ctypes.c_double), ("y0", ctypes.c_double), ("z0", ctypes.c_double),
+                ("dx", ctypes.c_double), ("dy", ctypes.c_double), ("dz", ctypes.c_double)]
+
+
+def build() -> str:
+    """Compile liboracle.so with the committed Makefile."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+    return _LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = ctypes.CDLL(_LIB_PATH)
+        P = ctypes.c_void_p
+        i32, i64, f64 = ctypes.c_int, ctypes.c_int64, ctypes.c_double
+        gp = ctypes.POINTER(OGrid)
+        L.oracle_gf.argtypes = [i32, i32, i32, i32, i32, i32, P, P, P, i32]
+        L.oracle_tfm_gravity.argtypes = [i32, i32, i32, i32, i32, i32, P, P, P, i64, P, i32]
+        L.oracle_tfm_gravity.restype = i32
+        L.oracle_pm_forward.argtypes = [gp, f64, f64, P, i64, P, P, i32]
+        L.oracle_pm_adjoint.argtypes = [gp, f64, i64, P, P, P, i32]
+        L.oracle_pm_forward_cells.argtypes = [gp, f64, P, P, i64, i64, P, P, i32]
+        L.oracle_pm_adjoint_cells.argtypes = [gp, f64, i64, P, P, P, i64, P, i32]
+        L.oracle_prism_forward.argtypes = [gp, f64, f64, P, i64, P, P, i32]
+        L.oracle_prism_adjoint.argtypes = [gp, f64, i64, P, P, P, i32]
+        L.oracle_prism_kernel.argtypes = [f64] * 9
+        L.oracle_prism_kernel.restype = f64
+        _lib = L
+    return _lib
+
+
+def _ptr(a: np.ndarray) -> ctypes.c_void_p:
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+def _grid(g) -> OGrid:
+    return OGrid(g.nx, g.ny, g.nz, g.x0, g.y0, g.z0, g.dx, g.dy, g.dz)
+
+
+def gf(bfr, di, dj, dk, fh, fv, rho, s, g=None, threads=DEFAULT_THREADS) -> np.ndarray:
+    """GFM_GPU.cu:16-22 in FP64 on float32 inputs; returns g[di*dj] (accumulated into g)."""
+    rho = np.ascontiguousarray(rho, dtype=np.float32)
+    s = np.ascontiguousarray(s, dtype=np.float32)
+    n = ((di - 1) * fh + 1) * ((dj - 1) * fh + 1) * ((dk - 1) * fv + 1)
+    if rho.size != n or s.size != di * dj:
+        raise ValueError(f"rho needs {n} values and s {di * dj}")
+    out = np.zeros(di * dj) if g is None else np.ascontiguousarray(g, dtype=np.float64)
+    lib().oracle_gf(bfr, di, dj, dk, fh, fv, _ptr(rho), _ptr(s), _ptr(out), threads)
+    return out
+
+
+def tfm_gravity(bfr, di, dj, dk, fh, fv, drho, surf, nn, G=None, threads=0) -> np.ndarray:
+    """TFM.cpp:95-99 CPU gravity (threads=0: one thread per station row)."""
+    drho = np.ascontiguousarray(drho, dtype=np.float64)
+    surf = np.ascontiguousarray(surf, dtype=np.float64)
+    nn = np.ascontiguousarray(nn, dtype=np.int32)
+    out = np.zeros(di * dj) if G is None else np.ascontiguousarray(G, dtype=np.float64)
+    lib().oracle_tfm_gravity(bfr, di, dj, dk, fh, fv, _ptr(drho), _ptr(surf), _ptr(nn),
+                             nn.size, _ptr(out), threads)
+    return out
+
+
+def pm_forward(grid, kappa, rho, xyz, threshold=0.0, threads=DEFAULT_THREADS) -> np.ndarray:
+    rho = np.ascontiguousarray(rho, dtype=np.float64).ravel()
+    xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
+    out = np.empty(xyz.shape[0])
+    g = _grid(grid)
+    lib().oracle_pm_forward(ctypes.byref(g), kappa, threshold, _ptr(rho), xyz.shape[0],
+                            _ptr(xyz), _ptr(out), threads)
+    return out
+
+
+def pm_forward_cells(grid, kappa, rho, cells, xyz, threads=DEFAULT_THREADS) -> np.ndarray:
+    """Forward restricted to the listed cells (bounded CPU-baseline sample)."""
+    rho = np.ascontiguousarray(rho, dtype=np.float64).ravel()
+    cells = np.ascontiguousarray(cells, dtype=np.int64)
+    xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
+    out = np.empty(xyz.shape[0])
+    g = _grid(grid)
+    lib().oracle_pm_forward_cells(ctypes.byref(g), kappa, _ptr(rho), _ptr(cells), cells.size,
+                                  xyz.shape[0], _ptr(xyz), _ptr(out), threads)
+    return out
+
+
+def pm_adjoint(grid, kappa, xyz, r, threads=DEFAULT_THREADS) -> np.ndarray:
+    xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
+    r = np.ascontiguousarray(r, dtype=np.float64).ravel()
+    out = np.empty(grid.nx * grid.ny * grid.nz)
+    g = _grid(grid)
+    lib().oracle_pm_adjoint(ctypes.byref(g), kappa, xyz.shape[0], _ptr(xyz), _ptr(r), _ptr(out),
+                            threads)
+    return out
+
+
+def pm_adjoint_cells(grid, kappa, xyz, r, cells, threads=DEFAULT_THREADS) -> np.ndarray:
+    """Adjoint restricted to the listed cells (bounded CPU-baseline sample)."""
+    xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
+    r = np.ascontiguousarray(r, dtype=np.float64).ravel()
+    cells = np.ascontiguousarray(cells, dtype=np.int64)
+    out = np.empty(cells.size)
+    g = _grid(grid)
+    lib().oracle_pm_adjoint_cells(ctypes.byref(g), kappa, xyz.shape[0], _ptr(xyz), _ptr(r),
+                                  _ptr(cells), cells.size, _ptr(out), threads)
+    return out
+
+
+def prism_forward(grid, kappa, rho, xyz, threshold=0.0, threads=DEFAULT_THREADS) -> np.ndarray:
+    rho = np.ascontiguousarray(rho, dtype=np.float64).ravel()
+    xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
+    out = np.empty(xyz.shape[0])
+    g = _grid(grid)
+    lib().oracle_prism_forward(ctypes.byref(g), kappa, threshold, _ptr(rho), xyz.shape[0],
+                               _ptr(xyz), _ptr(out), threads)
+    return out
+
+
+def prism_adjoint(grid, kappa, xyz, r, threads=DEFAULT_THREADS) -> np.ndarray:
+    xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
+    r = np.ascontiguousarray(r, dtype=np.float64).ravel()
+    out = np.empty(grid.nx * grid.ny * grid.nz)
+    g = _grid(grid)
+    lib().oracle_prism_adjoint(ctypes.byref(g), kappa, xyz.shape[0], _ptr(xyz), _ptr(r),
+                               _ptr(out), threads)
+    return out
+
+
+def prism_kernel(xs, ys, zs, x1, x2, y1, y2, z1, z2) -> float:
+    """Integral of (zs - z)/r^3 over [x1,x2]x[y1,y2]x[z1,z2]."""
+    return lib().oracle_prism_kernel(xs, ys, zs, x1, x2, y1, y2, z1, z2)

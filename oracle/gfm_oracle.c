@@ -1,0 +1,376 @@
+/*
+ * gfm_oracle.c -- FP64 CPU oracle for the gravity forward/adjoint path.
+ *
+ * TEST INFRASTRUCTURE ONLY. This file is the checker: only tests/, the smoke
+ * check and bench.py's CPU-baseline leg load it. The product (libztm_gfm.so,
+ * the GFM program, the ztm_amd package) never links, loads or calls it.
+ *
+ * What each function restates (abatchev/ZTM):
+ *   oracle_gf            GFM_GPU.cu:16-22 -- kernel GF on its own float
+ *                        inputs (raster decode :17-18, |rho|>.001f skip :19,
+ *                        interior-station loop :20-21, point-mass term :22),
+ *                        evaluated in FP64 with pow(r^2,1.5); the raster width
+ *                        is the producer's ii=(di-1)*fh+1 (TFM.cpp:83,
+ *                        GFM_GPU.cu:24), not the kernel's di*fh (:17), which
+ *                        agree for the only fh the reference uses (fh=1).
+ *   oracle_tfm_gravity   TFM.cpp:95-99 -- the CPU gravity lambda GFM(j0):
+ *                        FP64 drho/surf, the sparse cell list nn, G[m]=0 then
+ *                        += (66.7*drho*(k0-k)/pow(r2,1.5))/(fh*fh*fv), one
+ *                        std::thread per interior station row. Used as the
+ *                        timed CPU baseline ("port" of the reference path).
+ *   oracle_pm_forward    the same point-mass sum in explicit geometry:
+ *                        g[m] = kappa * sum_n [|rho_n|>thr] rho_n V dz/r^3.
+ *   oracle_pm_adjoint    its transpose, looped cells-outer (an independent
+ *                        loop nest from the forward, so a transposed index
+ *                        fails the dot-product test).
+ *   oracle_prism_*       right rectangular prism, exact volume integral of
+ *                        (z_m - z)/r^3 (Plouff 1976; Nagy, Papp & Benedek
+ *                        2000, eq. for g_z), no reference counterpart.
+ *
+ * Deliberate deviation: a station exactly at a cell centre (r = 0) gives
+ * 0*inf = NaN in the reference (GFM_GPU.cu:22, TFM.cpp:98); here that pair
+ * contributes 0, as in the HIP kernels.
+ *
+ * How each function is pinned (tests/test_oracle.py): single-cell closed
+ * forms, the 66.7 unit derivation, the solid-angle formula of a thin square
+ * layer, mirror symmetry, brute force on tiny grids, oracle_gf ==
+ * oracle_tfm_gravity on shared inputs, the dot-product identity and central
+ * finite differences for the adjoint; the prism against Gauss-Legendre
+ * quadrature of the point-mass kernel, its far-field point-mass limit,
+ * 8-subprism additivity and the infinite-slab limit 2*pi*G*rho*t.
+ */
+#include <math.h>
+#include <pthread.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+typedef struct og_grid {
+    int32_t nx, ny, nz;
+    double x0, y0, z0;
+    double dx, dy, dz;
+} og_grid;
+
+/* ------------------------------------------------------------ threading */
+typedef void (*range_fn)(void* ctx, int64_t lo, int64_t hi);
+typedef struct { range_fn fn; void* ctx; int64_t lo, hi; } range_job;
+
+static void* range_tramp(void* a) {
+    range_job* j = (range_job*)a;
+    j->fn(j->ctx, j->lo, j->hi);
+    return NULL;
+}
+
+/* Split [0,n) into nthreads contiguous chunks and run fn on each. */
+static void parallel_for(int64_t n, int nthreads, range_fn fn, void* ctx) {
+    if (nthreads <= 1 || n < 2) { fn(ctx, 0, n); return; }
+    if (nthreads > n) nthreads = (int)n;
+    pthread_t* th = (pthread_t*)malloc(sizeof(pthread_t) * nthreads);
+    range_job* jb = (range_job*)malloc(sizeof(range_job) * nthreads);
+    for (int t = 0; t < nthreads; ++t) {
+        jb[t].fn = fn; jb[t].ctx = ctx;
+        jb[t].lo = n * t / nthreads; jb[t].hi = n * (t + 1) / nthreads;
+        pthread_create(&th[t], NULL, range_tramp, &jb[t]);
+    }
+    for (int t = 0; t < nthreads; ++t) pthread_join(th[t], NULL);
+    free(th); free(jb);
+}
+
+/* ---------------------------------------------------- oracle_gf (GF kernel) */
+typedef struct {
+    int bfr, di, dj, fh, fv, ii, jj, kk;
+    const float* rho; const float* s; double* g;
+} gf_ctx;
+
+static void gf_rows(void* p, int64_t lo, int64_t hi) {
+    const gf_ctx* c = (const gf_ctx*)p;
+    const int64_t N = (int64_t)c->ii * c->jj * c->kk;
+    const double cst = 66.7 / ((double)c->fh * c->fh * c->fv);
+    const int ni_count = c->di - 2 * c->bfr;
+    for (int64_t t = lo; t < hi; ++t) {
+        const int ni = c->bfr + (int)(t % ni_count), nj = c->bfr + (int)(t / ni_count);
+        const int m = ni + nj * c->di;
+        const double i0 = ni, j0 = nj, k0 = c->s[m];
+        double acc = 0.0;
+        for (int64_t n = 0; n < N; ++n) {
+            const float rf = c->rho[n];
+            if (!(fabsf(rf) > .001f)) continue;                 /* GFM_GPU.cu:19 */
+            const double i = (double)(n % c->ii) / c->fh;
+            const double j = (double)((n / c->ii) % c->jj) / c->fh;
+            const double k = (double)(n / ((int64_t)c->ii * c->jj)) / c->fv;
+            const double r2 = (i0 - i) * (i0 - i) + (j0 - j) * (j0 - j) + (k0 - k) * (k0 - k);
+            if (r2 == 0.0) continue;
+            acc += cst * (double)rf * (k0 - k) / pow(r2, 1.5); /* GFM_GPU.cu:22 */
+        }
+        c->g[m] += acc;
+    }
+}
+
+/* g[di*dj] is accumulated into at interior stations, like the kernel. */
+void oracle_gf(int bfr, int di, int dj, int dk, int fh, int fv, const float* rho,
+               const float* s, double* g, int nthreads) {
+    gf_ctx c = {bfr, di, dj, fh, fv, (di - 1) * fh + 1, (dj - 1) * fh + 1, (dk - 1) * fv + 1,
+                rho, s, g};
+    const int ni = di - 2 * bfr, nj = dj - 2 * bfr;
+    if (ni <= 0 || nj <= 0) return;
+    parallel_for((int64_t)ni * nj, nthreads, gf_rows, &c);
+}
+
+/* ------------------------------------------ oracle_tfm_gravity (CPU path) */
+typedef struct {
+    int bfr, di, fh, fv, ii, jj;
+    const double* drho; const double* surf; const int32_t* nn; int64_t n_nn; double* G;
+} tfm_ctx;
+
+/* One interior station row j0 (TFM.cpp:96-98). */
+static void tfm_row(const tfm_ctx* c, int j0) {
+    for (int i0 = c->bfr; i0 < c->di - c->bfr; i0 += 1) {
+        const int m = i0 + j0 * c->di;
+        const double k0 = c->surf[m];
+        c->G[m] = 0;
+        for (int64_t q = 0; q < c->n_nn; ++q) {
+            const int n = c->nn[q];
+            const double i = (n % c->ii) / (1. * c->fh), j = ((n / c->ii) % c->jj) / (1. * c->fh),
+                         k = (n / (c->ii * c->jj)) / (1. * c->fv);
+            const double r2 = (i0 - i) * (i0 - i) + (j0 - j) * (j0 - j) + (k0 - k) * (k0 - k);
+            if (r2 == 0.0) continue;
+            c->G[m] += (66.7 * c->drho[n] * (k0 - k) / pow(r2, 1.5)) / (c->fh * c->fh * c->fv);
+        }
+    }
+}
+
+typedef struct { const tfm_ctx* c; int j0; } tfm_job;
+static void* tfm_tramp(void* a) { tfm_job* j = (tfm_job*)a; tfm_row(j->c, j->j0); return NULL; }
+static void tfm_range(void* p, int64_t lo, int64_t hi) {
+    const tfm_ctx* c = (const tfm_ctx*)p;
+    for (int64_t r = lo; r < hi; ++r) tfm_row(c, c->bfr + (int)r);
+}
+
+/* nthreads <= 0: one thread per interior station row, as TFM.cpp:99;
+ * otherwise rows are split over nthreads threads. Returns rows processed. */
+int oracle_tfm_gravity(int bfr, int di, int dj, int dk, int fh, int fv, const double* drho,
+                       const double* surf, const int32_t* nn, int64_t n_nn, double* G,
+                       int nthreads) {
+    (void)dk;
+    tfm_ctx c = {bfr, di, fh, fv, (di - 1) * fh + 1, (dj - 1) * fh + 1, drho, surf, nn, n_nn, G};
+    const int rows = dj - 2 * bfr;
+    if (rows <= 0 || di - 2 * bfr <= 0) return 0;
+    if (nthreads <= 0) {
+        pthread_t* th = (pthread_t*)malloc(sizeof(pthread_t) * rows);
+        tfm_job* jb = (tfm_job*)malloc(sizeof(tfm_job) * rows);
+        for (int j = 0; j < rows; ++j) {
+            jb[j].c = &c; jb[j].j0 = j + bfr;
+            pthread_create(&th[j], NULL, tfm_tramp, &jb[j]);
+        }
+        for (int j = 0; j < rows; ++j) pthread_join(th[j], NULL);
+        free(th); free(jb);
+    } else {
+        parallel_for(rows, nthreads, tfm_range, &c);
+    }
+    return rows;
+}
+
+/* ------------------------------------------------- general point mass */
+static inline void cell_centre(const og_grid* g, int64_t n, double* x, double* y, double* z) {
+    const int64_t i = n % g->nx, j = (n / g->nx) % g->ny, k = n / ((int64_t)g->nx * g->ny);
+    *x = g->x0 + i * g->dx; *y = g->y0 + j * g->dy; *z = g->z0 + k * g->dz;
+}
+
+typedef struct {
+    const og_grid* g; double kappa, thr; const double* rho; const double* xyz;
+    const double* r; double* out; int64_t m;
+} pm_ctx;
+
+static void pm_fwd_range(void* p, int64_t lo, int64_t hi) {
+    const pm_ctx* c = (const pm_ctx*)p;
+    const og_grid* g = c->g;
+    const int64_t N = (int64_t)g->nx * g->ny * g->nz;
+    const double V = fabs(g->dx * g->dy * g->dz);
+    for (int64_t m = lo; m < hi; ++m) {
+        const double xs = c->xyz[3 * m], ys = c->xyz[3 * m + 1], zs = c->xyz[3 * m + 2];
+        double acc = 0.0;
+        for (int64_t n = 0; n < N; ++n) {
+            const double rho = c->rho[n];
+            if (!(fabs(rho) > c->thr)) continue;
+            double x, y, z;
+            cell_centre(g, n, &x, &y, &z);
+            const double r2 = (xs - x) * (xs - x) + (ys - y) * (ys - y) + (zs - z) * (zs - z);
+            if (r2 == 0.0) continue;
+            acc += rho * V * (zs - z) / pow(r2, 1.5);
+        }
+        c->out[m] = c->kappa * acc;
+    }
+}
+
+void oracle_pm_forward(const og_grid* g, double kappa, double thr, const double* rho,
+                       int64_t m, const double* xyz, double* out, int nthreads) {
+    pm_ctx c = {g, kappa, thr, rho, xyz, NULL, out, m};
+    parallel_for(m, nthreads, pm_fwd_range, &c);
+}
+
+static void pm_adj_range(void* p, int64_t lo, int64_t hi) {
+    const pm_ctx* c = (const pm_ctx*)p;
+    const og_grid* g = c->g;
+    const double V = fabs(g->dx * g->dy * g->dz);
+    for (int64_t n = lo; n < hi; ++n) {
+        double x, y, z;
+        cell_centre(g, n, &x, &y, &z);
+        double acc = 0.0;
+        for (int64_t m = 0; m < c->m; ++m) {
+            const double dx = c->xyz[3 * m] - x, dy = c->xyz[3 * m + 1] - y,
+                         dz = c->xyz[3 * m + 2] - z;
+            const double r2 = dx * dx + dy * dy + dz * dz;
+            if (r2 == 0.0) continue;
+            acc += c->r[m] * V * dz / pow(r2, 1.5);
+        }
+        c->out[n] = c->kappa * acc;
+    }
+}
+
+void oracle_pm_adjoint(const og_grid* g, double kappa, int64_t m, const double* xyz,
+                       const double* r, double* out, int nthreads) {
+    pm_ctx c = {g, kappa, 0.0, NULL, xyz, r, out, m};
+    parallel_for((int64_t)g->nx * g->ny * g->nz, nthreads, pm_adj_range, &c);
+}
+
+/* Adjoint restricted to the listed cells: out[q] for cell cells[q]. */
+typedef struct {
+    const og_grid* g; double kappa; const int64_t* cells; const double* xyz; const double* r;
+    double* out; int64_t m;
+} pm_adj_sub_ctx;
+
+static void pm_adj_sub_range(void* p, int64_t lo, int64_t hi) {
+    const pm_adj_sub_ctx* c = (const pm_adj_sub_ctx*)p;
+    const double V = fabs(c->g->dx * c->g->dy * c->g->dz);
+    for (int64_t q = lo; q < hi; ++q) {
+        double x, y, z;
+        cell_centre(c->g, c->cells[q], &x, &y, &z);
+        double acc = 0.0;
+        for (int64_t m = 0; m < c->m; ++m) {
+            const double dx = c->xyz[3 * m] - x, dy = c->xyz[3 * m + 1] - y,
+                         dz = c->xyz[3 * m + 2] - z;
+            const double r2 = dx * dx + dy * dy + dz * dz;
+            if (r2 == 0.0) continue;
+            acc += c->r[m] * V * dz / pow(r2, 1.5);
+        }
+        c->out[q] = c->kappa * acc;
+    }
+}
+
+void oracle_pm_adjoint_cells(const og_grid* g, double kappa, int64_t m, const double* xyz,
+                             const double* r, const int64_t* cells, int64_t nc, double* out,
+                             int nthreads) {
+    pm_adj_sub_ctx c = {g, kappa, cells, xyz, r, out, m};
+    parallel_for(nc, nthreads, pm_adj_sub_range, &c);
+}
+
+/* Forward over a subset of cells (cells[q], q < nc) for all stations: the
+ * unit of the CPU baseline on a bounded sample. */
+typedef struct {
+    const og_grid* g; double kappa; const double* rho; const int64_t* cells; int64_t nc;
+    const double* xyz; double* out;
+} pm_sub_ctx;
+
+static void pm_sub_range(void* p, int64_t lo, int64_t hi) {
+    const pm_sub_ctx* c = (const pm_sub_ctx*)p;
+    const double V = fabs(c->g->dx * c->g->dy * c->g->dz);
+    for (int64_t m = lo; m < hi; ++m) {
+        const double xs = c->xyz[3 * m], ys = c->xyz[3 * m + 1], zs = c->xyz[3 * m + 2];
+        double acc = 0.0;
+        for (int64_t q = 0; q < c->nc; ++q) {
+            double x, y, z;
+            cell_centre(c->g, c->cells[q], &x, &y, &z);
+            const double r2 = (xs - x) * (xs - x) + (ys - y) * (ys - y) + (zs - z) * (zs - z);
+            if (r2 == 0.0) continue;
+            acc += c->rho[c->cells[q]] * V * (zs - z) / pow(r2, 1.5);
+        }
+        c->out[m] = c->kappa * acc;
+    }
+}
+
+void oracle_pm_forward_cells(const og_grid* g, double kappa, const double* rho,
+                             const int64_t* cells, int64_t nc, int64_t m, const double* xyz,
+                             double* out, int nthreads) {
+    pm_sub_ctx c = {g, kappa, rho, cells, nc, xyz, out};
+    parallel_for(m, nthreads, pm_sub_range, &c);
+}
+
+/* ------------------------------------------------------------ prism */
+/* Antiderivative of the g_z kernel over a box in station-relative
+ * coordinates (u,v,w) = (x,y,z) - station:
+ *   F = u ln(v + r) + v ln(u + r) - w atan(u v / (w r))
+ * with the principal-value arctangent (an atan2 would jump by pi across
+ * w r < 0). The 8-corner alternating sum sum (-1)^(a+b+c) F(u_a,v_b,w_c)
+ * equals the integral of w/r^3 over the box (Nagy et al. 2000), so the
+ * integral of (z_m - z)/r^3 = -w/r^3 is minus that sum.
+ * Terms whose prefactor is exactly 0 are dropped (their log may diverge). */
+static double prism_F(double u, double v, double w) {
+    const double r = sqrt(u * u + v * v + w * w);
+    double f = 0.0;
+    if (u != 0.0) { const double a = v + r; f += u * log(a > 0.0 ? a : 1e-300); }
+    if (v != 0.0) { const double a = u + r; f += v * log(a > 0.0 ? a : 1e-300); }
+    if (w != 0.0) f -= w * atan(u * v / (w * r));
+    return f;
+}
+
+/* Integral over the box of (z_s - z)/r^3 for station (xs,ys,zs). */
+double oracle_prism_kernel(double xs, double ys, double zs, double x1, double x2, double y1,
+                           double y2, double z1, double z2) {
+    const double u[2] = {x1 - xs, x2 - xs}, v[2] = {y1 - ys, y2 - ys}, w[2] = {z1 - zs, z2 - zs};
+    double s = 0.0;
+    for (int a = 0; a < 2; ++a)
+        for (int b = 0; b < 2; ++b)
+            for (int c = 0; c < 2; ++c) {
+                const double sg = ((a + b + c) & 1) ? -1.0 : 1.0;
+                s += sg * prism_F(u[a], v[b], w[c]);
+            }
+    return -s;
+}
+
+static void prism_fwd_range(void* p, int64_t lo, int64_t hi) {
+    const pm_ctx* c = (const pm_ctx*)p;
+    const og_grid* g = c->g;
+    const int64_t N = (int64_t)g->nx * g->ny * g->nz;
+    const double hx = 0.5 * fabs(g->dx), hy = 0.5 * fabs(g->dy), hz = 0.5 * fabs(g->dz);
+    for (int64_t m = lo; m < hi; ++m) {
+        const double xs = c->xyz[3 * m], ys = c->xyz[3 * m + 1], zs = c->xyz[3 * m + 2];
+        double acc = 0.0;
+        for (int64_t n = 0; n < N; ++n) {
+            const double rho = c->rho[n];
+            if (!(fabs(rho) > c->thr)) continue;
+            double x, y, z;
+            cell_centre(g, n, &x, &y, &z);
+            acc += rho * oracle_prism_kernel(xs, ys, zs, x - hx, x + hx, y - hy, y + hy, z - hz,
+                                             z + hz);
+        }
+        c->out[m] = c->kappa * acc;
+    }
+}
+
+void oracle_prism_forward(const og_grid* g, double kappa, double thr, const double* rho,
+                          int64_t m, const double* xyz, double* out, int nthreads) {
+    pm_ctx c = {g, kappa, thr, rho, xyz, NULL, out, m};
+    parallel_for(m, nthreads, prism_fwd_range, &c);
+}
+
+static void prism_adj_range(void* p, int64_t lo, int64_t hi) {
+    const pm_ctx* c = (const pm_ctx*)p;
+    const og_grid* g = c->g;
+    const double hx = 0.5 * fabs(g->dx), hy = 0.5 * fabs(g->dy), hz = 0.5 * fabs(g->dz);
+    for (int64_t n = lo; n < hi; ++n) {
+        double x, y, z;
+        cell_centre(g, n, &x, &y, &z);
+        double acc = 0.0;
+        for (int64_t m = 0; m < c->m; ++m)
+            acc += c->r[m] * oracle_prism_kernel(c->xyz[3 * m], c->xyz[3 * m + 1],
+                                                 c->xyz[3 * m + 2], x - hx, x + hx, y - hy,
+                                                 y + hy, z - hz, z + hz);
+        c->out[n] = c->kappa * acc;
+    }
+}
+
+void oracle_prism_adjoint(const og_grid* g, double kappa, int64_t m, const double* xyz,
+                          const double* r, double* out, int nthreads) {
+    pm_ctx c = {g, kappa, 0.0, NULL, xyz, r, out, m};
+    parallel_for((int64_t)g->nx * g->ny * g->nz, nthreads, prism_adj_range, &c);
+}

@@ -1,0 +1,69 @@
+"""Build libztm_gfm.so (HIP, gfx950) and the GFM program in-tree.
+
+    python -m ztm_amd.build
+
+Every HIP source is compiled for gfx950 only, with packed FP32 instructions
+disabled (-packed-fp32-ops), into position-independent objects that are linked
+into ztm_amd/libztm_gfm.so. The GFM program (ztm_amd/bin/GFM) is plain C++
+linked against that library with an $ORIGIN rpath, so the pair can be copied
+next to a TFM build as one unit.
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+from concurrent.futures import ThreadPoolExecutor
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(PKG)
+CSRC = os.path.join(PKG, "csrc")
+OBJ = os.path.join(PKG, "_build")
+LIB = os.path.join(PKG, "libztm_gfm.so")
+GFM = os.path.join(PKG, "bin", "GFM")
+
+HIP_SOURCES = ["gfm_abi.hip", "gfm_pointmass.hip"]
+ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
+HIPCC = os.path.join(ROCM, "bin", "hipcc")
+HIP_FLAGS = [
+    "-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wall", "-Wno-unused-result", "-Wno-unknown-pragmas",
+]
+
+
+def _run(cmd: list[str]) -> None:
+    proc = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    out = "\n".join(l for l in proc.stdout.splitlines()
+                    if "packed-fp32-ops' is not a recognized feature" not in l)
+    if proc.returncode != 0:
+        raise RuntimeError(f"command failed ({proc.returncode}): {' '.join(cmd)}\n{out}")
+    if out.strip():
+        sys.stderr.write(out + "\n")
+
+
+def _compile(src: str) -> str:
+    obj = os.path.join(OBJ, os.path.splitext(src)[0] + ".o")
+    _run([HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj])
+    return obj
+
+
+def build(verbose: bool = True) -> str:
+    """Compile every HIP source for gfx950 and link the library and GFM."""
+    os.makedirs(OBJ, exist_ok=True)
+    os.makedirs(os.path.dirname(GFM), exist_ok=True)
+    with ThreadPoolExecutor(max_workers=min(8, len(HIP_SOURCES))) as ex:
+        objs = list(ex.map(_compile, HIP_SOURCES))
+    tmp = LIB + ".tmp"
+    _run([HIPCC, "-shared", "-fPIC", "--offload-arch=gfx950", "-o", tmp, *objs])
+    os.replace(tmp, LIB)
+    _run(["g++", "-O2", "-std=c++17", "-Wall", "-o", GFM, os.path.join(CSRC, "gfm_main.cpp"),
+          "-L" + PKG, "-lztm_gfm", "-Wl,-rpath,$ORIGIN/..",
+          "-Wl,-rpath," + os.path.join(ROCM, "lib")])
+    if verbose:
+        print(f"built {os.path.relpath(LIB, ROOT)} and {os.path.relpath(GFM, ROOT)}")
+    return LIB
+
+
+if __name__ == "__main__":
+    build()

@@ -1,0 +1,545 @@
+// extern "C" layer of libztm_gfm: error reporting, device checks, plans, the
+// reference-signature entry points and dominant-kernel timing.
+#include "gfm_internal.h"
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <tuple>
+#include <vector>
+
+namespace ztm {
+
+namespace {
+thread_local std::string g_last_error;
+std::mutex g_mu;   // guards device-check cache, reference plans and profiling
+}  // namespace
+
+int fail(int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+    return code;
+}
+
+int hip_fail(hipError_t e, const char* what) {
+    const int code = (e == hipErrorOutOfMemory) ? ZTM_ENOMEM
+                     : (e == hipErrorNoDevice || e == hipErrorInvalidDevice) ? ZTM_ENODEV
+                                                                              : ZTM_EHIP;
+    return fail(code, "%s failed: %s", what, hipGetErrorString(e));
+}
+
+// ------------------------------------------------------------ profiling
+namespace {
+struct ProfRec {
+    std::string name;
+    hipEvent_t a, b;
+};
+bool g_prof_on = false;
+std::vector<ProfRec> g_prof;
+}  // namespace
+
+ProfScope::ProfScope(const char* name, hipStream_t s) : name_(name), stream_(s) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (!g_prof_on) return;
+    if (hipEventCreate(&a_) != hipSuccess || hipEventCreate(&b_) != hipSuccess) {
+        a_ = b_ = nullptr;
+        return;
+    }
+    (void)hipEventRecord(a_, stream_);
+}
+
+ProfScope::~ProfScope() {
+    if (!a_) return;
+    (void)hipEventRecord(b_, stream_);
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_prof.push_back({name_, a_, b_});
+}
+
+// ------------------------------------------------------------- devices
+namespace {
+std::map<int, int> g_dev_ok;   // device -> ZTM_OK / error code
+
+int check_device(int* dev_out) {
+    int dev = 0;
+    ZTM_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(g_mu);
+    auto it = g_dev_ok.find(dev);
+    if (it == g_dev_ok.end()) {
+        hipDeviceProp_t p;
+        hipError_t e = hipGetDeviceProperties(&p, dev);
+        if (e != hipSuccess) return hip_fail(e, "hipGetDeviceProperties");
+        const int ok = std::strncmp(p.gcnArchName, "gfx950", 6) == 0 ? ZTM_OK : ZTM_ENODEV;
+        it = g_dev_ok.emplace(dev, ok).first;
+        if (ok != ZTM_OK)
+            return fail(ZTM_ENODEV, "device %d is %s; libztm_gfm is built for gfx950 only",
+                        dev, p.gcnArchName);
+    }
+    if (it->second != ZTM_OK)
+        return fail(ZTM_ENODEV, "device %d is not gfx950", dev);
+    if (dev_out) *dev_out = dev;
+    return ZTM_OK;
+}
+}  // namespace
+
+}  // namespace ztm
+
+using namespace ztm;
+
+// ----------------------------------------------------------------- plans
+struct ztm_plan_s {
+    int device = 0;
+    ztm_problem prob{};
+    PmShape shape{};
+    PmTuning tuning{};
+    double L = 1.0, sy = 1.0, sz = 1.0;
+    float4* st_f = nullptr;      // [m_pad] prepared stations, FP32
+    double4* st_d = nullptr;     // [m_pad] prepared stations, FP64
+    int4* st_c = nullptr;        // [m_pad] nearest cell index per station
+    int* col_start = nullptr;    // near-field gather index (general plans)
+    int* col_st = nullptr;
+    int kmin = 0, kmax = -1;
+    int* out_map = nullptr;      // reference mode: slot -> g index
+    double* part = nullptr;      // split-K workspace
+    size_t part_cap = 0;         // doubles
+};
+
+namespace {
+
+int grid_check(const ztm_grid& g) {
+    if (g.nx <= 0 || g.ny <= 0 || g.nz <= 0)
+        return fail(ZTM_EINVAL, "grid dims must be positive (got %d x %d x %d)", g.nx, g.ny, g.nz);
+    if (!(std::isfinite(g.dx) && std::isfinite(g.dy) && std::isfinite(g.dz)) ||
+        g.dx == 0.0 || g.dy == 0.0 || g.dz == 0.0)
+        return fail(ZTM_EINVAL, "grid spacings must be finite and non-zero");
+    const double n = (double)g.nx * g.ny * g.nz;
+    if (n > 2147483647.0) return fail(ZTM_EINVAL, "grid has more than 2^31-1 cells");
+    return ZTM_OK;
+}
+
+int ensure_workspace(ztm_plan p, size_t doubles) {
+    if (doubles <= p->part_cap) return ZTM_OK;
+    if (p->part) (void)hipFree(p->part);
+    p->part = nullptr;
+    p->part_cap = 0;
+    ZTM_HIP(hipMalloc(&p->part, doubles * sizeof(double)));
+    p->part_cap = doubles;
+    return ZTM_OK;
+}
+
+void plan_free(ztm_plan p) {
+    if (!p) return;
+    if (p->st_f) (void)hipFree(p->st_f);
+    if (p->st_d) (void)hipFree(p->st_d);
+    if (p->st_c) (void)hipFree(p->st_c);
+    if (p->col_start) (void)hipFree(p->col_start);
+    if (p->col_st) (void)hipFree(p->col_st);
+    if (p->out_map) (void)hipFree(p->out_map);
+    if (p->part) (void)hipFree(p->part);
+    delete p;
+}
+
+int set_shape(ztm_plan p, int64_t m) {
+    const ztm_grid& g = p->prob.grid;
+    p->shape.nx = g.nx;
+    p->shape.ny = g.ny;
+    p->shape.nz = g.nz;
+    p->shape.n_rows = (int64_t)g.ny * g.nz;
+    p->shape.n_cells = (int64_t)g.nx * g.ny * g.nz;
+    if (m > (int64_t)1 << 30) return fail(ZTM_EINVAL, "too many stations (%lld)", (long long)m);
+    p->shape.m = (int)m;
+    p->shape.m_pad = (int)(((m + kStationPad - 1) / kStationPad) * kStationPad);
+    if (p->shape.m_pad == 0) p->shape.m_pad = kStationPad;
+    p->L = std::fabs(g.dx);
+    p->sy = g.dy / p->L;
+    p->sz = g.dz / p->L;
+    return ZTM_OK;
+}
+
+NearArgs near_args(const ztm_plan p) {
+    NearArgs a;
+    a.st_f = p->st_f;
+    a.st_d = p->st_d;
+    a.st_c = p->st_c;
+    a.col_start = p->col_start;
+    a.col_st = p->col_st;
+    a.nx = p->shape.nx;
+    a.ny = p->shape.ny;
+    a.nz = p->shape.nz;
+    a.syf = (float)p->sy;
+    a.szf = (float)p->sz;
+    a.thr = (float)p->prob.threshold;
+    a.syd = p->sy;
+    a.szd = p->sz;
+    a.kmin = p->kmin;
+    a.kmax = p->kmax;
+    return a;
+}
+
+double pm_scale(const ztm_plan p) {
+    // kappa * V / L^2 turns  sum rho dz'/r'^3  (cell units) into g.
+    const ztm_grid& g = p->prob.grid;
+    const double V = std::fabs(g.dx * g.dy * g.dz);
+    return p->prob.kappa * V / (p->L * p->L);
+}
+
+}  // namespace
+
+extern "C" {
+
+int ztm_version(void) { return ZTM_GFM_VERSION; }
+
+const char* ztm_strerror(int code) {
+    switch (code) {
+        case ZTM_OK: return "ok";
+        case ZTM_EINVAL: return "invalid argument";
+        case ZTM_EHIP: return "HIP runtime error";
+        case ZTM_ENOMEM: return "out of memory";
+        case ZTM_ENODEV: return "no usable gfx950 device";
+        case ZTM_EIO: return "I/O error";
+        default: return "unknown error";
+    }
+}
+
+const char* ztm_last_error(void) { return g_last_error.c_str(); }
+
+int ztm_init(void) { return check_device(nullptr); }
+
+int ztm_plan_create(ztm_plan* plan, const ztm_problem* problem, int64_t n_stations,
+                    const double* stations_host) {
+    if (!plan || !problem) return fail(ZTM_EINVAL, "null plan or problem");
+    *plan = nullptr;
+    if (int rc = grid_check(problem->grid)) return rc;
+    if (problem->formula != ZTM_POINT_MASS && problem->formula != ZTM_PRISM)
+        return fail(ZTM_EINVAL, "unknown formula %d", problem->formula);
+    if (problem->formula == ZTM_PRISM)
+        return fail(ZTM_EINVAL, "prism formula is not available in this build");
+    if (n_stations < 0) return fail(ZTM_EINVAL, "negative station count");
+    if (n_stations > 0 && !stations_host) return fail(ZTM_EINVAL, "null stations");
+    if (!std::isfinite(problem->kappa)) return fail(ZTM_EINVAL, "kappa must be finite");
+    int dev = 0;
+    if (int rc = check_device(&dev)) return rc;
+
+    ztm_plan p = new ztm_plan_s;
+    p->device = dev;
+    p->prob = *problem;
+    if (int rc = set_shape(p, n_stations)) { plan_free(p); return rc; }
+
+    const ztm_grid& g = problem->grid;
+    std::vector<float4> hf(p->shape.m_pad, make_float4(0.f, 0.f, 0.f, 0.f));
+    std::vector<double4> hd(p->shape.m_pad, make_double4(0.0, 0.0, 0.0, 0.0));
+    std::vector<int4> hc(p->shape.m_pad, make_int4(0, 0, 0, 0));
+    const int wcols = g.nx + 2 * kNear, hcols = g.ny + 2 * kNear;
+    std::vector<int> col_count((size_t)wcols * hcols + 1, 0);
+    for (int64_t m = 0; m < n_stations; ++m) {
+        const double x = stations_host[3 * m], y = stations_host[3 * m + 1],
+                     z = stations_host[3 * m + 2];
+        if (!(std::isfinite(x) && std::isfinite(y) && std::isfinite(z))) {
+            plan_free(p);
+            return fail(ZTM_EINVAL, "station %lld has a non-finite coordinate", (long long)m);
+        }
+        const double X = (x - g.x0) / g.dx;
+        const double Y = (y - g.y0) / p->L;
+        const double Z = (z - g.z0) / p->L;
+        hf[m] = make_float4((float)X, (float)Y, (float)Z, 0.f);
+        hd[m] = make_double4(X, Y, Z, 0.0);
+        // nearest cell index per axis, clamped far outside the grid so it fits int
+        auto idx = [](double v, int n) {
+            const double c = std::nearbyint(v);
+            return (int)std::max(-4.0 * kNear - 1.0, std::min(c, (double)n + 4.0 * kNear));
+        };
+        hc[m] = make_int4(idx(X, g.nx), idx(Y / p->sy, g.ny), idx(Z / p->sz, g.nz), 0);
+    }
+    // Stations grouped by nearest (ci, cj) column for the adjoint's near-field
+    // gather (counting sort, stable in station order).
+    std::vector<int> col_of((size_t)n_stations, -1);
+    bool any_near = false;
+    for (int64_t m = 0; m < n_stations; ++m) {
+        const int ci = hc[m].x + kNear, cj = hc[m].y + kNear;
+        if (ci < 0 || ci >= wcols || cj < 0 || cj >= hcols) continue;
+        col_of[m] = cj * wcols + ci;
+        ++col_count[col_of[m] + 1];
+        p->kmin = any_near ? std::min(p->kmin, hc[m].z) : hc[m].z;
+        p->kmax = any_near ? std::max(p->kmax, hc[m].z) : hc[m].z;
+        any_near = true;
+    }
+    if (!any_near) {   // no station is near any cell: the gather is skipped
+        p->kmin = 1 << 29;
+        p->kmax = -(1 << 29);
+    }
+    for (size_t c = 1; c < col_count.size(); ++c) col_count[c] += col_count[c - 1];
+    std::vector<int> col_st(std::max<int>(1, col_count.back()), 0);
+    {
+        std::vector<int> fill(col_count.begin(), col_count.end() - 1);
+        for (int64_t m = 0; m < n_stations; ++m)
+            if (col_of[m] >= 0) col_st[fill[col_of[m]]++] = (int)m;
+    }
+    hipError_t e = hipMalloc(&p->st_f, sizeof(float4) * p->shape.m_pad);
+    if (e == hipSuccess) e = hipMalloc(&p->st_d, sizeof(double4) * p->shape.m_pad);
+    if (e == hipSuccess) e = hipMalloc(&p->st_c, sizeof(int4) * p->shape.m_pad);
+    if (e == hipSuccess) e = hipMalloc(&p->col_start, sizeof(int) * col_count.size());
+    if (e == hipSuccess) e = hipMalloc(&p->col_st, sizeof(int) * col_st.size());
+    if (e == hipSuccess)
+        e = hipMemcpy(p->st_f, hf.data(), sizeof(float4) * p->shape.m_pad, hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = hipMemcpy(p->st_d, hd.data(), sizeof(double4) * p->shape.m_pad, hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = hipMemcpy(p->st_c, hc.data(), sizeof(int4) * p->shape.m_pad, hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = hipMemcpy(p->col_start, col_count.data(), sizeof(int) * col_count.size(),
+                      hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = hipMemcpy(p->col_st, col_st.data(), sizeof(int) * col_st.size(), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        plan_free(p);
+        return hip_fail(e, "plan station upload");
+    }
+    *plan = p;
+    return ZTM_OK;
+}
+
+int ztm_plan_destroy(ztm_plan plan) {
+    plan_free(plan);
+    return ZTM_OK;
+}
+
+int64_t ztm_plan_num_cells(ztm_plan plan) { return plan ? plan->shape.n_cells : -1; }
+int64_t ztm_plan_num_stations(ztm_plan plan) { return plan ? plan->shape.m : -1; }
+
+int ztm_plan_set_tuning(ztm_plan plan, int st_per_lane, int rows_per_slice,
+                        int cells_per_lane, int adj_splits) {
+    if (!plan) return fail(ZTM_EINVAL, "null plan");
+    if (st_per_lane && st_per_lane != 1 && st_per_lane != 2 && st_per_lane != 4 && st_per_lane != 8)
+        return fail(ZTM_EINVAL, "st_per_lane must be 1, 2, 4 or 8");
+    if (cells_per_lane && cells_per_lane != 4 && cells_per_lane != 8 && cells_per_lane != 16)
+        return fail(ZTM_EINVAL, "cells_per_lane must be 4, 8 or 16");
+    if (rows_per_slice < 0 || adj_splits < 0) return fail(ZTM_EINVAL, "negative tuning value");
+    if (st_per_lane) plan->tuning.st_per_lane = st_per_lane;
+    if (rows_per_slice) plan->tuning.rows_per_slice = rows_per_slice;
+    if (cells_per_lane) plan->tuning.cells_per_lane = cells_per_lane;
+    if (adj_splits) plan->tuning.adj_splits = adj_splits;
+    return ZTM_OK;
+}
+
+int ztm_plan_get_tuning(ztm_plan plan, int* st_per_lane, int* rows_per_slice,
+                        int* cells_per_lane, int* adj_splits) {
+    if (!plan) return fail(ZTM_EINVAL, "null plan");
+    if (st_per_lane) *st_per_lane = plan->tuning.st_per_lane;
+    if (rows_per_slice) *rows_per_slice = plan->tuning.rows_per_slice;
+    if (cells_per_lane) *cells_per_lane = plan->tuning.cells_per_lane;
+    if (adj_splits) *adj_splits = plan->tuning.adj_splits;
+    return ZTM_OK;
+}
+
+static PmTuning effective_tuning(ztm_plan plan, int precision) {
+    PmTuning t = plan->tuning;
+    if (precision == ZTM_FP64) {
+        if (t.st_per_lane > 4) t.st_per_lane = 4;
+        if (t.cells_per_lane > 8) t.cells_per_lane = 8;
+    }
+    return t;
+}
+
+int ztm_plan_forward(ztm_plan plan, int precision, const void* rho_dev, double* g_dev,
+                     void* stream) {
+    if (!plan) return fail(ZTM_EINVAL, "null plan");
+    if (precision != ZTM_FP32 && precision != ZTM_FP64)
+        return fail(ZTM_EINVAL, "unknown precision %d", precision);
+    if (plan->shape.m > 0 && (!rho_dev || !g_dev)) return fail(ZTM_EINVAL, "null rho or g");
+    int dev = 0;
+    ZTM_HIP(hipGetDevice(&dev));
+    if (dev != plan->device)
+        return fail(ZTM_EINVAL, "plan belongs to device %d, current device is %d", plan->device, dev);
+    const PmTuning t = effective_tuning(plan, precision);
+    if (int rc = ensure_workspace(plan, pm_fwd_workspace(plan->shape, t, precision))) return rc;
+    const void* st = precision == ZTM_FP64 ? (const void*)plan->st_d : (const void*)plan->st_f;
+    return pm_fwd_launch(plan->shape, t, precision, st, rho_dev, plan->sy, plan->sz,
+                         plan->prob.threshold, pm_scale(plan), plan->part, g_dev, nullptr,
+                         nullptr, near_args(plan), (hipStream_t)stream);
+}
+
+int ztm_plan_adjoint(ztm_plan plan, int precision, const void* r_dev, void* out_dev,
+                     void* stream) {
+    if (!plan) return fail(ZTM_EINVAL, "null plan");
+    if (precision != ZTM_FP32 && precision != ZTM_FP64)
+        return fail(ZTM_EINVAL, "unknown precision %d", precision);
+    if (!out_dev || (plan->shape.m > 0 && !r_dev)) return fail(ZTM_EINVAL, "null r or out");
+    int dev = 0;
+    ZTM_HIP(hipGetDevice(&dev));
+    if (dev != plan->device)
+        return fail(ZTM_EINVAL, "plan belongs to device %d, current device is %d", plan->device, dev);
+    const PmTuning t = effective_tuning(plan, precision);
+    if (int rc = ensure_workspace(
+            plan, std::max<size_t>(1, pm_adj_workspace(plan->shape, t, precision,
+                                                       precision == ZTM_FP32))))
+        return rc;
+    const void* st = precision == ZTM_FP64 ? (const void*)plan->st_d : (const void*)plan->st_f;
+    if (plan->shape.m == 0) {
+        const size_t bytes = (size_t)plan->shape.n_cells *
+                             (precision == ZTM_FP64 ? sizeof(double) : sizeof(float));
+        ZTM_HIP(hipMemsetAsync(out_dev, 0, bytes, (hipStream_t)stream));
+        return ZTM_OK;
+    }
+    return pm_adj_launch(plan->shape, t, precision, st, r_dev, plan->sy, plan->sz,
+                         pm_scale(plan), plan->part, out_dev, near_args(plan),
+                         (hipStream_t)stream);
+}
+
+// ------------------------------------------------ reference-signature ABI
+namespace {
+using RefKey = std::tuple<int, int, int, int, int, int, int>;   // dev,bfr,di,dj,dk,fh,fv
+std::map<RefKey, ztm_plan> g_ref_plans;
+
+int ref_check(int bfr, int di, int dj, int dk, int fh, int fv) {
+    if (di <= 0 || dj <= 0 || dk <= 0 || fh <= 0 || fv <= 0 || bfr < 0)
+        return fail(ZTM_EINVAL, "bad reference geometry bfr=%d di=%d dj=%d dk=%d fh=%d fv=%d",
+                    bfr, di, dj, dk, fh, fv);
+    const double n = ((double)(di - 1) * fh + 1) * ((double)(dj - 1) * fh + 1) *
+                     ((double)(dk - 1) * fv + 1);
+    if (n > 2147483647.0) return fail(ZTM_EINVAL, "reference grid too large");
+    return ZTM_OK;
+}
+
+// One cached plan per (device, geometry); station heights change per call and
+// are re-prepared on the device by prep_ref_stations().
+int ref_plan(int bfr, int di, int dj, int dk, int fh, int fv, ztm_plan* out) {
+    int dev = 0;
+    if (int rc = check_device(&dev)) return rc;
+    std::lock_guard<std::mutex> lk(g_mu);
+    const RefKey key{dev, bfr, di, dj, dk, fh, fv};
+    auto it = g_ref_plans.find(key);
+    if (it != g_ref_plans.end()) {
+        *out = it->second;
+        return ZTM_OK;
+    }
+    ztm_plan p = new ztm_plan_s;
+    p->device = dev;
+    ztm_problem& pr = p->prob;
+    pr.grid.nx = (di - 1) * fh + 1;
+    pr.grid.ny = (dj - 1) * fh + 1;
+    pr.grid.nz = (dk - 1) * fv + 1;
+    pr.grid.x0 = pr.grid.y0 = pr.grid.z0 = 0.0;
+    pr.grid.dx = pr.grid.dy = 1.0 / fh;
+    pr.grid.dz = 1.0 / fv;
+    pr.formula = ZTM_POINT_MASS;
+    pr.kappa = ZTM_KAPPA_REF;
+    pr.threshold = 0.001f;   // GFM_GPU.cu:19 compares in float against .001f
+    const int ni = std::max(0, di - 2 * bfr), nj = std::max(0, dj - 2 * bfr);
+    if (int rc = set_shape(p, (int64_t)ni * nj)) { plan_free(p); return rc; }
+    hipError_t e = hipMalloc(&p->st_f, sizeof(float4) * p->shape.m_pad);
+    if (e == hipSuccess) e = hipMalloc(&p->st_d, sizeof(double4) * p->shape.m_pad);
+    if (e == hipSuccess) e = hipMalloc(&p->st_c, sizeof(int4) * p->shape.m_pad);
+    if (e == hipSuccess) e = hipMalloc(&p->out_map, sizeof(int) * p->shape.m_pad);
+    if (e != hipSuccess) {
+        plan_free(p);
+        return hip_fail(e, "reference plan allocation");
+    }
+    if (int rc = ensure_workspace(p, pm_fwd_workspace(p->shape, p->tuning, ZTM_FP32))) {
+        plan_free(p);
+        return rc;
+    }
+    g_ref_plans.emplace(key, p);
+    *out = p;
+    return ZTM_OK;
+}
+}  // namespace
+
+int ztm_gf_async(int bfr, int di, int dj, int dk, int fh, int fv, float* g_dev,
+                 const float* rho_dev, const float* s_dev, void* stream) {
+    if (int rc = ref_check(bfr, di, dj, dk, fh, fv)) return rc;
+    if (!g_dev || !rho_dev || !s_dev) return fail(ZTM_EINVAL, "null g, rho or s");
+    ztm_plan p = nullptr;
+    if (int rc = ref_plan(bfr, di, dj, dk, fh, fv, &p)) return rc;
+    if (p->shape.m == 0) return ZTM_OK;   // no interior stations
+    hipStream_t s = (hipStream_t)stream;
+    if (int rc = prep_ref_stations(s_dev, bfr, di, dj, fh, fv, p->st_f, p->st_d, p->st_c,
+                                   p->out_map, p->shape.m, p->shape.m_pad, s))
+        return rc;
+    return pm_fwd_launch(p->shape, p->tuning, ZTM_FP32, p->st_f, rho_dev, p->sy, p->sz,
+                         p->prob.threshold, pm_scale(p), p->part, nullptr, g_dev, p->out_map,
+                         near_args(p), s);
+}
+
+int ztm_gf(int bfr, int di, int dj, int dk, int fh, int fv, float* g_dev,
+           const float* rho_dev, const float* s_dev) {
+    return ztm_gf_async(bfr, di, dj, dk, fh, fv, g_dev, rho_dev, s_dev, nullptr);
+}
+
+int ztm_gfm_host(int bfr, int di, int dj, int dk, int fh, int fv, float* g_host,
+                 const float* rho_host, int64_t n_rho, const float* s_host) {
+    if (int rc = ref_check(bfr, di, dj, dk, fh, fv)) return rc;
+    if (!g_host || !rho_host || !s_host) return fail(ZTM_EINVAL, "null host buffer");
+    const int64_t N = ((int64_t)(di - 1) * fh + 1) * ((int64_t)(dj - 1) * fh + 1) *
+                      ((int64_t)(dk - 1) * fv + 1);
+    if (n_rho != N)
+        return fail(ZTM_EINVAL, "rho has %lld values, geometry needs ii*jj*kk = %lld",
+                    (long long)n_rho, (long long)N);
+    if (int rc = check_device(nullptr)) return rc;
+    const size_t M = (size_t)di * dj;
+    float *r = nullptr, *g = nullptr, *s = nullptr;
+    int rc = ZTM_OK;
+    hipError_t e = hipMalloc(&r, sizeof(float) * N);
+    if (e == hipSuccess) e = hipMalloc(&g, sizeof(float) * M);
+    if (e == hipSuccess) e = hipMalloc(&s, sizeof(float) * M);
+    if (e == hipSuccess) e = hipMemcpy(r, rho_host, sizeof(float) * N, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(s, s_host, sizeof(float) * M, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemset(g, 0, sizeof(float) * M);
+    if (e != hipSuccess) rc = hip_fail(e, "GFM buffers");
+    if (rc == ZTM_OK) rc = ztm_gf(bfr, di, dj, dk, fh, fv, g, r, s);
+    if (rc == ZTM_OK) {
+        e = hipMemcpy(g_host, g, sizeof(float) * M, hipMemcpyDeviceToHost);
+        if (e != hipSuccess) rc = hip_fail(e, "GFM result copy");
+    }
+    if (r) (void)hipFree(r);
+    if (g) (void)hipFree(g);
+    if (s) (void)hipFree(s);
+    return rc;
+}
+
+// ----------------------------------------------------------- profiling
+int ztm_profile_enable(int on) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_prof_on = on != 0;
+    return ZTM_OK;
+}
+
+int ztm_profile_read(const char* kernel, double* total_ms, int64_t* count) {
+    if (!kernel || !total_ms || !count) return fail(ZTM_EINVAL, "null argument");
+    std::vector<ProfRec> recs;
+    {
+        std::lock_guard<std::mutex> lk(g_mu);
+        recs = g_prof;
+    }
+    double tot = 0.0;
+    int64_t n = 0;
+    for (auto& r : recs) {
+        if (r.name != kernel) continue;
+        ZTM_HIP(hipEventSynchronize(r.b));
+        float ms = 0.f;
+        ZTM_HIP(hipEventElapsedTime(&ms, r.a, r.b));
+        tot += ms;
+        ++n;
+    }
+    *total_ms = tot;
+    *count = n;
+    return ZTM_OK;
+}
+
+int ztm_profile_reset(void) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    for (auto& r : g_prof) {
+        (void)hipEventSynchronize(r.b);
+        (void)hipEventDestroy(r.a);
+        (void)hipEventDestroy(r.b);
+    }
+    g_prof.clear();
+    return ZTM_OK;
+}
+
+}  // extern "C"

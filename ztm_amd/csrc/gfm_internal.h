@@ -1,0 +1,115 @@
+// Internal declarations shared by the HIP translation units of libztm_gfm.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <string>
+
+#include "../../include/ztm_gfm.h"
+
+namespace ztm {
+
+// Thread-local error detail; returns `code` so callers can `return fail(...)`.
+int fail(int code, const char* fmt, ...);
+int hip_fail(hipError_t e, const char* what);
+
+#define ZTM_HIP(call)                                             \
+    do {                                                          \
+        hipError_t e_ = (call);                                   \
+        if (e_ != hipSuccess) return ::ztm::hip_fail(e_, #call);  \
+    } while (0)
+
+// Station records as the kernels read them: coordinates relative to the
+// centre of cell (0,0,0) in units of L = |dx|, scaled so that cell (i,j,k)
+// sits at (i, j*sy, k*sz). x is stored as (x - x0)/dx, so the sign of dx only
+// flips the sign of a squared term. w is unused by the prepared copy.
+struct GeomF {
+    float sy, sz;      // dy/L, dz/L (signed)
+    float thr;         // forward threshold on |rho|
+};
+struct GeomD {
+    double sy, sz, thr;
+};
+
+// Small floor keeps r^2 > 0 when a station coincides with a cell centre, so
+// the (zero-length) vertical offset contributes 0 instead of 0*inf = NaN.
+// In units of L^2 it is far below FP32 resolution of any real distance.
+constexpr float  kEpsF = 1e-20f;
+constexpr double kEpsD = 1e-20;
+
+// Padding of prepared station arrays (largest forward tile = 64 lanes * 8).
+constexpr int kStationPad = 512;
+
+// Near field: pairs whose cell index lies within kNear cells (per axis) of
+// the station's nearest cell index are re-evaluated in FP64 by the FP32 path.
+constexpr int kNear = 2;
+
+// Everything the FP32 near-field correction needs. st_d == nullptr disables it.
+// st_c[m] = (ci, cj, ck, 0): the station's nearest cell index per axis.
+// col_start/col_st: stations grouped by (ci, cj) column over the extended
+// range [-kNear, nx+kNear) x [-kNear, ny+kNear) (adjoint gather); stations
+// outside it are never near any cell. kmin/kmax: range of ck over them.
+struct NearArgs {
+    const float4* st_f = nullptr;
+    const double4* st_d = nullptr;
+    const int4* st_c = nullptr;
+    const int* col_start = nullptr;
+    const int* col_st = nullptr;
+    int nx = 0, ny = 0, nz = 0;
+    float syf = 1.f, szf = 1.f, thr = 0.f;
+    double syd = 1.0, szd = 1.0;
+    int kmin = 0, kmax = -1;
+};
+
+// Launch shapes of the point-mass kernels.
+struct PmTuning {
+    int st_per_lane = 4;      // forward: stations per lane (1,2,4,8)
+    int rows_per_slice = 0;   // forward: 0 = choose from the problem shape
+    int cells_per_lane = 16;  // adjoint: cells per lane (4,8,16)
+    int adj_splits = 0;       // adjoint: 0 = choose from the problem shape
+};
+
+struct PmShape {
+    int nx, ny, nz;
+    int64_t n_rows;           // ny*nz
+    int64_t n_cells;          // nx*ny*nz
+    int m, m_pad;             // stations, padded stations
+};
+
+// Kernel launchers (gfm_pointmass.hip). `part` must hold at least the number
+// of doubles returned by the matching *_workspace() call.
+size_t pm_fwd_workspace(const PmShape& s, const PmTuning& t, int precision);
+size_t pm_adj_workspace(const PmShape& s, const PmTuning& t, int precision,
+                        bool near);
+
+// Forward: writes g[m] = scale * sum (FP64). When out_map != nullptr the
+// result is instead added as float into gref[out_map[m]] (reference ABI).
+int pm_fwd_launch(const PmShape& s, const PmTuning& t, int precision,
+                  const void* st, const void* rho, double sy, double sz,
+                  double thr, double scale, double* part, double* g,
+                  float* gref, const int* out_map, const NearArgs& near,
+                  hipStream_t stream);
+
+// Adjoint: out[n] = scale * sum_m K(m,n) r[m] (float or double by precision).
+// With near.st_d set (FP32 only) the result always goes through `part`.
+int pm_adj_launch(const PmShape& s, const PmTuning& t, int precision,
+                  const void* st, const void* r, double sy, double sz,
+                  double scale, double* part, void* out, const NearArgs& near,
+                  hipStream_t stream);
+
+// Reference-mode station preparation on the device: interior stations of a
+// di x dj raster with heights s[m] -> float4 records + output map.
+int prep_ref_stations(const float* s, int bfr, int di, int dj, int fh, int fv,
+                      float4* st, double4* st_d, int4* st_c, int* out_map,
+                      int m_int, int m_pad, hipStream_t stream);
+
+// Dominant-kernel timing (gfm_abi.hip).
+struct ProfScope {
+    ProfScope(const char* name, hipStream_t s);
+    ~ProfScope();
+    const char* name_;
+    hipStream_t stream_;
+    hipEvent_t a_ = nullptr, b_ = nullptr;
+};
+
+}  // namespace ztm

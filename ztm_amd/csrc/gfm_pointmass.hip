@@ -1,0 +1,517 @@
+// Point-mass gravity forward (g = G rho) and adjoint (G^T r) kernels for
+// gfx950, FP32 and FP64.
+//
+// Cell formula of GFM_GPU.cu:22 and TFM.cpp:98: a cell of density rho at
+// distance r below/above a station adds  kappa * rho * V * dz / r^3  to g_z.
+// The reference scatters one thread per cell over all stations with a float
+// atomicAdd per pair (GFM_GPU.cu:17-22). Here both directions are gathers:
+//
+//   forward  (pm_fwd): a workgroup owns 64*S stations (S per lane, kept in
+//     registers) and a slice of grid rows (fixed j,k); the slice's densities
+//     are staged in LDS once (thresholded, GFM_GPU.cu:19) and read back as
+//     wave-uniform broadcasts. The four waves split the slice's rows and their
+//     partial sums are reduced across wavefronts through LDS at the end. Per
+//     row, dy^2+dz^2 is computed once per station, so a pair costs
+//     sub, fma, rsq, 2 mul, fma.
+//   adjoint  (pm_adj): a lane owns C consecutive cells of one row; station
+//     tiles of 256 (coordinates + residual) are staged in LDS and each wave
+//     walks a quarter of the tile; cross-wave LDS reduction at the end.
+//
+// Precision: coordinates are relative to the centre of cell (0,0,0) in cell
+// units, so station-cell offsets are differences of small numbers (exact for
+// the x offset). Inner sums run over one grid row (forward) or one 256-station
+// tile (adjoint) in the working precision and are then accumulated in FP64;
+// all cross-wave and cross-slice sums are FP64 in a fixed order, so results
+// are bitwise reproducible run to run (no atomics).
+#include "gfm_internal.h"
+
+#include <algorithm>
+
+namespace ztm {
+
+template <typename T> struct Vec4;
+template <> struct Vec4<float>  { using type = float4; };
+template <> struct Vec4<double> { using type = double4; };
+
+__device__ __forceinline__ float  rsq(float x)  { return __builtin_amdgcn_rsqf(x); }
+__device__ __forceinline__ double rsq(double x) { return rsqrt(x); }
+template <typename T> __device__ __forceinline__ T eps_r2();
+template <> __device__ __forceinline__ float  eps_r2<float>()  { return kEpsF; }
+template <> __device__ __forceinline__ double eps_r2<double>() { return kEpsD; }
+
+// ---------------------------------------------------------------- forward
+// grid = (m_pad / (64*S), n_slices); block = 256.
+// st:   prepared stations [m_pad] (x, y, z in cell units; see gfm_internal.h)
+// part: [n_slices][m_pad] FP64 partial sums  sum_n rho_n dz/r^3
+template <typename T, int S>
+__global__ __launch_bounds__(256) void pm_fwd_kernel(
+    const typename Vec4<T>::type* __restrict__ st, const T* __restrict__ rho,
+    int nx, int ny, int n_rows, T sy, T sz, T thr, int rows_per_slice,
+    double* __restrict__ part, int m_pad) {
+    using V4 = typename Vec4<T>::type;
+    extern __shared__ __align__(16) unsigned char smem[];
+    T* lrho = reinterpret_cast<T*>(smem);
+
+    const int nxp = (nx + 3) & ~3;
+    const int row0 = blockIdx.y * rows_per_slice;
+    const int nr = min(rows_per_slice, n_rows - row0);
+    const size_t base = (size_t)row0 * (size_t)nx;
+
+    // Stage this slice's densities, thresholded and zero-padded to nxp.
+    for (int t = threadIdx.x; t < nr * nxp; t += blockDim.x) {
+        const int r = t / nxp;
+        const int i = t - r * nxp;
+        T v = T(0);
+        if (i < nx) {
+            v = rho[base + (size_t)r * nx + i];
+            if (!(fabs(v) > thr)) v = T(0);   // NaN or |rho| <= thr: skipped
+        }
+        lrho[t] = v;
+    }
+    __syncthreads();
+
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int m0 = blockIdx.x * (64 * S);
+
+    T X[S], Y[S], Z[S];
+    double acc[S];
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        const V4 p = st[m0 + s * 64 + lane];
+        X[s] = p.x; Y[s] = p.y; Z[s] = p.z;
+        acc[s] = 0.0;
+    }
+
+    for (int r = wave; r < nr; r += 4) {
+        const int row = row0 + r;
+        const int k = row / ny;
+        const int j = row - k * ny;
+        const T yj = T(j) * sy;
+        const T zk = T(k) * sz;
+        T R[S], dz[S], ar[S], xm[S];
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+            const T dy = Y[s] - yj;
+            dz[s] = Z[s] - zk;
+            R[s] = fma(dy, dy, fma(dz[s], dz[s], eps_r2<T>()));
+            ar[s] = T(0);
+            xm[s] = X[s];
+        }
+        const V4* lrow = reinterpret_cast<const V4*>(lrho + r * nxp);
+        for (int i4 = 0; i4 < (nxp >> 2); ++i4) {
+            const V4 q = lrow[i4];   // wave-uniform LDS broadcast
+#pragma unroll
+            for (int s = 0; s < S; ++s) {
+                const T a0 = xm[s];
+                const T a1 = xm[s] - T(1);
+                const T a2 = xm[s] - T(2);
+                const T a3 = xm[s] - T(3);
+                const T v0 = rsq(fma(a0, a0, R[s]));
+                const T v1 = rsq(fma(a1, a1, R[s]));
+                const T v2 = rsq(fma(a2, a2, R[s]));
+                const T v3 = rsq(fma(a3, a3, R[s]));
+                T t = ar[s];
+                t = fma(q.x, v0 * v0 * v0, t);
+                t = fma(q.y, v1 * v1 * v1, t);
+                t = fma(q.z, v2 * v2 * v2, t);
+                t = fma(q.w, v3 * v3 * v3, t);
+                ar[s] = t;
+                xm[s] -= T(4);
+            }
+        }
+#pragma unroll
+        for (int s = 0; s < S; ++s) acc[s] += (double)(dz[s] * ar[s]);
+    }
+
+    // Cross-wavefront reduction (fixed order w0+w1+w2+w3).
+    __syncthreads();
+    double* red = reinterpret_cast<double*>(smem);
+#pragma unroll
+    for (int s = 0; s < S; ++s) red[(wave * S + s) * 64 + lane] = acc[s];
+    __syncthreads();
+    for (int t = threadIdx.x; t < 64 * S; t += blockDim.x) {
+        const double v = red[t] + red[S * 64 + t] + red[2 * S * 64 + t] +
+                         red[3 * S * 64 + t];
+        part[(size_t)blockIdx.y * m_pad + m0 + t] = v;
+    }
+}
+
+// Near-field correction of one station-cell pair for the FP32 path: the
+// exact FP64 term minus the FP32 term as the main kernels evaluate it (same
+// rounded coordinates, same v_rsq_f32), so adding it replaces the pair's FP32
+// value by its FP64 value. Terms are  w * dz / r^3  in cell units.
+__device__ __forceinline__ double near_pair(float w, const float4& pf, const double4& pd,
+                                            int i, int j, int k, const NearArgs& a) {
+    const float dyf = pf.y - (float)j * a.syf;
+    const float dzf = pf.z - (float)k * a.szf;
+    const float af = pf.x - (float)i;
+    const float v = rsq(fmaf(af, af, fmaf(dyf, dyf, fmaf(dzf, dzf, kEpsF))));
+    const double t32 = (double)dzf * (double)(w * (v * v * v));
+    const double ad = pd.x - (double)i;
+    const double dyd = pd.y - (double)j * a.syd;
+    const double dzd = pd.z - (double)k * a.szd;
+    const double r2 = ad * ad + dyd * dyd + dzd * dzd + kEpsD;
+    const double t64 = (double)w * dzd / (r2 * sqrt(r2));
+    return t64 - t32;
+}
+
+// g[m] = scale * (sum_slices part[slice][m] + near-field correction)
+//   (general API, FP64 out), or gref[map[m]] += (float)(...) (reference ABI).
+__global__ void pm_fwd_reduce_kernel(const double* __restrict__ part,
+                                     int n_slices, int m_pad, int m,
+                                     double scale, double* __restrict__ g,
+                                     float* __restrict__ gref,
+                                     const int* __restrict__ out_map,
+                                     const float* __restrict__ rho_f,
+                                     NearArgs na) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    double s = 0.0;
+    for (int k = 0; k < n_slices; ++k) s += part[(size_t)k * m_pad + i];
+    if (na.st_d) {
+        const float4 pf = na.st_f[i];
+        const double4 pd = na.st_d[i];
+        const int4 c = na.st_c[i];
+        double corr = 0.0;
+        const int k0 = max(c.z - kNear, 0), k1 = min(c.z + kNear, na.nz - 1);
+        const int j0 = max(c.y - kNear, 0), j1 = min(c.y + kNear, na.ny - 1);
+        const int i0 = max(c.x - kNear, 0), i1 = min(c.x + kNear, na.nx - 1);
+        for (int kk = k0; kk <= k1; ++kk)
+            for (int jj = j0; jj <= j1; ++jj)
+                for (int ii = i0; ii <= i1; ++ii) {
+                    const float w = rho_f[((size_t)kk * na.ny + jj) * na.nx + ii];
+                    if (!(fabsf(w) > na.thr)) continue;
+                    corr += near_pair(w, pf, pd, ii, jj, kk, na);
+                }
+        s += corr;
+    }
+    s *= scale;
+    if (out_map) {
+        gref[out_map[i]] += (float)s;
+    } else {
+        g[i] = s;
+    }
+}
+
+// ---------------------------------------------------------------- adjoint
+// grid = (ceil(n_segs/64), n_splits); block = 256; a segment is C
+// consecutive cells of one row (the last one of a row may be ragged).
+template <typename T, int C>
+__global__ __launch_bounds__(256) void pm_adj_kernel(
+    const typename Vec4<T>::type* __restrict__ st, const T* __restrict__ r,
+    int m, int nx, int ny, T sy, T sz, int segs_per_row, int64_t n_segs,
+    int st_per_split, int m_pad, double* __restrict__ part,
+    T* __restrict__ out, double scale, int64_t n_cells, int direct) {
+    using V4 = typename Vec4<T>::type;
+    extern __shared__ __align__(16) unsigned char smem[];
+    V4* tile = reinterpret_cast<V4*>(smem);
+
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int64_t seg = (int64_t)blockIdx.x * 64 + lane;
+    const int64_t segc = seg < n_segs ? seg : 0;
+    const int64_t row = segc / segs_per_row;
+    const int i0 = (int)(segc - row * segs_per_row) * C;
+    const int k = (int)(row / ny);
+    const int j = (int)(row - (int64_t)k * ny);
+    const T yj = T(j) * sy;
+    const T zk = T(k) * sz;
+    const T xi0 = T(i0);
+
+    double accd[C];
+#pragma unroll
+    for (int c = 0; c < C; ++c) accd[c] = 0.0;
+
+    const int mb = blockIdx.y * st_per_split;
+    const int me = min(mb + st_per_split, m_pad);
+    for (int t0 = mb; t0 < me; t0 += 256) {
+        __syncthreads();
+        {
+            const int mm = t0 + threadIdx.x;
+            V4 p = st[mm];
+            p.w = mm < m ? r[mm] : T(0);
+            tile[threadIdx.x] = p;
+        }
+        __syncthreads();
+        T acc[C];
+#pragma unroll
+        for (int c = 0; c < C; ++c) acc[c] = T(0);
+        const int sb = wave * 64;
+        for (int s = sb; s < sb + 64; ++s) {
+            const V4 p = tile[s];    // wave-uniform LDS broadcast
+            const T dy = p.y - yj;
+            const T dz = p.z - zk;
+            const T R = fma(dy, dy, fma(dz, dz, eps_r2<T>()));
+            const T q = p.w * dz;
+            const T xm = p.x - xi0;
+#pragma unroll
+            for (int c = 0; c < C; ++c) {
+                const T a = xm - T(c);
+                const T v = rsq(fma(a, a, R));
+                acc[c] = fma(q, v * v * v, acc[c]);
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < C; ++c) accd[c] += (double)acc[c];
+    }
+
+    // Cross-wavefront reduction (fixed order).
+    __syncthreads();
+    double* red = reinterpret_cast<double*>(smem);
+#pragma unroll
+    for (int c = 0; c < C; ++c) red[(wave * C + c) * 64 + lane] = accd[c];
+    __syncthreads();
+    for (int e = threadIdx.x; e < 64 * C; e += blockDim.x) {
+        const int c = e >> 6;
+        const int l = e & 63;
+        const double v = red[c * 64 + l] + red[(C + c) * 64 + l] +
+                         red[(2 * C + c) * 64 + l] + red[(3 * C + c) * 64 + l];
+        const int64_t sg = (int64_t)blockIdx.x * 64 + l;
+        if (sg >= n_segs) continue;
+        const int64_t rw = sg / segs_per_row;
+        const int ii = (int)(sg - rw * segs_per_row) * C + c;
+        if (ii >= nx) continue;
+        const int64_t cell = rw * nx + ii;
+        if (direct) {
+            out[cell] = (T)(scale * v);
+        } else {
+            part[(size_t)blockIdx.y * (size_t)n_cells + cell] = v;
+        }
+    }
+}
+
+// out[n] = scale * (sum_splits part[split][n] + near-field correction); the
+// correction gathers, per cell, the stations whose column lies within kNear
+// of the cell's column (fixed CSR order, so the result is deterministic).
+template <typename T>
+__global__ void pm_adj_reduce_kernel(const double* __restrict__ part,
+                                     int n_splits, int64_t n_cells,
+                                     double scale, T* __restrict__ out,
+                                     const float* __restrict__ r_f, NearArgs na) {
+    const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= n_cells) return;
+    double s = 0.0;
+    for (int k = 0; k < n_splits; ++k) s += part[(size_t)k * n_cells + n];
+    if (na.st_d) {
+        const int ii = (int)(n % na.nx);
+        const int64_t rw = n / na.nx;
+        const int jj = (int)(rw % na.ny);
+        const int kk = (int)(rw / na.ny);
+        if (kk >= na.kmin - kNear && kk <= na.kmax + kNear) {
+            const int wcols = na.nx + 2 * kNear;
+            double corr = 0.0;
+            for (int cj = jj - kNear; cj <= jj + kNear; ++cj) {
+                if (cj < -kNear || cj >= na.ny + kNear) continue;
+                for (int ci = ii - kNear; ci <= ii + kNear; ++ci) {
+                    if (ci < -kNear || ci >= na.nx + kNear) continue;
+                    const int col = (cj + kNear) * wcols + (ci + kNear);
+                    for (int q = na.col_start[col]; q < na.col_start[col + 1]; ++q) {
+                        const int m = na.col_st[q];
+                        const int4 c = na.st_c[m];
+                        if (c.z < kk - kNear || c.z > kk + kNear) continue;
+                        corr += near_pair(r_f[m], na.st_f[m], na.st_d[m], ii, jj, kk, na);
+                    }
+                }
+            }
+            s += corr;
+        }
+    }
+    out[n] = (T)(scale * s);
+}
+
+// ------------------------------------------------------- station prep
+__global__ void prep_ref_kernel(const float* __restrict__ s, int bfr, int di,
+                                int fh, int fv, int ni_count, int m_int,
+                                int m_pad, float4* __restrict__ st,
+                                double4* __restrict__ st_d, int4* __restrict__ st_c,
+                                int* __restrict__ out_map) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= m_pad) return;
+    if (t < m_int) {
+        const int ni = bfr + t % ni_count;
+        const int nj = bfr + t / ni_count;
+        const int m = ni + nj * di;
+        // Cell units (L = 1/fh): station (ni, nj, s) -> (ni*fh, nj*fh, s*fh);
+        // cell (i,j,k) sits at (i, j, k*fh/fv).
+        const double zd = (double)s[m] * fh;
+        st[t] = make_float4((float)(ni * fh), (float)(nj * fh), (float)zd, 0.f);
+        st_d[t] = make_double4((double)(ni * fh), (double)(nj * fh), zd, 0.0);
+        st_c[t] = make_int4(ni * fh, nj * fh, (int)rint(zd * fv / fh), 0);
+        out_map[t] = m;
+    } else {
+        st[t] = make_float4(0.f, 0.f, 0.f, 0.f);
+        st_d[t] = make_double4(0.0, 0.0, 0.0, 0.0);
+        st_c[t] = make_int4(0, 0, 0, 0);
+    }
+}
+
+int prep_ref_stations(const float* s, int bfr, int di, int dj, int fh, int fv,
+                      float4* st, double4* st_d, int4* st_c, int* out_map,
+                      int m_int, int m_pad, hipStream_t stream) {
+    (void)dj;
+    const int ni_count = di - 2 * bfr;
+    hipLaunchKernelGGL(prep_ref_kernel, dim3((m_pad + 255) / 256), dim3(256), 0,
+                       stream, s, bfr, di, fh, fv, ni_count, m_int, m_pad, st,
+                       st_d, st_c, out_map);
+    ZTM_HIP(hipGetLastError());
+    return ZTM_OK;
+}
+
+// ------------------------------------------------------------ launchers
+namespace {
+
+constexpr int kTargetWgs = 2048;   // 8 workgroups per CU on 256 CUs
+
+int fwd_rows_per_slice(const PmShape& s, const PmTuning& t, size_t elem) {
+    const int nxp = (s.nx + 3) & ~3;
+    const int lds_rows = std::max(1, (int)(32768 / ((size_t)nxp * elem)));
+    if (t.rows_per_slice > 0) return std::min(t.rows_per_slice, std::max(lds_rows, 1));
+    const int tiles = s.m_pad / (64 * t.st_per_lane);
+    const int64_t want_slices = std::max<int64_t>(1, (kTargetWgs + tiles - 1) / tiles);
+    int64_t rps = (s.n_rows + want_slices - 1) / want_slices;
+    rps = std::max<int64_t>(rps, 4);
+    rps = std::min<int64_t>(rps, lds_rows);
+    rps = std::min<int64_t>(rps, s.n_rows);
+    return (int)std::max<int64_t>(rps, 1);
+}
+
+int adj_splits(const PmShape& s, const PmTuning& t) {
+    const int C = t.cells_per_lane;
+    const int64_t segs = s.n_rows * ((s.nx + C - 1) / C);
+    const int64_t wgs = (segs + 63) / 64;
+    const int max_splits = s.m_pad / 256;
+    int sp = t.adj_splits;
+    if (sp <= 0) sp = (int)std::max<int64_t>(1, (kTargetWgs + wgs - 1) / wgs);
+    return std::max(1, std::min(sp, max_splits));
+}
+
+template <typename T, int S>
+void launch_fwd(const PmShape& s, int rps, int n_slices, const void* st,
+                const void* rho, T sy, T sz, T thr, double* part,
+                hipStream_t stream) {
+    using V4 = typename Vec4<T>::type;
+    const int nxp = (s.nx + 3) & ~3;
+    const size_t lds = std::max((size_t)rps * nxp * sizeof(T), (size_t)4 * S * 64 * sizeof(double));
+    hipLaunchKernelGGL((pm_fwd_kernel<T, S>), dim3(s.m_pad / (64 * S), n_slices),
+                       dim3(256), lds, stream, (const V4*)st, (const T*)rho, s.nx,
+                       s.ny, (int)s.n_rows, sy, sz, thr, rps, part, s.m_pad);
+}
+
+template <typename T, int C>
+void launch_adj(const PmShape& s, int splits, int direct, const void* st, const void* r,
+                T sy, T sz, double* part, void* out, double scale,
+                hipStream_t stream) {
+    using V4 = typename Vec4<T>::type;
+    const int segs_per_row = (s.nx + C - 1) / C;
+    const int64_t n_segs = s.n_rows * segs_per_row;
+    const int blocks_x = (int)((n_segs + 63) / 64);
+    const int tiles = s.m_pad / 256;
+    const int per_split = ((tiles + splits - 1) / splits) * 256;
+    const size_t lds = std::max((size_t)256 * sizeof(V4), (size_t)4 * C * 64 * sizeof(double));
+    hipLaunchKernelGGL((pm_adj_kernel<T, C>), dim3(blocks_x, splits), dim3(256),
+                       lds, stream, (const V4*)st, (const T*)r, s.m, s.nx, s.ny,
+                       sy, sz, segs_per_row, n_segs, per_split, s.m_pad, part,
+                       (T*)out, scale, s.n_cells, direct);
+}
+
+}  // namespace
+
+size_t pm_fwd_workspace(const PmShape& s, const PmTuning& t, int precision) {
+    const size_t elem = precision == ZTM_FP64 ? sizeof(double) : sizeof(float);
+    const int rps = fwd_rows_per_slice(s, t, elem);
+    const int64_t n_slices = (s.n_rows + rps - 1) / rps;
+    return (size_t)n_slices * (size_t)s.m_pad;
+}
+
+size_t pm_adj_workspace(const PmShape& s, const PmTuning& t, int precision, bool near) {
+    (void)precision;
+    const int sp = adj_splits(s, t);
+    return (sp > 1 || near) ? (size_t)sp * (size_t)s.n_cells : 0;
+}
+
+int pm_fwd_launch(const PmShape& s, const PmTuning& t, int precision,
+                  const void* st, const void* rho, double sy, double sz,
+                  double thr, double scale, double* part, double* g,
+                  float* gref, const int* out_map, const NearArgs& near,
+                  hipStream_t stream) {
+    const size_t elem = precision == ZTM_FP64 ? sizeof(double) : sizeof(float);
+    const int nxp = (s.nx + 3) & ~3;
+    if ((size_t)nxp * elem * 1 > 65536)
+        return fail(ZTM_EINVAL, "nx=%d too large for one LDS row (max %d)", s.nx,
+                    (int)(65536 / elem) - 3);
+    if (s.m == 0) return ZTM_OK;
+    const int rps = fwd_rows_per_slice(s, t, elem);
+    const int n_slices = (int)((s.n_rows + rps - 1) / rps);
+    if (n_slices > 65535) return fail(ZTM_EINVAL, "too many row slices (%d)", n_slices);
+    {
+        ProfScope prof("pm_fwd", stream);
+        if (precision == ZTM_FP64) {
+            const double a = sy, b = sz, c = thr;
+            switch (t.st_per_lane) {
+                case 1: launch_fwd<double, 1>(s, rps, n_slices, st, rho, a, b, c, part, stream); break;
+                case 2: launch_fwd<double, 2>(s, rps, n_slices, st, rho, a, b, c, part, stream); break;
+                default: launch_fwd<double, 4>(s, rps, n_slices, st, rho, a, b, c, part, stream); break;
+            }
+        } else {
+            const float a = (float)sy, b = (float)sz, c = (float)thr;
+            switch (t.st_per_lane) {
+                case 1: launch_fwd<float, 1>(s, rps, n_slices, st, rho, a, b, c, part, stream); break;
+                case 2: launch_fwd<float, 2>(s, rps, n_slices, st, rho, a, b, c, part, stream); break;
+                case 8: launch_fwd<float, 8>(s, rps, n_slices, st, rho, a, b, c, part, stream); break;
+                default: launch_fwd<float, 4>(s, rps, n_slices, st, rho, a, b, c, part, stream); break;
+            }
+        }
+        ZTM_HIP(hipGetLastError());
+    }
+    NearArgs na = near;
+    if (precision == ZTM_FP64) na.st_d = nullptr;   // FP64 needs no correction
+    hipLaunchKernelGGL(pm_fwd_reduce_kernel, dim3((s.m + 255) / 256), dim3(256), 0,
+                       stream, part, n_slices, s.m_pad, s.m, scale, g, gref, out_map,
+                       (const float*)rho, na);
+    ZTM_HIP(hipGetLastError());
+    return ZTM_OK;
+}
+
+int pm_adj_launch(const PmShape& s, const PmTuning& t, int precision,
+                  const void* st, const void* r, double sy, double sz,
+                  double scale, double* part, void* out, const NearArgs& near,
+                  hipStream_t stream) {
+    if (s.n_cells == 0) return ZTM_OK;
+    const int splits = adj_splits(s, t);
+    NearArgs na = near;
+    if (precision == ZTM_FP64) na.st_d = nullptr;
+    const int direct = (splits == 1 && !na.st_d) ? 1 : 0;
+    {
+        ProfScope prof("pm_adj", stream);
+        if (precision == ZTM_FP64) {
+            switch (t.cells_per_lane) {
+                case 4: launch_adj<double, 4>(s, splits, direct, st, r, sy, sz, part, out, scale, stream); break;
+                default: launch_adj<double, 8>(s, splits, direct, st, r, sy, sz, part, out, scale, stream); break;
+            }
+        } else {
+            const float a = (float)sy, b = (float)sz;
+            switch (t.cells_per_lane) {
+                case 4: launch_adj<float, 4>(s, splits, direct, st, r, a, b, part, out, scale, stream); break;
+                case 8: launch_adj<float, 8>(s, splits, direct, st, r, a, b, part, out, scale, stream); break;
+                default: launch_adj<float, 16>(s, splits, direct, st, r, a, b, part, out, scale, stream); break;
+            }
+        }
+        ZTM_HIP(hipGetLastError());
+    }
+    if (!direct) {
+        const unsigned blocks = (unsigned)((s.n_cells + 255) / 256);
+        if (precision == ZTM_FP64)
+            hipLaunchKernelGGL(pm_adj_reduce_kernel<double>, dim3(blocks), dim3(256), 0,
+                               stream, part, splits, s.n_cells, scale, (double*)out,
+                               (const float*)nullptr, na);
+        else
+            hipLaunchKernelGGL(pm_adj_reduce_kernel<float>, dim3(blocks), dim3(256), 0,
+                               stream, part, splits, s.n_cells, scale, (float*)out,
+                               (const float*)r, na);
+        ZTM_HIP(hipGetLastError());
+    }
+    return ZTM_OK;
+}
+
+}  // namespace ztm

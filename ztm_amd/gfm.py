@@ -1,0 +1,251 @@
+"""Python host mirror of libztm_gfm's C ABI (include/ztm_gfm.h).
+
+The reference's gravity path is called three ways, and each has a mirror here:
+
+  GF(bfr, di, dj, dk, fh, fv, g, rho, s)
+      the CUDA kernel's call (GFM_GPU.cu:16, launch :31) on device tensors;
+      g is accumulated into at interior stations.
+  gfm_host(bfr, di, dj, dk, fh, fv, rho, s) -> g
+      the GFM program body (GFM_GPU.cu:24-34) on host float32 arrays.
+  Plan(grid, stations, ...).forward(rho) / .adjoint(r)
+      the CPU gravity lambda (TFM.cpp:95-99) generalised to explicit geometry,
+      FP32/FP64 and its transpose.
+
+Everything runs through libztm_gfm.so; if it is missing or fails to load this
+module raises -- there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_PKG, "libztm_gfm.so")
+
+FP32, FP64 = 0, 1
+POINT_MASS, PRISM = 0, 1
+KAPPA_SI = 6.674e-6
+KAPPA_REF = 66.7
+_ERRNAMES = {1: "EINVAL", 2: "EHIP", 3: "ENOMEM", 4: "ENODEV", 5: "EIO"}
+
+
+class ZtmError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"ztm {_ERRNAMES.get(code, code)}: {msg}")
+        self.code = code
+
+
+class CGrid(ctypes.Structure):
+    _fields_ = [("nx", ctypes.c_int32), ("ny", ctypes.c_int32), ("nz", ctypes.c_int32),
+                ("x0", ctypes.c_double), ("y0", ctypes.c_double), ("z0", ctypes.c_double),
+                ("dx", ctypes.c_double), ("dy", ctypes.c_double), ("dz", ctypes.c_double)]
+
+
+class CProblem(ctypes.Structure):
+    _fields_ = [("grid", CGrid), ("formula", ctypes.c_int32), ("kappa", ctypes.c_double),
+                ("threshold", ctypes.c_double)]
+
+
+# name -> (restype, argtypes); every symbol declared in include/ztm_gfm.h.
+_P, _I, _I64, _D = ctypes.c_void_p, ctypes.c_int, ctypes.c_int64, ctypes.c_double
+_IP = ctypes.POINTER(ctypes.c_int)
+SIGNATURES = {
+    "ztm_version": (_I, []),
+    "ztm_strerror": (ctypes.c_char_p, [_I]),
+    "ztm_last_error": (ctypes.c_char_p, []),
+    "ztm_init": (_I, []),
+    "ztm_gf": (_I, [_I, _I, _I, _I, _I, _I, _P, _P, _P]),
+    "ztm_gf_async": (_I, [_I, _I, _I, _I, _I, _I, _P, _P, _P, _P]),
+    "ztm_gfm_host": (_I, [_I, _I, _I, _I, _I, _I, _P, _P, _I64, _P]),
+    "ztm_plan_create": (_I, [ctypes.POINTER(_P), ctypes.POINTER(CProblem), _I64, _P]),
+    "ztm_plan_destroy": (_I, [_P]),
+    "ztm_plan_num_cells": (_I64, [_P]),
+    "ztm_plan_num_stations": (_I64, [_P]),
+    "ztm_plan_forward": (_I, [_P, _I, _P, _P, _P]),
+    "ztm_plan_adjoint": (_I, [_P, _I, _P, _P, _P]),
+    "ztm_plan_set_tuning": (_I, [_P, _I, _I, _I, _I]),
+    "ztm_plan_get_tuning": (_I, [_P, _IP, _IP, _IP, _IP]),
+    "ztm_profile_enable": (_I, [_I]),
+    "ztm_profile_read": (_I, [ctypes.c_char_p, ctypes.POINTER(_D), ctypes.POINTER(_I64)]),
+    "ztm_profile_reset": (_I, []),
+}
+
+_lib = None
+
+
+def lib():
+    """Load libztm_gfm.so (raises if it is not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is not built; run `python -m ztm_amd.build`")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc: int) -> None:
+    if rc != 0:
+        raise ZtmError(rc, lib().ztm_last_error().decode())
+
+
+def _stream(stream) -> int | None:
+    if stream is None:
+        import torch
+        return torch.cuda.current_stream().cuda_stream
+    if isinstance(stream, int):
+        return stream
+    return stream.cuda_stream
+
+
+def _ptr(t) -> int:
+    return t.data_ptr()
+
+
+# ------------------------------------------------------- reference mirrors
+def GF(bfr, di, dj, dk, fh, fv, g, rho, s, stream=None) -> None:
+    """GFM_GPU.cu:16 on device float32 tensors; g += interior-station gravity."""
+    import torch
+    n = ((di - 1) * fh + 1) * ((dj - 1) * fh + 1) * ((dk - 1) * fv + 1)
+    for name, t, size in (("g", g, di * dj), ("rho", rho, n), ("s", s, di * dj)):
+        if not (isinstance(t, torch.Tensor) and t.is_cuda and t.dtype == torch.float32
+                and t.is_contiguous() and t.numel() == size):
+            raise ValueError(f"{name} must be a contiguous float32 CUDA tensor of {size} values")
+    check(lib().ztm_gf_async(bfr, di, dj, dk, fh, fv, _ptr(g), _ptr(rho), _ptr(s),
+                             _stream(stream)))
+
+
+def gfm_host(bfr, di, dj, dk, fh, fv, rho, s) -> np.ndarray:
+    """GFM_GPU.cu:24-34 on host float32 arrays; returns g[di*dj] (float32)."""
+    rho = np.ascontiguousarray(rho, dtype=np.float32)
+    s = np.ascontiguousarray(s, dtype=np.float32)
+    if s.size != di * dj:
+        raise ValueError(f"s needs di*dj = {di * dj} values")
+    g = np.zeros(di * dj, dtype=np.float32)
+    check(lib().ztm_gfm_host(bfr, di, dj, dk, fh, fv, g.ctypes.data, rho.ctypes.data, rho.size,
+                             s.ctypes.data))
+    return g
+
+
+# ------------------------------------------------------------------ plans
+@dataclass
+class Tuning:
+    st_per_lane: int = 0
+    rows_per_slice: int = 0
+    cells_per_lane: int = 0
+    adj_splits: int = 0
+
+
+class Plan:
+    """One grid + station set on the current device (ztm_plan_*)."""
+
+    def __init__(self, grid, stations, kappa=KAPPA_SI, threshold=0.0, formula=POINT_MASS):
+        st = np.ascontiguousarray(stations, dtype=np.float64).reshape(-1, 3)
+        prob = CProblem(CGrid(grid.nx, grid.ny, grid.nz, grid.x0, grid.y0, grid.z0,
+                              grid.dx, grid.dy, grid.dz), formula, kappa, threshold)
+        h = ctypes.c_void_p()
+        check(lib().ztm_plan_create(ctypes.byref(h), ctypes.byref(prob), st.shape[0],
+                                    st.ctypes.data if st.size else None))
+        self._h = h
+        self.grid = grid
+        self.n_stations = st.shape[0]
+        self.n_cells = grid.nx * grid.ny * grid.nz
+
+    def close(self) -> None:
+        if getattr(self, "_h", None) is not None and self._h.value:
+            lib().ztm_plan_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @property
+    def handle(self) -> int:
+        return self._h.value
+
+    def set_tuning(self, st_per_lane=0, rows_per_slice=0, cells_per_lane=0, adj_splits=0):
+        check(lib().ztm_plan_set_tuning(self._h, st_per_lane, rows_per_slice, cells_per_lane,
+                                        adj_splits))
+
+    def tuning(self) -> Tuning:
+        v = [ctypes.c_int() for _ in range(4)]
+        check(lib().ztm_plan_get_tuning(self._h, *[ctypes.byref(x) for x in v]))
+        return Tuning(*[x.value for x in v])
+
+    # raw device-pointer calls (stream-ordered)
+    def forward_ptr(self, precision: int, rho_ptr: int, g_ptr: int, stream: int | None) -> None:
+        check(lib().ztm_plan_forward(self._h, precision, rho_ptr, g_ptr, stream))
+
+    def adjoint_ptr(self, precision: int, r_ptr: int, out_ptr: int, stream: int | None) -> None:
+        check(lib().ztm_plan_adjoint(self._h, precision, r_ptr, out_ptr, stream))
+
+    # torch-tensor conveniences
+    def forward(self, rho, out=None, stream=None):
+        """g[M] (float64) = kappa * sum_n rho[n] K(m,n); rho float32 -> FP32 path,
+        float64 -> FP64 path."""
+        import torch
+        prec = _prec_of(rho, self.n_cells, "rho")
+        if out is None:
+            out = torch.empty(self.n_stations, dtype=torch.float64, device=rho.device)
+        _check_t(out, torch.float64, self.n_stations, "g")
+        self.forward_ptr(prec, _ptr(rho), _ptr(out), _stream(stream))
+        return out
+
+    def adjoint(self, r, out=None, stream=None):
+        """out[N] = kappa * sum_m K(m,n) r[m], in r's precision (float32/float64)."""
+        import torch
+        prec = _prec_of(r, self.n_stations, "r")
+        if out is None:
+            out = torch.empty(self.n_cells, dtype=r.dtype, device=r.device)
+        _check_t(out, r.dtype, self.n_cells, "out")
+        self.adjoint_ptr(prec, _ptr(r), _ptr(out), _stream(stream))
+        return out
+
+
+def _prec_of(t, n: int, name: str) -> int:
+    import torch
+    if not (isinstance(t, torch.Tensor) and t.is_cuda and t.is_contiguous() and t.numel() == n):
+        raise ValueError(f"{name} must be a contiguous CUDA tensor of {n} values")
+    if t.dtype == torch.float32:
+        return FP32
+    if t.dtype == torch.float64:
+        return FP64
+    raise ValueError(f"{name} must be float32 or float64, got {t.dtype}")
+
+
+def _check_t(t, dtype, n: int, name: str) -> None:
+    import torch
+    if not (isinstance(t, torch.Tensor) and t.is_cuda and t.is_contiguous()
+            and t.dtype == dtype and t.numel() == n):
+        raise ValueError(f"{name} must be a contiguous {dtype} CUDA tensor of {n} values")
+
+
+# --------------------------------------------------------------- profiling
+def profile_enable(on: bool = True) -> None:
+    check(lib().ztm_profile_enable(1 if on else 0))
+
+
+def profile_read(kernel: str) -> tuple[float, int]:
+    ms, n = ctypes.c_double(), ctypes.c_int64()
+    check(lib().ztm_profile_read(kernel.encode(), ctypes.byref(ms), ctypes.byref(n)))
+    return ms.value, n.value
+
+
+def profile_reset() -> None:
+    check(lib().ztm_profile_reset())

@@ -1,0 +1,186 @@
+"""Seeded synthetic gravity workloads (grids, stations, density models).
+
+Every random draw comes from a counter-based SplitMix64 stream, so any value
+can be regenerated from (seed, stream, index) alone and the host and any test
+see identical inputs. Seeds are 20190301 + workload id (pm_cpu_small 1,
+pm_blobs_1m 2, adjoint_small 3, ensemble_256 4, uneven_48 5).
+
+Coordinates are metres with z up; cells of edge h fill x in [0, nx h],
+y in [0, ny h], z in [-nz h, 0] (top at z = 0); density contrast in kg/m^3;
+the gravity output is in mGal (kappa = 1e5 * G).
+
+    pm_cpu_small   32x32x16 cells of 1 km, 1,024 stations at z=+10 m, FP64
+                   U[-300,300] densities (the CPU-runnable case)
+    pm_blobs_1m    128x128x64 cells of 500 m (1,048,576), 8 Gaussian blobs
+                   (+-400 kg/m^3, sigma 2-8 km) + N(0,20) noise; 16,384 stations
+                   on a 128x128 grid, heights 0-2,000 m from 5 Gaussian hills
+    adjoint_small  pm_cpu_small geometry, residual N(0, 0.5) mGal, lambda 1e-3
+    ensemble_256   96x96x48 cells of 1 km, 256 models of U[-300,300]
+                   smoothed by a clamped 3x3x3 box filter; 9,216 stations, z=+10
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+SEED0 = 20190301
+G_SI = 6.674e-11
+KAPPA_SI = 1e5 * G_SI          # mGal per (kg/m^3 * m^3 / m^2)
+
+_M1 = np.uint64(0x9E3779B97F4A7C15)
+_M2 = np.uint64(0xBF58476D1CE4E5B9)
+_M3 = np.uint64(0x94D049BB133111EB)
+
+
+def _splitmix(x: np.ndarray) -> np.ndarray:
+    z = x + _M1
+    z = (z ^ (z >> np.uint64(30))) * _M2
+    z = (z ^ (z >> np.uint64(27))) * _M3
+    return z ^ (z >> np.uint64(31))
+
+
+def uniform(seed: int, stream: int, n: int, lo: float = 0.0, hi: float = 1.0) -> np.ndarray:
+    """n uniforms in [lo, hi) from SplitMix64(seed, stream, index)."""
+    base = np.uint64((seed * 0x100000001B3 + stream * 0x9E3779B1) & 0xFFFFFFFFFFFFFFFF)
+    idx = np.arange(n, dtype=np.uint64) * _M1
+    with np.errstate(over="ignore"):
+        bits = _splitmix(idx + base)
+    u = (bits >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+    return lo + (hi - lo) * u
+
+
+def normal(seed: int, stream: int, n: int, sigma: float = 1.0) -> np.ndarray:
+    """n normals N(0, sigma^2) by Box-Muller on two uniform streams."""
+    u1 = uniform(seed, 2 * stream + 1000, n)
+    u2 = uniform(seed, 2 * stream + 1001, n)
+    return sigma * np.sqrt(-2.0 * np.log1p(-u1)) * np.cos(2.0 * np.pi * u2)
+
+
+@dataclass
+class Grid:
+    nx: int
+    ny: int
+    nz: int
+    x0: float
+    y0: float
+    z0: float
+    dx: float
+    dy: float
+    dz: float
+
+    @property
+    def n_cells(self) -> int:
+        return self.nx * self.ny * self.nz
+
+    def centres(self) -> np.ndarray:
+        """[N,3] cell centres in raster order n = i + nx*(j + ny*k)."""
+        k, j, i = np.meshgrid(np.arange(self.nz), np.arange(self.ny), np.arange(self.nx),
+                              indexing="ij")
+        return np.stack([self.x0 + i.ravel() * self.dx, self.y0 + j.ravel() * self.dy,
+                         self.z0 + k.ravel() * self.dz], axis=1)
+
+
+def cube_grid(nx: int, ny: int, nz: int, h: float) -> Grid:
+    """Cubic cells of edge h, top face at z = 0, k counting downwards."""
+    return Grid(nx, ny, nz, 0.5 * h, 0.5 * h, -0.5 * h, h, h, -h)
+
+
+@dataclass
+class Workload:
+    name: str
+    grid: Grid
+    stations: np.ndarray            # [M,3] float64
+    rho: np.ndarray                 # [N] or [B,N] float64
+    kappa: float = KAPPA_SI
+    extra: dict = field(default_factory=dict)
+
+    @property
+    def n_stations(self) -> int:
+        return int(self.stations.shape[0])
+
+    @property
+    def pairs(self) -> int:
+        return self.n_stations * self.grid.n_cells
+
+
+def _station_grid(g: Grid, z: np.ndarray | float) -> np.ndarray:
+    j, i = np.meshgrid(np.arange(g.ny), np.arange(g.nx), indexing="ij")
+    x = g.x0 + i.ravel() * g.dx
+    y = g.y0 + j.ravel() * g.dy
+    zz = np.broadcast_to(np.asarray(z, dtype=np.float64), x.shape).astype(np.float64)
+    return np.stack([x, y, zz], axis=1)
+
+
+def pm_cpu_small(seed: int = SEED0 + 1) -> Workload:
+    g = cube_grid(32, 32, 16, 1000.0)
+    rho = uniform(seed, 1, g.n_cells, -300.0, 300.0)
+    return Workload("pm_cpu_small", g, _station_grid(g, 10.0), rho)
+
+
+def _hills(seed: int, x: np.ndarray, y: np.ndarray, extent: float) -> np.ndarray:
+    u = uniform(seed, 20, 5 * 4).reshape(5, 4)
+    z = np.zeros_like(x)
+    for a, s, cx, cy in u:
+        amp = 500.0 + 1000.0 * a
+        sig = extent * (0.05 + 0.15 * s)
+        z += amp * np.exp(-((x - cx * extent) ** 2 + (y - cy * extent) ** 2) / (2 * sig * sig))
+    return np.clip(z, 0.0, 2000.0)
+
+
+def pm_blobs_1m(seed: int = SEED0 + 2, n: int = 128, nz: int = 64, h: float = 500.0) -> Workload:
+    g = cube_grid(n, n, nz, h)
+    ex, ez = n * h, nz * h
+    u = uniform(seed, 10, 8 * 6).reshape(8, 6)
+    c = g.centres()
+    rho = np.zeros(g.n_cells)
+    for sgn, sig, cx, cy, cz, _ in u:
+        amp = 400.0 if sgn < 0.5 else -400.0
+        s = 2000.0 + 6000.0 * sig
+        d2 = (c[:, 0] - cx * ex) ** 2 + (c[:, 1] - cy * ex) ** 2 + (c[:, 2] + cz * ez) ** 2
+        rho += amp * np.exp(-d2 / (2.0 * s * s))
+    rho += normal(seed, 11, g.n_cells, 20.0)
+    st = _station_grid(g, 0.0)
+    st[:, 2] = _hills(seed, st[:, 0], st[:, 1], ex)
+    return Workload("pm_blobs_1m", g, st, rho)
+
+
+def adjoint_small(seed: int = SEED0 + 3) -> Workload:
+    w = pm_cpu_small(SEED0 + 1)
+    r = normal(seed, 30, w.n_stations, 0.5)
+    return Workload("adjoint_small", w.grid, w.stations, w.rho,
+                    extra={"residual": r, "lambda": 1e-3})
+
+
+def _box3(field3: np.ndarray) -> np.ndarray:
+    """3x3x3 mean with clamped (edge-replicated) borders over the last 3 axes."""
+    out = field3
+    for ax in (-1, -2, -3):
+        p = np.concatenate([np.take(out, [0], axis=ax), out, np.take(out, [-1], axis=ax)],
+                           axis=ax)
+        n = out.shape[ax]
+        sl = [slice(None)] * out.ndim
+        acc = 0.0
+        for o in range(3):
+            sl[ax] = slice(o, o + n)
+            acc = acc + p[tuple(sl)]
+        out = acc / 3.0
+    return out
+
+
+def ensemble_256(seed: int = SEED0 + 4, models: int = 256, n: int = 96, nz: int = 48,
+                 h: float = 1000.0, dtype=np.float32) -> Workload:
+    g = cube_grid(n, n, nz, h)
+    rho = np.empty((models, g.n_cells), dtype=dtype)
+    for b in range(models):
+        f = uniform(seed, 100 + b, g.n_cells, -300.0, 300.0).reshape(nz, n, n)
+        rho[b] = _box3(f).ravel()
+    return Workload("ensemble_256", g, _station_grid(g, 10.0), rho)
+
+
+WORKLOADS = {
+    "pm_cpu_small": pm_cpu_small,
+    "pm_blobs_1m": pm_blobs_1m,
+    "adjoint_small": adjoint_small,
+    "ensemble_256": ensemble_256,
+}
