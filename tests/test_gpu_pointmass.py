@@ -26,18 +26,7 @@ def relerr(a, b):
 
 
 def ref_case(seed=7, di=61, dj=61, dk=33, fh=1, fv=2):
-    """Reference-shaped inputs: TFM's grid (TFM.cpp:190) rendered at fh=1, fv=2."""
-    ii, jj, kk = (di - 1) * fh + 1, (dj - 1) * fh + 1, (dk - 1) * fv + 1
-    n = ii * jj * kk
-    rho = W.uniform(seed, 1, n, -0.3, 0.3)
-    # a third of the cells sit at or under the .001 g/cc threshold (GFM_GPU.cu:19)
-    small = W.uniform(seed, 2, n) < 0.33
-    rho[small] = W.uniform(seed, 3, int(small.sum()), -0.001, 0.001)
-    rho = rho.astype(np.float32)
-    rho[5] = np.float32(0.001)            # exactly at the threshold: skipped
-    rho[6] = np.float32(-0.001)
-    s = W.uniform(seed, 4, di * dj, kk / fv - 4.0, kk / fv + 1.5).astype(np.float32)
-    return rho, s
+    return W.tfm_shaped(seed, di, dj, dk, fh, fv)
 
 
 # ------------------------------------------------------------ reference ABI

@@ -178,6 +178,25 @@ def ensemble_256(seed: int = SEED0 + 4, models: int = 256, n: int = 96, nz: int 
     return Workload("ensemble_256", g, _station_grid(g, 10.0), rho)
 
 
+def tfm_shaped(seed: int = 7, di: int = 61, dj: int = 61, dk: int = 33, fh: int = 1,
+               fv: int = 2):
+    """Inputs of the reference's own gravity call (TFM.cpp:83-94, GFM_GPU.cu:24):
+    float32 drho[ii*jj*kk] in g/cc on the fine raster, a third of it at or
+    below the .001 g/cc skip threshold (two cells exactly at +-.001), and
+    float32 station heights s[di*dj] in k-index units around the grid top.
+    Defaults are TFM's grid (di=dj=61, dk=33, fh=1, fv=2; TFM.cpp:190)."""
+    ii, jj, kk = (di - 1) * fh + 1, (dj - 1) * fh + 1, (dk - 1) * fv + 1
+    n = ii * jj * kk
+    rho = uniform(seed, 1, n, -0.3, 0.3)
+    small = uniform(seed, 2, n) < 0.33
+    rho[small] = uniform(seed, 3, int(small.sum()), -0.001, 0.001)
+    rho = rho.astype(np.float32)
+    rho[5] = np.float32(0.001)
+    rho[6] = np.float32(-0.001)
+    s = uniform(seed, 4, di * dj, kk / fv - 4.0, kk / fv + 1.5).astype(np.float32)
+    return rho, s
+
+
 WORKLOADS = {
     "pm_cpu_small": pm_cpu_small,
     "pm_blobs_1m": pm_blobs_1m,
