@@ -141,6 +141,15 @@ int ztm_plan_forward(ztm_plan plan, int precision, const void* rho_dev,
 int ztm_plan_adjoint(ztm_plan plan, int precision, const void* r_dev,
                      void* out_dev, void* stream);
 
+/* Smoothness operator of the regularised misfit 0.5||G rho-d||^2 +
+ * 0.5 lambda ||grad rho||^2 (forward differences on each axis, none across
+ * the boundary): out_dev = L^T L rho, so the misfit gradient is
+ * G^T (G rho - d) + lambda * out. Same precision convention as the adjoint
+ * (float or double rho/out). No reference counterpart (TFM.cpp:124-135 uses
+ * a finite-difference Jacobian and no smoothness prior). */
+int ztm_plan_smoothness(ztm_plan plan, int precision, const void* rho_dev,
+                        void* out_dev, void* stream);
+
 /* Tuning of the point-mass kernels (0 = keep the plan's current value).
  * st_per_lane: stations held in registers per lane of the forward kernel
  *              (1, 2, 4 or 8); rows_per_slice: grid rows (fixed j,k) staged in

@@ -22,7 +22,7 @@ OBJ = os.path.join(PKG, "_build")
 LIB = os.path.join(PKG, "libztm_gfm.so")
 GFM = os.path.join(PKG, "bin", "GFM")
 
-HIP_SOURCES = ["gfm_abi.hip", "gfm_pointmass.hip"]
+HIP_SOURCES = ["gfm_abi.hip", "gfm_pointmass.hip", "gfm_reg.hip"]
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 HIPCC = os.path.join(ROCM, "bin", "hipcc")
 HIP_FLAGS = [
@@ -48,8 +48,35 @@ def _compile(src: str) -> str:
     return obj
 
 
-def build(verbose: bool = True) -> str:
-    """Compile every HIP source for gfx950 and link the library and GFM."""
+def torch_ext_path() -> str:
+    import sysconfig
+    return os.path.join(PKG, "_ztm_torch" + sysconfig.get_config_var("EXT_SUFFIX"))
+
+
+def _build_torch_ext() -> str:
+    """libtorch autograd bindings: plain g++ against torch's headers and
+    libraries (the HIP stream comes from c10/hip), linked to libztm_gfm."""
+    import sysconfig
+    import torch
+    tdir = os.path.dirname(torch.__file__)
+    tinc, tlib = os.path.join(tdir, "include"), os.path.join(tdir, "lib")
+    abi = int(torch._C._GLIBCXX_USE_CXX11_ABI)
+    out = torch_ext_path()
+    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-w",
+          os.path.join(CSRC, "gfm_torch.cpp"), "-o", out + ".tmp",
+          "-I" + tinc, "-I" + os.path.join(tinc, "torch", "csrc", "api", "include"),
+          "-I" + sysconfig.get_paths()["include"], "-I" + os.path.join(ROCM, "include"),
+          "-D__HIP_PLATFORM_AMD__=1", "-DUSE_ROCM=1", "-DTORCH_EXTENSION_NAME=_ztm_torch",
+          "-DTORCH_API_INCLUDE_EXTENSION_H", f"-D_GLIBCXX_USE_CXX11_ABI={abi}",
+          "-L" + tlib, "-lc10", "-lc10_hip", "-ltorch", "-ltorch_cpu", "-ltorch_python",
+          "-L" + PKG, "-lztm_gfm", "-Wl,-rpath,$ORIGIN", "-Wl,-rpath," + tlib])
+    os.replace(out + ".tmp", out)
+    return out
+
+
+def build(verbose: bool = True, torch_ext: bool = True) -> str:
+    """Compile every HIP source for gfx950 and link the library, the GFM
+    program and the libtorch bindings."""
     os.makedirs(OBJ, exist_ok=True)
     os.makedirs(os.path.dirname(GFM), exist_ok=True)
     with ThreadPoolExecutor(max_workers=min(8, len(HIP_SOURCES))) as ex:
@@ -60,8 +87,11 @@ def build(verbose: bool = True) -> str:
     _run(["g++", "-O2", "-std=c++17", "-Wall", "-o", GFM, os.path.join(CSRC, "gfm_main.cpp"),
           "-L" + PKG, "-lztm_gfm", "-Wl,-rpath,$ORIGIN/..",
           "-Wl,-rpath," + os.path.join(ROCM, "lib")])
+    built = [LIB, GFM]
+    if torch_ext:
+        built.append(_build_torch_ext())
     if verbose:
-        print(f"built {os.path.relpath(LIB, ROOT)} and {os.path.relpath(GFM, ROOT)}")
+        print("built " + ", ".join(os.path.relpath(p, ROOT) for p in built))
     return LIB
 
 

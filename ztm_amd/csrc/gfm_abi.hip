@@ -391,6 +391,17 @@ int ztm_plan_adjoint(ztm_plan plan, int precision, const void* r_dev, void* out_
                          (hipStream_t)stream);
 }
 
+int ztm_plan_smoothness(ztm_plan plan, int precision, const void* rho_dev, void* out_dev,
+                        void* stream) {
+    if (!plan) return fail(ZTM_EINVAL, "null plan");
+    if (precision != ZTM_FP32 && precision != ZTM_FP64)
+        return fail(ZTM_EINVAL, "unknown precision %d", precision);
+    if (!rho_dev || !out_dev) return fail(ZTM_EINVAL, "null rho or out");
+    if (rho_dev == out_dev) return fail(ZTM_EINVAL, "rho and out must not alias");
+    return ltl_launch(precision, rho_dev, out_dev, plan->shape.nx, plan->shape.ny,
+                      plan->shape.nz, (hipStream_t)stream);
+}
+
 // ------------------------------------------------ reference-signature ABI
 namespace {
 using RefKey = std::tuple<int, int, int, int, int, int, int>;   // dev,bfr,di,dj,dk,fh,fv

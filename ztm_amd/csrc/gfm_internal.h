@@ -103,6 +103,11 @@ int prep_ref_stations(const float* s, int bfr, int di, int dj, int fh, int fv,
                       float4* st, double4* st_d, int4* st_c, int* out_map,
                       int m_int, int m_pad, hipStream_t stream);
 
+// out = L^T L rho for forward-difference gradients on an nx x ny x nz grid
+// (gfm_reg.hip).
+int ltl_launch(int precision, const void* rho, void* out, int nx, int ny, int nz,
+               hipStream_t stream);
+
 // Dominant-kernel timing (gfm_abi.hip).
 struct ProfScope {
     ProfScope(const char* name, hipStream_t s);

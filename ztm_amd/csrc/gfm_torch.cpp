@@ -1,0 +1,112 @@
+// libtorch autograd bindings of libztm_gfm (module ztm_amd._ztm_torch).
+//
+// GravityFn: g = G rho through ztm_plan_forward, with backward G^T (dL/dg)
+//   through the hand-written adjoint kernel (ztm_plan_adjoint), so a
+//   nonlinear or regularised inversion gets exact misfit gradients from
+//   torch.autograd. ZTM itself differentiates by finite differences, one full
+//   forward model per parameter (Jac, TFM.cpp:124-135).
+// SmoothnessFn: s = 0.5 ||grad rho||^2 (forward differences), backward
+//   (L^T L rho) * ds, through ztm_plan_smoothness.
+//
+// Both run on the current HIP stream of rho's device, so they compose with
+// torch work and with stream capture the same way torch's own ops do.
+#include <torch/extension.h>
+#include <c10/hip/HIPStream.h>
+
+#include "../../include/ztm_gfm.h"
+
+namespace {
+
+using torch::autograd::AutogradContext;
+using torch::autograd::variable_list;
+
+void check(int rc) {
+    TORCH_CHECK(rc == ZTM_OK, "libztm_gfm: ", ztm_strerror(rc), ": ", ztm_last_error());
+}
+
+int precision_of(const torch::Tensor& t, int64_t n, const char* name) {
+    TORCH_CHECK(t.is_cuda(), name, " must be a GPU tensor");
+    TORCH_CHECK(t.is_contiguous(), name, " must be contiguous");
+    TORCH_CHECK(t.numel() == n, name, " must have ", n, " elements, got ", t.numel());
+    if (t.scalar_type() == torch::kFloat32) return ZTM_FP32;
+    TORCH_CHECK(t.scalar_type() == torch::kFloat64, name, " must be float32 or float64");
+    return ZTM_FP64;
+}
+
+void* stream_of(const torch::Tensor& t) {
+    return (void*)c10::hip::getCurrentHIPStream(t.device().index()).stream();
+}
+
+ztm_plan as_plan(int64_t h) {
+    TORCH_CHECK(h != 0, "null plan handle");
+    return reinterpret_cast<ztm_plan>(h);
+}
+
+class GravityFn : public torch::autograd::Function<GravityFn> {
+  public:
+    static torch::Tensor forward(AutogradContext* ctx, torch::Tensor rho, int64_t plan) {
+        ztm_plan p = as_plan(plan);
+        const int64_t n = ztm_plan_num_cells(p), m = ztm_plan_num_stations(p);
+        const int prec = precision_of(rho, n, "rho");
+        auto g = torch::empty({m}, rho.options().dtype(torch::kFloat64));
+        check(ztm_plan_forward(p, prec, rho.data_ptr(), g.data_ptr<double>(), stream_of(rho)));
+        ctx->saved_data["plan"] = plan;
+        ctx->saved_data["dtype"] = static_cast<int64_t>(rho.scalar_type());
+        return g;
+    }
+
+    static variable_list backward(AutogradContext* ctx, variable_list grads) {
+        ztm_plan p = as_plan(ctx->saved_data["plan"].toInt());
+        const auto dtype = static_cast<torch::ScalarType>(ctx->saved_data["dtype"].toInt());
+        const int64_t n = ztm_plan_num_cells(p), m = ztm_plan_num_stations(p);
+        auto r = grads[0].to(dtype).contiguous();
+        const int prec = precision_of(r, m, "grad_g");
+        auto out = torch::empty({n}, r.options());
+        check(ztm_plan_adjoint(p, prec, r.data_ptr(), out.data_ptr(), stream_of(r)));
+        return {out, torch::Tensor()};
+    }
+};
+
+class SmoothnessFn : public torch::autograd::Function<SmoothnessFn> {
+  public:
+    static torch::Tensor forward(AutogradContext* ctx, torch::Tensor rho, int64_t plan) {
+        ztm_plan p = as_plan(plan);
+        const int prec = precision_of(rho, ztm_plan_num_cells(p), "rho");
+        auto ltl = torch::empty_like(rho);
+        check(ztm_plan_smoothness(p, prec, rho.data_ptr(), ltl.data_ptr(), stream_of(rho)));
+        ctx->save_for_backward({ltl});
+        // 0.5 ||L rho||^2 = 0.5 rho . (L^T L rho), accumulated in FP64
+        return 0.5 * torch::dot(rho.to(torch::kFloat64), ltl.to(torch::kFloat64));
+    }
+
+    static variable_list backward(AutogradContext* ctx, variable_list grads) {
+        auto ltl = ctx->get_saved_variables()[0];
+        return {ltl * grads[0].to(ltl.scalar_type()), torch::Tensor()};
+    }
+};
+
+torch::Tensor gravity(torch::Tensor rho, int64_t plan) { return GravityFn::apply(rho, plan); }
+
+torch::Tensor smoothness(torch::Tensor rho, int64_t plan) {
+    return SmoothnessFn::apply(rho, plan);
+}
+
+// G^T r without autograd (the adjoint on its own).
+torch::Tensor adjoint(torch::Tensor r, int64_t plan) {
+    ztm_plan p = as_plan(plan);
+    const int prec = precision_of(r, ztm_plan_num_stations(p), "r");
+    auto out = torch::empty({ztm_plan_num_cells(p)}, r.options());
+    check(ztm_plan_adjoint(p, prec, r.data_ptr(), out.data_ptr(), stream_of(r)));
+    return out;
+}
+
+}  // namespace
+
+PYBIND11_MODULE(_ztm_torch, m) {
+    m.doc() = "libtorch autograd bindings of libztm_gfm (gravity forward/adjoint)";
+    m.def("gravity", &gravity, "g = G rho (FP64 out); differentiable, backward = G^T grad",
+          pybind11::arg("rho"), pybind11::arg("plan"));
+    m.def("smoothness", &smoothness, "0.5 ||grad rho||^2; differentiable",
+          pybind11::arg("rho"), pybind11::arg("plan"));
+    m.def("adjoint", &adjoint, "G^T r", pybind11::arg("r"), pybind11::arg("plan"));
+}

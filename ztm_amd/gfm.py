@@ -66,6 +66,7 @@ SIGNATURES = {
     "ztm_plan_num_stations": (_I64, [_P]),
     "ztm_plan_forward": (_I, [_P, _I, _P, _P, _P]),
     "ztm_plan_adjoint": (_I, [_P, _I, _P, _P, _P]),
+    "ztm_plan_smoothness": (_I, [_P, _I, _P, _P, _P]),
     "ztm_plan_set_tuning": (_I, [_P, _I, _I, _I, _I]),
     "ztm_plan_get_tuning": (_I, [_P, _IP, _IP, _IP, _IP]),
     "ztm_profile_enable": (_I, [_I]),
@@ -205,6 +206,16 @@ class Plan:
             out = torch.empty(self.n_stations, dtype=torch.float64, device=rho.device)
         _check_t(out, torch.float64, self.n_stations, "g")
         self.forward_ptr(prec, _ptr(rho), _ptr(out), _stream(stream))
+        return out
+
+    def smoothness(self, rho, out=None, stream=None):
+        """out[N] = L^T L rho (gradient of 0.5 ||grad rho||^2), in rho's precision."""
+        import torch
+        prec = _prec_of(rho, self.n_cells, "rho")
+        if out is None:
+            out = torch.empty_like(rho)
+        _check_t(out, rho.dtype, self.n_cells, "out")
+        check(lib().ztm_plan_smoothness(self._h, prec, _ptr(rho), _ptr(out), _stream(stream)))
         return out
 
     def adjoint(self, r, out=None, stream=None):
