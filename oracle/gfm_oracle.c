@@ -304,11 +304,18 @@ void oracle_pm_forward_cells(const og_grid* g, double kappa, const double* rho,
  * equals the integral of w/r^3 over the box (Nagy et al. 2000), so the
  * integral of (z_m - z)/r^3 = -w/r^3 is minus that sum.
  * Terms whose prefactor is exactly 0 are dropped (their log may diverge). */
+/* ln(v + r) for r = |(u,v,w)|; for v < 0, v + r = (u^2 + w^2)/(r - v) avoids
+ * the cancellation of r and -v (stations beside the box, far away). */
+static double log_vr(double v, double r, double rest2) {
+    return v >= 0.0 ? log(v + r) : log(rest2 / (r - v));
+}
+
 static double prism_F(double u, double v, double w) {
-    const double r = sqrt(u * u + v * v + w * w);
+    const double u2 = u * u, v2 = v * v, w2 = w * w;
+    const double r = sqrt(u2 + v2 + w2);
     double f = 0.0;
-    if (u != 0.0) { const double a = v + r; f += u * log(a > 0.0 ? a : 1e-300); }
-    if (v != 0.0) { const double a = u + r; f += v * log(a > 0.0 ? a : 1e-300); }
+    if (u != 0.0) f += u * log_vr(v, r, u2 + w2);
+    if (v != 0.0) f += v * log_vr(u, r, v2 + w2);
     if (w != 0.0) f -= w * atan(u * v / (w * r));
     return f;
 }

@@ -164,6 +164,9 @@ int set_shape(ztm_plan p, int64_t m) {
 
 NearArgs near_args(const ztm_plan p) {
     NearArgs a;
+    const bool prism = p->prob.formula == ZTM_PRISM;
+    a.mode = prism ? kCorrPrism : kCorrPmNear;
+    a.radius = prism ? kMid : kNear;
     a.st_f = p->st_f;
     a.st_d = p->st_d;
     a.st_c = p->st_c;
@@ -177,8 +180,23 @@ NearArgs near_args(const ztm_plan p) {
     a.thr = (float)p->prob.threshold;
     a.syd = p->sy;
     a.szd = p->sz;
+    a.vol = std::fabs(p->sy * p->sz);
     a.kmin = p->kmin;
     a.kmax = p->kmax;
+    // Quadrupole term of the far field for non-cubic prism cells (half-widths
+    // in cell units); it vanishes identically for cubes.
+    const double hx = 0.5, hy = 0.5 * std::fabs(p->sy), hz = 0.5 * std::fabs(p->sz);
+    const bool cubic = std::fabs(hy - hx) <= 1e-12 && std::fabs(hz - hx) <= 1e-12;
+    a.quad = (prism && !cubic) ? 1 : 0;
+    if (a.quad) {
+        const double qxx = (2 * hx * hx - hy * hy - hz * hz) / 3.0;
+        const double qyy = (2 * hy * hy - hx * hx - hz * hz) / 3.0;
+        const double qzz = (2 * hz * hz - hx * hx - hy * hy) / 3.0;
+        a.q.q2x = (float)(2.5 * qxx);
+        a.q.q2y = (float)(2.5 * qyy);
+        a.q.q2z = (float)(2.5 * qzz);
+        a.q.qz = (float)qzz;
+    }
     return a;
 }
 
@@ -218,8 +236,6 @@ int ztm_plan_create(ztm_plan* plan, const ztm_problem* problem, int64_t n_statio
     if (int rc = grid_check(problem->grid)) return rc;
     if (problem->formula != ZTM_POINT_MASS && problem->formula != ZTM_PRISM)
         return fail(ZTM_EINVAL, "unknown formula %d", problem->formula);
-    if (problem->formula == ZTM_PRISM)
-        return fail(ZTM_EINVAL, "prism formula is not available in this build");
     if (n_stations < 0) return fail(ZTM_EINVAL, "negative station count");
     if (n_stations > 0 && !stations_host) return fail(ZTM_EINVAL, "null stations");
     if (!std::isfinite(problem->kappa)) return fail(ZTM_EINVAL, "kappa must be finite");
@@ -235,7 +251,7 @@ int ztm_plan_create(ztm_plan* plan, const ztm_problem* problem, int64_t n_statio
     std::vector<float4> hf(p->shape.m_pad, make_float4(0.f, 0.f, 0.f, 0.f));
     std::vector<double4> hd(p->shape.m_pad, make_double4(0.0, 0.0, 0.0, 0.0));
     std::vector<int4> hc(p->shape.m_pad, make_int4(0, 0, 0, 0));
-    const int wcols = g.nx + 2 * kNear, hcols = g.ny + 2 * kNear;
+    const int wcols = g.nx + 2 * kColMargin, hcols = g.ny + 2 * kColMargin;
     std::vector<int> col_count((size_t)wcols * hcols + 1, 0);
     for (int64_t m = 0; m < n_stations; ++m) {
         const double x = stations_host[3 * m], y = stations_host[3 * m + 1],
@@ -252,7 +268,7 @@ int ztm_plan_create(ztm_plan* plan, const ztm_problem* problem, int64_t n_statio
         // nearest cell index per axis, clamped far outside the grid so it fits int
         auto idx = [](double v, int n) {
             const double c = std::nearbyint(v);
-            return (int)std::max(-4.0 * kNear - 1.0, std::min(c, (double)n + 4.0 * kNear));
+            return (int)std::max(-(double)kColMargin - 1.0, std::min(c, (double)n + kColMargin));
         };
         hc[m] = make_int4(idx(X, g.nx), idx(Y / p->sy, g.ny), idx(Z / p->sz, g.nz), 0);
     }
@@ -261,7 +277,7 @@ int ztm_plan_create(ztm_plan* plan, const ztm_problem* problem, int64_t n_statio
     std::vector<int> col_of((size_t)n_stations, -1);
     bool any_near = false;
     for (int64_t m = 0; m < n_stations; ++m) {
-        const int ci = hc[m].x + kNear, cj = hc[m].y + kNear;
+        const int ci = hc[m].x + kColMargin, cj = hc[m].y + kColMargin;
         if (ci < 0 || ci >= wcols || cj < 0 || cj >= hcols) continue;
         col_of[m] = cj * wcols + ci;
         ++col_count[col_of[m] + 1];
@@ -357,11 +373,29 @@ int ztm_plan_forward(ztm_plan plan, int precision, const void* rho_dev, double* 
     if (dev != plan->device)
         return fail(ZTM_EINVAL, "plan belongs to device %d, current device is %d", plan->device, dev);
     const PmTuning t = effective_tuning(plan, precision);
+    hipStream_t s = (hipStream_t)stream;
+    if (plan->prob.formula == ZTM_PRISM) {
+        if (precision == ZTM_FP64) {
+            if (int rc = ensure_workspace(plan, prism64_workspace(plan->shape, t, false))) return rc;
+            return prism64_fwd_launch(plan->shape, t, plan->st_d, (const double*)rho_dev, plan->sy,
+                                      plan->sz, plan->prob.threshold, plan->prob.kappa * plan->L,
+                                      plan->part, g_dev, s);
+        }
+        const size_t ws = pm_fwd_workspace(plan->shape, t, precision);
+        if (int rc = ensure_workspace(plan, ws + plan->shape.m_pad)) return rc;
+        NearArgs na = near_args(plan);
+        na.corr = plan->part + ws;
+        if (int rc = prism_near_fwd_launch(plan->shape, na, (const float*)rho_dev, plan->part + ws, s))
+            return rc;
+        return pm_fwd_launch(plan->shape, t, precision, plan->st_f, rho_dev, plan->sy, plan->sz,
+                             plan->prob.threshold, pm_scale(plan), plan->part, g_dev, nullptr,
+                             nullptr, na, s);
+    }
     if (int rc = ensure_workspace(plan, pm_fwd_workspace(plan->shape, t, precision))) return rc;
     const void* st = precision == ZTM_FP64 ? (const void*)plan->st_d : (const void*)plan->st_f;
     return pm_fwd_launch(plan->shape, t, precision, st, rho_dev, plan->sy, plan->sz,
                          plan->prob.threshold, pm_scale(plan), plan->part, g_dev, nullptr,
-                         nullptr, near_args(plan), (hipStream_t)stream);
+                         nullptr, near_args(plan), s);
 }
 
 int ztm_plan_adjoint(ztm_plan plan, int precision, const void* r_dev, void* out_dev,
@@ -375,20 +409,26 @@ int ztm_plan_adjoint(ztm_plan plan, int precision, const void* r_dev, void* out_
     if (dev != plan->device)
         return fail(ZTM_EINVAL, "plan belongs to device %d, current device is %d", plan->device, dev);
     const PmTuning t = effective_tuning(plan, precision);
+    hipStream_t s = (hipStream_t)stream;
+    if (plan->shape.m == 0) {
+        const size_t bytes = (size_t)plan->shape.n_cells *
+                             (precision == ZTM_FP64 ? sizeof(double) : sizeof(float));
+        ZTM_HIP(hipMemsetAsync(out_dev, 0, bytes, s));
+        return ZTM_OK;
+    }
+    if (plan->prob.formula == ZTM_PRISM && precision == ZTM_FP64) {
+        if (int rc = ensure_workspace(plan, prism64_workspace(plan->shape, t, true))) return rc;
+        return prism64_adj_launch(plan->shape, t, plan->st_d, (const double*)r_dev, plan->sy,
+                                  plan->sz, plan->prob.kappa * plan->L, plan->part,
+                                  (double*)out_dev, s);
+    }
     if (int rc = ensure_workspace(
             plan, std::max<size_t>(1, pm_adj_workspace(plan->shape, t, precision,
                                                        precision == ZTM_FP32))))
         return rc;
     const void* st = precision == ZTM_FP64 ? (const void*)plan->st_d : (const void*)plan->st_f;
-    if (plan->shape.m == 0) {
-        const size_t bytes = (size_t)plan->shape.n_cells *
-                             (precision == ZTM_FP64 ? sizeof(double) : sizeof(float));
-        ZTM_HIP(hipMemsetAsync(out_dev, 0, bytes, (hipStream_t)stream));
-        return ZTM_OK;
-    }
     return pm_adj_launch(plan->shape, t, precision, st, r_dev, plan->sy, plan->sz,
-                         pm_scale(plan), plan->part, out_dev, near_args(plan),
-                         (hipStream_t)stream);
+                         pm_scale(plan), plan->part, out_dev, near_args(plan), s);
 }
 
 int ztm_plan_smoothness(ztm_plan plan, int precision, const void* rho_dev, void* out_dev,

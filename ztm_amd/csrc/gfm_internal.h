@@ -43,21 +43,43 @@ constexpr int kStationPad = 512;
 // Near field: pairs whose cell index lies within kNear cells (per axis) of
 // the station's nearest cell index are re-evaluated in FP64 by the FP32 path.
 constexpr int kNear = 2;
+// Prism mid zone: cells within kMid cells (per axis) of a station are
+// corrected from the far-field kernel to the prism integral (FP64 closed
+// form up to kNear, FP32 3x3x3 Gauss-Legendre beyond).
+constexpr int kMid = 7;
+// Column margin of the station-by-column index (covers kNear and kMid).
+constexpr int kColMargin = 8;
 
-// Everything the FP32 near-field correction needs. st_d == nullptr disables it.
+// Quadrupole term of a non-cubic cell's far field, cell units (L = |dx|),
+// divided by the cell volume: q2* = 2.5 * Q'_**, qz = Q'_zz with
+// Q'_xx = (2 hx^2 - hy^2 - hz^2)/3 etc. for half-widths (hx, hy, hz).
+struct QuadArgs {
+    float q2x = 0.f, q2y = 0.f, q2z = 0.f, qz = 0.f;
+};
+
+// Correction applied by the reduce kernels of the FP32 path.
+enum CorrMode { kCorrNone = 0, kCorrPmNear = 1, kCorrPrism = 2 };
+
+// Everything the FP32 corrections need (mode kCorrNone disables them).
 // st_c[m] = (ci, cj, ck, 0): the station's nearest cell index per axis.
 // col_start/col_st: stations grouped by (ci, cj) column over the extended
-// range [-kNear, nx+kNear) x [-kNear, ny+kNear) (adjoint gather); stations
-// outside it are never near any cell. kmin/kmax: range of ck over them.
+// range [-kColMargin, nx+kColMargin) x [-kColMargin, ny+kColMargin) (adjoint
+// gather); stations outside it are never near any cell. kmin/kmax: range of
+// ck over them.
 struct NearArgs {
+    int mode = kCorrNone;
+    int radius = kNear;
+    int quad = 0;
+    QuadArgs q;
     const float4* st_f = nullptr;
     const double4* st_d = nullptr;
     const int4* st_c = nullptr;
     const int* col_start = nullptr;
     const int* col_st = nullptr;
+    const double* corr = nullptr;   // forward prism: per-station correction
     int nx = 0, ny = 0, nz = 0;
     float syf = 1.f, szf = 1.f, thr = 0.f;
-    double syd = 1.0, szd = 1.0;
+    double syd = 1.0, szd = 1.0, vol = 1.0;   // vol = |sy*sz|, cell volume / L^3
     int kmin = 0, kmax = -1;
 };
 
@@ -91,11 +113,26 @@ int pm_fwd_launch(const PmShape& s, const PmTuning& t, int precision,
                   hipStream_t stream);
 
 // Adjoint: out[n] = scale * sum_m K(m,n) r[m] (float or double by precision).
-// With near.st_d set (FP32 only) the result always goes through `part`.
+// With a correction mode set (FP32 only) the result always goes through `part`.
 int pm_adj_launch(const PmShape& s, const PmTuning& t, int precision,
                   const void* st, const void* r, double sy, double sz,
                   double scale, double* part, void* out, const NearArgs& near,
                   hipStream_t stream);
+
+// Prism formula (gfm_prism.hip).
+// FP32: per-station mid-zone correction (prism integral minus the far-field
+// kernel) for cells within kMid of each station -> corr[m]; the forward then
+// runs pm_fwd_launch with near.mode = kCorrPrism and near.corr = corr.
+int prism_near_fwd_launch(const PmShape& s, const NearArgs& near, const float* rho,
+                          double* corr, hipStream_t stream);
+// FP64: exact closed form on every pair (the FP64 GPU path of the prism).
+size_t prism64_workspace(const PmShape& s, const PmTuning& t, bool adjoint);
+int prism64_fwd_launch(const PmShape& s, const PmTuning& t, const double4* st,
+                       const double* rho, double sy, double sz, double thr, double scale,
+                       double* part, double* g, hipStream_t stream);
+int prism64_adj_launch(const PmShape& s, const PmTuning& t, const double4* st,
+                       const double* r, double sy, double sz, double scale, double* part,
+                       double* out, hipStream_t stream);
 
 // Reference-mode station preparation on the device: interior stations of a
 // di x dj raster with heights s[m] -> float4 records + output map.

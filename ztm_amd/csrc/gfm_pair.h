@@ -1,0 +1,126 @@
+// Device-side pair terms shared by the point-mass and prism kernels.
+//
+// Units: lengths in L = |dx|; a station is (X, Y, Z) with cell (i,j,k) at
+// (i, j*sy, k*sz). The main FP32 kernels accumulate, per station-cell pair,
+//     w * dz * K32,  K32 = v^3 (point mass)  or  v^3 * (1 + v^2 (q v^2 - Qz))
+// (point mass plus the quadrupole term of a non-cubic box), v = 1/r.
+#pragma once
+
+#include "gfm_internal.h"
+
+namespace ztm {
+
+__device__ __forceinline__ float rsq_f(float x) { return __builtin_amdgcn_rsqf(x); }
+
+// Far-field kernel of one pair in FP32, exactly as the main kernels evaluate
+// it. a = X - i, R = dy^2 + dz^2 (+eps), qrow = q2y dy^2 + q2z dz^2.
+template <bool QUAD>
+__device__ __forceinline__ float pair_k32(float a, float R, float qrow, const QuadArgs& q) {
+    const float v = rsq_f(fmaf(a, a, R));
+    const float v2 = v * v;
+    const float v3 = v2 * v;
+    if constexpr (QUAD) {
+        const float qq = fmaf(a * q.q2x, a, qrow);
+        const float s = fmaf(qq, v2, -q.qz);
+        const float f = fmaf(s, v2, 1.0f);
+        return v3 * f;
+    } else {
+        return v3;
+    }
+}
+
+// ln(v + r) without cancellation when v < 0: v + r = (r^2 - v^2)/(r - v).
+__device__ __forceinline__ double log_vr(double v, double r, double rest2) {
+    return v >= 0.0 ? log(v + r) : log(rest2 / (r - v));
+}
+
+// Antiderivative of the prism g_z kernel (Nagy, Papp & Benedek 2000) in
+// station-relative coordinates; terms with an exactly zero prefactor are 0.
+__device__ __forceinline__ double prism_F(double u, double v, double w) {
+    const double u2 = u * u, v2 = v * v, w2 = w * w;
+    const double r = sqrt(u2 + v2 + w2);
+    double f = 0.0;
+    if (u != 0.0) f += u * log_vr(v, r, u2 + w2);
+    if (v != 0.0) f += v * log_vr(u, r, v2 + w2);
+    if (w != 0.0) f -= w * atan(u * v / (w * r));
+    return f;
+}
+
+// Exact integral of (Z - z)/r^3 over cell (i,j,k), in units of L, FP64.
+__device__ __forceinline__ double prism_cell(double X, double Y, double Z, int i, int j, int k,
+                                             double sy, double sz) {
+    const double hy = 0.5 * fabs(sy), hz = 0.5 * fabs(sz);
+    const double u0 = (double)i - 0.5 - X, u1 = (double)i + 0.5 - X;
+    const double v0 = (double)j * sy - hy - Y, v1 = (double)j * sy + hy - Y;
+    const double w0 = (double)k * sz - hz - Z, w1 = (double)k * sz + hz - Z;
+    double s = 0.0;
+    s += prism_F(u0, v0, w0) - prism_F(u0, v0, w1) - prism_F(u0, v1, w0) + prism_F(u0, v1, w1);
+    s -= prism_F(u1, v0, w0) - prism_F(u1, v0, w1) - prism_F(u1, v1, w0) + prism_F(u1, v1, w1);
+    return -s;
+}
+
+// Mean of the point-mass kernel dz/r^3 over cell (i,j,k) by 3x3x3
+// Gauss-Legendre quadrature, FP32 (accurate to ~2e-6 of the cell's 1/r^2
+// scale at 3 cells, ~4e-9 beyond 8).
+__device__ __forceinline__ float gl3_cell(float X, float Y, float Z, int i, int j, int k,
+                                          float sy, float sz) {
+    const float n = 0.38729833462074170f;   // sqrt(3/5)/2: nodes at +-n, 0 on a unit edge
+    const float wgt[3] = {5.0f / 18.0f, 8.0f / 18.0f, 5.0f / 18.0f};
+    const float off[3] = {-n, 0.0f, n};
+    const float ay = fabsf(sy), az = fabsf(sz);
+    float acc = 0.0f;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const float dz = Z - ((float)k * sz + off[c] * az);
+#pragma unroll
+        for (int b = 0; b < 3; ++b) {
+            const float dy = Y - ((float)j * sy + off[b] * ay);
+            const float R = fmaf(dy, dy, dz * dz);
+            float row = 0.0f;
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                const float dx = X - ((float)i + off[a]);
+                const float v = rsq_f(fmaf(dx, dx, R));
+                row = fmaf(wgt[a], v * v * v, row);
+            }
+            acc = fmaf(wgt[c] * wgt[b] * dz, row, acc);
+        }
+    }
+    return acc;
+}
+
+// Correction of one station-cell pair for the FP32 path: the better value of
+// the pair minus its FP32 far-field term as the main kernels evaluate it (same
+// rounded coordinates, same v_rsq_f32), so adding it replaces the pair's FP32
+// value. Terms are  w * dz * K  in cell units (K = 1/r^3 for a point mass).
+//   kCorrPmNear: better value = the point-mass term in FP64.
+//   kCorrPrism:  better value = the prism integral / cell volume, FP64 closed
+//                form within kNear cells, FP32 3x3x3 Gauss-Legendre beyond.
+template <bool QUAD>
+__device__ __forceinline__ double corr_pair(int mode, float w, const float4& pf,
+                                            const double4& pd, const int4& c, int i, int j,
+                                            int k, const NearArgs& a) {
+    const float dyf = pf.y - (float)j * a.syf;
+    const float dzf = pf.z - (float)k * a.szf;
+    const float af = pf.x - (float)i;
+    const float R = fmaf(dyf, dyf, fmaf(dzf, dzf, kEpsF));
+    const float qrow = QUAD ? fmaf(a.q.q2y, dyf * dyf, a.q.q2z * (dzf * dzf)) : 0.f;
+    const double t32 = (double)dzf * (double)(w * pair_k32<QUAD>(af, R, qrow, a.q));
+    double better;
+    if (mode == kCorrPmNear) {
+        const double ad = pd.x - (double)i;
+        const double dyd = pd.y - (double)j * a.syd;
+        const double dzd = pd.z - (double)k * a.szd;
+        const double r2 = ad * ad + dyd * dyd + dzd * dzd + kEpsD;
+        better = (double)w * dzd / (r2 * sqrt(r2));
+    } else {
+        const int cheb = max(abs(i - c.x), max(abs(j - c.y), abs(k - c.z)));
+        if (cheb <= kNear)
+            better = (double)w * prism_cell(pd.x, pd.y, pd.z, i, j, k, a.syd, a.szd) / a.vol;
+        else
+            better = (double)w * (double)gl3_cell(pf.x, pf.y, pf.z, i, j, k, a.syf, a.szf);
+    }
+    return better - t32;
+}
+
+}  // namespace ztm

@@ -24,6 +24,7 @@
 // all cross-wave and cross-slice sums are FP64 in a fixed order, so results
 // are bitwise reproducible run to run (no atomics).
 #include "gfm_internal.h"
+#include "gfm_pair.h"
 
 #include <algorithm>
 
@@ -39,15 +40,27 @@ template <typename T> __device__ __forceinline__ T eps_r2();
 template <> __device__ __forceinline__ float  eps_r2<float>()  { return kEpsF; }
 template <> __device__ __forceinline__ double eps_r2<double>() { return kEpsD; }
 
+// Far-field kernel of one pair: v^3 (point mass), or with the quadrupole term
+// of a non-cubic cell (FP32 prism path only; see gfm_pair.h).
+template <typename T, bool QUAD>
+__device__ __forceinline__ T kterm(T a, T R, T qrow, const QuadArgs& q) {
+    if constexpr (sizeof(T) == sizeof(float)) {
+        return pair_k32<QUAD>(a, R, qrow, q);
+    } else {
+        const T v = rsq(fma(a, a, R));
+        return v * v * v;
+    }
+}
+
 // ---------------------------------------------------------------- forward
 // grid = (m_pad / (64*S), n_slices); block = 256.
 // st:   prepared stations [m_pad] (x, y, z in cell units; see gfm_internal.h)
 // part: [n_slices][m_pad] FP64 partial sums  sum_n rho_n dz/r^3
-template <typename T, int S>
+template <typename T, int S, bool QUAD>
 __global__ __launch_bounds__(256) void pm_fwd_kernel(
     const typename Vec4<T>::type* __restrict__ st, const T* __restrict__ rho,
     int nx, int ny, int n_rows, T sy, T sz, T thr, int rows_per_slice,
-    double* __restrict__ part, int m_pad) {
+    double* __restrict__ part, int m_pad, QuadArgs qa) {
     using V4 = typename Vec4<T>::type;
     extern __shared__ __align__(16) unsigned char smem[];
     T* lrho = reinterpret_cast<T*>(smem);
@@ -89,12 +102,13 @@ __global__ __launch_bounds__(256) void pm_fwd_kernel(
         const int j = row - k * ny;
         const T yj = T(j) * sy;
         const T zk = T(k) * sz;
-        T R[S], dz[S], ar[S], xm[S];
+        T R[S], dz[S], ar[S], xm[S], QR[S];
 #pragma unroll
         for (int s = 0; s < S; ++s) {
             const T dy = Y[s] - yj;
             dz[s] = Z[s] - zk;
             R[s] = fma(dy, dy, fma(dz[s], dz[s], eps_r2<T>()));
+            QR[s] = QUAD ? fma(T(qa.q2y), dy * dy, T(qa.q2z) * (dz[s] * dz[s])) : T(0);
             ar[s] = T(0);
             xm[s] = X[s];
         }
@@ -107,15 +121,15 @@ __global__ __launch_bounds__(256) void pm_fwd_kernel(
                 const T a1 = xm[s] - T(1);
                 const T a2 = xm[s] - T(2);
                 const T a3 = xm[s] - T(3);
-                const T v0 = rsq(fma(a0, a0, R[s]));
-                const T v1 = rsq(fma(a1, a1, R[s]));
-                const T v2 = rsq(fma(a2, a2, R[s]));
-                const T v3 = rsq(fma(a3, a3, R[s]));
+                const T k0 = kterm<T, QUAD>(a0, R[s], QR[s], qa);
+                const T k1 = kterm<T, QUAD>(a1, R[s], QR[s], qa);
+                const T k2 = kterm<T, QUAD>(a2, R[s], QR[s], qa);
+                const T k3 = kterm<T, QUAD>(a3, R[s], QR[s], qa);
                 T t = ar[s];
-                t = fma(q.x, v0 * v0 * v0, t);
-                t = fma(q.y, v1 * v1 * v1, t);
-                t = fma(q.z, v2 * v2 * v2, t);
-                t = fma(q.w, v3 * v3 * v3, t);
+                t = fma(q.x, k0, t);
+                t = fma(q.y, k1, t);
+                t = fma(q.z, k2, t);
+                t = fma(q.w, k3, t);
                 ar[s] = t;
                 xm[s] -= T(4);
             }
@@ -137,28 +151,11 @@ __global__ __launch_bounds__(256) void pm_fwd_kernel(
     }
 }
 
-// Near-field correction of one station-cell pair for the FP32 path: the
-// exact FP64 term minus the FP32 term as the main kernels evaluate it (same
-// rounded coordinates, same v_rsq_f32), so adding it replaces the pair's FP32
-// value by its FP64 value. Terms are  w * dz / r^3  in cell units.
-__device__ __forceinline__ double near_pair(float w, const float4& pf, const double4& pd,
-                                            int i, int j, int k, const NearArgs& a) {
-    const float dyf = pf.y - (float)j * a.syf;
-    const float dzf = pf.z - (float)k * a.szf;
-    const float af = pf.x - (float)i;
-    const float v = rsq(fmaf(af, af, fmaf(dyf, dyf, fmaf(dzf, dzf, kEpsF))));
-    const double t32 = (double)dzf * (double)(w * (v * v * v));
-    const double ad = pd.x - (double)i;
-    const double dyd = pd.y - (double)j * a.syd;
-    const double dzd = pd.z - (double)k * a.szd;
-    const double r2 = ad * ad + dyd * dyd + dzd * dzd + kEpsD;
-    const double t64 = (double)w * dzd / (r2 * sqrt(r2));
-    return t64 - t32;
-}
-
-// g[m] = scale * (sum_slices part[slice][m] + near-field correction)
+// g[m] = scale * (sum_slices part[slice][m] + correction)
 //   (general API, FP64 out), or gref[map[m]] += (float)(...) (reference ABI).
-__global__ void pm_fwd_reduce_kernel(const double* __restrict__ part,
+// kCorrPmNear: FP64 near field over the (2 kNear + 1)^3 box around the
+// station; kCorrPrism: the per-station correction computed beforehand.
+__global__ __launch_bounds__(256) void pm_fwd_reduce_kernel(const double* __restrict__ part,
                                      int n_slices, int m_pad, int m,
                                      double scale, double* __restrict__ g,
                                      float* __restrict__ gref,
@@ -169,7 +166,7 @@ __global__ void pm_fwd_reduce_kernel(const double* __restrict__ part,
     if (i >= m) return;
     double s = 0.0;
     for (int k = 0; k < n_slices; ++k) s += part[(size_t)k * m_pad + i];
-    if (na.st_d) {
+    if (na.mode == kCorrPmNear) {
         const float4 pf = na.st_f[i];
         const double4 pd = na.st_d[i];
         const int4 c = na.st_c[i];
@@ -182,9 +179,11 @@ __global__ void pm_fwd_reduce_kernel(const double* __restrict__ part,
                 for (int ii = i0; ii <= i1; ++ii) {
                     const float w = rho_f[((size_t)kk * na.ny + jj) * na.nx + ii];
                     if (!(fabsf(w) > na.thr)) continue;
-                    corr += near_pair(w, pf, pd, ii, jj, kk, na);
+                    corr += corr_pair<false>(kCorrPmNear, w, pf, pd, c, ii, jj, kk, na);
                 }
         s += corr;
+    } else if (na.mode == kCorrPrism) {
+        s += na.corr[i];
     }
     s *= scale;
     if (out_map) {
@@ -197,12 +196,12 @@ __global__ void pm_fwd_reduce_kernel(const double* __restrict__ part,
 // ---------------------------------------------------------------- adjoint
 // grid = (ceil(n_segs/64), n_splits); block = 256; a segment is C
 // consecutive cells of one row (the last one of a row may be ragged).
-template <typename T, int C>
+template <typename T, int C, bool QUAD>
 __global__ __launch_bounds__(256) void pm_adj_kernel(
     const typename Vec4<T>::type* __restrict__ st, const T* __restrict__ r,
     int m, int nx, int ny, T sy, T sz, int segs_per_row, int64_t n_segs,
     int st_per_split, int m_pad, double* __restrict__ part,
-    T* __restrict__ out, double scale, int64_t n_cells, int direct) {
+    T* __restrict__ out, double scale, int64_t n_cells, int direct, QuadArgs qa) {
     using V4 = typename Vec4<T>::type;
     extern __shared__ __align__(16) unsigned char smem[];
     V4* tile = reinterpret_cast<V4*>(smem);
@@ -243,13 +242,13 @@ __global__ __launch_bounds__(256) void pm_adj_kernel(
             const T dy = p.y - yj;
             const T dz = p.z - zk;
             const T R = fma(dy, dy, fma(dz, dz, eps_r2<T>()));
+            const T qrow = QUAD ? fma(T(qa.q2y), dy * dy, T(qa.q2z) * (dz * dz)) : T(0);
             const T q = p.w * dz;
             const T xm = p.x - xi0;
 #pragma unroll
             for (int c = 0; c < C; ++c) {
                 const T a = xm - T(c);
-                const T v = rsq(fma(a, a, R));
-                acc[c] = fma(q, v * v * v, acc[c]);
+                acc[c] = fma(q, kterm<T, QUAD>(a, R, qrow, qa), acc[c]);
             }
         }
 #pragma unroll
@@ -281,11 +280,11 @@ __global__ __launch_bounds__(256) void pm_adj_kernel(
     }
 }
 
-// out[n] = scale * (sum_splits part[split][n] + near-field correction); the
-// correction gathers, per cell, the stations whose column lies within kNear
-// of the cell's column (fixed CSR order, so the result is deterministic).
-template <typename T>
-__global__ void pm_adj_reduce_kernel(const double* __restrict__ part,
+// out[n] = scale * (sum_splits part[split][n] + correction); the correction
+// gathers, per cell, the stations whose column lies within na.radius of the
+// cell's column (fixed CSR order, so the result is deterministic).
+template <typename T, bool QUAD>
+__global__ __launch_bounds__(256) void pm_adj_reduce_kernel(const double* __restrict__ part,
                                      int n_splits, int64_t n_cells,
                                      double scale, T* __restrict__ out,
                                      const float* __restrict__ r_f, NearArgs na) {
@@ -293,24 +292,26 @@ __global__ void pm_adj_reduce_kernel(const double* __restrict__ part,
     if (n >= n_cells) return;
     double s = 0.0;
     for (int k = 0; k < n_splits; ++k) s += part[(size_t)k * n_cells + n];
-    if (na.st_d) {
+    if (na.mode != kCorrNone) {
+        const int R = na.radius;
         const int ii = (int)(n % na.nx);
         const int64_t rw = n / na.nx;
         const int jj = (int)(rw % na.ny);
         const int kk = (int)(rw / na.ny);
-        if (kk >= na.kmin - kNear && kk <= na.kmax + kNear) {
-            const int wcols = na.nx + 2 * kNear;
+        if (kk >= na.kmin - R && kk <= na.kmax + R) {
+            const int wcols = na.nx + 2 * kColMargin;
             double corr = 0.0;
-            for (int cj = jj - kNear; cj <= jj + kNear; ++cj) {
-                if (cj < -kNear || cj >= na.ny + kNear) continue;
-                for (int ci = ii - kNear; ci <= ii + kNear; ++ci) {
-                    if (ci < -kNear || ci >= na.nx + kNear) continue;
-                    const int col = (cj + kNear) * wcols + (ci + kNear);
+            for (int cj = jj - R; cj <= jj + R; ++cj) {
+                if (cj < -kColMargin || cj >= na.ny + kColMargin) continue;
+                for (int ci = ii - R; ci <= ii + R; ++ci) {
+                    if (ci < -kColMargin || ci >= na.nx + kColMargin) continue;
+                    const int col = (cj + kColMargin) * wcols + (ci + kColMargin);
                     for (int q = na.col_start[col]; q < na.col_start[col + 1]; ++q) {
                         const int m = na.col_st[q];
                         const int4 c = na.st_c[m];
-                        if (c.z < kk - kNear || c.z > kk + kNear) continue;
-                        corr += near_pair(r_f[m], na.st_f[m], na.st_d[m], ii, jj, kk, na);
+                        if (c.z < kk - R || c.z > kk + R) continue;
+                        corr += corr_pair<QUAD>(na.mode, r_f[m], na.st_f[m], na.st_d[m], c, ii,
+                                                jj, kk, na);
                     }
                 }
             }
@@ -388,20 +389,26 @@ int adj_splits(const PmShape& s, const PmTuning& t) {
 
 template <typename T, int S>
 void launch_fwd(const PmShape& s, int rps, int n_slices, const void* st,
-                const void* rho, T sy, T sz, T thr, double* part,
-                hipStream_t stream) {
+                const void* rho, T sy, T sz, T thr, double* part, bool quad,
+                const QuadArgs& qa, hipStream_t stream) {
     using V4 = typename Vec4<T>::type;
     const int nxp = (s.nx + 3) & ~3;
     const size_t lds = std::max((size_t)rps * nxp * sizeof(T), (size_t)4 * S * 64 * sizeof(double));
-    hipLaunchKernelGGL((pm_fwd_kernel<T, S>), dim3(s.m_pad / (64 * S), n_slices),
-                       dim3(256), lds, stream, (const V4*)st, (const T*)rho, s.nx,
-                       s.ny, (int)s.n_rows, sy, sz, thr, rps, part, s.m_pad);
+    const dim3 grid(s.m_pad / (64 * S), n_slices);
+    if (quad && sizeof(T) == sizeof(float))
+        hipLaunchKernelGGL((pm_fwd_kernel<T, S, true>), grid, dim3(256), lds, stream,
+                           (const V4*)st, (const T*)rho, s.nx, s.ny, (int)s.n_rows, sy, sz,
+                           thr, rps, part, s.m_pad, qa);
+    else
+        hipLaunchKernelGGL((pm_fwd_kernel<T, S, false>), grid, dim3(256), lds, stream,
+                           (const V4*)st, (const T*)rho, s.nx, s.ny, (int)s.n_rows, sy, sz,
+                           thr, rps, part, s.m_pad, qa);
 }
 
 template <typename T, int C>
 void launch_adj(const PmShape& s, int splits, int direct, const void* st, const void* r,
-                T sy, T sz, double* part, void* out, double scale,
-                hipStream_t stream) {
+                T sy, T sz, double* part, void* out, double scale, bool quad,
+                const QuadArgs& qa, hipStream_t stream) {
     using V4 = typename Vec4<T>::type;
     const int segs_per_row = (s.nx + C - 1) / C;
     const int64_t n_segs = s.n_rows * segs_per_row;
@@ -409,10 +416,16 @@ void launch_adj(const PmShape& s, int splits, int direct, const void* st, const 
     const int tiles = s.m_pad / 256;
     const int per_split = ((tiles + splits - 1) / splits) * 256;
     const size_t lds = std::max((size_t)256 * sizeof(V4), (size_t)4 * C * 64 * sizeof(double));
-    hipLaunchKernelGGL((pm_adj_kernel<T, C>), dim3(blocks_x, splits), dim3(256),
-                       lds, stream, (const V4*)st, (const T*)r, s.m, s.nx, s.ny,
-                       sy, sz, segs_per_row, n_segs, per_split, s.m_pad, part,
-                       (T*)out, scale, s.n_cells, direct);
+    if (quad && sizeof(T) == sizeof(float))
+        hipLaunchKernelGGL((pm_adj_kernel<T, C, true>), dim3(blocks_x, splits), dim3(256),
+                           lds, stream, (const V4*)st, (const T*)r, s.m, s.nx, s.ny,
+                           sy, sz, segs_per_row, n_segs, per_split, s.m_pad, part,
+                           (T*)out, scale, s.n_cells, direct, qa);
+    else
+        hipLaunchKernelGGL((pm_adj_kernel<T, C, false>), dim3(blocks_x, splits), dim3(256),
+                           lds, stream, (const V4*)st, (const T*)r, s.m, s.nx, s.ny,
+                           sy, sz, segs_per_row, n_segs, per_split, s.m_pad, part,
+                           (T*)out, scale, s.n_cells, direct, qa);
 }
 
 }  // namespace
@@ -425,6 +438,7 @@ size_t pm_fwd_workspace(const PmShape& s, const PmTuning& t, int precision) {
 }
 
 size_t pm_adj_workspace(const PmShape& s, const PmTuning& t, int precision, bool near) {
+    // near: a correction will run, so the main kernel writes partials
     (void)precision;
     const int sp = adj_splits(s, t);
     return (sp > 1 || near) ? (size_t)sp * (size_t)s.n_cells : 0;
@@ -444,28 +458,29 @@ int pm_fwd_launch(const PmShape& s, const PmTuning& t, int precision,
     const int rps = fwd_rows_per_slice(s, t, elem);
     const int n_slices = (int)((s.n_rows + rps - 1) / rps);
     if (n_slices > 65535) return fail(ZTM_EINVAL, "too many row slices (%d)", n_slices);
+    NearArgs na = near;
+    if (precision == ZTM_FP64) na.mode = kCorrNone;   // FP64 needs no correction
+    const bool quad = na.quad != 0 && precision == ZTM_FP32;
     {
         ProfScope prof("pm_fwd", stream);
         if (precision == ZTM_FP64) {
             const double a = sy, b = sz, c = thr;
             switch (t.st_per_lane) {
-                case 1: launch_fwd<double, 1>(s, rps, n_slices, st, rho, a, b, c, part, stream); break;
-                case 2: launch_fwd<double, 2>(s, rps, n_slices, st, rho, a, b, c, part, stream); break;
-                default: launch_fwd<double, 4>(s, rps, n_slices, st, rho, a, b, c, part, stream); break;
+                case 1: launch_fwd<double, 1>(s, rps, n_slices, st, rho, a, b, c, part, false, na.q, stream); break;
+                case 2: launch_fwd<double, 2>(s, rps, n_slices, st, rho, a, b, c, part, false, na.q, stream); break;
+                default: launch_fwd<double, 4>(s, rps, n_slices, st, rho, a, b, c, part, false, na.q, stream); break;
             }
         } else {
             const float a = (float)sy, b = (float)sz, c = (float)thr;
             switch (t.st_per_lane) {
-                case 1: launch_fwd<float, 1>(s, rps, n_slices, st, rho, a, b, c, part, stream); break;
-                case 2: launch_fwd<float, 2>(s, rps, n_slices, st, rho, a, b, c, part, stream); break;
-                case 8: launch_fwd<float, 8>(s, rps, n_slices, st, rho, a, b, c, part, stream); break;
-                default: launch_fwd<float, 4>(s, rps, n_slices, st, rho, a, b, c, part, stream); break;
+                case 1: launch_fwd<float, 1>(s, rps, n_slices, st, rho, a, b, c, part, quad, na.q, stream); break;
+                case 2: launch_fwd<float, 2>(s, rps, n_slices, st, rho, a, b, c, part, quad, na.q, stream); break;
+                case 8: launch_fwd<float, 8>(s, rps, n_slices, st, rho, a, b, c, part, quad, na.q, stream); break;
+                default: launch_fwd<float, 4>(s, rps, n_slices, st, rho, a, b, c, part, quad, na.q, stream); break;
             }
         }
         ZTM_HIP(hipGetLastError());
     }
-    NearArgs na = near;
-    if (precision == ZTM_FP64) na.st_d = nullptr;   // FP64 needs no correction
     hipLaunchKernelGGL(pm_fwd_reduce_kernel, dim3((s.m + 255) / 256), dim3(256), 0,
                        stream, part, n_slices, s.m_pad, s.m, scale, g, gref, out_map,
                        (const float*)rho, na);
@@ -480,21 +495,22 @@ int pm_adj_launch(const PmShape& s, const PmTuning& t, int precision,
     if (s.n_cells == 0) return ZTM_OK;
     const int splits = adj_splits(s, t);
     NearArgs na = near;
-    if (precision == ZTM_FP64) na.st_d = nullptr;
-    const int direct = (splits == 1 && !na.st_d) ? 1 : 0;
+    if (precision == ZTM_FP64) na.mode = kCorrNone;
+    const bool quad = na.quad != 0 && precision == ZTM_FP32;
+    const int direct = (splits == 1 && na.mode == kCorrNone) ? 1 : 0;
     {
         ProfScope prof("pm_adj", stream);
         if (precision == ZTM_FP64) {
             switch (t.cells_per_lane) {
-                case 4: launch_adj<double, 4>(s, splits, direct, st, r, sy, sz, part, out, scale, stream); break;
-                default: launch_adj<double, 8>(s, splits, direct, st, r, sy, sz, part, out, scale, stream); break;
+                case 4: launch_adj<double, 4>(s, splits, direct, st, r, sy, sz, part, out, scale, false, na.q, stream); break;
+                default: launch_adj<double, 8>(s, splits, direct, st, r, sy, sz, part, out, scale, false, na.q, stream); break;
             }
         } else {
             const float a = (float)sy, b = (float)sz;
             switch (t.cells_per_lane) {
-                case 4: launch_adj<float, 4>(s, splits, direct, st, r, a, b, part, out, scale, stream); break;
-                case 8: launch_adj<float, 8>(s, splits, direct, st, r, a, b, part, out, scale, stream); break;
-                default: launch_adj<float, 16>(s, splits, direct, st, r, a, b, part, out, scale, stream); break;
+                case 4: launch_adj<float, 4>(s, splits, direct, st, r, a, b, part, out, scale, quad, na.q, stream); break;
+                case 8: launch_adj<float, 8>(s, splits, direct, st, r, a, b, part, out, scale, quad, na.q, stream); break;
+                default: launch_adj<float, 16>(s, splits, direct, st, r, a, b, part, out, scale, quad, na.q, stream); break;
             }
         }
         ZTM_HIP(hipGetLastError());
@@ -502,11 +518,15 @@ int pm_adj_launch(const PmShape& s, const PmTuning& t, int precision,
     if (!direct) {
         const unsigned blocks = (unsigned)((s.n_cells + 255) / 256);
         if (precision == ZTM_FP64)
-            hipLaunchKernelGGL(pm_adj_reduce_kernel<double>, dim3(blocks), dim3(256), 0,
+            hipLaunchKernelGGL((pm_adj_reduce_kernel<double, false>), dim3(blocks), dim3(256), 0,
                                stream, part, splits, s.n_cells, scale, (double*)out,
                                (const float*)nullptr, na);
+        else if (quad)
+            hipLaunchKernelGGL((pm_adj_reduce_kernel<float, true>), dim3(blocks), dim3(256), 0,
+                               stream, part, splits, s.n_cells, scale, (float*)out,
+                               (const float*)r, na);
         else
-            hipLaunchKernelGGL(pm_adj_reduce_kernel<float>, dim3(blocks), dim3(256), 0,
+            hipLaunchKernelGGL((pm_adj_reduce_kernel<float, false>), dim3(blocks), dim3(256), 0,
                                stream, part, splits, s.n_cells, scale, (float*)out,
                                (const float*)r, na);
         ZTM_HIP(hipGetLastError());
