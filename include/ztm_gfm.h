@@ -141,6 +141,25 @@ int ztm_plan_forward(ztm_plan plan, int precision, const void* rho_dev,
 int ztm_plan_adjoint(ztm_plan plan, int precision, const void* r_dev,
                      void* out_dev, void* stream);
 
+/* Stored-matrix variant (no reference counterpart; serves the reuse of one
+ * geometry's G across the forward models of an inversion step,
+ * TFM.cpp:124-135). ztm_plan_store materialises G once as FP32
+ * [M][ceil4(N)] in device memory owned by the plan (ztm_plan_store_bytes
+ * says how much; ZTM_ENOMEM if it does not fit), each entry evaluated in FP64
+ * with the plan's formula and rounded once. The stored forward/adjoint then
+ * stream G from HBM with FP64 accumulation:
+ *   forward: rho_dev float[N] (16-byte aligned) -> g_dev double[M]
+ *   adjoint: r_dev float[M] -> out_dev float[N]
+ * The forward applies the plan's threshold; the store is freed by
+ * ztm_plan_store_release or ztm_plan_destroy. */
+int64_t ztm_plan_store_bytes(ztm_plan plan);
+int ztm_plan_store(ztm_plan plan, void* stream);
+int ztm_plan_store_release(ztm_plan plan);
+int ztm_plan_forward_stored(ztm_plan plan, const float* rho_dev, double* g_dev,
+                            void* stream);
+int ztm_plan_adjoint_stored(ztm_plan plan, const float* r_dev, float* out_dev,
+                            void* stream);
+
 /* Smoothness operator of the regularised misfit 0.5||G rho-d||^2 +
  * 0.5 lambda ||grad rho||^2 (forward differences on each axis, none across
  * the boundary): out_dev = L^T L rho, so the misfit gradient is

@@ -108,6 +108,7 @@ struct ztm_plan_s {
     int* out_map = nullptr;      // reference mode: slot -> g index
     double* part = nullptr;      // split-K workspace
     size_t part_cap = 0;         // doubles
+    float* G = nullptr;          // stored matrix [m][stored_ld], or null
 };
 
 namespace {
@@ -142,6 +143,7 @@ void plan_free(ztm_plan p) {
     if (p->col_st) (void)hipFree(p->col_st);
     if (p->out_map) (void)hipFree(p->out_map);
     if (p->part) (void)hipFree(p->part);
+    if (p->G) (void)hipFree(p->G);
     delete p;
 }
 
@@ -429,6 +431,65 @@ int ztm_plan_adjoint(ztm_plan plan, int precision, const void* r_dev, void* out_
     const void* st = precision == ZTM_FP64 ? (const void*)plan->st_d : (const void*)plan->st_f;
     return pm_adj_launch(plan->shape, t, precision, st, r_dev, plan->sy, plan->sz,
                          pm_scale(plan), plan->part, out_dev, near_args(plan), s);
+}
+
+int64_t ztm_plan_store_bytes(ztm_plan plan) {
+    if (!plan) return -1;
+    return (int64_t)sizeof(float) * plan->shape.m * stored_ld(plan->shape);
+}
+
+int ztm_plan_store(ztm_plan plan, void* stream) {
+    if (!plan) return fail(ZTM_EINVAL, "null plan");
+    int dev = 0;
+    ZTM_HIP(hipGetDevice(&dev));
+    if (dev != plan->device)
+        return fail(ZTM_EINVAL, "plan belongs to device %d, current device is %d", plan->device, dev);
+    if (plan->G) return ZTM_OK;
+    const size_t bytes = (size_t)ztm_plan_store_bytes(plan);
+    if (bytes == 0) return ZTM_OK;
+    hipError_t e = hipMalloc(&plan->G, bytes);
+    if (e != hipSuccess) {
+        plan->G = nullptr;
+        (void)hipGetLastError();
+        return fail(ZTM_ENOMEM, "stored matrix needs %.3g GB of device memory: %s", bytes / 1e9,
+                    hipGetErrorString(e));
+    }
+    const bool prism = plan->prob.formula == ZTM_PRISM;
+    const double scale = prism ? plan->prob.kappa * plan->L : pm_scale(plan);
+    const int rc = stored_build_launch(plan->shape, prism, plan->st_d, plan->sy, plan->sz, scale,
+                                       plan->G, (hipStream_t)stream);
+    if (rc != ZTM_OK) {
+        (void)hipFree(plan->G);
+        plan->G = nullptr;
+    }
+    return rc;
+}
+
+int ztm_plan_store_release(ztm_plan plan) {
+    if (!plan) return fail(ZTM_EINVAL, "null plan");
+    if (plan->G) ZTM_HIP(hipFree(plan->G));
+    plan->G = nullptr;
+    return ZTM_OK;
+}
+
+int ztm_plan_forward_stored(ztm_plan plan, const float* rho_dev, double* g_dev, void* stream) {
+    if (!plan) return fail(ZTM_EINVAL, "null plan");
+    if (plan->shape.m == 0) return ZTM_OK;
+    if (!plan->G) return fail(ZTM_EINVAL, "no stored matrix: call ztm_plan_store first");
+    if (!rho_dev || !g_dev) return fail(ZTM_EINVAL, "null rho or g");
+    return stored_fwd_launch(plan->shape, plan->G, rho_dev, plan->prob.threshold, g_dev,
+                             (hipStream_t)stream);
+}
+
+int ztm_plan_adjoint_stored(ztm_plan plan, const float* r_dev, float* out_dev, void* stream) {
+    if (!plan) return fail(ZTM_EINVAL, "null plan");
+    if (!out_dev || (plan->shape.m > 0 && !r_dev)) return fail(ZTM_EINVAL, "null r or out");
+    if (plan->shape.m > 0 && !plan->G)
+        return fail(ZTM_EINVAL, "no stored matrix: call ztm_plan_store first");
+    if (int rc = ensure_workspace(plan, (size_t)stored_adj_splits(plan->shape) * plan->shape.n_cells))
+        return rc;
+    return stored_adj_launch(plan->shape, plan->G, r_dev, plan->part, out_dev,
+                             (hipStream_t)stream);
 }
 
 int ztm_plan_smoothness(ztm_plan plan, int precision, const void* rho_dev, void* out_dev,

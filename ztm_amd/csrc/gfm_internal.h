@@ -140,6 +140,17 @@ int prep_ref_stations(const float* s, int bfr, int di, int dj, int fh, int fv,
                       float4* st, double4* st_d, int4* st_c, int* out_map,
                       int m_int, int m_pad, hipStream_t stream);
 
+// Stored-matrix variant (gfm_stored.hip): G as FP32 [M][ld], ld = N rounded
+// up to 4 (zero padding), entries scale * K(m,n) evaluated in FP64.
+int64_t stored_ld(const PmShape& s);
+int stored_build_launch(const PmShape& s, bool prism, const double4* st, double sy, double sz,
+                        double scale, float* G, hipStream_t stream);
+int stored_fwd_launch(const PmShape& s, const float* G, const float* rho, double thr,
+                      double* g, hipStream_t stream);
+int stored_adj_splits(const PmShape& s);
+int stored_adj_launch(const PmShape& s, const float* G, const float* r, double* part, float* out,
+                      hipStream_t stream);
+
 // out = L^T L rho for forward-difference gradients on an nx x ny x nz grid
 // (gfm_reg.hip).
 int ltl_launch(int precision, const void* rho, void* out, int nx, int ny, int nz,

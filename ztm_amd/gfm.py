@@ -67,6 +67,11 @@ SIGNATURES = {
     "ztm_plan_forward": (_I, [_P, _I, _P, _P, _P]),
     "ztm_plan_adjoint": (_I, [_P, _I, _P, _P, _P]),
     "ztm_plan_smoothness": (_I, [_P, _I, _P, _P, _P]),
+    "ztm_plan_store_bytes": (_I64, [_P]),
+    "ztm_plan_store": (_I, [_P, _P]),
+    "ztm_plan_store_release": (_I, [_P]),
+    "ztm_plan_forward_stored": (_I, [_P, _P, _P, _P]),
+    "ztm_plan_adjoint_stored": (_I, [_P, _P, _P, _P]),
     "ztm_plan_set_tuning": (_I, [_P, _I, _I, _I, _I]),
     "ztm_plan_get_tuning": (_I, [_P, _IP, _IP, _IP, _IP]),
     "ztm_profile_enable": (_I, [_I]),
@@ -206,6 +211,35 @@ class Plan:
             out = torch.empty(self.n_stations, dtype=torch.float64, device=rho.device)
         _check_t(out, torch.float64, self.n_stations, "g")
         self.forward_ptr(prec, _ptr(rho), _ptr(out), _stream(stream))
+        return out
+
+    # stored-matrix variant
+    def store_bytes(self) -> int:
+        return int(lib().ztm_plan_store_bytes(self._h))
+
+    def store(self, stream=None) -> None:
+        """Materialise G (FP32, entries evaluated in FP64) in device memory."""
+        check(lib().ztm_plan_store(self._h, _stream(stream)))
+
+    def store_release(self) -> None:
+        check(lib().ztm_plan_store_release(self._h))
+
+    def forward_stored(self, rho, out=None, stream=None):
+        import torch
+        _check_t(rho, torch.float32, self.n_cells, "rho")
+        if out is None:
+            out = torch.empty(self.n_stations, dtype=torch.float64, device=rho.device)
+        _check_t(out, torch.float64, self.n_stations, "g")
+        check(lib().ztm_plan_forward_stored(self._h, _ptr(rho), _ptr(out), _stream(stream)))
+        return out
+
+    def adjoint_stored(self, r, out=None, stream=None):
+        import torch
+        _check_t(r, torch.float32, self.n_stations, "r")
+        if out is None:
+            out = torch.empty(self.n_cells, dtype=torch.float32, device=r.device)
+        _check_t(out, torch.float32, self.n_cells, "out")
+        check(lib().ztm_plan_adjoint_stored(self._h, _ptr(r), _ptr(out), _stream(stream)))
         return out
 
     def smoothness(self, rho, out=None, stream=None):
