@@ -105,6 +105,14 @@ struct ztm_plan_s {
     int* col_start = nullptr;    // near-field gather index (general plans)
     int* col_st = nullptr;
     int kmin = 0, kmax = -1;
+    int* row_start = nullptr;    // clamped-row gather index (general plans)
+    int* row_st = nullptr;
+    int jr = 1, kr = 1, rmy = kColMargin, rmz = kColMargin;
+    int rx = kNear, ry = kNear, rz = kNear;      // correction box radii
+    int nrx = kNear, nry = kNear, nrz = kNear;   // FP64 near-field radii
+    int cmx = kColMargin, cmy = kColMargin;      // column-index margins
+    int clamp_margin = kColMargin;               // nearest-index clamp
+    int rkmin = 1 << 29, rkmax = -(1 << 29);
     int* out_map = nullptr;      // reference mode: slot -> g index
     double* part = nullptr;      // split-K workspace
     size_t part_cap = 0;         // doubles
@@ -141,6 +149,8 @@ void plan_free(ztm_plan p) {
     if (p->st_c) (void)hipFree(p->st_c);
     if (p->col_start) (void)hipFree(p->col_start);
     if (p->col_st) (void)hipFree(p->col_st);
+    if (p->row_start) (void)hipFree(p->row_start);
+    if (p->row_st) (void)hipFree(p->row_st);
     if (p->out_map) (void)hipFree(p->out_map);
     if (p->part) (void)hipFree(p->part);
     if (p->G) (void)hipFree(p->G);
@@ -161,14 +171,64 @@ int set_shape(ztm_plan p, int64_t m) {
     p->L = std::fabs(g.dx);
     p->sy = g.dy / p->L;
     p->sz = g.dz / p->L;
+    // Rows whose FP32 term a station can clamp lie within |dy|,|dz| < 0.5 of
+    // it: at most ceil(0.5/|s| + 0.5) row indices from its nearest one (+1
+    // for rounding).
+    p->jr = (int)std::ceil(0.5 / std::fabs(p->sy) + 0.5) + 1;
+    p->kr = (int)std::ceil(0.5 / std::fabs(p->sz) + 0.5) + 1;
+    p->rmy = std::max(kColMargin, p->jr + 1);
+    p->rmz = std::max(kColMargin, p->kr + 1);
+    // Correction radii per axis in index units, scaled by the largest cell edge
+    // so that each zone covers the same distance on every axis.
+    const double ex = 1.0, ey = std::fabs(p->sy), ez = std::fabs(p->sz);
+    const double emax = std::max(ex, std::max(ey, ez));
+    auto rad = [&](int k, double e) { return (int)std::ceil(k * emax / e - 1e-9); };
+    p->nrx = rad(kNear, ex);
+    p->nry = rad(kNear, ey);
+    p->nrz = rad(kNear, ez);
+    const bool prism = p->prob.formula == ZTM_PRISM;
+    p->rx = prism ? rad(kMid, ex) : p->nrx;
+    p->ry = prism ? rad(kMid, ey) : p->nry;
+    p->rz = prism ? rad(kMid, ez) : p->nrz;
+    p->cmx = std::max(kColMargin, p->rx + 1);
+    p->cmy = std::max(kColMargin, p->ry + 1);
+    p->clamp_margin = std::max(std::max(p->cmx, p->cmy), std::max(p->rz + 1, std::max(p->rmy, p->rmz)));
     return ZTM_OK;
+}
+
+// Counting sort of stations by key (key < 0: left out), stable in station
+// order: start[nkeys + 1], list[start[nkeys]] (at least one entry).
+void build_csr(const std::vector<int>& key, int nkeys, std::vector<int>& start,
+               std::vector<int>& list) {
+    start.assign((size_t)nkeys + 1, 0);
+    for (int k : key)
+        if (k >= 0) ++start[(size_t)k + 1];
+    for (size_t c = 1; c < start.size(); ++c) start[c] += start[c - 1];
+    list.assign(std::max(1, start.back()), 0);
+    std::vector<int> fill(start.begin(), start.end() - 1);
+    for (size_t m = 0; m < key.size(); ++m)
+        if (key[m] >= 0) list[fill[key[m]]++] = (int)m;
+}
+
+template <typename T>
+hipError_t upload(T** dst, const std::vector<T>& src) {
+    hipError_t e = hipMalloc(dst, sizeof(T) * src.size());
+    if (e == hipSuccess) e = hipMemcpy(*dst, src.data(), sizeof(T) * src.size(), hipMemcpyHostToDevice);
+    return e;
 }
 
 NearArgs near_args(const ztm_plan p) {
     NearArgs a;
     const bool prism = p->prob.formula == ZTM_PRISM;
     a.mode = prism ? kCorrPrism : kCorrPmNear;
-    a.radius = prism ? kMid : kNear;
+    a.rx = p->rx;
+    a.ry = p->ry;
+    a.rz = p->rz;
+    a.nrx = p->nrx;
+    a.nry = p->nry;
+    a.nrz = p->nrz;
+    a.cmx = p->cmx;
+    a.cmy = p->cmy;
     a.st_f = p->st_f;
     a.st_d = p->st_d;
     a.st_c = p->st_c;
@@ -185,6 +245,14 @@ NearArgs near_args(const ztm_plan p) {
     a.vol = std::fabs(p->sy * p->sz);
     a.kmin = p->kmin;
     a.kmax = p->kmax;
+    a.row_start = p->row_start;
+    a.row_st = p->row_st;
+    a.jr = p->jr;
+    a.kr = p->kr;
+    a.rmy = p->rmy;
+    a.rmz = p->rmz;
+    a.rkmin = p->rkmin;
+    a.rkmax = p->rkmax;
     // Quadrupole term of the far field for non-cubic prism cells (half-widths
     // in cell units); it vanishes identically for cubes.
     const double hx = 0.5, hy = 0.5 * std::fabs(p->sy), hz = 0.5 * std::fabs(p->sz);
@@ -253,8 +321,6 @@ int ztm_plan_create(ztm_plan* plan, const ztm_problem* problem, int64_t n_statio
     std::vector<float4> hf(p->shape.m_pad, make_float4(0.f, 0.f, 0.f, 0.f));
     std::vector<double4> hd(p->shape.m_pad, make_double4(0.0, 0.0, 0.0, 0.0));
     std::vector<int4> hc(p->shape.m_pad, make_int4(0, 0, 0, 0));
-    const int wcols = g.nx + 2 * kColMargin, hcols = g.ny + 2 * kColMargin;
-    std::vector<int> col_count((size_t)wcols * hcols + 1, 0);
     for (int64_t m = 0; m < n_stations; ++m) {
         const double x = stations_host[3 * m], y = stations_host[3 * m + 1],
                      z = stations_host[3 * m + 2];
@@ -268,52 +334,63 @@ int ztm_plan_create(ztm_plan* plan, const ztm_problem* problem, int64_t n_statio
         hf[m] = make_float4((float)X, (float)Y, (float)Z, 0.f);
         hd[m] = make_double4(X, Y, Z, 0.0);
         // nearest cell index per axis, clamped far outside the grid so it fits int
-        auto idx = [](double v, int n) {
+        const double cm = p->clamp_margin;
+        auto idx = [cm](double v, int n) {
             const double c = std::nearbyint(v);
-            return (int)std::max(-(double)kColMargin - 1.0, std::min(c, (double)n + kColMargin));
+            return (int)std::max(-cm - 1.0, std::min(c, (double)n + cm));
         };
         hc[m] = make_int4(idx(X, g.nx), idx(Y / p->sy, g.ny), idx(Z / p->sz, g.nz), 0);
     }
-    // Stations grouped by nearest (ci, cj) column for the adjoint's near-field
-    // gather (counting sort, stable in station order).
-    std::vector<int> col_of((size_t)n_stations, -1);
-    bool any_near = false;
+    // Adjoint gather indexes (stable counting sorts by key):
+    //  columns: stations by nearest (ci, cj) over the (cmx, cmy)-extended range;
+    //  rows:    stations that may clamp some grid row, by nearest (cj, ck).
+    const int wcols = g.nx + 2 * p->cmx, hcols = g.ny + 2 * p->cmy;
+    const int wkey = g.ny + 2 * p->rmy, hkey = g.nz + 2 * p->rmz;
+    std::vector<int> col_key((size_t)n_stations, -1), row_key((size_t)n_stations, -1);
+    bool any_near = false, any_row = false;
     for (int64_t m = 0; m < n_stations; ++m) {
-        const int ci = hc[m].x + kColMargin, cj = hc[m].y + kColMargin;
-        if (ci < 0 || ci >= wcols || cj < 0 || cj >= hcols) continue;
-        col_of[m] = cj * wcols + ci;
-        ++col_count[col_of[m] + 1];
-        p->kmin = any_near ? std::min(p->kmin, hc[m].z) : hc[m].z;
-        p->kmax = any_near ? std::max(p->kmax, hc[m].z) : hc[m].z;
-        any_near = true;
+        const int ci = hc[m].x + p->cmx, cj = hc[m].y + p->cmy;
+        if (ci >= 0 && ci < wcols && cj >= 0 && cj < hcols) {
+            col_key[m] = cj * wcols + ci;
+            p->kmin = any_near ? std::min(p->kmin, hc[m].z) : hc[m].z;
+            p->kmax = any_near ? std::max(p->kmax, hc[m].z) : hc[m].z;
+            any_near = true;
+        }
+        const int rj = hc[m].y + p->rmy, rk = hc[m].z + p->rmz;
+        if (rj < 0 || rj >= wkey || rk < 0 || rk >= hkey) continue;
+        // superset of the kernels' FP32 test dy^2 + dz^2 < 0.25 (cell units)
+        bool clamps = false;
+        const double Y = hd[m].y, Z = hd[m].z;
+        for (int k = std::max(0, hc[m].z - p->kr); k <= std::min(g.nz - 1, hc[m].z + p->kr) && !clamps; ++k)
+            for (int j = std::max(0, hc[m].y - p->jr); j <= std::min(g.ny - 1, hc[m].y + p->jr); ++j) {
+                const double dy = Y - j * p->sy, dz = Z - k * p->sz;
+                const double tol = 1e-4 + 1e-6 * (std::fabs(Y) + std::fabs(Z) + std::fabs(j * p->sy) +
+                                                  std::fabs(k * p->sz));
+                if (std::fabs(dy) < 0.5 + tol && std::fabs(dz) < 0.5 + tol) {
+                    clamps = true;
+                    break;
+                }
+            }
+        if (!clamps) continue;
+        row_key[m] = rk * wkey + rj;
+        p->rkmin = any_row ? std::min(p->rkmin, hc[m].z) : hc[m].z;
+        p->rkmax = any_row ? std::max(p->rkmax, hc[m].z) : hc[m].z;
+        any_row = true;
     }
-    if (!any_near) {   // no station is near any cell: the gather is skipped
+    if (!any_near) {   // no station is near any cell: the column gather is skipped
         p->kmin = 1 << 29;
         p->kmax = -(1 << 29);
     }
-    for (size_t c = 1; c < col_count.size(); ++c) col_count[c] += col_count[c - 1];
-    std::vector<int> col_st(std::max<int>(1, col_count.back()), 0);
-    {
-        std::vector<int> fill(col_count.begin(), col_count.end() - 1);
-        for (int64_t m = 0; m < n_stations; ++m)
-            if (col_of[m] >= 0) col_st[fill[col_of[m]]++] = (int)m;
-    }
-    hipError_t e = hipMalloc(&p->st_f, sizeof(float4) * p->shape.m_pad);
-    if (e == hipSuccess) e = hipMalloc(&p->st_d, sizeof(double4) * p->shape.m_pad);
-    if (e == hipSuccess) e = hipMalloc(&p->st_c, sizeof(int4) * p->shape.m_pad);
-    if (e == hipSuccess) e = hipMalloc(&p->col_start, sizeof(int) * col_count.size());
-    if (e == hipSuccess) e = hipMalloc(&p->col_st, sizeof(int) * col_st.size());
-    if (e == hipSuccess)
-        e = hipMemcpy(p->st_f, hf.data(), sizeof(float4) * p->shape.m_pad, hipMemcpyHostToDevice);
-    if (e == hipSuccess)
-        e = hipMemcpy(p->st_d, hd.data(), sizeof(double4) * p->shape.m_pad, hipMemcpyHostToDevice);
-    if (e == hipSuccess)
-        e = hipMemcpy(p->st_c, hc.data(), sizeof(int4) * p->shape.m_pad, hipMemcpyHostToDevice);
-    if (e == hipSuccess)
-        e = hipMemcpy(p->col_start, col_count.data(), sizeof(int) * col_count.size(),
-                      hipMemcpyHostToDevice);
-    if (e == hipSuccess)
-        e = hipMemcpy(p->col_st, col_st.data(), sizeof(int) * col_st.size(), hipMemcpyHostToDevice);
+    std::vector<int> col_start, col_st, row_start, row_st;
+    build_csr(col_key, wcols * hcols, col_start, col_st);
+    build_csr(row_key, wkey * hkey, row_start, row_st);
+    hipError_t e = upload(&p->st_f, hf);
+    if (e == hipSuccess) e = upload(&p->st_d, hd);
+    if (e == hipSuccess) e = upload(&p->st_c, hc);
+    if (e == hipSuccess) e = upload(&p->col_start, col_start);
+    if (e == hipSuccess) e = upload(&p->col_st, col_st);
+    if (e == hipSuccess && any_row) e = upload(&p->row_start, row_start);
+    if (e == hipSuccess && any_row) e = upload(&p->row_st, row_st);
     if (e != hipSuccess) {
         plan_free(p);
         return hip_fail(e, "plan station upload");

@@ -21,33 +21,29 @@ int hip_fail(hipError_t e, const char* what);
 
 // Station records as the kernels read them: coordinates relative to the
 // centre of cell (0,0,0) in units of L = |dx|, scaled so that cell (i,j,k)
-// sits at (i, j*sy, k*sz). x is stored as (x - x0)/dx, so the sign of dx only
-// flips the sign of a squared term. w is unused by the prepared copy.
-struct GeomF {
-    float sy, sz;      // dy/L, dz/L (signed)
-    float thr;         // forward threshold on |rho|
-};
-struct GeomD {
-    double sy, sz, thr;
-};
+// sits at (i, j*sy, k*sz) with sy = dy/L, sz = dz/L (signed). x is stored as
+// (x - x0)/dx, so the sign of dx only flips the sign of a squared term.
 
-// Small floor keeps r^2 > 0 when a station coincides with a cell centre, so
-// the (zero-length) vertical offset contributes 0 instead of 0*inf = NaN.
-// In units of L^2 it is far below FP32 resolution of any real distance.
-constexpr float  kEpsF = 1e-20f;
+// Small floor keeps r^2 > 0 in FP64 when a station coincides with a cell
+// centre, so the (zero-length) vertical offset contributes 0 instead of
+// 0*inf = NaN. In units of L^2 it is far below any real distance.
 constexpr double kEpsD = 1e-20;
+// FP32 kernels clamp a row's dy^2 + dz^2 (cell units) below at this value:
+// half a cell width squared (see row_R32 in gfm_pair.h).
+constexpr float  kRmin2F = 0.25f;
 
 // Padding of prepared station arrays (largest forward tile = 64 lanes * 8).
 constexpr int kStationPad = 512;
 
-// Near field: pairs whose cell index lies within kNear cells (per axis) of
-// the station's nearest cell index are re-evaluated in FP64 by the FP32 path.
+// Near field: pairs within kNear of the station's nearest cell (measured in
+// units of the cell's largest edge, per axis) are re-evaluated in FP64 by the
+// FP32 path. Prism mid zone: pairs within kMid are corrected from the
+// far-field kernel to the prism integral (FP64 closed form in the near field,
+// FP32 3x3x3 Gauss-Legendre beyond). Per-axis index radii: r_a =
+// ceil(k * max_edge / edge_a), e.g. (2,2,2) and (7,7,7) for cubes.
 constexpr int kNear = 2;
-// Prism mid zone: cells within kMid cells (per axis) of a station are
-// corrected from the far-field kernel to the prism integral (FP64 closed
-// form up to kNear, FP32 3x3x3 Gauss-Legendre beyond).
 constexpr int kMid = 7;
-// Column margin of the station-by-column index (covers kNear and kMid).
+// Least margin of the station-by-column index.
 constexpr int kColMargin = 8;
 
 // Quadrupole term of a non-cubic cell's far field, cell units (L = |dx|),
@@ -63,12 +59,13 @@ enum CorrMode { kCorrNone = 0, kCorrPmNear = 1, kCorrPrism = 2 };
 // Everything the FP32 corrections need (mode kCorrNone disables them).
 // st_c[m] = (ci, cj, ck, 0): the station's nearest cell index per axis.
 // col_start/col_st: stations grouped by (ci, cj) column over the extended
-// range [-kColMargin, nx+kColMargin) x [-kColMargin, ny+kColMargin) (adjoint
-// gather); stations outside it are never near any cell. kmin/kmax: range of
-// ck over them.
+// range [-cmx, nx+cmx) x [-cmy, ny+cmy) (adjoint gather); stations outside it
+// are never near any cell. kmin/kmax: range of ck over them.
 struct NearArgs {
     int mode = kCorrNone;
-    int radius = kNear;
+    int rx = kNear, ry = kNear, rz = kNear;      // correction box radii
+    int nrx = kNear, nry = kNear, nrz = kNear;   // FP64 near-field radii
+    int cmx = kColMargin, cmy = kColMargin;      // column-index margins
     int quad = 0;
     QuadArgs q;
     const float4* st_f = nullptr;
@@ -77,6 +74,14 @@ struct NearArgs {
     const int* col_start = nullptr;
     const int* col_st = nullptr;
     const double* corr = nullptr;   // forward prism: per-station correction
+    // Stations that may have clamped rows, grouped by (cj, ck) row key over
+    // [-rmy, ny+rmy) x [-rmz, nz+rmz); jr/kr: the
+    // row-index window around (cj, ck) a clamped row can lie in; rkmin/rkmax:
+    // range of ck over them (adjoint gather).
+    const int* row_start = nullptr;
+    const int* row_st = nullptr;
+    int jr = 1, kr = 1, rkmin = 1 << 29, rkmax = -(1 << 29);
+    int rmy = kColMargin, rmz = kColMargin;   // row-key margins in j and k
     int nx = 0, ny = 0, nz = 0;
     float syf = 1.f, szf = 1.f, thr = 0.f;
     double syd = 1.0, szd = 1.0, vol = 1.0;   // vol = |sy*sz|, cell volume / L^3

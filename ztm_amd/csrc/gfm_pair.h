@@ -12,8 +12,19 @@ namespace ztm {
 
 __device__ __forceinline__ float rsq_f(float x) { return __builtin_amdgcn_rsqf(x); }
 
+// Row term of the FP32 kernels: R = dy^2 + dz^2, clamped below at kRmin2F.
+// A station closer than 0.5 cell widths to a row's axis would otherwise put a
+// near-singular term into that row's FP32 sum and swamp the rest of the row;
+// clamped rows are corrected in FP64 afterwards (row_clamped, corr_pair).
+__device__ __forceinline__ float row_R32(float dy, float dz) {
+    return fmaxf(fmaf(dy, dy, dz * dz), kRmin2F);
+}
+__device__ __forceinline__ bool row_clamped(float dy, float dz) {
+    return fmaf(dy, dy, dz * dz) < kRmin2F;
+}
+
 // Far-field kernel of one pair in FP32, exactly as the main kernels evaluate
-// it. a = X - i, R = dy^2 + dz^2 (+eps), qrow = q2y dy^2 + q2z dz^2.
+// it. a = X - i, R = row_R32(dy, dz), qrow = q2y dy^2 + q2z dz^2.
 template <bool QUAD>
 __device__ __forceinline__ float pair_k32(float a, float R, float qrow, const QuadArgs& q) {
     const float v = rsq_f(fmaf(a, a, R));
@@ -91,11 +102,14 @@ __device__ __forceinline__ float gl3_cell(float X, float Y, float Z, int i, int 
 
 // Correction of one station-cell pair for the FP32 path: the better value of
 // the pair minus its FP32 far-field term as the main kernels evaluate it (same
-// rounded coordinates, same v_rsq_f32), so adding it replaces the pair's FP32
-// value. Terms are  w * dz * K  in cell units (K = 1/r^3 for a point mass).
-//   kCorrPmNear: better value = the point-mass term in FP64.
-//   kCorrPrism:  better value = the prism integral / cell volume, FP64 closed
-//                form within kNear cells, FP32 3x3x3 Gauss-Legendre beyond.
+// rounded coordinates, same row clamp, same v_rsq_f32), so adding it replaces
+// the pair's FP32 value. Terms are  w * dz * K  in cell units (K = 1/r^3 for a
+// point mass). The better value depends on where the cell lies relative to
+// the station's nearest cell (per-axis index radii, NearArgs):
+//   near box (nr*):                FP64 point mass, or FP64 prism closed form;
+//   correction box (r*), prism:    FP32 3x3x3 Gauss-Legendre mean of dz/r^3;
+//   beyond (cells of a clamped row): FP64 far-field kernel (point mass, plus
+//                                 the quadrupole term for non-cubic prisms).
 template <bool QUAD>
 __device__ __forceinline__ double corr_pair(int mode, float w, const float4& pf,
                                             const double4& pd, const int4& c, int i, int j,
@@ -103,24 +117,59 @@ __device__ __forceinline__ double corr_pair(int mode, float w, const float4& pf,
     const float dyf = pf.y - (float)j * a.syf;
     const float dzf = pf.z - (float)k * a.szf;
     const float af = pf.x - (float)i;
-    const float R = fmaf(dyf, dyf, fmaf(dzf, dzf, kEpsF));
+    const float R = row_R32(dyf, dzf);
     const float qrow = QUAD ? fmaf(a.q.q2y, dyf * dyf, a.q.q2z * (dzf * dzf)) : 0.f;
     const double t32 = (double)dzf * (double)(w * pair_k32<QUAD>(af, R, qrow, a.q));
+    const int di = abs(i - c.x), dj = abs(j - c.y), dk = abs(k - c.z);
+    const bool in_near = di <= a.nrx && dj <= a.nry && dk <= a.nrz;
+    const bool in_box = di <= a.rx && dj <= a.ry && dk <= a.rz;
+    const double ad = pd.x - (double)i;
+    const double dyd = pd.y - (double)j * a.syd;
+    const double dzd = pd.z - (double)k * a.szd;
     double better;
-    if (mode == kCorrPmNear) {
-        const double ad = pd.x - (double)i;
-        const double dyd = pd.y - (double)j * a.syd;
-        const double dzd = pd.z - (double)k * a.szd;
-        const double r2 = ad * ad + dyd * dyd + dzd * dzd + kEpsD;
-        better = (double)w * dzd / (r2 * sqrt(r2));
+    if (mode == kCorrPrism && in_near) {
+        better = prism_cell(pd.x, pd.y, pd.z, i, j, k, a.syd, a.szd) / a.vol;
+    } else if (mode == kCorrPrism && in_box) {
+        better = (double)gl3_cell(pf.x, pf.y, pf.z, i, j, k, a.syf, a.szf);
     } else {
-        const int cheb = max(abs(i - c.x), max(abs(j - c.y), abs(k - c.z)));
-        if (cheb <= kNear)
-            better = (double)w * prism_cell(pd.x, pd.y, pd.z, i, j, k, a.syd, a.szd) / a.vol;
-        else
-            better = (double)w * (double)gl3_cell(pf.x, pf.y, pf.z, i, j, k, a.syf, a.szf);
+        const double r2 = ad * ad + dyd * dyd + dzd * dzd + kEpsD;
+        const double v2 = 1.0 / r2;
+        better = dzd * v2 * sqrt(v2);
+        if (QUAD && mode == kCorrPrism) {
+            const double q = (double)a.q.q2x * ad * ad + (double)a.q.q2y * dyd * dyd +
+                             (double)a.q.q2z * dzd * dzd;
+            better *= 1.0 + v2 * (q * v2 - (double)a.q.qz);
+        }
     }
-    return better - t32;
+    return (double)w * better - t32;
+}
+
+// Sum of corr_pair over the cells of the station's clamped rows that lie
+// outside its correction box (radii rx, ry, rz); the box itself is corrected by
+// the caller. Rows are the (j,k) within (jr, kr) of (c.y, c.z) whose FP32 row
+// term was clamped (row_clamped). Lanes [lane0, lane0 + nlanes) split the
+// cells of each row.
+template <bool QUAD>
+__device__ __forceinline__ double clamped_rows_corr(int mode, const float4& pf, const double4& pd,
+                                                    const int4& c, const float* __restrict__ rho,
+                                                    const NearArgs& a, int lane0, int nlanes) {
+    double acc = 0.0;
+    const int k0 = max(c.z - a.kr, 0), k1 = min(c.z + a.kr, a.nz - 1);
+    const int j0 = max(c.y - a.jr, 0), j1 = min(c.y + a.jr, a.ny - 1);
+    for (int kk = k0; kk <= k1; ++kk)
+        for (int jj = j0; jj <= j1; ++jj) {
+            const float dyf = pf.y - (float)jj * a.syf;
+            const float dzf = pf.z - (float)kk * a.szf;
+            if (!row_clamped(dyf, dzf)) continue;
+            const bool row_in_box = abs(jj - c.y) <= a.ry && abs(kk - c.z) <= a.rz;
+            for (int ii = lane0; ii < a.nx; ii += nlanes) {
+                if (row_in_box && abs(ii - c.x) <= a.rx) continue;
+                const float w = rho[((size_t)kk * a.ny + jj) * a.nx + ii];
+                if (!(fabsf(w) > a.thr)) continue;
+                acc += corr_pair<QUAD>(mode, w, pf, pd, c, ii, jj, kk, a);
+            }
+        }
+    return acc;
 }
 
 }  // namespace ztm

@@ -36,9 +36,12 @@ template <> struct Vec4<double> { using type = double4; };
 
 __device__ __forceinline__ float  rsq(float x)  { return __builtin_amdgcn_rsqf(x); }
 __device__ __forceinline__ double rsq(double x) { return rsqrt(x); }
-template <typename T> __device__ __forceinline__ T eps_r2();
-template <> __device__ __forceinline__ float  eps_r2<float>()  { return kEpsF; }
-template <> __device__ __forceinline__ double eps_r2<double>() { return kEpsD; }
+// Row term dy^2 + dz^2: clamped in FP32 (row_R32, gfm_pair.h; the pairs of a
+// clamped row are corrected in FP64 afterwards), floored by kEpsD in FP64.
+__device__ __forceinline__ float row_R(float dy, float dz) { return row_R32(dy, dz); }
+__device__ __forceinline__ double row_R(double dy, double dz) {
+    return fma(dy, dy, fma(dz, dz, kEpsD));
+}
 
 // Far-field kernel of one pair: v^3 (point mass), or with the quadrupole term
 // of a non-cubic cell (FP32 prism path only; see gfm_pair.h).
@@ -107,7 +110,7 @@ __global__ __launch_bounds__(256) void pm_fwd_kernel(
         for (int s = 0; s < S; ++s) {
             const T dy = Y[s] - yj;
             dz[s] = Z[s] - zk;
-            R[s] = fma(dy, dy, fma(dz[s], dz[s], eps_r2<T>()));
+            R[s] = row_R(dy, dz[s]);
             QR[s] = QUAD ? fma(T(qa.q2y), dy * dy, T(qa.q2z) * (dz[s] * dz[s])) : T(0);
             ar[s] = T(0);
             xm[s] = X[s];
@@ -171,9 +174,9 @@ __global__ __launch_bounds__(256) void pm_fwd_reduce_kernel(const double* __rest
         const double4 pd = na.st_d[i];
         const int4 c = na.st_c[i];
         double corr = 0.0;
-        const int k0 = max(c.z - kNear, 0), k1 = min(c.z + kNear, na.nz - 1);
-        const int j0 = max(c.y - kNear, 0), j1 = min(c.y + kNear, na.ny - 1);
-        const int i0 = max(c.x - kNear, 0), i1 = min(c.x + kNear, na.nx - 1);
+        const int k0 = max(c.z - na.rz, 0), k1 = min(c.z + na.rz, na.nz - 1);
+        const int j0 = max(c.y - na.ry, 0), j1 = min(c.y + na.ry, na.ny - 1);
+        const int i0 = max(c.x - na.rx, 0), i1 = min(c.x + na.rx, na.nx - 1);
         for (int kk = k0; kk <= k1; ++kk)
             for (int jj = j0; jj <= j1; ++jj)
                 for (int ii = i0; ii <= i1; ++ii) {
@@ -181,6 +184,7 @@ __global__ __launch_bounds__(256) void pm_fwd_reduce_kernel(const double* __rest
                     if (!(fabsf(w) > na.thr)) continue;
                     corr += corr_pair<false>(kCorrPmNear, w, pf, pd, c, ii, jj, kk, na);
                 }
+        corr += clamped_rows_corr<false>(kCorrPmNear, pf, pd, c, rho_f, na, 0, 1);
         s += corr;
     } else if (na.mode == kCorrPrism) {
         s += na.corr[i];
@@ -241,7 +245,7 @@ __global__ __launch_bounds__(256) void pm_adj_kernel(
             const V4 p = tile[s];    // wave-uniform LDS broadcast
             const T dy = p.y - yj;
             const T dz = p.z - zk;
-            const T R = fma(dy, dy, fma(dz, dz, eps_r2<T>()));
+            const T R = row_R(dy, dz);
             const T qrow = QUAD ? fma(T(qa.q2y), dy * dy, T(qa.q2z) * (dz * dz)) : T(0);
             const T q = p.w * dz;
             const T xm = p.x - xi0;
@@ -293,30 +297,54 @@ __global__ __launch_bounds__(256) void pm_adj_reduce_kernel(const double* __rest
     double s = 0.0;
     for (int k = 0; k < n_splits; ++k) s += part[(size_t)k * n_cells + n];
     if (na.mode != kCorrNone) {
-        const int R = na.radius;
         const int ii = (int)(n % na.nx);
         const int64_t rw = n / na.nx;
         const int jj = (int)(rw % na.ny);
         const int kk = (int)(rw / na.ny);
-        if (kk >= na.kmin - R && kk <= na.kmax + R) {
-            const int wcols = na.nx + 2 * kColMargin;
-            double corr = 0.0;
-            for (int cj = jj - R; cj <= jj + R; ++cj) {
-                if (cj < -kColMargin || cj >= na.ny + kColMargin) continue;
-                for (int ci = ii - R; ci <= ii + R; ++ci) {
-                    if (ci < -kColMargin || ci >= na.nx + kColMargin) continue;
-                    const int col = (cj + kColMargin) * wcols + (ci + kColMargin);
+        double corr = 0.0;
+        // (a) stations whose correction box holds this cell: via the column index
+        if (kk >= na.kmin - na.rz && kk <= na.kmax + na.rz) {
+            const int wcols = na.nx + 2 * na.cmx;
+            for (int cj = jj - na.ry; cj <= jj + na.ry; ++cj) {
+                if (cj < -na.cmy || cj >= na.ny + na.cmy) continue;
+                for (int ci = ii - na.rx; ci <= ii + na.rx; ++ci) {
+                    if (ci < -na.cmx || ci >= na.nx + na.cmx) continue;
+                    const int col = (cj + na.cmy) * wcols + (ci + na.cmx);
                     for (int q = na.col_start[col]; q < na.col_start[col + 1]; ++q) {
                         const int m = na.col_st[q];
                         const int4 c = na.st_c[m];
-                        if (c.z < kk - R || c.z > kk + R) continue;
+                        if (c.z < kk - na.rz || c.z > kk + na.rz) continue;
                         corr += corr_pair<QUAD>(na.mode, r_f[m], na.st_f[m], na.st_d[m], c, ii,
                                                 jj, kk, na);
                     }
                 }
             }
-            s += corr;
         }
+        // (b) stations whose FP32 row term for this cell's row was clamped,
+        //     outside (a): via the row-key index
+        if (na.row_start && kk >= na.rkmin - na.kr && kk <= na.rkmax + na.kr) {
+            const int wkey = na.ny + 2 * na.rmy;
+            for (int ck = kk - na.kr; ck <= kk + na.kr; ++ck) {
+                if (ck < -na.rmz || ck >= na.nz + na.rmz) continue;
+                for (int cj = jj - na.jr; cj <= jj + na.jr; ++cj) {
+                    if (cj < -na.rmy || cj >= na.ny + na.rmy) continue;
+                    const int key = (ck + na.rmz) * wkey + (cj + na.rmy);
+                    for (int q = na.row_start[key]; q < na.row_start[key + 1]; ++q) {
+                        const int m = na.row_st[q];
+                        const int4 c = na.st_c[m];
+                        if (abs(ii - c.x) <= na.rx && abs(jj - c.y) <= na.ry &&
+                            abs(kk - c.z) <= na.rz)
+                            continue;   // done in (a)
+                        const float4 pf = na.st_f[m];
+                        if (!row_clamped(pf.y - (float)jj * na.syf, pf.z - (float)kk * na.szf))
+                            continue;
+                        corr += corr_pair<QUAD>(na.mode, r_f[m], pf, na.st_d[m], c, ii, jj, kk,
+                                                na);
+                    }
+                }
+            }
+        }
+        s += corr;
     }
     out[n] = (T)(scale * s);
 }

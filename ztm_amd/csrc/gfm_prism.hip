@@ -28,8 +28,9 @@
 namespace ztm {
 
 // ------------------------------------------------ FP32 mid-zone correction
-// One wave per station; lanes stride over the (2 kMid + 1)^3 box (clamped to
-// the grid); fixed butterfly reduction, so corr[m] is deterministic.
+// One wave per station; lanes stride over the correction box (radii rx, ry,
+// rz, clamped to the grid) and then the cells of clamped rows outside it;
+// fixed butterfly reduction, so corr[m] is deterministic.
 template <bool QUAD>
 __global__ __launch_bounds__(256) void prism_near_fwd_kernel(NearArgs na,
                                                              const float* __restrict__ rho,
@@ -40,9 +41,9 @@ __global__ __launch_bounds__(256) void prism_near_fwd_kernel(NearArgs na,
     const float4 pf = na.st_f[st];
     const double4 pd = na.st_d[st];
     const int4 c = na.st_c[st];
-    const int i0 = max(c.x - kMid, 0), i1 = min(c.x + kMid, na.nx - 1);
-    const int j0 = max(c.y - kMid, 0), j1 = min(c.y + kMid, na.ny - 1);
-    const int k0 = max(c.z - kMid, 0), k1 = min(c.z + kMid, na.nz - 1);
+    const int i0 = max(c.x - na.rx, 0), i1 = min(c.x + na.rx, na.nx - 1);
+    const int j0 = max(c.y - na.ry, 0), j1 = min(c.y + na.ry, na.ny - 1);
+    const int k0 = max(c.z - na.rz, 0), k1 = min(c.z + na.rz, na.nz - 1);
     const int bi = i1 - i0 + 1, bj = j1 - j0 + 1, bk = k1 - k0 + 1;
     const int total = (bi > 0 && bj > 0 && bk > 0) ? bi * bj * bk : 0;
     double acc = 0.0;
@@ -55,6 +56,7 @@ __global__ __launch_bounds__(256) void prism_near_fwd_kernel(NearArgs na,
         if (!(fabsf(w) > na.thr)) continue;
         acc += corr_pair<QUAD>(kCorrPrism, w, pf, pd, c, ii, jj, kk, na);
     }
+    acc += clamped_rows_corr<QUAD>(kCorrPrism, pf, pd, c, rho, na, lane, 64);
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
     if (lane == 0) corr[st] = acc;
