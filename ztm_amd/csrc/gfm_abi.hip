@@ -117,6 +117,8 @@ struct ztm_plan_s {
     double* part = nullptr;      // split-K workspace
     size_t part_cap = 0;         // doubles
     float* G = nullptr;          // stored matrix [m][stored_ld], or null
+    void* bws = nullptr;         // batched forward workspace
+    size_t bws_cap = 0;          // bytes
 };
 
 namespace {
@@ -154,6 +156,7 @@ void plan_free(ztm_plan p) {
     if (p->out_map) (void)hipFree(p->out_map);
     if (p->part) (void)hipFree(p->part);
     if (p->G) (void)hipFree(p->G);
+    if (p->bws) (void)hipFree(p->bws);
     delete p;
 }
 
@@ -567,6 +570,42 @@ int ztm_plan_adjoint_stored(ztm_plan plan, const float* r_dev, float* out_dev, v
         return rc;
     return stored_adj_launch(plan->shape, plan->G, r_dev, plan->part, out_dev,
                              (hipStream_t)stream);
+}
+
+int ztm_plan_forward_batched(ztm_plan plan, int n_models, const float* rho_dev, double* g_dev,
+                             void* stream) {
+    if (!plan) return fail(ZTM_EINVAL, "null plan");
+    if (n_models < 0 || n_models > 256)
+        return fail(ZTM_EINVAL, "n_models must be in [0, 256] (got %d)", n_models);
+    if (plan->prob.formula != ZTM_POINT_MASS)
+        return fail(ZTM_EINVAL, "the batched forward supports the point-mass formula only");
+    if (n_models == 0 || plan->shape.m == 0) return ZTM_OK;
+    if (!rho_dev || !g_dev) return fail(ZTM_EINVAL, "null rho or g");
+    int dev = 0;
+    ZTM_HIP(hipGetDevice(&dev));
+    if (dev != plan->device)
+        return fail(ZTM_EINVAL, "plan belongs to device %d, current device is %d", plan->device, dev);
+    const int ks = plan->tuning.batched_slices > 0
+                       ? plan->tuning.batched_slices
+                       : batched_default_slices(plan->shape, n_models);
+    const size_t need = batched_workspace(plan->shape, n_models, ks);
+    if (need > plan->bws_cap) {
+        if (plan->bws) (void)hipFree(plan->bws);
+        plan->bws = nullptr;
+        plan->bws_cap = 0;
+        ZTM_HIP(hipMalloc(&plan->bws, need));
+        plan->bws_cap = need;
+    }
+    return batched_fwd_launch(plan->shape, n_models, ks, (const float*)plan->st_f, rho_dev,
+                              plan->prob.threshold, plan->sy, plan->sz, pm_scale(plan), plan->bws,
+                              g_dev, (hipStream_t)stream);
+}
+
+int ztm_plan_set_batched_slices(ztm_plan plan, int slices) {
+    if (!plan) return fail(ZTM_EINVAL, "null plan");
+    if (slices < 0) return fail(ZTM_EINVAL, "negative slice count");
+    plan->tuning.batched_slices = slices;
+    return ZTM_OK;
 }
 
 int ztm_plan_smoothness(ztm_plan plan, int precision, const void* rho_dev, void* out_dev,

@@ -1,0 +1,234 @@
+// Batched (ensemble) forward on MFMA: Y = G P for P = [B models x N cells],
+// with G (stations x cells) generated on the fly -- never stored.
+//
+// Split-BF16 scheme: G and P are each split into hi + lo BF16 parts
+// (hi = bf16(x), lo = bf16(x - hi), ~2^-16 relative together); the product
+// uses the three significant terms  hi*hi + hi*lo + lo*hi  on
+// v_mfma_f32_32x32x16_bf16 with FP32 accumulation (lo*lo, ~2^-32, dropped).
+// Each K-slice's FP32 accumulators are written out and the slices are summed
+// in FP64, which bounds the FP32 accumulation run to one slice.
+//
+// Work decomposition: a workgroup = 4 waves = 128 stations (32 per wave, the
+// MFMA's M); each wave keeps all B models' accumulators (B/32 tiles of 32x32,
+// 16 floats per lane each). The K loop walks the slice's cells 16 at a time:
+// the workgroup stages the chunk of packed P (hi and lo, B x 16 cells) in LDS
+// once, every wave generates its own 32 x 16 block of G in registers directly
+// in the MFMA A-operand layout (lane: station lane%32, cells 8*(lane/32)+0..7),
+// converts it to hi/lo and issues 3 MFMAs per model tile. Workgroups of one
+// K-slice differ only in stations; blockIdx.x runs over stations so that the
+// round-robin dispatch spreads one slice's tiles over the XCDs in sequence
+// and concurrently resident tiles of a slice share its P chunks in L2.
+//
+// Layouts (HBM):  Phi/Plo [N16/16][Bp][16] bf16, zero padded (N16 = N up to a
+//                 multiple of 16, Bp = B up to a multiple of 32);
+//                 part [KS][m_pad][Bp] FP32;  Y [B][M] FP64 (model-major).
+#include "gfm_internal.h"
+#include "gfm_pair.h"
+
+#include <algorithm>
+
+namespace ztm {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+constexpr int kBatchChunk = 16;   // cells per MFMA K step
+constexpr int kBatchTile = 128;   // stations per workgroup
+
+// ---------------------------------------------------------------- packing
+// One thread per packed element: t = ((chunk * Bp) + model) * 16 + c.
+__global__ void batched_pack_kernel(const float* __restrict__ P, int B, int64_t n_cells,
+                                    int Bp, int64_t n_packed, float thr,
+                                    __bf16* __restrict__ hi, __bf16* __restrict__ lo) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n_packed) return;
+    const int c = (int)(t & 15);
+    const int64_t cm = t >> 4;
+    const int model = (int)(cm % Bp);
+    const int64_t cell = (cm / Bp) * kBatchChunk + c;
+    float v = 0.f;
+    if (model < B && cell < n_cells) {
+        v = P[(size_t)model * n_cells + cell];
+        if (!(fabsf(v) > thr)) v = 0.f;
+    }
+    const __bf16 h = (__bf16)v;
+    hi[t] = h;
+    lo[t] = (__bf16)(v - (float)h);
+}
+
+// --------------------------------------------------------------- forward
+template <int NT>   // model tiles of 32 (B <= 32 * NT)
+__global__ __launch_bounds__(256) void batched_fwd_kernel(
+    const float4* __restrict__ st, const bf16x8* __restrict__ phi,
+    const bf16x8* __restrict__ plo, int nx, int ny, int64_t n_cells, float sy, float sz,
+    int64_t n_chunks, int chunks_per_slice, int m_pad, float* __restrict__ part) {
+    constexpr int Bp = 32 * NT;
+    __shared__ bf16x8 lhi[Bp * 2];
+    __shared__ bf16x8 llo[Bp * 2];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int row = lane & 31, kh = lane >> 5;
+    const int mbase = blockIdx.x * kBatchTile + wave * 32;
+    const float4 p = st[mbase + row];
+
+    f32x16 acc[NT];
+#pragma unroll
+    for (int t = 0; t < NT; ++t)
+#pragma unroll
+        for (int i = 0; i < 16; ++i) acc[t][i] = 0.f;
+
+    const int64_t c0 = (int64_t)blockIdx.y * chunks_per_slice;
+    const int64_t c1 = min(c0 + (int64_t)chunks_per_slice, n_chunks);
+    for (int64_t chunk = c0; chunk < c1; ++chunk) {
+        __syncthreads();
+        for (int v = threadIdx.x; v < Bp * 2; v += blockDim.x) {
+            lhi[v] = phi[chunk * (Bp * 2) + v];
+            llo[v] = plo[chunk * (Bp * 2) + v];
+        }
+        // this lane's 8 cells: chunk*16 + 8*kh + 0..7, decoded once, then stepped
+        int64_t cell = chunk * kBatchChunk + 8 * kh;
+        int i = (int)(cell % nx);
+        int64_t rw = cell / nx;
+        int j = (int)(rw % ny);
+        int k = (int)(rw / ny);
+        bf16x8 ahi, alo;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            float gv = 0.f;
+            if (cell + e < n_cells) {
+                const float dy = p.y - (float)j * sy;
+                const float dz = p.z - (float)k * sz;
+                const float a = p.x - (float)i;
+                // no near-field pass follows, so the exact row term (floored
+                // only against 0/0 at a coincident station), not row_R32
+                const float v = rsq_f(fmaf(a, a, fmaf(dy, dy, fmaf(dz, dz, 1e-20f))));
+                gv = dz * (v * v * v);
+            }
+            const __bf16 h = (__bf16)gv;
+            ahi[e] = h;
+            alo[e] = (__bf16)(gv - (float)h);
+            if (++i == nx) {
+                i = 0;
+                if (++j == ny) {
+                    j = 0;
+                    ++k;
+                }
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+            const bf16x8 bhi = lhi[(32 * t + row) * 2 + kh];
+            const bf16x8 blo = llo[(32 * t + row) * 2 + kh];
+            acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi, bhi, acc[t], 0, 0, 0);
+            acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi, blo, acc[t], 0, 0, 0);
+            acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(alo, bhi, acc[t], 0, 0, 0);
+        }
+    }
+    // D layout: lane holds model 32t + lane%32, stations 8*(i/4) + 4*(lane/32) + i%4
+    float* out = part + ((size_t)blockIdx.y * m_pad + mbase) * Bp;
+#pragma unroll
+    for (int t = 0; t < NT; ++t)
+#pragma unroll
+        for (int e = 0; e < 16; ++e) {
+            const int srow = 8 * (e >> 2) + 4 * kh + (e & 3);
+            out[(size_t)srow * Bp + 32 * t + row] = acc[t][e];
+        }
+}
+
+// Y[b][m] = scale * sum_ks part[ks][m][b]; 32x32 tiles transposed through LDS.
+__global__ __launch_bounds__(256) void batched_reduce_kernel(const float* __restrict__ part,
+                                                             int ks, int m, int m_pad, int B,
+                                                             int Bp, double scale,
+                                                             double* __restrict__ Y) {
+    __shared__ double tile[32][33];
+    const int mb = blockIdx.x * 32, bb = blockIdx.y * 32;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;   // 32 x 8
+    for (int r = ty; r < 32; r += 8) {
+        const int mm = mb + r, b = bb + tx;
+        double s = 0.0;
+        if (mm < m && b < B)
+            for (int q = 0; q < ks; ++q) s += (double)part[((size_t)q * m_pad + mm) * Bp + b];
+        tile[r][tx] = s;
+    }
+    __syncthreads();
+    for (int r = ty; r < 32; r += 8) {
+        const int b = bb + r, mm = mb + tx;
+        if (b < B && mm < m) Y[(size_t)b * m + mm] = scale * tile[tx][r];
+    }
+}
+
+// --------------------------------------------------------------- launchers
+namespace {
+// model tiles of 32, padded to a power of two (1, 2, 4 or 8 for B <= 256)
+int batched_nt(int B) {
+    const int t = (B + 31) / 32;
+    int p = 1;
+    while (p < t) p <<= 1;
+    return p;
+}
+}  // namespace
+
+size_t batched_workspace(const PmShape& s, int B, int ks) {
+    const int Bp = 32 * batched_nt(B);
+    const int64_t n16 = (s.n_cells + kBatchChunk - 1) / kBatchChunk * kBatchChunk;
+    const size_t packed = (size_t)n16 * Bp * 2 * sizeof(__bf16);      // hi + lo
+    const size_t partial = (size_t)ks * s.m_pad * Bp * sizeof(float);
+    return packed + partial;
+}
+
+int batched_default_slices(const PmShape& s, int B) {
+    (void)B;
+    const int tiles = (s.m + kBatchTile - 1) / kBatchTile;
+    const int64_t n_chunks = (s.n_cells + kBatchChunk - 1) / kBatchChunk;
+    // about two resident workgroups per CU, at least ~16 chunks per slice
+    int ks = std::max(1, (512 + tiles - 1) / tiles);
+    ks = (int)std::min<int64_t>(ks, std::max<int64_t>(1, n_chunks / 16));
+    return ks;
+}
+
+int batched_fwd_launch(const PmShape& s, int B, int ks, const float* st, const float* P,
+                       double thr, double sy, double sz, double scale, void* ws, double* Y,
+                       hipStream_t stream) {
+    if (B <= 0 || s.m == 0) return ZTM_OK;
+    const int nt = batched_nt(B);
+    if (nt > 8) return fail(ZTM_EINVAL, "batched forward supports at most 256 models per call (got %d)", B);
+    const int Bp = 32 * nt;
+    const int64_t n_chunks = (s.n_cells + kBatchChunk - 1) / kBatchChunk;
+    ks = std::max(1, std::min<int>(ks, (int)std::min<int64_t>(n_chunks, 65535)));
+    const int per = (int)((n_chunks + ks - 1) / ks);
+    ks = (int)((n_chunks + per - 1) / per);
+    const int64_t n_packed = n_chunks * Bp * kBatchChunk;
+    __bf16* hi = (__bf16*)ws;
+    __bf16* lo = hi + n_packed;
+    float* part = (float*)(lo + n_packed);
+    hipLaunchKernelGGL(batched_pack_kernel, dim3((unsigned)((n_packed + 255) / 256)), dim3(256), 0,
+                       stream, P, B, s.n_cells, Bp, n_packed, (float)thr, hi, lo);
+    ZTM_HIP(hipGetLastError());
+    const dim3 grid((unsigned)((s.m + kBatchTile - 1) / kBatchTile), (unsigned)ks);
+    {
+        ProfScope prof("batched_fwd", stream);
+#define ZTM_BATCHED_CASE(NT)                                                                   \
+    case NT:                                                                                   \
+        hipLaunchKernelGGL(batched_fwd_kernel<NT>, grid, dim3(256), 0, stream,                 \
+                           (const float4*)st, (const bf16x8*)hi, (const bf16x8*)lo, s.nx,      \
+                           s.ny, s.n_cells, (float)sy, (float)sz, n_chunks, per, s.m_pad,      \
+                           part);                                                              \
+        break;
+        switch (nt) {
+            ZTM_BATCHED_CASE(1)
+            ZTM_BATCHED_CASE(2)
+            ZTM_BATCHED_CASE(4)
+            ZTM_BATCHED_CASE(8)
+            default:
+                return fail(ZTM_EINVAL, "batched model count %d out of range", B);
+        }
+#undef ZTM_BATCHED_CASE
+        ZTM_HIP(hipGetLastError());
+    }
+    hipLaunchKernelGGL(batched_reduce_kernel, dim3((s.m + 31) / 32, (B + 31) / 32), dim3(256), 0,
+                       stream, part, ks, s.m, s.m_pad, B, Bp, scale, Y);
+    ZTM_HIP(hipGetLastError());
+    return ZTM_OK;
+}
+
+}  // namespace ztm

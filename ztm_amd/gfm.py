@@ -67,6 +67,8 @@ SIGNATURES = {
     "ztm_plan_forward": (_I, [_P, _I, _P, _P, _P]),
     "ztm_plan_adjoint": (_I, [_P, _I, _P, _P, _P]),
     "ztm_plan_smoothness": (_I, [_P, _I, _P, _P, _P]),
+    "ztm_plan_forward_batched": (_I, [_P, _I, _P, _P, _P]),
+    "ztm_plan_set_batched_slices": (_I, [_P, _I]),
     "ztm_plan_store_bytes": (_I64, [_P]),
     "ztm_plan_store": (_I, [_P, _P]),
     "ztm_plan_store_release": (_I, [_P]),
@@ -212,6 +214,23 @@ class Plan:
         _check_t(out, torch.float64, self.n_stations, "g")
         self.forward_ptr(prec, _ptr(rho), _ptr(out), _stream(stream))
         return out
+
+    # batched (ensemble) forward
+    def forward_batched(self, rho, out=None, stream=None):
+        """rho float32 [B, N] (B <= 256) -> g float64 [B, M] on split-BF16 MFMA."""
+        import torch
+        if not (isinstance(rho, torch.Tensor) and rho.is_cuda and rho.dtype == torch.float32
+                and rho.is_contiguous() and rho.dim() == 2 and rho.shape[1] == self.n_cells):
+            raise ValueError(f"rho must be a contiguous float32 CUDA tensor [B, {self.n_cells}]")
+        B = rho.shape[0]
+        if out is None:
+            out = torch.empty(B, self.n_stations, dtype=torch.float64, device=rho.device)
+        _check_t(out, torch.float64, B * self.n_stations, "g")
+        check(lib().ztm_plan_forward_batched(self._h, B, _ptr(rho), _ptr(out), _stream(stream)))
+        return out
+
+    def set_batched_slices(self, slices: int) -> None:
+        check(lib().ztm_plan_set_batched_slices(self._h, slices))
 
     # stored-matrix variant
     def store_bytes(self) -> int:
