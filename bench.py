@@ -222,6 +222,7 @@ def main() -> int:
     del g64, a64, rho64, r64
     if not args.no_extras:
         line["prism"] = prism_leg(w, r, stream, args)
+        line["batched"] = batched_leg(stream, args)
     print(json.dumps(line))
     return 0
 
@@ -272,6 +273,61 @@ def stored_leg(plan, rho32, r32, g64h, stream, args):
     if g64h is not None:
         out["rel_err_vs_fp64"] = relerr(g.cpu().numpy(), g64h)
     plan.store_release()
+    return out
+
+
+PEAK_BF16_TFLOPS = 2500.0     # MI355X dense BF16 MFMA, published
+
+
+def batched_leg(stream, args):
+    """Ensemble forward on split-BF16 MFMA (ensemble_256: 96x96x48 cells of 1 km,
+    9,216 stations at z = +10 m, 256 smoothed models), both splits, against the
+    FP64 single-model kernel on models 0, 85, 170, 255 (extrapolated to 256
+    models for the FP64 rate)."""
+    import torch
+    from ztm_amd import gfm, workloads as W
+    dev = torch.device("cuda:0")
+    w = W.ensemble_256()
+    B = w.rho.shape[0]
+    P = torch.from_numpy(w.rho).to(dev)
+    pm = w.n_stations * w.grid.n_cells * B                  # pair-models per call
+    out = {"workload": "ensemble_256: 96x96x48 cells (442,368) x 9,216 stations x 256 models",
+           "unit": "pair-models/s", "pair_models_per_call": pm}
+    with gfm.Plan(w.grid, w.stations, kappa=w.kappa) as p:
+        Y = torch.empty(B, w.n_stations, dtype=torch.float64, device=dev)
+        check = (0, 85, 170, 255)
+        g64 = {}
+        t64 = 0.0
+        for b in check:
+            x = torch.from_numpy(w.rho[b].astype(np.float64)).to(dev)
+            p.forward(x)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            g64[b] = p.forward(x).cpu().numpy()
+            t64 += time.perf_counter() - t0
+        out["fp64_gpu"] = {"value": pm / (t64 / len(check) * B), "unit": "pair-models/s",
+                           "kind": "FP64 single-model kernel, timed on 4 models"}
+        for split in (3, 2):
+            p.set_batched_tuning(split=split)
+            gfm.profile_reset()
+            gfm.profile_enable(True)
+            t = _timed(lambda: p.forward_batched(P, out=Y, stream=stream), 3,
+                       torch.cuda.synchronize)
+            gfm.profile_enable(False)
+            ms, n = gfm.profile_read("batched_fwd")
+            gfm.profile_reset()
+            kms = ms / max(n, 1)
+            mfma_flop = 2.0 * pm * (6 if split == 3 else 3)
+            Yh = Y.cpu().numpy()
+            leg = {"value": pm / t, "ms_per_call": t * 1e3, "kernel_ms": kms,
+                   "rel_err_vs_fp64": max(relerr(Yh[b], g64[b]) for b in check),
+                   "roofline": {"bound": "mfma", "unit": "TFLOP/s", "peak": PEAK_BF16_TFLOPS,
+                                "achieved": mfma_flop / (kms * 1e-3) / 1e12,
+                                "frac": mfma_flop / (kms * 1e-3) / 1e12 / PEAK_BF16_TFLOPS,
+                                "flop": "issued MFMA: 2 x pair-models x products",
+                                "products": 6 if split == 3 else 3}}
+            leg["speedup_vs_fp64_gpu"] = leg["value"] / out["fp64_gpu"]["value"]
+            out[f"split{split}"] = leg
     return out
 
 

@@ -165,15 +165,17 @@ int ztm_plan_adjoint_stored(ztm_plan plan, const float* r_dev, float* out_dev,
  * difference Jacobian's columns, TFM.cpp:124-135, or an ensemble).
  * rho_dev float[n_models][N] (each model contiguous) -> g_dev double
  * [n_models][M]. G is generated on the fly in FP32, G and rho are split into
- * hi + lo BF16 and hi*hi + hi*lo + lo*hi run on v_mfma_f32_32x32x16_bf16
- * with FP32 accumulation per K-slice and FP64 across slices. n_models <= 256
- * per call. The plan's threshold is applied to rho. There is no FP64
+ * BF16 parts -- 3-way (default, ~2^-24, 6 MFMA products) or 2-way hi/lo
+ * (~2^-16: hi*hi + hi*lo + lo*hi) -- run on v_mfma_f32_32x32x16_bf16 with
+ * FP32 accumulation per K-slice and FP64 across slices. n_models <= 256 per
+ * call. The plan's threshold is applied to rho. There is no FP64
  * near-field pass: results assume no station lies within about half a cell
  * of a cell centre (true for stations above the grid's top face). */
 int ztm_plan_forward_batched(ztm_plan plan, int n_models, const float* rho_dev,
                              double* g_dev, void* stream);
-/* K-slices of the batched forward (0 = choose from the problem shape). */
-int ztm_plan_set_batched_slices(ztm_plan plan, int slices);
+/* Batched forward tuning: K-slices (0 = choose from the problem shape) and
+ * BF16 split (2 or 3; 0 = keep). */
+int ztm_plan_set_batched_tuning(ztm_plan plan, int slices, int split);
 
 /* Smoothness operator of the regularised misfit 0.5||G rho-d||^2 +
  * 0.5 lambda ||grad rho||^2 (forward differences on each axis, none across

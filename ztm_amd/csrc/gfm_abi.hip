@@ -588,7 +588,8 @@ int ztm_plan_forward_batched(ztm_plan plan, int n_models, const float* rho_dev, 
     const int ks = plan->tuning.batched_slices > 0
                        ? plan->tuning.batched_slices
                        : batched_default_slices(plan->shape, n_models);
-    const size_t need = batched_workspace(plan->shape, n_models, ks);
+    const int split = plan->tuning.batched_split;
+    const size_t need = batched_workspace(plan->shape, n_models, ks, split);
     if (need > plan->bws_cap) {
         if (plan->bws) (void)hipFree(plan->bws);
         plan->bws = nullptr;
@@ -596,15 +597,17 @@ int ztm_plan_forward_batched(ztm_plan plan, int n_models, const float* rho_dev, 
         ZTM_HIP(hipMalloc(&plan->bws, need));
         plan->bws_cap = need;
     }
-    return batched_fwd_launch(plan->shape, n_models, ks, (const float*)plan->st_f, rho_dev,
+    return batched_fwd_launch(plan->shape, n_models, ks, split, (const float*)plan->st_f, rho_dev,
                               plan->prob.threshold, plan->sy, plan->sz, pm_scale(plan), plan->bws,
                               g_dev, (hipStream_t)stream);
 }
 
-int ztm_plan_set_batched_slices(ztm_plan plan, int slices) {
+int ztm_plan_set_batched_tuning(ztm_plan plan, int slices, int split) {
     if (!plan) return fail(ZTM_EINVAL, "null plan");
     if (slices < 0) return fail(ZTM_EINVAL, "negative slice count");
+    if (split != 0 && split != 2 && split != 3) return fail(ZTM_EINVAL, "split must be 2 or 3");
     plan->tuning.batched_slices = slices;
+    if (split) plan->tuning.batched_split = split;
     return ZTM_OK;
 }
 

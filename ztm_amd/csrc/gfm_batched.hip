@@ -1,12 +1,15 @@
 // Batched (ensemble) forward on MFMA: Y = G P for P = [B models x N cells],
 // with G (stations x cells) generated on the fly -- never stored.
 //
-// Split-BF16 scheme: G and P are each split into hi + lo BF16 parts
-// (hi = bf16(x), lo = bf16(x - hi), ~2^-16 relative together); the product
-// uses the three significant terms  hi*hi + hi*lo + lo*hi  on
-// v_mfma_f32_32x32x16_bf16 with FP32 accumulation (lo*lo, ~2^-32, dropped).
+// Split-BF16 scheme: G and P are each split into BF16 parts, x = x0 + x1 (+ x2)
+// with x0 = bf16(x), x1 = bf16(x - x0), x2 = bf16(x - x0 - x1), and the product
+// keeps the terms down to the split's precision on v_mfma_f32_32x32x16_bf16
+// with FP32 accumulation:
+//   2-way (hi/lo, ~2^-16):  x0 y0 + x0 y1 + x1 y0                 (3 MFMAs)
+//   3-way (~2^-24):         + x0 y2 + x1 y1 + x2 y0               (6 MFMAs)
 // Each K-slice's FP32 accumulators are written out and the slices are summed
-// in FP64, which bounds the FP32 accumulation run to one slice.
+// in FP64, which bounds the FP32 accumulation run to one slice (a few hundred
+// MFMA steps by default).
 //
 // Work decomposition: a workgroup = 4 waves = 128 stations (32 per wave, the
 // MFMA's M); each wave keeps all B models' accumulators (B/32 tiles of 32x32,
@@ -19,8 +22,8 @@
 // round-robin dispatch spreads one slice's tiles over the XCDs in sequence
 // and concurrently resident tiles of a slice share its P chunks in L2.
 //
-// Layouts (HBM):  Phi/Plo [N16/16][Bp][16] bf16, zero padded (N16 = N up to a
-//                 multiple of 16, Bp = B up to a multiple of 32);
+// Layouts (HBM):  P parts [N16/16][Bp][16] bf16 each, zero padded (N16 = N up
+//                 to a multiple of 16, Bp = B up to 32 x a power of two);
 //                 part [KS][m_pad][Bp] FP32;  Y [B][M] FP64 (model-major).
 #include "gfm_internal.h"
 #include "gfm_pair.h"
@@ -36,10 +39,12 @@ constexpr int kBatchChunk = 16;   // cells per MFMA K step
 constexpr int kBatchTile = 128;   // stations per workgroup
 
 // ---------------------------------------------------------------- packing
-// One thread per packed element: t = ((chunk * Bp) + model) * 16 + c.
+// One thread per packed element: t = ((chunk * Bp) + model) * 16 + c; writes
+// SPLIT parts at stride n_packed.
+template <int SPLIT>
 __global__ void batched_pack_kernel(const float* __restrict__ P, int B, int64_t n_cells,
                                     int Bp, int64_t n_packed, float thr,
-                                    __bf16* __restrict__ hi, __bf16* __restrict__ lo) {
+                                    __bf16* __restrict__ parts) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n_packed) return;
     const int c = (int)(t & 15);
@@ -51,20 +56,22 @@ __global__ void batched_pack_kernel(const float* __restrict__ P, int B, int64_t 
         v = P[(size_t)model * n_cells + cell];
         if (!(fabsf(v) > thr)) v = 0.f;
     }
-    const __bf16 h = (__bf16)v;
-    hi[t] = h;
-    lo[t] = (__bf16)(v - (float)h);
+#pragma unroll
+    for (int q = 0; q < SPLIT; ++q) {
+        const __bf16 h = (__bf16)v;
+        parts[q * n_packed + t] = h;
+        v -= (float)h;
+    }
 }
 
 // --------------------------------------------------------------- forward
-template <int NT>   // model tiles of 32 (B <= 32 * NT)
+template <int NT, int SPLIT>   // model tiles of 32 (B <= 32 * NT); 2- or 3-way split
 __global__ __launch_bounds__(256) void batched_fwd_kernel(
-    const float4* __restrict__ st, const bf16x8* __restrict__ phi,
-    const bf16x8* __restrict__ plo, int nx, int ny, int64_t n_cells, float sy, float sz,
-    int64_t n_chunks, int chunks_per_slice, int m_pad, float* __restrict__ part) {
+    const float4* __restrict__ st, const bf16x8* __restrict__ parts, int64_t part_stride,
+    int nx, int ny, int64_t n_cells, float sy, float sz, int64_t n_chunks,
+    int chunks_per_slice, int m_pad, float* __restrict__ part) {
     constexpr int Bp = 32 * NT;
-    __shared__ bf16x8 lhi[Bp * 2];
-    __shared__ bf16x8 llo[Bp * 2];
+    __shared__ bf16x8 lp[SPLIT][Bp * 2];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int row = lane & 31, kh = lane >> 5;
     const int mbase = blockIdx.x * kBatchTile + wave * 32;
@@ -80,32 +87,34 @@ __global__ __launch_bounds__(256) void batched_fwd_kernel(
     const int64_t c1 = min(c0 + (int64_t)chunks_per_slice, n_chunks);
     for (int64_t chunk = c0; chunk < c1; ++chunk) {
         __syncthreads();
-        for (int v = threadIdx.x; v < Bp * 2; v += blockDim.x) {
-            lhi[v] = phi[chunk * (Bp * 2) + v];
-            llo[v] = plo[chunk * (Bp * 2) + v];
-        }
+        for (int v = threadIdx.x; v < Bp * 2; v += blockDim.x)
+#pragma unroll
+            for (int q = 0; q < SPLIT; ++q) lp[q][v] = parts[q * part_stride + chunk * (Bp * 2) + v];
         // this lane's 8 cells: chunk*16 + 8*kh + 0..7, decoded once, then stepped
         int64_t cell = chunk * kBatchChunk + 8 * kh;
         int i = (int)(cell % nx);
         int64_t rw = cell / nx;
         int j = (int)(rw % ny);
         int k = (int)(rw / ny);
-        bf16x8 ahi, alo;
+        bf16x8 a[SPLIT];
 #pragma unroll
         for (int e = 0; e < 8; ++e) {
             float gv = 0.f;
             if (cell + e < n_cells) {
                 const float dy = p.y - (float)j * sy;
                 const float dz = p.z - (float)k * sz;
-                const float a = p.x - (float)i;
+                const float x = p.x - (float)i;
                 // no near-field pass follows, so the exact row term (floored
                 // only against 0/0 at a coincident station), not row_R32
-                const float v = rsq_f(fmaf(a, a, fmaf(dy, dy, fmaf(dz, dz, 1e-20f))));
+                const float v = rsq_f(fmaf(x, x, fmaf(dy, dy, fmaf(dz, dz, 1e-20f))));
                 gv = dz * (v * v * v);
             }
-            const __bf16 h = (__bf16)gv;
-            ahi[e] = h;
-            alo[e] = (__bf16)(gv - (float)h);
+#pragma unroll
+            for (int q = 0; q < SPLIT; ++q) {
+                const __bf16 h = (__bf16)gv;
+                a[q][e] = h;
+                gv -= (float)h;
+            }
             if (++i == nx) {
                 i = 0;
                 if (++j == ny) {
@@ -117,11 +126,17 @@ __global__ __launch_bounds__(256) void batched_fwd_kernel(
         __syncthreads();
 #pragma unroll
         for (int t = 0; t < NT; ++t) {
-            const bf16x8 bhi = lhi[(32 * t + row) * 2 + kh];
-            const bf16x8 blo = llo[(32 * t + row) * 2 + kh];
-            acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi, bhi, acc[t], 0, 0, 0);
-            acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi, blo, acc[t], 0, 0, 0);
-            acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(alo, bhi, acc[t], 0, 0, 0);
+            bf16x8 b[SPLIT];
+#pragma unroll
+            for (int q = 0; q < SPLIT; ++q) b[q] = lp[q][(32 * t + row) * 2 + kh];
+            acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[0], b[0], acc[t], 0, 0, 0);
+            acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[0], b[1], acc[t], 0, 0, 0);
+            acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[1], b[0], acc[t], 0, 0, 0);
+            if constexpr (SPLIT == 3) {
+                acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[0], b[2], acc[t], 0, 0, 0);
+                acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[1], b[1], acc[t], 0, 0, 0);
+                acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[2], b[0], acc[t], 0, 0, 0);
+            }
         }
     }
     // D layout: lane holds model 32t + lane%32, stations 8*(i/4) + 4*(lane/32) + i%4
@@ -168,10 +183,10 @@ int batched_nt(int B) {
 }
 }  // namespace
 
-size_t batched_workspace(const PmShape& s, int B, int ks) {
+size_t batched_workspace(const PmShape& s, int B, int ks, int split) {
     const int Bp = 32 * batched_nt(B);
     const int64_t n16 = (s.n_cells + kBatchChunk - 1) / kBatchChunk * kBatchChunk;
-    const size_t packed = (size_t)n16 * Bp * 2 * sizeof(__bf16);      // hi + lo
+    const size_t packed = (size_t)n16 * Bp * split * sizeof(__bf16);
     const size_t partial = (size_t)ks * s.m_pad * Bp * sizeof(float);
     return packed + partial;
 }
@@ -180,16 +195,19 @@ int batched_default_slices(const PmShape& s, int B) {
     (void)B;
     const int tiles = (s.m + kBatchTile - 1) / kBatchTile;
     const int64_t n_chunks = (s.n_cells + kBatchChunk - 1) / kBatchChunk;
-    // about two resident workgroups per CU, at least ~16 chunks per slice
+    // at least ~2 resident workgroups per CU, and at most ~384 chunks (MFMA
+    // accumulation steps) per slice to bound the FP32 accumulation error
     int ks = std::max(1, (512 + tiles - 1) / tiles);
-    ks = (int)std::min<int64_t>(ks, std::max<int64_t>(1, n_chunks / 16));
-    return ks;
+    ks = std::max<int64_t>(ks, (n_chunks + 383) / 384);
+    ks = (int)std::min<int64_t>(ks, std::max<int64_t>(1, n_chunks));
+    return std::min(ks, 65535);
 }
 
-int batched_fwd_launch(const PmShape& s, int B, int ks, const float* st, const float* P,
-                       double thr, double sy, double sz, double scale, void* ws, double* Y,
-                       hipStream_t stream) {
+int batched_fwd_launch(const PmShape& s, int B, int ks, int split, const float* st,
+                       const float* P, double thr, double sy, double sz, double scale, void* ws,
+                       double* Y, hipStream_t stream) {
     if (B <= 0 || s.m == 0) return ZTM_OK;
+    if (split != 2 && split != 3) return fail(ZTM_EINVAL, "split must be 2 or 3 (got %d)", split);
     const int nt = batched_nt(B);
     if (nt > 8) return fail(ZTM_EINVAL, "batched forward supports at most 256 models per call (got %d)", B);
     const int Bp = 32 * nt;
@@ -198,29 +216,37 @@ int batched_fwd_launch(const PmShape& s, int B, int ks, const float* st, const f
     const int per = (int)((n_chunks + ks - 1) / ks);
     ks = (int)((n_chunks + per - 1) / per);
     const int64_t n_packed = n_chunks * Bp * kBatchChunk;
-    __bf16* hi = (__bf16*)ws;
-    __bf16* lo = hi + n_packed;
-    float* part = (float*)(lo + n_packed);
-    hipLaunchKernelGGL(batched_pack_kernel, dim3((unsigned)((n_packed + 255) / 256)), dim3(256), 0,
-                       stream, P, B, s.n_cells, Bp, n_packed, (float)thr, hi, lo);
+    __bf16* parts = (__bf16*)ws;
+    float* part = (float*)(parts + split * n_packed);
+    if (split == 3)
+        hipLaunchKernelGGL(batched_pack_kernel<3>, dim3((unsigned)((n_packed + 255) / 256)),
+                           dim3(256), 0, stream, P, B, s.n_cells, Bp, n_packed, (float)thr, parts);
+    else
+        hipLaunchKernelGGL(batched_pack_kernel<2>, dim3((unsigned)((n_packed + 255) / 256)),
+                           dim3(256), 0, stream, P, B, s.n_cells, Bp, n_packed, (float)thr, parts);
     ZTM_HIP(hipGetLastError());
     const dim3 grid((unsigned)((s.m + kBatchTile - 1) / kBatchTile), (unsigned)ks);
+    const int64_t stride = n_packed / 8;   // in bf16x8 units
     {
         ProfScope prof("batched_fwd", stream);
-#define ZTM_BATCHED_CASE(NT)                                                                   \
-    case NT:                                                                                   \
-        hipLaunchKernelGGL(batched_fwd_kernel<NT>, grid, dim3(256), 0, stream,                 \
-                           (const float4*)st, (const bf16x8*)hi, (const bf16x8*)lo, s.nx,      \
-                           s.ny, s.n_cells, (float)sy, (float)sz, n_chunks, per, s.m_pad,      \
-                           part);                                                              \
-        break;
-        switch (nt) {
-            ZTM_BATCHED_CASE(1)
-            ZTM_BATCHED_CASE(2)
-            ZTM_BATCHED_CASE(4)
-            ZTM_BATCHED_CASE(8)
-            default:
-                return fail(ZTM_EINVAL, "batched model count %d out of range", B);
+#define ZTM_BATCHED_CASE(NT, SP)                                                               \
+    hipLaunchKernelGGL((batched_fwd_kernel<NT, SP>), grid, dim3(256), 0, stream,               \
+                       (const float4*)st, (const bf16x8*)parts, stride, s.nx, s.ny, s.n_cells, \
+                       (float)sy, (float)sz, n_chunks, per, s.m_pad, part)
+        if (split == 3) {
+            switch (nt) {
+                case 1: ZTM_BATCHED_CASE(1, 3); break;
+                case 2: ZTM_BATCHED_CASE(2, 3); break;
+                case 4: ZTM_BATCHED_CASE(4, 3); break;
+                default: ZTM_BATCHED_CASE(8, 3); break;
+            }
+        } else {
+            switch (nt) {
+                case 1: ZTM_BATCHED_CASE(1, 2); break;
+                case 2: ZTM_BATCHED_CASE(2, 2); break;
+                case 4: ZTM_BATCHED_CASE(4, 2); break;
+                default: ZTM_BATCHED_CASE(8, 2); break;
+            }
         }
 #undef ZTM_BATCHED_CASE
         ZTM_HIP(hipGetLastError());

@@ -95,6 +95,7 @@ struct PmTuning {
     int cells_per_lane = 16;  // adjoint: cells per lane (4,8,16)
     int adj_splits = 0;       // adjoint: 0 = choose from the problem shape
     int batched_slices = 0;   // batched forward K-slices: 0 = choose
+    int batched_split = 3;    // batched forward BF16 split: 2 (hi/lo) or 3
 };
 
 struct PmShape {
@@ -159,11 +160,11 @@ int stored_adj_launch(const PmShape& s, const float* G, const float* r, double* 
 
 // Batched split-BF16 MFMA forward (gfm_batched.hip): P float [B][N] -> Y
 // double [B][M]; ws must hold batched_workspace(s, B, ks) bytes.
-size_t batched_workspace(const PmShape& s, int B, int ks);
+size_t batched_workspace(const PmShape& s, int B, int ks, int split);
 int batched_default_slices(const PmShape& s, int B);
-int batched_fwd_launch(const PmShape& s, int B, int ks, const float* st, const float* P,
-                       double thr, double sy, double sz, double scale, void* ws, double* Y,
-                       hipStream_t stream);
+int batched_fwd_launch(const PmShape& s, int B, int ks, int split, const float* st,
+                       const float* P, double thr, double sy, double sz, double scale, void* ws,
+                       double* Y, hipStream_t stream);
 
 // out = L^T L rho for forward-difference gradients on an nx x ny x nz grid
 // (gfm_reg.hip).

@@ -68,7 +68,7 @@ SIGNATURES = {
     "ztm_plan_adjoint": (_I, [_P, _I, _P, _P, _P]),
     "ztm_plan_smoothness": (_I, [_P, _I, _P, _P, _P]),
     "ztm_plan_forward_batched": (_I, [_P, _I, _P, _P, _P]),
-    "ztm_plan_set_batched_slices": (_I, [_P, _I]),
+    "ztm_plan_set_batched_tuning": (_I, [_P, _I, _I]),
     "ztm_plan_store_bytes": (_I64, [_P]),
     "ztm_plan_store": (_I, [_P, _P]),
     "ztm_plan_store_release": (_I, [_P]),
@@ -229,8 +229,9 @@ class Plan:
         check(lib().ztm_plan_forward_batched(self._h, B, _ptr(rho), _ptr(out), _stream(stream)))
         return out
 
-    def set_batched_slices(self, slices: int) -> None:
-        check(lib().ztm_plan_set_batched_slices(self._h, slices))
+    def set_batched_tuning(self, slices: int = 0, split: int = 0) -> None:
+        """K-slices (0 = choose) and BF16 split (2 = hi/lo, 3 = three-way; 0 = keep)."""
+        check(lib().ztm_plan_set_batched_tuning(self._h, slices, split))
 
     # stored-matrix variant
     def store_bytes(self) -> int:
