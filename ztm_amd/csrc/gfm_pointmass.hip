@@ -59,17 +59,20 @@ __device__ __forceinline__ T kterm(T a, T R, T qrow, const QuadArgs& q) {
 // grid = (m_pad / (64*S), n_slices); block = 256.
 // st:   prepared stations [m_pad] (x, y, z in cell units; see gfm_internal.h)
 // part: [n_slices][m_pad] FP64 partial sums  sum_n rho_n dz/r^3
+// pm_fwd_block: one workgroup's work -- station tile bx, row slice by -- so
+// that the single-problem kernel and the grouped (multi-problem) kernel share
+// it; smem must hold max(rows_per_slice * nxp * sizeof(T), 4*S*64*8) bytes.
 template <typename T, int S, bool QUAD>
-__global__ __launch_bounds__(256) void pm_fwd_kernel(
+__device__ __forceinline__ void pm_fwd_block(
     const typename Vec4<T>::type* __restrict__ st, const T* __restrict__ rho,
     int nx, int ny, int n_rows, T sy, T sz, T thr, int rows_per_slice,
-    double* __restrict__ part, int m_pad, QuadArgs qa) {
+    double* __restrict__ part, int m_pad, const QuadArgs& qa, int bx, int by,
+    unsigned char* smem) {
     using V4 = typename Vec4<T>::type;
-    extern __shared__ __align__(16) unsigned char smem[];
     T* lrho = reinterpret_cast<T*>(smem);
 
     const int nxp = (nx + 3) & ~3;
-    const int row0 = blockIdx.y * rows_per_slice;
+    const int row0 = by * rows_per_slice;
     const int nr = min(rows_per_slice, n_rows - row0);
     const size_t base = (size_t)row0 * (size_t)nx;
 
@@ -88,7 +91,7 @@ __global__ __launch_bounds__(256) void pm_fwd_kernel(
 
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
-    const int m0 = blockIdx.x * (64 * S);
+    const int m0 = bx * (64 * S);
 
     T X[S], Y[S], Z[S];
     double acc[S];
@@ -150,23 +153,31 @@ __global__ __launch_bounds__(256) void pm_fwd_kernel(
     for (int t = threadIdx.x; t < 64 * S; t += blockDim.x) {
         const double v = red[t] + red[S * 64 + t] + red[2 * S * 64 + t] +
                          red[3 * S * 64 + t];
-        part[(size_t)blockIdx.y * m_pad + m0 + t] = v;
+        part[(size_t)by * m_pad + m0 + t] = v;
     }
+}
+
+// grid = (m_pad / (64*S), n_slices); block = 256.
+template <typename T, int S, bool QUAD>
+__global__ __launch_bounds__(256) void pm_fwd_kernel(
+    const typename Vec4<T>::type* __restrict__ st, const T* __restrict__ rho,
+    int nx, int ny, int n_rows, T sy, T sz, T thr, int rows_per_slice,
+    double* __restrict__ part, int m_pad, QuadArgs qa) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    pm_fwd_block<T, S, QUAD>(st, rho, nx, ny, n_rows, sy, sz, thr, rows_per_slice, part, m_pad,
+                             qa, blockIdx.x, blockIdx.y, smem);
 }
 
 // g[m] = scale * (sum_slices part[slice][m] + correction)
 //   (general API, FP64 out), or gref[map[m]] += (float)(...) (reference ABI).
 // kCorrPmNear: FP64 near field over the (2 kNear + 1)^3 box around the
 // station; kCorrPrism: the per-station correction computed beforehand.
-__global__ __launch_bounds__(256) void pm_fwd_reduce_kernel(const double* __restrict__ part,
-                                     int n_slices, int m_pad, int m,
-                                     double scale, double* __restrict__ g,
-                                     float* __restrict__ gref,
-                                     const int* __restrict__ out_map,
-                                     const float* __restrict__ rho_f,
-                                     NearArgs na) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= m) return;
+__device__ __forceinline__ void pm_fwd_reduce_one(const double* __restrict__ part, int n_slices,
+                                                  int m_pad, int i, double scale,
+                                                  double* __restrict__ g, float* __restrict__ gref,
+                                                  const int* __restrict__ out_map,
+                                                  const float* __restrict__ rho_f,
+                                                  const NearArgs& na) {
     double s = 0.0;
     for (int k = 0; k < n_slices; ++k) s += part[(size_t)k * m_pad + i];
     if (na.mode == kCorrPmNear) {
@@ -197,22 +208,36 @@ __global__ __launch_bounds__(256) void pm_fwd_reduce_kernel(const double* __rest
     }
 }
 
+__global__ __launch_bounds__(256) void pm_fwd_reduce_kernel(const double* __restrict__ part,
+                                     int n_slices, int m_pad, int m,
+                                     double scale, double* __restrict__ g,
+                                     float* __restrict__ gref,
+                                     const int* __restrict__ out_map,
+                                     const float* __restrict__ rho_f,
+                                     NearArgs na) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    pm_fwd_reduce_one(part, n_slices, m_pad, i, scale, g, gref, out_map, rho_f, na);
+}
+
 // ---------------------------------------------------------------- adjoint
 // grid = (ceil(n_segs/64), n_splits); block = 256; a segment is C
 // consecutive cells of one row (the last one of a row may be ragged).
+// pm_adj_block: one workgroup's work -- segment block bx, station split by;
+// smem must hold max(256 * sizeof(V4), 4*C*64*8) bytes.
 template <typename T, int C, bool QUAD>
-__global__ __launch_bounds__(256) void pm_adj_kernel(
+__device__ __forceinline__ void pm_adj_block(
     const typename Vec4<T>::type* __restrict__ st, const T* __restrict__ r,
     int m, int nx, int ny, T sy, T sz, int segs_per_row, int64_t n_segs,
     int st_per_split, int m_pad, double* __restrict__ part,
-    T* __restrict__ out, double scale, int64_t n_cells, int direct, QuadArgs qa) {
+    T* __restrict__ out, double scale, int64_t n_cells, int direct, const QuadArgs& qa,
+    int bx, int by, unsigned char* smem) {
     using V4 = typename Vec4<T>::type;
-    extern __shared__ __align__(16) unsigned char smem[];
     V4* tile = reinterpret_cast<V4*>(smem);
 
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
-    const int64_t seg = (int64_t)blockIdx.x * 64 + lane;
+    const int64_t seg = (int64_t)bx * 64 + lane;
     const int64_t segc = seg < n_segs ? seg : 0;
     const int64_t row = segc / segs_per_row;
     const int i0 = (int)(segc - row * segs_per_row) * C;
@@ -226,7 +251,7 @@ __global__ __launch_bounds__(256) void pm_adj_kernel(
 #pragma unroll
     for (int c = 0; c < C; ++c) accd[c] = 0.0;
 
-    const int mb = blockIdx.y * st_per_split;
+    const int mb = by * st_per_split;
     const int me = min(mb + st_per_split, m_pad);
     for (int t0 = mb; t0 < me; t0 += 256) {
         __syncthreads();
@@ -270,7 +295,7 @@ __global__ __launch_bounds__(256) void pm_adj_kernel(
         const int l = e & 63;
         const double v = red[c * 64 + l] + red[(C + c) * 64 + l] +
                          red[(2 * C + c) * 64 + l] + red[(3 * C + c) * 64 + l];
-        const int64_t sg = (int64_t)blockIdx.x * 64 + l;
+        const int64_t sg = (int64_t)bx * 64 + l;
         if (sg >= n_segs) continue;
         const int64_t rw = sg / segs_per_row;
         const int ii = (int)(sg - rw * segs_per_row) * C + c;
@@ -279,21 +304,32 @@ __global__ __launch_bounds__(256) void pm_adj_kernel(
         if (direct) {
             out[cell] = (T)(scale * v);
         } else {
-            part[(size_t)blockIdx.y * (size_t)n_cells + cell] = v;
+            part[(size_t)by * (size_t)n_cells + cell] = v;
         }
     }
+}
+
+// grid = (ceil(n_segs/64), n_splits); block = 256.
+template <typename T, int C, bool QUAD>
+__global__ __launch_bounds__(256) void pm_adj_kernel(
+    const typename Vec4<T>::type* __restrict__ st, const T* __restrict__ r,
+    int m, int nx, int ny, T sy, T sz, int segs_per_row, int64_t n_segs,
+    int st_per_split, int m_pad, double* __restrict__ part,
+    T* __restrict__ out, double scale, int64_t n_cells, int direct, QuadArgs qa) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    pm_adj_block<T, C, QUAD>(st, r, m, nx, ny, sy, sz, segs_per_row, n_segs, st_per_split, m_pad,
+                             part, out, scale, n_cells, direct, qa, blockIdx.x, blockIdx.y, smem);
 }
 
 // out[n] = scale * (sum_splits part[split][n] + correction); the correction
 // gathers, per cell, the stations whose column lies within na.radius of the
 // cell's column (fixed CSR order, so the result is deterministic).
 template <typename T, bool QUAD>
-__global__ __launch_bounds__(256) void pm_adj_reduce_kernel(const double* __restrict__ part,
-                                     int n_splits, int64_t n_cells,
-                                     double scale, T* __restrict__ out,
-                                     const float* __restrict__ r_f, NearArgs na) {
-    const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= n_cells) return;
+__device__ __forceinline__ void pm_adj_reduce_one(const double* __restrict__ part, int n_splits,
+                                                  int64_t n_cells, int64_t n, double scale,
+                                                  T* __restrict__ out,
+                                                  const float* __restrict__ r_f,
+                                                  const NearArgs& na) {
     double s = 0.0;
     for (int k = 0; k < n_splits; ++k) s += part[(size_t)k * n_cells + n];
     if (na.mode != kCorrNone) {
@@ -347,6 +383,16 @@ __global__ __launch_bounds__(256) void pm_adj_reduce_kernel(const double* __rest
         s += corr;
     }
     out[n] = (T)(scale * s);
+}
+
+template <typename T, bool QUAD>
+__global__ __launch_bounds__(256) void pm_adj_reduce_kernel(const double* __restrict__ part,
+                                     int n_splits, int64_t n_cells,
+                                     double scale, T* __restrict__ out,
+                                     const float* __restrict__ r_f, NearArgs na) {
+    const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= n_cells) return;
+    pm_adj_reduce_one<T, QUAD>(part, n_splits, n_cells, n, scale, out, r_f, na);
 }
 
 // ------------------------------------------------------- station prep
