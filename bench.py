@@ -223,6 +223,7 @@ def main() -> int:
     if not args.no_extras:
         line["prism"] = prism_leg(w, r, stream, args)
         line["batched"] = batched_leg(stream, args)
+        line["uneven_batch"] = uneven_leg(stream, args)
     print(json.dumps(line))
     return 0
 
@@ -328,6 +329,116 @@ def batched_leg(stream, args):
                                 "products": 6 if split == 3 else 3}}
             leg["speedup_vs_fp64_gpu"] = leg["value"] / out["fp64_gpu"]["value"]
             out[f"split{split}"] = leg
+    return out
+
+
+def uneven_leg(stream, args, iters=10, solves=2):
+    """uneven_48: 48 independent problems (cells 2e4-2e6, stations 1e3-4e4,
+    edges 250-2,000 m) solved together: 10 CG iterations of the regularised
+    normal equations per problem (one G^T d pass + 10 x (forward + adjoint)),
+    every pass one persistent launch over all 48, the whole solve one HIP
+    graph, tiles autotuned per size class. Compared with the same passes run
+    problem by problem (FP32 plans and FP64 plans, eager)."""
+    import torch
+    from ztm_amd import gfm, workloads as W
+    dev = torch.device("cuda:0")
+    ps = W.uneven_48()
+    pairs = sum(p.pairs for p in ps)
+    passes = 1 + 2 * iters
+    out = {"workload": "uneven_48: 48 problems, cells log-U[2e4,2e6], stations log-U[1e3,4e4], "
+                       f"edge log-U[250,2000] m; {iters} CG iterations (fwd + adj) each",
+           "unit": "pairs/s", "pairs_per_pass": pairs, "passes_per_solve": passes,
+           "cells": int(sum(p.grid.n_cells for p in ps)),
+           "stations": int(sum(p.stations.shape[0] for p in ps))}
+    lam = np.array([p.lam for p in ps])
+    with gfm.Batch([(p.grid, p.stations, p.kappa) for p in ps]) as b:
+        t0 = time.perf_counter()
+        b.autotune(reps=2, stream=stream)
+        torch.cuda.synchronize()
+        out["autotune_s"] = time.perf_counter() - t0
+        rho_t = b.pack_cells([p.rho for p in ps], device=dev)
+        g = b.forward(rho_t, stream=stream)
+        d = (g + b.pack_stations([p.noise for p in ps], dtype=torch.float64, device=dev)).float()
+        x = torch.zeros(b.total_cells, dtype=torch.float32, device=dev)
+        rr = torch.zeros(iters + 1, b.n_problems, dtype=torch.float64, device=dev)
+        b.solve(d, lam, iters, x=x, rr=rr, use_graph=True, stream=stream)   # capture + run
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(solves):
+            b.solve(d, lam, iters, x=x, rr=rr, use_graph=True, stream=stream)
+        torch.cuda.synchronize()
+        t_graph = (time.perf_counter() - t0) / solves
+        t_eager = _timed(lambda: b.solve(d, lam, iters, x=x, rr=rr, use_graph=False,
+                                         stream=stream), 1, torch.cuda.synchronize)
+        gfm.profile_reset()
+        gfm.profile_enable(True)
+        r_one = torch.ones(b.total_stations, dtype=torch.float32, device=dev)
+        gb = torch.empty(b.total_stations, dtype=torch.float64, device=dev)
+        ab = torch.empty(b.total_cells, dtype=torch.float32, device=dev)
+        t_pass = _timed(lambda: (b.forward(x, out=gb, stream=stream),
+                                 b.adjoint(r_one, out=ab, stream=stream)), 3,
+                        torch.cuda.synchronize)
+        gfm.profile_enable(False)
+        fms, fn = gfm.profile_read("pm_fwd_grouped")
+        ams, an = gfm.profile_read("pm_adj_grouped")
+        gfm.profile_reset()
+        # solution quality
+        gx = b.forward(x, stream=stream)
+        rrh = rr.cpu().numpy()
+        res = b.unpack_stations((gx - d.double()).cpu().numpy())
+        dn = b.unpack_stations(d.cpu().numpy())
+        xs = b.unpack_cells(x.cpu().numpy())
+        gbh = b.unpack_stations(gb.cpu().numpy())
+        xh = xs
+        tunings = [b.tuning(i) for i in range(len(ps))]
+
+    def colsum(v, grid):
+        return v.reshape(grid.nz, grid.ny, grid.nx).sum(axis=0).ravel()
+
+    corr = [float(np.corrcoef(colsum(xs[i], p.grid), colsum(p.rho, p.grid))[0, 1])
+            for i, p in enumerate(ps)]
+    chi2 = [float(res[i] @ res[i] / (len(res[i]) * 0.1 ** 2)) for i in range(len(ps))]
+    # problem by problem, eager: FP32 and FP64 plans, forward + adjoint of the
+    # same vectors, and the FP32 batch result against FP64
+    t32 = t64 = 0.0
+    err = 0.0
+    for i, p in enumerate(ps):
+        with gfm.Plan(p.grid, p.stations, kappa=p.kappa) as pl:
+            x32 = torch.from_numpy(xh[i]).to(dev)
+            x64 = x32.double()
+            r32 = torch.ones(p.stations.shape[0], dtype=torch.float32, device=dev)
+            r64 = r32.double()
+            t32 += _timed(lambda: (pl.forward(x32, stream=stream), pl.adjoint(r32, stream=stream)),
+                          1, torch.cuda.synchronize)
+            t64 += _timed(lambda: (pl.forward(x64, stream=stream), pl.adjoint(r64, stream=stream)),
+                          1, torch.cuda.synchronize)
+            err = max(err, relerr(gbh[i], pl.forward(x64, stream=stream).cpu().numpy()))
+    fwd_k, adj_k = fms / max(fn, 1), ams / max(an, 1)
+    out.update({
+        "value": passes * pairs / t_graph,
+        "ms_per_solve_graph": t_graph * 1e3,
+        "ms_per_solve_eager": t_eager * 1e3,
+        "ms_per_pass_batched": t_pass * 1e3,
+        "ms_per_pass_per_problem_fp32": t32 * 1e3,
+        "ms_per_pass_per_problem_fp64": t64 * 1e3,
+        "speedup_vs_fp64_gpu": t64 / t_pass,
+        "speedup_vs_per_problem_fp32": t32 / t_pass,
+        "rel_err_vs_fp64": err,
+        "kernels_ms": {"pm_fwd_grouped": fwd_k, "pm_adj_grouped": adj_k},
+        "roofline": {"bound": "valu", "unit": "TFLOP/s", "peak": PEAK_FP32_TFLOPS,
+                     "achieved": FLOP_PER_PAIR * pairs / (max(fwd_k, adj_k) * 1e-3) / 1e12,
+                     "frac": FLOP_PER_PAIR * pairs / (max(fwd_k, adj_k) * 1e-3) / 1e12
+                     / PEAK_FP32_TFLOPS,
+                     "kernel": "pm_adj_grouped" if adj_k >= fwd_k else "pm_fwd_grouped"},
+        "solution": {"rr_ratio_max": float(np.max(rrh[-1] / rrh[0])),
+                     "misfit_ratio_median": float(np.median(
+                         [np.linalg.norm(res[i]) / np.linalg.norm(dn[i]) for i in range(len(ps))])),
+                     "chi2_per_station_median": float(np.median(chi2)),
+                     "column_mass_corr_median": float(np.median(corr))},
+        "tiles": {"st_per_lane": sorted({t.st_per_lane for t in tunings}),
+                  "cells_per_lane": sorted({t.cells_per_lane for t in tunings}),
+                  "size_classes": sorted({t.size_class for t in tunings})},
+    })
     return out
 
 

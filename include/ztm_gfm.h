@@ -197,6 +197,47 @@ int ztm_plan_set_tuning(ztm_plan plan, int st_per_lane, int rows_per_slice,
 int ztm_plan_get_tuning(ztm_plan plan, int* st_per_lane, int* rows_per_slice,
                         int* cells_per_lane, int* adj_splits);
 
+/* ------------------------------------------------- uneven batches */
+/* A batch of independent point-mass problems (each its own grid, stations and
+ * kappa; FP32 path with the FP64 near-field corrections). Every forward or
+ * adjoint pass over the whole batch is one persistent launch that takes
+ * (problem, tile) tickets from a device counter, plus one reduce launch.
+ * Vectors are concatenated: problem s's cells start at cell_offsets[s], its
+ * stations at station_offsets[s] (ztm_batch_layout; n_problems + 1 entries,
+ * the last is the total length). Entries between one problem's end and the
+ * next start are padding: never read, never written.
+ * No reference counterpart: ZTM runs one CPU forward model per Jacobian
+ * column (TFM.cpp:124-135). Calls on one batch must not overlap in time. */
+typedef struct ztm_batch_s* ztm_batch;
+
+int ztm_batch_create(ztm_batch* batch, int n_problems, const ztm_problem* problems,
+                     const int64_t* n_stations, const double* const* stations_host);
+int ztm_batch_destroy(ztm_batch batch);
+int ztm_batch_num_problems(ztm_batch batch);
+int ztm_batch_layout(ztm_batch batch, int64_t* cell_offsets, int64_t* station_offsets);
+/* g_dev (FP64) at station_offsets; rho_dev (FP32) at cell_offsets. */
+int ztm_batch_forward(ztm_batch batch, const float* rho_dev, double* g_dev, void* stream);
+/* out_dev (FP32) at cell_offsets; r_dev (FP32) at station_offsets. */
+int ztm_batch_adjoint(ztm_batch batch, const float* r_dev, float* out_dev, void* stream);
+/* Conjugate gradients on (G^T G + lambda_s L^T L) x = G^T d for every problem
+ * at once (L = forward differences inside the grid), x(0) = 0, `iters`
+ * iterations; lambda is host double[n_problems]. rr_dev (optional, device
+ * double[(iters + 1) * n_problems]) receives ||r_k||^2 of the normal-equation
+ * residual per iteration k and problem. use_graph != 0 captures the whole
+ * solve into a HIP graph on the first call and replays it while (d_dev,
+ * x_dev, rr_dev, iters) are unchanged. */
+int ztm_batch_solve(ztm_batch batch, const float* d_dev, const double* lambda, int iters,
+                    float* x_dev, double* rr_dev, int use_graph, void* stream);
+/* Tiles: per problem (problem = -1: all). st_per_lane 2/4/8, cells_per_lane
+ * 8/16, rows_per_slice / adj_splits 0 = automatic. */
+int ztm_batch_set_tuning(ztm_batch batch, int problem, int st_per_lane, int rows_per_slice,
+                         int cells_per_lane, int adj_splits);
+int ztm_batch_get_tuning(ztm_batch batch, int problem, int* st_per_lane, int* rows_per_slice,
+                         int* cells_per_lane, int* adj_splits, int* size_class);
+/* Time the grouped kernels per size class (station count x cell count bands)
+ * over the tile candidates, `reps` passes each, and keep the fastest. */
+int ztm_batch_autotune(ztm_batch batch, int reps, void* stream);
+
 /* ------------------------------------------------------ kernel timing */
 /* When enabled, HIP events are recorded on the launch stream around the
  * dominant kernel of every entry point; ztm_profile_read() waits for them and

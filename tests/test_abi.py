@@ -76,3 +76,23 @@ def test_plan_rejects_bad_problems_without_a_device():
     ok = gfm.CProblem(gfm.CGrid(4, 4, 4, 0, 0, 0, 1, 1, 1), 0, 1.0, 0.0)
     assert L.ztm_plan_create(ctypes.byref(h), ctypes.byref(ok), 5, None) == 1   # null stations
     assert L.ztm_plan_create(ctypes.byref(h), ctypes.byref(ok), -1, None) == 1
+
+
+def test_batch_rejects_bad_problems_without_a_device():
+    L = gfm.lib()
+    h = ctypes.c_void_p()
+    ok = gfm.CProblem(gfm.CGrid(4, 4, 4, 0, 0, 0, 1, 1, 1), 0, 1.0, 0.0)
+    prism = gfm.CProblem(gfm.CGrid(4, 4, 4, 0, 0, 0, 1, 1, 1), 1, 1.0, 0.0)
+    st = (ctypes.c_double * 3)(0.0, 0.0, 1.0)
+    sp = (ctypes.c_void_p * 2)(ctypes.cast(st, ctypes.c_void_p), ctypes.cast(st, ctypes.c_void_p))
+    probs = (gfm.CProblem * 2)(ok, prism)
+    ns = (ctypes.c_int64 * 2)(1, 1)
+    assert L.ztm_batch_create(ctypes.byref(h), 2, probs, ns, sp) == 1          # prism
+    probs = (gfm.CProblem * 2)(ok, ok)
+    ns0 = (ctypes.c_int64 * 2)(1, 0)
+    assert L.ztm_batch_create(ctypes.byref(h), 2, probs, ns0, sp) == 1         # no stations
+    assert L.ztm_batch_create(ctypes.byref(h), 0, probs, ns, sp) == 1          # empty batch
+    assert L.ztm_batch_create(None, 2, probs, ns, sp) == 1
+    assert h.value is None
+    assert L.ztm_batch_forward(None, None, None, None) == 1
+    assert L.ztm_batch_solve(None, None, None, 1, None, None, 1, None) == 1

@@ -1,0 +1,214 @@
+"""Uneven batches: the grouped persistent forward/adjoint against the
+single-problem kernels (bitwise, same tiles) and the FP64 oracle (1e-5 of
+max |g|), the batched CG solve against the FP64 CPU CG checker, HIP-graph
+replays against eager runs, and independence of each problem from the rest
+of the batch."""
+import numpy as np
+import pytest
+
+from cg_ref import cg_normal, ltl
+from ztm_amd import gfm, workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def relerr(a, b):
+    return float(np.max(np.abs(np.asarray(a, np.float64) - b)) / np.max(np.abs(b)))
+
+
+def small_set(n, cells=(800, 20000), stations=(60, 1500), seed=W.SEED0 + 5):
+    return W.uneven_48(seed=seed, n_problems=n, cells=cells, stations=stations)
+
+
+def make_batch(ps):
+    return gfm.Batch([(p.grid, p.stations, p.kappa) for p in ps])
+
+
+def plan_like(b, i, p):
+    t = b.tuning(i)
+    pl = gfm.Plan(p.grid, p.stations, kappa=p.kappa)
+    pl.set_tuning(t.st_per_lane, t.rows_per_slice, t.cells_per_lane, t.adj_splits)
+    return pl
+
+
+def check_against_plans(dev, b, ps, rho, r):
+    import torch
+    g = b.forward(b.pack_cells(rho, device=dev)).cpu().numpy()
+    a = b.adjoint(b.pack_stations(r, device=dev)).cpu().numpy()
+    gs, as_ = b.unpack_stations(g), b.unpack_cells(a)
+    for i, p in enumerate(ps):
+        with plan_like(b, i, p) as pl:
+            g1 = pl.forward(torch.from_numpy(rho[i].astype(np.float32)).to(dev)).cpu().numpy()
+            a1 = pl.adjoint(torch.from_numpy(r[i].astype(np.float32)).to(dev)).cpu().numpy()
+        assert np.array_equal(gs[i], g1), i
+        assert np.array_equal(as_[i], a1), i
+    return gs, as_
+
+
+def test_batch_forward_adjoint_match_plans_and_oracle(dev, orc):
+    ps = small_set(7)
+    rho = [W.uniform(9, i, p.grid.n_cells, -300.0, 300.0) for i, p in enumerate(ps)]
+    r = [W.normal(9, 50 + i, p.stations.shape[0], 1.0) for i, p in enumerate(ps)]
+    with make_batch(ps) as b:
+        assert b.n_problems == 7
+        assert np.all(np.diff(b.cell_offsets) >= b.n_cells)
+        assert np.all(b.cell_offsets % 4096 == 0) and np.all(b.station_offsets % 256 == 0)
+        gs, as_ = check_against_plans(dev, b, ps, rho, r)
+    for i in (0, 3, 6):
+        p = ps[i]
+        want = orc.pm_forward(p.grid, p.kappa, rho[i].astype(np.float32).astype(np.float64),
+                              p.stations)
+        assert relerr(gs[i], want) < 1e-5, i
+        wa = orc.pm_adjoint(p.grid, p.kappa, p.stations, r[i].astype(np.float32).astype(np.float64))
+        assert relerr(as_[i], wa) < 1e-5, i
+
+
+def test_batch_padding_is_untouched(dev):
+    import torch
+    ps = small_set(3)
+    with make_batch(ps) as b:
+        rho = b.pack_cells([np.ones(p.grid.n_cells) for p in ps], device=dev)
+        g = torch.full((b.total_stations,), 123.0, dtype=torch.float64, device=dev)
+        b.forward(rho, out=g)
+        out = torch.full((b.total_cells,), 5.0, dtype=torch.float32, device=dev)
+        b.adjoint(torch.ones(b.total_stations, device=dev), out=out)
+        g, out = g.cpu().numpy(), out.cpu().numpy()
+        for i in range(3):
+            s0, s1 = b.station_offsets[i] + b.n_stations[i], b.station_offsets[i + 1]
+            c0, c1 = b.cell_offsets[i] + b.n_cells[i], b.cell_offsets[i + 1]
+            assert np.all(g[s0:s1] == 123.0) and np.all(out[c0:c1] == 5.0)
+
+
+def test_batch_tiles_and_autotune(dev):
+    """Every tile variant, then the autotuned one, reproduces the single-problem
+    kernels bit for bit; results across variants agree to 1e-6."""
+    ps = small_set(5, seed=W.SEED0 + 55)
+    rho = [W.uniform(8, i, p.grid.n_cells, -300.0, 300.0) for i, p in enumerate(ps)]
+    r = [W.normal(8, 50 + i, p.stations.shape[0], 1.0) for i, p in enumerate(ps)]
+    with make_batch(ps) as b:
+        base, abase = check_against_plans(dev, b, ps, rho, r)
+        for S, C, rps, sp in ((2, 8, 0, 0), (8, 16, 3, 1), (4, 8, 1, 2)):
+            b.set_tuning(-1, S, rps, C, sp)
+            gs, as_ = check_against_plans(dev, b, ps, rho, r)
+            for i in range(len(ps)):
+                assert relerr(gs[i], base[i]) < 1e-6 and relerr(as_[i], abase[i]) < 1e-6
+        b.autotune(reps=1)
+        gs, as_ = check_against_plans(dev, b, ps, rho, r)
+        for i in range(len(ps)):
+            assert relerr(gs[i], base[i]) < 1e-6 and relerr(as_[i], abase[i]) < 1e-6
+            t = b.tuning(i)
+            assert t.st_per_lane in (2, 4, 8) and t.cells_per_lane in (8, 16)
+
+
+def data_for(b, ps, dev):
+    """d = G rho_true + noise (float32), via the batch forward."""
+    import torch
+    g = b.forward(b.pack_cells([p.rho for p in ps], device=dev))
+    noise = b.pack_stations([p.noise for p in ps], dtype=torch.float64, device=dev)
+    return (g + noise).float()
+
+
+def test_batch_solve_matches_cpu_cg(dev, orc):
+    ps = small_set(4, cells=(500, 5000), stations=(40, 400), seed=W.SEED0 + 56)
+    lam = np.array([p.lam for p in ps])
+    iters = 10
+    with make_batch(ps) as b:
+        d = data_for(b, ps, dev)
+        x, rr = b.solve(d, lam, iters=iters, use_graph=False)
+        xs = b.unpack_cells(x.cpu().numpy())
+        rr = rr.cpu().numpy()
+        ds = b.unpack_stations(d.cpu().numpy().astype(np.float64))
+    for i, p in enumerate(ps):
+        g = p.grid
+
+        def fwd(v):
+            return orc.pm_forward(g, p.kappa, v, p.stations)
+
+        def adj(r_):
+            return orc.pm_adjoint(g, p.kappa, p.stations, r_)
+
+        xc, rrc = cg_normal(fwd, adj, lambda v: ltl(v, g.nx, g.ny, g.nz), ds[i], lam[i], iters)
+        assert np.linalg.norm(xs[i] - xc) <= 2e-3 * np.linalg.norm(xc), i
+        # the recursive residual agrees until FP32 rounding of r (~1e-6 of rr[0])
+        np.testing.assert_allclose(rr[:, i], rrc, rtol=2e-2, atol=1e-5 * rrc[0], err_msg=str(i))
+        assert rrc[-1] < 1e-2 * rrc[0]
+
+
+def test_batch_graph_replays_equal_eager_and_alone(dev):
+    """Graph replays (with eager work between them) equal the eager solve bit
+    for bit, and each problem solved in a batch of its own gives the same bits
+    as inside the batch."""
+    import torch
+    ps = small_set(5, seed=W.SEED0 + 57)
+    lam = np.array([p.lam for p in ps])
+    with make_batch(ps) as b:
+        d = data_for(b, ps, dev)
+        x0, rr0 = b.solve(d, lam, iters=10, use_graph=False)
+        x0, rr0 = x0.clone(), rr0.clone()
+        xe, rre = b.solve(d, lam, iters=10, use_graph=False)
+        torch.cuda.synchronize()
+        X = torch.zeros_like(x0)
+        RR = torch.zeros_like(rr0)
+        for rep in range(3):
+            X.fill_(7.0)
+            RR.fill_(-1.0)
+            b.solve(d, lam, iters=10, x=X, rr=RR, use_graph=True)
+            torch.cuda.synchronize()
+            assert torch.equal(xe, x0) and torch.equal(rre, rr0)
+            # padding between problems is never written (it keeps the 7.0)
+            for a, e in zip(b.unpack_cells(X), b.unpack_cells(x0)):
+                assert torch.equal(a, e), rep
+            assert torch.equal(RR, rr0), rep
+            b.adjoint(torch.ones(b.total_stations, device=dev))   # eager work between replays
+            torch.cuda.synchronize()
+        xs0 = b.unpack_cells(x0.cpu())
+        ds = b.unpack_stations(d.cpu())
+        tunings = [b.tuning(i) for i in range(len(ps))]
+    for i in (0, 2, 4):
+        with make_batch([ps[i]]) as one:
+            assert one.tuning(0) == tunings[i]
+            xi, rri = one.solve(one.pack_stations([ds[i]], device=dev), lam[i:i + 1], iters=10)
+            assert torch.equal(one.unpack_cells(xi.cpu())[0], xs0[i]), i
+            assert torch.equal(rri[:, 0].cpu(), rr0[:, i].cpu()), i
+
+
+def test_batch_uneven_recovers_blocks(dev):
+    """Twelve problems of the uneven workload at reduced size: 10 CG
+    iterations cut the normal-equation residual by 100x, halve the data
+    misfit, and the recovered mass per column (sum over depth) correlates with
+    that of the buried blocks (gravity fixes the horizontal distribution; the
+    depth is left to the smoothness prior)."""
+    import torch
+    ps = small_set(12, cells=(2000, 80000), stations=(100, 3000), seed=W.SEED0 + 5)
+    lam = np.array([p.lam for p in ps])
+    with make_batch(ps) as b:
+        d = data_for(b, ps, dev)
+        x, rr = b.solve(d, lam, iters=10)
+        gx = b.forward(x)
+        rr = rr.cpu().numpy()
+        res = b.unpack_stations((gx - d.double()).cpu().numpy())
+        dn = b.unpack_stations(d.cpu().numpy())
+        xs = b.unpack_cells(x.cpu().numpy())
+    def colsum(v, g):
+        return v.reshape(g.nz, g.ny, g.nx).sum(axis=0).ravel()
+
+    stats = [(float(rr[-1, i] / rr[0, i]), float(np.linalg.norm(res[i]) / np.linalg.norm(dn[i])),
+              float(np.corrcoef(colsum(xs[i], p.grid), colsum(p.rho, p.grid))[0, 1]))
+             for i, p in enumerate(ps)]
+    print("rr ratio, misfit ratio, column-mass correlation:", stats)
+    for i, (q, f, c) in enumerate(stats):
+        assert q < 1e-2 and f < 0.5 and c > 0.2, (i, q, f, c)
+
+
+def test_batch_rejects(dev):
+    import torch
+    ps = small_set(2)
+    with pytest.raises(gfm.ZtmError):
+        gfm.Batch([(ps[0].grid, ps[0].stations[:0], ps[0].kappa)])
+    with make_batch(ps) as b:
+        with pytest.raises(ValueError):
+            b.forward(torch.zeros(b.total_cells - 1, device=dev))
+        with pytest.raises(gfm.ZtmError):
+            b.set_tuning(-1, 3, 0, 16, 0)
+        with pytest.raises(gfm.ZtmError):
+            b.solve(torch.zeros(b.total_stations, device=dev), [-1.0, 1.0], iters=2)

@@ -48,6 +48,9 @@ std::vector<ProfRec> g_prof;
 ProfScope::ProfScope(const char* name, hipStream_t s) : name_(name), stream_(s) {
     std::lock_guard<std::mutex> lk(g_mu);
     if (!g_prof_on) return;
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(stream_, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone)
+        return;   // no timing events inside a graph capture
     if (hipEventCreate(&a_) != hipSuccess || hipEventCreate(&b_) != hipSuccess) {
         a_ = b_ = nullptr;
         return;
@@ -92,36 +95,8 @@ int check_device(int* dev_out) {
 
 using namespace ztm;
 
-// ----------------------------------------------------------------- plans
-struct ztm_plan_s {
-    int device = 0;
-    ztm_problem prob{};
-    PmShape shape{};
-    PmTuning tuning{};
-    double L = 1.0, sy = 1.0, sz = 1.0;
-    float4* st_f = nullptr;      // [m_pad] prepared stations, FP32
-    double4* st_d = nullptr;     // [m_pad] prepared stations, FP64
-    int4* st_c = nullptr;        // [m_pad] nearest cell index per station
-    int* col_start = nullptr;    // near-field gather index (general plans)
-    int* col_st = nullptr;
-    int kmin = 0, kmax = -1;
-    int* row_start = nullptr;    // clamped-row gather index (general plans)
-    int* row_st = nullptr;
-    int jr = 1, kr = 1, rmy = kColMargin, rmz = kColMargin;
-    int rx = kNear, ry = kNear, rz = kNear;      // correction box radii
-    int nrx = kNear, nry = kNear, nrz = kNear;   // FP64 near-field radii
-    int cmx = kColMargin, cmy = kColMargin;      // column-index margins
-    int clamp_margin = kColMargin;               // nearest-index clamp
-    int rkmin = 1 << 29, rkmax = -(1 << 29);
-    int* out_map = nullptr;      // reference mode: slot -> g index
-    double* part = nullptr;      // split-K workspace
-    size_t part_cap = 0;         // doubles
-    float* G = nullptr;          // stored matrix [m][stored_ld], or null
-    void* bws = nullptr;         // batched forward workspace
-    size_t bws_cap = 0;          // bytes
-};
 
-namespace {
+namespace ztm {
 
 int grid_check(const ztm_grid& g) {
     if (g.nx <= 0 || g.ny <= 0 || g.nz <= 0)
@@ -280,7 +255,16 @@ double pm_scale(const ztm_plan p) {
     return p->prob.kappa * V / (p->L * p->L);
 }
 
-}  // namespace
+}  // namespace ztm
+
+PmTuning ztm::effective_tuning(ztm_plan plan, int precision) {
+    PmTuning t = plan->tuning;
+    if (precision == ZTM_FP64) {
+        if (t.st_per_lane > 4) t.st_per_lane = 4;
+        if (t.cells_per_lane > 8) t.cells_per_lane = 8;
+    }
+    return t;
+}
 
 extern "C" {
 
@@ -435,14 +419,6 @@ int ztm_plan_get_tuning(ztm_plan plan, int* st_per_lane, int* rows_per_slice,
     return ZTM_OK;
 }
 
-static PmTuning effective_tuning(ztm_plan plan, int precision) {
-    PmTuning t = plan->tuning;
-    if (precision == ZTM_FP64) {
-        if (t.st_per_lane > 4) t.st_per_lane = 4;
-        if (t.cells_per_lane > 8) t.cells_per_lane = 8;
-    }
-    return t;
-}
 
 int ztm_plan_forward(ztm_plan plan, int precision, const void* rho_dev, double* g_dev,
                      void* stream) {

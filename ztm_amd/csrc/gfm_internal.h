@@ -108,6 +108,9 @@ struct PmShape {
 // Kernel launchers (gfm_pointmass.hip). `part` must hold at least the number
 // of doubles returned by the matching *_workspace() call.
 size_t pm_fwd_workspace(const PmShape& s, const PmTuning& t, int precision);
+// Forward rows per LDS slice / adjoint station splits actually used for t.
+int fwd_rows_per_slice(const PmShape& s, const PmTuning& t, size_t elem);
+int adj_splits(const PmShape& s, const PmTuning& t);
 size_t pm_adj_workspace(const PmShape& s, const PmTuning& t, int precision,
                         bool near);
 
@@ -125,6 +128,54 @@ int pm_adj_launch(const PmShape& s, const PmTuning& t, int precision,
                   const void* st, const void* r, double sy, double sz,
                   double scale, double* part, void* out, const NearArgs& near,
                   hipStream_t stream);
+
+// Grouped (multi-problem) point-mass launches, FP32 path. One persistent
+// grid takes tickets from a device counter; a ticket is one workgroup's block
+// of one problem. blk_start is the prefix sum of the blocks per descriptor,
+// item_start that of its stations (forward reduce) or cells (adjoint reduce).
+// Inputs and outputs are concatenated vectors: each descriptor carries its
+// problem's cell and station offsets. Descriptors live in device memory.
+struct GFwdDesc {
+    const float4* st;
+    double* part;
+    int64_t cell_off, st_off;
+    int nx, ny, n_rows, rps, m_pad, tiles_x, n_slices, m, S;
+    float sy, sz, thr;
+    double scale;
+    NearArgs na;
+};
+struct GAdjDesc {
+    const float4* st;
+    double* part;
+    int64_t cell_off, st_off;
+    int m, nx, ny, segs_per_row, st_per_split, m_pad, splits, blocks_x, C;
+    int64_t n_segs, n_cells;
+    float sy, sz;
+    double scale;
+    NearArgs na;
+};
+struct GroupedPass {
+    const void* desc = nullptr;          // GFwdDesc[np] or GAdjDesc[np] (device)
+    const int64_t* blk_start = nullptr;  // device, np + 1
+    const int64_t* item_start = nullptr; // device, np + 1
+    unsigned* ticket = nullptr;          // device counter, zero between passes
+    int np = 0;
+    int max_variant = 0;                 // largest S (forward) or C (adjoint) present
+    int per_cu = 8;                      // resident workgroups per CU launched
+    int64_t total_blocks = 0, total_items = 0;
+    size_t lds = 0;
+};
+// Sizes the plan's workspace for both grouped passes (call before building
+// descriptors: they hold the workspace address).
+int pm_grouped_workspace(ztm_plan p);
+int pm_fwd_desc(ztm_plan p, int64_t cell_off, int64_t st_off, GFwdDesc* d, size_t* lds,
+                int64_t* blocks);
+int pm_adj_desc(ztm_plan p, int64_t cell_off, int64_t st_off, GAdjDesc* d, size_t* lds,
+                int64_t* blocks);
+int pm_fwd_grouped_launch(const GroupedPass& gp, const float* rho, double* g, float* gf,
+                          hipStream_t stream);
+int pm_adj_grouped_launch(const GroupedPass& gp, const float* r, float* out,
+                          hipStream_t stream);
 
 // Prism formula (gfm_prism.hip).
 // FP32: per-station mid-zone correction (prism integral minus the far-field
@@ -180,4 +231,48 @@ struct ProfScope {
     hipEvent_t a_ = nullptr, b_ = nullptr;
 };
 
+}  // namespace ztm
+
+// A plan: one grid + one station set prepared on one device (ztm_plan_*).
+struct ztm_plan_s {
+    using PmShape = ztm::PmShape;
+    using PmTuning = ztm::PmTuning;
+    static constexpr int kColMargin = ztm::kColMargin;
+    static constexpr int kNear = ztm::kNear;
+    int device = 0;
+    ztm_problem prob{};
+    PmShape shape{};
+    PmTuning tuning{};
+    double L = 1.0, sy = 1.0, sz = 1.0;
+    float4* st_f = nullptr;      // [m_pad] prepared stations, FP32
+    double4* st_d = nullptr;     // [m_pad] prepared stations, FP64
+    int4* st_c = nullptr;        // [m_pad] nearest cell index per station
+    int* col_start = nullptr;    // near-field gather index (general plans)
+    int* col_st = nullptr;
+    int kmin = 0, kmax = -1;
+    int* row_start = nullptr;    // clamped-row gather index (general plans)
+    int* row_st = nullptr;
+    int jr = 1, kr = 1, rmy = kColMargin, rmz = kColMargin;
+    int rx = kNear, ry = kNear, rz = kNear;      // correction box radii
+    int nrx = kNear, nry = kNear, nrz = kNear;   // FP64 near-field radii
+    int cmx = kColMargin, cmy = kColMargin;      // column-index margins
+    int clamp_margin = kColMargin;               // nearest-index clamp
+    int rkmin = 1 << 29, rkmax = -(1 << 29);
+    int* out_map = nullptr;      // reference mode: slot -> g index
+    double* part = nullptr;      // split-K workspace
+    size_t part_cap = 0;         // doubles
+    float* G = nullptr;          // stored matrix [m][stored_ld], or null
+    void* bws = nullptr;         // batched forward workspace
+    size_t bws_cap = 0;          // bytes
+};
+
+namespace ztm {
+// Plan helpers (gfm_abi.hip).
+int grid_check(const ztm_grid& g);
+int ensure_workspace(ztm_plan p, size_t doubles);
+void plan_free(ztm_plan p);
+int set_shape(ztm_plan p, int64_t m);
+NearArgs near_args(const ztm_plan p);
+double pm_scale(const ztm_plan p);
+PmTuning effective_tuning(ztm_plan plan, int precision);
 }  // namespace ztm

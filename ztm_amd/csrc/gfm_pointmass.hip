@@ -172,12 +172,10 @@ __global__ __launch_bounds__(256) void pm_fwd_kernel(
 //   (general API, FP64 out), or gref[map[m]] += (float)(...) (reference ABI).
 // kCorrPmNear: FP64 near field over the (2 kNear + 1)^3 box around the
 // station; kCorrPrism: the per-station correction computed beforehand.
-__device__ __forceinline__ void pm_fwd_reduce_one(const double* __restrict__ part, int n_slices,
-                                                  int m_pad, int i, double scale,
-                                                  double* __restrict__ g, float* __restrict__ gref,
-                                                  const int* __restrict__ out_map,
-                                                  const float* __restrict__ rho_f,
-                                                  const NearArgs& na) {
+__device__ __forceinline__ double pm_fwd_reduce_value(const double* __restrict__ part,
+                                                     int n_slices, int m_pad, int i, double scale,
+                                                     const float* __restrict__ rho_f,
+                                                     const NearArgs& na) {
     double s = 0.0;
     for (int k = 0; k < n_slices; ++k) s += part[(size_t)k * m_pad + i];
     if (na.mode == kCorrPmNear) {
@@ -200,12 +198,7 @@ __device__ __forceinline__ void pm_fwd_reduce_one(const double* __restrict__ par
     } else if (na.mode == kCorrPrism) {
         s += na.corr[i];
     }
-    s *= scale;
-    if (out_map) {
-        gref[out_map[i]] += (float)s;
-    } else {
-        g[i] = s;
-    }
+    return s * scale;
 }
 
 __global__ __launch_bounds__(256) void pm_fwd_reduce_kernel(const double* __restrict__ part,
@@ -217,7 +210,11 @@ __global__ __launch_bounds__(256) void pm_fwd_reduce_kernel(const double* __rest
                                      NearArgs na) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= m) return;
-    pm_fwd_reduce_one(part, n_slices, m_pad, i, scale, g, gref, out_map, rho_f, na);
+    const double v = pm_fwd_reduce_value(part, n_slices, m_pad, i, scale, rho_f, na);
+    if (out_map)
+        gref[out_map[i]] += (float)v;
+    else
+        g[i] = v;
 }
 
 // ---------------------------------------------------------------- adjoint
@@ -395,6 +392,120 @@ __global__ __launch_bounds__(256) void pm_adj_reduce_kernel(const double* __rest
     pm_adj_reduce_one<T, QUAD>(part, n_splits, n_cells, n, scale, out, r_f, na);
 }
 
+
+// ------------------------------------------------------ grouped launches
+// Index of the problem owning item `b` in a prefix array start[0..np].
+__device__ __forceinline__ int find_problem(const int64_t* __restrict__ start, int np, int64_t b) {
+    int lo = 0, hi = np - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (start[mid] <= b) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+// Next ticket of a persistent workgroup, uniform across the workgroup.
+__device__ __forceinline__ int next_ticket(unsigned* ticket, int* slot) {
+    if (threadIdx.x == 0) *slot = (int)atomicAdd(ticket, 1u);
+    __syncthreads();
+    const int t = __builtin_amdgcn_readfirstlane(*slot);
+    __syncthreads();   // the slot is rewritten only after every lane has read it
+    return t;
+}
+
+// Persistent forward: each workgroup takes tickets (one ticket = one tile of
+// 64*S stations x one row slice of one problem) until the list is exhausted.
+// Problems are ordered by decreasing block cost, so the cheap blocks fill the
+// tail. MAXS bounds the station-per-lane variants compiled in (and so the
+// registers the kernel needs).
+template <int MAXS>
+__global__ __launch_bounds__(256) void pm_fwd_grouped_kernel(
+    const GFwdDesc* __restrict__ d, const int64_t* __restrict__ blk_start, int np, int total,
+    const float* __restrict__ rho, unsigned* __restrict__ ticket) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ int slot;
+    const QuadArgs qa{};
+    for (;;) {
+        const int b = next_ticket(ticket, &slot);
+        if (b >= total) break;
+        const int p = find_problem(blk_start, np, b);
+        const GFwdDesc& q = d[p];
+        const int local = b - (int)blk_start[p];
+        const int bx = local % q.tiles_x, by = local / q.tiles_x;
+        const float* rp = rho + q.cell_off;
+        if (MAXS >= 8 && q.S == 8)
+            pm_fwd_block<float, (MAXS >= 8 ? 8 : 4), false>(q.st, rp, q.nx, q.ny, q.n_rows, q.sy, q.sz,
+                                                         q.thr, q.rps, q.part, q.m_pad, qa, bx, by, smem);
+        else if (q.S == 2)
+            pm_fwd_block<float, 2, false>(q.st, rp, q.nx, q.ny, q.n_rows, q.sy, q.sz, q.thr, q.rps,
+                                          q.part, q.m_pad, qa, bx, by, smem);
+        else
+            pm_fwd_block<float, 4, false>(q.st, rp, q.nx, q.ny, q.n_rows, q.sy, q.sz, q.thr, q.rps,
+                                          q.part, q.m_pad, qa, bx, by, smem);
+        __syncthreads();   // LDS is reused by the next ticket
+    }
+}
+
+// Per station: slice sum + FP64 near-field correction; writes g (FP64) and/or
+// gf (FP32) at the problem's station offset. Resets the forward ticket for
+// the next pass (the persistent kernel before it has completed).
+__global__ __launch_bounds__(256) void pm_fwd_reduce_grouped_kernel(
+    const GFwdDesc* __restrict__ d, const int64_t* __restrict__ st_start, int np, int64_t total,
+    const float* __restrict__ rho, double* __restrict__ g, float* __restrict__ gf,
+    unsigned* __restrict__ ticket) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t == 0) *ticket = 0u;
+    if (t >= total) return;
+    const int p = find_problem(st_start, np, t);
+    const GFwdDesc& q = d[p];
+    const int i = (int)(t - st_start[p]);
+    const double v =
+        pm_fwd_reduce_value(q.part, q.n_slices, q.m_pad, i, q.scale, rho + q.cell_off, q.na);
+    if (g) g[q.st_off + i] = v;
+    if (gf) gf[q.st_off + i] = (float)v;
+}
+
+// Persistent adjoint: one ticket = 64 segments of C cells x one station split.
+template <int MAXC>
+__global__ __launch_bounds__(256) void pm_adj_grouped_kernel(
+    const GAdjDesc* __restrict__ d, const int64_t* __restrict__ blk_start, int np, int total,
+    const float* __restrict__ r, unsigned* __restrict__ ticket) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ int slot;
+    const QuadArgs qa{};
+    for (;;) {
+        const int b = next_ticket(ticket, &slot);
+        if (b >= total) break;
+        const int p = find_problem(blk_start, np, b);
+        const GAdjDesc& q = d[p];
+        const int local = b - (int)blk_start[p];
+        const int bx = local % q.blocks_x, by = local / q.blocks_x;
+        const float* rp = r + q.st_off;
+        if (MAXC >= 16 && q.C == 16)
+            pm_adj_block<float, (MAXC >= 16 ? 16 : 8), false>(
+                q.st, rp, q.m, q.nx, q.ny, q.sy, q.sz, q.segs_per_row, q.n_segs, q.st_per_split,
+                q.m_pad, q.part, nullptr, q.scale, q.n_cells, 0, qa, bx, by, smem);
+        else
+            pm_adj_block<float, 8, false>(q.st, rp, q.m, q.nx, q.ny, q.sy, q.sz, q.segs_per_row,
+                                          q.n_segs, q.st_per_split, q.m_pad, q.part, nullptr,
+                                          q.scale, q.n_cells, 0, qa, bx, by, smem);
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(256) void pm_adj_reduce_grouped_kernel(
+    const GAdjDesc* __restrict__ d, const int64_t* __restrict__ cell_start, int np,
+    int64_t total, const float* __restrict__ r, float* __restrict__ out,
+    unsigned* __restrict__ ticket) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t == 0) *ticket = 0u;
+    if (t >= total) return;
+    const int p = find_problem(cell_start, np, t);
+    const GAdjDesc& q = d[p];
+    pm_adj_reduce_one<float, false>(q.part, q.splits, q.n_cells, t - cell_start[p], q.scale,
+                                    out + q.cell_off, r + q.st_off, q.na);
+}
+
 // ------------------------------------------------------- station prep
 __global__ void prep_ref_kernel(const float* __restrict__ s, int bfr, int di,
                                 int fh, int fv, int ni_count, int m_int,
@@ -434,8 +545,6 @@ int prep_ref_stations(const float* s, int bfr, int di, int dj, int fh, int fv,
 }
 
 // ------------------------------------------------------------ launchers
-namespace {
-
 constexpr int kTargetWgs = 2048;   // 8 workgroups per CU on 256 CUs
 
 int fwd_rows_per_slice(const PmShape& s, const PmTuning& t, size_t elem) {
@@ -460,6 +569,8 @@ int adj_splits(const PmShape& s, const PmTuning& t) {
     if (sp <= 0) sp = (int)std::max<int64_t>(1, (kTargetWgs + wgs - 1) / wgs);
     return std::max(1, std::min(sp, max_splits));
 }
+
+namespace {
 
 template <typename T, int S>
 void launch_fwd(const PmShape& s, int rps, int n_slices, const void* st,
@@ -605,6 +716,140 @@ int pm_adj_launch(const PmShape& s, const PmTuning& t, int precision,
                                (const float*)r, na);
         ZTM_HIP(hipGetLastError());
     }
+    return ZTM_OK;
+}
+
+
+// ------------------------------------------------------ grouped launchers
+int pm_grouped_workspace(ztm_plan p) {
+    const PmTuning t = effective_tuning(p, ZTM_FP32);
+    return ensure_workspace(p, std::max(pm_fwd_workspace(p->shape, t, ZTM_FP32),
+                                        pm_adj_workspace(p->shape, t, ZTM_FP32, true)));
+}
+
+int pm_fwd_desc(ztm_plan p, int64_t cell_off, int64_t st_off, GFwdDesc* d, size_t* lds,
+                int64_t* blocks) {
+    const PmTuning t = effective_tuning(p, ZTM_FP32);
+    const PmShape& s = p->shape;
+    const int nxp = (s.nx + 3) & ~3;
+    if ((size_t)nxp * sizeof(float) > 65536) return fail(ZTM_EINVAL, "nx=%d too large", s.nx);
+    if (t.st_per_lane != 2 && t.st_per_lane != 4 && t.st_per_lane != 8)
+        return fail(ZTM_EINVAL, "grouped forward needs st_per_lane 2, 4 or 8 (got %d)",
+                    t.st_per_lane);
+    const int rps = fwd_rows_per_slice(s, t, sizeof(float));
+    d->st = p->st_f;
+    d->part = p->part;
+    d->cell_off = cell_off;
+    d->st_off = st_off;
+    d->nx = s.nx;
+    d->ny = s.ny;
+    d->n_rows = (int)s.n_rows;
+    d->rps = rps;
+    d->m_pad = s.m_pad;
+    d->tiles_x = s.m_pad / (64 * t.st_per_lane);
+    d->n_slices = (int)((s.n_rows + rps - 1) / rps);
+    d->m = s.m;
+    d->S = t.st_per_lane;
+    d->sy = (float)p->sy;
+    d->sz = (float)p->sz;
+    d->thr = (float)p->prob.threshold;
+    d->scale = pm_scale(p);
+    d->na = near_args(p);
+    *lds = std::max((size_t)rps * nxp * sizeof(float),
+                    (size_t)4 * t.st_per_lane * 64 * sizeof(double));
+    *blocks = s.m == 0 ? 0 : (int64_t)d->tiles_x * d->n_slices;
+    return ZTM_OK;
+}
+
+int pm_adj_desc(ztm_plan p, int64_t cell_off, int64_t st_off, GAdjDesc* d, size_t* lds,
+                int64_t* blocks) {
+    const PmTuning t = effective_tuning(p, ZTM_FP32);
+    const PmShape& s = p->shape;
+    const int C = t.cells_per_lane;
+    if (C != 8 && C != 16)
+        return fail(ZTM_EINVAL, "grouped adjoint needs cells_per_lane 8 or 16 (got %d)", C);
+    const int splits = adj_splits(s, t);
+    const int segs_per_row = (s.nx + C - 1) / C;
+    const int64_t n_segs = s.n_rows * segs_per_row;
+    const int tiles = s.m_pad / 256;
+    d->st = p->st_f;
+    d->part = p->part;
+    d->cell_off = cell_off;
+    d->st_off = st_off;
+    d->m = s.m;
+    d->nx = s.nx;
+    d->ny = s.ny;
+    d->segs_per_row = segs_per_row;
+    d->st_per_split = ((tiles + splits - 1) / splits) * 256;
+    d->m_pad = s.m_pad;
+    d->splits = splits;
+    d->blocks_x = (int)((n_segs + 63) / 64);
+    d->C = C;
+    d->n_segs = n_segs;
+    d->n_cells = s.n_cells;
+    d->sy = (float)p->sy;
+    d->sz = (float)p->sz;
+    d->scale = pm_scale(p);
+    d->na = near_args(p);
+    *lds = std::max((size_t)256 * sizeof(float4), (size_t)4 * C * 64 * sizeof(double));
+    *blocks = (int64_t)d->blocks_x * splits;
+    return ZTM_OK;
+}
+
+namespace {
+// Persistent grid: a few resident workgroups per CU (256 CUs).
+unsigned persistent_grid(int64_t total_blocks, int per_cu) {
+    return (unsigned)std::max<int64_t>(1, std::min<int64_t>(total_blocks, 256 * per_cu));
+}
+}  // namespace
+
+int pm_fwd_grouped_launch(const GroupedPass& gp, const float* rho, double* g, float* gf,
+                          hipStream_t stream) {
+    if (gp.np == 0) return ZTM_OK;
+    if (gp.total_blocks > INT32_MAX) return fail(ZTM_EINVAL, "too many forward blocks");
+    if (gp.total_blocks > 0) {
+        ProfScope prof("pm_fwd_grouped", stream);
+        const dim3 grid(persistent_grid(gp.total_blocks, gp.per_cu));
+        const int total = (int)gp.total_blocks;
+        const auto* dd = (const GFwdDesc*)gp.desc;
+        if (gp.max_variant > 4)
+            hipLaunchKernelGGL(pm_fwd_grouped_kernel<8>, grid, dim3(256), gp.lds, stream, dd,
+                               gp.blk_start, gp.np, total, rho, gp.ticket);
+        else
+            hipLaunchKernelGGL(pm_fwd_grouped_kernel<4>, grid, dim3(256), gp.lds, stream, dd,
+                               gp.blk_start, gp.np, total, rho, gp.ticket);
+        ZTM_HIP(hipGetLastError());
+    }
+    const int64_t items = std::max<int64_t>(gp.total_items, 1);
+    hipLaunchKernelGGL(pm_fwd_reduce_grouped_kernel, dim3((unsigned)((items + 255) / 256)),
+                       dim3(256), 0, stream, (const GFwdDesc*)gp.desc, gp.item_start, gp.np,
+                       gp.total_items, rho, g, gf, gp.ticket);
+    ZTM_HIP(hipGetLastError());
+    return ZTM_OK;
+}
+
+int pm_adj_grouped_launch(const GroupedPass& gp, const float* r, float* out,
+                          hipStream_t stream) {
+    if (gp.np == 0) return ZTM_OK;
+    if (gp.total_blocks > INT32_MAX) return fail(ZTM_EINVAL, "too many adjoint blocks");
+    if (gp.total_blocks > 0) {
+        ProfScope prof("pm_adj_grouped", stream);
+        const dim3 grid(persistent_grid(gp.total_blocks, gp.per_cu));
+        const int total = (int)gp.total_blocks;
+        const auto* dd = (const GAdjDesc*)gp.desc;
+        if (gp.max_variant > 8)
+            hipLaunchKernelGGL(pm_adj_grouped_kernel<16>, grid, dim3(256), gp.lds, stream, dd,
+                               gp.blk_start, gp.np, total, r, gp.ticket);
+        else
+            hipLaunchKernelGGL(pm_adj_grouped_kernel<8>, grid, dim3(256), gp.lds, stream, dd,
+                               gp.blk_start, gp.np, total, r, gp.ticket);
+        ZTM_HIP(hipGetLastError());
+    }
+    const int64_t items = std::max<int64_t>(gp.total_items, 1);
+    hipLaunchKernelGGL(pm_adj_reduce_grouped_kernel, dim3((unsigned)((items + 255) / 256)),
+                       dim3(256), 0, stream, (const GAdjDesc*)gp.desc, gp.item_start, gp.np,
+                       gp.total_items, r, out, gp.ticket);
+    ZTM_HIP(hipGetLastError());
     return ZTM_OK;
 }
 

@@ -76,6 +76,17 @@ SIGNATURES = {
     "ztm_plan_adjoint_stored": (_I, [_P, _P, _P, _P]),
     "ztm_plan_set_tuning": (_I, [_P, _I, _I, _I, _I]),
     "ztm_plan_get_tuning": (_I, [_P, _IP, _IP, _IP, _IP]),
+    "ztm_batch_create": (_I, [ctypes.POINTER(_P), _I, ctypes.POINTER(CProblem),
+                              ctypes.POINTER(_I64), ctypes.POINTER(_P)]),
+    "ztm_batch_destroy": (_I, [_P]),
+    "ztm_batch_num_problems": (_I, [_P]),
+    "ztm_batch_layout": (_I, [_P, ctypes.POINTER(_I64), ctypes.POINTER(_I64)]),
+    "ztm_batch_forward": (_I, [_P, _P, _P, _P]),
+    "ztm_batch_adjoint": (_I, [_P, _P, _P, _P]),
+    "ztm_batch_solve": (_I, [_P, _P, ctypes.POINTER(_D), _I, _P, _P, _I, _P]),
+    "ztm_batch_set_tuning": (_I, [_P, _I, _I, _I, _I, _I]),
+    "ztm_batch_get_tuning": (_I, [_P, _I, _IP, _IP, _IP, _IP, _IP]),
+    "ztm_batch_autotune": (_I, [_P, _I, _P]),
     "ztm_profile_enable": (_I, [_I]),
     "ztm_profile_read": (_I, [ctypes.c_char_p, ctypes.POINTER(_D), ctypes.POINTER(_I64)]),
     "ztm_profile_reset": (_I, []),
@@ -281,6 +292,153 @@ class Plan:
         _check_t(out, r.dtype, self.n_cells, "out")
         self.adjoint_ptr(prec, _ptr(r), _ptr(out), _stream(stream))
         return out
+
+
+# ---------------------------------------------------------- uneven batches
+@dataclass
+class BatchTuning:
+    st_per_lane: int
+    rows_per_slice: int
+    cells_per_lane: int
+    adj_splits: int
+    size_class: int
+
+
+class Batch:
+    """Independent point-mass problems run together (ztm_batch_*): one
+    persistent launch per forward/adjoint pass over all of them, and a batched
+    CG solve of the regularised normal equations, optionally as a HIP graph.
+
+    problems: sequence of (grid, stations[M,3], kappa) or (grid, stations,
+    kappa, threshold). Device vectors are concatenated; cell_offsets /
+    station_offsets give each problem's start (pack/unpack help)."""
+
+    def __init__(self, problems):
+        probs = []
+        sts = []
+        for pr in problems:
+            grid, stations, kappa = pr[0], pr[1], pr[2]
+            thr = pr[3] if len(pr) > 3 else 0.0
+            st = np.ascontiguousarray(stations, dtype=np.float64).reshape(-1, 3)
+            probs.append(CProblem(CGrid(grid.nx, grid.ny, grid.nz, grid.x0, grid.y0, grid.z0,
+                                        grid.dx, grid.dy, grid.dz), POINT_MASS, kappa, thr))
+            sts.append(st)
+        n = len(probs)
+        parr = (CProblem * n)(*probs)
+        narr = (ctypes.c_int64 * n)(*[st.shape[0] for st in sts])
+        sarr = (ctypes.c_void_p * n)(*[st.ctypes.data for st in sts])
+        h = ctypes.c_void_p()
+        check(lib().ztm_batch_create(ctypes.byref(h), n, parr, narr, sarr))
+        self._h = h
+        self.n_problems = n
+        self.n_cells = [int(pr[0].nx * pr[0].ny * pr[0].nz) for pr in problems]
+        self.n_stations = [st.shape[0] for st in sts]
+        co = (ctypes.c_int64 * (n + 1))()
+        so = (ctypes.c_int64 * (n + 1))()
+        check(lib().ztm_batch_layout(h, co, so))
+        self.cell_offsets = np.array(co[:], dtype=np.int64)
+        self.station_offsets = np.array(so[:], dtype=np.int64)
+
+    def close(self) -> None:
+        if getattr(self, "_h", None) is not None and self._h.value:
+            lib().ztm_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @property
+    def total_cells(self) -> int:
+        return int(self.cell_offsets[-1])
+
+    @property
+    def total_stations(self) -> int:
+        return int(self.station_offsets[-1])
+
+    def pack_cells(self, arrays, dtype=None, device=None):
+        """Concatenate per-problem cell vectors (padding = 0) into one tensor."""
+        return self._pack(arrays, self.cell_offsets, self.n_cells, dtype, device)
+
+    def pack_stations(self, arrays, dtype=None, device=None):
+        return self._pack(arrays, self.station_offsets, self.n_stations, dtype, device)
+
+    def unpack_cells(self, t):
+        return [t[int(o):int(o) + n] for o, n in zip(self.cell_offsets, self.n_cells)]
+
+    def unpack_stations(self, t):
+        return [t[int(o):int(o) + n] for o, n in zip(self.station_offsets, self.n_stations)]
+
+    @staticmethod
+    def _pack(arrays, offs, sizes, dtype, device):
+        import torch
+        dtype = dtype or torch.float32
+        out = torch.zeros(int(offs[-1]), dtype=dtype, device=device or "cuda")
+        for a, o, n in zip(arrays, offs, sizes):
+            a = torch.as_tensor(np.asarray(a) if not isinstance(a, torch.Tensor) else a)
+            if a.numel() != n:
+                raise ValueError(f"expected {n} values, got {a.numel()}")
+            out[int(o):int(o) + n] = a.reshape(-1).to(device=out.device, dtype=dtype)
+        return out
+
+    def forward(self, rho, out=None, stream=None):
+        """rho float32 [total_cells] -> g float64 [total_stations]."""
+        import torch
+        _check_t(rho, torch.float32, self.total_cells, "rho")
+        if out is None:
+            out = torch.zeros(self.total_stations, dtype=torch.float64, device=rho.device)
+        _check_t(out, torch.float64, self.total_stations, "g")
+        check(lib().ztm_batch_forward(self._h, _ptr(rho), _ptr(out), _stream(stream)))
+        return out
+
+    def adjoint(self, r, out=None, stream=None):
+        """r float32 [total_stations] -> out float32 [total_cells]."""
+        import torch
+        _check_t(r, torch.float32, self.total_stations, "r")
+        if out is None:
+            out = torch.zeros(self.total_cells, dtype=torch.float32, device=r.device)
+        _check_t(out, torch.float32, self.total_cells, "out")
+        check(lib().ztm_batch_adjoint(self._h, _ptr(r), _ptr(out), _stream(stream)))
+        return out
+
+    def solve(self, d, lam, iters=10, x=None, rr=None, use_graph=True, stream=None):
+        """CG on (G^T G + lam_s L^T L) x = G^T d per problem, x(0) = 0.
+        Returns (x float32 [total_cells], rr float64 [iters + 1, n_problems])."""
+        import torch
+        _check_t(d, torch.float32, self.total_stations, "d")
+        lam = np.ascontiguousarray(np.broadcast_to(np.asarray(lam, np.float64),
+                                                   (self.n_problems,)))
+        if x is None:
+            x = torch.zeros(self.total_cells, dtype=torch.float32, device=d.device)
+        _check_t(x, torch.float32, self.total_cells, "x")
+        if rr is None:
+            rr = torch.zeros(iters + 1, self.n_problems, dtype=torch.float64, device=d.device)
+        _check_t(rr, torch.float64, (iters + 1) * self.n_problems, "rr")
+        check(lib().ztm_batch_solve(self._h, _ptr(d), lam.ctypes.data_as(ctypes.POINTER(_D)),
+                                    iters, _ptr(x), _ptr(rr), int(bool(use_graph)),
+                                    _stream(stream)))
+        return x, rr
+
+    def set_tuning(self, problem=-1, st_per_lane=4, rows_per_slice=0, cells_per_lane=16,
+                   adj_splits=0) -> None:
+        check(lib().ztm_batch_set_tuning(self._h, problem, st_per_lane, rows_per_slice,
+                                         cells_per_lane, adj_splits))
+
+    def tuning(self, problem: int) -> BatchTuning:
+        v = [ctypes.c_int() for _ in range(5)]
+        check(lib().ztm_batch_get_tuning(self._h, problem, *[ctypes.byref(a) for a in v]))
+        return BatchTuning(*[a.value for a in v])
+
+    def autotune(self, reps: int = 3, stream=None) -> None:
+        check(lib().ztm_batch_autotune(self._h, reps, _stream(stream)))
 
 
 def _prec_of(t, n: int, name: str) -> int:

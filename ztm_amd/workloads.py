@@ -17,6 +17,10 @@ the gravity output is in mGal (kappa = 1e5 * G).
     adjoint_small  pm_cpu_small geometry, residual N(0, 0.5) mGal, lambda 1e-3
     ensemble_256   96x96x48 cells of 1 km, 256 models of U[-300,300]
                    smoothed by a clamped 3x3x3 box filter; 9,216 stations, z=+10
+    uneven_48      48 independent problems: cells log-U[2e4, 2e6], stations
+                   log-U[1e3, 4e4] scattered over the footprint at z=+10 m,
+                   cell edge log-U[250, 2000] m; 3 buried +250 kg/m^3 blocks
+                   each; data noise N(0, 0.1 mGal)
 """
 from __future__ import annotations
 
@@ -197,9 +201,68 @@ def tfm_shaped(seed: int = 7, di: int = 61, dj: int = 61, dk: int = 33, fh: int 
     return rho, s
 
 
+@dataclass
+class Subproblem:
+    grid: Grid
+    stations: np.ndarray            # [M,3] float64
+    rho: np.ndarray                 # [N] float64, the true model (blocks)
+    noise: np.ndarray               # [M] float64, mGal
+    kappa: float = KAPPA_SI
+
+    @property
+    def lam(self) -> float:
+        """Smoothness weight: 0.05 * 2 pi M (kappa h / nx)^2, a fixed fraction of
+        the squared column norm of G for a top cell under M stations spread
+        over the nx-cell footprint (recorded choice)."""
+        m = int(self.stations.shape[0])
+        return 0.05 * 2.0 * np.pi * m * (self.kappa * self.grid.dx / self.grid.nx) ** 2
+
+    @property
+    def pairs(self) -> int:
+        return int(self.stations.shape[0]) * self.grid.n_cells
+
+
+def _logu(seed: int, stream: int, lo: float, hi: float) -> float:
+    return float(np.exp(uniform(seed, stream, 1, np.log(lo), np.log(hi))[0]))
+
+
+def uneven_48(seed: int = SEED0 + 5, n_problems: int = 48, cells=(2e4, 2e6),
+              stations=(1e3, 4e4), edge=(250.0, 2000.0), noise: float = 0.1):
+    """Problems of uneven size for the batched solver. Problem s draws from
+    streams 100*s + (0..19): cell count N, station count M and edge h
+    log-uniform in the given ranges; the grid is nx = ny = 2 nz cubes with
+    nx*ny*nz ~ N; stations uniform over the footprint at z = +10 m; the true
+    model is three boxes of +250 kg/m^3 (each 10-30 % of every axis, tops at
+    least one cell below the surface) on zero background."""
+    out = []
+    for s in range(n_problems):
+        b = 100 * s
+        n = _logu(seed, b, *cells)
+        m = int(round(_logu(seed, b + 1, *stations)))
+        h = _logu(seed, b + 2, *edge)
+        nz = max(4, int(round((n / 4.0) ** (1.0 / 3.0))))
+        nx = max(4, int(round(np.sqrt(n / nz))))
+        g = cube_grid(nx, nx, nz, h)
+        xy = uniform(seed, b + 3, 2 * m, 0.0, nx * h).reshape(m, 2)
+        st = np.concatenate([xy, np.full((m, 1), 10.0)], axis=1)
+        rho = np.zeros((nz, nx, nx))
+        u = uniform(seed, b + 4, 3 * 6).reshape(3, 6)
+        for fx, fy, fz, px, py, pz in u:
+            sx = max(1, int(round(nx * (0.1 + 0.2 * fx))))
+            sy = max(1, int(round(nx * (0.1 + 0.2 * fy))))
+            sz = max(1, int(round(nz * (0.1 + 0.2 * fz))))
+            i0 = int(px * (nx - sx + 1))
+            j0 = int(py * (nx - sy + 1))
+            k0 = 1 + int(pz * max(1, nz - sz))
+            rho[k0:k0 + sz, j0:j0 + sy, i0:i0 + sx] = 250.0
+        out.append(Subproblem(g, st, rho.ravel(), normal(seed, b + 5, m, noise)))
+    return out
+
+
 WORKLOADS = {
     "pm_cpu_small": pm_cpu_small,
     "pm_blobs_1m": pm_blobs_1m,
     "adjoint_small": adjoint_small,
     "ensemble_256": ensemble_256,
+    "uneven_48": uneven_48,
 }
