@@ -92,6 +92,8 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the stored-matrix and prism legs")
+    ap.add_argument("--leg", choices=["prism", "batched", "uneven"], default=None,
+                    help="run only this extra leg and print its JSON object (for profiling)")
     ap.add_argument("--copy-calib-mib", type=int, default=0,
                     help="only run 5 device-to-device copies of this many MiB "
                          "(calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE against known bytes)")
@@ -117,9 +119,16 @@ def main() -> int:
         torch.cuda.synchronize()
         print(json.dumps({"copy_calibration_bytes": 4 * n, "copies": 5}))
         return 0
+    if args.leg in ("batched", "uneven"):
+        leg = batched_leg if args.leg == "batched" else uneven_leg
+        print(json.dumps({args.leg: leg(torch.cuda.current_stream(), args)}))
+        return 0
     w = W.pm_blobs_1m()
     r = W.normal(W.SEED0 + 2, 40, w.n_stations, 0.5)
     pairs = w.pairs
+    if args.leg == "prism":
+        print(json.dumps({"prism": prism_leg(w, r, torch.cuda.current_stream(), args)}))
+        return 0
 
     plan = gfm.Plan(w.grid, w.stations, kappa=w.kappa)
     rho32 = torch.from_numpy(w.rho.astype(np.float32)).to(dev)

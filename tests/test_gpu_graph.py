@@ -1,0 +1,62 @@
+"""Plan calls captured by the caller in a torch CUDA (HIP) graph: every replay,
+with eager work between replays, equals the eager result bit for bit."""
+import numpy as np
+import pytest
+
+from ztm_amd import gfm, workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("formula", [gfm.POINT_MASS, gfm.PRISM])
+def test_plan_forward_adjoint_in_graph(dev, formula):
+    import torch
+    w = W.pm_cpu_small()
+    rho = torch.from_numpy(w.rho.astype(np.float32)).to(dev)
+    r = torch.from_numpy(W.normal(3, 7, w.n_stations, 0.5).astype(np.float32)).to(dev)
+    with gfm.Plan(w.grid, w.stations, kappa=w.kappa, formula=formula) as p:
+        g = torch.empty(w.n_stations, dtype=torch.float64, device=dev)
+        a = torch.empty(w.grid.n_cells, dtype=torch.float32, device=dev)
+        p.forward(rho, out=g)                 # grows the workspace before capture
+        p.adjoint(r, out=a)
+        torch.cuda.synchronize()
+        g0, a0 = g.clone(), a.clone()
+        graph = torch.cuda.CUDAGraph()
+        side = torch.cuda.Stream()
+        side.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(side):
+            with torch.cuda.graph(graph):
+                p.forward(rho, out=g)
+                p.adjoint(r, out=a)
+        torch.cuda.current_stream().wait_stream(side)
+        for rep in range(3):
+            g.fill_(-1.0)
+            a.fill_(-1.0)
+            graph.replay()
+            torch.cuda.synchronize()
+            assert torch.equal(g, g0), rep
+            assert torch.equal(a, a0), rep
+            p.forward(2.0 * rho)              # eager work between replays
+            torch.cuda.synchronize()
+
+
+def test_batched_forward_in_graph(dev):
+    import torch
+    w = W.pm_cpu_small()
+    P = torch.from_numpy(np.stack([W.uniform(5, b, w.grid.n_cells, -300, 300)
+                                   for b in range(40)]).astype(np.float32)).to(dev)
+    with gfm.Plan(w.grid, w.stations, kappa=w.kappa) as p:
+        Y = torch.empty(40, w.n_stations, dtype=torch.float64, device=dev)
+        p.forward_batched(P, out=Y)
+        torch.cuda.synchronize()
+        Y0 = Y.clone()
+        graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(graph):
+            p.forward_batched(P, out=Y)
+        for rep in range(3):
+            Y.zero_()
+            graph.replay()
+            torch.cuda.synchronize()
+            assert torch.equal(Y, Y0), rep
+            p.forward_batched(P[:8].contiguous())
+            torch.cuda.synchronize()
