@@ -22,8 +22,10 @@
 // round-robin dispatch spreads one slice's tiles over the XCDs in sequence
 // and concurrently resident tiles of a slice share its P chunks in L2.
 //
-// Layouts (HBM):  P parts [N16/16][Bp][16] bf16 each, zero padded (N16 = N up
-//                 to a multiple of 16, Bp = B up to 32 x a power of two);
+// Layouts (HBM):  P parts [N16/16][2][Bp][8] bf16 each (chunk, cell half,
+//                 model, cell), zero padded (N16 = N up to a multiple of 16,
+//                 Bp = B up to 32 x a power of two), so that the B operands
+//                 of one half-wave are consecutive 16-byte slots in LDS;
 //                 part [KS][m_pad][Bp] FP32;  Y [B][M] FP64 (model-major).
 #include "gfm_internal.h"
 #include "gfm_pair.h"
@@ -39,18 +41,19 @@ constexpr int kBatchChunk = 16;   // cells per MFMA K step
 constexpr int kBatchTile = 128;   // stations per workgroup
 
 // ---------------------------------------------------------------- packing
-// One thread per packed element: t = ((chunk * Bp) + model) * 16 + c; writes
-// SPLIT parts at stride n_packed.
+// One thread per packed element: t = ((chunk * 2 + h) * Bp + model) * 8 + e
+// holds cell chunk * 16 + 8 h + e; writes SPLIT parts at stride n_packed.
 template <int SPLIT>
 __global__ void batched_pack_kernel(const float* __restrict__ P, int B, int64_t n_cells,
                                     int Bp, int64_t n_packed, float thr,
                                     __bf16* __restrict__ parts) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n_packed) return;
-    const int c = (int)(t & 15);
-    const int64_t cm = t >> 4;
-    const int model = (int)(cm % Bp);
-    const int64_t cell = (cm / Bp) * kBatchChunk + c;
+    const int e = (int)(t & 7);
+    const int64_t hm = t >> 3;
+    const int model = (int)(hm % Bp);
+    const int64_t ch = hm / Bp;   // chunk * 2 + h
+    const int64_t cell = (ch >> 1) * kBatchChunk + (ch & 1) * 8 + e;
     float v = 0.f;
     if (model < B && cell < n_cells) {
         v = P[(size_t)model * n_cells + cell];
@@ -65,13 +68,93 @@ __global__ void batched_pack_kernel(const float* __restrict__ P, int B, int64_t 
 }
 
 // --------------------------------------------------------------- forward
-template <int NT, int SPLIT>   // model tiles of 32 (B <= 32 * NT); 2- or 3-way split
+// q = n / d, r = n % d for 0 <= n < 2^31 through a double reciprocal and one
+// correction step (the product's error is far below one unit for n < 2^31).
+__device__ __forceinline__ void divmod_pos(int n, int d, double inv, int& q, int& r) {
+    q = (int)((double)n * inv);
+    r = n - q * d;
+    const bool lo = r < 0, hi = r >= d;
+    q += hi ? 1 : (lo ? -1 : 0);
+    r += hi ? -d : (lo ? d : 0);
+}
+
+// This lane's 8 G values of a chunk (cells cell0 .. cell0 + 7, raster order),
+// split into SPLIT BF16 parts in the MFMA A-operand layout. WIDE (nx >= 8):
+// the 8 cells span at most two grid rows, so both row terms are formed once
+// and each cell selects one. Cells past the grid give finite values (r^2 >=
+// 1e-20, so |dz r^-3| <= 1e36), which the zero-padded P cancels.
+template <int SPLIT, bool WIDE>
+__device__ __forceinline__ void batched_gen(const float4 p, int cell0, int nx, int ny,
+                                            double inv_nx, double inv_ny, float sy, float sz,
+                                            bf16x8 (&a)[SPLIT]) {
+    int rw, i0, k0, j0;
+    divmod_pos(cell0, nx, inv_nx, rw, i0);
+    divmod_pos(rw, ny, inv_ny, k0, j0);
+    if constexpr (WIDE) {
+        const bool wrap = j0 + 1 == ny;
+        const int j1 = wrap ? 0 : j0 + 1, k1 = wrap ? k0 + 1 : k0;
+        const float dy0 = p.y - (float)j0 * sy, dz0 = p.z - (float)k0 * sz;
+        const float dy1 = p.y - (float)j1 * sy, dz1 = p.z - (float)k1 * sz;
+        // no near-field pass follows, so the exact row term (floored only
+        // against 0/0 at a coincident station), not row_R32
+        const float R0 = fmaf(dy0, dy0, fmaf(dz0, dz0, 1e-20f));
+        const float R1 = fmaf(dy1, dy1, fmaf(dz1, dz1, 1e-20f));
+        const int ew = nx - i0;   // cells e >= ew lie on the next row
+        const float xa = p.x - (float)i0, nxf = (float)nx;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const bool w = e >= ew;
+            const float xe = xa - (float)e;
+            const float x = w ? xe + nxf : xe;
+            const float v = rsq_f(fmaf(x, x, w ? R1 : R0));
+            float gv = (w ? dz1 : dz0) * (v * v * v);
+#pragma unroll
+            for (int q = 0; q < SPLIT; ++q) {
+                const __bf16 h = (__bf16)gv;
+                a[q][e] = h;
+                gv -= (float)h;
+            }
+        }
+    } else {
+        int i = i0, j = j0, k = k0;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const float dy = p.y - (float)j * sy;
+            const float dz = p.z - (float)k * sz;
+            const float x = p.x - (float)i;
+            const float v = rsq_f(fmaf(x, x, fmaf(dy, dy, fmaf(dz, dz, 1e-20f))));
+            float gv = dz * (v * v * v);
+#pragma unroll
+            for (int q = 0; q < SPLIT; ++q) {
+                const __bf16 h = (__bf16)gv;
+                a[q][e] = h;
+                gv -= (float)h;
+            }
+            const bool wi = ++i == nx;
+            i = wi ? 0 : i;
+            j += wi ? 1 : 0;
+            const bool wj = j == ny;
+            j = wj ? 0 : j;
+            k += wj ? 1 : 0;
+        }
+    }
+}
+
+// grid = (station tiles of 128, K-slices); block = 256 (4 waves x 32 stations).
+// Software pipeline over the slice's chunks, one barrier per chunk: while the
+// MFMAs of chunk c run on its G block (registers) and P parts (LDS buffer
+// c&1), the next chunk's P parts are loaded from global memory into
+// registers and its G block is generated; the P parts then go to the other
+// LDS buffer. The last iteration repeats its own chunk's loads (unused).
+template <int NT, int SPLIT, bool WIDE>   // model tiles of 32 (B <= 32 * NT); 2- or 3-way split
 __global__ __launch_bounds__(256) void batched_fwd_kernel(
     const float4* __restrict__ st, const bf16x8* __restrict__ parts, int64_t part_stride,
-    int nx, int ny, int64_t n_cells, float sy, float sz, int64_t n_chunks,
+    int nx, int ny, double inv_nx, double inv_ny, float sy, float sz, int n_chunks,
     int chunks_per_slice, int m_pad, float* __restrict__ part) {
     constexpr int Bp = 32 * NT;
-    __shared__ bf16x8 lp[SPLIT][Bp * 2];
+    constexpr int V = Bp * 2;                          // bf16x8 per part per chunk
+    constexpr int LOADS = (V * SPLIT + 255) / 256;     // per thread per chunk
+    __shared__ bf16x8 lp[2][SPLIT][V];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int row = lane & 31, kh = lane >> 5;
     const int mbase = blockIdx.x * kBatchTile + wave * 32;
@@ -83,61 +166,76 @@ __global__ __launch_bounds__(256) void batched_fwd_kernel(
 #pragma unroll
         for (int i = 0; i < 16; ++i) acc[t][i] = 0.f;
 
-    const int64_t c0 = (int64_t)blockIdx.y * chunks_per_slice;
-    const int64_t c1 = min(c0 + (int64_t)chunks_per_slice, n_chunks);
-    for (int64_t chunk = c0; chunk < c1; ++chunk) {
-        __syncthreads();
-        for (int v = threadIdx.x; v < Bp * 2; v += blockDim.x)
+    const int c0 = blockIdx.y * chunks_per_slice;
+    const int c1 = min(c0 + chunks_per_slice, n_chunks);
+    bf16x8 g[LOADS];
+    auto fetch = [&](int chunk) {
 #pragma unroll
-            for (int q = 0; q < SPLIT; ++q) lp[q][v] = parts[q * part_stride + chunk * (Bp * 2) + v];
-        // this lane's 8 cells: chunk*16 + 8*kh + 0..7, decoded once, then stepped
-        int64_t cell = chunk * kBatchChunk + 8 * kh;
-        int i = (int)(cell % nx);
-        int64_t rw = cell / nx;
-        int j = (int)(rw % ny);
-        int k = (int)(rw / ny);
-        bf16x8 a[SPLIT];
+        for (int u = 0; u < LOADS; ++u) {
+            const int idx = threadIdx.x + 256 * u;
+            if ((V * SPLIT) % 256 == 0 || idx < V * SPLIT)
+                g[u] = parts[(idx / V) * part_stride + (int64_t)chunk * V + (idx % V)];
+        }
+    };
+    auto stash = [&](int buf) {
 #pragma unroll
-        for (int e = 0; e < 8; ++e) {
-            float gv = 0.f;
-            if (cell + e < n_cells) {
-                const float dy = p.y - (float)j * sy;
-                const float dz = p.z - (float)k * sz;
-                const float x = p.x - (float)i;
-                // no near-field pass follows, so the exact row term (floored
-                // only against 0/0 at a coincident station), not row_R32
-                const float v = rsq_f(fmaf(x, x, fmaf(dy, dy, fmaf(dz, dz, 1e-20f))));
-                gv = dz * (v * v * v);
-            }
+        for (int u = 0; u < LOADS; ++u) {
+            const int idx = threadIdx.x + 256 * u;
+            if ((V * SPLIT) % 256 == 0 || idx < V * SPLIT) lp[buf][idx / V][idx % V] = g[u];
+        }
+    };
+    bf16x8 a[SPLIT];
+    if (c0 < c1) {
+        fetch(c0);
+        stash(0);
+        batched_gen<SPLIT, WIDE>(p, c0 * kBatchChunk + 8 * kh, nx, ny, inv_nx, inv_ny, sy, sz, a);
+    }
+    __syncthreads();
+    for (int c = c0; c < c1; ++c) {
+        const int buf = (c - c0) & 1;
+        const int cn = min(c + 1, c1 - 1);
+        fetch(cn);
+        bf16x8 cur[SPLIT];
 #pragma unroll
-            for (int q = 0; q < SPLIT; ++q) {
-                const __bf16 h = (__bf16)gv;
-                a[q][e] = h;
-                gv -= (float)h;
-            }
-            if (++i == nx) {
-                i = 0;
-                if (++j == ny) {
-                    j = 0;
-                    ++k;
-                }
+        for (int q = 0; q < SPLIT; ++q) cur[q] = a[q];
+        batched_gen<SPLIT, WIDE>(p, cn * kBatchChunk + 8 * kh, nx, ny, inv_nx, inv_ny, sy, sz, a);
+        // Tiles in pairs so that consecutive MFMAs update different
+        // accumulators (no back-to-back dependence on one accumulator).
+        constexpr int TP = NT >= 2 ? 2 : 1;
+#pragma unroll
+        for (int t = 0; t < NT; t += TP) {
+            bf16x8 b[TP][SPLIT];
+#pragma unroll
+            for (int u = 0; u < TP; ++u)
+#pragma unroll
+                for (int q = 0; q < SPLIT; ++q) b[u][q] = lp[buf][q][kh * Bp + 32 * (t + u) + row];
+            // products x_qa * y_qb with qa + qb < SPLIT
+#pragma unroll
+            for (int s2 = 0; s2 < SPLIT; ++s2)
+#pragma unroll
+                for (int qa = 0; qa <= s2; ++qa)
+#pragma unroll
+                    for (int u = 0; u < TP; ++u)
+                        acc[t + u] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
+                            cur[qa], b[u][s2 - qa], acc[t + u], 0, 0, 0);
+        }
+        stash(buf ^ 1);
+        // Schedule: the next chunk's global loads first, then per tile pair
+        // its LDS reads and its MFMAs, each MFMA followed by a few VALU
+        // instructions of the next chunk's G generation; the LDS writes last.
+        constexpr int kPerPair = TP * SPLIT * (SPLIT + 1) / 2;   // MFMAs per tile pair
+        __builtin_amdgcn_sched_group_barrier(0x020, LOADS, 0);   // VMEM read
+#pragma unroll
+        for (int t = 0; t < NT; t += TP) {
+            __builtin_amdgcn_sched_group_barrier(0x100, TP * SPLIT, 0);   // DS read
+#pragma unroll
+            for (int i = 0; i < kPerPair; ++i) {
+                __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);        // MFMA
+                __builtin_amdgcn_sched_group_barrier(0x002, SPLIT == 3 ? 5 : 7, 0);   // VALU
             }
         }
+        __builtin_amdgcn_sched_group_barrier(0x200, LOADS, 0);   // DS write
         __syncthreads();
-#pragma unroll
-        for (int t = 0; t < NT; ++t) {
-            bf16x8 b[SPLIT];
-#pragma unroll
-            for (int q = 0; q < SPLIT; ++q) b[q] = lp[q][(32 * t + row) * 2 + kh];
-            acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[0], b[0], acc[t], 0, 0, 0);
-            acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[0], b[1], acc[t], 0, 0, 0);
-            acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[1], b[0], acc[t], 0, 0, 0);
-            if constexpr (SPLIT == 3) {
-                acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[0], b[2], acc[t], 0, 0, 0);
-                acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[1], b[1], acc[t], 0, 0, 0);
-                acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[2], b[0], acc[t], 0, 0, 0);
-            }
-        }
     }
     // D layout: lane holds model 32t + lane%32, stations 8*(i/4) + 4*(lane/32) + i%4
     float* out = part + ((size_t)blockIdx.y * m_pad + mbase) * Bp;
@@ -211,6 +309,8 @@ int batched_fwd_launch(const PmShape& s, int B, int ks, int split, const float* 
     const int nt = batched_nt(B);
     if (nt > 8) return fail(ZTM_EINVAL, "batched forward supports at most 256 models per call (got %d)", B);
     const int Bp = 32 * nt;
+    if (s.n_cells > (int64_t)INT32_MAX - 2 * kBatchChunk)
+        return fail(ZTM_EINVAL, "batched forward supports up to 2^31 - 32 cells");
     const int64_t n_chunks = (s.n_cells + kBatchChunk - 1) / kBatchChunk;
     ks = std::max(1, std::min<int>(ks, (int)std::min<int64_t>(n_chunks, 65535)));
     const int per = (int)((n_chunks + ks - 1) / ks);
@@ -230,9 +330,16 @@ int batched_fwd_launch(const PmShape& s, int B, int ks, int split, const float* 
     {
         ProfScope prof("batched_fwd", stream);
 #define ZTM_BATCHED_CASE(NT, SP)                                                               \
-    hipLaunchKernelGGL((batched_fwd_kernel<NT, SP>), grid, dim3(256), 0, stream,               \
-                       (const float4*)st, (const bf16x8*)parts, stride, s.nx, s.ny, s.n_cells, \
-                       (float)sy, (float)sz, n_chunks, per, s.m_pad, part)
+    if (s.nx >= 8)                                                                             \
+        hipLaunchKernelGGL((batched_fwd_kernel<NT, SP, true>), grid, dim3(256), 0, stream,     \
+                           (const float4*)st, (const bf16x8*)parts, stride, s.nx, s.ny,        \
+                           1.0 / s.nx, 1.0 / s.ny, (float)sy, (float)sz, (int)n_chunks, per,   \
+                           s.m_pad, part);                                                     \
+    else                                                                                       \
+        hipLaunchKernelGGL((batched_fwd_kernel<NT, SP, false>), grid, dim3(256), 0, stream,    \
+                           (const float4*)st, (const bf16x8*)parts, stride, s.nx, s.ny,        \
+                           1.0 / s.nx, 1.0 / s.ny, (float)sy, (float)sz, (int)n_chunks, per,   \
+                           s.m_pad, part)
         if (split == 3) {
             switch (nt) {
                 case 1: ZTM_BATCHED_CASE(1, 3); break;
