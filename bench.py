@@ -92,7 +92,7 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the stored-matrix and prism legs")
-    ap.add_argument("--leg", choices=["prism", "batched", "uneven"], default=None,
+    ap.add_argument("--leg", choices=["prism", "batched", "uneven", "tfm"], default=None,
                     help="run only this extra leg and print its JSON object (for profiling)")
     ap.add_argument("--copy-calib-mib", type=int, default=0,
                     help="only run 5 device-to-device copies of this many MiB "
@@ -119,8 +119,8 @@ def main() -> int:
         torch.cuda.synchronize()
         print(json.dumps({"copy_calibration_bytes": 4 * n, "copies": 5}))
         return 0
-    if args.leg in ("batched", "uneven"):
-        leg = batched_leg if args.leg == "batched" else uneven_leg
+    if args.leg in ("batched", "uneven", "tfm"):
+        leg = {"batched": batched_leg, "uneven": uneven_leg, "tfm": tfm_leg}[args.leg]
         print(json.dumps({args.leg: leg(torch.cuda.current_stream(), args)}))
         return 0
     w = W.pm_blobs_1m()
@@ -233,6 +233,7 @@ def main() -> int:
         line["prism"] = prism_leg(w, r, stream, args)
         line["batched"] = batched_leg(stream, args)
         line["uneven_batch"] = uneven_leg(stream, args)
+        line["tfm_fm"] = tfm_leg(stream, args)
     print(json.dumps(line))
     return 0
 
@@ -448,6 +449,103 @@ def uneven_leg(stream, args, iters=10, solves=2):
                   "cells_per_lane": sorted({t.cells_per_lane for t in tunings}),
                   "size_classes": sorted({t.size_class for t in tunings})},
     })
+    return out
+
+
+def tfm_leg(stream, args, reps=20):
+    """One gravity forward model of a ZTM FM call at the reference's own grid
+    (TFM.cpp:83-94; tfm_fm: 61x61x33 coarse, fh=1, fv=2 -> 241,865 fine cells,
+    1,681 interior stations): the density producer and GF on the device with
+    inputs resident in HBM, the same through the host entry point (PCIe in
+    and out), the GFM program behind the reference's file contract (process
+    start, files, PCIe), and the reference's CPU path restated (producer +
+    TFM.cpp:95-99 gravity with one thread per station row, and on 16
+    threads) on this host."""
+    import subprocess
+    import tempfile
+    import torch
+    import oracle
+    from ztm_amd import gfm, workloads as W
+    dev = torch.device("cuda:0")
+    w = W.tfm_fm()
+    M = w.di * w.dj
+    ni, nj = w.di - 2 * w.bfr, w.dj - 2 * w.bfr
+    pairs = w.n_fine * ni * nj
+    t64 = lambda a: torch.from_numpy(np.ascontiguousarray(a, np.float64)).to(dev)
+    sea, moho, vel = t64(w.sea), t64(w.moho), t64(w.vel)
+    s32 = torch.from_numpy(w.surf.astype(np.float32)).to(dev)
+    drho = torch.empty(w.n_fine, dtype=torch.float32, device=dev)
+    g = torch.zeros(M, dtype=torch.float32, device=dev)
+
+    def fm_device():
+        gfm.tfm_drho(w.di, w.dj, w.dk, w.fh, w.fv, w.rc, sea, moho, vel, w.vm, w.ro, w.dr,
+                     out=drho, stream=stream)
+        g.zero_()
+        gfm.GF(w.bfr, w.di, w.dj, w.dk, w.fh, w.fv, g, drho, s32, stream=stream)
+
+    gfm.profile_reset()
+    gfm.profile_enable(True)
+    t_dev = _timed(fm_device, reps, torch.cuda.synchronize)
+    gfm.profile_enable(False)
+    p_ms, p_n = gfm.profile_read("tfm_drho")
+    f_ms, f_n = gfm.profile_read("pm_fwd")
+    gfm.profile_reset()
+    g_dev = g.cpu().numpy().astype(np.float64)
+    host_args = (w.bfr, w.di, w.dj, w.dk, w.fh, w.fv, w.rc, w.sea, w.moho, w.vel, w.surf, w.vm,
+                 w.ro, w.dr)
+    t_host = _timed(lambda: gfm.tfm_gravity_host(*host_args), reps, lambda: None)
+    G_host = gfm.tfm_gravity_host(*host_args)
+    # the GFM program on the reference's files (R.dat / S.dat -> G.dat)
+    exe = os.path.join(ROOT, "ztm_amd", "bin", "GFM")
+    t_prog = None
+    with tempfile.TemporaryDirectory() as tmp:
+        drho.cpu().numpy().tofile(os.path.join(tmp, "R.dat"))
+        w.surf.astype(np.float32).tofile(os.path.join(tmp, "S.dat"))
+        env = dict(os.environ, ZTM_GFM_DIR=tmp)
+        runs = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            ok = subprocess.run([exe], env=env, capture_output=True).returncode == 0
+            runs.append(time.perf_counter() - t0)
+            if not ok:
+                break
+        if ok:
+            t_prog = min(runs)
+            G_prog = np.fromfile(os.path.join(tmp, "G.dat"), dtype=np.float32)
+    # the reference's CPU path, restated (producer + gravity lambda)
+    t0 = time.perf_counter()
+    d64, nn = oracle.tfm_drho(w.di, w.dj, w.dk, w.fh, w.fv, w.rc, w.sea, w.moho, w.vel, w.vm,
+                              w.ro, w.dr)
+    t_prod_cpu = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    G_cpu = oracle.tfm_gravity(w.bfr, w.di, w.dj, w.dk, w.fh, w.fv, d64, w.surf, nn, threads=0)
+    t_grav_rows = time.perf_counter() - t0
+    cores = min(16, os.cpu_count() or 1)
+    t0 = time.perf_counter()
+    oracle.tfm_gravity(w.bfr, w.di, w.dj, w.dk, w.fh, w.fv, d64, w.surf, nn, threads=cores)
+    t_grav_cores = time.perf_counter() - t0
+    interior = np.zeros((w.dj, w.di), bool)
+    interior[w.bfr:w.dj - w.bfr, w.bfr:w.di - w.bfr] = True
+    interior = interior.ravel()
+    err = relerr(g_dev[interior], G_cpu[interior])
+    out = {
+        "workload": "tfm_fm: TFM.cpp:83-94 at the reference grid, 241,865 fine cells x 1,681 "
+                    "interior stations, producer + gravity per FM call",
+        "unit": "ms per FM call", "pairs_per_call": pairs, "kept_cells": int(nn.size),
+        "ms_device_resident": t_dev * 1e3,
+        "kernels_ms": {"tfm_drho": p_ms / max(p_n, 1), "pm_fwd": f_ms / max(f_n, 1)},
+        "ms_host_api": t_host * 1e3,
+        "ms_gfm_program": None if t_prog is None else t_prog * 1e3,
+        "cpu_port": {"ms_producer": t_prod_cpu * 1e3,
+                     "ms_gravity_thread_per_row": t_grav_rows * 1e3, "rows": nj,
+                     "ms_gravity_cores": t_grav_cores * 1e3, "cores": cores,
+                     "kind": "port", "sample": "full call (producer + gravity), FP64 pow"},
+        "speedup_vs_cpu_port": (t_prod_cpu + t_grav_rows) / t_dev,
+        "rel_err_vs_cpu_path": err,
+        "host_api_equals_device": bool(np.array_equal(G_host, g_dev)),
+    }
+    if t_prog is not None:
+        out["gfm_program_equals_device"] = bool(np.array_equal(G_prog.astype(np.float64), g_dev))
     return out
 
 

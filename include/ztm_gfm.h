@@ -197,6 +197,29 @@ int ztm_plan_set_tuning(ztm_plan plan, int st_per_lane, int rows_per_slice,
 int ztm_plan_get_tuning(ztm_plan plan, int* st_per_lane, int* rows_per_slice,
                         int* cells_per_lane, int* adj_splits);
 
+/* ------------------------------------ gravity producer (TFM.cpp:83-94) */
+/* The density perturbation ZTM renders before every gravity call
+ * (TFM.cpp:83-89), on the device: fine raster ii x jj x kk (ii=(di-1)fh+1,
+ * jj=(dj-1)fh+1, kk=(dk-1)fv+1), coarse grids sea[di*dj] (sea-level k),
+ * moho[di*dj] (rendered Moho k, TFM.cpp:66 M), vel[di*dj*dk] (velocity,
+ * TFM.cpp:65 v), parameters vm, ro (already negated, TFM.cpp:84), dr, and
+ * rc = RC[0..6] of TFM.cpp:211 (lat_ref, lon_ref, d_ref, ENU offsets, h).
+ * Cells failing |r0-r1| > .001 or depth > 10 km are 0. drho_dev[N] is float,
+ * equal to (float) of the reference's double value. Device pointers. */
+int ztm_tfm_drho(int di, int dj, int dk, int fh, int fv, const double* rc, const double* sea_dev,
+                 const double* moho_dev, const double* vel_dev, double vm, double ro, double dr,
+                 float* drho_dev, void* stream);
+/* The whole gravity forward model of one FM call, TFM.cpp:83-94 (GPU
+ * branch) in process: producer and GF on the device, no files, no popen.
+ * Host arrays in; G[di*dj] out (gravity at interior stations, 0 outside, as
+ * the reference's G after :94); drho_out (optional, float[N]) receives the
+ * rendered field. surf[di*dj] = topo + sea (station k heights, TFM.cpp:83).
+ * Not re-entrant per geometry (cached device buffers). */
+int ztm_tfm_gravity_host(int bfr, int di, int dj, int dk, int fh, int fv, const double* rc,
+                         const double* sea, const double* moho, const double* vel,
+                         const double* surf, double vm, double ro, double dr, double* G,
+                         float* drho_out);
+
 /* ------------------------------------------------- uneven batches */
 /* A batch of independent point-mass problems (each its own grid, stations and
  * kappa; FP32 path with the FP64 near-field corrections). Every forward or

@@ -55,6 +55,14 @@ def lib():
         L.oracle_pm_adjoint_cells.argtypes = [gp, f64, i64, P, P, P, i64, P, i32]
         L.oracle_prism_forward.argtypes = [gp, f64, f64, P, i64, P, P, i32]
         L.oracle_prism_adjoint.argtypes = [gp, f64, i64, P, P, P, i32]
+        L.oracle_interp2.argtypes = [P, f64, f64, i32, i32]
+        L.oracle_interp2.restype = f64
+        L.oracle_interp3.argtypes = [P, f64, f64, f64, i32, i32, i32]
+        L.oracle_interp3.restype = f64
+        L.oracle_c2g_depth.argtypes = [P, f64, f64, f64]
+        L.oracle_c2g_depth.restype = f64
+        L.oracle_tfm_drho.argtypes = [i32, i32, i32, i32, i32, P, P, P, P, f64, f64, f64, P, P]
+        L.oracle_tfm_drho.restype = i64
         L.oracle_prism_kernel.argtypes = [f64] * 9
         L.oracle_prism_kernel.restype = f64
         _lib = L
@@ -90,6 +98,40 @@ def tfm_gravity(bfr, di, dj, dk, fh, fv, drho, surf, nn, G=None, threads=0) -> n
     lib().oracle_tfm_gravity(bfr, di, dj, dk, fh, fv, _ptr(drho), _ptr(surf), _ptr(nn),
                              nn.size, _ptr(out), threads)
     return out
+
+
+def interp2(f, x, y, di, dj) -> float:
+    """zmath.cpp:225-233 (unit spacing) on f[di*dj]."""
+    f = np.ascontiguousarray(f, dtype=np.float64)
+    return lib().oracle_interp2(_ptr(f), x, y, di, dj)
+
+
+def interp3(f, x, y, z, di, dj, dk) -> float:
+    """zmath.cpp:236-249 (unit spacing) on f[di*dj*dk]."""
+    f = np.ascontiguousarray(f, dtype=np.float64)
+    return lib().oracle_interp3(_ptr(f), x, y, z, di, dj, dk)
+
+
+def c2g_depth(rc, i, j, k) -> float:
+    """Depth (km) of grid point (i, j, k): TFM.cpp:56 + SET.cpp:86-99."""
+    rc = np.ascontiguousarray(rc, dtype=np.float64)
+    return lib().oracle_c2g_depth(_ptr(rc), i, j, k)
+
+
+def tfm_drho(di, dj, dk, fh, fv, rc, sea, moho, vel, vm, ro, dr):
+    """TFM.cpp:83-89: returns (drho[N] float64, nn int32 ascending)."""
+    rc = np.ascontiguousarray(rc, dtype=np.float64)
+    sea = np.ascontiguousarray(sea, dtype=np.float64)
+    moho = np.ascontiguousarray(moho, dtype=np.float64)
+    vel = np.ascontiguousarray(vel, dtype=np.float64)
+    if sea.size != di * dj or moho.size != di * dj or vel.size != di * dj * dk or rc.size < 7:
+        raise ValueError("sea/moho need di*dj values, vel di*dj*dk, rc 7+")
+    n = ((di - 1) * fh + 1) * ((dj - 1) * fh + 1) * ((dk - 1) * fv + 1)
+    drho = np.empty(n)
+    nn = np.empty(n, dtype=np.int32)
+    cnt = lib().oracle_tfm_drho(di, dj, dk, fh, fv, _ptr(rc), _ptr(sea), _ptr(moho), _ptr(vel),
+                                vm, ro, dr, _ptr(drho), _ptr(nn))
+    return drho, nn[:cnt].copy()
 
 
 def pm_forward(grid, kappa, rho, xyz, threshold=0.0, threads=DEFAULT_THREADS) -> np.ndarray:

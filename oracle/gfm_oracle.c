@@ -23,11 +23,27 @@
  *   oracle_pm_adjoint    its transpose, looped cells-outer (an independent
  *                        loop nest from the forward, so a transposed index
  *                        fails the dot-product test).
+ *   oracle_tfm_drho      TFM.cpp:83-89 -- the producer of the gravity path:
+ *                        the density perturbation on the fine raster from the
+ *                        surface (sea), Moho (M) and velocity (v) grids and
+ *                        (vm, ro, dr), with the |r0-r1|>.001 and depth>10 km
+ *                        cell filter. Its parts are separate functions:
+ *   oracle_interp2       zmath.cpp:225-233, bilinear interp, unit spacing;
+ *   oracle_interp3       zmath.cpp:236-249, trilinear interp, unit spacing;
+ *   oracle_c2g_depth     TFM.cpp:56 + SET.cpp:86-99, the depth (km) of the
+ *                        ENU grid point c, with the reference's sqrtf.
+ *                        Both FP64 expressions keep the reference's operation
+ *                        order, and the file is built without FMA contraction,
+ *                        so results are the reference's bit for bit.
  *   oracle_prism_*       right rectangular prism, exact volume integral of
  *                        (z_m - z)/r^3 (Plouff 1976; Nagy, Papp & Benedek
  *                        2000, eq. for g_z), no reference counterpart.
  *
- * Deliberate deviation: a station exactly at a cell centre (r = 0) gives
+ * Deliberate deviations: (1) oracle_interp2 reads a neighbour outside the
+ * grid (i = di-1 or j = dj-1, where its weight is exactly 0) as 0; the
+ * reference reads past the row or the array there (zmath.cpp:229-230) and
+ * multiplies by 0, which equals ours whenever that value is finite.
+ * (2) a station exactly at a cell centre (r = 0) gives
  * 0*inf = NaN in the reference (GFM_GPU.cu:22, TFM.cpp:98); here that pair
  * contributes 0, as in the HIP kernels.
  *
@@ -168,6 +184,85 @@ int oracle_tfm_gravity(int bfr, int di, int dj, int dk, int fh, int fv, const do
         parallel_for(rows, nthreads, tfm_range, &c);
     }
     return rows;
+}
+
+/* -------------------------------------------- oracle_tfm_drho (producer) */
+/* zmath.cpp:225-233 with hi = 1 (hj = hi), x0 = y0 = 0. */
+double oracle_interp2(const double* f, double x, double y, int di, int dj) {
+    const double ii = fmin(fmax(x / 1.0, 0), (di - 1)), jj = fmin(fmax(y / 1.0, 0), (dj - 1));
+    const int i = (int)ii, j = (int)jj;
+    const double si = ii - i, sj = jj - j;
+    const double c00 = f[i + (j + 0) * di];
+    const double c10 = i + 1 < di ? f[i + 1 + (j + 0) * di] : 0.0;
+    const double c01 = j + 1 < dj ? f[i + (j + 1) * di] : 0.0;
+    const double c11 = (i + 1 < di && j + 1 < dj) ? f[i + 1 + (j + 1) * di] : 0.0;
+    const double c0 = c00 * (1 - si) + c10 * si, c1 = c01 * (1 - si) + c11 * si;
+    return c0 * (1 - sj) + c1 * sj;
+}
+
+/* zmath.cpp:236-249 with hi = 1 (hj = hk = hi), zero offsets. */
+double oracle_interp3(const double* f, double x, double y, double z, int di, int dj, int dk) {
+    const double ii = fmin(fmax(x / 1.0, 0), (di - 2)), jj = fmin(fmax(y / 1.0, 0), (dj - 2)),
+                 kk = fmin(fmax(z / 1.0, 0), (dk - 2));
+    const int i = (int)ii, j = (int)jj, k = (int)kk;
+    const double si = ii - i, sj = jj - j, sk = kk - k;
+    const int64_t p = (int64_t)di * dj;
+    const double c000 = f[i + (j + 0) * di + (k + 0) * p], c100 = f[i + 1 + (j + 0) * di + (k + 0) * p];
+    const double c010 = f[i + (j + 1) * di + (k + 0) * p], c110 = f[i + 1 + (j + 1) * di + (k + 0) * p];
+    const double c001 = f[i + (j + 0) * di + (k + 1) * p], c101 = f[i + 1 + (j + 0) * di + (k + 1) * p];
+    const double c011 = f[i + (j + 1) * di + (k + 1) * p], c111 = f[i + 1 + (j + 1) * di + (k + 1) * p];
+    const double c00 = c000 * (1 - si) + c100 * si, c01 = c001 * (1 - si) + c101 * si;
+    const double c10 = c010 * (1 - si) + c110 * si, c11 = c011 * (1 - si) + c111 * si;
+    const double c0 = c00 * (1 - sj) + c10 * sj, c1 = c01 * (1 - sj) + c11 * sj;
+    return c0 * (1 - sk) + c1 * sk;
+}
+
+/* TFM.cpp:56 c2g(c) = enu2geo(RC[0..2], c*RC[6] - RC[3..5]), SET.cpp:86-99,
+ * depth component only: R - sqrtf(|ECEF point|^2). rc = RC[0..6]. */
+double oracle_c2g_depth(const double* rc, double ci, double cj, double ck) {
+    const double rad = 57.295, R = 6371;
+    const double lat_ref = rc[0], lon_ref = rc[1], d_ref = rc[2];
+    const double x = ci * rc[6] - rc[3], y = cj * rc[6] - rc[4], z = ck * rc[6] - rc[5];
+    const double xr = (R - d_ref) * cos(lon_ref / rad) * cos(lat_ref / rad);
+    const double yr = (R - d_ref) * sin(lon_ref / rad) * cos(lat_ref / rad);
+    const double zr = (R - d_ref) * sin(lat_ref / rad);
+    const double phi = lat_ref / rad, theta = lon_ref / rad;
+    const double m[9] = {-sin(theta), -sin(phi) * cos(theta), cos(phi) * cos(theta),
+                         cos(theta),  -sin(phi) * sin(theta), cos(phi) * sin(theta),
+                         0,           cos(phi),               sin(phi)};
+    const double xi = m[0] * x + m[1] * y + m[2] * z + xr;
+    const double yi = m[3] * x + m[4] * y + m[5] * z + yr;
+    const double zi = m[6] * x + m[7] * y + m[8] * z + zr;
+    const double r = sqrtf(xi * xi + yi * yi + zi * zi);
+    return R - r;
+}
+
+/* TFM.cpp:83-89: drho[N] (zero outside the kept set) and the kept-cell list
+ * nn (ascending n); returns the number of kept cells. nn may be NULL. */
+int64_t oracle_tfm_drho(int di, int dj, int dk, int fh, int fv, const double* rc,
+                        const double* sea, const double* M, const double* v, double vm,
+                        double ro, double dr, double* drho, int32_t* nn) {
+    const int ii = (di - 1) * fh + 1, jj = (dj - 1) * fh + 1, kk = (dk - 1) * fv + 1;
+    const int64_t N = (int64_t)ii * jj * kk;
+    int64_t count = 0;
+    for (int64_t n = 0; n < N; ++n) {
+        const double i = (n % ii) / (1. * fh), j = ((n / ii) % jj) / (1. * fh),
+                     k = (n / ((int64_t)ii * jj)) / (1. * fv), li = 1. / fv;
+        double m0, m1, r0, r1;
+        m0 = oracle_interp2(sea, i, j, di, dj) - 3.5;
+        m1 = oracle_interp2(M, i, j, di, dj);
+        r0 = 0;
+        r1 = dr * (oracle_interp3(v, i, j, k, di, dj, dk) - vm);
+        if (k > m0 + li) { r0 = ro; } else if (fabs(k - m0) < li) { r0 = r0 + (k - m0 + li) * (r0 + ro) / (2 * li); }
+        if (k > m1 + li) { r1 = ro; } else if (fabs(k - m1) < li) { r1 = r1 + (k - m1 + li) * (r1 + ro) / (2 * li); }
+        drho[n] = 0;
+        if (fabs(r0 - r1) > .001 && oracle_c2g_depth(rc, i, j, k) > 10) {
+            drho[n] = r1 - r0;
+            if (nn) nn[count] = (int32_t)n;
+            ++count;
+        }
+    }
+    return count;
 }
 
 /* ------------------------------------------------- general point mass */

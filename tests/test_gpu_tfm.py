@@ -1,0 +1,74 @@
+"""The gravity FM of one ZTM forward-model call on the device (TFM.cpp:83-94):
+the rendered density perturbation equals the FP64 restatement's value
+narrowed to float bit for bit, and the in-process gravity equals the FP64
+oracle of GF on that field (1e-6 of max |g|, as the reference-ABI tests) and
+the reference's CPU path on the double field (1e-5)."""
+import numpy as np
+import pytest
+
+from ztm_amd import gfm, workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def relerr(a, b):
+    return float(np.max(np.abs(np.asarray(a, np.float64) - b)) / np.max(np.abs(b)))
+
+
+def device_drho(w, dev):
+    import torch
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a, np.float64)).to(dev)
+    return gfm.tfm_drho(w.di, w.dj, w.dk, w.fh, w.fv, w.rc, t(w.sea), t(w.moho), t(w.vel), w.vm,
+                        w.ro, w.dr).cpu().numpy()
+
+
+@pytest.mark.parametrize("geom", [dict(), dict(fh=2, fv=1, di=23, dj=19, dk=14),
+                                  dict(fh=1, fv=3, di=31, dj=27, dk=17)])
+def test_drho_bitwise_equals_restatement(dev, orc, geom):
+    w = W.tfm_fm(**geom)
+    want, nn = orc.tfm_drho(w.di, w.dj, w.dk, w.fh, w.fv, w.rc, w.sea, w.moho, w.vel, w.vm,
+                            w.ro, w.dr)
+    got = device_drho(w, dev)
+    assert got.dtype == np.float32 and got.size == want.size
+    assert np.array_equal(got, want.astype(np.float32))
+    assert nn.size > 1000
+
+
+def test_gravity_fm_host_matches_oracles(dev, orc):
+    w = W.tfm_fm()
+    G, drho = gfm.tfm_gravity_host(w.bfr, w.di, w.dj, w.dk, w.fh, w.fv, w.rc, w.sea, w.moho,
+                                   w.vel, w.surf, w.vm, w.ro, w.dr, want_drho=True)
+    want_d, nn = orc.tfm_drho(w.di, w.dj, w.dk, w.fh, w.fv, w.rc, w.sea, w.moho, w.vel, w.vm,
+                              w.ro, w.dr)
+    assert np.array_equal(drho, want_d.astype(np.float32))
+    # GF on the float field (the reference's GPU branch, TFM.cpp:90-94)
+    g_ref = orc.gf(w.bfr, w.di, w.dj, w.dk, w.fh, w.fv, drho, w.surf.astype(np.float32))
+    assert relerr(G, g_ref) < 1e-6
+    # the CPU branch (TFM.cpp:95-99) on the double field and its kept list
+    g_cpu = orc.tfm_gravity(w.bfr, w.di, w.dj, w.dk, w.fh, w.fv, want_d, w.surf, nn, threads=16)
+    interior = np.zeros((w.dj, w.di), bool)
+    interior[w.bfr:w.dj - w.bfr, w.bfr:w.di - w.bfr] = True
+    interior = interior.ravel()
+    assert relerr(G[interior], g_cpu[interior]) < 1e-5
+    assert np.all(G[~interior] == 0)
+    assert np.abs(G).max() > 10.0     # mGal-scale signal of a crustal root
+
+
+def test_gravity_fm_repeated_calls_and_errors(dev, orc):
+    a = W.tfm_fm(seed=W.SEED0 + 16)
+    b = W.tfm_fm(seed=W.SEED0 + 26)
+    args = lambda w: (w.bfr, w.di, w.dj, w.dk, w.fh, w.fv, w.rc, w.sea, w.moho, w.vel, w.surf,
+                      w.vm, w.ro, w.dr)
+    Ga1 = gfm.tfm_gravity_host(*args(a))
+    Gb = gfm.tfm_gravity_host(*args(b))
+    Ga2 = gfm.tfm_gravity_host(*args(a))
+    assert np.array_equal(Ga1, Ga2) and not np.array_equal(Ga1, Gb)
+    w = a
+    with pytest.raises(gfm.ZtmError):
+        gfm.tfm_gravity_host(w.bfr, 1, w.dj, w.dk, w.fh, w.fv, w.rc, w.sea[:w.dj],
+                             w.moho[:w.dj], w.vel[:w.dj * w.dk], w.surf[:w.dj], w.vm, w.ro, w.dr)
+    bad_rc = w.rc.copy()
+    bad_rc[6] = np.nan
+    with pytest.raises(gfm.ZtmError):
+        gfm.tfm_gravity_host(w.bfr, w.di, w.dj, w.dk, w.fh, w.fv, bad_rc, w.sea, w.moho, w.vel,
+                             w.surf, w.vm, w.ro, w.dr)

@@ -655,6 +655,56 @@ int ref_plan(int bfr, int di, int dj, int dk, int fh, int fv, ztm_plan* out) {
     *out = p;
     return ZTM_OK;
 }
+
+// Producer arguments: the interpolation stencils need at least 2 points per
+// axis; RC[0..6] must be finite.
+int tfm_check(int di, int dj, int dk, int fh, int fv, const double* rc) {
+    if (di < 2 || dj < 2 || dk < 2 || fh <= 0 || fv <= 0)
+        return fail(ZTM_EINVAL, "producer grid needs di, dj, dk >= 2 and fh, fv >= 1 "
+                                "(di=%d dj=%d dk=%d fh=%d fv=%d)", di, dj, dk, fh, fv);
+    if (int e = ref_check(0, di, dj, dk, fh, fv)) return e;
+    if (!rc) return fail(ZTM_EINVAL, "null RC frame");
+    for (int q = 0; q < 7; ++q)
+        if (!std::isfinite(rc[q])) return fail(ZTM_EINVAL, "RC[%d] is not finite", q);
+    return ZTM_OK;
+}
+
+// Device buffers of the in-process gravity FM, one set per (device,
+// geometry), kept for the process like the reference-ABI plans.
+struct TfmBuffers {
+    double *sea = nullptr, *moho = nullptr, *vel = nullptr;
+    float *s = nullptr, *drho = nullptr, *g = nullptr;
+    int64_t n = 0;
+};
+std::map<RefKey, TfmBuffers> g_tfm_bufs;
+
+int tfm_buffers(int dev, int di, int dj, int dk, int fh, int fv, TfmBuffers* out) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    const RefKey key{dev, 0, di, dj, dk, fh, fv};
+    auto it = g_tfm_bufs.find(key);
+    if (it != g_tfm_bufs.end()) {
+        *out = it->second;
+        return ZTM_OK;
+    }
+    TfmBuffers b;
+    const size_t M = (size_t)di * dj;
+    b.n = ((int64_t)(di - 1) * fh + 1) * ((int64_t)(dj - 1) * fh + 1) * ((int64_t)(dk - 1) * fv + 1);
+    hipError_t e = hipMalloc(&b.sea, sizeof(double) * M);
+    if (e == hipSuccess) e = hipMalloc(&b.moho, sizeof(double) * M);
+    if (e == hipSuccess) e = hipMalloc(&b.vel, sizeof(double) * M * dk);
+    if (e == hipSuccess) e = hipMalloc(&b.s, sizeof(float) * M);
+    if (e == hipSuccess) e = hipMalloc(&b.g, sizeof(float) * M);
+    if (e == hipSuccess) e = hipMalloc(&b.drho, sizeof(float) * b.n);
+    if (e != hipSuccess) {
+        for (void* q : {(void*)b.sea, (void*)b.moho, (void*)b.vel, (void*)b.s, (void*)b.g,
+                        (void*)b.drho})
+            if (q) (void)hipFree(q);
+        return hip_fail(e, "gravity FM buffers");
+    }
+    g_tfm_bufs.emplace(key, b);
+    *out = b;
+    return ZTM_OK;
+}
 }  // namespace
 
 int ztm_gf_async(int bfr, int di, int dj, int dk, int fh, int fv, float* g_dev,
@@ -707,6 +757,48 @@ int ztm_gfm_host(int bfr, int di, int dj, int dk, int fh, int fv, float* g_host,
     if (g) (void)hipFree(g);
     if (s) (void)hipFree(s);
     return rc;
+}
+
+// ------------------------------------- gravity producer (TFM.cpp:83-94)
+int ztm_tfm_drho(int di, int dj, int dk, int fh, int fv, const double* rc, const double* sea_dev,
+                 const double* moho_dev, const double* vel_dev, double vm, double ro, double dr,
+                 float* drho_dev, void* stream) {
+    if (int e = tfm_check(di, dj, dk, fh, fv, rc)) return e;
+    if (!sea_dev || !moho_dev || !vel_dev || !drho_dev) return fail(ZTM_EINVAL, "null argument");
+    if (int e = check_device(nullptr)) return e;
+    return tfm_drho_launch(di, dj, dk, fh, fv, rc, sea_dev, moho_dev, vel_dev, vm, ro, dr,
+                           drho_dev, (hipStream_t)stream);
+}
+
+int ztm_tfm_gravity_host(int bfr, int di, int dj, int dk, int fh, int fv, const double* rc,
+                         const double* sea, const double* moho, const double* vel,
+                         const double* surf, double vm, double ro, double dr, double* G,
+                         float* drho_out) {
+    if (int e = tfm_check(di, dj, dk, fh, fv, rc)) return e;
+    if (int e = ref_check(bfr, di, dj, dk, fh, fv)) return e;
+    if (!sea || !moho || !vel || !surf || !G) return fail(ZTM_EINVAL, "null host buffer");
+    int dev = 0;
+    if (int e = check_device(&dev)) return e;
+    TfmBuffers b;
+    if (int e = tfm_buffers(dev, di, dj, dk, fh, fv, &b)) return e;
+    const size_t M = (size_t)di * dj, NV = M * dk;
+    std::vector<float> s32(surf, surf + M), g32(M);   // S.dat narrowing, TFM.cpp:92
+    hipError_t e = hipMemcpy(b.sea, sea, sizeof(double) * M, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(b.moho, moho, sizeof(double) * M, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(b.vel, vel, sizeof(double) * NV, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(b.s, s32.data(), sizeof(float) * M, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemset(b.g, 0, sizeof(float) * M);
+    if (e != hipSuccess) return hip_fail(e, "gravity FM inputs");
+    if (int rc2 = tfm_drho_launch(di, dj, dk, fh, fv, rc, b.sea, b.moho, b.vel, vm, ro, dr,
+                                  b.drho, nullptr))
+        return rc2;
+    if (int rc2 = ztm_gf_async(bfr, di, dj, dk, fh, fv, b.g, b.drho, b.s, nullptr)) return rc2;
+    e = hipMemcpy(g32.data(), b.g, sizeof(float) * M, hipMemcpyDeviceToHost);
+    if (e == hipSuccess && drho_out)
+        e = hipMemcpy(drho_out, b.drho, sizeof(float) * b.n, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return hip_fail(e, "gravity FM result copy");
+    for (size_t m = 0; m < M; ++m) G[m] = g32[m];   // TFM.cpp:94
+    return ZTM_OK;
 }
 
 // ----------------------------------------------------------- profiling

@@ -177,6 +177,20 @@ int pm_fwd_grouped_launch(const GroupedPass& gp, const float* rho, double* g, fl
 int pm_adj_grouped_launch(const GroupedPass& gp, const float* r, float* out,
                           hipStream_t stream);
 
+// Producer of the gravity path (gfm_tfm.hip): TFM.cpp:83-89 on the device.
+// Frame of the reference's ENU grid: ENU -> ECEF rotation m (row-major),
+// ECEF reference point, ENU offsets RC[3..5] and cell edge RC[6] (km).
+struct TfmFrame {
+    double m[9];
+    double ref[3];
+    double off[3];
+    double h;
+};
+TfmFrame tfm_frame(const double* rc);
+int tfm_drho_launch(int di, int dj, int dk, int fh, int fv, const double* rc, const double* sea,
+                    const double* moho, const double* vel, double vm, double ro, double dr,
+                    float* drho, hipStream_t stream);
+
 // Prism formula (gfm_prism.hip).
 // FP32: per-station mid-zone correction (prism integral minus the far-field
 // kernel) for cells within kMid of each station -> corr[m]; the forward then

@@ -1,0 +1,138 @@
+// Producer of the gravity path on the device: the density perturbation drho
+// that ZTM's forward model renders on the fine raster before every gravity
+// call (TFM.cpp:83-89), so the rendered field never leaves the GPU.
+//
+// Per fine cell n (ii x jj x kk raster, coarse-index coordinates i, j, k):
+//   m0 = interp2(sea, i, j) - 3.5           reference Moho (k units)
+//   m1 = interp2(M, i, j)                   rendered Moho
+//   r0 = 0 below m0, ro above, linear band of half-width 1/fv between
+//   r1 = dr (interp3(v, i, j, k) - vm) below m1, ro above, band as written
+//        in TFM.cpp:88 (r1 + t (r1 + ro), t in [0, 1))
+//   drho = r1 - r0 if |r0 - r1| > .001 and depth(i, j, k) > 10 km, else 0
+// with interp2/interp3 the unit-spacing bilinear/trilinear interpolation of
+// zmath.cpp:225-249 and depth = R - sqrtf(|ECEF|^2) of the ENU point
+// (TFM.cpp:56, SET.cpp:86-99). All FP64 in the reference's operation order,
+// FMA contraction off, so drho equals the reference's value bit for bit; it
+// is written as float, the narrowing the reference applies before its GPU
+// call (TFM.cpp:91). One thread per fine cell; the coarse grids are read
+// through the cache (the kernel is HBM-bound on its 4-byte output).
+#include "gfm_internal.h"
+
+#include <cmath>
+
+namespace ztm {
+
+__device__ __forceinline__ double interp2_unit(const double* __restrict__ f, double x, double y,
+                                               int di, int dj) {
+#pragma clang fp contract(off)
+    const double ii = fmin(fmax(x / 1.0, 0.0), (double)(di - 1));
+    const double jj = fmin(fmax(y / 1.0, 0.0), (double)(dj - 1));
+    const int i = (int)ii, j = (int)jj;
+    const double si = ii - i, sj = jj - j;
+    // a neighbour outside the grid has weight exactly 0: read as 0
+    const double c00 = f[i + (j + 0) * di];
+    const double c10 = i + 1 < di ? f[i + 1 + (j + 0) * di] : 0.0;
+    const double c01 = j + 1 < dj ? f[i + (j + 1) * di] : 0.0;
+    const double c11 = (i + 1 < di && j + 1 < dj) ? f[i + 1 + (j + 1) * di] : 0.0;
+    const double c0 = c00 * (1 - si) + c10 * si, c1 = c01 * (1 - si) + c11 * si;
+    return c0 * (1 - sj) + c1 * sj;
+}
+
+__device__ __forceinline__ double interp3_unit(const double* __restrict__ f, double x, double y,
+                                               double z, int di, int dj, int dk) {
+#pragma clang fp contract(off)
+    const double ii = fmin(fmax(x / 1.0, 0.0), (double)(di - 2));
+    const double jj = fmin(fmax(y / 1.0, 0.0), (double)(dj - 2));
+    const double kk = fmin(fmax(z / 1.0, 0.0), (double)(dk - 2));
+    const int i = (int)ii, j = (int)jj, k = (int)kk;
+    const double si = ii - i, sj = jj - j, sk = kk - k;
+    const int64_t p = (int64_t)di * dj;
+    const int64_t b = i + (int64_t)j * di + k * p;
+    const double c000 = f[b], c100 = f[b + 1];
+    const double c010 = f[b + di], c110 = f[b + di + 1];
+    const double c001 = f[b + p], c101 = f[b + p + 1];
+    const double c011 = f[b + p + di], c111 = f[b + p + di + 1];
+    const double c00 = c000 * (1 - si) + c100 * si, c01 = c001 * (1 - si) + c101 * si;
+    const double c10 = c010 * (1 - si) + c110 * si, c11 = c011 * (1 - si) + c111 * si;
+    const double c0 = c00 * (1 - sj) + c10 * sj, c1 = c01 * (1 - sj) + c11 * sj;
+    return c0 * (1 - sk) + c1 * sk;
+}
+
+__device__ __forceinline__ double depth_km(const TfmFrame& fr, double ci, double cj, double ck) {
+#pragma clang fp contract(off)
+    const double x = ci * fr.h - fr.off[0], y = cj * fr.h - fr.off[1], z = ck * fr.h - fr.off[2];
+    const double xi = fr.m[0] * x + fr.m[1] * y + fr.m[2] * z + fr.ref[0];
+    const double yi = fr.m[3] * x + fr.m[4] * y + fr.m[5] * z + fr.ref[1];
+    const double zi = fr.m[6] * x + fr.m[7] * y + fr.m[8] * z + fr.ref[2];
+    const double r = sqrtf((float)(xi * xi + yi * yi + zi * zi));
+    return 6371.0 - r;
+}
+
+__global__ __launch_bounds__(256) void tfm_drho_kernel(int di, int dj, int dk, int fh, int fv,
+                                                       TfmFrame fr,
+                                                       const double* __restrict__ sea,
+                                                       const double* __restrict__ moho,
+                                                       const double* __restrict__ vel, double vm,
+                                                       double ro, double dr,
+                                                       float* __restrict__ drho, int64_t N) {
+#pragma clang fp contract(off)
+    const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= N) return;
+    const int ii = (di - 1) * fh + 1, jj = (dj - 1) * fh + 1;
+    const double i = (double)(n % ii) / (1. * fh), j = (double)((n / ii) % jj) / (1. * fh),
+                 k = (double)(n / ((int64_t)ii * jj)) / (1. * fv), li = 1. / fv;
+    const double m0 = interp2_unit(sea, i, j, di, dj) - 3.5;
+    const double m1 = interp2_unit(moho, i, j, di, dj);
+    double r0 = 0;
+    double r1 = dr * (interp3_unit(vel, i, j, k, di, dj, dk) - vm);
+    if (k > m0 + li) {
+        r0 = ro;
+    } else if (fabs(k - m0) < li) {
+        r0 = r0 + (k - m0 + li) * (r0 + ro) / (2 * li);
+    }
+    if (k > m1 + li) {
+        r1 = ro;
+    } else if (fabs(k - m1) < li) {
+        r1 = r1 + (k - m1 + li) * (r1 + ro) / (2 * li);
+    }
+    float out = 0.0f;
+    if (fabs(r0 - r1) > .001 && depth_km(fr, i, j, k) > 10) out = (float)(r1 - r0);
+    drho[n] = out;
+}
+
+TfmFrame tfm_frame(const double* rc) {
+    // SET.cpp:86-96: ENU -> ECEF rotation and the reference point, from RC
+    // (TFM.cpp:211): lat_ref, lon_ref, d_ref, ENU offsets, cell edge h.
+    const double rad = 57.295, R = 6371;
+    const double lat_ref = rc[0], lon_ref = rc[1], d_ref = rc[2];
+    TfmFrame f;
+    f.ref[0] = (R - d_ref) * std::cos(lon_ref / rad) * std::cos(lat_ref / rad);
+    f.ref[1] = (R - d_ref) * std::sin(lon_ref / rad) * std::cos(lat_ref / rad);
+    f.ref[2] = (R - d_ref) * std::sin(lat_ref / rad);
+    const double phi = lat_ref / rad, theta = lon_ref / rad;
+    const double m[9] = {-std::sin(theta), -std::sin(phi) * std::cos(theta), std::cos(phi) * std::cos(theta),
+                         std::cos(theta),  -std::sin(phi) * std::sin(theta), std::cos(phi) * std::sin(theta),
+                         0,                std::cos(phi),                    std::sin(phi)};
+    for (int q = 0; q < 9; ++q) f.m[q] = m[q];
+    f.off[0] = rc[3];
+    f.off[1] = rc[4];
+    f.off[2] = rc[5];
+    f.h = rc[6];
+    return f;
+}
+
+int tfm_drho_launch(int di, int dj, int dk, int fh, int fv, const double* rc, const double* sea,
+                    const double* moho, const double* vel, double vm, double ro, double dr,
+                    float* drho, hipStream_t stream) {
+    const int64_t N = ((int64_t)(di - 1) * fh + 1) * ((int64_t)(dj - 1) * fh + 1) *
+                      ((int64_t)(dk - 1) * fv + 1);
+    if (N == 0) return ZTM_OK;
+    const TfmFrame fr = tfm_frame(rc);
+    ProfScope prof("tfm_drho", stream);
+    hipLaunchKernelGGL(tfm_drho_kernel, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, stream,
+                       di, dj, dk, fh, fv, fr, sea, moho, vel, vm, ro, dr, drho, N);
+    ZTM_HIP(hipGetLastError());
+    return ZTM_OK;
+}
+
+}  // namespace ztm

@@ -76,6 +76,9 @@ SIGNATURES = {
     "ztm_plan_adjoint_stored": (_I, [_P, _P, _P, _P]),
     "ztm_plan_set_tuning": (_I, [_P, _I, _I, _I, _I]),
     "ztm_plan_get_tuning": (_I, [_P, _IP, _IP, _IP, _IP]),
+    "ztm_tfm_drho": (_I, [_I, _I, _I, _I, _I, _P, _P, _P, _P, _D, _D, _D, _P, _P]),
+    "ztm_tfm_gravity_host": (_I, [_I, _I, _I, _I, _I, _I, _P, _P, _P, _P, _P, _D, _D, _D, _P,
+                                  _P]),
     "ztm_batch_create": (_I, [ctypes.POINTER(_P), _I, ctypes.POINTER(CProblem),
                               ctypes.POINTER(_I64), ctypes.POINTER(_P)]),
     "ztm_batch_destroy": (_I, [_P]),
@@ -151,6 +154,55 @@ def gfm_host(bfr, di, dj, dk, fh, fv, rho, s) -> np.ndarray:
     check(lib().ztm_gfm_host(bfr, di, dj, dk, fh, fv, g.ctypes.data, rho.ctypes.data, rho.size,
                              s.ctypes.data))
     return g
+
+
+# ------------------------------------------ gravity producer (TFM.cpp:83-94)
+def _host_f64(a, n: int, name: str) -> np.ndarray:
+    a = np.ascontiguousarray(a, dtype=np.float64).ravel()
+    if a.size != n:
+        raise ValueError(f"{name} needs {n} values, got {a.size}")
+    return a
+
+
+def tfm_drho(di, dj, dk, fh, fv, rc, sea, moho, vel, vm, ro, dr, out=None, stream=None):
+    """TFM.cpp:83-89 on the device: sea/moho float64 CUDA tensors [di*dj], vel
+    [di*dj*dk]; returns drho float32 [ii*jj*kk] (the reference's value, narrowed
+    as its GPU path does)."""
+    import torch
+    rc = _host_f64(rc, 7, "rc") if np.size(rc) == 7 else _host_f64(rc, 10, "rc")
+    n = ((di - 1) * fh + 1) * ((dj - 1) * fh + 1) * ((dk - 1) * fv + 1)
+    for name, t, size in (("sea", sea, di * dj), ("moho", moho, di * dj),
+                          ("vel", vel, di * dj * dk)):
+        _check_t(t, torch.float64, size, name)
+    if out is None:
+        out = torch.empty(n, dtype=torch.float32, device=sea.device)
+    _check_t(out, torch.float32, n, "drho")
+    check(lib().ztm_tfm_drho(di, dj, dk, fh, fv, rc.ctypes.data, _ptr(sea), _ptr(moho),
+                             _ptr(vel), vm, ro, dr, _ptr(out), _stream(stream)))
+    return out
+
+
+def tfm_gravity_host(bfr, di, dj, dk, fh, fv, rc, sea, moho, vel, surf, vm, ro, dr,
+                     want_drho=False):
+    """One gravity forward model of TFM.cpp:83-94 in process on host arrays:
+    returns G[di*dj] (float64) and, if asked, drho (float32)."""
+    rc = np.ascontiguousarray(rc, dtype=np.float64).ravel()
+    if rc.size < 7:
+        raise ValueError("rc needs RC[0..6]")
+    sea = _host_f64(sea, di * dj, "sea")
+    moho = _host_f64(moho, di * dj, "moho")
+    vel = _host_f64(vel, di * dj * dk, "vel")
+    surf = _host_f64(surf, di * dj, "surf")
+    G = np.empty(di * dj)
+    drho = None
+    if want_drho:
+        drho = np.empty(((di - 1) * fh + 1) * ((dj - 1) * fh + 1) * ((dk - 1) * fv + 1),
+                        dtype=np.float32)
+    check(lib().ztm_tfm_gravity_host(bfr, di, dj, dk, fh, fv, rc.ctypes.data, sea.ctypes.data,
+                                     moho.ctypes.data, vel.ctypes.data, surf.ctypes.data, vm, ro,
+                                     dr, G.ctypes.data,
+                                     drho.ctypes.data if drho is not None else None))
+    return (G, drho) if want_drho else G
 
 
 # ------------------------------------------------------------------ plans

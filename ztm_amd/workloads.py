@@ -3,7 +3,7 @@
 Every random draw comes from a counter-based SplitMix64 stream, so any value
 can be regenerated from (seed, stream, index) alone and the host and any test
 see identical inputs. Seeds are 20190301 + workload id (pm_cpu_small 1,
-pm_blobs_1m 2, adjoint_small 3, ensemble_256 4, uneven_48 5).
+pm_blobs_1m 2, adjoint_small 3, ensemble_256 4, uneven_48 5, tfm_fm 6).
 
 Coordinates are metres with z up; cells of edge h fill x in [0, nx h],
 y in [0, ny h], z in [-nz h, 0] (top at z = 0); density contrast in kg/m^3;
@@ -17,6 +17,11 @@ the gravity output is in mGal (kappa = 1e5 * G).
     adjoint_small  pm_cpu_small geometry, residual N(0, 0.5) mGal, lambda 1e-3
     ensemble_256   96x96x48 cells of 1 km, 256 models of U[-300,300]
                    smoothed by a clamped 3x3x3 box filter; 9,216 stations, z=+10
+    tfm_fm         the reference's own gravity FM call (TFM.cpp:83-94) at its
+                   grid (61 x 61 x 33 coarse, h = 10 km, fh = 1, fv = 2, bfr 10,
+                   reference point (-15.25, -72.5)): sea-level k from the ENU
+                   frame, seeded topography, Airy Moho + perturbation,
+                   perturbed mantle velocity; drho rendered from them
     uneven_48      48 independent problems: cells log-U[2e4, 2e6], stations
                    log-U[1e3, 4e4] scattered over the footprint at z=+10 m,
                    cell edge log-U[250, 2000] m; 3 buried +250 kg/m^3 blocks
@@ -259,10 +264,109 @@ def uneven_48(seed: int = SEED0 + 5, n_problems: int = 48, cells=(2e4, 2e6),
     return out
 
 
+_RAD, _R_EARTH = 57.295, 6371.0     # SET.cpp:67,87
+
+
+def _geo2enu(ref, lat, lon, d):
+    """SET.cpp:66-82 (vectorised): geodetic (deg, deg, km depth) -> ENU km."""
+    lat_r, lon_r, d_r = ref
+    xi = (_R_EARTH - d) * np.cos(lon / _RAD) * np.cos(lat / _RAD)
+    yi = (_R_EARTH - d) * np.sin(lon / _RAD) * np.cos(lat / _RAD)
+    zi = (_R_EARTH - d) * np.sin(lat / _RAD)
+    xr = (_R_EARTH - d_r) * np.cos(lon_r / _RAD) * np.cos(lat_r / _RAD)
+    yr = (_R_EARTH - d_r) * np.sin(lon_r / _RAD) * np.cos(lat_r / _RAD)
+    zr = (_R_EARTH - d_r) * np.sin(lat_r / _RAD)
+    ph, th = lat_r / _RAD, lon_r / _RAD
+    v = (xi - xr, yi - yr, zi - zr)
+    x = -np.sin(th) * v[0] + np.cos(th) * v[1]
+    y = -np.sin(ph) * np.cos(th) * v[0] - np.sin(ph) * np.sin(th) * v[1] + np.cos(ph) * v[2]
+    z = np.cos(ph) * np.cos(th) * v[0] + np.cos(ph) * np.sin(th) * v[1] + np.sin(ph) * v[2]
+    return x, y, z
+
+
+def _enu2geo(ref, x, y, z):
+    """SET.cpp:86-99 (vectorised, FP64 sqrt): ENU km -> geodetic."""
+    lat_r, lon_r, d_r = ref
+    xr = (_R_EARTH - d_r) * np.cos(lon_r / _RAD) * np.cos(lat_r / _RAD)
+    yr = (_R_EARTH - d_r) * np.sin(lon_r / _RAD) * np.cos(lat_r / _RAD)
+    zr = (_R_EARTH - d_r) * np.sin(lat_r / _RAD)
+    ph, th = lat_r / _RAD, lon_r / _RAD
+    xi = -np.sin(th) * x - np.sin(ph) * np.cos(th) * y + np.cos(ph) * np.cos(th) * z + xr
+    yi = np.cos(th) * x - np.sin(ph) * np.sin(th) * y + np.cos(ph) * np.sin(th) * z + yr
+    zi = np.cos(ph) * y + np.sin(ph) * z + zr
+    r = np.sqrt(xi * xi + yi * yi + zi * zi)
+    return _RAD * np.arcsin(zi / r), _RAD * np.arctan2(yi, xi), _R_EARTH - r
+
+
+@dataclass
+class TfmInputs:
+    di: int
+    dj: int
+    dk: int
+    fh: int
+    fv: int
+    bfr: int
+    rc: np.ndarray        # RC[10], TFM.cpp:211
+    sea: np.ndarray       # [di*dj] sea-level k
+    topo: np.ndarray      # [di*dj] k units
+    moho: np.ndarray      # [di*dj] rendered Moho k
+    vel: np.ndarray       # [di*dj*dk] velocity, km/s
+    vm: float
+    ro: float
+    dr: float
+
+    @property
+    def surf(self) -> np.ndarray:
+        return self.topo + self.sea
+
+    @property
+    def n_fine(self) -> int:
+        return (((self.di - 1) * self.fh + 1) * ((self.dj - 1) * self.fh + 1)
+                * ((self.dk - 1) * self.fv + 1))
+
+
+def _smooth2(seed, stream, di, dj, n_bumps, width):
+    u = uniform(seed, stream, n_bumps * 4).reshape(n_bumps, 4)
+    j, i = np.meshgrid(np.arange(dj), np.arange(di), indexing="ij")
+    f = np.zeros((dj, di))
+    for a, s_, cx, cy in u:
+        w = width * (0.5 + s_)
+        f += (2 * a - 1) * np.exp(-((i - cx * di) ** 2 + (j - cy * dj) ** 2) / (2 * w * w))
+    return f.ravel()
+
+
+def tfm_fm(seed: int = SEED0 + 6, di: int = 61, dj: int = 61, dk: int = 33, h: float = 10.0,
+           fh: int = 1, fv: int = 2, bfr: int = 10, ref=(-15.25, -72.5)) -> TfmInputs:
+    """Inputs of one gravity FM call at the reference's grid (TFM.cpp:190-225):
+    sea[n] = k of sea level under column n (TFM.cpp:224), topography from 6
+    seeded Gaussian ranges (<= 0.5 k units = 5 km, floored at -0.5, :225), Airy
+    Moho M0 = sea - 35/h - 5 topo (:225) plus a +-0.5 k perturbation, mantle
+    velocity 8 km/s +- 0.2 (3 seeded smooth layers), vm = 8, ro = -0.4
+    (-drhomc), dr = 0.3 (drhov) (:195-196)."""
+    rc = np.array([ref[0], ref[1], (dk - 3) * h / 2 - 5, (di - 1) * h / 2, (dj - 1) * h / 2,
+                   (dk - 1) * h / 2, h, float(di), float(dj), float(dk)])
+    j, i = np.meshgrid(np.arange(dj, dtype=np.float64), np.arange(di, dtype=np.float64),
+                       indexing="ij")
+    i, j = i.ravel(), j.ravel()
+    lat, lon, _ = _enu2geo(rc[:3], i * h - rc[3], j * h - rc[4], (dk - 3.0) * h - rc[5])
+    _, _, z = _geo2enu(rc[:3], lat, lon, np.zeros_like(lat))
+    sea = (z + rc[5]) / h
+    topo = np.maximum(0.5 * np.abs(_smooth2(seed, 1, di, dj, 6, 8.0)), -0.5)
+    moho = sea - 35.0 / h - 5.0 * topo + 0.5 * np.tanh(_smooth2(seed, 2, di, dj, 8, 6.0))
+    vel = np.empty(dk * dj * di)
+    for k in range(dk):
+        a = (k % 3) / 3.0
+        vel[k * di * dj:(k + 1) * di * dj] = 8.0 + 0.2 * np.tanh(
+            (1 - a) * _smooth2(seed, 10 + k // 3, di, dj, 8, 5.0)
+            + a * _smooth2(seed, 11 + k // 3, di, dj, 8, 5.0))
+    return TfmInputs(di, dj, dk, fh, fv, bfr, rc, sea, topo, moho, vel, 8.0, -0.4, 0.3)
+
+
 WORKLOADS = {
     "pm_cpu_small": pm_cpu_small,
     "pm_blobs_1m": pm_blobs_1m,
     "adjoint_small": adjoint_small,
     "ensemble_256": ensemble_256,
     "uneven_48": uneven_48,
+    "tfm_fm": tfm_fm,
 }
