@@ -528,6 +528,19 @@ def tfm_leg(stream, args, reps=20):
     interior[w.bfr:w.dj - w.bfr, w.bfr:w.di - w.bfr] = True
     interior = interior.ravel()
     err = relerr(g_dev[interior], G_cpu[interior])
+    # finite-difference Jacobian columns: 256 variants (gravity parameters and
+    # Moho perturbed per variant) in one batched call
+    B = 256
+    prm = np.array([[w.vm + 1e-3 * (b % 7), w.ro - 1e-3 * (b % 5), w.dr + 1e-3 * (b % 3)]
+                    for b in range(B)])
+    bump = np.sin(np.arange(M) * 0.37)
+    mohoB = t64(np.stack([w.moho + 0.01 * (b % 11) * bump for b in range(B)]))
+    prmB = t64(prm)
+    gB = torch.empty(B, M, dtype=torch.float64, device=dev)
+    dB = torch.empty(B, w.n_fine, dtype=torch.float32, device=dev)
+    t_batch = _timed(lambda: gfm.tfm_gravity_batched(w.bfr, w.di, w.dj, w.dk, w.fh, w.fv, w.rc,
+                                                     sea, mohoB, vel, prmB, s32, drho=dB, g=gB,
+                                                     stream=stream), 3, torch.cuda.synchronize)
     out = {
         "workload": "tfm_fm: TFM.cpp:83-94 at the reference grid, 241,865 fine cells x 1,681 "
                     "interior stations, producer + gravity per FM call",
@@ -535,6 +548,9 @@ def tfm_leg(stream, args, reps=20):
         "ms_device_resident": t_dev * 1e3,
         "kernels_ms": {"tfm_drho": p_ms / max(p_n, 1), "pm_fwd": f_ms / max(f_n, 1)},
         "ms_host_api": t_host * 1e3,
+        "jacobian_256_variants": {"ms_per_call": t_batch * 1e3,
+                                  "ms_per_variant": t_batch * 1e3 / B,
+                                  "vs_single_device_calls": t_dev * B / t_batch},
         "ms_gfm_program": None if t_prog is None else t_prog * 1e3,
         "cpu_port": {"ms_producer": t_prod_cpu * 1e3,
                      "ms_gravity_thread_per_row": t_grav_rows * 1e3, "rows": nj,

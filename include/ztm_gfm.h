@@ -15,6 +15,8 @@
  *                    generalised to explicit geometry, SI units, FP32/FP64,
  *                    the point-mass and the rectangular-prism cell formula,
  *                    and its adjoint (no reference counterpart for the adjoint).
+ *   ztm_tfm_*     <- the gravity part of FM::FM: the drho producer and the
+ *                    file/popen hop to GFM                     TFM.cpp:83-94
  *
  * Conventions
  *   - All functions return ZTM_OK (0) or a positive ZTM_E* code; the detail of
@@ -219,6 +221,24 @@ int ztm_tfm_gravity_host(int bfr, int di, int dj, int dk, int fh, int fv, const 
                          const double* sea, const double* moho, const double* vel,
                          const double* surf, double vm, double ro, double dr, double* G,
                          float* drho_out);
+
+/* The gravity of n_models <= 256 FM variants in one call: the columns a
+ * finite-difference Jacobian (TFM.cpp:124-135) needs for the parameters the
+ * gravity depends on. Variant b renders drho from moho_dev + b*moho_stride,
+ * vel_dev + b*vel_stride (strides in elements; 0 = shared) and
+ * params_dev[3b..3b+2] = (vm, ro, dr), into drho_dev[b][N] (float, as
+ * ztm_tfm_drho); its gravity at the interior stations (station heights s_dev,
+ * float[di*dj]) goes to g_dev[b][di*dj] (FP64; 0 outside the interior),
+ * through the split-BF16 MFMA ensemble forward: no near-field pass, so its
+ * ~1e-6 relative accuracy holds for stations at least half a cell from any
+ * kept cell; the 10 km depth filter keeps cells that far below stations at
+ * or above the -0.5 topography floor (TFM.cpp:89,225). No reference
+ * counterpart. Device pointers. */
+int ztm_tfm_gravity_batched(int bfr, int di, int dj, int dk, int fh, int fv, int n_models,
+                            const double* rc, const double* sea_dev, const double* moho_dev,
+                            int64_t moho_stride, const double* vel_dev, int64_t vel_stride,
+                            const double* params_dev, const float* s_dev, float* drho_dev,
+                            double* g_dev, void* stream);
 
 /* ------------------------------------------------- uneven batches */
 /* A batch of independent point-mass problems (each its own grid, stations and

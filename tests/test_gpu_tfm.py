@@ -72,3 +72,35 @@ def test_gravity_fm_repeated_calls_and_errors(dev, orc):
     with pytest.raises(gfm.ZtmError):
         gfm.tfm_gravity_host(w.bfr, w.di, w.dj, w.dk, w.fh, w.fv, bad_rc, w.sea, w.moho, w.vel,
                              w.surf, w.vm, w.ro, w.dr)
+
+
+def test_batched_variants_match_single_calls(dev, orc):
+    """Six FM variants (gravity parameters and Moho perturbed per variant) in
+    one call: drho bitwise equal to the single-call producer, gravity within
+    5e-6 of the single FP32 path and of the FP64 oracle (split-BF16 MFMA)."""
+    import torch
+    w = W.tfm_fm()
+    B, M = 6, w.di * w.dj
+    params = np.array([[w.vm + 0.01 * b, w.ro - 0.005 * b, w.dr + 0.002 * b] for b in range(B)])
+    bump = np.sin(np.arange(M) * 0.37)
+    moho = np.stack([w.moho + 0.1 * b * bump for b in range(B)])
+    t64 = lambda a: torch.from_numpy(np.ascontiguousarray(a, np.float64)).to(dev)
+    s32 = torch.from_numpy(w.surf.astype(np.float32)).to(dev)
+    g, drho = gfm.tfm_gravity_batched(w.bfr, w.di, w.dj, w.dk, w.fh, w.fv, w.rc, t64(w.sea),
+                                      t64(moho), t64(w.vel), t64(params), s32)
+    g, drho = g.cpu().numpy(), drho.cpu().numpy()
+    for b in range(B):
+        Gb, db = gfm.tfm_gravity_host(w.bfr, w.di, w.dj, w.dk, w.fh, w.fv, w.rc, w.sea, moho[b],
+                                      w.vel, w.surf, *params[b], want_drho=True)
+        assert np.array_equal(drho[b], db), b
+        assert relerr(g[b], Gb) < 5e-6, b
+        assert np.all(g[b][Gb == 0] == 0)
+    want = orc.gf(w.bfr, w.di, w.dj, w.dk, w.fh, w.fv, drho[B - 1], w.surf.astype(np.float32))
+    assert relerr(g[B - 1], want) < 5e-6
+    # shared Moho and velocity (stride 0): three gravity-parameter columns
+    g3, _ = gfm.tfm_gravity_batched(w.bfr, w.di, w.dj, w.dk, w.fh, w.fv, w.rc, t64(w.sea),
+                                    t64(w.moho), t64(w.vel), t64(params[:3]), s32)
+    g3 = g3.cpu().numpy()
+    G0 = gfm.tfm_gravity_host(w.bfr, w.di, w.dj, w.dk, w.fh, w.fv, w.rc, w.sea, w.moho, w.vel,
+                              w.surf, *params[2])
+    assert relerr(g3[2], G0) < 5e-6

@@ -801,6 +801,48 @@ int ztm_tfm_gravity_host(int bfr, int di, int dj, int dk, int fh, int fv, const 
     return ZTM_OK;
 }
 
+int ztm_tfm_gravity_batched(int bfr, int di, int dj, int dk, int fh, int fv, int n_models,
+                            const double* rc, const double* sea_dev, const double* moho_dev,
+                            int64_t moho_stride, const double* vel_dev, int64_t vel_stride,
+                            const double* params_dev, const float* s_dev, float* drho_dev,
+                            double* g_dev, void* stream) {
+    if (int e = tfm_check(di, dj, dk, fh, fv, rc)) return e;
+    if (int e = ref_check(bfr, di, dj, dk, fh, fv)) return e;
+    if (n_models < 1 || n_models > 256)
+        return fail(ZTM_EINVAL, "n_models must be in [1, 256] (got %d)", n_models);
+    if (!sea_dev || !moho_dev || !vel_dev || !params_dev || !s_dev || !drho_dev || !g_dev)
+        return fail(ZTM_EINVAL, "null argument");
+    if (moho_stride < 0 || vel_stride < 0) return fail(ZTM_EINVAL, "negative stride");
+    ztm_plan p = nullptr;
+    if (int e = ref_plan(bfr, di, dj, dk, fh, fv, &p)) return e;
+    hipStream_t s = (hipStream_t)stream;
+    if (int e = tfm_drho_batched_launch(di, dj, dk, fh, fv, n_models, rc, sea_dev, moho_dev,
+                                        moho_stride, vel_dev, vel_stride, params_dev, drho_dev, s))
+        return e;
+    const int M = di * dj, m = p->shape.m;
+    if (m == 0) return tfm_scatter_launch(nullptr, nullptr, 0, M, n_models, g_dev, s);
+    if (int e = prep_ref_stations(s_dev, bfr, di, dj, fh, fv, p->st_f, p->st_d, p->st_c,
+                                  p->out_map, m, p->shape.m_pad, s))
+        return e;
+    const int ks = p->tuning.batched_slices > 0 ? p->tuning.batched_slices
+                                                : batched_default_slices(p->shape, n_models);
+    const int split = p->tuning.batched_split;
+    const size_t ws = (batched_workspace(p->shape, n_models, ks, split) + 255) / 256 * 256;
+    const size_t need = ws + sizeof(double) * (size_t)n_models * m;
+    if (need > p->bws_cap) {
+        if (p->bws) (void)hipFree(p->bws);
+        p->bws = nullptr;
+        p->bws_cap = 0;
+        ZTM_HIP(hipMalloc(&p->bws, need));
+        p->bws_cap = need;
+    }
+    double* Y = (double*)((char*)p->bws + ws);
+    if (int e = batched_fwd_launch(p->shape, n_models, ks, split, (const float*)p->st_f, drho_dev,
+                                   p->prob.threshold, p->sy, p->sz, pm_scale(p), p->bws, Y, s))
+        return e;
+    return tfm_scatter_launch(Y, p->out_map, m, M, n_models, g_dev, s);
+}
+
 // ----------------------------------------------------------- profiling
 int ztm_profile_enable(int on) {
     std::lock_guard<std::mutex> lk(g_mu);

@@ -190,6 +190,15 @@ TfmFrame tfm_frame(const double* rc);
 int tfm_drho_launch(int di, int dj, int dk, int fh, int fv, const double* rc, const double* sea,
                     const double* moho, const double* vel, double vm, double ro, double dr,
                     float* drho, hipStream_t stream);
+// B variants at once: moho/vel advance by their strides (elements, 0 =
+// shared) per variant, params[3b..3b+2] = (vm, ro, dr); drho [B][N].
+int tfm_drho_batched_launch(int di, int dj, int dk, int fh, int fv, int B, const double* rc,
+                            const double* sea, const double* moho, int64_t moho_stride,
+                            const double* vel, int64_t vel_stride, const double* params,
+                            float* drho, hipStream_t stream);
+// g [B][M] = 0 except g[b][out_map[i]] = Y[b][i], i < m.
+int tfm_scatter_launch(const double* Y, const int* out_map, int m, int M, int B, double* g,
+                       hipStream_t stream);
 
 // Prism formula (gfm_prism.hip).
 // FP32: per-station mid-zone correction (prism integral minus the far-field

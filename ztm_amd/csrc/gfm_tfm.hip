@@ -68,16 +68,12 @@ __device__ __forceinline__ double depth_km(const TfmFrame& fr, double ci, double
     return 6371.0 - r;
 }
 
-__global__ __launch_bounds__(256) void tfm_drho_kernel(int di, int dj, int dk, int fh, int fv,
-                                                       TfmFrame fr,
-                                                       const double* __restrict__ sea,
-                                                       const double* __restrict__ moho,
-                                                       const double* __restrict__ vel, double vm,
-                                                       double ro, double dr,
-                                                       float* __restrict__ drho, int64_t N) {
+__device__ __forceinline__ float drho_at(int64_t n, int di, int dj, int dk, int fh, int fv,
+                                         const TfmFrame& fr, const double* __restrict__ sea,
+                                         const double* __restrict__ moho,
+                                         const double* __restrict__ vel, double vm, double ro,
+                                         double dr) {
 #pragma clang fp contract(off)
-    const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= N) return;
     const int ii = (di - 1) * fh + 1, jj = (dj - 1) * fh + 1;
     const double i = (double)(n % ii) / (1. * fh), j = (double)((n / ii) % jj) / (1. * fh),
                  k = (double)(n / ((int64_t)ii * jj)) / (1. * fv), li = 1. / fv;
@@ -97,7 +93,47 @@ __global__ __launch_bounds__(256) void tfm_drho_kernel(int di, int dj, int dk, i
     }
     float out = 0.0f;
     if (fabs(r0 - r1) > .001 && depth_km(fr, i, j, k) > 10) out = (float)(r1 - r0);
-    drho[n] = out;
+    return out;
+}
+
+__global__ __launch_bounds__(256) void tfm_drho_kernel(int di, int dj, int dk, int fh, int fv,
+                                                       TfmFrame fr,
+                                                       const double* __restrict__ sea,
+                                                       const double* __restrict__ moho,
+                                                       const double* __restrict__ vel, double vm,
+                                                       double ro, double dr,
+                                                       float* __restrict__ drho, int64_t N) {
+    const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= N) return;
+    drho[n] = drho_at(n, di, dj, dk, fh, fv, fr, sea, moho, vel, vm, ro, dr);
+}
+
+// Variant b = blockIdx.y: its own Moho and velocity grids (strides in
+// elements, 0 = shared) and (vm, ro, dr) = params[3b .. 3b+2]; drho [B][N].
+__global__ __launch_bounds__(256) void tfm_drho_batched_kernel(
+    int di, int dj, int dk, int fh, int fv, TfmFrame fr, const double* __restrict__ sea,
+    const double* __restrict__ moho, int64_t moho_stride, const double* __restrict__ vel,
+    int64_t vel_stride, const double* __restrict__ params, float* __restrict__ drho, int64_t N) {
+    const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= N) return;
+    const int b = blockIdx.y;
+    drho[(size_t)b * N + n] =
+        drho_at(n, di, dj, dk, fh, fv, fr, sea, moho + b * moho_stride, vel + b * vel_stride,
+                params[3 * b], params[3 * b + 1], params[3 * b + 2]);
+}
+
+// g[b][*] = 0, then g[b][out_map[i]] = Y[b][i] for the m prepared stations.
+__global__ void tfm_fill_kernel(double* __restrict__ g, int64_t n) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) g[t] = 0.0;
+}
+
+__global__ void tfm_scatter_kernel(const double* __restrict__ Y, const int* __restrict__ out_map,
+                                   int m, int M, int B, double* __restrict__ g) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)m * B) return;
+    const int b = (int)(t / m), i = (int)(t % m);
+    g[(size_t)b * M + out_map[i]] = Y[t];
 }
 
 TfmFrame tfm_frame(const double* rc) {
@@ -131,6 +167,36 @@ int tfm_drho_launch(int di, int dj, int dk, int fh, int fv, const double* rc, co
     ProfScope prof("tfm_drho", stream);
     hipLaunchKernelGGL(tfm_drho_kernel, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, stream,
                        di, dj, dk, fh, fv, fr, sea, moho, vel, vm, ro, dr, drho, N);
+    ZTM_HIP(hipGetLastError());
+    return ZTM_OK;
+}
+
+int tfm_drho_batched_launch(int di, int dj, int dk, int fh, int fv, int B, const double* rc,
+                            const double* sea, const double* moho, int64_t moho_stride,
+                            const double* vel, int64_t vel_stride, const double* params,
+                            float* drho, hipStream_t stream) {
+    const int64_t N = ((int64_t)(di - 1) * fh + 1) * ((int64_t)(dj - 1) * fh + 1) *
+                      ((int64_t)(dk - 1) * fv + 1);
+    if (N == 0 || B <= 0) return ZTM_OK;
+    const TfmFrame fr = tfm_frame(rc);
+    ProfScope prof("tfm_drho", stream);
+    hipLaunchKernelGGL(tfm_drho_batched_kernel, dim3((unsigned)((N + 255) / 256), (unsigned)B),
+                       dim3(256), 0, stream, di, dj, dk, fh, fv, fr, sea, moho, moho_stride, vel,
+                       vel_stride, params, drho, N);
+    ZTM_HIP(hipGetLastError());
+    return ZTM_OK;
+}
+
+int tfm_scatter_launch(const double* Y, const int* out_map, int m, int M, int B, double* g,
+                       hipStream_t stream) {
+    const int64_t n = (int64_t)M * B;
+    if (n == 0) return ZTM_OK;
+    hipLaunchKernelGGL(tfm_fill_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream,
+                       g, n);
+    const int64_t ns = (int64_t)m * B;
+    if (ns > 0)
+        hipLaunchKernelGGL(tfm_scatter_kernel, dim3((unsigned)((ns + 255) / 256)), dim3(256), 0,
+                           stream, Y, out_map, m, M, B, g);
     ZTM_HIP(hipGetLastError());
     return ZTM_OK;
 }

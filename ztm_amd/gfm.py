@@ -79,6 +79,8 @@ SIGNATURES = {
     "ztm_tfm_drho": (_I, [_I, _I, _I, _I, _I, _P, _P, _P, _P, _D, _D, _D, _P, _P]),
     "ztm_tfm_gravity_host": (_I, [_I, _I, _I, _I, _I, _I, _P, _P, _P, _P, _P, _D, _D, _D, _P,
                                   _P]),
+    "ztm_tfm_gravity_batched": (_I, [_I, _I, _I, _I, _I, _I, _I, _P, _P, _P, _I64, _P, _I64,
+                                     _P, _P, _P, _P, _P]),
     "ztm_batch_create": (_I, [ctypes.POINTER(_P), _I, ctypes.POINTER(CProblem),
                               ctypes.POINTER(_I64), ctypes.POINTER(_P)]),
     "ztm_batch_destroy": (_I, [_P]),
@@ -203,6 +205,36 @@ def tfm_gravity_host(bfr, di, dj, dk, fh, fv, rc, sea, moho, vel, surf, vm, ro, 
                                      dr, G.ctypes.data,
                                      drho.ctypes.data if drho is not None else None))
     return (G, drho) if want_drho else G
+
+
+def tfm_gravity_batched(bfr, di, dj, dk, fh, fv, rc, sea, moho, vel, params, s, drho=None,
+                        g=None, stream=None):
+    """B FM variants in one call (FD Jacobian columns): sea float64 [di*dj];
+    moho float64 [di*dj] (shared) or [B, di*dj]; vel [di*dj*dk] or
+    [B, di*dj*dk]; params float64 [B, 3] = (vm, ro, dr) on the device; s
+    float32 [di*dj]. Returns (g float64 [B, di*dj], drho float32 [B, N])."""
+    import torch
+    rc = np.ascontiguousarray(rc, dtype=np.float64).ravel()
+    B = int(params.shape[0])
+    n = ((di - 1) * fh + 1) * ((dj - 1) * fh + 1) * ((dk - 1) * fv + 1)
+    M = di * dj
+    _check_t(params, torch.float64, 3 * B, "params")
+    _check_t(sea, torch.float64, M, "sea")
+    _check_t(s, torch.float32, M, "s")
+    ms = M if moho.numel() == B * M and B > 1 else 0
+    vs = M * dk if vel.numel() == B * M * dk and B > 1 else 0
+    _check_t(moho, torch.float64, B * M if ms else M, "moho")
+    _check_t(vel, torch.float64, B * M * dk if vs else M * dk, "vel")
+    if drho is None:
+        drho = torch.empty(B, n, dtype=torch.float32, device=sea.device)
+    if g is None:
+        g = torch.empty(B, M, dtype=torch.float64, device=sea.device)
+    _check_t(drho, torch.float32, B * n, "drho")
+    _check_t(g, torch.float64, B * M, "g")
+    check(lib().ztm_tfm_gravity_batched(bfr, di, dj, dk, fh, fv, B, rc.ctypes.data, _ptr(sea),
+                                        _ptr(moho), ms, _ptr(vel), vs, _ptr(params), _ptr(s),
+                                        _ptr(drho), _ptr(g), _stream(stream)))
+    return g, drho
 
 
 # ------------------------------------------------------------------ plans
