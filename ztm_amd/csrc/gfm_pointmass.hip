@@ -170,37 +170,44 @@ __global__ __launch_bounds__(256) void pm_fwd_kernel(
 
 // g[m] = scale * (sum_slices part[slice][m] + correction)
 //   (general API, FP64 out), or gref[map[m]] += (float)(...) (reference ABI).
-// kCorrPmNear: FP64 near field over the (2 kNear + 1)^3 box around the
-// station; kCorrPrism: the per-station correction computed beforehand.
-__device__ __forceinline__ double pm_fwd_reduce_value(const double* __restrict__ part,
-                                                     int n_slices, int m_pad, int i, double scale,
-                                                     const float* __restrict__ rho_f,
-                                                     const NearArgs& na) {
+// kCorrPmNear: FP64 near field over the station's correction box and its
+// clamped rows; kCorrPrism: the per-station correction computed beforehand.
+// One wave per station: the lanes stride over the slice partials and the
+// correction pairs, then a fixed butterfly sums them (deterministic; every
+// lane returns the result).
+__device__ __forceinline__ double pm_fwd_reduce_wave(const double* __restrict__ part,
+                                                    int n_slices, int m_pad, int i, double scale,
+                                                    const float* __restrict__ rho_f,
+                                                    const NearArgs& na, int lane) {
     double s = 0.0;
-    for (int k = 0; k < n_slices; ++k) s += part[(size_t)k * m_pad + i];
+    for (int k = lane; k < n_slices; k += 64) s += part[(size_t)k * m_pad + i];
     if (na.mode == kCorrPmNear) {
         const float4 pf = na.st_f[i];
         const double4 pd = na.st_d[i];
         const int4 c = na.st_c[i];
-        double corr = 0.0;
-        const int k0 = max(c.z - na.rz, 0), k1 = min(c.z + na.rz, na.nz - 1);
-        const int j0 = max(c.y - na.ry, 0), j1 = min(c.y + na.ry, na.ny - 1);
         const int i0 = max(c.x - na.rx, 0), i1 = min(c.x + na.rx, na.nx - 1);
-        for (int kk = k0; kk <= k1; ++kk)
-            for (int jj = j0; jj <= j1; ++jj)
-                for (int ii = i0; ii <= i1; ++ii) {
-                    const float w = rho_f[((size_t)kk * na.ny + jj) * na.nx + ii];
-                    if (!(fabsf(w) > na.thr)) continue;
-                    corr += corr_pair<false>(kCorrPmNear, w, pf, pd, c, ii, jj, kk, na);
-                }
-        corr += clamped_rows_corr<false>(kCorrPmNear, pf, pd, c, rho_f, na, 0, 1);
-        s += corr;
-    } else if (na.mode == kCorrPrism) {
-        s += na.corr[i];
+        const int j0 = max(c.y - na.ry, 0), j1 = min(c.y + na.ry, na.ny - 1);
+        const int k0 = max(c.z - na.rz, 0), k1 = min(c.z + na.rz, na.nz - 1);
+        const int bi = i1 - i0 + 1, bj = j1 - j0 + 1, bk = k1 - k0 + 1;
+        const int total = (bi > 0 && bj > 0 && bk > 0) ? bi * bj * bk : 0;
+        for (int t = lane; t < total; t += 64) {
+            const int ii = i0 + t % bi;
+            const int rest = t / bi;
+            const int jj = j0 + rest % bj;
+            const int kk = k0 + rest / bj;
+            const float w = rho_f[((size_t)kk * na.ny + jj) * na.nx + ii];
+            if (!(fabsf(w) > na.thr)) continue;
+            s += corr_pair<false>(kCorrPmNear, w, pf, pd, c, ii, jj, kk, na);
+        }
+        s += clamped_rows_corr<false>(kCorrPmNear, pf, pd, c, rho_f, na, lane, 64);
     }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+    if (na.mode == kCorrPrism) s += na.corr[i];
     return s * scale;
 }
 
+// block = 256 = 4 stations (one wave each)
 __global__ __launch_bounds__(256) void pm_fwd_reduce_kernel(const double* __restrict__ part,
                                      int n_slices, int m_pad, int m,
                                      double scale, double* __restrict__ g,
@@ -208,9 +215,11 @@ __global__ __launch_bounds__(256) void pm_fwd_reduce_kernel(const double* __rest
                                      const int* __restrict__ out_map,
                                      const float* __restrict__ rho_f,
                                      NearArgs na) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= m) return;
-    const double v = pm_fwd_reduce_value(part, n_slices, m_pad, i, scale, rho_f, na);
+    const int lane = threadIdx.x & 63;
+    const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (i >= m) return;   // uniform per wave
+    const double v = pm_fwd_reduce_wave(part, n_slices, m_pad, i, scale, rho_f, na, lane);
+    if (lane != 0) return;
     if (out_map)
         gref[out_map[i]] += (float)v;
     else
@@ -453,14 +462,16 @@ __global__ __launch_bounds__(256) void pm_fwd_reduce_grouped_kernel(
     const GFwdDesc* __restrict__ d, const int64_t* __restrict__ st_start, int np, int64_t total,
     const float* __restrict__ rho, double* __restrict__ g, float* __restrict__ gf,
     unsigned* __restrict__ ticket) {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t == 0) *ticket = 0u;
+    const int lane = threadIdx.x & 63;
+    const int64_t t = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);   // one wave per station
+    if (blockIdx.x == 0 && threadIdx.x == 0) *ticket = 0u;
     if (t >= total) return;
     const int p = find_problem(st_start, np, t);
     const GFwdDesc& q = d[p];
     const int i = (int)(t - st_start[p]);
-    const double v =
-        pm_fwd_reduce_value(q.part, q.n_slices, q.m_pad, i, q.scale, rho + q.cell_off, q.na);
+    const double v = pm_fwd_reduce_wave(q.part, q.n_slices, q.m_pad, i, q.scale,
+                                        rho + q.cell_off, q.na, lane);
+    if (lane != 0) return;
     if (g) g[q.st_off + i] = v;
     if (gf) gf[q.st_off + i] = (float)v;
 }
@@ -666,7 +677,7 @@ int pm_fwd_launch(const PmShape& s, const PmTuning& t, int precision,
         }
         ZTM_HIP(hipGetLastError());
     }
-    hipLaunchKernelGGL(pm_fwd_reduce_kernel, dim3((s.m + 255) / 256), dim3(256), 0,
+    hipLaunchKernelGGL(pm_fwd_reduce_kernel, dim3((s.m + 3) / 4), dim3(256), 0,
                        stream, part, n_slices, s.m_pad, s.m, scale, g, gref, out_map,
                        (const float*)rho, na);
     ZTM_HIP(hipGetLastError());
@@ -821,7 +832,7 @@ int pm_fwd_grouped_launch(const GroupedPass& gp, const float* rho, double* g, fl
         ZTM_HIP(hipGetLastError());
     }
     const int64_t items = std::max<int64_t>(gp.total_items, 1);
-    hipLaunchKernelGGL(pm_fwd_reduce_grouped_kernel, dim3((unsigned)((items + 255) / 256)),
+    hipLaunchKernelGGL(pm_fwd_reduce_grouped_kernel, dim3((unsigned)((items + 3) / 4)),
                        dim3(256), 0, stream, (const GFwdDesc*)gp.desc, gp.item_start, gp.np,
                        gp.total_items, rho, g, gf, gp.ticket);
     ZTM_HIP(hipGetLastError());
