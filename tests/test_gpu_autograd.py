@@ -117,5 +117,5 @@ def test_smoothness_value_and_gradient(dev, case):
         x = torch.from_numpy(w.rho).to(dev).requires_grad_(True)
         s = torch_ops.smoothness(x, p)
         s.backward()
-        assert abs(float(s) - want) <= 1e-12 * want
+        assert abs(float(s.detach()) - want) <= 1e-12 * want
         assert np.max(np.abs(x.grad.cpu().numpy() - grad_ltl(w.rho, shape))) < 1e-9

@@ -161,6 +161,16 @@ def test_batch_graph_replays_equal_eager_and_alone(dev):
             assert torch.equal(RR, rr0), rep
             b.adjoint(torch.ones(b.total_stations, device=dev))   # eager work between replays
             torch.cuda.synchronize()
+        # the captured graph reads lambda from the device at each launch: a new
+        # lambda on the same graph equals an eager solve with it
+        xe2, rre2 = b.solve(d, 2.0 * lam, iters=10, use_graph=False)
+        xe2, rre2 = xe2.clone(), rre2.clone()
+        b.solve(d, 2.0 * lam, iters=10, x=X, rr=RR, use_graph=True)
+        torch.cuda.synchronize()
+        assert torch.equal(RR, rre2)
+        assert not torch.equal(rre2, rr0)
+        for a, e in zip(b.unpack_cells(X), b.unpack_cells(xe2)):
+            assert torch.equal(a, e)
         xs0 = b.unpack_cells(x0.cpu())
         ds = b.unpack_stations(d.cpu())
         tunings = [b.tuning(i) for i in range(len(ps))]

@@ -231,6 +231,9 @@ struct ztm_batch_s {
     int n_chunks = 0;
     float *r = nullptr, *p = nullptr, *ap = nullptr, *qf = nullptr;
     double *part_a = nullptr, *part_b = nullptr, *lam = nullptr, *rr = nullptr;
+    double* lam_host = nullptr;          // pinned staging of lambda
+    hipEvent_t lam_copied = nullptr;     // the last staging copy has run
+    bool lam_recorded = false;
     int rr_cap = 0;   // entries
     hipStream_t cap = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -257,6 +260,8 @@ void batch_free(ztm_batch b) {
                     (void*)b->lam, (void*)b->rr})
         if (q) (void)hipFree(q);
     if (b->cap) (void)hipStreamDestroy(b->cap);
+    if (b->lam_copied) (void)hipEventDestroy(b->lam_copied);
+    if (b->lam_host) (void)hipHostFree(b->lam_host);
     for (ztm_plan p : b->plans) plan_free(p);
     delete b;
 }
@@ -451,6 +456,11 @@ int ztm_batch_create(ztm_batch* batch, int n_problems, const ztm_problem* proble
     if (rc == ZTM_OK) rc = upload_vec(&b->chunk_prob, chunk_prob);
     if (rc == ZTM_OK && hipStreamCreateWithFlags(&b->cap, hipStreamNonBlocking) != hipSuccess)
         rc = fail(ZTM_EHIP, "hipStreamCreate failed");
+    if (rc == ZTM_OK && hipHostMalloc(&b->lam_host, sizeof(double) * n_problems) != hipSuccess)
+        rc = fail(ZTM_ENOMEM, "hipHostMalloc failed");
+    if (rc == ZTM_OK &&
+        hipEventCreateWithFlags(&b->lam_copied, hipEventDisableTiming) != hipSuccess)
+        rc = fail(ZTM_EHIP, "hipEventCreate failed");
     if (rc == ZTM_OK) rc = rebuild(b);
     if (rc) {
         batch_free(b);
@@ -508,7 +518,13 @@ int ztm_batch_solve(ztm_batch batch, const float* d_dev, const double* lambda, i
         }
         rr = b->rr;
     }
-    ZTM_HIP(hipMemcpyAsync(b->lam, lambda, b->np * sizeof(double), hipMemcpyHostToDevice, s));
+    // Stage lambda in pinned memory (the caller's array may change as soon as
+    // this returns); the previous staging copy must have run before reuse.
+    if (b->lam_recorded) ZTM_HIP(hipEventSynchronize(b->lam_copied));
+    std::memcpy(b->lam_host, lambda, b->np * sizeof(double));
+    ZTM_HIP(hipMemcpyAsync(b->lam, b->lam_host, b->np * sizeof(double), hipMemcpyHostToDevice, s));
+    ZTM_HIP(hipEventRecord(b->lam_copied, s));
+    b->lam_recorded = true;
     if (!use_graph) return enqueue_solve(b, d_dev, iters, x_dev, rr, s);
     const void* key[4] = {d_dev, x_dev, rr, nullptr};
     if (!b->exec || b->key_iters != iters || std::memcmp(key, b->key, sizeof key) != 0) {

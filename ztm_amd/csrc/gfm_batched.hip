@@ -290,15 +290,21 @@ size_t batched_workspace(const PmShape& s, int B, int ks, int split) {
 }
 
 int batched_default_slices(const PmShape& s, int B) {
-    (void)B;
     const int tiles = (s.m + kBatchTile - 1) / kBatchTile;
     const int64_t n_chunks = (s.n_cells + kBatchChunk - 1) / kBatchChunk;
-    // at least ~2 resident workgroups per CU, and at most ~384 chunks (MFMA
-    // accumulation steps) per slice to bound the FP32 accumulation error
-    int ks = std::max(1, (512 + tiles - 1) / tiles);
-    ks = std::max<int64_t>(ks, (n_chunks + 383) / 384);
-    ks = (int)std::min<int64_t>(ks, std::max<int64_t>(1, n_chunks));
-    return std::min(ks, 65535);
+    // Resident workgroups on 256 CUs for the kernel instance B selects
+    // (registers: 2 per CU at 256 models, more for fewer model tiles).
+    const int nt = batched_nt(std::max(B, 1));
+    const int64_t cap = 256 * (nt >= 8 ? 2 : nt == 4 ? 3 : nt == 2 ? 5 : 8);
+    // At least one full round of workgroups and at most ~384 chunks (MFMA
+    // accumulation steps) per slice, to bound the FP32 accumulation error;
+    // then as many slices as fit the same number of rounds, so the last round
+    // is not left mostly empty (time ~ rounds x chunks per slice).
+    int64_t ks = std::max<int64_t>((cap + tiles - 1) / tiles, (n_chunks + 383) / 384);
+    const int64_t rounds = (tiles * ks + cap - 1) / cap;
+    ks = std::max<int64_t>(ks, rounds * cap / tiles);
+    ks = std::min<int64_t>(ks, std::max<int64_t>(1, n_chunks));
+    return (int)std::min<int64_t>(ks, 65535);
 }
 
 int batched_fwd_launch(const PmShape& s, int B, int ks, int split, const float* st,
