@@ -222,3 +222,30 @@ def test_batch_rejects(dev):
             b.set_tuning(-1, 3, 0, 16, 0)
         with pytest.raises(gfm.ZtmError):
             b.solve(torch.zeros(b.total_stations, device=dev), [-1.0, 1.0], iters=2)
+
+
+def test_batch_tiny_and_odd_problems(dev, orc):
+    """One station, three stations, a 3-cell-wide grid and a flat slab, mixed
+    with an ordinary problem: grouped results equal the single-problem
+    kernels bit for bit and the oracle to 1e-5."""
+    ps = small_set(1)
+    g1 = W.Grid(3, 4, 5, 0.0, 0.0, -50.0, 100.0, 100.0, -100.0)
+    g2 = W.Grid(40, 30, 1, 0.0, 0.0, -25.0, 50.0, 50.0, -50.0)
+    extra = [
+        W.Subproblem(g1, np.array([[100.0, 150.0, 20.0]]), np.zeros(g1.n_cells), np.zeros(1)),
+        W.Subproblem(g2, np.array([[0.0, 0.0, 5.0], [1000.0, 700.0, 30.0], [1950.0, 1450.0, 1.0]]),
+                     np.zeros(g2.n_cells), np.zeros(3)),
+        W.Subproblem(g1, np.array([[-500.0, 900.0, 300.0], [120.0, 80.0, 0.5]]),
+                     np.zeros(g1.n_cells), np.zeros(2)),
+    ]
+    ps = ps + extra
+    rho = [W.uniform(21, i, p.grid.n_cells, -300.0, 300.0) for i, p in enumerate(ps)]
+    r = [W.normal(21, 50 + i, p.stations.shape[0], 1.0) for i, p in enumerate(ps)]
+    with make_batch(ps) as b:
+        gs, as_ = check_against_plans(dev, b, ps, rho, r)
+    for i, p in enumerate(ps):
+        want = orc.pm_forward(p.grid, p.kappa, rho[i].astype(np.float32).astype(np.float64),
+                              p.stations)
+        assert relerr(gs[i], want) < 1e-5, i
+        wa = orc.pm_adjoint(p.grid, p.kappa, p.stations, r[i].astype(np.float32).astype(np.float64))
+        assert relerr(as_[i], wa) < 1e-5, i

@@ -78,3 +78,20 @@ def test_batched_rejects(dev):
     with gfm.Plan(w.grid, w.stations, kappa=w.kappa) as p:
         with pytest.raises(gfm.ZtmError):
             p.forward_batched(torch.zeros(257, w.grid.n_cells, device=dev))
+
+
+def test_batched_narrow_rows(dev, orc):
+    """nx < 8 (a lane's 8 cells cross several rows: the generic stepping
+    path), nx = 8 (the two-row path at its limit), against the oracle."""
+    import torch
+    for nx, ny, nz in ((5, 7, 9), (3, 4, 20), (8, 6, 5)):
+        g = W.Grid(nx, ny, nz, 10.0, 20.0, -15.0, 30.0, 35.0, -25.0)
+        c = g.centres()
+        st = np.stack([W.uniform(6, a, 40, c[:, a].min() - 50, c[:, a].max() + 50)
+                       for a in range(2)] + [W.uniform(6, 2, 40, 20.0, 200.0)], axis=1)
+        P = models(7 + nx, 9, g.n_cells)
+        with gfm.Plan(g, st, kappa=W.KAPPA_SI) as p:
+            Y = p.forward_batched(torch.from_numpy(P).to(dev)).cpu().numpy()
+        for b in (0, 4, 8):
+            want = orc.pm_forward(g, W.KAPPA_SI, P[b].astype(np.float64), st)
+            assert relerr(Y[b], want) < 2e-6, (nx, b)
