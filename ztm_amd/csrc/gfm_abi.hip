@@ -439,17 +439,22 @@ int ztm_plan_forward(ztm_plan plan, int precision, const void* rho_dev, double* 
                                       plan->sz, plan->prob.threshold, plan->prob.kappa * plan->L,
                                       plan->part, g_dev, s);
         }
-        const size_t ws = pm_fwd_workspace(plan->shape, t, precision);
-        if (int rc = ensure_workspace(plan, ws + plan->shape.m_pad)) return rc;
         NearArgs na = near_args(plan);
+        const bool quad = na.quad != 0;
+        const size_t ws = pm_fwd_workspace(plan->shape, t, precision, quad);
+        if (int rc = ensure_workspace(plan, ws + plan->shape.m_pad)) return rc;
         na.corr = plan->part + ws;
+        // the main kernel's FP32 rounding, which the corrections replicate
+        na.sep = fwd_group(fwd_adjust(plan->shape, t, sizeof(float), quad).st_per_lane,
+                           sizeof(float), quad) > 1 ? 1 : 0;
         if (int rc = prism_near_fwd_launch(plan->shape, na, (const float*)rho_dev, plan->part + ws, s))
             return rc;
         return pm_fwd_launch(plan->shape, t, precision, plan->st_f, rho_dev, plan->sy, plan->sz,
                              plan->prob.threshold, pm_scale(plan), plan->part, g_dev, nullptr,
                              nullptr, na, s);
     }
-    if (int rc = ensure_workspace(plan, pm_fwd_workspace(plan->shape, t, precision))) return rc;
+    if (int rc = ensure_workspace(plan, pm_fwd_workspace(plan->shape, t, precision, false)))
+        return rc;
     const void* st = precision == ZTM_FP64 ? (const void*)plan->st_d : (const void*)plan->st_f;
     return pm_fwd_launch(plan->shape, t, precision, st, rho_dev, plan->sy, plan->sz,
                          plan->prob.threshold, pm_scale(plan), plan->part, g_dev, nullptr,
@@ -647,7 +652,7 @@ int ref_plan(int bfr, int di, int dj, int dk, int fh, int fv, ztm_plan* out) {
         plan_free(p);
         return hip_fail(e, "reference plan allocation");
     }
-    if (int rc = ensure_workspace(p, pm_fwd_workspace(p->shape, p->tuning, ZTM_FP32))) {
+    if (int rc = ensure_workspace(p, pm_fwd_workspace(p->shape, p->tuning, ZTM_FP32, false))) {
         plan_free(p);
         return rc;
     }

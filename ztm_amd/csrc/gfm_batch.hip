@@ -576,7 +576,8 @@ int ztm_batch_get_tuning(ztm_batch batch, int problem, int* st_per_lane, int* ro
         return fail(ZTM_EINVAL, "bad batch or problem index");
     ztm_plan p = batch->plans[problem];
     const PmTuning t = effective_tuning(p, ZTM_FP32);
-    if (st_per_lane) *st_per_lane = t.st_per_lane;
+    if (st_per_lane)   // the stations per lane the forward runs with (wide rows: 8)
+        *st_per_lane = fwd_adjust(p->shape, t, sizeof(float), false).st_per_lane;
     if (rows_per_slice) *rows_per_slice = fwd_rows_per_slice(p->shape, t, sizeof(float));
     if (cells_per_lane) *cells_per_lane = t.cells_per_lane;
     if (adj_splits) *adj_splits = ztm::adj_splits(p->shape, t);

@@ -86,11 +86,15 @@ struct NearArgs {
     float syf = 1.f, szf = 1.f, thr = 0.f;
     double syd = 1.0, szd = 1.0, vol = 1.0;   // vol = |sy*sz|, cell volume / L^3
     int kmin = 0, kmax = -1;
+    // The FP32 forward kernel forms r^2 = (a*a) + R with two roundings (it
+    // shares a*a across a group of rows) instead of fma(a, a, R); the
+    // forward's FP32 replicas follow suit when set.
+    int sep = 0;
 };
 
 // Launch shapes of the point-mass kernels.
 struct PmTuning {
-    int st_per_lane = 4;      // forward: stations per lane (1,2,4,8)
+    int st_per_lane = 2;      // forward: stations per lane (1,2,4,8)
     int rows_per_slice = 0;   // forward: 0 = choose from the problem shape
     int cells_per_lane = 16;  // adjoint: cells per lane (4,8,16)
     int adj_splits = 0;       // adjoint: 0 = choose from the problem shape
@@ -107,9 +111,13 @@ struct PmShape {
 
 // Kernel launchers (gfm_pointmass.hip). `part` must hold at least the number
 // of doubles returned by the matching *_workspace() call.
-size_t pm_fwd_workspace(const PmShape& s, const PmTuning& t, int precision);
+size_t pm_fwd_workspace(const PmShape& s, const PmTuning& t, int precision, bool quad);
 // Forward rows per LDS slice / adjoint station splits actually used for t.
-int fwd_rows_per_slice(const PmShape& s, const PmTuning& t, size_t elem);
+// FP32 point-mass forward: rows per group sharing a*a (1 otherwise), and
+// the tuning actually run (rows too wide for a row group fall back to S = 8).
+int fwd_group(int S, size_t elem, bool quad);
+PmTuning fwd_adjust(const PmShape& s, PmTuning t, size_t elem, bool quad);
+int fwd_rows_per_slice(const PmShape& s, const PmTuning& t, size_t elem, bool quad = false);
 int adj_splits(const PmShape& s, const PmTuning& t);
 size_t pm_adj_workspace(const PmShape& s, const PmTuning& t, int precision,
                         bool near);

@@ -62,6 +62,14 @@ __device__ __forceinline__ T kterm(T a, T R, T qrow, const QuadArgs& q) {
 // pm_fwd_block: one workgroup's work -- station tile bx, row slice by -- so
 // that the single-problem kernel and the grouped (multi-problem) kernel share
 // it; smem must hold max(rows_per_slice * nxp * sizeof(T), 4*S*64*8) bytes.
+// Rows a wave processes together in the FP32 point-mass forward, sharing
+// a*a across them (fewer for more stations per lane, to bound the
+// registers); 1 for FP64 and the quadrupole kernel. Matches fwd_group().
+template <typename T, int S, bool QUAD>
+constexpr int kFwdGroup = (sizeof(T) == sizeof(float) && !QUAD)
+                              ? (S <= 1 ? 8 : S == 2 ? 4 : S == 4 ? 2 : 1)
+                              : 1;
+
 template <typename T, int S, bool QUAD>
 __device__ __forceinline__ void pm_fwd_block(
     const typename Vec4<T>::type* __restrict__ st, const T* __restrict__ rho,
@@ -69,19 +77,21 @@ __device__ __forceinline__ void pm_fwd_block(
     double* __restrict__ part, int m_pad, const QuadArgs& qa, int bx, int by,
     unsigned char* smem) {
     using V4 = typename Vec4<T>::type;
+    constexpr int G = kFwdGroup<T, S, QUAD>;
     T* lrho = reinterpret_cast<T*>(smem);
 
     const int nxp = (nx + 3) & ~3;
     const int row0 = by * rows_per_slice;
     const int nr = min(rows_per_slice, n_rows - row0);
+    const int nr_pad = (nr + G - 1) / G * G;   // whole row groups; extra rows are 0
     const size_t base = (size_t)row0 * (size_t)nx;
 
     // Stage this slice's densities, thresholded and zero-padded to nxp.
-    for (int t = threadIdx.x; t < nr * nxp; t += blockDim.x) {
+    for (int t = threadIdx.x; t < nr_pad * nxp; t += blockDim.x) {
         const int r = t / nxp;
         const int i = t - r * nxp;
         T v = T(0);
-        if (i < nx) {
+        if (i < nx && r < nr) {
             v = rho[base + (size_t)r * nx + i];
             if (!(fabs(v) > thr)) v = T(0);   // NaN or |rho| <= thr: skipped
         }
@@ -102,6 +112,64 @@ __device__ __forceinline__ void pm_fwd_block(
         acc[s] = 0.0;
     }
 
+    if constexpr (G > 1) {
+        // Groups of G consecutive rows per wave: per station and cell column
+        // a = X - i and a*a are formed once and shared by the G rows, so a
+        // pair costs r^2 = a*a + R (two roundings, as pair_k32<.., true>
+        // replicates), rsq, v*v*v and the accumulate. Padded rows hold 0.
+        for (int rg = wave * G; rg < nr; rg += 4 * G) {
+            T R[G][S], ar[G][S], xm[S];
+#pragma unroll
+            for (int g = 0; g < G; ++g) {
+                const int row = row0 + rg + g;
+                const int k = row / ny;
+                const int j = row - k * ny;
+                const T yj = T(j) * sy;
+                const T zk = T(k) * sz;
+#pragma unroll
+                for (int s = 0; s < S; ++s) {
+                    R[g][s] = row_R(Y[s] - yj, Z[s] - zk);
+                    ar[g][s] = T(0);
+                }
+            }
+#pragma unroll
+            for (int s = 0; s < S; ++s) xm[s] = X[s];
+            const int nx4 = nxp >> 2;
+            const V4* lrow = reinterpret_cast<const V4*>(lrho + rg * nxp);
+            for (int i4 = 0; i4 < nx4; ++i4) {
+                V4 q[G];
+#pragma unroll
+                for (int g = 0; g < G; ++g) q[g] = lrow[g * nx4 + i4];   // wave-uniform
+#pragma unroll
+                for (int s = 0; s < S; ++s) {
+                    const T a0 = xm[s], a1 = xm[s] - T(1), a2 = xm[s] - T(2), a3 = xm[s] - T(3);
+                    const T A0 = __fmul_rn(a0, a0), A1 = __fmul_rn(a1, a1);
+                    const T A2 = __fmul_rn(a2, a2), A3 = __fmul_rn(a3, a3);
+#pragma unroll
+                    for (int g = 0; g < G; ++g) {
+                        const T v0 = rsq(__fadd_rn(A0, R[g][s]));
+                        const T v1 = rsq(__fadd_rn(A1, R[g][s]));
+                        const T v2 = rsq(__fadd_rn(A2, R[g][s]));
+                        const T v3 = rsq(__fadd_rn(A3, R[g][s]));
+                        T t = ar[g][s];
+                        t = fma(q[g].x, (v0 * v0) * v0, t);
+                        t = fma(q[g].y, (v1 * v1) * v1, t);
+                        t = fma(q[g].z, (v2 * v2) * v2, t);
+                        t = fma(q[g].w, (v3 * v3) * v3, t);
+                        ar[g][s] = t;
+                    }
+                    xm[s] -= T(4);
+                }
+            }
+#pragma unroll
+            for (int g = 0; g < G; ++g) {
+                const int row = row0 + rg + g;
+                const T zk = T(row / ny) * sz;
+#pragma unroll
+                for (int s = 0; s < S; ++s) acc[s] += (double)((Z[s] - zk) * ar[g][s]);
+            }
+        }
+    } else
     for (int r = wave; r < nr; r += 4) {
         const int row = row0 + r;
         const int k = row / ny;
@@ -197,9 +265,11 @@ __device__ __forceinline__ double pm_fwd_reduce_wave(const double* __restrict__ 
             const int kk = k0 + rest / bj;
             const float w = rho_f[((size_t)kk * na.ny + jj) * na.nx + ii];
             if (!(fabsf(w) > na.thr)) continue;
-            s += corr_pair<false>(kCorrPmNear, w, pf, pd, c, ii, jj, kk, na);
+            s += na.sep ? corr_pair<false, true>(kCorrPmNear, w, pf, pd, c, ii, jj, kk, na)
+                        : corr_pair<false, false>(kCorrPmNear, w, pf, pd, c, ii, jj, kk, na);
         }
-        s += clamped_rows_corr<false>(kCorrPmNear, pf, pd, c, rho_f, na, lane, 64);
+        s += na.sep ? clamped_rows_corr<false, true>(kCorrPmNear, pf, pd, c, rho_f, na, lane, 64)
+                    : clamped_rows_corr<false, false>(kCorrPmNear, pf, pd, c, rho_f, na, lane, 64);
     }
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
@@ -558,16 +628,37 @@ int prep_ref_stations(const float* s, int bfr, int di, int dj, int fh, int fv,
 // ------------------------------------------------------------ launchers
 constexpr int kTargetWgs = 2048;   // 8 workgroups per CU on 256 CUs
 
-int fwd_rows_per_slice(const PmShape& s, const PmTuning& t, size_t elem) {
+int fwd_group(int S, size_t elem, bool quad) {
+    if (elem != sizeof(float) || quad) return 1;
+    return S <= 1 ? 8 : S == 2 ? 4 : S == 4 ? 2 : 1;
+}
+
+PmTuning fwd_adjust(const PmShape& s, PmTuning t, size_t elem, bool quad) {
+    // a row group of G rows must fit the 32 KB LDS budget; wide rows run S = 8
+    const size_t nxp = (size_t)((s.nx + 3) & ~3);
+    if (fwd_group(t.st_per_lane, elem, quad) > 1 &&
+        nxp * elem * fwd_group(t.st_per_lane, elem, quad) > 32768)
+        t.st_per_lane = 8;
+    return t;
+}
+
+int fwd_rows_per_slice(const PmShape& s, const PmTuning& t0, size_t elem, bool quad) {
+    const PmTuning t = fwd_adjust(s, t0, elem, quad);
+    const int G = fwd_group(t.st_per_lane, elem, quad);
     const int nxp = (s.nx + 3) & ~3;
     const int lds_rows = std::max(1, (int)(32768 / ((size_t)nxp * elem)));
-    if (t.rows_per_slice > 0) return std::min(t.rows_per_slice, std::max(lds_rows, 1));
-    const int tiles = s.m_pad / (64 * t.st_per_lane);
-    const int64_t want_slices = std::max<int64_t>(1, (kTargetWgs + tiles - 1) / tiles);
-    int64_t rps = (s.n_rows + want_slices - 1) / want_slices;
-    rps = std::max<int64_t>(rps, 4);
-    rps = std::min<int64_t>(rps, lds_rows);
-    rps = std::min<int64_t>(rps, s.n_rows);
+    int64_t rps;
+    if (t.rows_per_slice > 0) {
+        rps = std::min(t.rows_per_slice, lds_rows);
+    } else {
+        const int tiles = s.m_pad / (64 * t.st_per_lane);
+        const int64_t want_slices = std::max<int64_t>(1, (kTargetWgs + tiles - 1) / tiles);
+        rps = (s.n_rows + want_slices - 1) / want_slices;
+        rps = std::max<int64_t>(rps, 4);
+        rps = std::min<int64_t>(rps, lds_rows);
+        rps = std::min<int64_t>(rps, s.n_rows);
+    }
+    if (rps > G) rps = rps / G * G;   // whole row groups per slice where possible
     return (int)std::max<int64_t>(rps, 1);
 }
 
@@ -589,7 +680,9 @@ void launch_fwd(const PmShape& s, int rps, int n_slices, const void* st,
                 const QuadArgs& qa, hipStream_t stream) {
     using V4 = typename Vec4<T>::type;
     const int nxp = (s.nx + 3) & ~3;
-    const size_t lds = std::max((size_t)rps * nxp * sizeof(T), (size_t)4 * S * 64 * sizeof(double));
+    const int G = fwd_group(S, sizeof(T), quad);
+    const size_t lds = std::max((size_t)((rps + G - 1) / G * G) * nxp * sizeof(T),
+                                (size_t)4 * S * 64 * sizeof(double));
     const dim3 grid(s.m_pad / (64 * S), n_slices);
     if (quad && sizeof(T) == sizeof(float))
         hipLaunchKernelGGL((pm_fwd_kernel<T, S, true>), grid, dim3(256), lds, stream,
@@ -626,9 +719,9 @@ void launch_adj(const PmShape& s, int splits, int direct, const void* st, const 
 
 }  // namespace
 
-size_t pm_fwd_workspace(const PmShape& s, const PmTuning& t, int precision) {
+size_t pm_fwd_workspace(const PmShape& s, const PmTuning& t, int precision, bool quad) {
     const size_t elem = precision == ZTM_FP64 ? sizeof(double) : sizeof(float);
-    const int rps = fwd_rows_per_slice(s, t, elem);
+    const int rps = fwd_rows_per_slice(s, t, elem, quad && precision == ZTM_FP32);
     const int64_t n_slices = (s.n_rows + rps - 1) / rps;
     return (size_t)n_slices * (size_t)s.m_pad;
 }
@@ -640,7 +733,7 @@ size_t pm_adj_workspace(const PmShape& s, const PmTuning& t, int precision, bool
     return (sp > 1 || near) ? (size_t)sp * (size_t)s.n_cells : 0;
 }
 
-int pm_fwd_launch(const PmShape& s, const PmTuning& t, int precision,
+int pm_fwd_launch(const PmShape& s, const PmTuning& t0, int precision,
                   const void* st, const void* rho, double sy, double sz,
                   double thr, double scale, double* part, double* g,
                   float* gref, const int* out_map, const NearArgs& near,
@@ -651,12 +744,14 @@ int pm_fwd_launch(const PmShape& s, const PmTuning& t, int precision,
         return fail(ZTM_EINVAL, "nx=%d too large for one LDS row (max %d)", s.nx,
                     (int)(65536 / elem) - 3);
     if (s.m == 0) return ZTM_OK;
-    const int rps = fwd_rows_per_slice(s, t, elem);
-    const int n_slices = (int)((s.n_rows + rps - 1) / rps);
-    if (n_slices > 65535) return fail(ZTM_EINVAL, "too many row slices (%d)", n_slices);
     NearArgs na = near;
     if (precision == ZTM_FP64) na.mode = kCorrNone;   // FP64 needs no correction
     const bool quad = na.quad != 0 && precision == ZTM_FP32;
+    const PmTuning t = fwd_adjust(s, t0, elem, quad);
+    const int rps = fwd_rows_per_slice(s, t, elem, quad);
+    const int n_slices = (int)((s.n_rows + rps - 1) / rps);
+    if (n_slices > 65535) return fail(ZTM_EINVAL, "too many row slices (%d)", n_slices);
+    na.sep = fwd_group(t.st_per_lane, elem, quad) > 1 ? 1 : 0;
     {
         ProfScope prof("pm_fwd", stream);
         if (precision == ZTM_FP64) {
@@ -734,20 +829,21 @@ int pm_adj_launch(const PmShape& s, const PmTuning& t, int precision,
 // ------------------------------------------------------ grouped launchers
 int pm_grouped_workspace(ztm_plan p) {
     const PmTuning t = effective_tuning(p, ZTM_FP32);
-    return ensure_workspace(p, std::max(pm_fwd_workspace(p->shape, t, ZTM_FP32),
+    return ensure_workspace(p, std::max(pm_fwd_workspace(p->shape, t, ZTM_FP32, false),
                                         pm_adj_workspace(p->shape, t, ZTM_FP32, true)));
 }
 
 int pm_fwd_desc(ztm_plan p, int64_t cell_off, int64_t st_off, GFwdDesc* d, size_t* lds,
                 int64_t* blocks) {
-    const PmTuning t = effective_tuning(p, ZTM_FP32);
     const PmShape& s = p->shape;
+    const PmTuning t = fwd_adjust(s, effective_tuning(p, ZTM_FP32), sizeof(float), false);
     const int nxp = (s.nx + 3) & ~3;
     if ((size_t)nxp * sizeof(float) > 65536) return fail(ZTM_EINVAL, "nx=%d too large", s.nx);
     if (t.st_per_lane != 2 && t.st_per_lane != 4 && t.st_per_lane != 8)
         return fail(ZTM_EINVAL, "grouped forward needs st_per_lane 2, 4 or 8 (got %d)",
                     t.st_per_lane);
-    const int rps = fwd_rows_per_slice(s, t, sizeof(float));
+    const int rps = fwd_rows_per_slice(s, t, sizeof(float), false);
+    const int G = fwd_group(t.st_per_lane, sizeof(float), false);
     d->st = p->st_f;
     d->part = p->part;
     d->cell_off = cell_off;
@@ -766,7 +862,8 @@ int pm_fwd_desc(ztm_plan p, int64_t cell_off, int64_t st_off, GFwdDesc* d, size_
     d->thr = (float)p->prob.threshold;
     d->scale = pm_scale(p);
     d->na = near_args(p);
-    *lds = std::max((size_t)rps * nxp * sizeof(float),
+    d->na.sep = G > 1 ? 1 : 0;
+    *lds = std::max((size_t)((rps + G - 1) / G * G) * nxp * sizeof(float),
                     (size_t)4 * t.st_per_lane * 64 * sizeof(double));
     *blocks = s.m == 0 ? 0 : (int64_t)d->tiles_x * d->n_slices;
     return ZTM_OK;
