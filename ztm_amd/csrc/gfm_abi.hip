@@ -444,9 +444,6 @@ int ztm_plan_forward(ztm_plan plan, int precision, const void* rho_dev, double* 
         const size_t ws = pm_fwd_workspace(plan->shape, t, precision, quad);
         if (int rc = ensure_workspace(plan, ws + plan->shape.m_pad)) return rc;
         na.corr = plan->part + ws;
-        // the main kernel's FP32 rounding, which the corrections replicate
-        na.sep = fwd_group(fwd_adjust(plan->shape, t, sizeof(float), quad).st_per_lane,
-                           sizeof(float), quad) > 1 ? 1 : 0;
         if (int rc = prism_near_fwd_launch(plan->shape, na, (const float*)rho_dev, plan->part + ws, s))
             return rc;
         return pm_fwd_launch(plan->shape, t, precision, plan->st_f, rho_dev, plan->sy, plan->sz,
@@ -487,7 +484,8 @@ int ztm_plan_adjoint(ztm_plan plan, int precision, const void* r_dev, void* out_
     }
     if (int rc = ensure_workspace(
             plan, std::max<size_t>(1, pm_adj_workspace(plan->shape, t, precision,
-                                                       precision == ZTM_FP32))))
+                                                       precision == ZTM_FP32,
+                                                       near_args(plan).quad != 0))))
         return rc;
     const void* st = precision == ZTM_FP64 ? (const void*)plan->st_d : (const void*)plan->st_f;
     return pm_adj_launch(plan->shape, t, precision, st, r_dev, plan->sy, plan->sz,

@@ -580,7 +580,7 @@ int ztm_batch_get_tuning(ztm_batch batch, int problem, int* st_per_lane, int* ro
         *st_per_lane = fwd_adjust(p->shape, t, sizeof(float), false).st_per_lane;
     if (rows_per_slice) *rows_per_slice = fwd_rows_per_slice(p->shape, t, sizeof(float));
     if (cells_per_lane) *cells_per_lane = t.cells_per_lane;
-    if (adj_splits) *adj_splits = ztm::adj_splits(p->shape, t);
+    if (adj_splits) *adj_splits = ztm::adj_splits(p->shape, t, sizeof(float), false);
     if (size_class_out) *size_class_out = batch->cls[problem];
     return ZTM_OK;
 }
@@ -640,7 +640,7 @@ int ztm_batch_autotune(ztm_batch batch, int reps, void* stream) {
                     PmTuning& t = b->plans[i]->tuning;
                     t.cells_per_lane = C;
                     t.adj_splits = 0;
-                    const int auto_sp = ztm::adj_splits(b->plans[i]->shape, t);
+                    const int auto_sp = ztm::adj_splits(b->plans[i]->shape, t, sizeof(float), false);
                     t.adj_splits = std::max(1, (int)std::lround(auto_sp * f));
                 }
                 float ms = 0.0f;
@@ -656,7 +656,7 @@ int ztm_batch_autotune(ztm_batch batch, int reps, void* stream) {
             PmTuning& t = b->plans[i]->tuning;
             t.cells_per_lane = best_c;
             t.adj_splits = 0;
-            const int auto_sp = ztm::adj_splits(b->plans[i]->shape, t);
+            const int auto_sp = ztm::adj_splits(b->plans[i]->shape, t, sizeof(float), false);
             t.adj_splits = std::max(1, (int)std::lround(auto_sp * best_f));
         }
     }

@@ -86,17 +86,13 @@ struct NearArgs {
     float syf = 1.f, szf = 1.f, thr = 0.f;
     double syd = 1.0, szd = 1.0, vol = 1.0;   // vol = |sy*sz|, cell volume / L^3
     int kmin = 0, kmax = -1;
-    // The FP32 forward kernel forms r^2 = (a*a) + R with two roundings (it
-    // shares a*a across a group of rows) instead of fma(a, a, R); the
-    // forward's FP32 replicas follow suit when set.
-    int sep = 0;
 };
 
 // Launch shapes of the point-mass kernels.
 struct PmTuning {
     int st_per_lane = 2;      // forward: stations per lane (1,2,4,8)
     int rows_per_slice = 0;   // forward: 0 = choose from the problem shape
-    int cells_per_lane = 16;  // adjoint: cells per lane (4,8,16)
+    int cells_per_lane = 8;   // adjoint: cells per lane (4,8,16)
     int adj_splits = 0;       // adjoint: 0 = choose from the problem shape
     int batched_slices = 0;   // batched forward K-slices: 0 = choose
     int batched_split = 3;    // batched forward BF16 split: 2 (hi/lo) or 3
@@ -113,14 +109,19 @@ struct PmShape {
 // of doubles returned by the matching *_workspace() call.
 size_t pm_fwd_workspace(const PmShape& s, const PmTuning& t, int precision, bool quad);
 // Forward rows per LDS slice / adjoint station splits actually used for t.
-// FP32 point-mass forward: rows per group sharing a*a (1 otherwise), and
-// the tuning actually run (rows too wide for a row group fall back to S = 8).
+// FP32 point-mass forward: rows per group sharing the per-column offset
+// a = X - i (1 otherwise), and the tuning actually run (rows too wide for a
+// row group fall back to S = 8).
 int fwd_group(int S, size_t elem, bool quad);
 PmTuning fwd_adjust(const PmShape& s, PmTuning t, size_t elem, bool quad);
 int fwd_rows_per_slice(const PmShape& s, const PmTuning& t, size_t elem, bool quad = false);
-int adj_splits(const PmShape& s, const PmTuning& t);
+// FP32 point-mass adjoint: rows a lane takes together (1 otherwise), the
+// number of lane segments (C cells of G rows) and the station splits.
+int adj_group(int C, size_t elem, bool quad);
+int64_t adj_segments(const PmShape& s, int C, int G, int* segs_per_row);
+int adj_splits(const PmShape& s, const PmTuning& t, size_t elem, bool quad);
 size_t pm_adj_workspace(const PmShape& s, const PmTuning& t, int precision,
-                        bool near);
+                        bool near, bool quad);
 
 // Forward: writes g[m] = scale * sum (FP64). When out_map != nullptr the
 // result is instead added as float into gref[out_map[m]] (reference ABI).

@@ -24,12 +24,10 @@ __device__ __forceinline__ bool row_clamped(float dy, float dz) {
 }
 
 // Far-field kernel of one pair in FP32, exactly as the main kernels evaluate
-// it. a = X - i, R = row_R32(dy, dz), qrow = q2y dy^2 + q2z dz^2. SEP: r^2 as
-// (a*a) + R with two roundings (the row-grouped forward kernel), else
-// fma(a, a, R).
-template <bool QUAD, bool SEP = false>
+// it. a = X - i, R = row_R32(dy, dz), qrow = q2y dy^2 + q2z dz^2.
+template <bool QUAD>
 __device__ __forceinline__ float pair_k32(float a, float R, float qrow, const QuadArgs& q) {
-    const float v = rsq_f(SEP ? __fadd_rn(__fmul_rn(a, a), R) : fmaf(a, a, R));
+    const float v = rsq_f(fmaf(a, a, R));
     const float v2 = v * v;
     const float v3 = v2 * v;
     if constexpr (QUAD) {
@@ -112,7 +110,7 @@ __device__ __forceinline__ float gl3_cell(float X, float Y, float Z, int i, int 
 //   correction box (r*), prism:    FP32 3x3x3 Gauss-Legendre mean of dz/r^3;
 //   beyond (cells of a clamped row): FP64 far-field kernel (point mass, plus
 //                                 the quadrupole term for non-cubic prisms).
-template <bool QUAD, bool SEP = false>
+template <bool QUAD>
 __device__ __forceinline__ double corr_pair(int mode, float w, const float4& pf,
                                             const double4& pd, const int4& c, int i, int j,
                                             int k, const NearArgs& a) {
@@ -121,7 +119,7 @@ __device__ __forceinline__ double corr_pair(int mode, float w, const float4& pf,
     const float af = pf.x - (float)i;
     const float R = row_R32(dyf, dzf);
     const float qrow = QUAD ? fmaf(a.q.q2y, dyf * dyf, a.q.q2z * (dzf * dzf)) : 0.f;
-    const double t32 = (double)dzf * (double)(w * pair_k32<QUAD, SEP>(af, R, qrow, a.q));
+    const double t32 = (double)dzf * (double)(w * pair_k32<QUAD>(af, R, qrow, a.q));
     const int di = abs(i - c.x), dj = abs(j - c.y), dk = abs(k - c.z);
     const bool in_near = di <= a.nrx && dj <= a.nry && dk <= a.nrz;
     const bool in_box = di <= a.rx && dj <= a.ry && dk <= a.rz;
@@ -151,7 +149,7 @@ __device__ __forceinline__ double corr_pair(int mode, float w, const float4& pf,
 // the caller. Rows are the (j,k) within (jr, kr) of (c.y, c.z) whose FP32 row
 // term was clamped (row_clamped). Lanes [lane0, lane0 + nlanes) split the
 // cells of each row.
-template <bool QUAD, bool SEP = false>
+template <bool QUAD>
 __device__ __forceinline__ double clamped_rows_corr(int mode, const float4& pf, const double4& pd,
                                                     const int4& c, const float* __restrict__ rho,
                                                     const NearArgs& a, int lane0, int nlanes) {
@@ -168,7 +166,7 @@ __device__ __forceinline__ double clamped_rows_corr(int mode, const float4& pf, 
                 if (row_in_box && abs(ii - c.x) <= a.rx) continue;
                 const float w = rho[((size_t)kk * a.ny + jj) * a.nx + ii];
                 if (!(fabsf(w) > a.thr)) continue;
-                acc += corr_pair<QUAD, SEP>(mode, w, pf, pd, c, ii, jj, kk, a);
+                acc += corr_pair<QUAD>(mode, w, pf, pd, c, ii, jj, kk, a);
             }
         }
     return acc;

@@ -62,9 +62,10 @@ __device__ __forceinline__ T kterm(T a, T R, T qrow, const QuadArgs& q) {
 // pm_fwd_block: one workgroup's work -- station tile bx, row slice by -- so
 // that the single-problem kernel and the grouped (multi-problem) kernel share
 // it; smem must hold max(rows_per_slice * nxp * sizeof(T), 4*S*64*8) bytes.
-// Rows a wave processes together in the FP32 point-mass forward, sharing
-// a*a across them (fewer for more stations per lane, to bound the
-// registers); 1 for FP64 and the quadrupole kernel. Matches fwd_group().
+// Rows a wave processes together in the FP32 point-mass forward, sharing the
+// per-column offset a = X - i across them (fewer for more stations per lane,
+// to bound the registers); 1 for FP64 and the quadrupole kernel. Matches
+// fwd_group().
 template <typename T, int S, bool QUAD>
 constexpr int kFwdGroup = (sizeof(T) == sizeof(float) && !QUAD)
                               ? (S <= 1 ? 8 : S == 2 ? 4 : S == 4 ? 2 : 1)
@@ -114,9 +115,10 @@ __device__ __forceinline__ void pm_fwd_block(
 
     if constexpr (G > 1) {
         // Groups of G consecutive rows per wave: per station and cell column
-        // a = X - i and a*a are formed once and shared by the G rows, so a
-        // pair costs r^2 = a*a + R (two roundings, as pair_k32<.., true>
-        // replicates), rsq, v*v*v and the accumulate. Padded rows hold 0.
+        // a = X - i is formed once and shared by the G rows, so a pair costs
+        // r^2 = fma(a, a, R), rsq, v*v*v and the accumulate (the same
+        // arithmetic as pair_k32, which the near-field corrections replicate).
+        // Padded rows hold 0.
         for (int rg = wave * G; rg < nr; rg += 4 * G) {
             T R[G][S], ar[G][S], xm[S];
 #pragma unroll
@@ -143,14 +145,12 @@ __device__ __forceinline__ void pm_fwd_block(
 #pragma unroll
                 for (int s = 0; s < S; ++s) {
                     const T a0 = xm[s], a1 = xm[s] - T(1), a2 = xm[s] - T(2), a3 = xm[s] - T(3);
-                    const T A0 = __fmul_rn(a0, a0), A1 = __fmul_rn(a1, a1);
-                    const T A2 = __fmul_rn(a2, a2), A3 = __fmul_rn(a3, a3);
 #pragma unroll
                     for (int g = 0; g < G; ++g) {
-                        const T v0 = rsq(__fadd_rn(A0, R[g][s]));
-                        const T v1 = rsq(__fadd_rn(A1, R[g][s]));
-                        const T v2 = rsq(__fadd_rn(A2, R[g][s]));
-                        const T v3 = rsq(__fadd_rn(A3, R[g][s]));
+                        const T v0 = rsq(fma(a0, a0, R[g][s]));
+                        const T v1 = rsq(fma(a1, a1, R[g][s]));
+                        const T v2 = rsq(fma(a2, a2, R[g][s]));
+                        const T v3 = rsq(fma(a3, a3, R[g][s]));
                         T t = ar[g][s];
                         t = fma(q[g].x, (v0 * v0) * v0, t);
                         t = fma(q[g].y, (v1 * v1) * v1, t);
@@ -265,11 +265,9 @@ __device__ __forceinline__ double pm_fwd_reduce_wave(const double* __restrict__ 
             const int kk = k0 + rest / bj;
             const float w = rho_f[((size_t)kk * na.ny + jj) * na.nx + ii];
             if (!(fabsf(w) > na.thr)) continue;
-            s += na.sep ? corr_pair<false, true>(kCorrPmNear, w, pf, pd, c, ii, jj, kk, na)
-                        : corr_pair<false, false>(kCorrPmNear, w, pf, pd, c, ii, jj, kk, na);
+            s += corr_pair<false>(kCorrPmNear, w, pf, pd, c, ii, jj, kk, na);
         }
-        s += na.sep ? clamped_rows_corr<false, true>(kCorrPmNear, pf, pd, c, rho_f, na, lane, 64)
-                    : clamped_rows_corr<false, false>(kCorrPmNear, pf, pd, c, rho_f, na, lane, 64);
+        s += clamped_rows_corr<false>(kCorrPmNear, pf, pd, c, rho_f, na, lane, 64);
     }
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
@@ -297,8 +295,17 @@ __global__ __launch_bounds__(256) void pm_fwd_reduce_kernel(const double* __rest
 }
 
 // ---------------------------------------------------------------- adjoint
+// Rows a lane takes together in the FP32 point-mass adjoint (same k, G
+// consecutive j): per station, dz, q = r dz and the column offsets a = X - i
+// are formed once for all G rows. 1 for FP64 and the quadrupole kernel.
+// Matches adj_group().
+template <typename T, int C, bool QUAD>
+constexpr int kAdjGroup = (sizeof(T) == sizeof(float) && !QUAD) ? (C >= 16 ? 2 : 4) : 1;
+
 // grid = (ceil(n_segs/64), n_splits); block = 256; a segment is C
-// consecutive cells of one row (the last one of a row may be ragged).
+// consecutive cells of G consecutive rows of one k plane (the last segment of
+// a row and the last row group of a plane may be ragged); segments run
+// i fastest, then the row group, then k.
 // pm_adj_block: one workgroup's work -- segment block bx, station split by;
 // smem must hold max(256 * sizeof(V4), 4*C*64*8) bytes.
 template <typename T, int C, bool QUAD>
@@ -309,23 +316,29 @@ __device__ __forceinline__ void pm_adj_block(
     T* __restrict__ out, double scale, int64_t n_cells, int direct, const QuadArgs& qa,
     int bx, int by, unsigned char* smem) {
     using V4 = typename Vec4<T>::type;
+    constexpr int G = kAdjGroup<T, C, QUAD>;
     V4* tile = reinterpret_cast<V4*>(smem);
 
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
+    const int rgroups = (ny + G - 1) / G;
     const int64_t seg = (int64_t)bx * 64 + lane;
     const int64_t segc = seg < n_segs ? seg : 0;
-    const int64_t row = segc / segs_per_row;
-    const int i0 = (int)(segc - row * segs_per_row) * C;
-    const int k = (int)(row / ny);
-    const int j = (int)(row - (int64_t)k * ny);
-    const T yj = T(j) * sy;
+    const int64_t rowg = segc / segs_per_row;
+    const int i0 = (int)(segc - rowg * segs_per_row) * C;
+    const int k = (int)(rowg / rgroups);
+    const int j0 = (int)(rowg - (int64_t)k * rgroups) * G;
     const T zk = T(k) * sz;
     const T xi0 = T(i0);
-
-    double accd[C];
+    T yj[G];
 #pragma unroll
-    for (int c = 0; c < C; ++c) accd[c] = 0.0;
+    for (int g = 0; g < G; ++g) yj[g] = T(j0 + g) * sy;
+
+    double accd[G][C];
+#pragma unroll
+    for (int g = 0; g < G; ++g)
+#pragma unroll
+        for (int c = 0; c < C; ++c) accd[g][c] = 0.0;
 
     const int mb = by * st_per_split;
     const int me = min(mb + st_per_split, m_pad);
@@ -338,49 +351,64 @@ __device__ __forceinline__ void pm_adj_block(
             tile[threadIdx.x] = p;
         }
         __syncthreads();
-        T acc[C];
+        T acc[G][C];
 #pragma unroll
-        for (int c = 0; c < C; ++c) acc[c] = T(0);
+        for (int g = 0; g < G; ++g)
+#pragma unroll
+            for (int c = 0; c < C; ++c) acc[g][c] = T(0);
         const int sb = wave * 64;
         for (int s = sb; s < sb + 64; ++s) {
             const V4 p = tile[s];    // wave-uniform LDS broadcast
-            const T dy = p.y - yj;
             const T dz = p.z - zk;
-            const T R = row_R(dy, dz);
-            const T qrow = QUAD ? fma(T(qa.q2y), dy * dy, T(qa.q2z) * (dz * dz)) : T(0);
             const T q = p.w * dz;
             const T xm = p.x - xi0;
+            T R[G], qrow[G];
+#pragma unroll
+            for (int g = 0; g < G; ++g) {
+                const T dy = p.y - yj[g];
+                R[g] = row_R(dy, dz);
+                qrow[g] = QUAD ? fma(T(qa.q2y), dy * dy, T(qa.q2z) * (dz * dz)) : T(0);
+            }
 #pragma unroll
             for (int c = 0; c < C; ++c) {
                 const T a = xm - T(c);
-                acc[c] = fma(q, kterm<T, QUAD>(a, R, qrow, qa), acc[c]);
+#pragma unroll
+                for (int g = 0; g < G; ++g)
+                    acc[g][c] = fma(q, kterm<T, QUAD>(a, R[g], qrow[g], qa), acc[g][c]);
             }
         }
 #pragma unroll
-        for (int c = 0; c < C; ++c) accd[c] += (double)acc[c];
+        for (int g = 0; g < G; ++g)
+#pragma unroll
+            for (int c = 0; c < C; ++c) accd[g][c] += (double)acc[g][c];
     }
 
-    // Cross-wavefront reduction (fixed order).
-    __syncthreads();
+    // Cross-wavefront reduction (fixed order), one row of the group at a time.
     double* red = reinterpret_cast<double*>(smem);
 #pragma unroll
-    for (int c = 0; c < C; ++c) red[(wave * C + c) * 64 + lane] = accd[c];
-    __syncthreads();
-    for (int e = threadIdx.x; e < 64 * C; e += blockDim.x) {
-        const int c = e >> 6;
-        const int l = e & 63;
-        const double v = red[c * 64 + l] + red[(C + c) * 64 + l] +
-                         red[(2 * C + c) * 64 + l] + red[(3 * C + c) * 64 + l];
-        const int64_t sg = (int64_t)bx * 64 + l;
-        if (sg >= n_segs) continue;
-        const int64_t rw = sg / segs_per_row;
-        const int ii = (int)(sg - rw * segs_per_row) * C + c;
-        if (ii >= nx) continue;
-        const int64_t cell = rw * nx + ii;
-        if (direct) {
-            out[cell] = (T)(scale * v);
-        } else {
-            part[(size_t)by * (size_t)n_cells + cell] = v;
+    for (int g = 0; g < G; ++g) {
+        __syncthreads();
+#pragma unroll
+        for (int c = 0; c < C; ++c) red[(wave * C + c) * 64 + lane] = accd[g][c];
+        __syncthreads();
+        for (int e = threadIdx.x; e < 64 * C; e += blockDim.x) {
+            const int c = e >> 6;
+            const int l = e & 63;
+            const double v = red[c * 64 + l] + red[(C + c) * 64 + l] +
+                             red[(2 * C + c) * 64 + l] + red[(3 * C + c) * 64 + l];
+            const int64_t sg = (int64_t)bx * 64 + l;
+            if (sg >= n_segs) continue;
+            const int64_t rg = sg / segs_per_row;
+            const int ii = (int)(sg - rg * segs_per_row) * C + c;
+            const int kk = (int)(rg / rgroups);
+            const int jj = (int)(rg - (int64_t)kk * rgroups) * G + g;
+            if (ii >= nx || jj >= ny) continue;
+            const int64_t cell = ((int64_t)kk * ny + jj) * nx + ii;
+            if (direct) {
+                out[cell] = (T)(scale * v);
+            } else {
+                part[(size_t)by * (size_t)n_cells + cell] = v;
+            }
         }
     }
 }
@@ -662,9 +690,20 @@ int fwd_rows_per_slice(const PmShape& s, const PmTuning& t0, size_t elem, bool q
     return (int)std::max<int64_t>(rps, 1);
 }
 
-int adj_splits(const PmShape& s, const PmTuning& t) {
+int adj_group(int C, size_t elem, bool quad) {
+    if (elem != sizeof(float) || quad) return 1;
+    return C >= 16 ? 2 : 4;
+}
+
+int64_t adj_segments(const PmShape& s, int C, int G, int* segs_per_row) {
+    const int spr = (s.nx + C - 1) / C;
+    if (segs_per_row) *segs_per_row = spr;
+    return (int64_t)s.nz * ((s.ny + G - 1) / G) * spr;
+}
+
+int adj_splits(const PmShape& s, const PmTuning& t, size_t elem, bool quad) {
     const int C = t.cells_per_lane;
-    const int64_t segs = s.n_rows * ((s.nx + C - 1) / C);
+    const int64_t segs = adj_segments(s, C, adj_group(C, elem, quad), nullptr);
     const int64_t wgs = (segs + 63) / 64;
     const int max_splits = s.m_pad / 256;
     int sp = t.adj_splits;
@@ -699,8 +738,8 @@ void launch_adj(const PmShape& s, int splits, int direct, const void* st, const 
                 T sy, T sz, double* part, void* out, double scale, bool quad,
                 const QuadArgs& qa, hipStream_t stream) {
     using V4 = typename Vec4<T>::type;
-    const int segs_per_row = (s.nx + C - 1) / C;
-    const int64_t n_segs = s.n_rows * segs_per_row;
+    int segs_per_row = 0;
+    const int64_t n_segs = adj_segments(s, C, adj_group(C, sizeof(T), quad), &segs_per_row);
     const int blocks_x = (int)((n_segs + 63) / 64);
     const int tiles = s.m_pad / 256;
     const int per_split = ((tiles + splits - 1) / splits) * 256;
@@ -726,10 +765,11 @@ size_t pm_fwd_workspace(const PmShape& s, const PmTuning& t, int precision, bool
     return (size_t)n_slices * (size_t)s.m_pad;
 }
 
-size_t pm_adj_workspace(const PmShape& s, const PmTuning& t, int precision, bool near) {
+size_t pm_adj_workspace(const PmShape& s, const PmTuning& t, int precision, bool near,
+                        bool quad) {
     // near: a correction will run, so the main kernel writes partials
-    (void)precision;
-    const int sp = adj_splits(s, t);
+    const size_t elem = precision == ZTM_FP64 ? sizeof(double) : sizeof(float);
+    const int sp = adj_splits(s, t, elem, quad && precision == ZTM_FP32);
     return (sp > 1 || near) ? (size_t)sp * (size_t)s.n_cells : 0;
 }
 
@@ -751,7 +791,6 @@ int pm_fwd_launch(const PmShape& s, const PmTuning& t0, int precision,
     const int rps = fwd_rows_per_slice(s, t, elem, quad);
     const int n_slices = (int)((s.n_rows + rps - 1) / rps);
     if (n_slices > 65535) return fail(ZTM_EINVAL, "too many row slices (%d)", n_slices);
-    na.sep = fwd_group(t.st_per_lane, elem, quad) > 1 ? 1 : 0;
     {
         ProfScope prof("pm_fwd", stream);
         if (precision == ZTM_FP64) {
@@ -784,10 +823,11 @@ int pm_adj_launch(const PmShape& s, const PmTuning& t, int precision,
                   double scale, double* part, void* out, const NearArgs& near,
                   hipStream_t stream) {
     if (s.n_cells == 0) return ZTM_OK;
-    const int splits = adj_splits(s, t);
     NearArgs na = near;
     if (precision == ZTM_FP64) na.mode = kCorrNone;
     const bool quad = na.quad != 0 && precision == ZTM_FP32;
+    const size_t elem = precision == ZTM_FP64 ? sizeof(double) : sizeof(float);
+    const int splits = adj_splits(s, t, elem, quad);
     const int direct = (splits == 1 && na.mode == kCorrNone) ? 1 : 0;
     {
         ProfScope prof("pm_adj", stream);
@@ -830,7 +870,7 @@ int pm_adj_launch(const PmShape& s, const PmTuning& t, int precision,
 int pm_grouped_workspace(ztm_plan p) {
     const PmTuning t = effective_tuning(p, ZTM_FP32);
     return ensure_workspace(p, std::max(pm_fwd_workspace(p->shape, t, ZTM_FP32, false),
-                                        pm_adj_workspace(p->shape, t, ZTM_FP32, true)));
+                                        pm_adj_workspace(p->shape, t, ZTM_FP32, true, false)));
 }
 
 int pm_fwd_desc(ztm_plan p, int64_t cell_off, int64_t st_off, GFwdDesc* d, size_t* lds,
@@ -862,7 +902,6 @@ int pm_fwd_desc(ztm_plan p, int64_t cell_off, int64_t st_off, GFwdDesc* d, size_
     d->thr = (float)p->prob.threshold;
     d->scale = pm_scale(p);
     d->na = near_args(p);
-    d->na.sep = G > 1 ? 1 : 0;
     *lds = std::max((size_t)((rps + G - 1) / G * G) * nxp * sizeof(float),
                     (size_t)4 * t.st_per_lane * 64 * sizeof(double));
     *blocks = s.m == 0 ? 0 : (int64_t)d->tiles_x * d->n_slices;
@@ -876,9 +915,9 @@ int pm_adj_desc(ztm_plan p, int64_t cell_off, int64_t st_off, GAdjDesc* d, size_
     const int C = t.cells_per_lane;
     if (C != 8 && C != 16)
         return fail(ZTM_EINVAL, "grouped adjoint needs cells_per_lane 8 or 16 (got %d)", C);
-    const int splits = adj_splits(s, t);
-    const int segs_per_row = (s.nx + C - 1) / C;
-    const int64_t n_segs = s.n_rows * segs_per_row;
+    const int splits = adj_splits(s, t, sizeof(float), false);
+    int segs_per_row = 0;
+    const int64_t n_segs = adj_segments(s, C, adj_group(C, sizeof(float), false), &segs_per_row);
     const int tiles = s.m_pad / 256;
     d->st = p->st_f;
     d->part = p->part;

@@ -54,15 +54,9 @@ __global__ __launch_bounds__(256) void prism_near_fwd_kernel(NearArgs na,
         const int kk = k0 + rest / bj;
         const float w = rho[((size_t)kk * na.ny + jj) * na.nx + ii];
         if (!(fabsf(w) > na.thr)) continue;
-        if (!QUAD && na.sep)
-            acc += corr_pair<QUAD, true>(kCorrPrism, w, pf, pd, c, ii, jj, kk, na);
-        else
-            acc += corr_pair<QUAD, false>(kCorrPrism, w, pf, pd, c, ii, jj, kk, na);
+        acc += corr_pair<QUAD>(kCorrPrism, w, pf, pd, c, ii, jj, kk, na);
     }
-    if (!QUAD && na.sep)
-        acc += clamped_rows_corr<QUAD, true>(kCorrPrism, pf, pd, c, rho, na, lane, 64);
-    else
-        acc += clamped_rows_corr<QUAD, false>(kCorrPrism, pf, pd, c, rho, na, lane, 64);
+    acc += clamped_rows_corr<QUAD>(kCorrPrism, pf, pd, c, rho, na, lane, 64);
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
     if (lane == 0) corr[st] = acc;

@@ -511,7 +511,7 @@ class Batch:
                                     _stream(stream)))
         return x, rr
 
-    def set_tuning(self, problem=-1, st_per_lane=4, rows_per_slice=0, cells_per_lane=16,
+    def set_tuning(self, problem=-1, st_per_lane=2, rows_per_slice=0, cells_per_lane=8,
                    adj_splits=0) -> None:
         check(lib().ztm_batch_set_tuning(self._h, problem, st_per_lane, rows_per_slice,
                                          cells_per_lane, adj_splits))
