@@ -189,7 +189,7 @@ namespace {
 struct PassBuf {
     GroupedPass gp;
     void* desc = nullptr;
-    int64_t* starts = nullptr;   // blk_start[np+1] then item_start[np+1]
+    int64_t* starts = nullptr;   // blk_start[np+1], item_start[np+1], near_start[np+1]
     void release() {
         if (desc) (void)hipFree(desc);
         if (starts) (void)hipFree(starts);
@@ -273,7 +273,7 @@ int build_pass(ztm_batch b, const std::vector<int>& members, bool fwd, PassBuf& 
     const int np = (int)members.size();
     std::vector<GFwdDesc> fd(np);
     std::vector<GAdjDesc> ad(np);
-    std::vector<int64_t> blocks(np), items(np);
+    std::vector<int64_t> blocks(np), items(np), nears(np, 0);
     std::vector<double> cost(np);
     size_t lds = 0;
     int maxv = 0;
@@ -292,6 +292,7 @@ int build_pass(ztm_batch b, const std::vector<int>& members, bool fwd, PassBuf& 
             if (int rc = pm_adj_desc(p, b->cell_off[s], b->st_off[s], &ad[a], &l, &blocks[a]))
                 return rc;
             items[a] = p->shape.n_cells;
+            nears[a] = ad[a].n_near;
             cost[a] = 64.0 * ad[a].C * ad[a].st_per_split;
             maxv = std::max(maxv, ad[a].C);
         }
@@ -300,13 +301,14 @@ int build_pass(ztm_batch b, const std::vector<int>& members, bool fwd, PassBuf& 
     std::vector<int> order(np);
     std::iota(order.begin(), order.end(), 0);
     std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return cost[x] > cost[y]; });
-    std::vector<int64_t> starts(2 * (np + 1), 0);
+    std::vector<int64_t> starts(3 * (np + 1), 0);
     std::vector<GFwdDesc> fo;
     std::vector<GAdjDesc> ao;
     for (int a = 0; a < np; ++a) {
         const int o = order[a];
         starts[a + 1] = starts[a] + blocks[o];
         starts[np + 1 + a + 1] = starts[np + 1 + a] + items[o];
+        starts[2 * (np + 1) + a + 1] = starts[2 * (np + 1) + a] + nears[o];
         if (fwd) fo.push_back(fd[o]); else ao.push_back(ad[o]);
     }
     const size_t desc_bytes = fwd ? np * sizeof(GFwdDesc) : np * sizeof(GAdjDesc);
@@ -319,11 +321,13 @@ int build_pass(ztm_batch b, const std::vector<int>& members, bool fwd, PassBuf& 
     pb.gp.desc = pb.desc;
     pb.gp.blk_start = pb.starts;
     pb.gp.item_start = pb.starts + (np + 1);
+    pb.gp.near_start = pb.starts + 2 * (np + 1);
     pb.gp.ticket = b->ticket + (fwd ? 0 : 1);
     pb.gp.np = np;
     pb.gp.max_variant = maxv;
     pb.gp.total_blocks = starts[np];
     pb.gp.total_items = starts[2 * np + 1];
+    pb.gp.total_near = starts[3 * np + 2];
     pb.gp.lds = lds;
     if (lds > 65536) return fail(ZTM_EINVAL, "grouped pass needs %zu bytes of LDS", lds);
     return ZTM_OK;

@@ -88,6 +88,25 @@ struct NearArgs {
     int kmin = 0, kmax = -1;
 };
 
+// Cell layers [*klo, *khi] of an nz-layer grid in which the adjoint gather
+// may apply a near-field correction (empty, *klo > *khi, without one).
+// Cells outside them take the plain split-K sum.
+inline void near_layers(const NearArgs& na, int nz, int* klo, int* khi) {
+    int lo = 1 << 30, hi = -(1 << 30);
+    if (na.mode != kCorrNone) {
+        if (na.kmin <= na.kmax) {
+            lo = na.kmin - na.rz;
+            hi = na.kmax + na.rz;
+        }
+        if (na.row_start && na.rkmin <= na.rkmax) {
+            lo = lo < na.rkmin - na.kr ? lo : na.rkmin - na.kr;
+            hi = hi > na.rkmax + na.kr ? hi : na.rkmax + na.kr;
+        }
+    }
+    *klo = lo < 0 ? 0 : lo;
+    *khi = hi > nz - 1 ? nz - 1 : hi;
+}
+
 // Launch shapes of the point-mass kernels.
 struct PmTuning {
     int st_per_lane = 2;      // forward: stations per lane (1,2,4,8)
@@ -158,7 +177,9 @@ struct GAdjDesc {
     double* part;
     int64_t cell_off, st_off;
     int m, nx, ny, segs_per_row, st_per_split, m_pad, splits, blocks_x, C;
+    int klo, khi;            // near-field layers (near_layers)
     int64_t n_segs, n_cells;
+    int64_t near_off, n_near; // cells of layers klo..khi: first cell, count
     float sy, sz;
     double scale;
     NearArgs na;
@@ -167,11 +188,12 @@ struct GroupedPass {
     const void* desc = nullptr;          // GFwdDesc[np] or GAdjDesc[np] (device)
     const int64_t* blk_start = nullptr;  // device, np + 1
     const int64_t* item_start = nullptr; // device, np + 1
+    const int64_t* near_start = nullptr; // device, np + 1 (adjoint: near-field cells)
     unsigned* ticket = nullptr;          // device counter, zero between passes
     int np = 0;
     int max_variant = 0;                 // largest S (forward) or C (adjoint) present
     int per_cu = 8;                      // resident workgroups per CU launched
-    int64_t total_blocks = 0, total_items = 0;
+    int64_t total_blocks = 0, total_items = 0, total_near = 0;
     size_t lds = 0;
 };
 // Sizes the plan's workspace for both grouped passes (call before building

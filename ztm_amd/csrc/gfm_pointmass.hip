@@ -441,78 +441,110 @@ __global__ __launch_bounds__(256) void pm_adj_kernel(
                              part, out, scale, n_cells, direct, qa, blockIdx.x, blockIdx.y, smem);
 }
 
-// out[n] = scale * (sum_splits part[split][n] + correction); the correction
-// gathers, per cell, the stations whose column lies within na.radius of the
-// cell's column (fixed CSR order, so the result is deterministic).
-template <typename T, bool QUAD>
-__device__ __forceinline__ void pm_adj_reduce_one(const double* __restrict__ part, int n_splits,
-                                                  int64_t n_cells, int64_t n, double scale,
-                                                  T* __restrict__ out,
-                                                  const float* __restrict__ r_f,
-                                                  const NearArgs& na) {
+// One wave per cell of the near-field layers: out[n] = scale * (sum over
+// splits of part[split][n] + correction). The lanes stride over the split
+// partials, over the station columns within (rx, ry) of the cell's column
+// (stations of a column in CSR order) and over the row keys of stations with
+// clamped FP32 row terms; a fixed butterfly then sums the lanes, so the
+// result is deterministic. Every lane returns it. MODE (kCorrPmNear or
+// kCorrPrism) is na.mode fixed at compile time, which keeps the FP64 prism
+// formula out of the point-mass kernel's registers.
+template <bool QUAD, int MODE>
+__device__ __forceinline__ double pm_adj_reduce_wave(const double* __restrict__ part,
+                                                     int n_splits, int64_t n_cells, int64_t n,
+                                                     const float* __restrict__ r_f,
+                                                     const NearArgs& na, int lane) {
+    double s = 0.0;
+    for (int k = lane; k < n_splits; k += 64) s += part[(size_t)k * n_cells + n];
+    const int ii = (int)(n % na.nx);
+    const int64_t rw = n / na.nx;
+    const int jj = (int)(rw % na.ny);
+    const int kk = (int)(rw / na.ny);
+    // (a) stations whose correction box holds this cell: via the column index
+    if (kk >= na.kmin - na.rz && kk <= na.kmax + na.rz) {
+        const int wcols = na.nx + 2 * na.cmx;
+        const int bw = 2 * na.rx + 1;
+        const int total = bw * (2 * na.ry + 1);
+        for (int t = lane; t < total; t += 64) {
+            const int ci = ii - na.rx + t % bw;
+            const int cj = jj - na.ry + t / bw;
+            if (cj < -na.cmy || cj >= na.ny + na.cmy) continue;
+            if (ci < -na.cmx || ci >= na.nx + na.cmx) continue;
+            const int col = (cj + na.cmy) * wcols + (ci + na.cmx);
+            for (int q = na.col_start[col]; q < na.col_start[col + 1]; ++q) {
+                const int m = na.col_st[q];
+                const int4 c = na.st_c[m];
+                if (c.z < kk - na.rz || c.z > kk + na.rz) continue;
+                s += corr_pair<QUAD>(MODE, r_f[m], na.st_f[m], na.st_d[m], c, ii, jj, kk, na);
+            }
+        }
+    }
+    // (b) stations whose FP32 row term for this cell's row was clamped,
+    //     outside (a): via the row-key index
+    if (na.row_start && kk >= na.rkmin - na.kr && kk <= na.rkmax + na.kr) {
+        const int wkey = na.ny + 2 * na.rmy;
+        const int bj = 2 * na.jr + 1;
+        const int total = bj * (2 * na.kr + 1);
+        for (int t = lane; t < total; t += 64) {
+            const int cj = jj - na.jr + t % bj;
+            const int ck = kk - na.kr + t / bj;
+            if (ck < -na.rmz || ck >= na.nz + na.rmz) continue;
+            if (cj < -na.rmy || cj >= na.ny + na.rmy) continue;
+            const int key = (ck + na.rmz) * wkey + (cj + na.rmy);
+            for (int q = na.row_start[key]; q < na.row_start[key + 1]; ++q) {
+                const int m = na.row_st[q];
+                const int4 c = na.st_c[m];
+                if (abs(ii - c.x) <= na.rx && abs(jj - c.y) <= na.ry && abs(kk - c.z) <= na.rz)
+                    continue;   // done in (a)
+                const float4 pf = na.st_f[m];
+                if (!row_clamped(pf.y - (float)jj * na.syf, pf.z - (float)kk * na.szf)) continue;
+                s += corr_pair<QUAD>(MODE, r_f[m], pf, na.st_d[m], c, ii, jj, kk, na);
+            }
+        }
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+    return s;
+}
+
+// Cells outside the near-field layers [klo, khi]: the split-K sum only. The
+// near-field walk is kept out of this kernel so that it runs at full
+// occupancy; pm_adj_reduce_kernel does the layers klo..khi.
+template <typename T>
+__device__ __forceinline__ void pm_adj_sum_one(const double* __restrict__ part, int n_splits,
+                                               int64_t n_cells, int64_t n, double scale,
+                                               T* __restrict__ out) {
     double s = 0.0;
     for (int k = 0; k < n_splits; ++k) s += part[(size_t)k * n_cells + n];
-    if (na.mode != kCorrNone) {
-        const int ii = (int)(n % na.nx);
-        const int64_t rw = n / na.nx;
-        const int jj = (int)(rw % na.ny);
-        const int kk = (int)(rw / na.ny);
-        double corr = 0.0;
-        // (a) stations whose correction box holds this cell: via the column index
-        if (kk >= na.kmin - na.rz && kk <= na.kmax + na.rz) {
-            const int wcols = na.nx + 2 * na.cmx;
-            for (int cj = jj - na.ry; cj <= jj + na.ry; ++cj) {
-                if (cj < -na.cmy || cj >= na.ny + na.cmy) continue;
-                for (int ci = ii - na.rx; ci <= ii + na.rx; ++ci) {
-                    if (ci < -na.cmx || ci >= na.nx + na.cmx) continue;
-                    const int col = (cj + na.cmy) * wcols + (ci + na.cmx);
-                    for (int q = na.col_start[col]; q < na.col_start[col + 1]; ++q) {
-                        const int m = na.col_st[q];
-                        const int4 c = na.st_c[m];
-                        if (c.z < kk - na.rz || c.z > kk + na.rz) continue;
-                        corr += corr_pair<QUAD>(na.mode, r_f[m], na.st_f[m], na.st_d[m], c, ii,
-                                                jj, kk, na);
-                    }
-                }
-            }
-        }
-        // (b) stations whose FP32 row term for this cell's row was clamped,
-        //     outside (a): via the row-key index
-        if (na.row_start && kk >= na.rkmin - na.kr && kk <= na.rkmax + na.kr) {
-            const int wkey = na.ny + 2 * na.rmy;
-            for (int ck = kk - na.kr; ck <= kk + na.kr; ++ck) {
-                if (ck < -na.rmz || ck >= na.nz + na.rmz) continue;
-                for (int cj = jj - na.jr; cj <= jj + na.jr; ++cj) {
-                    if (cj < -na.rmy || cj >= na.ny + na.rmy) continue;
-                    const int key = (ck + na.rmz) * wkey + (cj + na.rmy);
-                    for (int q = na.row_start[key]; q < na.row_start[key + 1]; ++q) {
-                        const int m = na.row_st[q];
-                        const int4 c = na.st_c[m];
-                        if (abs(ii - c.x) <= na.rx && abs(jj - c.y) <= na.ry &&
-                            abs(kk - c.z) <= na.rz)
-                            continue;   // done in (a)
-                        const float4 pf = na.st_f[m];
-                        if (!row_clamped(pf.y - (float)jj * na.syf, pf.z - (float)kk * na.szf))
-                            continue;
-                        corr += corr_pair<QUAD>(na.mode, r_f[m], pf, na.st_d[m], c, ii, jj, kk,
-                                                na);
-                    }
-                }
-            }
-        }
-        s += corr;
-    }
     out[n] = (T)(scale * s);
 }
 
-template <typename T, bool QUAD>
-__global__ __launch_bounds__(256) void pm_adj_reduce_kernel(const double* __restrict__ part,
-                                     int n_splits, int64_t n_cells,
-                                     double scale, T* __restrict__ out,
-                                     const float* __restrict__ r_f, NearArgs na) {
+template <typename T>
+__global__ __launch_bounds__(256) void pm_adj_sum_kernel(const double* __restrict__ part,
+                                                         int n_splits, int64_t n_cells,
+                                                         int64_t plane, int klo, int khi,
+                                                         double scale, T* __restrict__ out) {
     const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= n_cells) return;
-    pm_adj_reduce_one<T, QUAD>(part, n_splits, n_cells, n, scale, out, r_f, na);
+    const int64_t kk = n / plane;
+    if (kk >= klo && kk <= khi) return;
+    pm_adj_sum_one<T>(part, n_splits, n_cells, n, scale, out);
+}
+
+// Cells n0 .. n0+count-1 (the near-field layers), one wave each (block =
+// 256 = 4 cells): sum plus corrections.
+template <bool QUAD, int MODE>
+__global__ __launch_bounds__(256) void pm_adj_reduce_kernel(const double* __restrict__ part,
+                                     int n_splits, int64_t n_cells, int64_t n0, int64_t count,
+                                     double scale, float* __restrict__ out,
+                                     const float* __restrict__ r_f, NearArgs na) {
+    const int lane = threadIdx.x & 63;
+    for (int64_t t = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); t < count;
+         t += (int64_t)gridDim.x * 4) {   // uniform per wave
+        const double v = pm_adj_reduce_wave<QUAD, MODE>(part, n_splits, n_cells, n0 + t, r_f, na,
+                                                        lane);
+        if (lane == 0) out[n0 + t] = (float)(scale * v);
+    }
 }
 
 
@@ -618,17 +650,37 @@ __global__ __launch_bounds__(256) void pm_adj_grouped_kernel(
     }
 }
 
-__global__ __launch_bounds__(256) void pm_adj_reduce_grouped_kernel(
+// Grouped adjoint reduce, in two launches like the single-problem one: the
+// split-K sum over every cell outside its problem's near-field layers (this
+// launch also resets the ticket counter), then the near-field layers.
+__global__ __launch_bounds__(256) void pm_adj_sum_grouped_kernel(
     const GAdjDesc* __restrict__ d, const int64_t* __restrict__ cell_start, int np,
-    int64_t total, const float* __restrict__ r, float* __restrict__ out,
-    unsigned* __restrict__ ticket) {
+    int64_t total, float* __restrict__ out, unsigned* __restrict__ ticket) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t == 0) *ticket = 0u;
     if (t >= total) return;
     const int p = find_problem(cell_start, np, t);
     const GAdjDesc& q = d[p];
-    pm_adj_reduce_one<float, false>(q.part, q.splits, q.n_cells, t - cell_start[p], q.scale,
-                                    out + q.cell_off, r + q.st_off, q.na);
+    const int64_t n = t - cell_start[p];
+    const int64_t kk = n / ((int64_t)q.nx * q.ny);
+    if (kk >= q.klo && kk <= q.khi) return;
+    pm_adj_sum_one<float>(q.part, q.splits, q.n_cells, n, q.scale, out + q.cell_off);
+}
+
+// One wave per near-field cell (block = 256 = 4 cells).
+__global__ __launch_bounds__(256) void pm_adj_reduce_grouped_kernel(
+    const GAdjDesc* __restrict__ d, const int64_t* __restrict__ near_start, int np,
+    int64_t total, const float* __restrict__ r, float* __restrict__ out) {
+    const int lane = threadIdx.x & 63;
+    for (int64_t t = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); t < total;
+         t += (int64_t)gridDim.x * 4) {   // uniform per wave
+        const int p = find_problem(near_start, np, t);
+        const GAdjDesc& q = d[p];
+        const int64_t n = q.near_off + (t - near_start[p]);
+        const double v = pm_adj_reduce_wave<false, kCorrPmNear>(q.part, q.splits, q.n_cells, n,
+                                                                r + q.st_off, q.na, lane);
+        if (lane == 0) out[q.cell_off + n] = (float)(q.scale * v);
+    }
 }
 
 // ------------------------------------------------------- station prep
@@ -863,20 +915,31 @@ int pm_adj_launch(const PmShape& s, const PmTuning& t, int precision,
         ZTM_HIP(hipGetLastError());
     }
     if (!direct) {
+        int klo = 0, khi = -1;
+        near_layers(na, s.nz, &klo, &khi);
+        const int64_t plane = (int64_t)s.nx * s.ny;
         const unsigned blocks = (unsigned)((s.n_cells + 255) / 256);
         if (precision == ZTM_FP64)
-            hipLaunchKernelGGL((pm_adj_reduce_kernel<double, false>), dim3(blocks), dim3(256), 0,
-                               stream, part, splits, s.n_cells, scale, (double*)out,
-                               (const float*)nullptr, na);
-        else if (quad)
-            hipLaunchKernelGGL((pm_adj_reduce_kernel<float, true>), dim3(blocks), dim3(256), 0,
-                               stream, part, splits, s.n_cells, scale, (float*)out,
-                               (const float*)r, na);
+            hipLaunchKernelGGL((pm_adj_sum_kernel<double>), dim3(blocks), dim3(256), 0, stream,
+                               part, splits, s.n_cells, plane, klo, khi, scale, (double*)out);
         else
-            hipLaunchKernelGGL((pm_adj_reduce_kernel<float, false>), dim3(blocks), dim3(256), 0,
-                               stream, part, splits, s.n_cells, scale, (float*)out,
-                               (const float*)r, na);
+            hipLaunchKernelGGL((pm_adj_sum_kernel<float>), dim3(blocks), dim3(256), 0, stream,
+                               part, splits, s.n_cells, plane, klo, khi, scale, (float*)out);
         ZTM_HIP(hipGetLastError());
+        if (klo <= khi && precision == ZTM_FP32) {
+            const int64_t n0 = klo * plane, count = (int64_t)(khi - klo + 1) * plane;
+            const unsigned nb = (unsigned)std::min<int64_t>((count + 3) / 4, 1 << 20);
+#define ZTM_ADJ_NEAR(Q, M)                                                                 \
+    hipLaunchKernelGGL((pm_adj_reduce_kernel<Q, M>), dim3(nb), dim3(256), 0, stream, part,  \
+                       splits, s.n_cells, n0, count, scale, (float*)out, (const float*)r, na)
+            if (na.mode == kCorrPrism) {
+                if (quad) ZTM_ADJ_NEAR(true, kCorrPrism); else ZTM_ADJ_NEAR(false, kCorrPrism);
+            } else {
+                if (quad) ZTM_ADJ_NEAR(true, kCorrPmNear); else ZTM_ADJ_NEAR(false, kCorrPmNear);
+            }
+#undef ZTM_ADJ_NEAR
+            ZTM_HIP(hipGetLastError());
+        }
     }
     return ZTM_OK;
 }
@@ -954,6 +1017,10 @@ int pm_adj_desc(ztm_plan p, int64_t cell_off, int64_t st_off, GAdjDesc* d, size_
     d->sz = (float)p->sz;
     d->scale = pm_scale(p);
     d->na = near_args(p);
+    near_layers(d->na, s.nz, &d->klo, &d->khi);
+    const int64_t plane = (int64_t)s.nx * s.ny;
+    d->near_off = d->klo <= d->khi ? d->klo * plane : 0;
+    d->n_near = d->klo <= d->khi ? (int64_t)(d->khi - d->klo + 1) * plane : 0;
     *lds = std::max((size_t)256 * sizeof(float4), (size_t)4 * C * 64 * sizeof(double));
     *blocks = (int64_t)d->blocks_x * splits;
     return ZTM_OK;
@@ -1009,10 +1076,18 @@ int pm_adj_grouped_launch(const GroupedPass& gp, const float* r, float* out,
         ZTM_HIP(hipGetLastError());
     }
     const int64_t items = std::max<int64_t>(gp.total_items, 1);
-    hipLaunchKernelGGL(pm_adj_reduce_grouped_kernel, dim3((unsigned)((items + 255) / 256)),
+    hipLaunchKernelGGL(pm_adj_sum_grouped_kernel, dim3((unsigned)((items + 255) / 256)),
                        dim3(256), 0, stream, (const GAdjDesc*)gp.desc, gp.item_start, gp.np,
-                       gp.total_items, r, out, gp.ticket);
+                       gp.total_items, out, gp.ticket);
     ZTM_HIP(hipGetLastError());
+    if (gp.total_near > 0) {
+        hipLaunchKernelGGL(pm_adj_reduce_grouped_kernel,
+                           dim3((unsigned)std::min<int64_t>((gp.total_near + 3) / 4, 1 << 20)),
+                           dim3(256), 0, stream,
+                           (const GAdjDesc*)gp.desc, gp.near_start, gp.np, gp.total_near, r,
+                           out);
+        ZTM_HIP(hipGetLastError());
+    }
     return ZTM_OK;
 }
 
