@@ -14,8 +14,8 @@
 // (TFM.cpp:56, SET.cpp:86-99). All FP64 in the reference's operation order,
 // FMA contraction off, so drho equals the reference's value bit for bit; it
 // is written as float, the narrowing the reference applies before its GPU
-// call (TFM.cpp:91). One thread per fine cell; the coarse grids are read
-// through the cache (the kernel is HBM-bound on its 4-byte output).
+// call (TFM.cpp:91). One thread per fine cell (and, batched, per group of
+// variants); the coarse grids are read through the cache.
 #include "gfm_internal.h"
 
 #include <cmath>
@@ -68,19 +68,35 @@ __device__ __forceinline__ double depth_km(const TfmFrame& fr, double ci, double
     return 6371.0 - r;
 }
 
-__device__ __forceinline__ float drho_at(int64_t n, int di, int dj, int dk, int fh, int fv,
-                                         const TfmFrame& fr, const double* __restrict__ sea,
-                                         const double* __restrict__ moho,
-                                         const double* __restrict__ vel, double vm, double ro,
-                                         double dr) {
+// The per-cell terms that do not depend on the model parameters: the
+// coarse-index coordinates, the reference Moho m0 and the depth filter.
+struct DrhoCell {
+    double i, j, k, li, m0;
+    bool deep;
+};
+
+__device__ __forceinline__ DrhoCell drho_cell(int64_t n, int di, int dj, int fh, int fv,
+                                              const TfmFrame& fr,
+                                              const double* __restrict__ sea) {
 #pragma clang fp contract(off)
     const int ii = (di - 1) * fh + 1, jj = (dj - 1) * fh + 1;
-    const double i = (double)(n % ii) / (1. * fh), j = (double)((n / ii) % jj) / (1. * fh),
-                 k = (double)(n / ((int64_t)ii * jj)) / (1. * fv), li = 1. / fv;
-    const double m0 = interp2_unit(sea, i, j, di, dj) - 3.5;
-    const double m1 = interp2_unit(moho, i, j, di, dj);
+    DrhoCell c;
+    c.i = (double)(n % ii) / (1. * fh);
+    c.j = (double)((n / ii) % jj) / (1. * fh);
+    c.k = (double)(n / ((int64_t)ii * jj)) / (1. * fv);
+    c.li = 1. / fv;
+    c.m0 = interp2_unit(sea, c.i, c.j, di, dj) - 3.5;
+    c.deep = depth_km(fr, c.i, c.j, c.k) > 10;
+    return c;
+}
+
+// drho of one cell from its rendered Moho m1 and velocity v (TFM.cpp:86-89).
+__device__ __forceinline__ float drho_from(const DrhoCell& c, double m1, double v, double vm,
+                                           double ro, double dr) {
+#pragma clang fp contract(off)
+    const double k = c.k, li = c.li, m0 = c.m0;
     double r0 = 0;
-    double r1 = dr * (interp3_unit(vel, i, j, k, di, dj, dk) - vm);
+    double r1 = dr * (v - vm);
     if (k > m0 + li) {
         r0 = ro;
     } else if (fabs(k - m0) < li) {
@@ -92,8 +108,19 @@ __device__ __forceinline__ float drho_at(int64_t n, int di, int dj, int dk, int 
         r1 = r1 + (k - m1 + li) * (r1 + ro) / (2 * li);
     }
     float out = 0.0f;
-    if (fabs(r0 - r1) > .001 && depth_km(fr, i, j, k) > 10) out = (float)(r1 - r0);
+    if (fabs(r0 - r1) > .001 && c.deep) out = (float)(r1 - r0);
     return out;
+}
+
+__device__ __forceinline__ float drho_at(int64_t n, int di, int dj, int dk, int fh, int fv,
+                                         const TfmFrame& fr, const double* __restrict__ sea,
+                                         const double* __restrict__ moho,
+                                         const double* __restrict__ vel, double vm, double ro,
+                                         double dr) {
+    const DrhoCell c = drho_cell(n, di, dj, fh, fv, fr, sea);
+    const double m1 = interp2_unit(moho, c.i, c.j, di, dj);
+    const double v = interp3_unit(vel, c.i, c.j, c.k, di, dj, dk);
+    return drho_from(c, m1, v, vm, ro, dr);
 }
 
 __global__ __launch_bounds__(256) void tfm_drho_kernel(int di, int dj, int dk, int fh, int fv,
@@ -108,18 +135,33 @@ __global__ __launch_bounds__(256) void tfm_drho_kernel(int di, int dj, int dk, i
     drho[n] = drho_at(n, di, dj, dk, fh, fv, fr, sea, moho, vel, vm, ro, dr);
 }
 
-// Variant b = blockIdx.y: its own Moho and velocity grids (strides in
-// elements, 0 = shared) and (vm, ro, dr) = params[3b .. 3b+2]; drho [B][N].
+// Variants kDrhoVariants * blockIdx.y onwards, each with its own Moho and
+// velocity grids (strides in elements, 0 = shared) and (vm, ro, dr) =
+// params[3b .. 3b+2]; drho [B][N]. The parameter-free terms of a cell, and a
+// shared grid's interpolation, are formed once per thread for all its
+// variants (the same operations, so each variant equals drho_at).
+constexpr int kDrhoVariants = 16;
+
 __global__ __launch_bounds__(256) void tfm_drho_batched_kernel(
     int di, int dj, int dk, int fh, int fv, TfmFrame fr, const double* __restrict__ sea,
     const double* __restrict__ moho, int64_t moho_stride, const double* __restrict__ vel,
-    int64_t vel_stride, const double* __restrict__ params, float* __restrict__ drho, int64_t N) {
+    int64_t vel_stride, const double* __restrict__ params, float* __restrict__ drho, int64_t N,
+    int B) {
     const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= N) return;
-    const int b = blockIdx.y;
-    drho[(size_t)b * N + n] =
-        drho_at(n, di, dj, dk, fh, fv, fr, sea, moho + b * moho_stride, vel + b * vel_stride,
-                params[3 * b], params[3 * b + 1], params[3 * b + 2]);
+    const DrhoCell c = drho_cell(n, di, dj, fh, fv, fr, sea);
+    const double m1s = moho_stride == 0 ? interp2_unit(moho, c.i, c.j, di, dj) : 0.0;
+    const double vs = vel_stride == 0 ? interp3_unit(vel, c.i, c.j, c.k, di, dj, dk) : 0.0;
+    const int b0 = blockIdx.y * kDrhoVariants;
+    const int b1 = min(b0 + kDrhoVariants, B);
+    for (int b = b0; b < b1; ++b) {
+        const double m1 =
+            moho_stride == 0 ? m1s : interp2_unit(moho + b * moho_stride, c.i, c.j, di, dj);
+        const double v =
+            vel_stride == 0 ? vs : interp3_unit(vel + b * vel_stride, c.i, c.j, c.k, di, dj, dk);
+        drho[(size_t)b * N + n] =
+            drho_from(c, m1, v, params[3 * b], params[3 * b + 1], params[3 * b + 2]);
+    }
 }
 
 // g[b][*] = 0, then g[b][out_map[i]] = Y[b][i] for the m prepared stations.
@@ -180,9 +222,10 @@ int tfm_drho_batched_launch(int di, int dj, int dk, int fh, int fv, int B, const
     if (N == 0 || B <= 0) return ZTM_OK;
     const TfmFrame fr = tfm_frame(rc);
     ProfScope prof("tfm_drho", stream);
-    hipLaunchKernelGGL(tfm_drho_batched_kernel, dim3((unsigned)((N + 255) / 256), (unsigned)B),
+    const unsigned gy = (unsigned)((B + kDrhoVariants - 1) / kDrhoVariants);
+    hipLaunchKernelGGL(tfm_drho_batched_kernel, dim3((unsigned)((N + 255) / 256), gy),
                        dim3(256), 0, stream, di, dj, dk, fh, fv, fr, sea, moho, moho_stride, vel,
-                       vel_stride, params, drho, N);
+                       vel_stride, params, drho, N, B);
     ZTM_HIP(hipGetLastError());
     return ZTM_OK;
 }
