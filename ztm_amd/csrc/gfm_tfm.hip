@@ -22,40 +22,97 @@
 
 namespace ztm {
 
-__device__ __forceinline__ double interp2_unit(const double* __restrict__ f, double x, double y,
-                                               int di, int dj) {
+// Unit-spacing bilinear interpolation (zmath.cpp:225-236) in two parts: the
+// weights and corner offsets of a point, which depend on the point only, and
+// the evaluation on one grid, so that several grids sampled at the same point
+// (the variants of a batch) share the first part. The operations and their
+// order are those of one plain interpolation.
+struct Interp2W {
+    int64_t b;                  // offset of the (i, j) corner
+    int di;                     // row stride
+    double si, sj, ri, rj;      // fractions and 1 - fractions
+    bool hi, hj;                // the i+1 / j+1 neighbours lie inside the grid
+};
+
+__device__ __forceinline__ Interp2W interp2_weights(double x, double y, int di, int dj) {
 #pragma clang fp contract(off)
     const double ii = fmin(fmax(x / 1.0, 0.0), (double)(di - 1));
     const double jj = fmin(fmax(y / 1.0, 0.0), (double)(dj - 1));
     const int i = (int)ii, j = (int)jj;
-    const double si = ii - i, sj = jj - j;
-    // a neighbour outside the grid has weight exactly 0: read as 0
-    const double c00 = f[i + (j + 0) * di];
-    const double c10 = i + 1 < di ? f[i + 1 + (j + 0) * di] : 0.0;
-    const double c01 = j + 1 < dj ? f[i + (j + 1) * di] : 0.0;
-    const double c11 = (i + 1 < di && j + 1 < dj) ? f[i + 1 + (j + 1) * di] : 0.0;
-    const double c0 = c00 * (1 - si) + c10 * si, c1 = c01 * (1 - si) + c11 * si;
-    return c0 * (1 - sj) + c1 * sj;
+    Interp2W w;
+    w.b = i + (int64_t)j * di;
+    w.di = di;
+    w.si = ii - i;
+    w.sj = jj - j;
+    w.ri = 1 - w.si;
+    w.rj = 1 - w.sj;
+    w.hi = i + 1 < di;
+    w.hj = j + 1 < dj;
+    return w;
 }
 
-__device__ __forceinline__ double interp3_unit(const double* __restrict__ f, double x, double y,
-                                               double z, int di, int dj, int dk) {
+__device__ __forceinline__ double interp2_eval(const double* __restrict__ f, const Interp2W& w) {
+#pragma clang fp contract(off)
+    // a neighbour outside the grid has weight exactly 0: read as 0
+    const double* q = f + w.b;
+    const double c00 = q[0];
+    const double c10 = w.hi ? q[1] : 0.0;
+    const double c01 = w.hj ? q[w.di] : 0.0;
+    const double c11 = (w.hi && w.hj) ? q[w.di + 1] : 0.0;
+    const double c0 = c00 * w.ri + c10 * w.si, c1 = c01 * w.ri + c11 * w.si;
+    return c0 * w.rj + c1 * w.sj;
+}
+
+__device__ __forceinline__ double interp2_unit(const double* __restrict__ f, double x, double y,
+                                               int di, int dj) {
+    return interp2_eval(f, interp2_weights(x, y, di, dj));
+}
+
+// Trilinear counterpart (zmath.cpp:238-249); the base corner is clamped to
+// d-2 per axis, so all 8 corners lie inside the grid.
+struct Interp3W {
+    int64_t b, p;               // base corner offset, plane stride
+    int di;
+    double si, sj, sk, ri, rj, rk;
+};
+
+__device__ __forceinline__ Interp3W interp3_weights(double x, double y, double z, int di, int dj,
+                                                   int dk) {
 #pragma clang fp contract(off)
     const double ii = fmin(fmax(x / 1.0, 0.0), (double)(di - 2));
     const double jj = fmin(fmax(y / 1.0, 0.0), (double)(dj - 2));
     const double kk = fmin(fmax(z / 1.0, 0.0), (double)(dk - 2));
     const int i = (int)ii, j = (int)jj, k = (int)kk;
-    const double si = ii - i, sj = jj - j, sk = kk - k;
-    const int64_t p = (int64_t)di * dj;
-    const int64_t b = i + (int64_t)j * di + k * p;
-    const double c000 = f[b], c100 = f[b + 1];
-    const double c010 = f[b + di], c110 = f[b + di + 1];
-    const double c001 = f[b + p], c101 = f[b + p + 1];
-    const double c011 = f[b + p + di], c111 = f[b + p + di + 1];
-    const double c00 = c000 * (1 - si) + c100 * si, c01 = c001 * (1 - si) + c101 * si;
-    const double c10 = c010 * (1 - si) + c110 * si, c11 = c011 * (1 - si) + c111 * si;
-    const double c0 = c00 * (1 - sj) + c10 * sj, c1 = c01 * (1 - sj) + c11 * sj;
-    return c0 * (1 - sk) + c1 * sk;
+    Interp3W w;
+    w.p = (int64_t)di * dj;
+    w.b = i + (int64_t)j * di + k * w.p;
+    w.di = di;
+    w.si = ii - i;
+    w.sj = jj - j;
+    w.sk = kk - k;
+    w.ri = 1 - w.si;
+    w.rj = 1 - w.sj;
+    w.rk = 1 - w.sk;
+    return w;
+}
+
+__device__ __forceinline__ double interp3_eval(const double* __restrict__ f, const Interp3W& w) {
+#pragma clang fp contract(off)
+    const double* q = f + w.b;
+    const int64_t di = w.di, p = w.p;
+    const double c000 = q[0], c100 = q[1];
+    const double c010 = q[di], c110 = q[di + 1];
+    const double c001 = q[p], c101 = q[p + 1];
+    const double c011 = q[p + di], c111 = q[p + di + 1];
+    const double c00 = c000 * w.ri + c100 * w.si, c01 = c001 * w.ri + c101 * w.si;
+    const double c10 = c010 * w.ri + c110 * w.si, c11 = c011 * w.ri + c111 * w.si;
+    const double c0 = c00 * w.rj + c10 * w.sj, c1 = c01 * w.rj + c11 * w.sj;
+    return c0 * w.rk + c1 * w.sk;
+}
+
+__device__ __forceinline__ double interp3_unit(const double* __restrict__ f, double x, double y,
+                                               double z, int di, int dj, int dk) {
+    return interp3_eval(f, interp3_weights(x, y, z, di, dj, dk));
 }
 
 __device__ __forceinline__ double depth_km(const TfmFrame& fr, double ci, double cj, double ck) {
@@ -137,9 +194,10 @@ __global__ __launch_bounds__(256) void tfm_drho_kernel(int di, int dj, int dk, i
 
 // Variants kDrhoVariants * blockIdx.y onwards, each with its own Moho and
 // velocity grids (strides in elements, 0 = shared) and (vm, ro, dr) =
-// params[3b .. 3b+2]; drho [B][N]. The parameter-free terms of a cell, and a
-// shared grid's interpolation, are formed once per thread for all its
-// variants (the same operations, so each variant equals drho_at).
+// params[3b .. 3b+2]; drho [B][N]. The parameter-free terms of a cell, the
+// interpolation weights and a shared grid's interpolation are formed once per
+// thread for all its variants (the same operations, so each variant equals
+// drho_at).
 constexpr int kDrhoVariants = 16;
 
 __global__ __launch_bounds__(256) void tfm_drho_batched_kernel(
@@ -150,15 +208,15 @@ __global__ __launch_bounds__(256) void tfm_drho_batched_kernel(
     const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= N) return;
     const DrhoCell c = drho_cell(n, di, dj, fh, fv, fr, sea);
-    const double m1s = moho_stride == 0 ? interp2_unit(moho, c.i, c.j, di, dj) : 0.0;
-    const double vs = vel_stride == 0 ? interp3_unit(vel, c.i, c.j, c.k, di, dj, dk) : 0.0;
+    const Interp2W w2 = interp2_weights(c.i, c.j, di, dj);
+    const Interp3W w3 = interp3_weights(c.i, c.j, c.k, di, dj, dk);
+    const double m1s = moho_stride == 0 ? interp2_eval(moho, w2) : 0.0;
+    const double vs = vel_stride == 0 ? interp3_eval(vel, w3) : 0.0;
     const int b0 = blockIdx.y * kDrhoVariants;
     const int b1 = min(b0 + kDrhoVariants, B);
     for (int b = b0; b < b1; ++b) {
-        const double m1 =
-            moho_stride == 0 ? m1s : interp2_unit(moho + b * moho_stride, c.i, c.j, di, dj);
-        const double v =
-            vel_stride == 0 ? vs : interp3_unit(vel + b * vel_stride, c.i, c.j, c.k, di, dj, dk);
+        const double m1 = moho_stride == 0 ? m1s : interp2_eval(moho + b * moho_stride, w2);
+        const double v = vel_stride == 0 ? vs : interp3_eval(vel + b * vel_stride, w3);
         drho[(size_t)b * N + n] =
             drho_from(c, m1, v, params[3 * b], params[3 * b + 1], params[3 * b + 2]);
     }
