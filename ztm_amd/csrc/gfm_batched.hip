@@ -248,26 +248,32 @@ __global__ __launch_bounds__(256) void batched_fwd_kernel(
         }
 }
 
-// Y[b][m] = scale * sum_ks part[ks][m][b]; 32x32 tiles transposed through LDS.
+// Y[b][m] = scale * sum_ks part[ks][m][b]. A block sums an 8 (stations) x 32
+// (models) tile, one entry per thread (reads coalesced along the models), and
+// writes it transposed through LDS.
 __global__ __launch_bounds__(256) void batched_reduce_kernel(const float* __restrict__ part,
                                                              int ks, int m, int m_pad, int B,
                                                              int Bp, double scale,
                                                              double* __restrict__ Y) {
-    __shared__ double tile[32][33];
-    const int mb = blockIdx.x * 32, bb = blockIdx.y * 32;
+    __shared__ double tile[8][33];
+    const int mb = blockIdx.x * 8, bb = blockIdx.y * 32;
     const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;   // 32 x 8
-    for (int r = ty; r < 32; r += 8) {
-        const int mm = mb + r, b = bb + tx;
+    {
+        const int mm = mb + ty, b = bb + tx;
         double s = 0.0;
-        if (mm < m && b < B)
-            for (int q = 0; q < ks; ++q) s += (double)part[((size_t)q * m_pad + mm) * Bp + b];
-        tile[r][tx] = s;
+        if (mm < m && b < B) {
+            const float* p = part + (size_t)mm * Bp + b;
+            const size_t step = (size_t)m_pad * Bp;
+#pragma unroll 8
+            for (int q = 0; q < ks; ++q) s += (double)p[(size_t)q * step];
+        }
+        tile[ty][tx] = s;
     }
     __syncthreads();
-    for (int r = ty; r < 32; r += 8) {
-        const int b = bb + r, mm = mb + tx;
-        if (b < B && mm < m) Y[(size_t)b * m + mm] = scale * tile[tx][r];
-    }
+    // 32 models x 8 stations out: thread (b = t / 8, station t % 8)
+    const int t = threadIdx.x;
+    const int b = bb + (t >> 3), mm = mb + (t & 7);
+    if (b < B && mm < m) Y[(size_t)b * m + mm] = scale * tile[t & 7][t >> 3];
 }
 
 // --------------------------------------------------------------- launchers
@@ -364,7 +370,7 @@ int batched_fwd_launch(const PmShape& s, int B, int ks, int split, const float* 
 #undef ZTM_BATCHED_CASE
         ZTM_HIP(hipGetLastError());
     }
-    hipLaunchKernelGGL(batched_reduce_kernel, dim3((s.m + 31) / 32, (B + 31) / 32), dim3(256), 0,
+    hipLaunchKernelGGL(batched_reduce_kernel, dim3((s.m + 7) / 8, (B + 31) / 32), dim3(256), 0,
                        stream, part, ks, s.m, s.m_pad, B, Bp, scale, Y);
     ZTM_HIP(hipGetLastError());
     return ZTM_OK;
