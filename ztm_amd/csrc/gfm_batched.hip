@@ -41,30 +41,57 @@ constexpr int kBatchChunk = 16;   // cells per MFMA K step
 constexpr int kBatchTile = 128;   // stations per workgroup
 
 // ---------------------------------------------------------------- packing
-// One thread per packed element: t = ((chunk * 2 + h) * Bp + model) * 8 + e
-// holds cell chunk * 16 + 8 h + e; writes SPLIT parts at stride n_packed.
+// A block packs 32 models x 64 cells (4 chunks): it reads the tile with
+// 256-byte row reads into LDS, then each thread converts the 8 cells of one
+// (chunk, half, model) slot and writes them as one 16-byte store per part,
+// contiguous across the models of a slot. grid = (ceil(n_chunks / 4), Bp / 32).
+// Element v is split as h_1 = bf16(v), h_2 = bf16(v - h_1), ...; entries
+// outside the model or cell range, and |v| <= thr, are 0.
+constexpr int kPackCells = 64;
+
 template <int SPLIT>
-__global__ void batched_pack_kernel(const float* __restrict__ P, int B, int64_t n_cells,
-                                    int Bp, int64_t n_packed, float thr,
-                                    __bf16* __restrict__ parts) {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n_packed) return;
-    const int e = (int)(t & 7);
-    const int64_t hm = t >> 3;
-    const int model = (int)(hm % Bp);
-    const int64_t ch = hm / Bp;   // chunk * 2 + h
-    const int64_t cell = (ch >> 1) * kBatchChunk + (ch & 1) * 8 + e;
-    float v = 0.f;
-    if (model < B && cell < n_cells) {
-        v = P[(size_t)model * n_cells + cell];
-        if (!(fabsf(v) > thr)) v = 0.f;
-    }
+__global__ __launch_bounds__(256) void batched_pack_kernel(const float* __restrict__ P, int B,
+                                                           int64_t n_cells, int Bp,
+                                                           int64_t n_chunks, int64_t n_packed,
+                                                           float thr,
+                                                           __bf16* __restrict__ parts) {
+    __shared__ float tile[32][kPackCells + 1];
+    const int t = threadIdx.x;
+    const int64_t c0 = (int64_t)blockIdx.x * kPackCells;
+    const int m0 = blockIdx.y * 32;
 #pragma unroll
-    for (int q = 0; q < SPLIT; ++q) {
-        const __bf16 h = (__bf16)v;
-        parts[q * n_packed + t] = h;
-        v -= (float)h;
+    for (int i = 0; i < 8; ++i) {
+        const int idx = i * 256 + t;
+        const int row = idx >> 6, col = idx & 63;
+        const int model = m0 + row;
+        const int64_t cell = c0 + col;
+        float v = 0.f;
+        if (model < B && cell < n_cells) {
+            v = P[(size_t)model * n_cells + cell];
+            if (!(fabsf(v) > thr)) v = 0.f;
+        }
+        tile[row][col] = v;
     }
+    __syncthreads();
+    const int slot = t >> 5;          // 8 slots: chunk (c0/16 + slot/2), half slot%2
+    const int mrow = t & 31;
+    const int64_t chunk = c0 / kBatchChunk + (slot >> 1);
+    if (chunk >= n_chunks) return;
+    bf16x8 h[SPLIT];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+        float v = tile[mrow][slot * 8 + e];
+#pragma unroll
+        for (int q = 0; q < SPLIT; ++q) {
+            const __bf16 b = (__bf16)v;
+            h[q][e] = b;
+            v -= (float)b;
+        }
+    }
+    const int64_t off = ((chunk * 2 + (slot & 1)) * Bp + m0 + mrow) * 8;
+#pragma unroll
+    for (int q = 0; q < SPLIT; ++q)
+        *reinterpret_cast<bf16x8*>(parts + q * n_packed + off) = h[q];
 }
 
 // --------------------------------------------------------------- forward
@@ -330,12 +357,13 @@ int batched_fwd_launch(const PmShape& s, int B, int ks, int split, const float* 
     const int64_t n_packed = n_chunks * Bp * kBatchChunk;
     __bf16* parts = (__bf16*)ws;
     float* part = (float*)(parts + split * n_packed);
+    const dim3 pgrid((unsigned)((s.n_cells + kPackCells - 1) / kPackCells), (unsigned)(Bp / 32));
     if (split == 3)
-        hipLaunchKernelGGL(batched_pack_kernel<3>, dim3((unsigned)((n_packed + 255) / 256)),
-                           dim3(256), 0, stream, P, B, s.n_cells, Bp, n_packed, (float)thr, parts);
+        hipLaunchKernelGGL(batched_pack_kernel<3>, pgrid, dim3(256), 0, stream, P, B, s.n_cells,
+                           Bp, n_chunks, n_packed, (float)thr, parts);
     else
-        hipLaunchKernelGGL(batched_pack_kernel<2>, dim3((unsigned)((n_packed + 255) / 256)),
-                           dim3(256), 0, stream, P, B, s.n_cells, Bp, n_packed, (float)thr, parts);
+        hipLaunchKernelGGL(batched_pack_kernel<2>, pgrid, dim3(256), 0, stream, P, B, s.n_cells,
+                           Bp, n_chunks, n_packed, (float)thr, parts);
     ZTM_HIP(hipGetLastError());
     const dim3 grid((unsigned)((s.m + kBatchTile - 1) / kBatchTile), (unsigned)ks);
     const int64_t stride = n_packed / 8;   // in bf16x8 units
