@@ -109,6 +109,32 @@ def test_adjoint_small_vs_oracle(dev, orc, cells_per_lane):
     assert relerr(a64, want) < TOL64
 
 
+@pytest.mark.parametrize("cells_per_lane", [8, 16])
+@pytest.mark.parametrize("shape", [(128, 18, 3, 500), (200, 20, 2, 700), (121, 16, 2, 300)])
+def test_adjoint_row_table_shapes(dev, orc, shape, cells_per_lane):
+    """Shapes whose adjoint workgroups mix the row-term table path (segments
+    in one k plane over at most 16 rows) and the general path, with ragged
+    row groups and lanes past the last segment; stations inside the grid
+    exercise the clamped rows and the near field."""
+    import torch
+    nx, ny, nz, m = shape
+    print(f"[adj-table] {shape} C={cells_per_lane}", flush=True)
+    g = W.Grid(nx, ny, nz, 100.0, -50.0, 30.0, 110.0, -80.0, -60.0)
+    c = g.centres()
+    lo, hi = c.min(0) - 150, c.max(0) + 150
+    st = np.stack([W.uniform(11, a, m, lo[a], hi[a]) for a in range(3)], axis=1)
+    r = W.normal(11, 4, m, 1.0)
+    want = orc.pm_adjoint(g, W.KAPPA_SI, st, r)
+    want32 = orc.pm_adjoint(g, W.KAPPA_SI, st, r.astype(np.float32))
+    with gfm.Plan(g, st, kappa=W.KAPPA_SI) as p:
+        p.set_tuning(cells_per_lane=cells_per_lane)
+        a32 = p.adjoint(_torch(r, dev, torch.float32)).cpu().numpy()
+        a64 = p.adjoint(_torch(r, dev, torch.float64)).cpu().numpy()
+    assert np.all(np.isfinite(a32))
+    assert relerr(a32, want32) < TOL32
+    assert relerr(a64, want) < TOL64
+
+
 @pytest.mark.parametrize("shape", [(37, 5, 3, 13), (1, 1, 1, 1), (5, 1, 7, 600),
                                    (130, 3, 2, 1030), (64, 64, 1, 70)])
 def test_ragged_and_anisotropic(dev, orc, shape):

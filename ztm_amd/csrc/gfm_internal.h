@@ -137,6 +137,7 @@ int fwd_rows_per_slice(const PmShape& s, const PmTuning& t, size_t elem, bool qu
 // FP32 point-mass adjoint: rows a lane takes together (1 otherwise), the
 // number of lane segments (C cells of G rows) and the station splits.
 int adj_group(int C, size_t elem, bool quad);
+size_t adj_lds(int C, size_t elem, bool quad);   // dynamic LDS of one adjoint workgroup
 int64_t adj_segments(const PmShape& s, int C, int G, int* segs_per_row);
 int adj_splits(const PmShape& s, const PmTuning& t, size_t elem, bool quad);
 size_t pm_adj_workspace(const PmShape& s, const PmTuning& t, int precision,

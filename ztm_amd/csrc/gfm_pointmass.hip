@@ -317,12 +317,22 @@ __global__ __launch_bounds__(256) void pm_fwd_reduce_kernel(const double* __rest
 template <typename T, int C, bool QUAD>
 constexpr int kAdjGroup = (sizeof(T) == sizeof(float) && !QUAD) ? (C >= 16 ? 2 : 4) : 1;
 
+// Rows of the row-term table of the FP32 point-mass adjoint (see pm_adj_block).
+constexpr int kAdjRows = 16;
+
 // grid = (ceil(n_segs/64), n_splits); block = 256; a segment is C
 // consecutive cells of G consecutive rows of one k plane (the last segment of
 // a row and the last row group of a plane may be ragged); segments run
 // i fastest, then the row group, then k.
 // pm_adj_block: one workgroup's work -- segment block bx, station split by;
-// smem must hold max(256 * sizeof(V4), 4*C*64*8) bytes.
+// smem must hold adj_lds() bytes.
+//
+// Row-term table (FP32 point mass, G > 1): when the workgroup's segments lie
+// in one k plane and span at most kAdjRows rows, the station staging also
+// forms q = r dz and dz^2 once per station, and R = max(dy^2 + dz^2, 1/4) once
+// per (station, row) into an LDS table, instead of once per lane. Lanes then
+// read their G row terms with one LDS load. The values are those of the
+// general path, operation for operation, so both give identical results.
 template <typename T, int C, bool QUAD>
 __device__ __forceinline__ void pm_adj_block(
     const typename Vec4<T>::type* __restrict__ st, const T* __restrict__ r,
@@ -338,7 +348,7 @@ __device__ __forceinline__ void pm_adj_block(
     const int wave = threadIdx.x >> 6;
     const int rgroups = (ny + G - 1) / G;
     const int64_t seg = (int64_t)bx * 64 + lane;
-    const int64_t segc = seg < n_segs ? seg : 0;
+    const int64_t segc = seg < n_segs ? seg : n_segs - 1;   // past the end: computed, not written
     const int64_t rowg = segc / segs_per_row;
     const int i0 = (int)(segc - rowg * segs_per_row) * C;
     const int k = (int)(rowg / rgroups);
@@ -355,6 +365,35 @@ __device__ __forceinline__ void pm_adj_block(
 #pragma unroll
         for (int c = 0; c < C; ++c) accd[g][c] = 0.0;
 
+    // workgroup-uniform choice of the row-term table
+    constexpr bool kTable = sizeof(T) == sizeof(float) && !QUAD && G > 1;
+    bool table = false;
+    int rowoff = 0;
+    T yfirst = T(0);
+    if constexpr (kTable) {
+        const int64_t s0 = (int64_t)bx * 64;
+        const int64_t s1 = min(s0 + 63, n_segs - 1);
+        const int64_t rg0 = s0 / segs_per_row, rg1 = s1 / segs_per_row;
+        const int kw = (int)(rg0 / rgroups);
+        table = kw == (int)(rg1 / rgroups) && (rg1 - rg0 + 1) * G <= kAdjRows;
+        const int jfirst = (int)(rg0 - (int64_t)kw * rgroups) * G;
+        rowoff = j0 - jfirst;   // in [0, kAdjRows - G] when table
+        yfirst = T(jfirst);
+    }
+    T* rt = reinterpret_cast<T*>(smem + 256 * sizeof(V4));   // [256][kAdjRows]
+
+    // the pairs of one station with this lane's G x C cells
+    auto pairs = [&](T (&acc)[G][C], T q, T xm, const T (&R)[G], const T (&qrow)[G]) {
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+            const T a = xm - T(c);
+#pragma unroll
+            for (int g = 0; g < G; ++g)
+                acc[g][c] = fma(q, kterm<T, QUAD>(a, R[g], qrow[g], qa), acc[g][c]);
+        }
+        if constexpr (sizeof(T) == sizeof(float) && !QUAD) interleave_trans<C * G, 4>();
+    };
+
     const int mb = by * st_per_split;
     const int me = min(mb + st_per_split, m_pad);
     for (int t0 = mb; t0 < me; t0 += 256) {
@@ -363,6 +402,10 @@ __device__ __forceinline__ void pm_adj_block(
             const int mm = t0 + threadIdx.x;
             V4 p = st[mm];
             p.w = mm < m ? r[mm] : T(0);
+            if (kTable && table) {
+                const T dz = p.z - zk;
+                p = V4{p.x, p.w * dz, dz * dz, p.y};   // (x, q, dz^2, y)
+            }
             tile[threadIdx.x] = p;
         }
         __syncthreads();
@@ -372,26 +415,43 @@ __device__ __forceinline__ void pm_adj_block(
 #pragma unroll
             for (int c = 0; c < C; ++c) acc[g][c] = T(0);
         const int sb = wave * 64;
-        for (int s = sb; s < sb + 64; ++s) {
-            const V4 p = tile[s];    // wave-uniform LDS broadcast
-            const T dz = p.z - zk;
-            const T q = p.w * dz;
-            const T xm = p.x - xi0;
-            T R[G], qrow[G];
-#pragma unroll
-            for (int g = 0; g < G; ++g) {
-                const T dy = p.y - yj[g];
-                R[g] = row_R(dy, dz);
-                qrow[g] = QUAD ? fma(T(qa.q2y), dy * dy, T(qa.q2z) * (dz * dz)) : T(0);
+        if (kTable && table) {
+            {   // R for kAdjRows rows from the workgroup's first row: thread ->
+                // row t % kAdjRows, stations t / kAdjRows + (256 / kAdjRows) u
+                const int rho = threadIdx.x % kAdjRows;
+                const T yr = (yfirst + T(rho)) * sy;
+#pragma unroll 4
+                for (int u = 0; u < kAdjRows; ++u) {
+                    const int s = threadIdx.x / kAdjRows + (256 / kAdjRows) * u;
+                    const V4 p = tile[s];
+                    const T dy = p.w - yr;
+                    rt[s * kAdjRows + rho] = fmax(fma(dy, dy, p.z), T(kRmin2F));
+                }
             }
+            __syncthreads();
+            const T qrow[G] = {};
+            for (int s = sb; s < sb + 64; ++s) {
+                const V4 p = tile[s];    // wave-uniform LDS broadcast
+                T R[G];
 #pragma unroll
-            for (int c = 0; c < C; ++c) {
-                const T a = xm - T(c);
-#pragma unroll
-                for (int g = 0; g < G; ++g)
-                    acc[g][c] = fma(q, kterm<T, QUAD>(a, R[g], qrow[g], qa), acc[g][c]);
+                for (int g = 0; g < G; ++g) R[g] = rt[s * kAdjRows + rowoff + g];
+                pairs(acc, p.y, p.x - xi0, R, qrow);
             }
-            if constexpr (sizeof(T) == sizeof(float) && !QUAD) interleave_trans<C * G, 4>();
+        } else {
+            for (int s = sb; s < sb + 64; ++s) {
+                const V4 p = tile[s];    // wave-uniform LDS broadcast
+                const T dz = p.z - zk;
+                const T q = p.w * dz;
+                const T xm = p.x - xi0;
+                T R[G], qrow[G];
+#pragma unroll
+                for (int g = 0; g < G; ++g) {
+                    const T dy = p.y - yj[g];
+                    R[g] = row_R(dy, dz);
+                    qrow[g] = QUAD ? fma(T(qa.q2y), dy * dy, T(qa.q2z) * (dz * dz)) : T(0);
+                }
+                pairs(acc, q, xm, R, qrow);
+            }
         }
 #pragma unroll
         for (int g = 0; g < G; ++g)
@@ -763,6 +823,15 @@ int adj_group(int C, size_t elem, bool quad) {
     return C >= 16 ? 2 : 4;
 }
 
+size_t adj_lds(int C, size_t elem, bool quad) {
+    // station tile, then (FP32 point mass with row groups) the row-term
+    // table; the cross-wave FP64 reduction reuses the same memory
+    size_t b = std::max((size_t)256 * 4 * elem, (size_t)4 * C * 64 * sizeof(double));
+    if (elem == sizeof(float) && !quad && adj_group(C, elem, quad) > 1)
+        b = std::max(b, (size_t)256 * 4 * elem + (size_t)256 * kAdjRows * elem);
+    return b;
+}
+
 int64_t adj_segments(const PmShape& s, int C, int G, int* segs_per_row) {
     const int spr = (s.nx + C - 1) / C;
     if (segs_per_row) *segs_per_row = spr;
@@ -811,7 +880,7 @@ void launch_adj(const PmShape& s, int splits, int direct, const void* st, const 
     const int blocks_x = (int)((n_segs + 63) / 64);
     const int tiles = s.m_pad / 256;
     const int per_split = ((tiles + splits - 1) / splits) * 256;
-    const size_t lds = std::max((size_t)256 * sizeof(V4), (size_t)4 * C * 64 * sizeof(double));
+    const size_t lds = adj_lds(C, sizeof(T), quad);
     if (quad && sizeof(T) == sizeof(float))
         hipLaunchKernelGGL((pm_adj_kernel<T, C, true>), dim3(blocks_x, splits), dim3(256),
                            lds, stream, (const V4*)st, (const T*)r, s.m, s.nx, s.ny,
@@ -1021,7 +1090,7 @@ int pm_adj_desc(ztm_plan p, int64_t cell_off, int64_t st_off, GAdjDesc* d, size_
     const int64_t plane = (int64_t)s.nx * s.ny;
     d->near_off = d->klo <= d->khi ? d->klo * plane : 0;
     d->n_near = d->klo <= d->khi ? (int64_t)(d->khi - d->klo + 1) * plane : 0;
-    *lds = std::max((size_t)256 * sizeof(float4), (size_t)4 * C * 64 * sizeof(double));
+    *lds = adj_lds(C, sizeof(float), false);
     *blocks = (int64_t)d->blocks_x * splits;
     return ZTM_OK;
 }
