@@ -38,6 +38,9 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 constexpr int kBatchChunk = 16;   // cells per MFMA K step
+// s_waitcnt immediate (gfx9 encoding): vmcnt(0), expcnt and lgkmcnt left at
+// their maxima, i.e. wait only for this wave's vector memory operations.
+constexpr int kWaitVm0 = 0x0F70;
 constexpr int kBatchTile = 128;   // stations per workgroup
 
 // ---------------------------------------------------------------- packing
@@ -195,47 +198,54 @@ __global__ __launch_bounds__(256) void batched_fwd_kernel(
 
     const int c0 = blockIdx.y * chunks_per_slice;
     const int c1 = min(c0 + chunks_per_slice, n_chunks);
-    bf16x8 g[LOADS];
-    auto fetch = [&](int chunk) {
+    // P parts of a chunk go from global memory straight into LDS buffer buf
+    // (global_load_lds_dwordx4: lane l of a wave writes 16 bytes at the wave's
+    // LDS base + 16 l). Element idx = q V + v of the flattened [SPLIT][V]
+    // buffer is part q, slot v; a wave's 64 lanes cover 64 consecutive idx,
+    // and V * SPLIT is a multiple of 64, so the guard is wave-uniform.
+    auto fetch = [&](int chunk, int buf) {
 #pragma unroll
         for (int u = 0; u < LOADS; ++u) {
             const int idx = threadIdx.x + 256 * u;
             if ((V * SPLIT) % 256 == 0 || idx < V * SPLIT)
-                g[u] = parts[(idx / V) * part_stride + (int64_t)chunk * V + (idx % V)];
-        }
-    };
-    auto stash = [&](int buf) {
-#pragma unroll
-        for (int u = 0; u < LOADS; ++u) {
-            const int idx = threadIdx.x + 256 * u;
-            if ((V * SPLIT) % 256 == 0 || idx < V * SPLIT) lp[buf][idx / V][idx % V] = g[u];
+                __builtin_amdgcn_global_load_lds(
+                    (const void*)(parts + (idx / V) * part_stride + (int64_t)chunk * V + (idx % V)),
+                    (__attribute__((address_space(3))) void*)(&lp[buf][0][0] + (idx - lane)),
+                    16, 0, 0);
         }
     };
     bf16x8 a[SPLIT];
     if (c0 < c1) {
-        fetch(c0);
-        stash(0);
+        fetch(c0, 0);
         batched_gen<SPLIT, WIDE>(p, c0 * kBatchChunk + 8 * kh, nx, ny, inv_nx, inv_ny, sy, sz, a);
     }
+    __builtin_amdgcn_s_waitcnt(kWaitVm0);   // this wave's LDS loads have landed
     __syncthreads();
     for (int c = c0; c < c1; ++c) {
         const int buf = (c - c0) & 1;
         const int cn = min(c + 1, c1 - 1);
-        fetch(cn);
+        fetch(cn, buf ^ 1);   // buffer buf ^ 1 was last read before the previous barrier
         bf16x8 cur[SPLIT];
 #pragma unroll
         for (int q = 0; q < SPLIT; ++q) cur[q] = a[q];
         batched_gen<SPLIT, WIDE>(p, cn * kBatchChunk + 8 * kh, nx, ny, inv_nx, inv_ny, sy, sz, a);
         // Tiles in pairs so that consecutive MFMAs update different
         // accumulators (no back-to-back dependence on one accumulator).
+        // The B operands of the next tile pair are read from LDS while the
+        // current pair's MFMAs run (two register sets).
         constexpr int TP = NT >= 2 ? 2 : 1;
-#pragma unroll
-        for (int t = 0; t < NT; t += TP) {
-            bf16x8 b[TP][SPLIT];
+        bf16x8 b[2][TP][SPLIT];
+        auto load_b = [&](int t, bf16x8 (&bb)[TP][SPLIT]) {
 #pragma unroll
             for (int u = 0; u < TP; ++u)
 #pragma unroll
-                for (int q = 0; q < SPLIT; ++q) b[u][q] = lp[buf][q][kh * Bp + 32 * (t + u) + row];
+                for (int q = 0; q < SPLIT; ++q) bb[u][q] = lp[buf][q][kh * Bp + 32 * (t + u) + row];
+        };
+        load_b(0, b[0]);
+#pragma unroll
+        for (int t = 0; t < NT; t += TP) {
+            const int cb = (t / TP) & 1;
+            if (t + TP < NT) load_b(t + TP, b[cb ^ 1]);
             // products x_qa * y_qb with qa + qb < SPLIT
 #pragma unroll
             for (int s2 = 0; s2 < SPLIT; ++s2)
@@ -244,24 +254,26 @@ __global__ __launch_bounds__(256) void batched_fwd_kernel(
 #pragma unroll
                     for (int u = 0; u < TP; ++u)
                         acc[t + u] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
-                            cur[qa], b[u][s2 - qa], acc[t + u], 0, 0, 0);
+                            cur[qa], b[cb][u][s2 - qa], acc[t + u], 0, 0, 0);
         }
-        stash(buf ^ 1);
-        // Schedule: the next chunk's global loads first, then per tile pair
-        // its LDS reads and its MFMAs, each MFMA followed by a few VALU
-        // instructions of the next chunk's G generation; the LDS writes last.
+        // Schedule: the next chunk's global-to-LDS loads first, the first
+        // tile pair's LDS reads, then per tile pair the next pair's LDS reads
+        // and its own MFMAs, each MFMA followed by a few VALU instructions of
+        // the next chunk's G generation.
         constexpr int kPerPair = TP * SPLIT * (SPLIT + 1) / 2;   // MFMAs per tile pair
         __builtin_amdgcn_sched_group_barrier(0x020, LOADS, 0);   // VMEM read
+        __builtin_amdgcn_sched_group_barrier(0x100, TP * SPLIT, 0);   // DS read, pair 0
 #pragma unroll
         for (int t = 0; t < NT; t += TP) {
-            __builtin_amdgcn_sched_group_barrier(0x100, TP * SPLIT, 0);   // DS read
+            if (t + TP < NT)
+                __builtin_amdgcn_sched_group_barrier(0x100, TP * SPLIT, 0);   // DS read, next pair
 #pragma unroll
             for (int i = 0; i < kPerPair; ++i) {
                 __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);        // MFMA
                 __builtin_amdgcn_sched_group_barrier(0x002, SPLIT == 3 ? 5 : 7, 0);   // VALU
             }
         }
-        __builtin_amdgcn_sched_group_barrier(0x200, LOADS, 0);   // DS write
+        __builtin_amdgcn_s_waitcnt(kWaitVm0);
         __syncthreads();
     }
     // D layout: lane holds model 32t + lane%32, stations 8*(i/4) + 4*(lane/32) + i%4
