@@ -109,18 +109,37 @@ __device__ __forceinline__ void divmod_pos(int n, int d, double inv, int& q, int
 }
 
 // This lane's 8 G values of a chunk (cells cell0 .. cell0 + 7, raster order),
-// split into SPLIT BF16 parts in the MFMA A-operand layout. WIDE (nx >= 8):
+// split into SPLIT BF16 parts in the MFMA A-operand layout. MODE 1 (nx >= 8):
 // the 8 cells span at most two grid rows, so both row terms are formed once
-// and each cell selects one. Cells past the grid give finite values (r^2 >=
-// 1e-20, so |dz r^-3| <= 1e36), which the zero-padded P cancels.
-template <int SPLIT, bool WIDE>
+// and each cell selects one. MODE 2 (nx a multiple of 8): the 8 cells lie in
+// one row (cell0 is a multiple of 8), so there is nothing to select; for such
+// grids it gives the same values as MODE 1. MODE 0: cell-by-cell stepping.
+// Cells past the grid give finite values (r^2 >= 1e-20, so |dz r^-3| <=
+// 1e36), which the zero-padded P cancels.
+template <int SPLIT, int MODE>
 __device__ __forceinline__ void batched_gen(const float4 p, int cell0, int nx, int ny,
                                             double inv_nx, double inv_ny, float sy, float sz,
                                             bf16x8 (&a)[SPLIT]) {
     int rw, i0, k0, j0;
     divmod_pos(cell0, nx, inv_nx, rw, i0);
     divmod_pos(rw, ny, inv_ny, k0, j0);
-    if constexpr (WIDE) {
+    if constexpr (MODE == 2) {
+        const float dy0 = p.y - (float)j0 * sy, dz0 = p.z - (float)k0 * sz;
+        const float R0 = fmaf(dy0, dy0, fmaf(dz0, dz0, 1e-20f));
+        const float xa = p.x - (float)i0;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const float x = xa - (float)e;
+            const float v = rsq_f(fmaf(x, x, R0));
+            float gv = dz0 * (v * v * v);
+#pragma unroll
+            for (int q = 0; q < SPLIT; ++q) {
+                const __bf16 h = (__bf16)gv;
+                a[q][e] = h;
+                gv -= (float)h;
+            }
+        }
+    } else if constexpr (MODE == 1) {
         const bool wrap = j0 + 1 == ny;
         const int j1 = wrap ? 0 : j0 + 1, k1 = wrap ? k0 + 1 : k0;
         const float dy0 = p.y - (float)j0 * sy, dz0 = p.z - (float)k0 * sz;
@@ -176,7 +195,7 @@ __device__ __forceinline__ void batched_gen(const float4 p, int cell0, int nx, i
 // c&1), the next chunk's P parts are loaded from global memory into
 // registers and its G block is generated; the P parts then go to the other
 // LDS buffer. The last iteration repeats its own chunk's loads (unused).
-template <int NT, int SPLIT, bool WIDE>   // model tiles of 32 (B <= 32 * NT); 2- or 3-way split
+template <int NT, int SPLIT, int MODE>   // model tiles of 32 (B <= 32 * NT); 2- or 3-way split; batched_gen mode
 __global__ __launch_bounds__(256) void batched_fwd_kernel(
     const float4* __restrict__ st, const bf16x8* __restrict__ parts, int64_t part_stride,
     int nx, int ny, double inv_nx, double inv_ny, float sy, float sz, int n_chunks,
@@ -217,7 +236,7 @@ __global__ __launch_bounds__(256) void batched_fwd_kernel(
     bf16x8 a[SPLIT];
     if (c0 < c1) {
         fetch(c0, 0);
-        batched_gen<SPLIT, WIDE>(p, c0 * kBatchChunk + 8 * kh, nx, ny, inv_nx, inv_ny, sy, sz, a);
+        batched_gen<SPLIT, MODE>(p, c0 * kBatchChunk + 8 * kh, nx, ny, inv_nx, inv_ny, sy, sz, a);
     }
     __builtin_amdgcn_s_waitcnt(kWaitVm0);   // this wave's LDS loads have landed
     __syncthreads();
@@ -228,7 +247,7 @@ __global__ __launch_bounds__(256) void batched_fwd_kernel(
         bf16x8 cur[SPLIT];
 #pragma unroll
         for (int q = 0; q < SPLIT; ++q) cur[q] = a[q];
-        batched_gen<SPLIT, WIDE>(p, cn * kBatchChunk + 8 * kh, nx, ny, inv_nx, inv_ny, sy, sz, a);
+        batched_gen<SPLIT, MODE>(p, cn * kBatchChunk + 8 * kh, nx, ny, inv_nx, inv_ny, sy, sz, a);
         // Tiles in pairs so that consecutive MFMAs update different
         // accumulators (no back-to-back dependence on one accumulator).
         // The B operands of the next tile pair are read from LDS while the
@@ -381,17 +400,17 @@ int batched_fwd_launch(const PmShape& s, int B, int ks, int split, const float* 
     const int64_t stride = n_packed / 8;   // in bf16x8 units
     {
         ProfScope prof("batched_fwd", stream);
+#define ZTM_BATCHED_LAUNCH(NT, SP, MODE)                                                        \
+    hipLaunchKernelGGL((batched_fwd_kernel<NT, SP, MODE>), grid, dim3(256), 0, stream,         \
+                       (const float4*)st, (const bf16x8*)parts, stride, s.nx, s.ny, 1.0 / s.nx, \
+                       1.0 / s.ny, (float)sy, (float)sz, (int)n_chunks, per, s.m_pad, part)
 #define ZTM_BATCHED_CASE(NT, SP)                                                               \
-    if (s.nx >= 8)                                                                             \
-        hipLaunchKernelGGL((batched_fwd_kernel<NT, SP, true>), grid, dim3(256), 0, stream,     \
-                           (const float4*)st, (const bf16x8*)parts, stride, s.nx, s.ny,        \
-                           1.0 / s.nx, 1.0 / s.ny, (float)sy, (float)sz, (int)n_chunks, per,   \
-                           s.m_pad, part);                                                     \
+    if (s.nx % 8 == 0)                                                                         \
+        ZTM_BATCHED_LAUNCH(NT, SP, 2);                                                         \
+    else if (s.nx >= 8)                                                                        \
+        ZTM_BATCHED_LAUNCH(NT, SP, 1);                                                         \
     else                                                                                       \
-        hipLaunchKernelGGL((batched_fwd_kernel<NT, SP, false>), grid, dim3(256), 0, stream,    \
-                           (const float4*)st, (const bf16x8*)parts, stride, s.nx, s.ny,        \
-                           1.0 / s.nx, 1.0 / s.ny, (float)sy, (float)sz, (int)n_chunks, per,   \
-                           s.m_pad, part)
+        ZTM_BATCHED_LAUNCH(NT, SP, 0)
         if (split == 3) {
             switch (nt) {
                 case 1: ZTM_BATCHED_CASE(1, 3); break;
@@ -408,6 +427,7 @@ int batched_fwd_launch(const PmShape& s, int B, int ks, int split, const float* 
             }
         }
 #undef ZTM_BATCHED_CASE
+#undef ZTM_BATCHED_LAUNCH
         ZTM_HIP(hipGetLastError());
     }
     hipLaunchKernelGGL(batched_reduce_kernel, dim3((s.m + 7) / 8, (B + 31) / 32), dim3(256), 0,
