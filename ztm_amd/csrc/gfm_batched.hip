@@ -192,9 +192,9 @@ __device__ __forceinline__ void batched_gen(const float4 p, int cell0, int nx, i
 // grid = (station tiles of 128, K-slices); block = 256 (4 waves x 32 stations).
 // Software pipeline over the slice's chunks, one barrier per chunk: while the
 // MFMAs of chunk c run on its G block (registers) and P parts (LDS buffer
-// c&1), the next chunk's P parts are loaded from global memory into
-// registers and its G block is generated; the P parts then go to the other
-// LDS buffer. The last iteration repeats its own chunk's loads (unused).
+// c&1), the next chunk's P parts load from global memory straight into the
+// other LDS buffer and its G block is generated. The last iteration repeats
+// its own chunk's loads into the buffer that is not read again.
 template <int NT, int SPLIT, int MODE>   // model tiles of 32 (B <= 32 * NT); 2- or 3-way split; batched_gen mode
 __global__ __launch_bounds__(256) void batched_fwd_kernel(
     const float4* __restrict__ st, const bf16x8* __restrict__ parts, int64_t part_stride,
