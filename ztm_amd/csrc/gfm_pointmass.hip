@@ -81,9 +81,9 @@ __device__ __forceinline__ T kterm(T a, T R, T qrow, const QuadArgs& q) {
 // to bound the registers); 1 for FP64 and the quadrupole kernel. Matches
 // fwd_group().
 template <typename T, int S, bool QUAD>
-constexpr int kFwdGroup = (sizeof(T) == sizeof(float) && !QUAD)
-                              ? (S <= 1 ? 8 : S == 2 ? 4 : S == 4 ? 2 : 1)
-                              : 1;
+constexpr int kFwdGroup = QUAD ? 1
+                          : sizeof(T) == sizeof(float) ? (S <= 1 ? 8 : S == 2 ? 4 : S == 4 ? 2 : 1)
+                                                       : (S == 2 ? 4 : 1);
 
 template <typename T, int S, bool QUAD>
 __device__ __forceinline__ void pm_fwd_block(
@@ -174,7 +174,7 @@ __device__ __forceinline__ void pm_fwd_block(
                     }
                     xm[s] -= T(4);
                 }
-                interleave_trans<S * G * 4, 6>();
+                if constexpr (sizeof(T) == sizeof(float)) interleave_trans<S * G * 4, 6>();
             }
 #pragma unroll
             for (int g = 0; g < G; ++g) {
@@ -310,12 +310,15 @@ __global__ __launch_bounds__(256) void pm_fwd_reduce_kernel(const double* __rest
 }
 
 // ---------------------------------------------------------------- adjoint
-// Rows a lane takes together in the FP32 point-mass adjoint (same k, G
+// Rows a lane takes together in the point-mass adjoint (same k, G
 // consecutive j): per station, dz, q = r dz and the column offsets a = X - i
-// are formed once for all G rows. 1 for FP64 and the quadrupole kernel.
+// are formed once for all G rows. FP64 uses half the FP32 group at C = 8 (its
+// accumulators take twice the registers); the quadrupole kernel uses 1.
 // Matches adj_group().
 template <typename T, int C, bool QUAD>
-constexpr int kAdjGroup = (sizeof(T) == sizeof(float) && !QUAD) ? (C >= 16 ? 2 : 4) : 1;
+constexpr int kAdjGroup = QUAD ? 1
+                          : sizeof(T) == sizeof(float) ? (C >= 16 ? 2 : 4)
+                                                       : (C >= 8 ? 2 : 4);
 
 // Rows of the row-term table of the FP32 point-mass adjoint (see pm_adj_block).
 constexpr int kAdjRows = 16;
@@ -785,16 +788,18 @@ int prep_ref_stations(const float* s, int bfr, int di, int dj, int fh, int fv,
 constexpr int kTargetWgs = 2048;   // 8 workgroups per CU on 256 CUs
 
 int fwd_group(int S, size_t elem, bool quad) {
-    if (elem != sizeof(float) || quad) return 1;
+    if (quad) return 1;
+    if (elem != sizeof(float)) return S == 2 ? 4 : 1;
     return S <= 1 ? 8 : S == 2 ? 4 : S == 4 ? 2 : 1;
 }
 
 PmTuning fwd_adjust(const PmShape& s, PmTuning t, size_t elem, bool quad) {
-    // a row group of G rows must fit the 32 KB LDS budget; wide rows run S = 8
+    // a row group of G rows must fit the 32 KB LDS budget; wide rows run
+    // without groups (S = 8 in FP32, S = 4 in FP64)
     const size_t nxp = (size_t)((s.nx + 3) & ~3);
     if (fwd_group(t.st_per_lane, elem, quad) > 1 &&
         nxp * elem * fwd_group(t.st_per_lane, elem, quad) > 32768)
-        t.st_per_lane = 8;
+        t.st_per_lane = elem == sizeof(float) ? 8 : 4;
     return t;
 }
 
@@ -819,7 +824,8 @@ int fwd_rows_per_slice(const PmShape& s, const PmTuning& t0, size_t elem, bool q
 }
 
 int adj_group(int C, size_t elem, bool quad) {
-    if (elem != sizeof(float) || quad) return 1;
+    if (quad) return 1;
+    if (elem != sizeof(float)) return C >= 8 ? 2 : 4;
     return C >= 16 ? 2 : 4;
 }
 
