@@ -541,6 +541,13 @@ def tfm_leg(stream, args, reps=20):
     t_batch = _timed(lambda: gfm.tfm_gravity_batched(w.bfr, w.di, w.dj, w.dk, w.fh, w.fv, w.rc,
                                                      sea, mohoB, vel, prmB, s32, drho=dB, g=gB,
                                                      stream=stream), 3, torch.cuda.synchronize)
+    # exact gravity columns for (vm, ro, dr): derivative fields + one forward
+    ddB = torch.empty(3, w.n_fine, dtype=torch.float32, device=dev)
+    dgB = torch.empty(3, M, dtype=torch.float64, device=dev)
+    t_jvp = _timed(lambda: gfm.tfm_gravity_jvp(w.bfr, w.di, w.dj, w.dk, w.fh, w.fv, w.rc, sea,
+                                               t64(w.moho), vel, w.vm, w.ro, w.dr, s32,
+                                               ddrho=ddB, dg=dgB, stream=stream), 5,
+                   torch.cuda.synchronize)
     out = {
         "workload": "tfm_fm: TFM.cpp:83-94 at the reference grid, 241,865 fine cells x 1,681 "
                     "interior stations, producer + gravity per FM call",
@@ -551,6 +558,8 @@ def tfm_leg(stream, args, reps=20):
         "jacobian_256_variants": {"ms_per_call": t_batch * 1e3,
                                   "ms_per_variant": t_batch * 1e3 / B,
                                   "vs_single_device_calls": t_dev * B / t_batch},
+        "gravity_jvp_3_params": {"ms_per_call": t_jvp * 1e3,
+                                 "note": "exact dG/d(vm, ro, dr): derivative fields + forward"},
         "ms_gfm_program": None if t_prog is None else t_prog * 1e3,
         "cpu_port": {"ms_producer": t_prod_cpu * 1e3,
                      "ms_gravity_thread_per_row": t_grav_rows * 1e3, "rows": nj,

@@ -240,6 +240,28 @@ int ztm_tfm_gravity_batched(int bfr, int di, int dj, int dk, int fh, int fv, int
                             const double* params_dev, const float* s_dev, float* drho_dev,
                             double* g_dev, void* stream);
 
+/* Exact gravity columns of the Jacobian for the three gravity parameters.
+ * drho is piecewise linear in (vm, ro, dr) (TFM.cpp:86-88), so its
+ * derivative at one parameter point is rendered directly:
+ * ddrho_dev[3][N] = d drho / d(vm, ro, dr) (float), on the cells the producer
+ * keeps (TFM.cpp:89) and GF does not skip (|float drho| > .001,
+ * GFM_GPU.cu:19), decided at that point; 0 elsewhere. ro is the value the
+ * producer uses (ro = -B[4nl+nt+2], TFM.cpp:84), so the column for
+ * B[4nl+nt+2] is minus the ro column. No reference counterpart: TFM.cpp:135
+ * builds these columns by finite differences, which differ only by the cells
+ * whose piece or filter changes within the step. Device pointers. */
+int ztm_tfm_drho_jvp(int di, int dj, int dk, int fh, int fv, const double* rc,
+                     const double* sea_dev, const double* moho_dev, const double* vel_dev,
+                     double vm, double ro, double dr, float* ddrho_dev, void* stream);
+/* The gravity of those three fields at the interior stations (station
+ * heights s_dev, float[di*dj]): dg_dev[3][di*dj] = dG / d(vm, ro, dr) (FP64;
+ * 0 outside the interior), through the split-BF16 ensemble forward as in
+ * ztm_tfm_gravity_batched; ddrho_dev[3][N] receives the fields. */
+int ztm_tfm_gravity_jvp(int bfr, int di, int dj, int dk, int fh, int fv, const double* rc,
+                        const double* sea_dev, const double* moho_dev, const double* vel_dev,
+                        double vm, double ro, double dr, const float* s_dev, float* ddrho_dev,
+                        double* dg_dev, void* stream);
+
 /* ------------------------------------------------- uneven batches */
 /* A batch of independent point-mass problems (each its own grid, stations and
  * kappa; FP32 path with the FP64 near-field corrections). Every forward or

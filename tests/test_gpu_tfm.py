@@ -104,3 +104,67 @@ def test_batched_variants_match_single_calls(dev, orc):
     G0 = gfm.tfm_gravity_host(w.bfr, w.di, w.dj, w.dk, w.fh, w.fv, w.rc, w.sea, w.moho, w.vel,
                               w.surf, *params[2])
     assert relerr(g3[2], G0) < 5e-6
+
+
+def _t64(a, dev):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(a, np.float64)).to(dev)
+
+
+@pytest.mark.parametrize("geom", [dict(), dict(fh=2, fv=1, di=23, dj=19, dk=14)])
+def test_drho_jvp_matches_central_differences(dev, orc, geom):
+    """d drho / d(vm, ro, dr) against central differences of the FP64
+    restatement of TFM.cpp:86-89 on the cells that stay in one linear piece
+    and stay kept over [p - h, p + h] (there the difference is exact up to
+    rounding); cells the producer or GF drops at p must give exactly 0."""
+    w = W.tfm_fm(**geom)
+    args = (w.di, w.dj, w.dk, w.fh, w.fv, w.rc, w.sea, w.moho, w.vel)
+    dd = gfm.tfm_drho_jvp(w.di, w.dj, w.dk, w.fh, w.fv, w.rc, _t64(w.sea, dev),
+                          _t64(w.moho, dev), _t64(w.vel, dev), w.vm, w.ro, w.dr).cpu().numpy()
+    p0 = np.array([w.vm, w.ro, w.dr])
+    d0, _ = orc.tfm_drho(*args, *p0)
+    kept = np.abs(d0.astype(np.float32)) > np.float32(0.001)
+    h = 1e-4
+    for q in range(3):
+        e = np.zeros(3)
+        e[q] = h
+        dp, _ = orc.tfm_drho(*args, *(p0 + e))
+        dm, _ = orc.tfm_drho(*args, *(p0 - e))
+        right, left = (dp - d0) / h, (d0 - dm) / h
+        keep3 = kept & (np.abs(dp.astype(np.float32)) > 0.001) & (np.abs(dm.astype(np.float32)) > 0.001)
+        smooth = keep3 & (np.abs(right - left) <= 1e-7 * (1 + np.abs(right)))
+        fd = (dp - dm) / (2 * h)
+        assert smooth.sum() > 1000, q
+        scale = max(np.abs(fd[smooth]).max(), 1e-30)
+        assert np.max(np.abs(dd[q][smooth] - fd[smooth])) / scale < 1e-6, q
+        assert np.all(dd[q][~kept] == 0), q
+    # somewhere the ro derivative is the band fraction, not just 0 or +-1
+    band = kept & (np.abs(dd[1]) > 1e-3) & (np.abs(np.abs(dd[1]) - 1) > 1e-3)
+    assert band.sum() > 0
+
+
+def test_gravity_jvp_matches_gf_oracle(dev, orc):
+    """dG/d(vm, ro, dr): the split-BF16 forward of the derivative fields
+    against the FP64 GF oracle on the same fields (scaled by 2^20, exact in
+    float, so that GF's .001 skip drops none of them), 5e-6 as the batched
+    FM; exterior stations 0."""
+    import torch
+    w = W.tfm_fm()
+    s32 = torch.from_numpy(w.surf.astype(np.float32)).to(dev)
+    dg, dd = gfm.tfm_gravity_jvp(w.bfr, w.di, w.dj, w.dk, w.fh, w.fv, w.rc, _t64(w.sea, dev),
+                                 _t64(w.moho, dev), _t64(w.vel, dev), w.vm, w.ro, w.dr, s32)
+    dg, dd = dg.cpu().numpy(), dd.cpu().numpy()
+    interior = np.zeros((w.dj, w.di), bool)
+    interior[w.bfr:w.dj - w.bfr, w.bfr:w.di - w.bfr] = True
+    interior = interior.ravel()
+    c = np.float32(2.0 ** 20)
+    for q in range(3):
+        want = orc.gf(w.bfr, w.di, w.dj, w.dk, w.fh, w.fv, dd[q] * c, w.surf.astype(np.float32))
+        want = np.asarray(want, np.float64) / float(c)
+        assert np.abs(want).max() > 0, q
+        assert relerr(dg[q], want) < 5e-6, q
+        assert np.all(dg[q][~interior] == 0), q
+    # the fields equal the stand-alone derivative call bit for bit
+    dd2 = gfm.tfm_drho_jvp(w.di, w.dj, w.dk, w.fh, w.fv, w.rc, _t64(w.sea, dev),
+                           _t64(w.moho, dev), _t64(w.vel, dev), w.vm, w.ro, w.dr).cpu().numpy()
+    assert np.array_equal(dd, dd2)

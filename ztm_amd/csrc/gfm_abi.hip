@@ -804,6 +804,38 @@ int ztm_tfm_gravity_host(int bfr, int di, int dj, int dk, int fh, int fv, const 
     return ZTM_OK;
 }
 
+namespace {
+// Gravity at the interior stations of B float fields drho_dev[B][N] on the
+// reference plan p (stations prepared from s_dev): the split-BF16 ensemble
+// forward with cell threshold thr, scattered to g_dev[B][di*dj] (0 outside).
+int tfm_gravity_of_fields(ztm_plan p, int bfr, int di, int dj, int fh, int fv, int B,
+                          const float* drho_dev, double thr, const float* s_dev, double* g_dev,
+                          hipStream_t s) {
+    const int M = di * dj, m = p->shape.m;
+    if (m == 0) return tfm_scatter_launch(nullptr, nullptr, 0, M, B, g_dev, s);
+    if (int e = prep_ref_stations(s_dev, bfr, di, dj, fh, fv, p->st_f, p->st_d, p->st_c,
+                                  p->out_map, m, p->shape.m_pad, s))
+        return e;
+    const int ks = p->tuning.batched_slices > 0 ? p->tuning.batched_slices
+                                                : batched_default_slices(p->shape, B);
+    const int split = p->tuning.batched_split;
+    const size_t ws = (batched_workspace(p->shape, B, ks, split) + 255) / 256 * 256;
+    const size_t need = ws + sizeof(double) * (size_t)B * m;
+    if (need > p->bws_cap) {
+        if (p->bws) (void)hipFree(p->bws);
+        p->bws = nullptr;
+        p->bws_cap = 0;
+        ZTM_HIP(hipMalloc(&p->bws, need));
+        p->bws_cap = need;
+    }
+    double* Y = (double*)((char*)p->bws + ws);
+    if (int e = batched_fwd_launch(p->shape, B, ks, split, (const float*)p->st_f, drho_dev, thr,
+                                   p->sy, p->sz, pm_scale(p), p->bws, Y, s))
+        return e;
+    return tfm_scatter_launch(Y, p->out_map, m, M, B, g_dev, s);
+}
+}  // namespace
+
 int ztm_tfm_gravity_batched(int bfr, int di, int dj, int dk, int fh, int fv, int n_models,
                             const double* rc, const double* sea_dev, const double* moho_dev,
                             int64_t moho_stride, const double* vel_dev, int64_t vel_stride,
@@ -822,28 +854,40 @@ int ztm_tfm_gravity_batched(int bfr, int di, int dj, int dk, int fh, int fv, int
     if (int e = tfm_drho_batched_launch(di, dj, dk, fh, fv, n_models, rc, sea_dev, moho_dev,
                                         moho_stride, vel_dev, vel_stride, params_dev, drho_dev, s))
         return e;
-    const int M = di * dj, m = p->shape.m;
-    if (m == 0) return tfm_scatter_launch(nullptr, nullptr, 0, M, n_models, g_dev, s);
-    if (int e = prep_ref_stations(s_dev, bfr, di, dj, fh, fv, p->st_f, p->st_d, p->st_c,
-                                  p->out_map, m, p->shape.m_pad, s))
+    return tfm_gravity_of_fields(p, bfr, di, dj, fh, fv, n_models, drho_dev, p->prob.threshold,
+                                 s_dev, g_dev, s);
+}
+
+int ztm_tfm_drho_jvp(int di, int dj, int dk, int fh, int fv, const double* rc,
+                     const double* sea_dev, const double* moho_dev, const double* vel_dev,
+                     double vm, double ro, double dr, float* ddrho_dev, void* stream) {
+    if (int e = tfm_check(di, dj, dk, fh, fv, rc)) return e;
+    if (!sea_dev || !moho_dev || !vel_dev || !ddrho_dev) return fail(ZTM_EINVAL, "null argument");
+    if (!std::isfinite(vm) || !std::isfinite(ro) || !std::isfinite(dr))
+        return fail(ZTM_EINVAL, "vm, ro and dr must be finite");
+    if (int e = check_device(nullptr)) return e;
+    return tfm_drho_jvp_launch(di, dj, dk, fh, fv, rc, sea_dev, moho_dev, vel_dev, vm, ro, dr,
+                               ddrho_dev, (hipStream_t)stream);
+}
+
+int ztm_tfm_gravity_jvp(int bfr, int di, int dj, int dk, int fh, int fv, const double* rc,
+                        const double* sea_dev, const double* moho_dev, const double* vel_dev,
+                        double vm, double ro, double dr, const float* s_dev, float* ddrho_dev,
+                        double* dg_dev, void* stream) {
+    if (int e = tfm_check(di, dj, dk, fh, fv, rc)) return e;
+    if (int e = ref_check(bfr, di, dj, dk, fh, fv)) return e;
+    if (!sea_dev || !moho_dev || !vel_dev || !s_dev || !ddrho_dev || !dg_dev)
+        return fail(ZTM_EINVAL, "null argument");
+    if (!std::isfinite(vm) || !std::isfinite(ro) || !std::isfinite(dr))
+        return fail(ZTM_EINVAL, "vm, ro and dr must be finite");
+    ztm_plan p = nullptr;
+    if (int e = ref_plan(bfr, di, dj, dk, fh, fv, &p)) return e;
+    hipStream_t s = (hipStream_t)stream;
+    if (int e = tfm_drho_jvp_launch(di, dj, dk, fh, fv, rc, sea_dev, moho_dev, vel_dev, vm, ro,
+                                    dr, ddrho_dev, s))
         return e;
-    const int ks = p->tuning.batched_slices > 0 ? p->tuning.batched_slices
-                                                : batched_default_slices(p->shape, n_models);
-    const int split = p->tuning.batched_split;
-    const size_t ws = (batched_workspace(p->shape, n_models, ks, split) + 255) / 256 * 256;
-    const size_t need = ws + sizeof(double) * (size_t)n_models * m;
-    if (need > p->bws_cap) {
-        if (p->bws) (void)hipFree(p->bws);
-        p->bws = nullptr;
-        p->bws_cap = 0;
-        ZTM_HIP(hipMalloc(&p->bws, need));
-        p->bws_cap = need;
-    }
-    double* Y = (double*)((char*)p->bws + ws);
-    if (int e = batched_fwd_launch(p->shape, n_models, ks, split, (const float*)p->st_f, drho_dev,
-                                   p->prob.threshold, p->sy, p->sz, pm_scale(p), p->bws, Y, s))
-        return e;
-    return tfm_scatter_launch(Y, p->out_map, m, M, n_models, g_dev, s);
+    // the derivative fields are already masked by GF's skip at this point
+    return tfm_gravity_of_fields(p, bfr, di, dj, fh, fv, 3, ddrho_dev, 0.0, s_dev, dg_dev, s);
 }
 
 // ----------------------------------------------------------- profiling

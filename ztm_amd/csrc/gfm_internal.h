@@ -229,6 +229,10 @@ int tfm_drho_batched_launch(int di, int dj, int dk, int fh, int fv, int B, const
                             const double* vel, int64_t vel_stride, const double* params,
                             float* drho, hipStream_t stream);
 // g [B][M] = 0 except g[b][out_map[i]] = Y[b][i], i < m.
+// d drho / d(vm, ro, dr) [3][N] at one parameter point (float).
+int tfm_drho_jvp_launch(int di, int dj, int dk, int fh, int fv, const double* rc,
+                        const double* sea, const double* moho, const double* vel, double vm,
+                        double ro, double dr, float* out, hipStream_t stream);
 int tfm_scatter_launch(const double* Y, const int* out_map, int m, int M, int B, double* g,
                        hipStream_t stream);
 

@@ -222,6 +222,65 @@ __global__ __launch_bounds__(256) void tfm_drho_batched_kernel(
     }
 }
 
+// Derivatives of drho with respect to (vm, ro, dr) at one parameter point,
+// out [3][N] = d drho / d(vm, ro, dr) (float). Within each piece of
+// TFM.cpp:86-88 drho is linear in the three parameters:
+//   r0 = ro above m0 + li, t0 ro in the band (t0 = (k - m0 + li) / 2li), else 0;
+//   r1 = ro above m1 + li, dr u (1 + t1) + t1 ro in the band, else dr u
+//   (u = v - vm). The derivative is that of r1 - r0 where the cell is kept:
+// the producer's own filter (|r0 - r1| > .001 and depth > 10 km) and GF's skip
+// of |float drho| <= .001 (GFM_GPU.cu:19), both decided on the value at this
+// point; 0 elsewhere. A finite-difference column differs from it only by
+// cells whose piece or filter changes within the step.
+__global__ __launch_bounds__(256) void tfm_drho_jvp_kernel(
+    int di, int dj, int dk, int fh, int fv, TfmFrame fr, const double* __restrict__ sea,
+    const double* __restrict__ moho, const double* __restrict__ vel, double vm, double ro,
+    double dr, float* __restrict__ out, int64_t N) {
+#pragma clang fp contract(off)
+    const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= N) return;
+    const DrhoCell c = drho_cell(n, di, dj, fh, fv, fr, sea);
+    const double m1 = interp2_unit(moho, c.i, c.j, di, dj);
+    const double v = interp3_unit(vel, c.i, c.j, c.k, di, dj, dk);
+    const float base = drho_from(c, m1, v, vm, ro, dr);
+    double d_vm = 0.0, d_ro = 0.0, d_dr = 0.0;
+    if (fabsf(base) > .001f) {
+        const double k = c.k, li = c.li, m0 = c.m0;
+        // r0
+        if (k > m0 + li) d_ro -= 1.0;
+        else if (fabs(k - m0) < li) d_ro -= (k - m0 + li) / (2 * li);
+        // r1
+        const double u = v - vm;
+        if (k > m1 + li) {
+            d_ro += 1.0;
+        } else if (fabs(k - m1) < li) {
+            const double t1 = (k - m1 + li) / (2 * li);
+            d_vm += -dr * (1 + t1);
+            d_ro += t1;
+            d_dr += u * (1 + t1);
+        } else {
+            d_vm += -dr;
+            d_dr += u;
+        }
+    }
+    out[n] = (float)d_vm;
+    out[N + n] = (float)d_ro;
+    out[2 * N + n] = (float)d_dr;
+}
+
+int tfm_drho_jvp_launch(int di, int dj, int dk, int fh, int fv, const double* rc,
+                        const double* sea, const double* moho, const double* vel, double vm,
+                        double ro, double dr, float* out, hipStream_t stream) {
+    const int64_t N = ((int64_t)(di - 1) * fh + 1) * ((int64_t)(dj - 1) * fh + 1) *
+                      ((int64_t)(dk - 1) * fv + 1);
+    if (N == 0) return ZTM_OK;
+    const TfmFrame fr = tfm_frame(rc);
+    hipLaunchKernelGGL(tfm_drho_jvp_kernel, dim3((unsigned)((N + 255) / 256)), dim3(256), 0,
+                       stream, di, dj, dk, fh, fv, fr, sea, moho, vel, vm, ro, dr, out, N);
+    ZTM_HIP(hipGetLastError());
+    return ZTM_OK;
+}
+
 // g[b][*] = 0, then g[b][out_map[i]] = Y[b][i] for the m prepared stations.
 __global__ void tfm_fill_kernel(double* __restrict__ g, int64_t n) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;

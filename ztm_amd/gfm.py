@@ -81,6 +81,9 @@ SIGNATURES = {
                                   _P]),
     "ztm_tfm_gravity_batched": (_I, [_I, _I, _I, _I, _I, _I, _I, _P, _P, _P, _I64, _P, _I64,
                                      _P, _P, _P, _P, _P]),
+    "ztm_tfm_drho_jvp": (_I, [_I, _I, _I, _I, _I, _P, _P, _P, _P, _D, _D, _D, _P, _P]),
+    "ztm_tfm_gravity_jvp": (_I, [_I, _I, _I, _I, _I, _I, _P, _P, _P, _P, _D, _D, _D, _P, _P,
+                                 _P, _P]),
     "ztm_batch_create": (_I, [ctypes.POINTER(_P), _I, ctypes.POINTER(CProblem),
                               ctypes.POINTER(_I64), ctypes.POINTER(_P)]),
     "ztm_batch_destroy": (_I, [_P]),
@@ -235,6 +238,47 @@ def tfm_gravity_batched(bfr, di, dj, dk, fh, fv, rc, sea, moho, vel, params, s, 
                                         _ptr(moho), ms, _ptr(vel), vs, _ptr(params), _ptr(s),
                                         _ptr(drho), _ptr(g), _stream(stream)))
     return g, drho
+
+
+def tfm_drho_jvp(di, dj, dk, fh, fv, rc, sea, moho, vel, vm, ro, dr, out=None, stream=None):
+    """d drho / d(vm, ro, dr) at one parameter point on the device: returns
+    float32 [3, N] (0 where the producer or GF drops the cell)."""
+    import torch
+    rc = _host_f64(rc, 7, "rc") if np.size(rc) == 7 else _host_f64(rc, 10, "rc")
+    n = ((di - 1) * fh + 1) * ((dj - 1) * fh + 1) * ((dk - 1) * fv + 1)
+    for name, t, size in (("sea", sea, di * dj), ("moho", moho, di * dj),
+                          ("vel", vel, di * dj * dk)):
+        _check_t(t, torch.float64, size, name)
+    if out is None:
+        out = torch.empty(3, n, dtype=torch.float32, device=sea.device)
+    _check_t(out, torch.float32, 3 * n, "ddrho")
+    check(lib().ztm_tfm_drho_jvp(di, dj, dk, fh, fv, rc.ctypes.data, _ptr(sea), _ptr(moho),
+                                 _ptr(vel), vm, ro, dr, _ptr(out), _stream(stream)))
+    return out
+
+
+def tfm_gravity_jvp(bfr, di, dj, dk, fh, fv, rc, sea, moho, vel, vm, ro, dr, s, ddrho=None,
+                    dg=None, stream=None):
+    """dG / d(vm, ro, dr) at the interior stations: sea/moho float64 [di*dj],
+    vel [di*dj*dk], s float32 [di*dj] on the device. Returns (dg float64
+    [3, di*dj], ddrho float32 [3, N])."""
+    import torch
+    rc = _host_f64(rc, 7, "rc") if np.size(rc) == 7 else _host_f64(rc, 10, "rc")
+    n = ((di - 1) * fh + 1) * ((dj - 1) * fh + 1) * ((dk - 1) * fv + 1)
+    M = di * dj
+    for name, t, size in (("sea", sea, M), ("moho", moho, M), ("vel", vel, M * dk)):
+        _check_t(t, torch.float64, size, name)
+    _check_t(s, torch.float32, M, "s")
+    if ddrho is None:
+        ddrho = torch.empty(3, n, dtype=torch.float32, device=sea.device)
+    if dg is None:
+        dg = torch.empty(3, M, dtype=torch.float64, device=sea.device)
+    _check_t(ddrho, torch.float32, 3 * n, "ddrho")
+    _check_t(dg, torch.float64, 3 * M, "dg")
+    check(lib().ztm_tfm_gravity_jvp(bfr, di, dj, dk, fh, fv, rc.ctypes.data, _ptr(sea),
+                                    _ptr(moho), _ptr(vel), vm, ro, dr, _ptr(s), _ptr(ddrho),
+                                    _ptr(dg), _stream(stream)))
+    return dg, ddrho
 
 
 # ------------------------------------------------------------------ plans
