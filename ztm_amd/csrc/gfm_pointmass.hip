@@ -504,21 +504,24 @@ __global__ __launch_bounds__(256) void pm_adj_kernel(
                              part, out, scale, n_cells, direct, qa, blockIdx.x, blockIdx.y, smem);
 }
 
-// One wave per cell of the near-field layers: out[n] = scale * (sum over
-// splits of part[split][n] + correction). The lanes stride over the split
-// partials, over the station columns within (rx, ry) of the cell's column
-// (stations of a column in CSR order) and over the row keys of stations with
-// clamped FP32 row terms; a fixed butterfly then sums the lanes, so the
-// result is deterministic. Every lane returns it. MODE (kCorrPmNear or
-// kCorrPrism) is na.mode fixed at compile time, which keeps the FP64 prism
-// formula out of the point-mass kernel's registers.
+// One half-wave (32 lanes) per cell of the near-field layers: out[n] =
+// scale * (sum over splits of part[split][n] + correction). The lanes stride
+// over the split partials, over the station columns within (rx, ry) of the
+// cell's column (stations of a column in CSR order) and over the row keys of
+// stations with clamped FP32 row terms; a fixed butterfly within the half
+// then sums them, so the result is deterministic. Every lane of the half
+// returns it. MODE (kCorrPmNear or kCorrPrism) is na.mode fixed at compile
+// time, which keeps the FP64 prism formula out of the point-mass kernel's
+// registers.
+constexpr int kNearLanes = 32;   // lanes per near-field cell
+
 template <bool QUAD, int MODE>
 __device__ __forceinline__ double pm_adj_reduce_wave(const double* __restrict__ part,
                                                      int n_splits, int64_t n_cells, int64_t n,
                                                      const float* __restrict__ r_f,
                                                      const NearArgs& na, int lane) {
     double s = 0.0;
-    for (int k = lane; k < n_splits; k += 64) s += part[(size_t)k * n_cells + n];
+    for (int k = lane; k < n_splits; k += kNearLanes) s += part[(size_t)k * n_cells + n];
     const int ii = (int)(n % na.nx);
     const int64_t rw = n / na.nx;
     const int jj = (int)(rw % na.ny);
@@ -528,7 +531,7 @@ __device__ __forceinline__ double pm_adj_reduce_wave(const double* __restrict__ 
         const int wcols = na.nx + 2 * na.cmx;
         const int bw = 2 * na.rx + 1;
         const int total = bw * (2 * na.ry + 1);
-        for (int t = lane; t < total; t += 64) {
+        for (int t = lane; t < total; t += kNearLanes) {
             const int ci = ii - na.rx + t % bw;
             const int cj = jj - na.ry + t / bw;
             if (cj < -na.cmy || cj >= na.ny + na.cmy) continue;
@@ -548,7 +551,7 @@ __device__ __forceinline__ double pm_adj_reduce_wave(const double* __restrict__ 
         const int wkey = na.ny + 2 * na.rmy;
         const int bj = 2 * na.jr + 1;
         const int total = bj * (2 * na.kr + 1);
-        for (int t = lane; t < total; t += 64) {
+        for (int t = lane; t < total; t += kNearLanes) {
             const int cj = jj - na.jr + t % bj;
             const int ck = kk - na.kr + t / bj;
             if (ck < -na.rmz || ck >= na.nz + na.rmz) continue;
@@ -566,7 +569,7 @@ __device__ __forceinline__ double pm_adj_reduce_wave(const double* __restrict__ 
         }
     }
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+    for (int off = kNearLanes / 2; off > 0; off >>= 1) s += __shfl_xor(s, off);
     return s;
 }
 
@@ -594,16 +597,17 @@ __global__ __launch_bounds__(256) void pm_adj_sum_kernel(const double* __restric
     pm_adj_sum_one<T>(part, n_splits, n_cells, n, scale, out);
 }
 
-// Cells n0 .. n0+count-1 (the near-field layers), one wave each (block =
-// 256 = 4 cells): sum plus corrections.
+// Cells n0 .. n0+count-1 (the near-field layers), one half-wave each (block
+// = 256 = 8 cells): sum plus corrections.
 template <bool QUAD, int MODE>
 __global__ __launch_bounds__(256) void pm_adj_reduce_kernel(const double* __restrict__ part,
                                      int n_splits, int64_t n_cells, int64_t n0, int64_t count,
                                      double scale, float* __restrict__ out,
                                      const float* __restrict__ r_f, NearArgs na) {
-    const int lane = threadIdx.x & 63;
-    for (int64_t t = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); t < count;
-         t += (int64_t)gridDim.x * 4) {   // uniform per wave
+    const int lane = threadIdx.x % kNearLanes;
+    constexpr int kPerBlock = 256 / kNearLanes;
+    for (int64_t t = (int64_t)blockIdx.x * kPerBlock + threadIdx.x / kNearLanes; t < count;
+         t += (int64_t)gridDim.x * kPerBlock) {   // uniform per half-wave
         const double v = pm_adj_reduce_wave<QUAD, MODE>(part, n_splits, n_cells, n0 + t, r_f, na,
                                                         lane);
         if (lane == 0) out[n0 + t] = (float)(scale * v);
@@ -730,13 +734,14 @@ __global__ __launch_bounds__(256) void pm_adj_sum_grouped_kernel(
     pm_adj_sum_one<float>(q.part, q.splits, q.n_cells, n, q.scale, out + q.cell_off);
 }
 
-// One wave per near-field cell (block = 256 = 4 cells).
+// One half-wave per near-field cell (block = 256 = 8 cells).
 __global__ __launch_bounds__(256) void pm_adj_reduce_grouped_kernel(
     const GAdjDesc* __restrict__ d, const int64_t* __restrict__ near_start, int np,
     int64_t total, const float* __restrict__ r, float* __restrict__ out) {
-    const int lane = threadIdx.x & 63;
-    for (int64_t t = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); t < total;
-         t += (int64_t)gridDim.x * 4) {   // uniform per wave
+    const int lane = threadIdx.x % kNearLanes;
+    constexpr int kPerBlock = 256 / kNearLanes;
+    for (int64_t t = (int64_t)blockIdx.x * kPerBlock + threadIdx.x / kNearLanes; t < total;
+         t += (int64_t)gridDim.x * kPerBlock) {   // uniform per half-wave
         const int p = find_problem(near_start, np, t);
         const GAdjDesc& q = d[p];
         const int64_t n = q.near_off + (t - near_start[p]);
@@ -1003,7 +1008,7 @@ int pm_adj_launch(const PmShape& s, const PmTuning& t, int precision,
         ZTM_HIP(hipGetLastError());
         if (klo <= khi && precision == ZTM_FP32) {
             const int64_t n0 = klo * plane, count = (int64_t)(khi - klo + 1) * plane;
-            const unsigned nb = (unsigned)std::min<int64_t>((count + 3) / 4, 1 << 20);
+            const unsigned nb = (unsigned)std::min<int64_t>((count + 7) / 8, 1 << 20);
 #define ZTM_ADJ_NEAR(Q, M)                                                                 \
     hipLaunchKernelGGL((pm_adj_reduce_kernel<Q, M>), dim3(nb), dim3(256), 0, stream, part,  \
                        splits, s.n_cells, n0, count, scale, (float*)out, (const float*)r, na)
@@ -1157,7 +1162,7 @@ int pm_adj_grouped_launch(const GroupedPass& gp, const float* r, float* out,
     ZTM_HIP(hipGetLastError());
     if (gp.total_near > 0) {
         hipLaunchKernelGGL(pm_adj_reduce_grouped_kernel,
-                           dim3((unsigned)std::min<int64_t>((gp.total_near + 3) / 4, 1 << 20)),
+                           dim3((unsigned)std::min<int64_t>((gp.total_near + 7) / 8, 1 << 20)),
                            dim3(256), 0, stream,
                            (const GAdjDesc*)gp.desc, gp.near_start, gp.np, gp.total_near, r,
                            out);
