@@ -289,7 +289,7 @@ __global__ __launch_bounds__(256) void batched_fwd_kernel(
 #pragma unroll
             for (int i = 0; i < kPerPair; ++i) {
                 __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);        // MFMA
-                __builtin_amdgcn_sched_group_barrier(0x002, SPLIT == 3 ? 5 : 7, 0);   // VALU
+                __builtin_amdgcn_sched_group_barrier(0x002, SPLIT == 3 ? 4 : 6, 0);   // VALU
             }
         }
         __builtin_amdgcn_s_waitcnt(kWaitVm0);
