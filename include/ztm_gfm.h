@@ -255,8 +255,9 @@ int ztm_tfm_drho_jvp(int di, int dj, int dk, int fh, int fv, const double* rc,
                      double vm, double ro, double dr, float* ddrho_dev, void* stream);
 /* The gravity of those three fields at the interior stations (station
  * heights s_dev, float[di*dj]): dg_dev[3][di*dj] = dG / d(vm, ro, dr) (FP64;
- * 0 outside the interior), through the split-BF16 ensemble forward as in
- * ztm_tfm_gravity_batched; ddrho_dev[3][N] receives the fields. */
+ * 0 outside the interior), one FP32 GF forward per field with its FP64
+ * near-field correction (as ztm_gf, without GF's skip, which the fields have
+ * already had); ddrho_dev[3][N] receives the fields. */
 int ztm_tfm_gravity_jvp(int bfr, int di, int dj, int dk, int fh, int fv, const double* rc,
                         const double* sea_dev, const double* moho_dev, const double* vel_dev,
                         double vm, double ro, double dr, const float* s_dev, float* ddrho_dev,

@@ -886,8 +886,32 @@ int ztm_tfm_gravity_jvp(int bfr, int di, int dj, int dk, int fh, int fv, const d
     if (int e = tfm_drho_jvp_launch(di, dj, dk, fh, fv, rc, sea_dev, moho_dev, vel_dev, vm, ro,
                                     dr, ddrho_dev, s))
         return e;
-    // the derivative fields are already masked by GF's skip at this point
-    return tfm_gravity_of_fields(p, bfr, di, dj, fh, fv, 3, ddrho_dev, 0.0, s_dev, dg_dev, s);
+    // One FP32 GF forward (with its FP64 near-field correction) per field.
+    // The fields are already masked by GF's skip at the parameter point, so
+    // neither the main kernel nor the correction thresholds them again.
+    const int M = di * dj, m = p->shape.m;
+    if (m == 0) return tfm_scatter_launch(nullptr, nullptr, 0, M, 3, dg_dev, s);
+    if (int e = prep_ref_stations(s_dev, bfr, di, dj, fh, fv, p->st_f, p->st_d, p->st_c,
+                                  p->out_map, m, p->shape.m_pad, s))
+        return e;
+    const size_t need = sizeof(double) * 3 * (size_t)m;
+    if (need > p->bws_cap) {
+        if (p->bws) (void)hipFree(p->bws);
+        p->bws = nullptr;
+        p->bws_cap = 0;
+        ZTM_HIP(hipMalloc(&p->bws, need));
+        p->bws_cap = need;
+    }
+    double* Y = (double*)p->bws;
+    NearArgs na = near_args(p);
+    na.thr = 0.f;
+    const int64_t N = p->shape.n_cells;
+    for (int f = 0; f < 3; ++f)
+        if (int e = pm_fwd_launch(p->shape, p->tuning, ZTM_FP32, p->st_f, ddrho_dev + f * N,
+                                  p->sy, p->sz, 0.0, pm_scale(p), p->part, Y + (size_t)f * m,
+                                  nullptr, nullptr, na, s))
+            return e;
+    return tfm_scatter_launch(Y, p->out_map, m, M, 3, dg_dev, s);
 }
 
 // ----------------------------------------------------------- profiling
