@@ -548,6 +548,12 @@ def tfm_leg(stream, args, reps=20):
                                                t64(w.moho), vel, w.vm, w.ro, w.dr, s32,
                                                ddrho=ddB, dg=dgB, stream=stream), 5,
                    torch.cuda.synchronize)
+    rB = t64(W.normal(W.SEED0 + 31, 1, M, 1.0))
+    vjB = torch.empty(3, dtype=torch.float64, device=dev)
+    t_vjp = _timed(lambda: gfm.tfm_gravity_vjp(w.bfr, w.di, w.dj, w.dk, w.fh, w.fv, w.rc, sea,
+                                               t64(w.moho), vel, w.vm, w.ro, w.dr, s32, rB,
+                                               ddrho=ddB, out=vjB, stream=stream), 5,
+                   torch.cuda.synchronize)
     out = {
         "workload": "tfm_fm: TFM.cpp:83-94 at the reference grid, 241,865 fine cells x 1,681 "
                     "interior stations, producer + gravity per FM call",
@@ -560,6 +566,8 @@ def tfm_leg(stream, args, reps=20):
                                   "vs_single_device_calls": t_dev * B / t_batch},
         "gravity_jvp_3_params": {"ms_per_call": t_jvp * 1e3,
                                  "note": "exact dG/d(vm, ro, dr): derivative fields + forward"},
+        "gravity_vjp_3_params": {"ms_per_call": t_vjp * 1e3,
+                                 "note": "r . dG/d(vm, ro, dr): derivative fields + adjoint"},
         "ms_gfm_program": None if t_prog is None else t_prog * 1e3,
         "cpu_port": {"ms_producer": t_prod_cpu * 1e3,
                      "ms_gravity_thread_per_row": t_grav_rows * 1e3, "rows": nj,

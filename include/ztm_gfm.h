@@ -262,6 +262,15 @@ int ztm_tfm_gravity_jvp(int bfr, int di, int dj, int dk, int fh, int fv, const d
                         const double* sea_dev, const double* moho_dev, const double* vel_dev,
                         double vm, double ro, double dr, const float* s_dev, float* ddrho_dev,
                         double* dg_dev, void* stream);
+/* Its adjoint: out_dev[3] = sum over interior stations m of r_dev[m] *
+ * dG[m]/d(vm, ro, dr) (FP64), the gradient of r . G for a station residual
+ * r_dev[di*dj] (exterior entries ignored), computed as the FP64 dot products
+ * of G^T r (FP32 adjoint) with the three derivative fields (ddrho_dev[3][N]
+ * receives them). One adjoint pass instead of three forwards. */
+int ztm_tfm_gravity_vjp(int bfr, int di, int dj, int dk, int fh, int fv, const double* rc,
+                        const double* sea_dev, const double* moho_dev, const double* vel_dev,
+                        double vm, double ro, double dr, const float* s_dev, const double* r_dev,
+                        float* ddrho_dev, double* out_dev, void* stream);
 
 /* ------------------------------------------------- uneven batches */
 /* A batch of independent point-mass problems (each its own grid, stations and

@@ -168,3 +168,30 @@ def test_gravity_jvp_matches_gf_oracle(dev, orc):
     dd2 = gfm.tfm_drho_jvp(w.di, w.dj, w.dk, w.fh, w.fv, w.rc, _t64(w.sea, dev),
                            _t64(w.moho, dev), _t64(w.vel, dev), w.vm, w.ro, w.dr).cpu().numpy()
     assert np.array_equal(dd, dd2)
+
+
+def test_gravity_vjp_is_the_adjoint_of_the_jvp(dev, orc):
+    """r . (dG/db) from the adjoint path (G^T r dotted with the derivative
+    fields) equals the same sum formed from the forward columns, for a
+    seeded residual (exterior entries must not matter)."""
+    import torch
+    w = W.tfm_fm()
+    s32 = torch.from_numpy(w.surf.astype(np.float32)).to(dev)
+    args = (w.bfr, w.di, w.dj, w.dk, w.fh, w.fv, w.rc, _t64(w.sea, dev), _t64(w.moho, dev),
+            _t64(w.vel, dev), w.vm, w.ro, w.dr, s32)
+    dg, dd = gfm.tfm_gravity_jvp(*args)
+    r = W.normal(W.SEED0 + 31, 1, w.di * w.dj, 1.0)
+    out, dd2 = gfm.tfm_gravity_vjp(*args, _t64(r, dev))
+    out, dg = out.cpu().numpy(), dg.cpu().numpy()
+    assert np.array_equal(dd.cpu().numpy(), dd2.cpu().numpy())
+    interior = np.zeros((w.dj, w.di), bool)
+    interior[w.bfr:w.dj - w.bfr, w.bfr:w.di - w.bfr] = True
+    interior = interior.ravel()
+    for q in range(3):
+        terms = r[interior] * dg[q][interior]
+        assert abs(out[q] - terms.sum()) <= 1e-6 * np.abs(terms).sum(), q
+    # exterior residuals are ignored
+    r2 = r.copy()
+    r2[~interior] = 1e3
+    out2, _ = gfm.tfm_gravity_vjp(*args, _t64(r2, dev))
+    assert np.array_equal(out2.cpu().numpy(), out)

@@ -914,6 +914,54 @@ int ztm_tfm_gravity_jvp(int bfr, int di, int dj, int dk, int fh, int fv, const d
     return tfm_scatter_launch(Y, p->out_map, m, M, 3, dg_dev, s);
 }
 
+int ztm_tfm_gravity_vjp(int bfr, int di, int dj, int dk, int fh, int fv, const double* rc,
+                        const double* sea_dev, const double* moho_dev, const double* vel_dev,
+                        double vm, double ro, double dr, const float* s_dev, const double* r_dev,
+                        float* ddrho_dev, double* out_dev, void* stream) {
+    if (int e = tfm_check(di, dj, dk, fh, fv, rc)) return e;
+    if (int e = ref_check(bfr, di, dj, dk, fh, fv)) return e;
+    if (!sea_dev || !moho_dev || !vel_dev || !s_dev || !r_dev || !ddrho_dev || !out_dev)
+        return fail(ZTM_EINVAL, "null argument");
+    if (!std::isfinite(vm) || !std::isfinite(ro) || !std::isfinite(dr))
+        return fail(ZTM_EINVAL, "vm, ro and dr must be finite");
+    ztm_plan p = nullptr;
+    if (int e = ref_plan(bfr, di, dj, dk, fh, fv, &p)) return e;
+    hipStream_t s = (hipStream_t)stream;
+    if (int e = tfm_drho_jvp_launch(di, dj, dk, fh, fv, rc, sea_dev, moho_dev, vel_dev, vm, ro,
+                                    dr, ddrho_dev, s))
+        return e;
+    const int64_t N = p->shape.n_cells;
+    const int m = p->shape.m;
+    if (m == 0) return tfm_scatter_launch(nullptr, nullptr, 0, 3, 1, out_dev, s);   // zeros
+    if (int e = prep_ref_stations(s_dev, bfr, di, dj, fh, fv, p->st_f, p->st_d, p->st_c,
+                                  p->out_map, m, p->shape.m_pad, s))
+        return e;
+    // a = G^T r on the kept-cell raster (FP32 adjoint; the producer's depth
+    // filter keeps cells at least a coarse cell below every station, where
+    // the FP32 pair kernel needs no near-field correction), then the three
+    // FP64 dot products with the derivative fields
+    const size_t need = tfm_vjp_workspace(N, m);
+    if (need > p->bws_cap) {
+        if (p->bws) (void)hipFree(p->bws);
+        p->bws = nullptr;
+        p->bws_cap = 0;
+        ZTM_HIP(hipMalloc(&p->bws, need));
+        p->bws_cap = need;
+    }
+    float* a = (float*)p->bws;
+    float* r_int = a + N;
+    double* part = (double*)((char*)p->bws + ((size_t)(N + m) * sizeof(float) + 255) / 256 * 256);
+    if (int e = ensure_workspace(p, std::max(pm_fwd_workspace(p->shape, p->tuning, ZTM_FP32, false),
+                                             pm_adj_workspace(p->shape, p->tuning, ZTM_FP32, true,
+                                                              false))))
+        return e;
+    if (int e = tfm_gather_launch(r_dev, p->out_map, m, r_int, s)) return e;
+    if (int e = pm_adj_launch(p->shape, p->tuning, ZTM_FP32, p->st_f, r_int, p->sy, p->sz,
+                              pm_scale(p), p->part, a, near_args(p), s))
+        return e;
+    return tfm_dot3_launch(a, ddrho_dev, N, part, out_dev, s);
+}
+
 // ----------------------------------------------------------- profiling
 int ztm_profile_enable(int on) {
     std::lock_guard<std::mutex> lk(g_mu);

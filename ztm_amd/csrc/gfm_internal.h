@@ -233,6 +233,14 @@ int tfm_drho_batched_launch(int di, int dj, int dk, int fh, int fv, int B, const
 int tfm_drho_jvp_launch(int di, int dj, int dk, int fh, int fv, const double* rc,
                         const double* sea, const double* moho, const double* vel, double vm,
                         double ro, double dr, float* out, hipStream_t stream);
+// Residual gather to the reference plan's station order, and FP64 dot
+// products out[q] = sum_n a[n] f[q][n] (q < 3) in a fixed order;
+// tfm_vjp_workspace: bytes for a[N], r_int[m] and the dot partials.
+int tfm_gather_launch(const double* r, const int* out_map, int m, float* r_int,
+                      hipStream_t stream);
+int tfm_dot3_launch(const float* a, const float* f, int64_t N, double* part, double* out,
+                    hipStream_t stream);
+size_t tfm_vjp_workspace(int64_t N, int m);
 int tfm_scatter_launch(const double* Y, const int* out_map, int m, int M, int B, double* g,
                        hipStream_t stream);
 

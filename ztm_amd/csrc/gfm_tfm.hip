@@ -281,6 +281,76 @@ int tfm_drho_jvp_launch(int di, int dj, int dk, int fh, int fv, const double* rc
     return ZTM_OK;
 }
 
+// r_int[i] = r[out_map[i]] (float) for the m prepared stations: a residual on
+// the di x dj station raster in the order of the reference plan's stations.
+__global__ void tfm_gather_kernel(const double* __restrict__ r, const int* __restrict__ out_map,
+                                  int m, float* __restrict__ r_int) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < m) r_int[i] = (float)r[out_map[i]];
+}
+
+// Dot products of a[N] with the three fields f[3][N], FP64, in a fixed
+// order: block b sums the elements b, b + kDotBlocks*256, ... per thread,
+// then through LDS (tree); tfm_dot_final adds the kDotBlocks partials in
+// block order.
+constexpr int kDotBlocks = 512;
+
+__global__ __launch_bounds__(256) void tfm_dot3_kernel(const float* __restrict__ a,
+                                                       const float* __restrict__ f, int64_t N,
+                                                       double* __restrict__ part) {
+    __shared__ double red[3][256];
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+    for (int64_t n = (int64_t)blockIdx.x * 256 + threadIdx.x; n < N;
+         n += (int64_t)kDotBlocks * 256) {
+        const double an = a[n];
+        s0 += an * (double)f[n];
+        s1 += an * (double)f[N + n];
+        s2 += an * (double)f[2 * N + n];
+    }
+    red[0][threadIdx.x] = s0;
+    red[1][threadIdx.x] = s1;
+    red[2][threadIdx.x] = s2;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w)
+#pragma unroll
+            for (int q = 0; q < 3; ++q) red[q][threadIdx.x] += red[q][threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x < 3) part[threadIdx.x * kDotBlocks + blockIdx.x] = red[threadIdx.x][0];
+}
+
+__global__ void tfm_dot_final_kernel(const double* __restrict__ part, double* __restrict__ out) {
+    const int q = threadIdx.x;
+    if (q >= 3) return;
+    double s = 0.0;
+    for (int b = 0; b < kDotBlocks; ++b) s += part[q * kDotBlocks + b];
+    out[q] = s;
+}
+
+size_t tfm_vjp_workspace(int64_t N, int m) {
+    // a[N] float, r_int[m] float, block partials [3][kDotBlocks] double
+    return (((size_t)N + (size_t)m) * sizeof(float) + 255) / 256 * 256 +
+           (size_t)3 * kDotBlocks * sizeof(double);
+}
+
+int tfm_gather_launch(const double* r, const int* out_map, int m, float* r_int,
+                      hipStream_t stream) {
+    if (m == 0) return ZTM_OK;
+    hipLaunchKernelGGL(tfm_gather_kernel, dim3((m + 255) / 256), dim3(256), 0, stream, r,
+                       out_map, m, r_int);
+    ZTM_HIP(hipGetLastError());
+    return ZTM_OK;
+}
+
+int tfm_dot3_launch(const float* a, const float* f, int64_t N, double* part, double* out,
+                    hipStream_t stream) {
+    hipLaunchKernelGGL(tfm_dot3_kernel, dim3(kDotBlocks), dim3(256), 0, stream, a, f, N, part);
+    hipLaunchKernelGGL(tfm_dot_final_kernel, dim3(1), dim3(64), 0, stream, part, out);
+    ZTM_HIP(hipGetLastError());
+    return ZTM_OK;
+}
+
 // g[b][*] = 0, then g[b][out_map[i]] = Y[b][i] for the m prepared stations.
 __global__ void tfm_fill_kernel(double* __restrict__ g, int64_t n) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;

@@ -84,6 +84,8 @@ SIGNATURES = {
     "ztm_tfm_drho_jvp": (_I, [_I, _I, _I, _I, _I, _P, _P, _P, _P, _D, _D, _D, _P, _P]),
     "ztm_tfm_gravity_jvp": (_I, [_I, _I, _I, _I, _I, _I, _P, _P, _P, _P, _D, _D, _D, _P, _P,
                                  _P, _P]),
+    "ztm_tfm_gravity_vjp": (_I, [_I, _I, _I, _I, _I, _I, _P, _P, _P, _P, _D, _D, _D, _P, _P,
+                                 _P, _P, _P]),
     "ztm_batch_create": (_I, [ctypes.POINTER(_P), _I, ctypes.POINTER(CProblem),
                               ctypes.POINTER(_I64), ctypes.POINTER(_P)]),
     "ztm_batch_destroy": (_I, [_P]),
@@ -279,6 +281,31 @@ def tfm_gravity_jvp(bfr, di, dj, dk, fh, fv, rc, sea, moho, vel, vm, ro, dr, s, 
                                     _ptr(moho), _ptr(vel), vm, ro, dr, _ptr(s), _ptr(ddrho),
                                     _ptr(dg), _stream(stream)))
     return dg, ddrho
+
+
+def tfm_gravity_vjp(bfr, di, dj, dk, fh, fv, rc, sea, moho, vel, vm, ro, dr, s, r, ddrho=None,
+                    out=None, stream=None):
+    """Gradient of r . G with respect to (vm, ro, dr) for a station residual r
+    (float64 [di*dj] on the device; exterior entries ignored). Returns (out
+    float64 [3], ddrho float32 [3, N])."""
+    import torch
+    rc = _host_f64(rc, 7, "rc") if np.size(rc) == 7 else _host_f64(rc, 10, "rc")
+    n = ((di - 1) * fh + 1) * ((dj - 1) * fh + 1) * ((dk - 1) * fv + 1)
+    M = di * dj
+    for name, t, size in (("sea", sea, M), ("moho", moho, M), ("vel", vel, M * dk),
+                          ("r", r, M)):
+        _check_t(t, torch.float64, size, name)
+    _check_t(s, torch.float32, M, "s")
+    if ddrho is None:
+        ddrho = torch.empty(3, n, dtype=torch.float32, device=sea.device)
+    if out is None:
+        out = torch.empty(3, dtype=torch.float64, device=sea.device)
+    _check_t(ddrho, torch.float32, 3 * n, "ddrho")
+    _check_t(out, torch.float64, 3, "out")
+    check(lib().ztm_tfm_gravity_vjp(bfr, di, dj, dk, fh, fv, rc.ctypes.data, _ptr(sea),
+                                    _ptr(moho), _ptr(vel), vm, ro, dr, _ptr(s), _ptr(r),
+                                    _ptr(ddrho), _ptr(out), _stream(stream)))
+    return out, ddrho
 
 
 # ------------------------------------------------------------------ plans
