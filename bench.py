@@ -291,10 +291,11 @@ PEAK_BF16_TFLOPS = 2500.0     # MI355X dense BF16 MFMA, published
 
 
 def batched_leg(stream, args):
-    """Ensemble forward on split-BF16 MFMA (ensemble_256: 96x96x48 cells of 1 km,
-    9,216 stations at z = +10 m, 256 smoothed models), both splits, against the
-    FP64 single-model kernel on models 0, 85, 170, 255 (extrapolated to 256
-    models for the FP64 rate)."""
+    """Ensemble forward and adjoint on split-BF16 MFMA (ensemble_256: 96x96x48
+    cells of 1 km, 9,216 stations at z = +10 m, 256 smoothed models, 256
+    residual vectors N(0, 0.5)), both splits, against the FP64 single-model
+    kernels on models / vectors 0, 85, 170, 255 (extrapolated to 256 for the
+    FP64 rates)."""
     import torch
     from ztm_amd import gfm, workloads as W
     dev = torch.device("cuda:0")
@@ -318,6 +319,23 @@ def batched_leg(stream, args):
             t64 += time.perf_counter() - t0
         out["fp64_gpu"] = {"value": pm / (t64 / len(check) * B), "unit": "pair-models/s",
                            "kind": "FP64 single-model kernel, timed on 4 models"}
+        R = np.stack([W.normal(W.SEED0 + 40, b, w.n_stations, 0.5) for b in range(B)]
+                     ).astype(np.float32)
+        Rt = torch.from_numpy(R).to(dev)
+        X = torch.empty(B, w.grid.n_cells, dtype=torch.float32, device=dev)
+        a64 = {}
+        ta64 = 0.0
+        for b in check:
+            x = torch.from_numpy(R[b].astype(np.float64)).to(dev)
+            p.adjoint(x)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            a64[b] = p.adjoint(x).cpu().numpy()
+            ta64 += time.perf_counter() - t0
+        out["fp64_gpu_adjoint"] = {"value": pm / (ta64 / len(check) * B),
+                                   "unit": "pair-models/s",
+                                   "kind": "FP64 single-vector adjoint, timed on 4 vectors"}
+        fp64_fa = 2.0 * pm / ((t64 + ta64) / len(check) * B)
         for split in (3, 2):
             p.set_batched_tuning(split=split)
             gfm.profile_reset()
@@ -338,6 +356,25 @@ def batched_leg(stream, args):
                                 "flop": "issued MFMA: 2 x pair-models x products",
                                 "products": 6 if split == 3 else 3}}
             leg["speedup_vs_fp64_gpu"] = leg["value"] / out["fp64_gpu"]["value"]
+            gfm.profile_reset()
+            gfm.profile_enable(True)
+            ta = _timed(lambda: p.adjoint_batched(Rt, out=X, stream=stream), 3,
+                        torch.cuda.synchronize)
+            gfm.profile_enable(False)
+            ms, n = gfm.profile_read("batched_adj")
+            gfm.profile_reset()
+            kma = ms / max(n, 1)
+            Xh = X.cpu().numpy()
+            leg["adjoint"] = {
+                "value": pm / ta, "ms_per_call": ta * 1e3, "kernel_ms": kma,
+                "rel_err_vs_fp64": max(relerr(Xh[b], a64[b]) for b in check),
+                "roofline": {"bound": "mfma", "unit": "TFLOP/s", "peak": PEAK_BF16_TFLOPS,
+                             "achieved": mfma_flop / (kma * 1e-3) / 1e12,
+                             "frac": mfma_flop / (kma * 1e-3) / 1e12 / PEAK_BF16_TFLOPS},
+                "speedup_vs_fp64_gpu": (pm / ta) / out["fp64_gpu_adjoint"]["value"]}
+            fa = 2.0 * pm / (t + ta)
+            leg["forward_plus_adjoint"] = {"value": fa, "unit": "pair-models/s",
+                                           "speedup_vs_fp64_gpu": fa / fp64_fa}
             out[f"split{split}"] = leg
     return out
 

@@ -175,8 +175,20 @@ int ztm_plan_adjoint_stored(ztm_plan plan, const float* r_dev, float* out_dev,
  * of a cell centre (true for stations above the grid's top face). */
 int ztm_plan_forward_batched(ztm_plan plan, int n_models, const float* rho_dev,
                              double* g_dev, void* stream);
+/* Batched (ensemble) adjoint on MFMA, point-mass formula (no reference
+ * counterpart; the transpose of ztm_plan_forward_batched, e.g. misfit
+ * gradients of an ensemble, or J^T r over many residuals): r_dev float
+ * [n_vectors][M] -> out_dev float[n_vectors][N], out[b] = G^T r[b] with G
+ * the plan's kernel (scale included, no threshold: the exact transpose of
+ * G). Same split-BF16 scheme and split setting as the batched forward, the
+ * same G entries (bit-identical FP32 evaluation); K-slices over stations
+ * are chosen from the problem shape. n_vectors <= 256 per call; no FP64
+ * near-field pass (same assumption as the batched forward). */
+int ztm_plan_adjoint_batched(ztm_plan plan, int n_vectors, const float* r_dev,
+                             float* out_dev, void* stream);
 /* Batched forward tuning: K-slices (0 = choose from the problem shape) and
- * BF16 split (2 or 3; 0 = keep). */
+ * BF16 split (2 or 3; 0 = keep). The split applies to the batched adjoint
+ * too. */
 int ztm_plan_set_batched_tuning(ztm_plan plan, int slices, int split);
 
 /* Smoothness operator of the regularised misfit 0.5||G rho-d||^2 +

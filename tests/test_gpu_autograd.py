@@ -119,3 +119,27 @@ def test_smoothness_value_and_gradient(dev, case):
         s.backward()
         assert abs(float(s.detach()) - want) <= 1e-12 * want
         assert np.max(np.abs(x.grad.cpu().numpy() - grad_ltl(w.rho, shape))) < 1e-9
+
+
+def test_gravity_batched_backward_is_batched_adjoint(dev):
+    """Ensemble autograd: the forward equals ztm_plan_forward_batched and the
+    backward of sum(W * Y) equals the batched adjoint of W (and, to the
+    split's precision, the FP64 single-vector adjoint)."""
+    import torch
+    w = W.pm_cpu_small()
+    B = 7
+    P = np.stack([W.uniform(21, b, w.grid.n_cells, -300.0, 300.0) for b in range(B)]
+                 ).astype(np.float32)
+    Wt = np.stack([W.normal(22, b, w.n_stations, 0.5) for b in range(B)])
+    with gfm.Plan(w.grid, w.stations, kappa=w.kappa) as p:
+        x = torch.from_numpy(P).to(dev).requires_grad_(True)
+        Y = torch_ops.gravity_batched(x, p)
+        (Y * torch.from_numpy(Wt).to(dev)).sum().backward()
+        grad = x.grad.cpu().numpy()
+        Yb = p.forward_batched(torch.from_numpy(P).to(dev)).cpu().numpy()
+        Xb = p.adjoint_batched(torch.from_numpy(Wt.astype(np.float32)).to(dev)).cpu().numpy()
+        a64 = [p.adjoint(torch.from_numpy(Wt[b]).to(dev)).cpu().numpy() for b in range(B)]
+    assert np.array_equal(Y.detach().cpu().numpy(), Yb)
+    assert np.array_equal(grad, Xb)
+    for b in range(B):
+        assert np.max(np.abs(grad[b] - a64[b])) < 1e-5 * np.max(np.abs(a64[b])), b

@@ -95,3 +95,93 @@ def test_batched_narrow_rows(dev, orc):
         for b in (0, 4, 8):
             want = orc.pm_forward(g, W.KAPPA_SI, P[b].astype(np.float64), st)
             assert relerr(Y[b], want) < 2e-6, (nx, b)
+
+
+# ------------------------------------------------------------ batched adjoint
+# G^T R on the same split-BF16 scheme. The 3-way split is held to the product
+# bar (1e-5 of max |G^T r| per vector); hi/lo (~2^-16 per entry) to 1e-4,
+# since zero-mean residuals cancel in the sum while the entry errors do not.
+
+def residuals(seed, B, m, sigma=0.5):
+    return np.stack([W.normal(seed, 300 + b, m, sigma) for b in range(B)]).astype(np.float32)
+
+
+def test_batched_adjoint_small_vs_single_and_oracle(dev, orc):
+    import torch
+    w = W.pm_cpu_small()
+    R = residuals(11, 37, w.n_stations)
+    with gfm.Plan(w.grid, w.stations, kappa=w.kappa) as p:
+        X = p.adjoint_batched(torch.from_numpy(R).to(dev)).cpu().numpy()
+        singles = [p.adjoint(torch.from_numpy(R[b].astype(np.float64)).to(dev)).cpu().numpy()
+                   for b in range(37)]
+    assert X.shape == (37, w.grid.n_cells) and X.dtype == np.float32
+    for b in range(37):
+        assert relerr(X[b], singles[b]) < 1e-5, b
+    for b in (0, 36):
+        want = orc.pm_adjoint(w.grid, w.kappa, w.stations, R[b].astype(np.float64))
+        assert relerr(X[b], want) < 1e-5, b
+
+
+def test_batched_adjoint_ragged(dev, orc):
+    """N = 555 (a partial tile of 128 cells), 13 and 40 stations (partial
+    station chunks, one or three chunks), 5 vectors, negative spacings; and
+    nx < 8 grids."""
+    import torch
+    cases = [((37, 5, 3, 100.0, -50.0, 30.0, -70.0, 45.0, -25.0), 13),
+             ((37, 5, 3, 100.0, -50.0, 30.0, -70.0, 45.0, -25.0), 40),
+             ((5, 7, 9, 10.0, 20.0, -15.0, 30.0, 35.0, -25.0), 40),
+             ((3, 4, 20, 10.0, 20.0, -15.0, 30.0, 35.0, -25.0), 77)]
+    for gi, (gargs, m) in enumerate(cases):
+        g = W.Grid(*gargs)
+        c = g.centres()
+        st = np.stack([W.uniform(8 + gi, a, m, c[:, a].min() - 200, c[:, a].max() + 200)
+                       for a in range(2)] + [W.uniform(8 + gi, 2, m, 80.0, 300.0)], axis=1)
+        R = residuals(12 + gi, 5, m)
+        with gfm.Plan(g, st, kappa=W.KAPPA_SI) as p:
+            X = p.adjoint_batched(torch.from_numpy(R).to(dev)).cpu().numpy()
+        for b in range(5):
+            want = orc.pm_adjoint(g, W.KAPPA_SI, st, R[b].astype(np.float64))
+            assert relerr(X[b], want) < 1e-5, (gi, b)
+
+
+def test_batched_adjoint_256_half_size(dev):
+    """256 vectors, 48x48x24 cells of 1 km, 2,304 stations (144 station
+    chunks, 72 per K-slice): both splits against the FP64 single-vector
+    adjoint on 4 vectors, bitwise repeatability, and the dot-product identity
+    <G P, R> = <P, G^T R> between the batched forward and adjoint per model."""
+    import torch
+    w = W.ensemble_256(n=48, nz=24)
+    Pt = torch.from_numpy(w.rho).to(dev)
+    R = residuals(13, 256, w.n_stations)
+    Rt = torch.from_numpy(R).to(dev)
+    with gfm.Plan(w.grid, w.stations, kappa=w.kappa) as p:
+        X = p.adjoint_batched(Rt).cpu().numpy()
+        X2 = p.adjoint_batched(Rt).cpu().numpy()
+        Y = p.forward_batched(Pt).cpu().numpy()
+        p.set_batched_tuning(split=2)
+        Xh = p.adjoint_batched(Rt).cpu().numpy()
+        p.set_batched_tuning(split=3)
+        for b in (0, 85, 170, 255):
+            a64 = p.adjoint(torch.from_numpy(R[b].astype(np.float64)).to(dev)).cpu().numpy()
+            assert relerr(X[b], a64) < 1e-5, b
+            assert relerr(Xh[b], a64) < 1e-4, b
+    assert np.array_equal(X, X2)
+    lhs = np.einsum("bm,bm->b", Y, R.astype(np.float64))
+    rhs = np.einsum("bn,bn->b", w.rho.astype(np.float64), X.astype(np.float64))
+    scale = np.einsum("bm,bm->b", np.abs(Y), np.abs(R.astype(np.float64)))
+    assert np.max(np.abs(lhs - rhs) / scale) < 1e-5
+
+
+def test_batched_adjoint_rejects_and_empty(dev):
+    import torch
+    w = W.pm_cpu_small()
+    with gfm.Plan(w.grid, w.stations, kappa=w.kappa, formula=gfm.PRISM) as p:
+        with pytest.raises(gfm.ZtmError):
+            p.adjoint_batched(torch.zeros(2, w.n_stations, device=dev))
+    with gfm.Plan(w.grid, w.stations, kappa=w.kappa) as p:
+        with pytest.raises(gfm.ZtmError):
+            p.adjoint_batched(torch.zeros(257, w.n_stations, device=dev))
+        out = p.adjoint_batched(torch.zeros(0, w.n_stations, device=dev))
+        assert out.shape == (0, w.grid.n_cells)
+        X = p.adjoint_batched(torch.zeros(3, w.n_stations, device=dev)).cpu().numpy()
+        assert not np.any(X)

@@ -119,6 +119,17 @@ int ensure_workspace(ztm_plan p, size_t doubles) {
     return ZTM_OK;
 }
 
+// Grows the batched (ensemble) workspace to at least `bytes`.
+int ensure_batched_workspace(ztm_plan p, size_t bytes) {
+    if (bytes <= p->bws_cap) return ZTM_OK;
+    if (p->bws) (void)hipFree(p->bws);
+    p->bws = nullptr;
+    p->bws_cap = 0;
+    ZTM_HIP(hipMalloc(&p->bws, bytes));
+    p->bws_cap = bytes;
+    return ZTM_OK;
+}
+
 void plan_free(ztm_plan p) {
     if (!p) return;
     if (p->st_f) (void)hipFree(p->st_f);
@@ -569,16 +580,32 @@ int ztm_plan_forward_batched(ztm_plan plan, int n_models, const float* rho_dev, 
                        : batched_default_slices(plan->shape, n_models);
     const int split = plan->tuning.batched_split;
     const size_t need = batched_workspace(plan->shape, n_models, ks, split);
-    if (need > plan->bws_cap) {
-        if (plan->bws) (void)hipFree(plan->bws);
-        plan->bws = nullptr;
-        plan->bws_cap = 0;
-        ZTM_HIP(hipMalloc(&plan->bws, need));
-        plan->bws_cap = need;
-    }
+    if (int rc = ensure_batched_workspace(plan, need)) return rc;
     return batched_fwd_launch(plan->shape, n_models, ks, split, (const float*)plan->st_f, rho_dev,
                               plan->prob.threshold, plan->sy, plan->sz, pm_scale(plan), plan->bws,
                               g_dev, (hipStream_t)stream);
+}
+
+int ztm_plan_adjoint_batched(ztm_plan plan, int n_vectors, const float* r_dev, float* out_dev,
+                             void* stream) {
+    if (!plan) return fail(ZTM_EINVAL, "null plan");
+    if (n_vectors < 0 || n_vectors > 256)
+        return fail(ZTM_EINVAL, "n_vectors must be in [0, 256] (got %d)", n_vectors);
+    if (plan->prob.formula != ZTM_POINT_MASS)
+        return fail(ZTM_EINVAL, "the batched adjoint supports the point-mass formula only");
+    if (n_vectors == 0 || plan->shape.n_cells == 0) return ZTM_OK;
+    if (!out_dev || (plan->shape.m > 0 && !r_dev)) return fail(ZTM_EINVAL, "null r or out");
+    int dev = 0;
+    ZTM_HIP(hipGetDevice(&dev));
+    if (dev != plan->device)
+        return fail(ZTM_EINVAL, "plan belongs to device %d, current device is %d", plan->device, dev);
+    const int ks = batched_adj_default_slices(plan->shape, n_vectors);
+    const int split = plan->tuning.batched_split;
+    const size_t need = batched_adj_workspace(plan->shape, n_vectors, ks, split);
+    if (int rc = ensure_batched_workspace(plan, need)) return rc;
+    return batched_adj_launch(plan->shape, n_vectors, ks, split, (const float*)plan->st_f, r_dev,
+                              plan->sy, plan->sz, pm_scale(plan), plan->bws, out_dev,
+                              (hipStream_t)stream);
 }
 
 int ztm_plan_set_batched_tuning(ztm_plan plan, int slices, int split) {
@@ -821,13 +848,7 @@ int tfm_gravity_of_fields(ztm_plan p, int bfr, int di, int dj, int fh, int fv, i
     const int split = p->tuning.batched_split;
     const size_t ws = (batched_workspace(p->shape, B, ks, split) + 255) / 256 * 256;
     const size_t need = ws + sizeof(double) * (size_t)B * m;
-    if (need > p->bws_cap) {
-        if (p->bws) (void)hipFree(p->bws);
-        p->bws = nullptr;
-        p->bws_cap = 0;
-        ZTM_HIP(hipMalloc(&p->bws, need));
-        p->bws_cap = need;
-    }
+    if (int rc = ensure_batched_workspace(p, need)) return rc;
     double* Y = (double*)((char*)p->bws + ws);
     if (int e = batched_fwd_launch(p->shape, B, ks, split, (const float*)p->st_f, drho_dev, thr,
                                    p->sy, p->sz, pm_scale(p), p->bws, Y, s))
@@ -895,13 +916,7 @@ int ztm_tfm_gravity_jvp(int bfr, int di, int dj, int dk, int fh, int fv, const d
                                   p->out_map, m, p->shape.m_pad, s))
         return e;
     const size_t need = sizeof(double) * 3 * (size_t)m;
-    if (need > p->bws_cap) {
-        if (p->bws) (void)hipFree(p->bws);
-        p->bws = nullptr;
-        p->bws_cap = 0;
-        ZTM_HIP(hipMalloc(&p->bws, need));
-        p->bws_cap = need;
-    }
+    if (int rc = ensure_batched_workspace(p, need)) return rc;
     double* Y = (double*)p->bws;
     NearArgs na = near_args(p);
     na.thr = 0.f;
@@ -941,13 +956,7 @@ int ztm_tfm_gravity_vjp(int bfr, int di, int dj, int dk, int fh, int fv, const d
     // the FP32 pair kernel needs no near-field correction), then the three
     // FP64 dot products with the derivative fields
     const size_t need = tfm_vjp_workspace(N, m);
-    if (need > p->bws_cap) {
-        if (p->bws) (void)hipFree(p->bws);
-        p->bws = nullptr;
-        p->bws_cap = 0;
-        ZTM_HIP(hipMalloc(&p->bws, need));
-        p->bws_cap = need;
-    }
+    if (int rc = ensure_batched_workspace(p, need)) return rc;
     float* a = (float*)p->bws;
     float* r_int = a + N;
     double* part = (double*)((char*)p->bws + ((size_t)(N + m) * sizeof(float) + 255) / 256 * 256);

@@ -42,6 +42,8 @@ constexpr int kBatchChunk = 16;   // cells per MFMA K step
 // their maxima, i.e. wait only for this wave's vector memory operations.
 constexpr int kWaitVm0 = 0x0F70;
 constexpr int kBatchTile = 128;   // stations per workgroup
+// the adjoint's station chunks end inside the padded station records
+static_assert(kStationPad % 16 == 0, "station padding must hold whole chunks");
 
 // ---------------------------------------------------------------- packing
 // A block packs 32 models x 64 cells (4 chunks): it reads the tile with
@@ -306,13 +308,171 @@ __global__ __launch_bounds__(256) void batched_fwd_kernel(
         }
 }
 
+// --------------------------------------------------------------- adjoint
+// X = G^T R for R = [B residual vectors x M stations]: the same MFMA scheme
+// with the roles of stations and cells exchanged. A wave owns 32 cells (the
+// MFMA's M; the cell's grid coordinates stay in registers) and the K loop
+// walks the slice's stations 16 at a time. The B operand is the packed
+// residual chunk (the forward's packing over stations instead of cells), and
+// the lane's 8 G values of a chunk are those of its cell with the stations
+// 8 (lane / 32) + 0..7 of the chunk, evaluated with exactly the forward's
+// arithmetic (MODE 0), so both directions use bit-identical G entries.
+//
+// The stations of a chunk are staged in LDS next to the residual parts. G
+// of chunk c + 1 is generated while chunk c's MFMAs run, i.e. before the
+// barrier that publishes chunk c + 1's parts, so the stations run one chunk
+// further ahead: iteration i (chunk c0 + i) loads the stations of chunk
+// c0 + i + 2 into ring slot (i + 2) % 3 and reads those of chunk c0 + i + 1
+// from slot (i + 1) % 3. A slot is overwritten two iterations after its last
+// read, and every wave loads (and reads) a copy of its own, 4 copies of the
+// 16 records across the 64 lanes, so no load needs a lane mask.
+template <int NT, int SPLIT>   // model tiles of 32 (B <= 32 * NT); 2- or 3-way split
+__global__ __launch_bounds__(256) void batched_adj_kernel(
+    const float4* __restrict__ st, const bf16x8* __restrict__ parts, int64_t part_stride,
+    int nx, int ny, double inv_nx, double inv_ny, float sy, float sz, int n_chunks,
+    int chunks_per_slice, int n_pad, float* __restrict__ part) {
+    constexpr int Bp = 32 * NT;
+    constexpr int V = Bp * 2;                          // bf16x8 per part per chunk
+    constexpr int LOADS = (V * SPLIT + 255) / 256;     // per thread per chunk
+    __shared__ bf16x8 lp[2][SPLIT][V];
+    __shared__ float4 ls[3][4][64];                    // [ring slot][wave][lane]
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int row = lane & 31, kh = lane >> 5;
+    const int nbase = blockIdx.x * kBatchTile + wave * 32;
+    float cx, cy, cz;
+    {
+        int rw, i, j, k;
+        divmod_pos(nbase + row, nx, inv_nx, rw, i);
+        divmod_pos(rw, ny, inv_ny, k, j);
+        cx = (float)i;
+        cy = (float)j * sy;
+        cz = (float)k * sz;
+    }
+
+    f32x16 acc[NT];
+#pragma unroll
+    for (int t = 0; t < NT; ++t)
+#pragma unroll
+        for (int i = 0; i < 16; ++i) acc[t][i] = 0.f;
+
+    const int c0 = blockIdx.y * chunks_per_slice;
+    const int c1 = min(c0 + chunks_per_slice, n_chunks);
+    auto fetch = [&](int chunk, int buf) {
+#pragma unroll
+        for (int u = 0; u < LOADS; ++u) {
+            const int idx = threadIdx.x + 256 * u;
+            if ((V * SPLIT) % 256 == 0 || idx < V * SPLIT)
+                __builtin_amdgcn_global_load_lds(
+                    (const void*)(parts + (idx / V) * part_stride + (int64_t)chunk * V + (idx % V)),
+                    (__attribute__((address_space(3))) void*)(&lp[buf][0][0] + (idx - lane)),
+                    16, 0, 0);
+        }
+    };
+    // lane l of each wave: record 16 chunk + l % 16 into ls[slot][wave][l]
+    auto fetch_st = [&](int chunk, int slot) {
+        __builtin_amdgcn_global_load_lds(
+            (const void*)(st + (int64_t)chunk * kBatchChunk + (lane & 15)),
+            (__attribute__((address_space(3))) void*)(&ls[slot][wave][0]), 16, 0, 0);
+    };
+    auto gen = [&](int slot, bf16x8 (&a)[SPLIT]) {
+        const float4* s = &ls[slot][wave][8 * kh];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const float4 p = s[e];
+            const float dy = p.y - cy;
+            const float dz = p.z - cz;
+            const float x = p.x - cx;
+            const float v = rsq_f(fmaf(x, x, fmaf(dy, dy, fmaf(dz, dz, 1e-20f))));
+            float gv = dz * (v * v * v);
+#pragma unroll
+            for (int q = 0; q < SPLIT; ++q) {
+                const __bf16 h = (__bf16)gv;
+                a[q][e] = h;
+                gv -= (float)h;
+            }
+        }
+    };
+    bf16x8 a[SPLIT];
+    if (c0 < c1) {
+        fetch(c0, 0);
+        fetch_st(c0, 0);
+        fetch_st(min(c0 + 1, c1 - 1), 1);
+    }
+    __builtin_amdgcn_s_waitcnt(kWaitVm0);
+    __syncthreads();
+    if (c0 < c1) gen(0, a);
+    int rd = 1, wr = 2;   // ring slots read and written by the current iteration
+    for (int c = c0; c < c1; ++c) {
+        const int buf = (c - c0) & 1;
+        fetch(min(c + 1, c1 - 1), buf ^ 1);
+        fetch_st(min(c + 2, c1 - 1), wr);
+        bf16x8 cur[SPLIT];
+#pragma unroll
+        for (int q = 0; q < SPLIT; ++q) cur[q] = a[q];
+        gen(rd, a);
+        constexpr int TP = NT >= 2 ? 2 : 1;
+        bf16x8 b[2][TP][SPLIT];
+        auto load_b = [&](int t, bf16x8 (&bb)[TP][SPLIT]) {
+#pragma unroll
+            for (int u = 0; u < TP; ++u)
+#pragma unroll
+                for (int q = 0; q < SPLIT; ++q) bb[u][q] = lp[buf][q][kh * Bp + 32 * (t + u) + row];
+        };
+        load_b(0, b[0]);
+#pragma unroll
+        for (int t = 0; t < NT; t += TP) {
+            const int cb = (t / TP) & 1;
+            if (t + TP < NT) load_b(t + TP, b[cb ^ 1]);
+#pragma unroll
+            for (int s2 = 0; s2 < SPLIT; ++s2)
+#pragma unroll
+                for (int qa = 0; qa <= s2; ++qa)
+#pragma unroll
+                    for (int u = 0; u < TP; ++u)
+                        acc[t + u] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
+                            cur[qa], b[cb][u][s2 - qa], acc[t + u], 0, 0, 0);
+        }
+        // Same issue order as the forward: global-to-LDS loads, the first
+        // tile pair's LDS reads, then per tile pair the next pair's reads and
+        // its MFMAs, each followed by a few VALU instructions of the next
+        // chunk's G block.
+        constexpr int kPerPair = TP * SPLIT * (SPLIT + 1) / 2;
+        __builtin_amdgcn_sched_group_barrier(0x020, LOADS + 1, 0);   // VMEM read
+        __builtin_amdgcn_sched_group_barrier(0x100, TP * SPLIT, 0);
+#pragma unroll
+        for (int t = 0; t < NT; t += TP) {
+            if (t + TP < NT) __builtin_amdgcn_sched_group_barrier(0x100, TP * SPLIT, 0);
+#pragma unroll
+            for (int i = 0; i < kPerPair; ++i) {
+                __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+                __builtin_amdgcn_sched_group_barrier(0x002, SPLIT == 3 ? 4 : 6, 0);
+            }
+        }
+        rd = rd == 2 ? 0 : rd + 1;
+        wr = wr == 2 ? 0 : wr + 1;
+        __builtin_amdgcn_s_waitcnt(kWaitVm0);
+        __syncthreads();
+    }
+    // D layout: lane holds model 32t + lane%32, cells 8*(i/4) + 4*(lane/32) + i%4
+    float* out = part + ((size_t)blockIdx.y * n_pad + nbase) * Bp;
+#pragma unroll
+    for (int t = 0; t < NT; ++t)
+#pragma unroll
+        for (int e = 0; e < 16; ++e) {
+            const int crow = 8 * (e >> 2) + 4 * kh + (e & 3);
+            out[(size_t)crow * Bp + 32 * t + row] = acc[t][e];
+        }
+}
+
 // Y[b][m] = scale * sum_ks part[ks][m][b]. A block sums an 8 (stations) x 32
 // (models) tile, one entry per thread (reads coalesced along the models), and
-// writes it transposed through LDS.
+// writes it transposed through LDS. The adjoint uses it with cells in place
+// of stations and a float output.
+template <typename TO>
 __global__ __launch_bounds__(256) void batched_reduce_kernel(const float* __restrict__ part,
                                                              int ks, int m, int m_pad, int B,
                                                              int Bp, double scale,
-                                                             double* __restrict__ Y) {
+                                                             TO* __restrict__ Y) {
     __shared__ double tile[8][33];
     const int mb = blockIdx.x * 8, bb = blockIdx.y * 32;
     const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;   // 32 x 8
@@ -331,7 +491,7 @@ __global__ __launch_bounds__(256) void batched_reduce_kernel(const float* __rest
     // 32 models x 8 stations out: thread (b = t / 8, station t % 8)
     const int t = threadIdx.x;
     const int b = bb + (t >> 3), mm = mb + (t & 7);
-    if (b < B && mm < m) Y[(size_t)b * m + mm] = scale * tile[t & 7][t >> 3];
+    if (b < B && mm < m) Y[(size_t)b * m + mm] = (TO)(scale * tile[t & 7][t >> 3]);
 }
 
 // --------------------------------------------------------------- launchers
@@ -353,9 +513,10 @@ size_t batched_workspace(const PmShape& s, int B, int ks, int split) {
     return packed + partial;
 }
 
-int batched_default_slices(const PmShape& s, int B) {
-    const int tiles = (s.m + kBatchTile - 1) / kBatchTile;
-    const int64_t n_chunks = (s.n_cells + kBatchChunk - 1) / kBatchChunk;
+namespace {
+// K-slices for a grid of `tiles` row tiles (workgroups per slice) over
+// n_chunks K chunks.
+int default_slices(int64_t tiles, int64_t n_chunks, int B) {
     // Resident workgroups on 256 CUs for the kernel instance B selects
     // (registers: 2 per CU at 256 models, more for fewer model tiles).
     const int nt = batched_nt(std::max(B, 1));
@@ -369,6 +530,17 @@ int batched_default_slices(const PmShape& s, int B) {
     ks = std::max<int64_t>(ks, rounds * cap / tiles);
     ks = std::min<int64_t>(ks, std::max<int64_t>(1, n_chunks));
     return (int)std::min<int64_t>(ks, 65535);
+}
+}  // namespace
+
+int batched_default_slices(const PmShape& s, int B) {
+    return default_slices((s.m + kBatchTile - 1) / kBatchTile,
+                          (s.n_cells + kBatchChunk - 1) / kBatchChunk, B);
+}
+
+int batched_adj_default_slices(const PmShape& s, int B) {
+    return default_slices((s.n_cells + kBatchTile - 1) / kBatchTile,
+                          ((int64_t)s.m + kBatchChunk - 1) / kBatchChunk, B);
 }
 
 int batched_fwd_launch(const PmShape& s, int B, int ks, int split, const float* st,
@@ -430,8 +602,94 @@ int batched_fwd_launch(const PmShape& s, int B, int ks, int split, const float* 
 #undef ZTM_BATCHED_LAUNCH
         ZTM_HIP(hipGetLastError());
     }
-    hipLaunchKernelGGL(batched_reduce_kernel, dim3((s.m + 7) / 8, (B + 31) / 32), dim3(256), 0,
+    hipLaunchKernelGGL(batched_reduce_kernel<double>, dim3((s.m + 7) / 8, (B + 31) / 32), dim3(256), 0,
                        stream, part, ks, s.m, s.m_pad, B, Bp, scale, Y);
+    ZTM_HIP(hipGetLastError());
+    return ZTM_OK;
+}
+
+
+namespace {
+__global__ void batched_zero_kernel(float* __restrict__ x, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] = 0.f;
+}
+}  // namespace
+
+size_t batched_adj_workspace(const PmShape& s, int B, int ks, int split) {
+    const int Bp = 32 * batched_nt(B);
+    const int64_t m16 = ((int64_t)s.m + kBatchChunk - 1) / kBatchChunk * kBatchChunk;
+    const int64_t n_pad = (s.n_cells + kBatchTile - 1) / kBatchTile * kBatchTile;
+    const size_t packed = (size_t)m16 * Bp * split * sizeof(__bf16);
+    const size_t partial = (size_t)ks * n_pad * Bp * sizeof(float);
+    return packed + partial;
+}
+
+int batched_adj_launch(const PmShape& s, int B, int ks, int split, const float* st,
+                       const float* R, double sy, double sz, double scale, void* ws, float* X,
+                       hipStream_t stream) {
+    if (B <= 0 || s.n_cells == 0) return ZTM_OK;
+    if (split != 2 && split != 3) return fail(ZTM_EINVAL, "split must be 2 or 3 (got %d)", split);
+    const int nt = batched_nt(B);
+    if (nt > 8) return fail(ZTM_EINVAL, "batched adjoint supports at most 256 vectors per call (got %d)", B);
+    if (s.n_cells > (int64_t)INT32_MAX - 2 * kBatchTile)
+        return fail(ZTM_EINVAL, "batched adjoint supports up to 2^31 - 256 cells");
+    if (s.m == 0) {   // no stations: G^T R = 0
+        const int64_t n = (int64_t)B * s.n_cells;
+        hipLaunchKernelGGL(batched_zero_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                           stream, X, n);
+        ZTM_HIP(hipGetLastError());
+        return ZTM_OK;
+    }
+    const int Bp = 32 * nt;
+    // station chunks read records up to 16 n_chunks - 1 < m_pad (a multiple of 16)
+    const int64_t n_chunks = ((int64_t)s.m + kBatchChunk - 1) / kBatchChunk;
+    ks = std::max(1, std::min<int>(ks, (int)std::min<int64_t>(n_chunks, 65535)));
+    const int per = (int)((n_chunks + ks - 1) / ks);
+    ks = (int)((n_chunks + per - 1) / per);
+    const int64_t n_packed = n_chunks * Bp * kBatchChunk;
+    const int64_t tiles = (s.n_cells + kBatchTile - 1) / kBatchTile;
+    const int n_pad = (int)(tiles * kBatchTile);
+    __bf16* parts = (__bf16*)ws;
+    float* part = (float*)(parts + split * n_packed);
+    // the residuals are packed like the forward's models, stations in place of cells
+    const dim3 pgrid((unsigned)((s.m + kPackCells - 1) / kPackCells), (unsigned)(Bp / 32));
+    if (split == 3)
+        hipLaunchKernelGGL(batched_pack_kernel<3>, pgrid, dim3(256), 0, stream, R, B,
+                           (int64_t)s.m, Bp, n_chunks, n_packed, 0.f, parts);
+    else
+        hipLaunchKernelGGL(batched_pack_kernel<2>, pgrid, dim3(256), 0, stream, R, B,
+                           (int64_t)s.m, Bp, n_chunks, n_packed, 0.f, parts);
+    ZTM_HIP(hipGetLastError());
+    const dim3 grid((unsigned)tiles, (unsigned)ks);
+    const int64_t stride = n_packed / 8;   // in bf16x8 units
+    {
+        ProfScope prof("batched_adj", stream);
+#define ZTM_BATCHED_ADJ(NT, SP)                                                                  \
+    hipLaunchKernelGGL((batched_adj_kernel<NT, SP>), grid, dim3(256), 0, stream,                 \
+                       (const float4*)st, (const bf16x8*)parts, stride, s.nx, s.ny, 1.0 / s.nx,  \
+                       1.0 / s.ny, (float)sy, (float)sz, (int)n_chunks, per, n_pad, part)
+        if (split == 3) {
+            switch (nt) {
+                case 1: ZTM_BATCHED_ADJ(1, 3); break;
+                case 2: ZTM_BATCHED_ADJ(2, 3); break;
+                case 4: ZTM_BATCHED_ADJ(4, 3); break;
+                default: ZTM_BATCHED_ADJ(8, 3); break;
+            }
+        } else {
+            switch (nt) {
+                case 1: ZTM_BATCHED_ADJ(1, 2); break;
+                case 2: ZTM_BATCHED_ADJ(2, 2); break;
+                case 4: ZTM_BATCHED_ADJ(4, 2); break;
+                default: ZTM_BATCHED_ADJ(8, 2); break;
+            }
+        }
+#undef ZTM_BATCHED_ADJ
+        ZTM_HIP(hipGetLastError());
+    }
+    hipLaunchKernelGGL(batched_reduce_kernel<float>,
+                       dim3((unsigned)((s.n_cells + 7) / 8), (B + 31) / 32), dim3(256), 0, stream,
+                       part, ks, (int)s.n_cells, n_pad, B, Bp, scale, X);
     ZTM_HIP(hipGetLastError());
     return ZTM_OK;
 }

@@ -283,6 +283,13 @@ int batched_default_slices(const PmShape& s, int B);
 int batched_fwd_launch(const PmShape& s, int B, int ks, int split, const float* st,
                        const float* P, double thr, double sy, double sz, double scale, void* ws,
                        double* Y, hipStream_t stream);
+// Batched adjoint on the same scheme: R float [B][M] -> X float [B][N];
+// ws must hold batched_adj_workspace(s, B, ks, split) bytes.
+size_t batched_adj_workspace(const PmShape& s, int B, int ks, int split);
+int batched_adj_default_slices(const PmShape& s, int B);
+int batched_adj_launch(const PmShape& s, int B, int ks, int split, const float* st,
+                       const float* R, double sy, double sz, double scale, void* ws, float* X,
+                       hipStream_t stream);
 
 // out = L^T L rho for forward-difference gradients on an nx x ny x nz grid
 // (gfm_reg.hip).

@@ -5,10 +5,13 @@
 //   nonlinear or regularised inversion gets exact misfit gradients from
 //   torch.autograd. ZTM itself differentiates by finite differences, one full
 //   forward model per parameter (Jac, TFM.cpp:124-135).
+// GravityBatchedFn: the same for an ensemble, Y = G P for P float32
+//   [B, N] on split-BF16 MFMA (ztm_plan_forward_batched), backward
+//   G^T dY through the batched adjoint (ztm_plan_adjoint_batched).
 // SmoothnessFn: s = 0.5 ||grad rho||^2 (forward differences), backward
 //   (L^T L rho) * ds, through ztm_plan_smoothness.
 //
-// Both run on the current HIP stream of rho's device, so they compose with
+// All run on the current HIP stream of rho's device, so they compose with
 // torch work and with stream capture the same way torch's own ops do.
 #include <torch/extension.h>
 #include <c10/hip/HIPStream.h>
@@ -67,6 +70,34 @@ class GravityFn : public torch::autograd::Function<GravityFn> {
     }
 };
 
+class GravityBatchedFn : public torch::autograd::Function<GravityBatchedFn> {
+  public:
+    static torch::Tensor forward(AutogradContext* ctx, torch::Tensor rho, int64_t plan) {
+        ztm_plan p = as_plan(plan);
+        const int64_t n = ztm_plan_num_cells(p), m = ztm_plan_num_stations(p);
+        TORCH_CHECK(rho.is_cuda() && rho.is_contiguous() && rho.dim() == 2 && rho.size(1) == n &&
+                        rho.scalar_type() == torch::kFloat32,
+                    "rho must be a contiguous float32 GPU tensor [B, ", n, "]");
+        const int64_t B = rho.size(0);
+        TORCH_CHECK(B <= 256, "at most 256 models per call, got ", B);
+        auto g = torch::empty({B, m}, rho.options().dtype(torch::kFloat64));
+        check(ztm_plan_forward_batched(p, (int)B, rho.data_ptr<float>(), g.data_ptr<double>(),
+                                       stream_of(rho)));
+        ctx->saved_data["plan"] = plan;
+        return g;
+    }
+
+    static variable_list backward(AutogradContext* ctx, variable_list grads) {
+        ztm_plan p = as_plan(ctx->saved_data["plan"].toInt());
+        const int64_t n = ztm_plan_num_cells(p);
+        auto r = grads[0].to(torch::kFloat32).contiguous();
+        auto out = torch::empty({r.size(0), n}, r.options());
+        check(ztm_plan_adjoint_batched(p, (int)r.size(0), r.data_ptr<float>(),
+                                       out.data_ptr<float>(), stream_of(r)));
+        return {out, torch::Tensor()};
+    }
+};
+
 class SmoothnessFn : public torch::autograd::Function<SmoothnessFn> {
   public:
     static torch::Tensor forward(AutogradContext* ctx, torch::Tensor rho, int64_t plan) {
@@ -87,6 +118,10 @@ class SmoothnessFn : public torch::autograd::Function<SmoothnessFn> {
 
 torch::Tensor gravity(torch::Tensor rho, int64_t plan) { return GravityFn::apply(rho, plan); }
 
+torch::Tensor gravity_batched(torch::Tensor rho, int64_t plan) {
+    return GravityBatchedFn::apply(rho, plan);
+}
+
 torch::Tensor smoothness(torch::Tensor rho, int64_t plan) {
     return SmoothnessFn::apply(rho, plan);
 }
@@ -105,6 +140,10 @@ torch::Tensor adjoint(torch::Tensor r, int64_t plan) {
 PYBIND11_MODULE(_ztm_torch, m) {
     m.doc() = "libtorch autograd bindings of libztm_gfm (gravity forward/adjoint)";
     m.def("gravity", &gravity, "g = G rho (FP64 out); differentiable, backward = G^T grad",
+          pybind11::arg("rho"), pybind11::arg("plan"));
+    m.def("gravity_batched", &gravity_batched,
+          "Y = G P for P float32 [B, N] (FP64 out) on split-BF16 MFMA; differentiable, "
+          "backward = batched G^T grad",
           pybind11::arg("rho"), pybind11::arg("plan"));
     m.def("smoothness", &smoothness, "0.5 ||grad rho||^2; differentiable",
           pybind11::arg("rho"), pybind11::arg("plan"));

@@ -68,6 +68,7 @@ SIGNATURES = {
     "ztm_plan_adjoint": (_I, [_P, _I, _P, _P, _P]),
     "ztm_plan_smoothness": (_I, [_P, _I, _P, _P, _P]),
     "ztm_plan_forward_batched": (_I, [_P, _I, _P, _P, _P]),
+    "ztm_plan_adjoint_batched": (_I, [_P, _I, _P, _P, _P]),
     "ztm_plan_set_batched_tuning": (_I, [_P, _I, _I]),
     "ztm_plan_store_bytes": (_I64, [_P]),
     "ztm_plan_store": (_I, [_P, _P]),
@@ -395,8 +396,22 @@ class Plan:
         check(lib().ztm_plan_forward_batched(self._h, B, _ptr(rho), _ptr(out), _stream(stream)))
         return out
 
+    def adjoint_batched(self, r, out=None, stream=None):
+        """r float32 [B, M] (B <= 256) -> G^T r float32 [B, N] on split-BF16 MFMA."""
+        import torch
+        if not (isinstance(r, torch.Tensor) and r.is_cuda and r.dtype == torch.float32
+                and r.is_contiguous() and r.dim() == 2 and r.shape[1] == self.n_stations):
+            raise ValueError(f"r must be a contiguous float32 CUDA tensor [B, {self.n_stations}]")
+        B = r.shape[0]
+        if out is None:
+            out = torch.empty(B, self.n_cells, dtype=torch.float32, device=r.device)
+        _check_t(out, torch.float32, B * self.n_cells, "out")
+        check(lib().ztm_plan_adjoint_batched(self._h, B, _ptr(r), _ptr(out), _stream(stream)))
+        return out
+
     def set_batched_tuning(self, slices: int = 0, split: int = 0) -> None:
-        """K-slices (0 = choose) and BF16 split (2 = hi/lo, 3 = three-way; 0 = keep)."""
+        """Forward K-slices (0 = choose) and BF16 split (2 = hi/lo, 3 = three-way;
+        0 = keep); the split also applies to adjoint_batched."""
         check(lib().ztm_plan_set_batched_tuning(self._h, slices, split))
 
     # stored-matrix variant

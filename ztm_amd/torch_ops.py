@@ -2,6 +2,7 @@
 extension ztm_amd._ztm_torch (ztm_amd/csrc/gfm_torch.cpp).
 
     g    = gravity(rho, plan)          # G rho, backward G^T grad (adjoint kernel)
+    Y    = gravity_batched(P, plan)    # G P for P [B, N], backward batched G^T
     s    = smoothness(rho, plan)       # 0.5 ||grad rho||^2, backward L^T L rho
     loss = misfit(rho, plan, d, lam)   # 0.5 ||G rho - d||^2 + lam * s
 
@@ -30,6 +31,10 @@ def ext():
 
 def gravity(rho, plan: Plan):
     return ext().gravity(rho, plan.handle)
+
+
+def gravity_batched(rho, plan: Plan):
+    return ext().gravity_batched(rho, plan.handle)
 
 
 def smoothness(rho, plan: Plan):
