@@ -146,7 +146,8 @@ def test_batched_adjoint_ragged(dev, orc):
 
 def test_batched_adjoint_256_half_size(dev):
     """256 vectors, 48x48x24 cells of 1 km, 2,304 stations (144 station
-    chunks, 72 per K-slice): both splits against the FP64 single-vector
+    chunks in one K-slice, stored straight from the accumulators with 16-byte
+    stores): both splits against the FP64 single-vector
     adjoint on 4 vectors, bitwise repeatability, and the dot-product identity
     <G P, R> = <P, G^T R> between the batched forward and adjoint per model."""
     import torch
@@ -170,6 +171,24 @@ def test_batched_adjoint_256_half_size(dev):
     rhs = np.einsum("bn,bn->b", w.rho.astype(np.float64), X.astype(np.float64))
     scale = np.einsum("bm,bm->b", np.abs(Y), np.abs(R.astype(np.float64)))
     assert np.max(np.abs(lhs - rhs) / scale) < 1e-5
+
+
+def test_batched_adjoint_one_slice_ragged(dev, orc):
+    """One K-slice stored from the accumulators with N % 4 != 0 (per-element
+    stores) and a partial last cell tile: 101x53x13 cells, 300 stations (19
+    chunks), 250 vectors (8 tiles of 32, the last partial); 4 vectors against
+    the oracle."""
+    import torch
+    g = W.Grid(101, 53, 13, -500.0, 200.0, -40.0, 40.0, 45.0, 30.0)
+    c = g.centres()
+    st = np.stack([W.uniform(31, a, 300, c[:, a].min(), c[:, a].max()) for a in range(2)]
+                  + [W.uniform(31, 2, 300, 350.0, 450.0)], axis=1)
+    R = residuals(32, 250, 300)
+    with gfm.Plan(g, st, kappa=W.KAPPA_SI) as p:
+        X = p.adjoint_batched(torch.from_numpy(R).to(dev)).cpu().numpy()
+    for b in (0, 100, 224, 249):
+        want = orc.pm_adjoint(g, W.KAPPA_SI, st, R[b].astype(np.float64))
+        assert relerr(X[b], want) < 1e-5, b
 
 
 def test_batched_adjoint_rejects_and_empty(dev):

@@ -50,15 +50,15 @@ static_assert(kStationPad % 16 == 0, "station padding must hold whole chunks");
 // 256-byte row reads into LDS, then each thread converts the 8 cells of one
 // (chunk, half, model) slot and writes them as one 16-byte store per part,
 // contiguous across the models of a slot. grid = (ceil(n_chunks / 4), Bp / 32).
-// Element v is split as h_1 = bf16(v), h_2 = bf16(v - h_1), ...; entries
-// outside the model or cell range, and |v| <= thr, are 0.
+// Element v (times mul) is split as h_1 = bf16(v), h_2 = bf16(v - h_1), ...;
+// entries outside the model or cell range, and |v| <= thr, are 0.
 constexpr int kPackCells = 64;
 
 template <int SPLIT>
 __global__ __launch_bounds__(256) void batched_pack_kernel(const float* __restrict__ P, int B,
                                                            int64_t n_cells, int Bp,
                                                            int64_t n_chunks, int64_t n_packed,
-                                                           float thr,
+                                                           float thr, float mul,
                                                            __bf16* __restrict__ parts) {
     __shared__ float tile[32][kPackCells + 1];
     const int t = threadIdx.x;
@@ -73,7 +73,7 @@ __global__ __launch_bounds__(256) void batched_pack_kernel(const float* __restri
         float v = 0.f;
         if (model < B && cell < n_cells) {
             v = P[(size_t)model * n_cells + cell];
-            if (!(fabsf(v) > thr)) v = 0.f;
+            v = fabsf(v) > thr ? v * mul : 0.f;
         }
         tile[row][col] = v;
     }
@@ -326,11 +326,19 @@ __global__ __launch_bounds__(256) void batched_fwd_kernel(
 // from slot (i + 1) % 3. A slot is overwritten two iterations after its last
 // read, and every wave loads (and reads) a copy of its own, 4 copies of the
 // 16 records across the 64 lanes, so no load needs a lane mask.
+//
+// With one K-slice (X != nullptr) the kernel stores X[b][n] itself: lane
+// (row, kh) holds vector 32 t + row at cells 8 g + 4 kh + 0..3 of its wave's
+// 32, i.e. 4 consecutive cells per (t, g) -- one 16-byte store when vec4
+// (N % 4 == 0 and X 16-byte aligned). The scale is then folded into the
+// packed residuals, so the accumulators are stored as they are (no register
+// work that would cost occupancy).
 template <int NT, int SPLIT>   // model tiles of 32 (B <= 32 * NT); 2- or 3-way split
 __global__ __launch_bounds__(256) void batched_adj_kernel(
     const float4* __restrict__ st, const bf16x8* __restrict__ parts, int64_t part_stride,
     int nx, int ny, double inv_nx, double inv_ny, float sy, float sz, int n_chunks,
-    int chunks_per_slice, int n_pad, float* __restrict__ part) {
+    int chunks_per_slice, int n_pad, float* __restrict__ part, float* __restrict__ X,
+    int n_cells, int B, int vec4) {
     constexpr int Bp = 32 * NT;
     constexpr int V = Bp * 2;                          // bf16x8 per part per chunk
     constexpr int LOADS = (V * SPLIT + 255) / 256;     // per thread per chunk
@@ -454,6 +462,28 @@ __global__ __launch_bounds__(256) void batched_adj_kernel(
         __syncthreads();
     }
     // D layout: lane holds model 32t + lane%32, cells 8*(i/4) + 4*(lane/32) + i%4
+    if (X) {
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+            const int b = 32 * t + row;
+            if (b >= B) continue;
+            float* xb = X + (size_t)b * n_cells;
+#pragma unroll
+            for (int g = 0; g < 4; ++g) {
+                const int n = nbase + 8 * g + 4 * kh;
+                if (vec4 && n + 3 < n_cells) {
+                    *reinterpret_cast<float4*>(xb + n) =
+                        make_float4(acc[t][4 * g], acc[t][4 * g + 1], acc[t][4 * g + 2],
+                                    acc[t][4 * g + 3]);
+                } else {
+#pragma unroll
+                    for (int i = 0; i < 4; ++i)
+                        if (n + i < n_cells) xb[n + i] = acc[t][4 * g + i];
+                }
+            }
+        }
+        return;
+    }
     float* out = part + ((size_t)blockIdx.y * n_pad + nbase) * Bp;
 #pragma unroll
     for (int t = 0; t < NT; ++t)
@@ -516,11 +546,15 @@ size_t batched_workspace(const PmShape& s, int B, int ks, int split) {
 namespace {
 // K-slices for a grid of `tiles` row tiles (workgroups per slice) over
 // n_chunks K chunks.
-int default_slices(int64_t tiles, int64_t n_chunks, int B) {
-    // Resident workgroups on 256 CUs for the kernel instance B selects
-    // (registers: 2 per CU at 256 models, more for fewer model tiles).
+// Resident workgroups on 256 CUs for the kernel instance B selects
+// (registers: 2 per CU at 256 models, more for fewer model tiles).
+int64_t resident_groups(int B) {
     const int nt = batched_nt(std::max(B, 1));
-    const int64_t cap = 256 * (nt >= 8 ? 2 : nt == 4 ? 3 : nt == 2 ? 5 : 8);
+    return 256 * (nt >= 8 ? 2 : nt == 4 ? 3 : nt == 2 ? 5 : 8);
+}
+
+int default_slices(int64_t tiles, int64_t n_chunks, int B) {
+    const int64_t cap = resident_groups(B);
     // At least one full round of workgroups and at most ~384 chunks (MFMA
     // accumulation steps) per slice, to bound the FP32 accumulation error;
     // then as many slices as fit the same number of rounds, so the last round
@@ -539,8 +573,20 @@ int batched_default_slices(const PmShape& s, int B) {
 }
 
 int batched_adj_default_slices(const PmShape& s, int B) {
-    return default_slices((s.n_cells + kBatchTile - 1) / kBatchTile,
-                          ((int64_t)s.m + kBatchChunk - 1) / kBatchChunk, B);
+    const int64_t tiles = (s.n_cells + kBatchTile - 1) / kBatchTile;
+    const int64_t n_chunks = ((int64_t)s.m + kBatchChunk - 1) / kBatchChunk;
+    const int ks = default_slices(tiles, n_chunks, B);
+    // One slice stores G^T R from the accumulators (no partials, no reduce
+    // pass). Take it when it costs no more rounds x chunks per slice than the
+    // split and its FP32 accumulation run stays within 768 chunks (12,288
+    // stations).
+    if (ks > 1 && n_chunks <= 768) {
+        const int64_t cap = resident_groups(B);
+        const int64_t one = (tiles + cap - 1) / cap * n_chunks;
+        const int64_t split = (tiles * ks + cap - 1) / cap * ((n_chunks + ks - 1) / ks);
+        if (one <= split) return 1;
+    }
+    return ks;
 }
 
 int batched_fwd_launch(const PmShape& s, int B, int ks, int split, const float* st,
@@ -563,10 +609,10 @@ int batched_fwd_launch(const PmShape& s, int B, int ks, int split, const float* 
     const dim3 pgrid((unsigned)((s.n_cells + kPackCells - 1) / kPackCells), (unsigned)(Bp / 32));
     if (split == 3)
         hipLaunchKernelGGL(batched_pack_kernel<3>, pgrid, dim3(256), 0, stream, P, B, s.n_cells,
-                           Bp, n_chunks, n_packed, (float)thr, parts);
+                           Bp, n_chunks, n_packed, (float)thr, 1.f, parts);
     else
         hipLaunchKernelGGL(batched_pack_kernel<2>, pgrid, dim3(256), 0, stream, P, B, s.n_cells,
-                           Bp, n_chunks, n_packed, (float)thr, parts);
+                           Bp, n_chunks, n_packed, (float)thr, 1.f, parts);
     ZTM_HIP(hipGetLastError());
     const dim3 grid((unsigned)((s.m + kBatchTile - 1) / kBatchTile), (unsigned)ks);
     const int64_t stride = n_packed / 8;   // in bf16x8 units
@@ -621,7 +667,7 @@ size_t batched_adj_workspace(const PmShape& s, int B, int ks, int split) {
     const int64_t m16 = ((int64_t)s.m + kBatchChunk - 1) / kBatchChunk * kBatchChunk;
     const int64_t n_pad = (s.n_cells + kBatchTile - 1) / kBatchTile * kBatchTile;
     const size_t packed = (size_t)m16 * Bp * split * sizeof(__bf16);
-    const size_t partial = (size_t)ks * n_pad * Bp * sizeof(float);
+    const size_t partial = ks > 1 ? (size_t)ks * n_pad * Bp * sizeof(float) : 0;   // 1: direct
     return packed + partial;
 }
 
@@ -652,23 +698,28 @@ int batched_adj_launch(const PmShape& s, int B, int ks, int split, const float* 
     const int n_pad = (int)(tiles * kBatchTile);
     __bf16* parts = (__bf16*)ws;
     float* part = (float*)(parts + split * n_packed);
-    // the residuals are packed like the forward's models, stations in place of cells
+    float* direct = ks == 1 ? X : nullptr;
+    // The residuals are packed like the forward's models, stations in place
+    // of cells; with one slice they carry the scale (kernel stores as is).
+    const float mul = direct ? (float)scale : 1.f;
     const dim3 pgrid((unsigned)((s.m + kPackCells - 1) / kPackCells), (unsigned)(Bp / 32));
     if (split == 3)
         hipLaunchKernelGGL(batched_pack_kernel<3>, pgrid, dim3(256), 0, stream, R, B,
-                           (int64_t)s.m, Bp, n_chunks, n_packed, 0.f, parts);
+                           (int64_t)s.m, Bp, n_chunks, n_packed, 0.f, mul, parts);
     else
         hipLaunchKernelGGL(batched_pack_kernel<2>, pgrid, dim3(256), 0, stream, R, B,
-                           (int64_t)s.m, Bp, n_chunks, n_packed, 0.f, parts);
+                           (int64_t)s.m, Bp, n_chunks, n_packed, 0.f, mul, parts);
     ZTM_HIP(hipGetLastError());
     const dim3 grid((unsigned)tiles, (unsigned)ks);
     const int64_t stride = n_packed / 8;   // in bf16x8 units
+    const int vec4 = s.n_cells % 4 == 0 && (uintptr_t)X % 16 == 0;
     {
         ProfScope prof("batched_adj", stream);
 #define ZTM_BATCHED_ADJ(NT, SP)                                                                  \
     hipLaunchKernelGGL((batched_adj_kernel<NT, SP>), grid, dim3(256), 0, stream,                 \
                        (const float4*)st, (const bf16x8*)parts, stride, s.nx, s.ny, 1.0 / s.nx,  \
-                       1.0 / s.ny, (float)sy, (float)sz, (int)n_chunks, per, n_pad, part)
+                       1.0 / s.ny, (float)sy, (float)sz, (int)n_chunks, per, n_pad, part, direct, \
+                       (int)s.n_cells, B, vec4)
         if (split == 3) {
             switch (nt) {
                 case 1: ZTM_BATCHED_ADJ(1, 3); break;
@@ -687,6 +738,7 @@ int batched_adj_launch(const PmShape& s, int B, int ks, int split, const float* 
 #undef ZTM_BATCHED_ADJ
         ZTM_HIP(hipGetLastError());
     }
+    if (direct) return ZTM_OK;
     hipLaunchKernelGGL(batched_reduce_kernel<float>,
                        dim3((unsigned)((s.n_cells + 7) / 8), (B + 31) / 32), dim3(256), 0, stream,
                        part, ks, (int)s.n_cells, n_pad, B, Bp, scale, X);
