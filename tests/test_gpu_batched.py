@@ -144,14 +144,16 @@ def test_batched_adjoint_ragged(dev, orc):
             assert relerr(X[b], want) < 1e-5, (gi, b)
 
 
-def test_batched_adjoint_256_half_size(dev):
-    """256 vectors, 48x48x24 cells of 1 km, 2,304 stations (144 station
+@pytest.mark.parametrize("n,nz", [(48, 24), (64, 12)])
+def test_batched_adjoint_256_half_size(dev, n, nz):
+    """256 vectors, n x n x nz cells of 1 km, n^2 stations (144 / 256 station
     chunks in one K-slice, stored straight from the accumulators with 16-byte
-    stores): both splits against the FP64 single-vector
+    stores; nx = 64 takes the shared-row generation): both splits against the
+    FP64 single-vector
     adjoint on 4 vectors, bitwise repeatability, and the dot-product identity
     <G P, R> = <P, G^T R> between the batched forward and adjoint per model."""
     import torch
-    w = W.ensemble_256(n=48, nz=24)
+    w = W.ensemble_256(n=n, nz=nz)
     Pt = torch.from_numpy(w.rho).to(dev)
     R = residuals(13, 256, w.n_stations)
     Rt = torch.from_numpy(R).to(dev)

@@ -320,12 +320,20 @@ __global__ __launch_bounds__(256) void batched_fwd_kernel(
 //
 // The stations of a chunk are staged in LDS next to the residual parts. G
 // of chunk c + 1 is generated while chunk c's MFMAs run, i.e. before the
-// barrier that publishes chunk c + 1's parts, so the stations run one chunk
-// further ahead: iteration i (chunk c0 + i) loads the stations of chunk
-// c0 + i + 2 into ring slot (i + 2) % 3 and reads those of chunk c0 + i + 1
-// from slot (i + 1) % 3. A slot is overwritten two iterations after its last
-// read, and every wave loads (and reads) a copy of its own, 4 copies of the
-// 16 records across the 64 lanes, so no load needs a lane mask.
+// barrier that publishes chunk c + 1's parts, so the stations run further
+// ahead in a ring of 3 slots, chunk c0 + r in slot r % 3. Every wave loads
+// (and reads) a copy of its own, 4 copies of the 16 records across the 64
+// lanes, so no load needs a lane mask.
+//   ROW 0: iteration i loads chunk c0 + i + 2 and generates from the raw
+//          records of chunk c0 + i + 1.
+//   ROW 1 (nx % 32 == 0: a wave's 32 cells lie in one grid row, so they
+//          share y and z): iteration i loads chunk c0 + i + 3, turns the
+//          records of chunk c0 + i + 2 into (x, dy^2 + dz^2 + 1e-20, dz) for
+//          the wave's row in place (each lane its own record), and generates
+//          from chunk c0 + i + 1's, transformed one barrier earlier. A lane's
+//          G entry then costs 1 sub + 1 FMA + rsq + 3 mul instead of
+//          3 sub + 3 FMA + rsq + 3 mul, with the same roundings.
+// A slot is reloaded only after the barrier that follows its last read.
 //
 // With one K-slice (X != nullptr) the kernel stores X[b][n] itself: lane
 // (row, kh) holds vector 32 t + row at cells 8 g + 4 kh + 0..3 of its wave's
@@ -333,7 +341,7 @@ __global__ __launch_bounds__(256) void batched_fwd_kernel(
 // (N % 4 == 0 and X 16-byte aligned). The scale is then folded into the
 // packed residuals, so the accumulators are stored as they are (no register
 // work that would cost occupancy).
-template <int NT, int SPLIT>   // model tiles of 32 (B <= 32 * NT); 2- or 3-way split
+template <int NT, int SPLIT, int ROW>   // model tiles of 32 (B <= 32 * NT); 2- or 3-way split
 __global__ __launch_bounds__(256) void batched_adj_kernel(
     const float4* __restrict__ st, const bf16x8* __restrict__ parts, int64_t part_stride,
     int nx, int ny, double inv_nx, double inv_ny, float sy, float sz, int n_chunks,
@@ -342,6 +350,7 @@ __global__ __launch_bounds__(256) void batched_adj_kernel(
     constexpr int Bp = 32 * NT;
     constexpr int V = Bp * 2;                          // bf16x8 per part per chunk
     constexpr int LOADS = (V * SPLIT + 255) / 256;     // per thread per chunk
+    constexpr int AHEAD = ROW ? 3 : 2;                 // station chunks loaded ahead
     __shared__ bf16x8 lp[2][SPLIT][V];
     __shared__ float4 ls[3][4][64];                    // [ring slot][wave][lane]
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -382,15 +391,29 @@ __global__ __launch_bounds__(256) void batched_adj_kernel(
             (const void*)(st + (int64_t)chunk * kBatchChunk + (lane & 15)),
             (__attribute__((address_space(3))) void*)(&ls[slot][wave][0]), 16, 0, 0);
     };
+    // ROW 1: record (x, y, z, w) -> (x, dy^2 + dz^2 + 1e-20, dz, w), this lane's own
+    auto transform = [&](int slot) {
+        const float4 p = ls[slot][wave][lane];
+        const float dy = p.y - cy;
+        const float dz = p.z - cz;
+        ls[slot][wave][lane] = make_float4(p.x, fmaf(dy, dy, fmaf(dz, dz, 1e-20f)), dz, p.w);
+    };
     auto gen = [&](int slot, bf16x8 (&a)[SPLIT]) {
         const float4* s = &ls[slot][wave][8 * kh];
 #pragma unroll
         for (int e = 0; e < 8; ++e) {
             const float4 p = s[e];
-            const float dy = p.y - cy;
-            const float dz = p.z - cz;
+            float r2, dz;
             const float x = p.x - cx;
-            const float v = rsq_f(fmaf(x, x, fmaf(dy, dy, fmaf(dz, dz, 1e-20f))));
+            if constexpr (ROW) {
+                r2 = fmaf(x, x, p.y);
+                dz = p.z;
+            } else {
+                const float dy = p.y - cy;
+                dz = p.z - cz;
+                r2 = fmaf(x, x, fmaf(dy, dy, fmaf(dz, dz, 1e-20f)));
+            }
+            const float v = rsq_f(r2);
             float gv = dz * (v * v * v);
 #pragma unroll
             for (int q = 0; q < SPLIT; ++q) {
@@ -403,17 +426,26 @@ __global__ __launch_bounds__(256) void batched_adj_kernel(
     bf16x8 a[SPLIT];
     if (c0 < c1) {
         fetch(c0, 0);
-        fetch_st(c0, 0);
-        fetch_st(min(c0 + 1, c1 - 1), 1);
+#pragma unroll
+        for (int r = 0; r < AHEAD; ++r) fetch_st(min(c0 + r, c1 - 1), r);
     }
     __builtin_amdgcn_s_waitcnt(kWaitVm0);
     __syncthreads();
+    if constexpr (ROW) {
+        if (c0 < c1) {
+            transform(0);
+            transform(1);
+        }
+        __syncthreads();
+    }
     if (c0 < c1) gen(0, a);
-    int rd = 1, wr = 2;   // ring slots read and written by the current iteration
+    if constexpr (ROW) __syncthreads();   // slot 0 is reloaded by the first iteration
+    int rd = 1, tf = 2, wr = ROW ? 0 : 2;   // ring slots: generate, transform, load
     for (int c = c0; c < c1; ++c) {
         const int buf = (c - c0) & 1;
         fetch(min(c + 1, c1 - 1), buf ^ 1);
-        fetch_st(min(c + 2, c1 - 1), wr);
+        fetch_st(min(c + AHEAD, c1 - 1), wr);
+        if constexpr (ROW) transform(tf);
         bf16x8 cur[SPLIT];
 #pragma unroll
         for (int q = 0; q < SPLIT; ++q) cur[q] = a[q];
@@ -446,6 +478,7 @@ __global__ __launch_bounds__(256) void batched_adj_kernel(
         // chunk's G block.
         constexpr int kPerPair = TP * SPLIT * (SPLIT + 1) / 2;
         __builtin_amdgcn_sched_group_barrier(0x020, LOADS + 1, 0);   // VMEM read
+        if constexpr (ROW) __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);   // own record
         __builtin_amdgcn_sched_group_barrier(0x100, TP * SPLIT, 0);
 #pragma unroll
         for (int t = 0; t < NT; t += TP) {
@@ -457,6 +490,7 @@ __global__ __launch_bounds__(256) void batched_adj_kernel(
             }
         }
         rd = rd == 2 ? 0 : rd + 1;
+        tf = tf == 2 ? 0 : tf + 1;
         wr = wr == 2 ? 0 : wr + 1;
         __builtin_amdgcn_s_waitcnt(kWaitVm0);
         __syncthreads();
@@ -715,11 +749,16 @@ int batched_adj_launch(const PmShape& s, int B, int ks, int split, const float* 
     const int vec4 = s.n_cells % 4 == 0 && (uintptr_t)X % 16 == 0;
     {
         ProfScope prof("batched_adj", stream);
-#define ZTM_BATCHED_ADJ(NT, SP)                                                                  \
-    hipLaunchKernelGGL((batched_adj_kernel<NT, SP>), grid, dim3(256), 0, stream,                 \
+#define ZTM_BATCHED_ADJ_ROW(NT, SP, ROW)                                                         \
+    hipLaunchKernelGGL((batched_adj_kernel<NT, SP, ROW>), grid, dim3(256), 0, stream,            \
                        (const float4*)st, (const bf16x8*)parts, stride, s.nx, s.ny, 1.0 / s.nx,  \
                        1.0 / s.ny, (float)sy, (float)sz, (int)n_chunks, per, n_pad, part, direct, \
                        (int)s.n_cells, B, vec4)
+#define ZTM_BATCHED_ADJ(NT, SP)                                                                  \
+    if (s.nx % 32 == 0)                                                                        \
+        ZTM_BATCHED_ADJ_ROW(NT, SP, 1);                                                        \
+    else                                                                                       \
+        ZTM_BATCHED_ADJ_ROW(NT, SP, 0)
         if (split == 3) {
             switch (nt) {
                 case 1: ZTM_BATCHED_ADJ(1, 3); break;
@@ -736,6 +775,7 @@ int batched_adj_launch(const PmShape& s, int B, int ks, int split, const float* 
             }
         }
 #undef ZTM_BATCHED_ADJ
+#undef ZTM_BATCHED_ADJ_ROW
         ZTM_HIP(hipGetLastError());
     }
     if (direct) return ZTM_OK;
