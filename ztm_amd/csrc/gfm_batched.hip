@@ -31,6 +31,7 @@
 #include "gfm_pair.h"
 
 #include <algorithm>
+#include <type_traits>
 
 namespace ztm {
 
@@ -318,22 +319,20 @@ __global__ __launch_bounds__(256) void batched_fwd_kernel(
 // 8 (lane / 32) + 0..7 of the chunk, evaluated with exactly the forward's
 // arithmetic (MODE 0), so both directions use bit-identical G entries.
 //
-// The stations of a chunk are staged in LDS next to the residual parts. G
-// of chunk c + 1 is generated while chunk c's MFMAs run, i.e. before the
-// barrier that publishes chunk c + 1's parts, so the stations run further
-// ahead in a ring of 3 slots, chunk c0 + r in slot r % 3. Every wave loads
-// (and reads) a copy of its own, 4 copies of the 16 records across the 64
-// lanes, so no load needs a lane mask.
-//   ROW 0: iteration i loads chunk c0 + i + 2 and generates from the raw
-//          records of chunk c0 + i + 1.
-//   ROW 1 (nx % 32 == 0: a wave's 32 cells lie in one grid row, so they
-//          share y and z): iteration i loads chunk c0 + i + 3, turns the
-//          records of chunk c0 + i + 2 into (x, dy^2 + dz^2 + 1e-20, dz) for
-//          the wave's row in place (each lane its own record), and generates
-//          from chunk c0 + i + 1's, transformed one barrier earlier. A lane's
-//          G entry then costs 1 sub + 1 FMA + rsq + 3 mul instead of
-//          3 sub + 3 FMA + rsq + 3 mul, with the same roundings.
-// A slot is reloaded only after the barrier that follows its last read.
+// The stations of a chunk never touch LDS: lane l holds the record of
+// station 16 c + 8 (l / 32) + l % 8 of the chunk in registers, and a lane's
+// 8 stations 8 (l / 32) + e are broadcast to it by DPP row_newbcast:e (lane
+// e of each 16-lane row to the whole row), a register move that needs no
+// memory wait. G of chunk c + 1 is generated while chunk c's MFMAs run, so
+// the record of chunk c + 2 is loaded (plain global load) during iteration c
+// and used in iteration c + 1. The only LDS traffic is the residual parts,
+// which load straight from global memory into the buffer that is not being
+// read, as in the forward.
+//   ROW 1 (nx % 32 == 0: a wave's 32 cells lie in one grid row and share y
+//          and z): each lane first turns its record into (x, dy^2 + dz^2 +
+//          1e-20, dz) for the wave's row, so a G entry costs 1 sub + 1 FMA +
+//          rsq + 3 mul instead of 3 sub + 3 FMA + rsq + 3 mul, with the same
+//          roundings.
 //
 // With one K-slice (X != nullptr) the kernel stores X[b][n] itself: lane
 // (row, kh) holds vector 32 t + row at cells 8 g + 4 kh + 0..3 of its wave's
@@ -341,6 +340,20 @@ __global__ __launch_bounds__(256) void batched_fwd_kernel(
 // (N % 4 == 0 and X 16-byte aligned). The scale is then folded into the
 // packed residuals, so the accumulators are stored as they are (no register
 // work that would cost occupancy).
+template <int E>
+__device__ __forceinline__ float row_bcast(float v) {   // lane E of this lane's 16-lane row
+    return __builtin_bit_cast(
+        float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), 0x150 + E, 0xF, 0xF, false));
+}
+
+template <int E, int N, typename F>
+__device__ __forceinline__ void static_for(F&& f) {
+    if constexpr (E < N) {
+        f(std::integral_constant<int, E>{});
+        static_for<E + 1, N>(f);
+    }
+}
+
 template <int NT, int SPLIT, int ROW>   // model tiles of 32 (B <= 32 * NT); 2- or 3-way split
 __global__ __launch_bounds__(256) void batched_adj_kernel(
     const float4* __restrict__ st, const bf16x8* __restrict__ parts, int64_t part_stride,
@@ -350,9 +363,7 @@ __global__ __launch_bounds__(256) void batched_adj_kernel(
     constexpr int Bp = 32 * NT;
     constexpr int V = Bp * 2;                          // bf16x8 per part per chunk
     constexpr int LOADS = (V * SPLIT + 255) / 256;     // per thread per chunk
-    constexpr int AHEAD = ROW ? 3 : 2;                 // station chunks loaded ahead
     __shared__ bf16x8 lp[2][SPLIT][V];
-    __shared__ float4 ls[3][4][64];                    // [ring slot][wave][lane]
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int row = lane & 31, kh = lane >> 5;
     const int nbase = blockIdx.x * kBatchTile + wave * 32;
@@ -385,32 +396,29 @@ __global__ __launch_bounds__(256) void batched_adj_kernel(
                     16, 0, 0);
         }
     };
-    // lane l of each wave: record 16 chunk + l % 16 into ls[slot][wave][l]
-    auto fetch_st = [&](int chunk, int slot) {
-        __builtin_amdgcn_global_load_lds(
-            (const void*)(st + (int64_t)chunk * kBatchChunk + (lane & 15)),
-            (__attribute__((address_space(3))) void*)(&ls[slot][wave][0]), 16, 0, 0);
+    // x, y, z only: a load whose unused fourth register is reallocated while
+    // the load is in flight would need a wait for it
+    auto record = [&](int chunk) {
+        const float* r = &st[(int64_t)chunk * kBatchChunk + 8 * kh + (lane & 7)].x;
+        return make_float3(r[0], r[1], r[2]);
     };
-    // ROW 1: record (x, y, z, w) -> (x, dy^2 + dz^2 + 1e-20, dz, w), this lane's own
-    auto transform = [&](int slot) {
-        const float4 p = ls[slot][wave][lane];
-        const float dy = p.y - cy;
-        const float dz = p.z - cz;
-        ls[slot][wave][lane] = make_float4(p.x, fmaf(dy, dy, fmaf(dz, dz, 1e-20f)), dz, p.w);
-    };
-    auto gen = [&](int slot, bf16x8 (&a)[SPLIT]) {
-        const float4* s = &ls[slot][wave][8 * kh];
-#pragma unroll
-        for (int e = 0; e < 8; ++e) {
-            const float4 p = s[e];
+    auto gen = [&](float3 p, bf16x8 (&a)[SPLIT]) {
+        if constexpr (ROW) {   // (x, R0, dz) of this lane's station for the wave's row
+            const float dy = p.y - cy;
+            const float dz = p.z - cz;
+            p.y = fmaf(dy, dy, fmaf(dz, dz, 1e-20f));
+            p.z = dz;
+        }
+        static_for<0, 8>([&](auto ec) {
+            constexpr int e = decltype(ec)::value;
             float r2, dz;
-            const float x = p.x - cx;
+            const float x = row_bcast<e>(p.x) - cx;
             if constexpr (ROW) {
-                r2 = fmaf(x, x, p.y);
-                dz = p.z;
+                r2 = fmaf(x, x, row_bcast<e>(p.y));
+                dz = row_bcast<e>(p.z);
             } else {
-                const float dy = p.y - cy;
-                dz = p.z - cz;
+                const float dy = row_bcast<e>(p.y) - cy;
+                dz = row_bcast<e>(p.z) - cz;
                 r2 = fmaf(x, x, fmaf(dy, dy, fmaf(dz, dz, 1e-20f)));
             }
             const float v = rsq_f(r2);
@@ -421,35 +429,26 @@ __global__ __launch_bounds__(256) void batched_adj_kernel(
                 a[q][e] = h;
                 gv -= (float)h;
             }
-        }
+        });
     };
     bf16x8 a[SPLIT];
+    float3 rec = make_float3(0.f, 0.f, 0.f);
     if (c0 < c1) {
         fetch(c0, 0);
-#pragma unroll
-        for (int r = 0; r < AHEAD; ++r) fetch_st(min(c0 + r, c1 - 1), r);
+        gen(record(c0), a);
+        rec = record(min(c0 + 1, c1 - 1));
     }
     __builtin_amdgcn_s_waitcnt(kWaitVm0);
     __syncthreads();
-    if constexpr (ROW) {
-        if (c0 < c1) {
-            transform(0);
-            transform(1);
-        }
-        __syncthreads();
-    }
-    if (c0 < c1) gen(0, a);
-    if constexpr (ROW) __syncthreads();   // slot 0 is reloaded by the first iteration
-    int rd = 1, tf = 2, wr = ROW ? 0 : 2;   // ring slots: generate, transform, load
     for (int c = c0; c < c1; ++c) {
         const int buf = (c - c0) & 1;
         fetch(min(c + 1, c1 - 1), buf ^ 1);
-        fetch_st(min(c + AHEAD, c1 - 1), wr);
-        if constexpr (ROW) transform(tf);
+        const float3 now = rec;                 // chunk c + 1
+        rec = record(min(c + 2, c1 - 1));
         bf16x8 cur[SPLIT];
 #pragma unroll
         for (int q = 0; q < SPLIT; ++q) cur[q] = a[q];
-        gen(rd, a);
+        gen(now, a);
         constexpr int TP = NT >= 2 ? 2 : 1;
         bf16x8 b[2][TP][SPLIT];
         auto load_b = [&](int t, bf16x8 (&bb)[TP][SPLIT]) {
@@ -478,7 +477,6 @@ __global__ __launch_bounds__(256) void batched_adj_kernel(
         // chunk's G block.
         constexpr int kPerPair = TP * SPLIT * (SPLIT + 1) / 2;
         __builtin_amdgcn_sched_group_barrier(0x020, LOADS + 1, 0);   // VMEM read
-        if constexpr (ROW) __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);   // own record
         __builtin_amdgcn_sched_group_barrier(0x100, TP * SPLIT, 0);
 #pragma unroll
         for (int t = 0; t < NT; t += TP) {
@@ -489,9 +487,6 @@ __global__ __launch_bounds__(256) void batched_adj_kernel(
                 __builtin_amdgcn_sched_group_barrier(0x002, SPLIT == 3 ? 4 : 6, 0);
             }
         }
-        rd = rd == 2 ? 0 : rd + 1;
-        tf = tf == 2 ? 0 : tf + 1;
-        wr = wr == 2 ? 0 : wr + 1;
         __builtin_amdgcn_s_waitcnt(kWaitVm0);
         __syncthreads();
     }
