@@ -206,3 +206,29 @@ def test_batched_adjoint_rejects_and_empty(dev):
         assert out.shape == (0, w.grid.n_cells)
         X = p.adjoint_batched(torch.zeros(3, w.n_stations, device=dev)).cpu().numpy()
         assert not np.any(X)
+
+
+def test_batched_forward_adjoint_full_size(dev):
+    """ensemble_256 at its full size (96x96x48 cells of 1 km, 9,216 stations,
+    256 models and 256 residual vectors), 3-way split: forward and adjoint
+    against the FP64 single-model kernels on models / vectors 0 and 255
+    (product bar 1e-5 of max |.|), and <G P, R> = <P, G^T R> for every model
+    (size-independent; 1e-5 of sum |G P| |R|)."""
+    import torch
+    w = W.ensemble_256()
+    R = residuals(14, 256, w.n_stations)
+    with gfm.Plan(w.grid, w.stations, kappa=w.kappa) as p:
+        Pt = torch.from_numpy(w.rho).to(dev)
+        Y = p.forward_batched(Pt).cpu().numpy()
+        del Pt
+        X = p.adjoint_batched(torch.from_numpy(R).to(dev)).cpu().numpy()
+        for b in (0, 255):
+            g64 = p.forward(torch.from_numpy(w.rho[b].astype(np.float64)).to(dev)).cpu().numpy()
+            a64 = p.adjoint(torch.from_numpy(R[b].astype(np.float64)).to(dev)).cpu().numpy()
+            assert relerr(Y[b], g64) < 1e-5, b
+            assert relerr(X[b], a64) < 1e-5, b
+    R64 = R.astype(np.float64)
+    lhs = np.einsum("bm,bm->b", Y, R64)
+    rhs = np.einsum("bn,bn->b", w.rho.astype(np.float64), X.astype(np.float64))
+    scale = np.einsum("bm,bm->b", np.abs(Y), np.abs(R64))
+    assert np.max(np.abs(lhs - rhs) / scale) < 1e-5

@@ -112,76 +112,81 @@ __device__ __forceinline__ void divmod_pos(int n, int d, double inv, int& q, int
 }
 
 // This lane's 8 G values of a chunk (cells cell0 .. cell0 + 7, raster order),
-// split into SPLIT BF16 parts in the MFMA A-operand layout. MODE 1 (nx >= 8):
+// split into SPLIT BF16 parts in the MFMA A-operand layout: begin() forms
+// the chunk's shared terms, elem<E>() the value of cell cell0 + E, so that
+// the kernel can spread the 8 elements between its MFMAs. MODE 1 (nx >= 8):
 // the 8 cells span at most two grid rows, so both row terms are formed once
 // and each cell selects one. MODE 2 (nx a multiple of 8): the 8 cells lie in
 // one row (cell0 is a multiple of 8), so there is nothing to select; for such
-// grids it gives the same values as MODE 1. MODE 0: cell-by-cell stepping.
-// Cells past the grid give finite values (r^2 >= 1e-20, so |dz r^-3| <=
-// 1e36), which the zero-padded P cancels.
+// grids it gives the same values as MODE 1. MODE 0: cell-by-cell stepping
+// (elements in increasing E). Cells past the grid give finite values
+// (r^2 >= 1e-20, so |dz r^-3| <= 1e36), which the zero-padded P cancels.
 template <int SPLIT, int MODE>
-__device__ __forceinline__ void batched_gen(const float4 p, int cell0, int nx, int ny,
-                                            double inv_nx, double inv_ny, float sy, float sz,
-                                            bf16x8 (&a)[SPLIT]) {
-    int rw, i0, k0, j0;
-    divmod_pos(cell0, nx, inv_nx, rw, i0);
-    divmod_pos(rw, ny, inv_ny, k0, j0);
-    if constexpr (MODE == 2) {
-        const float dy0 = p.y - (float)j0 * sy, dz0 = p.z - (float)k0 * sz;
-        const float R0 = fmaf(dy0, dy0, fmaf(dz0, dz0, 1e-20f));
-        const float xa = p.x - (float)i0;
-#pragma unroll
-        for (int e = 0; e < 8; ++e) {
-            const float x = xa - (float)e;
-            const float v = rsq_f(fmaf(x, x, R0));
-            float gv = dz0 * (v * v * v);
-#pragma unroll
-            for (int q = 0; q < SPLIT; ++q) {
-                const __bf16 h = (__bf16)gv;
-                a[q][e] = h;
-                gv -= (float)h;
+struct BatchedGen {
+    float xa, R0, dz0, R1, dz1, nxf;   // MODE 1, 2
+    int ew;                            // MODE 1: cells e >= ew lie on the next row
+    float px, py, pz, sy, sz;          // MODE 0
+    int i, j, k, nx, ny;               // MODE 0
+
+    __device__ __forceinline__ void begin(const float4 p, int cell0, int nx_, int ny_,
+                                          double inv_nx, double inv_ny, float sy_, float sz_) {
+        int rw, i0, k0, j0;
+        divmod_pos(cell0, nx_, inv_nx, rw, i0);
+        divmod_pos(rw, ny_, inv_ny, k0, j0);
+        if constexpr (MODE == 0) {
+            px = p.x; py = p.y; pz = p.z; sy = sy_; sz = sz_;
+            i = i0; j = j0; k = k0; nx = nx_; ny = ny_;
+        } else {
+            dz0 = p.z - (float)k0 * sz_;
+            const float dy0 = p.y - (float)j0 * sy_;
+            // no near-field pass follows, so the exact row term (floored only
+            // against 0/0 at a coincident station), not row_R32
+            R0 = fmaf(dy0, dy0, fmaf(dz0, dz0, 1e-20f));
+            xa = p.x - (float)i0;
+            if constexpr (MODE == 1) {
+                const bool wrap = j0 + 1 == ny_;
+                const int j1 = wrap ? 0 : j0 + 1, k1 = wrap ? k0 + 1 : k0;
+                const float dy1 = p.y - (float)j1 * sy_;
+                dz1 = p.z - (float)k1 * sz_;
+                R1 = fmaf(dy1, dy1, fmaf(dz1, dz1, 1e-20f));
+                ew = nx_ - i0;
+                nxf = (float)nx_;
             }
         }
-    } else if constexpr (MODE == 1) {
-        const bool wrap = j0 + 1 == ny;
-        const int j1 = wrap ? 0 : j0 + 1, k1 = wrap ? k0 + 1 : k0;
-        const float dy0 = p.y - (float)j0 * sy, dz0 = p.z - (float)k0 * sz;
-        const float dy1 = p.y - (float)j1 * sy, dz1 = p.z - (float)k1 * sz;
-        // no near-field pass follows, so the exact row term (floored only
-        // against 0/0 at a coincident station), not row_R32
-        const float R0 = fmaf(dy0, dy0, fmaf(dz0, dz0, 1e-20f));
-        const float R1 = fmaf(dy1, dy1, fmaf(dz1, dz1, 1e-20f));
-        const int ew = nx - i0;   // cells e >= ew lie on the next row
-        const float xa = p.x - (float)i0, nxf = (float)nx;
-#pragma unroll
-        for (int e = 0; e < 8; ++e) {
-            const bool w = e >= ew;
-            const float xe = xa - (float)e;
+    }
+
+    // Empty asm operands that tie the chunk's shared terms to this point of
+    // the instruction stream, so that an element computed from them cannot
+    // be hoisted above it.
+    __device__ __forceinline__ void pin() {
+        if constexpr (MODE == 2) {
+            asm volatile("" : "+v"(xa), "+v"(R0), "+v"(dz0));
+        } else if constexpr (MODE == 1) {
+            asm volatile("" : "+v"(xa), "+v"(R0), "+v"(dz0), "+v"(R1), "+v"(dz1));
+        } else {
+            asm volatile("" : "+v"(px), "+v"(py), "+v"(pz));
+        }
+    }
+
+    template <int E>
+    __device__ __forceinline__ void elem(bf16x8 (&a)[SPLIT]) {
+        float gv;
+        if constexpr (MODE == 2) {
+            const float x = xa - (float)E;
+            const float v = rsq_f(fmaf(x, x, R0));
+            gv = dz0 * (v * v * v);
+        } else if constexpr (MODE == 1) {
+            const bool w = E >= ew;
+            const float xe = xa - (float)E;
             const float x = w ? xe + nxf : xe;
             const float v = rsq_f(fmaf(x, x, w ? R1 : R0));
-            float gv = (w ? dz1 : dz0) * (v * v * v);
-#pragma unroll
-            for (int q = 0; q < SPLIT; ++q) {
-                const __bf16 h = (__bf16)gv;
-                a[q][e] = h;
-                gv -= (float)h;
-            }
-        }
-    } else {
-        int i = i0, j = j0, k = k0;
-#pragma unroll
-        for (int e = 0; e < 8; ++e) {
-            const float dy = p.y - (float)j * sy;
-            const float dz = p.z - (float)k * sz;
-            const float x = p.x - (float)i;
+            gv = (w ? dz1 : dz0) * (v * v * v);
+        } else {
+            const float dy = py - (float)j * sy;
+            const float dz = pz - (float)k * sz;
+            const float x = px - (float)i;
             const float v = rsq_f(fmaf(x, x, fmaf(dy, dy, fmaf(dz, dz, 1e-20f))));
-            float gv = dz * (v * v * v);
-#pragma unroll
-            for (int q = 0; q < SPLIT; ++q) {
-                const __bf16 h = (__bf16)gv;
-                a[q][e] = h;
-                gv -= (float)h;
-            }
+            gv = dz * (v * v * v);
             const bool wi = ++i == nx;
             i = wi ? 0 : i;
             j += wi ? 1 : 0;
@@ -189,7 +194,43 @@ __device__ __forceinline__ void batched_gen(const float4 p, int cell0, int nx, i
             j = wj ? 0 : j;
             k += wj ? 1 : 0;
         }
+#pragma unroll
+        for (int q = 0; q < SPLIT; ++q) {
+            const __bf16 h = (__bf16)gv;
+            a[q][E] = h;
+            gv -= (float)h;
+        }
     }
+};
+
+template <int E, int N, typename F>
+__device__ __forceinline__ void static_for(F&& f) {
+    if constexpr (E < N) {
+        f(std::integral_constant<int, E>{});
+        static_for<E + 1, N>(f);
+    }
+}
+
+template <int SPLIT, int MODE>
+__device__ __forceinline__ void batched_gen(const float4 p, int cell0, int nx, int ny,
+                                            double inv_nx, double inv_ny, float sy, float sz,
+                                            bf16x8 (&a)[SPLIT]) {
+    BatchedGen<SPLIT, MODE> g;
+    g.begin(p, cell0, nx, ny, inv_nx, inv_ny, sy, sz);
+    static_for<0, 8>([&](auto ec) { g.template elem<decltype(ec)::value>(a); });
+}
+
+// Issue order within a chunk: after MFMA m (of the chunk's TOTAL) come the G
+// elements e with e * TOTAL / 8 == m (integer division: the 8 elements of
+// the next chunk's G block spread evenly over the MFMAs), fenced by
+// scheduling barriers, so that the wave issues independent VALU work while
+// each MFMA runs.
+template <int TOTAL, typename F>
+__device__ __forceinline__ void gen_after_mfma(int m, F&& elem) {
+    static_for<0, 8>([&](auto ec) {
+        constexpr int e = decltype(ec)::value;
+        if ((e * TOTAL) / 8 == m) elem(ec);
+    });
 }
 
 // grid = (station tiles of 128, K-slices); block = 256 (4 waves x 32 stations).
@@ -250,12 +291,16 @@ __global__ __launch_bounds__(256) void batched_fwd_kernel(
         bf16x8 cur[SPLIT];
 #pragma unroll
         for (int q = 0; q < SPLIT; ++q) cur[q] = a[q];
-        batched_gen<SPLIT, MODE>(p, cn * kBatchChunk + 8 * kh, nx, ny, inv_nx, inv_ny, sy, sz, a);
+        BatchedGen<SPLIT, MODE> g;
+        g.begin(p, cn * kBatchChunk + 8 * kh, nx, ny, inv_nx, inv_ny, sy, sz);
         // Tiles in pairs so that consecutive MFMAs update different
         // accumulators (no back-to-back dependence on one accumulator).
         // The B operands of the next tile pair are read from LDS while the
-        // current pair's MFMAs run (two register sets).
+        // current pair's MFMAs run (two register sets); the next chunk's G
+        // elements go between the MFMAs.
         constexpr int TP = NT >= 2 ? 2 : 1;
+        constexpr int kPerPair = TP * SPLIT * (SPLIT + 1) / 2;   // MFMAs per tile pair
+        constexpr int TOTAL = NT / TP * kPerPair;
         bf16x8 b[2][TP][SPLIT];
         auto load_b = [&](int t, bf16x8 (&bb)[TP][SPLIT]) {
 #pragma unroll
@@ -274,26 +319,17 @@ __global__ __launch_bounds__(256) void batched_fwd_kernel(
 #pragma unroll
                 for (int qa = 0; qa <= s2; ++qa)
 #pragma unroll
-                    for (int u = 0; u < TP; ++u)
+                    for (int u = 0; u < TP; ++u) {
                         acc[t + u] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
                             cur[qa], b[cb][u][s2 - qa], acc[t + u], 0, 0, 0);
-        }
-        // Schedule: the next chunk's global-to-LDS loads first, the first
-        // tile pair's LDS reads, then per tile pair the next pair's LDS reads
-        // and its own MFMAs, each MFMA followed by a few VALU instructions of
-        // the next chunk's G generation.
-        constexpr int kPerPair = TP * SPLIT * (SPLIT + 1) / 2;   // MFMAs per tile pair
-        __builtin_amdgcn_sched_group_barrier(0x020, LOADS, 0);   // VMEM read
-        __builtin_amdgcn_sched_group_barrier(0x100, TP * SPLIT, 0);   // DS read, pair 0
+                        const int m = t / TP * kPerPair + (s2 * (s2 + 1) / 2 + qa) * TP + u;
+                        gen_after_mfma<TOTAL>(m, [&](auto ec) {
+                            g.pin();
+                            g.template elem<decltype(ec)::value>(a);
 #pragma unroll
-        for (int t = 0; t < NT; t += TP) {
-            if (t + TP < NT)
-                __builtin_amdgcn_sched_group_barrier(0x100, TP * SPLIT, 0);   // DS read, next pair
-#pragma unroll
-            for (int i = 0; i < kPerPair; ++i) {
-                __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);        // MFMA
-                __builtin_amdgcn_sched_group_barrier(0x002, SPLIT == 3 ? 4 : 6, 0);   // VALU
-            }
+                            for (int q = 0; q < SPLIT; ++q) asm volatile("" : "+v"(a[q]));
+                        });
+                    }
         }
         __builtin_amdgcn_s_waitcnt(kWaitVm0);
         __syncthreads();
@@ -344,14 +380,6 @@ template <int E>
 __device__ __forceinline__ float row_bcast(float v) {   // lane E of this lane's 16-lane row
     return __builtin_bit_cast(
         float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), 0x150 + E, 0xF, 0xF, false));
-}
-
-template <int E, int N, typename F>
-__device__ __forceinline__ void static_for(F&& f) {
-    if constexpr (E < N) {
-        f(std::integral_constant<int, E>{});
-        static_for<E + 1, N>(f);
-    }
 }
 
 template <int NT, int SPLIT, int ROW>   // model tiles of 32 (B <= 32 * NT); 2- or 3-way split
@@ -471,10 +499,12 @@ __global__ __launch_bounds__(256) void batched_adj_kernel(
                         acc[t + u] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
                             cur[qa], b[cb][u][s2 - qa], acc[t + u], 0, 0, 0);
         }
-        // Same issue order as the forward: global-to-LDS loads, the first
-        // tile pair's LDS reads, then per tile pair the next pair's reads and
-        // its MFMAs, each followed by a few VALU instructions of the next
-        // chunk's G block.
+        // Issue-order hints: global loads, the first tile pair's LDS reads,
+        // then per tile pair the next pair's reads and its MFMAs, each
+        // followed by a few VALU instructions of the next chunk's G block.
+        // (The forward's source-order interleave measured the same on this
+        // kernel's shapes, and here it moved the copy of the prefetched
+        // record into the iteration, with a wait for every load.)
         constexpr int kPerPair = TP * SPLIT * (SPLIT + 1) / 2;
         __builtin_amdgcn_sched_group_barrier(0x020, LOADS + 1, 0);   // VMEM read
         __builtin_amdgcn_sched_group_barrier(0x100, TP * SPLIT, 0);
