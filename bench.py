@@ -99,8 +99,8 @@ def main() -> int:
                          "(calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE against known bytes)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(sharding is described in DESIGN.md); run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU only: there is no multi-GPU path; "
+              "run with --gpus 1", file=sys.stderr)
         return 2
 
     import torch
@@ -234,8 +234,45 @@ def main() -> int:
         line["batched"] = batched_leg(stream, args)
         line["uneven_batch"] = uneven_leg(stream, args)
         line["tfm_fm"] = tfm_leg(stream, args)
+    line["summary"] = summary(line)
     print(json.dumps(line))
     return 0
+
+
+def summary(line):
+    """The speedups of every forward + adjoint path over its FP64 GPU
+    counterpart and over the CPU baseline, with the largest relative error of
+    each, gathered from the legs above (None where a leg did not run)."""
+    def get(d, *keys):
+        for k in keys:
+            if not isinstance(d, dict) or k not in d:
+                return None
+            d = d[k]
+        return d
+
+    def worst(e):
+        return max(e.values()) if isinstance(e, dict) else e
+
+    out = {"fwd_adj_vs_fp64_gpu": {
+               "point_mass_fp32": line.get("speedup_vs_fp64_gpu"),
+               "prism_fp32": get(line, "prism", "speedup_vs_fp64_gpu"),
+               "ensemble_split3": get(line, "batched", "split3", "forward_plus_adjoint",
+                                      "speedup_vs_fp64_gpu"),
+               "ensemble_split2": get(line, "batched", "split2", "forward_plus_adjoint",
+                                      "speedup_vs_fp64_gpu")},
+           "point_mass_vs_cpu_baseline": line.get("speedup_vs_cpu"),
+           "max_rel_err_vs_fp64": {
+               "point_mass_fp32": worst(line.get("rel_err_vs_fp64")),
+               "prism_fp32": worst(get(line, "prism", "rel_err_vs_fp64")),
+               "ensemble_split3": max((v for v in (get(line, "batched", "split3", "rel_err_vs_fp64"),
+                                                   get(line, "batched", "split3", "adjoint",
+                                                       "rel_err_vs_fp64")) if v is not None),
+                                      default=None),
+               "ensemble_split2": max((v for v in (get(line, "batched", "split2", "rel_err_vs_fp64"),
+                                                   get(line, "batched", "split2", "adjoint",
+                                                       "rel_err_vs_fp64")) if v is not None),
+                                      default=None)}}
+    return out
 
 
 def _timed(fn, reps, sync):
