@@ -907,7 +907,9 @@ int ztm_tfm_gravity_jvp(int bfr, int di, int dj, int dk, int fh, int fv, const d
     if (int e = tfm_drho_jvp_launch(di, dj, dk, fh, fv, rc, sea_dev, moho_dev, vel_dev, vm, ro,
                                     dr, ddrho_dev, s))
         return e;
-    // One FP32 GF forward (with its FP64 near-field correction) per field.
+    // The three fields through one FP32 GF forward that forms each pair's
+    // r^-3 once (each field with its FP64 near-field correction), or one
+    // forward per field when three density tiles of a row do not fit LDS.
     // The fields are already masked by GF's skip at the parameter point, so
     // neither the main kernel nor the correction thresholds them again.
     const int M = di * dj, m = p->shape.m;
@@ -921,11 +923,18 @@ int ztm_tfm_gravity_jvp(int bfr, int di, int dj, int dk, int fh, int fv, const d
     NearArgs na = near_args(p);
     na.thr = 0.f;
     const int64_t N = p->shape.n_cells;
-    for (int f = 0; f < 3; ++f)
-        if (int e = pm_fwd_launch(p->shape, p->tuning, ZTM_FP32, p->st_f, ddrho_dev + f * N,
-                                  p->sy, p->sz, 0.0, pm_scale(p), p->part, Y + (size_t)f * m,
-                                  nullptr, nullptr, na, s))
+    if (pm_fwd_fields_supported(p->shape, 3) && !na.quad) {
+        if (int rc = ensure_workspace(p, pm_fwd_fields_workspace(p->shape, 3))) return rc;
+        if (int e = pm_fwd_fields_launch(p->shape, 3, (const float4*)p->st_f, ddrho_dev, N, p->sy,
+                                         p->sz, pm_scale(p), p->part, Y, na, s))
             return e;
+    } else {
+        for (int f = 0; f < 3; ++f)
+            if (int e = pm_fwd_launch(p->shape, p->tuning, ZTM_FP32, p->st_f, ddrho_dev + f * N,
+                                      p->sy, p->sz, 0.0, pm_scale(p), p->part,
+                                      Y + (size_t)f * m, nullptr, nullptr, na, s))
+                return e;
+    }
     return tfm_scatter_launch(Y, p->out_map, m, M, 3, dg_dev, s);
 }
 

@@ -85,14 +85,19 @@ constexpr int kFwdGroup = QUAD ? 1
                           : sizeof(T) == sizeof(float) ? (S <= 1 ? 8 : S == 2 ? 4 : S == 4 ? 2 : 1)
                                                        : (S == 2 ? 4 : 1);
 
-template <typename T, int S, bool QUAD>
+// F > 1 (FP32 row groups only): F density fields, field f at rho + f *
+// field_stride, share every pair's r^-3; their partials go to
+// part[(by * F + f) * m_pad + m] and smem holds F density tiles.
+template <typename T, int S, bool QUAD, int F = 1>
 __device__ __forceinline__ void pm_fwd_block(
     const typename Vec4<T>::type* __restrict__ st, const T* __restrict__ rho,
     int nx, int ny, int n_rows, T sy, T sz, T thr, int rows_per_slice,
     double* __restrict__ part, int m_pad, const QuadArgs& qa, int bx, int by,
-    unsigned char* smem) {
+    unsigned char* smem, int64_t field_stride = 0) {
     using V4 = typename Vec4<T>::type;
     constexpr int G = kFwdGroup<T, S, QUAD>;
+    static_assert(F == 1 || (G > 1 && sizeof(T) == sizeof(float)),
+                  "several fields need the FP32 row-group path");
     T* lrho = reinterpret_cast<T*>(smem);
 
     const int nxp = (nx + 3) & ~3;
@@ -102,15 +107,19 @@ __device__ __forceinline__ void pm_fwd_block(
     const size_t base = (size_t)row0 * (size_t)nx;
 
     // Stage this slice's densities, thresholded and zero-padded to nxp.
+    const int tile = nr_pad * nxp;   // one field's tile
     for (int t = threadIdx.x; t < nr_pad * nxp; t += blockDim.x) {
         const int r = t / nxp;
         const int i = t - r * nxp;
-        T v = T(0);
-        if (i < nx && r < nr) {
-            v = rho[base + (size_t)r * nx + i];
-            if (!(fabs(v) > thr)) v = T(0);   // NaN or |rho| <= thr: skipped
+#pragma unroll
+        for (int f = 0; f < F; ++f) {
+            T v = T(0);
+            if (i < nx && r < nr) {
+                v = rho[(size_t)f * field_stride + base + (size_t)r * nx + i];
+                if (!(fabs(v) > thr)) v = T(0);   // NaN or |rho| <= thr: skipped
+            }
+            lrho[f * tile + t] = v;
         }
-        lrho[t] = v;
     }
     __syncthreads();
 
@@ -119,12 +128,13 @@ __device__ __forceinline__ void pm_fwd_block(
     const int m0 = bx * (64 * S);
 
     T X[S], Y[S], Z[S];
-    double acc[S];
+    double acc[F][S];
 #pragma unroll
     for (int s = 0; s < S; ++s) {
         const V4 p = st[m0 + s * 64 + lane];
         X[s] = p.x; Y[s] = p.y; Z[s] = p.z;
-        acc[s] = 0.0;
+#pragma unroll
+        for (int f = 0; f < F; ++f) acc[f][s] = 0.0;
     }
 
     if constexpr (G > 1) {
@@ -134,7 +144,7 @@ __device__ __forceinline__ void pm_fwd_block(
         // arithmetic as pair_k32, which the near-field corrections replicate).
         // Padded rows hold 0.
         for (int rg = wave * G; rg < nr; rg += 4 * G) {
-            T R[G][S], ar[G][S], xm[S];
+            T R[G][S], ar[F][G][S], xm[S];
 #pragma unroll
             for (int g = 0; g < G; ++g) {
                 const int row = row0 + rg + g;
@@ -145,7 +155,8 @@ __device__ __forceinline__ void pm_fwd_block(
 #pragma unroll
                 for (int s = 0; s < S; ++s) {
                     R[g][s] = row_R(Y[s] - yj, Z[s] - zk);
-                    ar[g][s] = T(0);
+#pragma unroll
+                    for (int f = 0; f < F; ++f) ar[f][g][s] = T(0);
                 }
             }
 #pragma unroll
@@ -153,9 +164,12 @@ __device__ __forceinline__ void pm_fwd_block(
             const int nx4 = nxp >> 2;
             const V4* lrow = reinterpret_cast<const V4*>(lrho + rg * nxp);
             for (int i4 = 0; i4 < nx4; ++i4) {
-                V4 q[G];
+                V4 q[F][G];
 #pragma unroll
-                for (int g = 0; g < G; ++g) q[g] = lrow[g * nx4 + i4];   // wave-uniform
+                for (int f = 0; f < F; ++f)
+#pragma unroll
+                    for (int g = 0; g < G; ++g)
+                        q[f][g] = lrow[f * (tile >> 2) + g * nx4 + i4];   // wave-uniform
 #pragma unroll
                 for (int s = 0; s < S; ++s) {
                     const T a0 = xm[s], a1 = xm[s] - T(1), a2 = xm[s] - T(2), a3 = xm[s] - T(3);
@@ -165,23 +179,31 @@ __device__ __forceinline__ void pm_fwd_block(
                         const T v1 = rsq(fma(a1, a1, R[g][s]));
                         const T v2 = rsq(fma(a2, a2, R[g][s]));
                         const T v3 = rsq(fma(a3, a3, R[g][s]));
-                        T t = ar[g][s];
-                        t = fma(q[g].x, (v0 * v0) * v0, t);
-                        t = fma(q[g].y, (v1 * v1) * v1, t);
-                        t = fma(q[g].z, (v2 * v2) * v2, t);
-                        t = fma(q[g].w, (v3 * v3) * v3, t);
-                        ar[g][s] = t;
+                        const T k0 = (v0 * v0) * v0, k1 = (v1 * v1) * v1;
+                        const T k2 = (v2 * v2) * v2, k3 = (v3 * v3) * v3;
+#pragma unroll
+                        for (int f = 0; f < F; ++f) {
+                            T t = ar[f][g][s];
+                            t = fma(q[f][g].x, k0, t);
+                            t = fma(q[f][g].y, k1, t);
+                            t = fma(q[f][g].z, k2, t);
+                            t = fma(q[f][g].w, k3, t);
+                            ar[f][g][s] = t;
+                        }
                     }
                     xm[s] -= T(4);
                 }
-                if constexpr (sizeof(T) == sizeof(float)) interleave_trans<S * G * 4, 6>();
+                if constexpr (sizeof(T) == sizeof(float)) interleave_trans<S * G * 4, 5 + F>();
             }
 #pragma unroll
             for (int g = 0; g < G; ++g) {
                 const int row = row0 + rg + g;
                 const T zk = T(row / ny) * sz;
 #pragma unroll
-                for (int s = 0; s < S; ++s) acc[s] += (double)((Z[s] - zk) * ar[g][s]);
+                for (int s = 0; s < S; ++s)
+#pragma unroll
+                    for (int f = 0; f < F; ++f)
+                        acc[f][s] += (double)((Z[s] - zk) * ar[f][g][s]);
             }
         }
     } else
@@ -224,20 +246,36 @@ __device__ __forceinline__ void pm_fwd_block(
             }
         }
 #pragma unroll
-        for (int s = 0; s < S; ++s) acc[s] += (double)(dz[s] * ar[s]);
+        for (int s = 0; s < S; ++s) acc[0][s] += (double)(dz[s] * ar[s]);
     }
 
     // Cross-wavefront reduction (fixed order w0+w1+w2+w3).
     __syncthreads();
     double* red = reinterpret_cast<double*>(smem);
 #pragma unroll
-    for (int s = 0; s < S; ++s) red[(wave * S + s) * 64 + lane] = acc[s];
+    for (int f = 0; f < F; ++f)
+#pragma unroll
+        for (int s = 0; s < S; ++s) red[((wave * F + f) * S + s) * 64 + lane] = acc[f][s];
     __syncthreads();
-    for (int t = threadIdx.x; t < 64 * S; t += blockDim.x) {
-        const double v = red[t] + red[S * 64 + t] + red[2 * S * 64 + t] +
-                         red[3 * S * 64 + t];
-        part[(size_t)by * m_pad + m0 + t] = v;
+    for (int t = threadIdx.x; t < 64 * S * F; t += blockDim.x) {
+        const int f = t / (64 * S), u = t - f * (64 * S);
+        const double* rf = red + f * S * 64 + u;
+        const double v = rf[0] + rf[F * S * 64] + rf[2 * F * S * 64] + rf[3 * F * S * 64];
+        part[((size_t)by * F + f) * m_pad + m0 + u] = v;
     }
+}
+
+// F FP32 density fields through one pass (S = 2, row groups of 4).
+// grid = (m_pad / 128, n_slices); block = 256.
+template <int F>
+__global__ __launch_bounds__(256) void pm_fwd_fields_kernel(
+    const float4* __restrict__ st, const float* __restrict__ rho, int64_t field_stride, int nx,
+    int ny, int n_rows, float sy, float sz, int rows_per_slice, double* __restrict__ part,
+    int m_pad) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    pm_fwd_block<float, 2, false, F>(st, rho, nx, ny, n_rows, sy, sz, 0.f, rows_per_slice, part,
+                                     m_pad, QuadArgs{}, blockIdx.x, blockIdx.y, smem,
+                                     field_stride);
 }
 
 // grid = (m_pad / (64*S), n_slices); block = 256.
@@ -258,12 +296,13 @@ __global__ __launch_bounds__(256) void pm_fwd_kernel(
 // One wave per station: the lanes stride over the slice partials and the
 // correction pairs, then a fixed butterfly sums them (deterministic; every
 // lane returns the result).
+// Slice k's partial of station i is part[k * stride + i].
 __device__ __forceinline__ double pm_fwd_reduce_wave(const double* __restrict__ part,
-                                                    int n_slices, int m_pad, int i, double scale,
+                                                    int n_slices, int stride, int i, double scale,
                                                     const float* __restrict__ rho_f,
                                                     const NearArgs& na, int lane) {
     double s = 0.0;
-    for (int k = lane; k < n_slices; k += 64) s += part[(size_t)k * m_pad + i];
+    for (int k = lane; k < n_slices; k += 64) s += part[(size_t)k * stride + i];
     if (na.mode == kCorrPmNear) {
         const float4 pf = na.st_f[i];
         const double4 pd = na.st_d[i];
@@ -292,7 +331,7 @@ __device__ __forceinline__ double pm_fwd_reduce_wave(const double* __restrict__ 
 
 // block = 256 = 4 stations (one wave each)
 __global__ __launch_bounds__(256) void pm_fwd_reduce_kernel(const double* __restrict__ part,
-                                     int n_slices, int m_pad, int m,
+                                     int n_slices, int stride, int m,
                                      double scale, double* __restrict__ g,
                                      float* __restrict__ gref,
                                      const int* __restrict__ out_map,
@@ -301,7 +340,7 @@ __global__ __launch_bounds__(256) void pm_fwd_reduce_kernel(const double* __rest
     const int lane = threadIdx.x & 63;
     const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (i >= m) return;   // uniform per wave
-    const double v = pm_fwd_reduce_wave(part, n_slices, m_pad, i, scale, rho_f, na, lane);
+    const double v = pm_fwd_reduce_wave(part, n_slices, stride, i, scale, rho_f, na, lane);
     if (lane != 0) return;
     if (out_map)
         gref[out_map[i]] += (float)v;
@@ -963,6 +1002,72 @@ int pm_fwd_launch(const PmShape& s, const PmTuning& t0, int precision,
                        stream, part, n_slices, s.m_pad, s.m, scale, g, gref, out_map,
                        (const float*)rho, na);
     ZTM_HIP(hipGetLastError());
+    return ZTM_OK;
+}
+
+namespace {
+constexpr int kFieldsMax = 4;
+// Rows per slice for the F-field forward: the single-field choice for S = 2,
+// held to F density tiles within the 32 KB LDS budget, in whole row groups.
+int fields_rows_per_slice(const PmShape& s, int F) {
+    PmTuning t;
+    t.st_per_lane = 2;
+    t.rows_per_slice = 0;
+    const int nxp = (s.nx + 3) & ~3;
+    const int G = fwd_group(2, sizeof(float), false);
+    int rps = fwd_rows_per_slice(s, t, sizeof(float), false);
+    rps = std::min(rps, std::max(G, 32768 / (nxp * (int)sizeof(float) * F) / G * G));
+    return std::max(rps, 1);
+}
+}  // namespace
+
+bool pm_fwd_fields_supported(const PmShape& s, int F) {
+    const int nxp = (s.nx + 3) & ~3;
+    const int G = fwd_group(2, sizeof(float), false);
+    return F >= 1 && F <= kFieldsMax && (size_t)nxp * sizeof(float) * G * F <= 32768;
+}
+
+size_t pm_fwd_fields_workspace(const PmShape& s, int F) {
+    const int rps = fields_rows_per_slice(s, F);
+    return (size_t)((s.n_rows + rps - 1) / rps) * F * (size_t)s.m_pad;
+}
+
+int pm_fwd_fields_launch(const PmShape& s, int F, const float4* st, const float* rho,
+                         int64_t field_stride, double sy, double sz, double scale,
+                         double* part, double* g, const NearArgs& near, hipStream_t stream) {
+    if (!pm_fwd_fields_supported(s, F))
+        return fail(ZTM_EINVAL, "%d fields of rows of %d cells exceed the LDS budget", F, s.nx);
+    if (near.quad) return fail(ZTM_EINVAL, "the multi-field forward has no quadrupole path");
+    if (s.m == 0) return ZTM_OK;
+    const int rps = fields_rows_per_slice(s, F);
+    const int n_slices = (int)((s.n_rows + rps - 1) / rps);
+    if (n_slices > 65535) return fail(ZTM_EINVAL, "too many row slices (%d)", n_slices);
+    const int nxp = (s.nx + 3) & ~3;
+    const int G = fwd_group(2, sizeof(float), false);
+    const size_t lds = std::max((size_t)((rps + G - 1) / G * G) * nxp * sizeof(float) * F,
+                                (size_t)4 * 2 * 64 * sizeof(double) * F);
+    const dim3 grid(s.m_pad / 128, n_slices);
+    {
+        ProfScope prof("pm_fwd_fields", stream);
+#define ZTM_FIELDS(FF)                                                                          \
+    hipLaunchKernelGGL(pm_fwd_fields_kernel<FF>, grid, dim3(256), lds, stream, st, rho,         \
+                       field_stride, s.nx, s.ny, (int)s.n_rows, (float)sy, (float)sz, rps,       \
+                       part, s.m_pad)
+        switch (F) {
+            case 1: ZTM_FIELDS(1); break;
+            case 2: ZTM_FIELDS(2); break;
+            case 3: ZTM_FIELDS(3); break;
+            default: ZTM_FIELDS(4); break;
+        }
+#undef ZTM_FIELDS
+        ZTM_HIP(hipGetLastError());
+    }
+    for (int f = 0; f < F; ++f) {
+        hipLaunchKernelGGL(pm_fwd_reduce_kernel, dim3((s.m + 3) / 4), dim3(256), 0, stream,
+                           part + (size_t)f * s.m_pad, n_slices, F * s.m_pad, s.m, scale,
+                           g + (size_t)f * s.m, nullptr, nullptr, rho + f * field_stride, near);
+        ZTM_HIP(hipGetLastError());
+    }
     return ZTM_OK;
 }
 
