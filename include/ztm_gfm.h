@@ -172,7 +172,11 @@ int ztm_plan_adjoint_stored(ztm_plan plan, const float* r_dev, float* out_dev,
  * FP32 accumulation per K-slice and FP64 across slices. n_models <= 256 per
  * call. The plan's threshold is applied to rho. There is no FP64
  * near-field pass: results assume no station lies within about half a cell
- * of a cell centre (true for stations above the grid's top face). */
+ * of a cell centre (true for stations above the grid's top face). Up to 4
+ * models instead take one FP32 point-mass pass that forms each pair's
+ * r^-3 once for all of them, with the plan's FP64 near-field correction
+ * (the result of ztm_plan_forward per model, to FP64 rounding of the
+ * partial sums), when 4 density rows of the grid fit its LDS tile. */
 int ztm_plan_forward_batched(ztm_plan plan, int n_models, const float* rho_dev,
                              double* g_dev, void* stream);
 /* Batched (ensemble) adjoint on MFMA, point-mass formula (no reference

@@ -232,3 +232,25 @@ def test_batched_forward_adjoint_full_size(dev):
     rhs = np.einsum("bn,bn->b", w.rho.astype(np.float64), X.astype(np.float64))
     scale = np.einsum("bm,bm->b", np.abs(Y), np.abs(R64))
     assert np.max(np.abs(lhs - rhs) / scale) < 1e-5
+
+
+def test_small_ensembles_take_the_multi_field_forward(dev, orc):
+    """1-4 models go through the multi-field FP32 forward (pair terms formed
+    once, the plan's FP64 near-field correction per model): each model equals
+    the single-model FP32 forward up to FP64 rounding of the partial sums,
+    and the oracle to 1e-6; on a ragged grid with a threshold and stations
+    inside the grid's footprint, close to the top face."""
+    import torch
+    g = W.Grid(37, 29, 11, 100.0, -50.0, 30.0, -70.0, 45.0, -25.0)
+    c = g.centres()
+    st = np.stack([W.uniform(41, a, 300, c[:, a].min(), c[:, a].max()) for a in range(2)]
+                  + [W.uniform(41, 2, 300, 50.0, 80.0)], axis=1)
+    for B in (1, 2, 3, 4):
+        P = models(42 + B, B, g.n_cells)
+        with gfm.Plan(g, st, kappa=W.KAPPA_SI, threshold=5.0) as p:
+            Y = p.forward_batched(torch.from_numpy(P).to(dev)).cpu().numpy()
+            singles = [p.forward(torch.from_numpy(P[b]).to(dev)).cpu().numpy() for b in range(B)]
+        for b in range(B):
+            assert relerr(Y[b], singles[b]) < 1e-12, (B, b)
+            want = orc.pm_forward(g, W.KAPPA_SI, P[b].astype(np.float64), st, threshold=5.0)
+            assert relerr(Y[b], want) < 1e-6, (B, b)

@@ -575,6 +575,18 @@ int ztm_plan_forward_batched(ztm_plan plan, int n_models, const float* rho_dev, 
     ZTM_HIP(hipGetDevice(&dev));
     if (dev != plan->device)
         return fail(ZTM_EINVAL, "plan belongs to device %d, current device is %d", plan->device, dev);
+    // Up to 4 models: the multi-field FP32 forward (each pair's r^-3 formed
+    // once, the plan's FP64 near-field correction per model), which costs
+    // less than one 32-model MFMA tile's G generation.
+    NearArgs na = near_args(plan);
+    if (n_models <= 4 && pm_fwd_fields_supported(plan->shape, n_models) && !na.quad) {
+        if (int rc = ensure_workspace(plan, pm_fwd_fields_workspace(plan->shape, n_models)))
+            return rc;
+        return pm_fwd_fields_launch(plan->shape, n_models, (const float4*)plan->st_f, rho_dev,
+                                    plan->shape.n_cells, plan->sy, plan->sz,
+                                    plan->prob.threshold, pm_scale(plan), plan->part, g_dev, na,
+                                    (hipStream_t)stream);
+    }
     const int ks = plan->tuning.batched_slices > 0
                        ? plan->tuning.batched_slices
                        : batched_default_slices(plan->shape, n_models);
@@ -926,7 +938,7 @@ int ztm_tfm_gravity_jvp(int bfr, int di, int dj, int dk, int fh, int fv, const d
     if (pm_fwd_fields_supported(p->shape, 3) && !na.quad) {
         if (int rc = ensure_workspace(p, pm_fwd_fields_workspace(p->shape, 3))) return rc;
         if (int e = pm_fwd_fields_launch(p->shape, 3, (const float4*)p->st_f, ddrho_dev, N, p->sy,
-                                         p->sz, pm_scale(p), p->part, Y, na, s))
+                                         p->sz, 0.0, pm_scale(p), p->part, Y, na, s))
             return e;
     } else {
         for (int f = 0; f < 3; ++f)

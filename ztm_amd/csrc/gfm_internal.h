@@ -278,13 +278,13 @@ int stored_adj_launch(const PmShape& s, const float* G, const float* r, double* 
 
 // F <= 4 FP32 density fields (field f at rho + f * field_stride) through one
 // point-mass forward that forms every pair's r^-3 once (gfm_pointmass.hip);
-// g[f * m + i], each field with its FP64 near-field correction (thresholds
-// from near.thr; the main pass applies none). part must hold
-// pm_fwd_fields_workspace(s, F) doubles.
+// g[f * m + i], each field with its FP64 near-field correction. |rho| <= thr
+// is skipped by the main pass, |rho| <= near.thr by the correction. part must
+// hold pm_fwd_fields_workspace(s, F) doubles.
 bool pm_fwd_fields_supported(const PmShape& s, int F);
 size_t pm_fwd_fields_workspace(const PmShape& s, int F);
 int pm_fwd_fields_launch(const PmShape& s, int F, const float4* st, const float* rho,
-                         int64_t field_stride, double sy, double sz, double scale,
+                         int64_t field_stride, double sy, double sz, double thr, double scale,
                          double* part, double* g, const NearArgs& near, hipStream_t stream);
 
 // Batched split-BF16 MFMA forward (gfm_batched.hip): P float [B][N] -> Y

@@ -270,10 +270,10 @@ __device__ __forceinline__ void pm_fwd_block(
 template <int F>
 __global__ __launch_bounds__(256) void pm_fwd_fields_kernel(
     const float4* __restrict__ st, const float* __restrict__ rho, int64_t field_stride, int nx,
-    int ny, int n_rows, float sy, float sz, int rows_per_slice, double* __restrict__ part,
-    int m_pad) {
+    int ny, int n_rows, float sy, float sz, float thr, int rows_per_slice,
+    double* __restrict__ part, int m_pad) {
     extern __shared__ __align__(16) unsigned char smem[];
-    pm_fwd_block<float, 2, false, F>(st, rho, nx, ny, n_rows, sy, sz, 0.f, rows_per_slice, part,
+    pm_fwd_block<float, 2, false, F>(st, rho, nx, ny, n_rows, sy, sz, thr, rows_per_slice, part,
                                      m_pad, QuadArgs{}, blockIdx.x, blockIdx.y, smem,
                                      field_stride);
 }
@@ -1033,7 +1033,7 @@ size_t pm_fwd_fields_workspace(const PmShape& s, int F) {
 }
 
 int pm_fwd_fields_launch(const PmShape& s, int F, const float4* st, const float* rho,
-                         int64_t field_stride, double sy, double sz, double scale,
+                         int64_t field_stride, double sy, double sz, double thr, double scale,
                          double* part, double* g, const NearArgs& near, hipStream_t stream) {
     if (!pm_fwd_fields_supported(s, F))
         return fail(ZTM_EINVAL, "%d fields of rows of %d cells exceed the LDS budget", F, s.nx);
@@ -1051,8 +1051,8 @@ int pm_fwd_fields_launch(const PmShape& s, int F, const float4* st, const float*
         ProfScope prof("pm_fwd_fields", stream);
 #define ZTM_FIELDS(FF)                                                                          \
     hipLaunchKernelGGL(pm_fwd_fields_kernel<FF>, grid, dim3(256), lds, stream, st, rho,         \
-                       field_stride, s.nx, s.ny, (int)s.n_rows, (float)sy, (float)sz, rps,       \
-                       part, s.m_pad)
+                       field_stride, s.nx, s.ny, (int)s.n_rows, (float)sy, (float)sz,            \
+                       (float)thr, rps, part, s.m_pad)
         switch (F) {
             case 1: ZTM_FIELDS(1); break;
             case 2: ZTM_FIELDS(2); break;
