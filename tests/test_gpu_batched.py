@@ -206,6 +206,11 @@ def test_batched_adjoint_rejects_and_empty(dev):
         assert out.shape == (0, w.grid.n_cells)
         X = p.adjoint_batched(torch.zeros(3, w.n_stations, device=dev)).cpu().numpy()
         assert not np.any(X)
+    # no stations: G^T R is zero (the output is cleared, not left as it was)
+    with gfm.Plan(w.grid, np.zeros((0, 3)), kappa=w.kappa) as p:
+        out = torch.full((5, w.grid.n_cells), 7.0, device=dev)
+        p.adjoint_batched(torch.zeros(5, 0, device=dev), out=out)
+        assert not torch.any(out).item()
 
 
 def test_batched_forward_adjoint_full_size(dev):

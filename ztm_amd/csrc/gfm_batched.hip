@@ -716,8 +716,9 @@ int batched_fwd_launch(const PmShape& s, int B, int ks, int split, const float* 
 
 namespace {
 __global__ void batched_zero_kernel(float* __restrict__ x, int64_t n) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) x[i] = 0.f;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        x[i] = 0.f;
 }
 }  // namespace
 
@@ -741,8 +742,9 @@ int batched_adj_launch(const PmShape& s, int B, int ks, int split, const float* 
         return fail(ZTM_EINVAL, "batched adjoint supports up to 2^31 - 256 cells");
     if (s.m == 0) {   // no stations: G^T R = 0
         const int64_t n = (int64_t)B * s.n_cells;
-        hipLaunchKernelGGL(batched_zero_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
-                           stream, X, n);
+        const int64_t blocks = std::min<int64_t>((n + 255) / 256, 1 << 20);
+        hipLaunchKernelGGL(batched_zero_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, X,
+                           n);
         ZTM_HIP(hipGetLastError());
         return ZTM_OK;
     }
