@@ -187,7 +187,9 @@ int ztm_plan_forward_batched(ztm_plan plan, int n_models, const float* rho_dev,
  * G). Same split-BF16 scheme and split setting as the batched forward, the
  * same G entries (bit-identical FP32 evaluation); K-slices over stations
  * are chosen from the problem shape. n_vectors <= 256 per call; no FP64
- * near-field pass (same assumption as the batched forward). */
+ * near-field pass (same assumption as the batched forward). Up to 4 vectors,
+ * where the batched forward runs its FP32 pass, instead run
+ * ztm_plan_adjoint (FP32) per vector: the transpose of that forward. */
 int ztm_plan_adjoint_batched(ztm_plan plan, int n_vectors, const float* r_dev,
                              float* out_dev, void* stream);
 /* Batched forward tuning: K-slices (0 = choose from the problem shape) and

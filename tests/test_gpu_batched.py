@@ -259,3 +259,24 @@ def test_small_ensembles_take_the_multi_field_forward(dev, orc):
             assert relerr(Y[b], singles[b]) < 1e-12, (B, b)
             want = orc.pm_forward(g, W.KAPPA_SI, P[b].astype(np.float64), st, threshold=5.0)
             assert relerr(Y[b], want) < 1e-6, (B, b)
+
+
+def test_small_batched_adjoint_is_the_fp32_adjoint(dev):
+    """1-4 residual vectors run the FP32 adjoint per vector (the transpose of
+    the batched forward's FP32 pass for up to 4 models): bitwise equal to
+    ztm_plan_adjoint, and <G P, R> = <P, G^T R> to FP32 rounding."""
+    import torch
+    w = W.pm_cpu_small()
+    for B in (1, 3, 4):
+        R = residuals(50 + B, B, w.n_stations)
+        P = models(60 + B, B, w.grid.n_cells)
+        with gfm.Plan(w.grid, w.stations, kappa=w.kappa) as p:
+            X = p.adjoint_batched(torch.from_numpy(R).to(dev)).cpu().numpy()
+            Y = p.forward_batched(torch.from_numpy(P).to(dev)).cpu().numpy()
+            singles = [p.adjoint(torch.from_numpy(R[b]).to(dev)).cpu().numpy() for b in range(B)]
+        for b in range(B):
+            assert np.array_equal(X[b], singles[b]), (B, b)
+            lhs = float(np.dot(Y[b], R[b].astype(np.float64)))
+            rhs = float(np.dot(P[b].astype(np.float64), X[b].astype(np.float64)))
+            scale = float(np.dot(np.abs(Y[b]), np.abs(R[b].astype(np.float64))))
+            assert abs(lhs - rhs) < 1e-6 * scale, (B, b)

@@ -611,6 +611,17 @@ int ztm_plan_adjoint_batched(ztm_plan plan, int n_vectors, const float* r_dev, f
     ZTM_HIP(hipGetDevice(&dev));
     if (dev != plan->device)
         return fail(ZTM_EINVAL, "plan belongs to device %d, current device is %d", plan->device, dev);
+    // Up to 4 vectors: the FP32 adjoint per vector, the transpose of what the
+    // batched forward runs for up to 4 models (one vector costs less than a
+    // quarter of a 32-vector MFMA tile's G generation).
+    if (n_vectors <= 4 && pm_fwd_fields_supported(plan->shape, n_vectors) &&
+        !near_args(plan).quad) {
+        for (int b = 0; b < n_vectors; ++b)
+            if (int rc = ztm_plan_adjoint(plan, ZTM_FP32, r_dev + (size_t)b * plan->shape.m,
+                                          out_dev + (size_t)b * plan->shape.n_cells, stream))
+                return rc;
+        return ZTM_OK;
+    }
     const int ks = batched_adj_default_slices(plan->shape, n_vectors);
     const int split = plan->tuning.batched_split;
     const size_t need = batched_adj_workspace(plan->shape, n_vectors, ks, split);
