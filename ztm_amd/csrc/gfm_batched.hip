@@ -1,5 +1,7 @@
-// Batched (ensemble) forward on MFMA: Y = G P for P = [B models x N cells],
-// with G (stations x cells) generated on the fly -- never stored.
+// Batched (ensemble) forward and adjoint on MFMA: Y = G P for P = [B models x
+// N cells] and X = G^T R for R = [B residual vectors x M stations], with G
+// (stations x cells) generated on the fly -- never stored. (Up to 4 models
+// or vectors take the FP32 point-mass kernels instead; see gfm_abi.hip.)
 //
 // Split-BF16 scheme: G and P are each split into BF16 parts, x = x0 + x1 (+ x2)
 // with x0 = bf16(x), x1 = bf16(x - x0), x2 = bf16(x - x0 - x1), and the product
@@ -27,6 +29,9 @@
 //                 Bp = B up to 32 x a power of two), so that the B operands
 //                 of one half-wave are consecutive 16-byte slots in LDS;
 //                 part [KS][m_pad][Bp] FP32;  Y [B][M] FP64 (model-major).
+//                 The adjoint packs R the same way over stations, and writes
+//                 X [B][N] FP32 directly (one K-slice) or through part
+//                 [KS][n_pad][Bp] and the reduce.
 #include "gfm_internal.h"
 #include "gfm_pair.h"
 
