@@ -280,3 +280,34 @@ def test_small_batched_adjoint_is_the_fp32_adjoint(dev):
             rhs = float(np.dot(P[b].astype(np.float64), X[b].astype(np.float64)))
             scale = float(np.dot(np.abs(Y[b]), np.abs(R[b].astype(np.float64))))
             assert abs(lhs - rhs) < 1e-6 * scale, (B, b)
+
+
+def test_batched_random_geometries(dev, orc):
+    """Seeded sweep over ragged grids (signed spacings, nx from 3 to 70),
+    stations above the top face, 1-40 models / vectors: the batched forward
+    (MFMA or, for up to 4 models, the multi-field FP32 pass) and the batched
+    adjoint against the oracle on the first and last model / vector."""
+    import torch
+    rng = np.random.default_rng(20190317)
+    for case in range(6):
+        nx, ny, nz = (int(v) for v in rng.integers((3, 2, 2), (71, 40, 12)))
+        d = rng.uniform(20.0, 80.0, 3) * rng.choice((-1.0, 1.0), 3)
+        g = W.Grid(nx, ny, nz, float(rng.uniform(-1e3, 1e3)), float(rng.uniform(-1e3, 1e3)),
+                   0.0, float(d[0]), float(d[1]), float(d[2]))
+        c = g.centres()
+        m = int(rng.integers(1, 300))
+        top = c[:, 2].max() + abs(d[2]) / 2
+        st = np.stack([rng.uniform(c[:, 0].min(), c[:, 0].max(), m),
+                       rng.uniform(c[:, 1].min(), c[:, 1].max(), m),
+                       rng.uniform(top + 5.0, top + 100.0, m)], axis=1)
+        B = int(rng.choice((1, 3, 4, 5, 17, 40)))
+        P = rng.uniform(-300, 300, (B, g.n_cells)).astype(np.float32)
+        R = rng.normal(0.0, 0.5, (B, m)).astype(np.float32)
+        with gfm.Plan(g, st, kappa=W.KAPPA_SI) as p:
+            Y = p.forward_batched(torch.from_numpy(P).to(dev)).cpu().numpy()
+            X = p.adjoint_batched(torch.from_numpy(R).to(dev)).cpu().numpy()
+        for b in sorted({0, B - 1}):
+            want = orc.pm_forward(g, W.KAPPA_SI, P[b].astype(np.float64), st)
+            assert relerr(Y[b], want) < 1e-5, (case, nx, ny, nz, m, B, b)
+            want = orc.pm_adjoint(g, W.KAPPA_SI, st, R[b].astype(np.float64))
+            assert relerr(X[b], want) < 1e-5, (case, nx, ny, nz, m, B, b)
