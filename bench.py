@@ -259,6 +259,8 @@ def summary(line):
                "ensemble_split3": get(line, "batched", "split3", "forward_plus_adjoint",
                                       "speedup_vs_fp64_gpu"),
                "ensemble_split2": get(line, "batched", "split2", "forward_plus_adjoint",
+                                      "speedup_vs_fp64_gpu"),
+               "ensemble_fp16x2": get(line, "batched", "fp16x2", "forward_plus_adjoint",
                                       "speedup_vs_fp64_gpu")},
            "point_mass_vs_cpu_baseline": line.get("speedup_vs_cpu"),
            "max_rel_err_vs_fp64": {
@@ -270,6 +272,10 @@ def summary(line):
                                       default=None),
                "ensemble_split2": max((v for v in (get(line, "batched", "split2", "rel_err_vs_fp64"),
                                                    get(line, "batched", "split2", "adjoint",
+                                                       "rel_err_vs_fp64")) if v is not None),
+                                      default=None),
+               "ensemble_fp16x2": max((v for v in (get(line, "batched", "fp16x2", "rel_err_vs_fp64"),
+                                                   get(line, "batched", "fp16x2", "adjoint",
                                                        "rel_err_vs_fp64")) if v is not None),
                                       default=None)}}
     return out
@@ -324,13 +330,14 @@ def stored_leg(plan, rho32, r32, g64h, stream, args):
     return out
 
 
-PEAK_BF16_TFLOPS = 2500.0     # MI355X dense BF16 MFMA, published
+PEAK_BF16_TFLOPS = 2500.0     # MI355X dense BF16 (and FP16) MFMA, published
 
 
 def batched_leg(stream, args):
     """Ensemble forward and adjoint on split-BF16 MFMA (ensemble_256: 96x96x48
     cells of 1 km, 9,216 stations at z = +10 m, 256 smoothed models, 256
-    residual vectors N(0, 0.5)), both splits, against the FP64 single-model
+    residual vectors N(0, 0.5)), BF16 split 3 and 2 and the FP16 hi/lo split,
+    against the FP64 single-model
     kernels on models / vectors 0, 85, 170, 255 (extrapolated to 256 for the
     FP64 rates)."""
     import torch
@@ -373,8 +380,9 @@ def batched_leg(stream, args):
                                    "unit": "pair-models/s",
                                    "kind": "FP64 single-vector adjoint, timed on 4 vectors"}
         fp64_fa = 2.0 * pm / ((t64 + ta64) / len(check) * B)
-        for split in (3, 2):
+        for split in (3, 2, 4):
             p.set_batched_tuning(split=split)
+            products = 6 if split == 3 else 3
             gfm.profile_reset()
             gfm.profile_enable(True)
             t = _timed(lambda: p.forward_batched(P, out=Y, stream=stream), 3,
@@ -383,7 +391,7 @@ def batched_leg(stream, args):
             ms, n = gfm.profile_read("batched_fwd")
             gfm.profile_reset()
             kms = ms / max(n, 1)
-            mfma_flop = 2.0 * pm * (6 if split == 3 else 3)
+            mfma_flop = 2.0 * pm * products
             Yh = Y.cpu().numpy()
             leg = {"value": pm / t, "ms_per_call": t * 1e3, "kernel_ms": kms,
                    "rel_err_vs_fp64": max(relerr(Yh[b], g64[b]) for b in check),
@@ -391,7 +399,7 @@ def batched_leg(stream, args):
                                 "achieved": mfma_flop / (kms * 1e-3) / 1e12,
                                 "frac": mfma_flop / (kms * 1e-3) / 1e12 / PEAK_BF16_TFLOPS,
                                 "flop": "issued MFMA: 2 x pair-models x products",
-                                "products": 6 if split == 3 else 3}}
+                                "products": products}}
             leg["speedup_vs_fp64_gpu"] = leg["value"] / out["fp64_gpu"]["value"]
             gfm.profile_reset()
             gfm.profile_enable(True)
@@ -412,7 +420,8 @@ def batched_leg(stream, args):
             fa = 2.0 * pm / (t + ta)
             leg["forward_plus_adjoint"] = {"value": fa, "unit": "pair-models/s",
                                            "speedup_vs_fp64_gpu": fa / fp64_fa}
-            out[f"split{split}"] = leg
+            out["fp16x2" if split == 4 else f"split{split}"] = leg
+        p.set_batched_tuning(split=3)
     return out
 
 

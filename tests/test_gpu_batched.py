@@ -311,3 +311,66 @@ def test_batched_random_geometries(dev, orc):
             assert relerr(Y[b], want) < 1e-5, (case, nx, ny, nz, m, B, b)
             want = orc.pm_adjoint(g, W.KAPPA_SI, st, R[b].astype(np.float64))
             assert relerr(X[b], want) < 1e-5, (case, nx, ny, nz, m, B, b)
+
+
+@pytest.mark.parametrize("n,nz", [(48, 24), (64, 12)])
+def test_batched_fp16_split_half_size(dev, n, nz):
+    """Split code 4 (FP16 hi/lo parts under power-of-two scales: G by the
+    plan's bound on |G|, each model / vector by its largest entry): forward
+    and adjoint of 256 models against the FP64 single-model kernels on 4 of
+    them, bitwise repeatability and the dot-product identity."""
+    import torch
+    w = W.ensemble_256(n=n, nz=nz)
+    Pt = torch.from_numpy(w.rho).to(dev)
+    R = residuals(13, 256, w.n_stations)
+    Rt = torch.from_numpy(R).to(dev)
+    with gfm.Plan(w.grid, w.stations, kappa=w.kappa) as p:
+        p.set_batched_tuning(split=4)
+        Y = p.forward_batched(Pt).cpu().numpy()
+        X = p.adjoint_batched(Rt).cpu().numpy()
+        assert np.array_equal(Y, p.forward_batched(Pt).cpu().numpy())
+        assert np.array_equal(X, p.adjoint_batched(Rt).cpu().numpy())
+        for b in (0, 85, 170, 255):
+            g64 = p.forward(torch.from_numpy(w.rho[b].astype(np.float64)).to(dev)).cpu().numpy()
+            a64 = p.adjoint(torch.from_numpy(R[b].astype(np.float64)).to(dev)).cpu().numpy()
+            assert relerr(Y[b], g64) < 5e-6, b
+            assert relerr(X[b], a64) < 5e-6, b
+    lhs = np.einsum("bm,bm->b", Y, R.astype(np.float64))
+    rhs = np.einsum("bn,bn->b", w.rho.astype(np.float64), X.astype(np.float64))
+    scale = np.einsum("bm,bm->b", np.abs(Y), np.abs(R.astype(np.float64)))
+    assert np.max(np.abs(lhs - rhs) / scale) < 1e-5
+
+
+def test_batched_fp16_split_random_geometries(dev, orc):
+    """Split code 4 over seeded ragged grids whose stations range from a
+    hundredth of a cell above the top face (a large bound on |G|) to far
+    above it (a small one), models of very different magnitudes and a zero
+    model: forward and adjoint against the oracle."""
+    import torch
+    rng = np.random.default_rng(6021023)
+    for case in range(6):
+        nx, ny, nz = (int(v) for v in rng.integers((3, 2, 2), (71, 40, 12)))
+        d = rng.uniform(20.0, 80.0, 3) * rng.choice((-1.0, 1.0), 3)
+        g = W.Grid(nx, ny, nz, 0.0, 0.0, 0.0, float(d[0]), float(d[1]), float(d[2]))
+        c = g.centres()
+        m = int(rng.integers(5, 300))
+        top = c[:, 2].max() + abs(d[2]) / 2
+        lift = abs(d[2]) * (0.01 if case % 2 == 0 else 20.0)
+        st = np.stack([rng.uniform(c[:, 0].min(), c[:, 0].max(), m),
+                       rng.uniform(c[:, 1].min(), c[:, 1].max(), m),
+                       rng.uniform(top + lift, top + 2 * lift, m)], axis=1)
+        B = int(rng.choice((9, 33, 64)))
+        mag = 10.0 ** rng.uniform(-6, 6, (B, 1))
+        P = (rng.uniform(-1, 1, (B, g.n_cells)) * mag).astype(np.float32)
+        P[B // 2] = 0.0
+        R = (rng.normal(0.0, 1.0, (B, m)) * mag).astype(np.float32)
+        with gfm.Plan(g, st, kappa=W.KAPPA_SI) as p:
+            p.set_batched_tuning(split=4)
+            Y = p.forward_batched(torch.from_numpy(P).to(dev)).cpu().numpy()
+            X = p.adjoint_batched(torch.from_numpy(R).to(dev)).cpu().numpy()
+        assert not Y[B // 2].any()
+        for b in sorted({0, B - 1}):
+            want = orc.pm_forward(g, W.KAPPA_SI, P[b].astype(np.float64), st)
+            assert relerr(Y[b], want) < 1e-5, (case, nx, ny, nz, m, B, b)
+            want = orc.pm_adjoint(g, W.KAPPA_SI, st, R[b].astype(np.float64))
+            assert relerr(X[b], want) < 1e-5, (case, nx, ny, nz, m, B, b)

@@ -297,6 +297,38 @@ const char* ztm_last_error(void) { return g_last_error.c_str(); }
 
 int ztm_init(void) { return check_device(nullptr); }
 
+namespace {
+// Largest |dz| r^-3 over all station-cell pairs, in the kernels' units and
+// from the FP32 station records they read. For one station the distance h
+// to the nearest cell column (nearest index per axis) is the same in every
+// layer, and |dz| (h^2 + dz^2)^-1.5 rises then falls in |dz| on either side
+// of the station, peaking at |dz| = h / sqrt(2); so the largest value lies
+// within a layer of Z -+ h / sqrt(2), and two layers either side of each are
+// tried. 0 without stations or cells.
+double g_bound(const std::vector<float4>& hf, int64_t n, int nx, int ny, int nz, double sy,
+               double sz) {
+    double best = 0.0;
+    if (nx <= 0 || ny <= 0 || nz <= 0) return 0.0;
+    auto nearest = [](double v, int cnt) {
+        return std::max(0.0, std::min((double)(cnt - 1), std::nearbyint(v)));
+    };
+    for (int64_t m = 0; m < n; ++m) {
+        const double X = hf[m].x, Y = hf[m].y, Z = hf[m].z;
+        const double dx = X - nearest(X, nx), dy = Y - sy * nearest(Y / sy, ny);
+        const double h2 = dx * dx + dy * dy, t = std::sqrt(h2 * 0.5);
+        for (int side = -1; side <= 1; side += 2) {
+            const double k0 = std::nearbyint((Z + side * t) / sz);
+            for (int d = -2; d <= 2; ++d) {
+                const double k = std::max(0.0, std::min((double)(nz - 1), k0 + d));
+                const double dz = Z - k * sz, r2 = h2 + dz * dz;
+                if (r2 > 0.0) best = std::max(best, std::fabs(dz) / (r2 * std::sqrt(r2)));
+            }
+        }
+    }
+    return best;
+}
+}  // namespace
+
 int ztm_plan_create(ztm_plan* plan, const ztm_problem* problem, int64_t n_stations,
                     const double* stations_host) {
     if (!plan || !problem) return fail(ZTM_EINVAL, "null plan or problem");
@@ -339,6 +371,7 @@ int ztm_plan_create(ztm_plan* plan, const ztm_problem* problem, int64_t n_statio
         };
         hc[m] = make_int4(idx(X, g.nx), idx(Y / p->sy, g.ny), idx(Z / p->sz, g.nz), 0);
     }
+    p->gmax = g_bound(hf, n_stations, g.nx, g.ny, g.nz, p->sy, p->sz);
     // Adjoint gather indexes (stable counting sorts by key):
     //  columns: stations by nearest (ci, cj) over the (cmx, cmy)-extended range;
     //  rows:    stations that may clamp some grid row, by nearest (cj, ck).
@@ -594,8 +627,8 @@ int ztm_plan_forward_batched(ztm_plan plan, int n_models, const float* rho_dev, 
     const size_t need = batched_workspace(plan->shape, n_models, ks, split);
     if (int rc = ensure_batched_workspace(plan, need)) return rc;
     return batched_fwd_launch(plan->shape, n_models, ks, split, (const float*)plan->st_f, rho_dev,
-                              plan->prob.threshold, plan->sy, plan->sz, pm_scale(plan), plan->bws,
-                              g_dev, (hipStream_t)stream);
+                              plan->prob.threshold, plan->sy, plan->sz, pm_scale(plan),
+                              plan->gmax, plan->bws, g_dev, (hipStream_t)stream);
 }
 
 int ztm_plan_adjoint_batched(ztm_plan plan, int n_vectors, const float* r_dev, float* out_dev,
@@ -627,14 +660,17 @@ int ztm_plan_adjoint_batched(ztm_plan plan, int n_vectors, const float* r_dev, f
     const size_t need = batched_adj_workspace(plan->shape, n_vectors, ks, split);
     if (int rc = ensure_batched_workspace(plan, need)) return rc;
     return batched_adj_launch(plan->shape, n_vectors, ks, split, (const float*)plan->st_f, r_dev,
-                              plan->sy, plan->sz, pm_scale(plan), plan->bws, out_dev,
+                              plan->sy, plan->sz, pm_scale(plan), plan->gmax, plan->bws, out_dev,
                               (hipStream_t)stream);
 }
 
 int ztm_plan_set_batched_tuning(ztm_plan plan, int slices, int split) {
     if (!plan) return fail(ZTM_EINVAL, "null plan");
     if (slices < 0) return fail(ZTM_EINVAL, "negative slice count");
-    if (split != 0 && split != 2 && split != 3) return fail(ZTM_EINVAL, "split must be 2 or 3");
+    if (split != 0 && split != 2 && split != 3 && split != 4)
+        return fail(ZTM_EINVAL, "split must be 2, 3 or 4");
+    if (split == 4 && !(plan->gmax > 0.0))
+        return fail(ZTM_EINVAL, "the FP16 split (4) needs a plan made by ztm_plan_create");
     plan->tuning.batched_slices = slices;
     if (split) plan->tuning.batched_split = split;
     return ZTM_OK;
@@ -874,7 +910,7 @@ int tfm_gravity_of_fields(ztm_plan p, int bfr, int di, int dj, int fh, int fv, i
     if (int rc = ensure_batched_workspace(p, need)) return rc;
     double* Y = (double*)((char*)p->bws + ws);
     if (int e = batched_fwd_launch(p->shape, B, ks, split, (const float*)p->st_f, drho_dev, thr,
-                                   p->sy, p->sz, pm_scale(p), p->bws, Y, s))
+                                   p->sy, p->sz, pm_scale(p), 0.0, p->bws, Y, s))
         return e;
     return tfm_scatter_launch(Y, p->out_map, m, M, B, g_dev, s);
 }

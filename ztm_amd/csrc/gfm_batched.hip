@@ -36,12 +36,66 @@
 #include "gfm_pair.h"
 
 #include <algorithm>
+#include <cmath>
 #include <type_traits>
 
 namespace ztm {
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+// Element type of the split parts: BF16 (8 significant bits, the FP32 range)
+// or FP16 (11 significant bits below 65504; used with power-of-two scales
+// that bring every operand's largest magnitude to at most 2^kHalfTop).
+template <typename H> struct Half;
+template <> struct Half<__bf16> {
+    using v8 = bf16x8;
+    static constexpr bool scaled = false;
+    static __device__ __forceinline__ f32x16 mfma(v8 a, v8 b, f32x16 c) {
+        return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
+    }
+};
+template <> struct Half<_Float16> {
+    using v8 = f16x8;
+    static constexpr bool scaled = true;
+    static __device__ __forceinline__ f32x16 mfma(v8 a, v8 b, f32x16 c) {
+        return __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, c, 0, 0, 0);
+    }
+};
+constexpr int kHalfTop = 14;
+
+// Power-of-two factor that brings a largest magnitude a (given by its float
+// bits) to at most 2^kHalfTop; 1 for a = 0.
+__device__ __forceinline__ float half_scale(unsigned abits) {
+    const float a = __uint_as_float(abits);
+    if (!(a > 0.f)) return 1.f;
+    int e;
+    frexpf(a, &e);   // a <= 2^e
+    return ldexpf(1.f, kHalfTop - e);
+}
+
+// amax[b] |= bits of max over n of |X[b][n]| among entries with |X| > thr
+// (non-negative floats order like their bits); amax must start at 0.
+// grid = (blocks, B); block = 256.
+__global__ __launch_bounds__(256) void batched_absmax_kernel(const float* __restrict__ X,
+                                                             int64_t n, float thr,
+                                                             unsigned* __restrict__ amax) {
+    const int b = blockIdx.y;
+    float m = 0.f;
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+        const float v = fabsf(X[(size_t)b * n + i]);
+        if (v > thr) m = fmaxf(m, v);
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
+    if ((threadIdx.x & 63) == 0) atomicMax(&amax[b], __float_as_uint(m));
+}
+
+__global__ void batched_clear_kernel(unsigned* __restrict__ x, int n) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) x[i] = 0u;
+}
 
 constexpr int kBatchChunk = 16;   // cells per MFMA K step
 // s_waitcnt immediate (gfx9 encoding): vmcnt(0), expcnt and lgkmcnt left at
@@ -56,16 +110,19 @@ static_assert(kStationPad % 16 == 0, "station padding must hold whole chunks");
 // 256-byte row reads into LDS, then each thread converts the 8 cells of one
 // (chunk, half, model) slot and writes them as one 16-byte store per part,
 // contiguous across the models of a slot. grid = (ceil(n_chunks / 4), Bp / 32).
-// Element v (times mul) is split as h_1 = bf16(v), h_2 = bf16(v - h_1), ...;
-// entries outside the model or cell range, and |v| <= thr, are 0.
+// Element v (times mul, and times the model's half_scale(amax[model]) when
+// amax is given) is split as h_1 = H(v), h_2 = H(v - h_1), ...; entries
+// outside the model or cell range, and |v| <= thr, are 0.
 constexpr int kPackCells = 64;
 
-template <int SPLIT>
+template <int SPLIT, typename H>
 __global__ __launch_bounds__(256) void batched_pack_kernel(const float* __restrict__ P, int B,
                                                            int64_t n_cells, int Bp,
                                                            int64_t n_chunks, int64_t n_packed,
                                                            float thr, float mul,
-                                                           __bf16* __restrict__ parts) {
+                                                           const unsigned* __restrict__ amax,
+                                                           H* __restrict__ parts) {
+    using v8 = typename Half<H>::v8;
     __shared__ float tile[32][kPackCells + 1];
     const int t = threadIdx.x;
     const int64_t c0 = (int64_t)blockIdx.x * kPackCells;
@@ -79,7 +136,8 @@ __global__ __launch_bounds__(256) void batched_pack_kernel(const float* __restri
         float v = 0.f;
         if (model < B && cell < n_cells) {
             v = P[(size_t)model * n_cells + cell];
-            v = fabsf(v) > thr ? v * mul : 0.f;
+            const float f = amax ? mul * half_scale(amax[model]) : mul;
+            v = fabsf(v) > thr ? v * f : 0.f;
         }
         tile[row][col] = v;
     }
@@ -88,13 +146,13 @@ __global__ __launch_bounds__(256) void batched_pack_kernel(const float* __restri
     const int mrow = t & 31;
     const int64_t chunk = c0 / kBatchChunk + (slot >> 1);
     if (chunk >= n_chunks) return;
-    bf16x8 h[SPLIT];
+    v8 h[SPLIT];
 #pragma unroll
     for (int e = 0; e < 8; ++e) {
         float v = tile[mrow][slot * 8 + e];
 #pragma unroll
         for (int q = 0; q < SPLIT; ++q) {
-            const __bf16 b = (__bf16)v;
+            const H b = (H)v;
             h[q][e] = b;
             v -= (float)b;
         }
@@ -102,7 +160,7 @@ __global__ __launch_bounds__(256) void batched_pack_kernel(const float* __restri
     const int64_t off = ((chunk * 2 + (slot & 1)) * Bp + m0 + mrow) * 8;
 #pragma unroll
     for (int q = 0; q < SPLIT; ++q)
-        *reinterpret_cast<bf16x8*>(parts + q * n_packed + off) = h[q];
+        *reinterpret_cast<v8*>(parts + q * n_packed + off) = h[q];
 }
 
 // --------------------------------------------------------------- forward
@@ -116,6 +174,22 @@ __device__ __forceinline__ void divmod_pos(int n, int d, double inv, int& q, int
     r += hi ? -d : (lo ? d : 0);
 }
 
+// One G value into element e of the SPLIT parts. FP16 parts take the value
+// times the power-of-two G scale gs, held to +-2^15: cells past the grid and
+// the padding station records, whose |G| the scale does not bound, stay
+// finite, so that their products with the zero-padded operand stay 0.
+template <int SPLIT, typename H>
+__device__ __forceinline__ void store_split(float gv, float gs,
+                                            typename Half<H>::v8 (&a)[SPLIT], int e) {
+    if constexpr (Half<H>::scaled) gv = __builtin_amdgcn_fmed3f(gv * gs, -32768.f, 32768.f);
+#pragma unroll
+    for (int q = 0; q < SPLIT; ++q) {
+        const H h = (H)gv;
+        a[q][e] = h;
+        gv -= (float)h;
+    }
+}
+
 // This lane's 8 G values of a chunk (cells cell0 .. cell0 + 7, raster order),
 // split into SPLIT BF16 parts in the MFMA A-operand layout: begin() forms
 // the chunk's shared terms, elem<E>() the value of cell cell0 + E, so that
@@ -126,8 +200,10 @@ __device__ __forceinline__ void divmod_pos(int n, int d, double inv, int& q, int
 // grids it gives the same values as MODE 1. MODE 0: cell-by-cell stepping
 // (elements in increasing E). Cells past the grid give finite values
 // (r^2 >= 1e-20, so |dz r^-3| <= 1e36), which the zero-padded P cancels.
-template <int SPLIT, int MODE>
+template <int SPLIT, int MODE, typename H = __bf16>
 struct BatchedGen {
+    using v8 = typename Half<H>::v8;
+    float gs = 1.f;                    // G scale (FP16 parts)
     float xa, R0, dz0, R1, dz1, nxf;   // MODE 1, 2
     int ew;                            // MODE 1: cells e >= ew lie on the next row
     float px, py, pz, sy, sz;          // MODE 0
@@ -174,7 +250,7 @@ struct BatchedGen {
     }
 
     template <int E>
-    __device__ __forceinline__ void elem(bf16x8 (&a)[SPLIT]) {
+    __device__ __forceinline__ void elem(v8 (&a)[SPLIT]) {
         float gv;
         if constexpr (MODE == 2) {
             const float x = xa - (float)E;
@@ -199,12 +275,7 @@ struct BatchedGen {
             j = wj ? 0 : j;
             k += wj ? 1 : 0;
         }
-#pragma unroll
-        for (int q = 0; q < SPLIT; ++q) {
-            const __bf16 h = (__bf16)gv;
-            a[q][E] = h;
-            gv -= (float)h;
-        }
+        store_split<SPLIT, H>(gv, gs, a, E);
     }
 };
 
@@ -216,11 +287,12 @@ __device__ __forceinline__ void static_for(F&& f) {
     }
 }
 
-template <int SPLIT, int MODE>
+template <int SPLIT, int MODE, typename H>
 __device__ __forceinline__ void batched_gen(const float4 p, int cell0, int nx, int ny,
                                             double inv_nx, double inv_ny, float sy, float sz,
-                                            bf16x8 (&a)[SPLIT]) {
-    BatchedGen<SPLIT, MODE> g;
+                                            float gs, typename Half<H>::v8 (&a)[SPLIT]) {
+    BatchedGen<SPLIT, MODE, H> g;
+    g.gs = gs;
     g.begin(p, cell0, nx, ny, inv_nx, inv_ny, sy, sz);
     static_for<0, 8>([&](auto ec) { g.template elem<decltype(ec)::value>(a); });
 }
@@ -244,15 +316,17 @@ __device__ __forceinline__ void gen_after_mfma(int m, F&& elem) {
 // c&1), the next chunk's P parts load from global memory straight into the
 // other LDS buffer and its G block is generated. The last iteration repeats
 // its own chunk's loads into the buffer that is not read again.
-template <int NT, int SPLIT, int MODE>   // model tiles of 32 (B <= 32 * NT); 2- or 3-way split; batched_gen mode
+// H: part type (BF16, or FP16 with the G scale gs applied to G).
+template <int NT, int SPLIT, int MODE, typename H>   // model tiles of 32 (B <= 32 * NT); 2- or 3-way split; batched_gen mode
 __global__ __launch_bounds__(256) void batched_fwd_kernel(
-    const float4* __restrict__ st, const bf16x8* __restrict__ parts, int64_t part_stride,
-    int nx, int ny, double inv_nx, double inv_ny, float sy, float sz, int n_chunks,
-    int chunks_per_slice, int m_pad, float* __restrict__ part) {
+    const float4* __restrict__ st, const typename Half<H>::v8* __restrict__ parts,
+    int64_t part_stride, int nx, int ny, double inv_nx, double inv_ny, float sy, float sz,
+    float gs, int n_chunks, int chunks_per_slice, int m_pad, float* __restrict__ part) {
+    using v8 = typename Half<H>::v8;
     constexpr int Bp = 32 * NT;
-    constexpr int V = Bp * 2;                          // bf16x8 per part per chunk
+    constexpr int V = Bp * 2;                          // 16-byte slots per part per chunk
     constexpr int LOADS = (V * SPLIT + 255) / 256;     // per thread per chunk
-    __shared__ bf16x8 lp[2][SPLIT][V];
+    __shared__ v8 lp[2][SPLIT][V];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int row = lane & 31, kh = lane >> 5;
     const int mbase = blockIdx.x * kBatchTile + wave * 32;
@@ -282,10 +356,11 @@ __global__ __launch_bounds__(256) void batched_fwd_kernel(
                     16, 0, 0);
         }
     };
-    bf16x8 a[SPLIT];
+    v8 a[SPLIT];
     if (c0 < c1) {
         fetch(c0, 0);
-        batched_gen<SPLIT, MODE>(p, c0 * kBatchChunk + 8 * kh, nx, ny, inv_nx, inv_ny, sy, sz, a);
+        batched_gen<SPLIT, MODE, H>(p, c0 * kBatchChunk + 8 * kh, nx, ny, inv_nx, inv_ny, sy, sz,
+                                    gs, a);
     }
     __builtin_amdgcn_s_waitcnt(kWaitVm0);   // this wave's LDS loads have landed
     __syncthreads();
@@ -293,10 +368,11 @@ __global__ __launch_bounds__(256) void batched_fwd_kernel(
         const int buf = (c - c0) & 1;
         const int cn = min(c + 1, c1 - 1);
         fetch(cn, buf ^ 1);   // buffer buf ^ 1 was last read before the previous barrier
-        bf16x8 cur[SPLIT];
+        v8 cur[SPLIT];
 #pragma unroll
         for (int q = 0; q < SPLIT; ++q) cur[q] = a[q];
-        BatchedGen<SPLIT, MODE> g;
+        BatchedGen<SPLIT, MODE, H> g;
+        g.gs = gs;
         g.begin(p, cn * kBatchChunk + 8 * kh, nx, ny, inv_nx, inv_ny, sy, sz);
         // Tiles in pairs so that consecutive MFMAs update different
         // accumulators (no back-to-back dependence on one accumulator).
@@ -306,8 +382,8 @@ __global__ __launch_bounds__(256) void batched_fwd_kernel(
         constexpr int TP = NT >= 2 ? 2 : 1;
         constexpr int kPerPair = TP * SPLIT * (SPLIT + 1) / 2;   // MFMAs per tile pair
         constexpr int TOTAL = NT / TP * kPerPair;
-        bf16x8 b[2][TP][SPLIT];
-        auto load_b = [&](int t, bf16x8 (&bb)[TP][SPLIT]) {
+        v8 b[2][TP][SPLIT];
+        auto load_b = [&](int t, v8 (&bb)[TP][SPLIT]) {
 #pragma unroll
             for (int u = 0; u < TP; ++u)
 #pragma unroll
@@ -325,8 +401,7 @@ __global__ __launch_bounds__(256) void batched_fwd_kernel(
                 for (int qa = 0; qa <= s2; ++qa)
 #pragma unroll
                     for (int u = 0; u < TP; ++u) {
-                        acc[t + u] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
-                            cur[qa], b[cb][u][s2 - qa], acc[t + u], 0, 0, 0);
+                        acc[t + u] = Half<H>::mfma(cur[qa], b[cb][u][s2 - qa], acc[t + u]);
                         const int m = t / TP * kPerPair + (s2 * (s2 + 1) / 2 + qa) * TP + u;
                         gen_after_mfma<TOTAL>(m, [&](auto ec) {
                             g.pin();
@@ -387,16 +462,17 @@ __device__ __forceinline__ float row_bcast(float v) {   // lane E of this lane's
         float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), 0x150 + E, 0xF, 0xF, false));
 }
 
-template <int NT, int SPLIT, int ROW>   // model tiles of 32 (B <= 32 * NT); 2- or 3-way split
+template <int NT, int SPLIT, int ROW, typename H>   // model tiles of 32 (B <= 32 * NT); 2- or 3-way split; part type
 __global__ __launch_bounds__(256) void batched_adj_kernel(
-    const float4* __restrict__ st, const bf16x8* __restrict__ parts, int64_t part_stride,
-    int nx, int ny, double inv_nx, double inv_ny, float sy, float sz, int n_chunks,
-    int chunks_per_slice, int n_pad, float* __restrict__ part, float* __restrict__ X,
-    int n_cells, int B, int vec4) {
+    const float4* __restrict__ st, const typename Half<H>::v8* __restrict__ parts,
+    int64_t part_stride, int nx, int ny, double inv_nx, double inv_ny, float sy, float sz,
+    float gs, int n_chunks, int chunks_per_slice, int n_pad, float* __restrict__ part,
+    float* __restrict__ X, int n_cells, int B, int vec4) {
+    using v8 = typename Half<H>::v8;
     constexpr int Bp = 32 * NT;
-    constexpr int V = Bp * 2;                          // bf16x8 per part per chunk
+    constexpr int V = Bp * 2;                          // 16-byte slots per part per chunk
     constexpr int LOADS = (V * SPLIT + 255) / 256;     // per thread per chunk
-    __shared__ bf16x8 lp[2][SPLIT][V];
+    __shared__ v8 lp[2][SPLIT][V];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int row = lane & 31, kh = lane >> 5;
     const int nbase = blockIdx.x * kBatchTile + wave * 32;
@@ -435,7 +511,7 @@ __global__ __launch_bounds__(256) void batched_adj_kernel(
         const float* r = &st[(int64_t)chunk * kBatchChunk + 8 * kh + (lane & 7)].x;
         return make_float3(r[0], r[1], r[2]);
     };
-    auto gen = [&](float3 p, bf16x8 (&a)[SPLIT]) {
+    auto gen = [&](float3 p, v8 (&a)[SPLIT]) {
         if constexpr (ROW) {   // (x, R0, dz) of this lane's station for the wave's row
             const float dy = p.y - cy;
             const float dz = p.z - cz;
@@ -455,16 +531,10 @@ __global__ __launch_bounds__(256) void batched_adj_kernel(
                 r2 = fmaf(x, x, fmaf(dy, dy, fmaf(dz, dz, 1e-20f)));
             }
             const float v = rsq_f(r2);
-            float gv = dz * (v * v * v);
-#pragma unroll
-            for (int q = 0; q < SPLIT; ++q) {
-                const __bf16 h = (__bf16)gv;
-                a[q][e] = h;
-                gv -= (float)h;
-            }
+            store_split<SPLIT, H>(dz * (v * v * v), gs, a, e);
         });
     };
-    bf16x8 a[SPLIT];
+    v8 a[SPLIT];
     float3 rec = make_float3(0.f, 0.f, 0.f);
     if (c0 < c1) {
         fetch(c0, 0);
@@ -478,13 +548,13 @@ __global__ __launch_bounds__(256) void batched_adj_kernel(
         fetch(min(c + 1, c1 - 1), buf ^ 1);
         const float3 now = rec;                 // chunk c + 1
         rec = record(min(c + 2, c1 - 1));
-        bf16x8 cur[SPLIT];
+        v8 cur[SPLIT];
 #pragma unroll
         for (int q = 0; q < SPLIT; ++q) cur[q] = a[q];
         gen(now, a);
         constexpr int TP = NT >= 2 ? 2 : 1;
-        bf16x8 b[2][TP][SPLIT];
-        auto load_b = [&](int t, bf16x8 (&bb)[TP][SPLIT]) {
+        v8 b[2][TP][SPLIT];
+        auto load_b = [&](int t, v8 (&bb)[TP][SPLIT]) {
 #pragma unroll
             for (int u = 0; u < TP; ++u)
 #pragma unroll
@@ -501,8 +571,7 @@ __global__ __launch_bounds__(256) void batched_adj_kernel(
                 for (int qa = 0; qa <= s2; ++qa)
 #pragma unroll
                     for (int u = 0; u < TP; ++u)
-                        acc[t + u] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
-                            cur[qa], b[cb][u][s2 - qa], acc[t + u], 0, 0, 0);
+                        acc[t + u] = Half<H>::mfma(cur[qa], b[cb][u][s2 - qa], acc[t + u]);
         }
         // Issue-order hints: global loads, the first tile pair's LDS reads,
         // then per tile pair the next pair's reads and its MFMAs, each
@@ -562,11 +631,14 @@ __global__ __launch_bounds__(256) void batched_adj_kernel(
 // (models) tile, one entry per thread (reads coalesced along the models), and
 // writes it transposed through LDS. The adjoint uses it with cells in place
 // of stations and a float output.
+// With FP16 parts (amax given) the sums carry the G scale gs and model b's
+// operand scale half_scale(amax[b]), which the FP64 scale takes out.
 template <typename TO>
 __global__ __launch_bounds__(256) void batched_reduce_kernel(const float* __restrict__ part,
                                                              int ks, int m, int m_pad, int B,
                                                              int Bp, double scale,
-                                                             TO* __restrict__ Y) {
+                                                             const unsigned* __restrict__ amax,
+                                                             double gs, TO* __restrict__ Y) {
     __shared__ double tile[8][33];
     const int mb = blockIdx.x * 8, bb = blockIdx.y * 32;
     const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;   // 32 x 8
@@ -585,7 +657,10 @@ __global__ __launch_bounds__(256) void batched_reduce_kernel(const float* __rest
     // 32 models x 8 stations out: thread (b = t / 8, station t % 8)
     const int t = threadIdx.x;
     const int b = bb + (t >> 3), mm = mb + (t & 7);
-    if (b < B && mm < m) Y[(size_t)b * m + mm] = (TO)(scale * tile[t & 7][t >> 3]);
+    if (b < B && mm < m) {
+        const double f = amax ? scale / (gs * (double)half_scale(amax[b])) : scale;
+        Y[(size_t)b * m + mm] = (TO)(f * tile[t & 7][t >> 3]);
+    }
 }
 
 // --------------------------------------------------------------- launchers
@@ -599,12 +674,34 @@ int batched_nt(int B) {
 }
 }  // namespace
 
+namespace {
+// Split codes: 2 and 3 = BF16 parts, 4 = two FP16 parts.
+int split_parts(int split) { return split == 4 ? 2 : split; }
+constexpr size_t kAmaxBytes = 256 * sizeof(unsigned);   // per-model absmax (FP16)
+
+// Power-of-two G scale of the FP16 parts: brings gmax (a bound on |G|) to at
+// most 2^kHalfTop.
+float g_scale(double gmax) {
+    int e;
+    std::frexp(gmax, &e);   // gmax < 2^e
+    return (float)std::ldexp(1.0, std::max(-120, std::min(120, kHalfTop - e)));
+}
+
+int check_split(int split, double gmax) {
+    if (split != 2 && split != 3 && split != 4)
+        return fail(ZTM_EINVAL, "split must be 2, 3 or 4 (got %d)", split);
+    if (split == 4 && !(gmax > 0.0 && std::isfinite(gmax)))
+        return fail(ZTM_EINVAL, "the FP16 split needs the plan's bound on |G| (none for this plan)");
+    return ZTM_OK;
+}
+}  // namespace
+
 size_t batched_workspace(const PmShape& s, int B, int ks, int split) {
     const int Bp = 32 * batched_nt(B);
     const int64_t n16 = (s.n_cells + kBatchChunk - 1) / kBatchChunk * kBatchChunk;
-    const size_t packed = (size_t)n16 * Bp * split * sizeof(__bf16);
+    const size_t packed = (size_t)n16 * Bp * split_parts(split) * sizeof(__bf16);
     const size_t partial = (size_t)ks * s.m_pad * Bp * sizeof(float);
-    return packed + partial;
+    return packed + partial + kAmaxBytes;
 }
 
 namespace {
@@ -654,10 +751,10 @@ int batched_adj_default_slices(const PmShape& s, int B) {
 }
 
 int batched_fwd_launch(const PmShape& s, int B, int ks, int split, const float* st,
-                       const float* P, double thr, double sy, double sz, double scale, void* ws,
-                       double* Y, hipStream_t stream) {
+                       const float* P, double thr, double sy, double sz, double scale,
+                       double gmax, void* ws, double* Y, hipStream_t stream) {
     if (B <= 0 || s.m == 0) return ZTM_OK;
-    if (split != 2 && split != 3) return fail(ZTM_EINVAL, "split must be 2 or 3 (got %d)", split);
+    if (int rc = check_split(split, gmax)) return rc;
     const int nt = batched_nt(B);
     if (nt > 8) return fail(ZTM_EINVAL, "batched forward supports at most 256 models per call (got %d)", B);
     const int Bp = 32 * nt;
@@ -668,52 +765,68 @@ int batched_fwd_launch(const PmShape& s, int B, int ks, int split, const float* 
     const int per = (int)((n_chunks + ks - 1) / ks);
     ks = (int)((n_chunks + per - 1) / per);
     const int64_t n_packed = n_chunks * Bp * kBatchChunk;
+    const int np = split_parts(split);
     __bf16* parts = (__bf16*)ws;
-    float* part = (float*)(parts + split * n_packed);
+    float* part = (float*)(parts + np * n_packed);
+    unsigned* amax = nullptr;
+    float gs = 1.f;
     const dim3 pgrid((unsigned)((s.n_cells + kPackCells - 1) / kPackCells), (unsigned)(Bp / 32));
-    if (split == 3)
-        hipLaunchKernelGGL(batched_pack_kernel<3>, pgrid, dim3(256), 0, stream, P, B, s.n_cells,
-                           Bp, n_chunks, n_packed, (float)thr, 1.f, parts);
-    else
-        hipLaunchKernelGGL(batched_pack_kernel<2>, pgrid, dim3(256), 0, stream, P, B, s.n_cells,
-                           Bp, n_chunks, n_packed, (float)thr, 1.f, parts);
+    if (split == 4) {
+        amax = (unsigned*)(part + (size_t)ks * s.m_pad * Bp);
+        gs = g_scale(gmax);
+        hipLaunchKernelGGL(batched_clear_kernel, dim3(1), dim3(256), 0, stream, amax, Bp);
+        const int64_t blocks = std::min<int64_t>(64, (s.n_cells + 1023) / 1024);
+        hipLaunchKernelGGL(batched_absmax_kernel, dim3((unsigned)blocks, (unsigned)B), dim3(256),
+                           0, stream, P, s.n_cells, (float)thr, amax);
+        hipLaunchKernelGGL((batched_pack_kernel<2, _Float16>), pgrid, dim3(256), 0, stream, P, B,
+                           s.n_cells, Bp, n_chunks, n_packed, (float)thr, 1.f,
+                           (const unsigned*)amax, (_Float16*)parts);
+    } else if (split == 3) {
+        hipLaunchKernelGGL((batched_pack_kernel<3, __bf16>), pgrid, dim3(256), 0, stream, P, B,
+                           s.n_cells, Bp, n_chunks, n_packed, (float)thr, 1.f, nullptr, parts);
+    } else {
+        hipLaunchKernelGGL((batched_pack_kernel<2, __bf16>), pgrid, dim3(256), 0, stream, P, B,
+                           s.n_cells, Bp, n_chunks, n_packed, (float)thr, 1.f, nullptr, parts);
+    }
     ZTM_HIP(hipGetLastError());
     const dim3 grid((unsigned)((s.m + kBatchTile - 1) / kBatchTile), (unsigned)ks);
-    const int64_t stride = n_packed / 8;   // in bf16x8 units
+    const int64_t stride = n_packed / 8;   // in 16-byte units
     {
         ProfScope prof("batched_fwd", stream);
-#define ZTM_BATCHED_LAUNCH(NT, SP, MODE)                                                        \
-    hipLaunchKernelGGL((batched_fwd_kernel<NT, SP, MODE>), grid, dim3(256), 0, stream,         \
-                       (const float4*)st, (const bf16x8*)parts, stride, s.nx, s.ny, 1.0 / s.nx, \
-                       1.0 / s.ny, (float)sy, (float)sz, (int)n_chunks, per, s.m_pad, part)
-#define ZTM_BATCHED_CASE(NT, SP)                                                               \
+#define ZTM_BATCHED_LAUNCH(NT, SP, MODE, H)                                                     \
+    hipLaunchKernelGGL((batched_fwd_kernel<NT, SP, MODE, H>), grid, dim3(256), 0, stream,      \
+                       (const float4*)st, (const typename Half<H>::v8*)parts, stride, s.nx,     \
+                       s.ny, 1.0 / s.nx, 1.0 / s.ny, (float)sy, (float)sz, gs, (int)n_chunks,   \
+                       per, s.m_pad, part)
+#define ZTM_BATCHED_CASE(NT, SP, H)                                                            \
     if (s.nx % 8 == 0)                                                                         \
-        ZTM_BATCHED_LAUNCH(NT, SP, 2);                                                         \
+        ZTM_BATCHED_LAUNCH(NT, SP, 2, H);                                                      \
     else if (s.nx >= 8)                                                                        \
-        ZTM_BATCHED_LAUNCH(NT, SP, 1);                                                         \
+        ZTM_BATCHED_LAUNCH(NT, SP, 1, H);                                                      \
     else                                                                                       \
-        ZTM_BATCHED_LAUNCH(NT, SP, 0)
-        if (split == 3) {
-            switch (nt) {
-                case 1: ZTM_BATCHED_CASE(1, 3); break;
-                case 2: ZTM_BATCHED_CASE(2, 3); break;
-                case 4: ZTM_BATCHED_CASE(4, 3); break;
-                default: ZTM_BATCHED_CASE(8, 3); break;
-            }
+        ZTM_BATCHED_LAUNCH(NT, SP, 0, H)
+#define ZTM_BATCHED_NT(SP, H)                                                                  \
+    switch (nt) {                                                                              \
+        case 1: ZTM_BATCHED_CASE(1, SP, H); break;                                             \
+        case 2: ZTM_BATCHED_CASE(2, SP, H); break;                                             \
+        case 4: ZTM_BATCHED_CASE(4, SP, H); break;                                             \
+        default: ZTM_BATCHED_CASE(8, SP, H); break;                                            \
+    }
+        if (split == 4) {
+            ZTM_BATCHED_NT(2, _Float16)
+        } else if (split == 3) {
+            ZTM_BATCHED_NT(3, __bf16)
         } else {
-            switch (nt) {
-                case 1: ZTM_BATCHED_CASE(1, 2); break;
-                case 2: ZTM_BATCHED_CASE(2, 2); break;
-                case 4: ZTM_BATCHED_CASE(4, 2); break;
-                default: ZTM_BATCHED_CASE(8, 2); break;
-            }
+            ZTM_BATCHED_NT(2, __bf16)
         }
+#undef ZTM_BATCHED_NT
 #undef ZTM_BATCHED_CASE
 #undef ZTM_BATCHED_LAUNCH
         ZTM_HIP(hipGetLastError());
     }
     hipLaunchKernelGGL(batched_reduce_kernel<double>, dim3((s.m + 7) / 8, (B + 31) / 32), dim3(256), 0,
-                       stream, part, ks, s.m, s.m_pad, B, Bp, scale, Y);
+                       stream, part, ks, s.m, s.m_pad, B, Bp, scale, (const unsigned*)amax,
+                       (double)gs, Y);
     ZTM_HIP(hipGetLastError());
     return ZTM_OK;
 }
@@ -731,16 +844,19 @@ size_t batched_adj_workspace(const PmShape& s, int B, int ks, int split) {
     const int Bp = 32 * batched_nt(B);
     const int64_t m16 = ((int64_t)s.m + kBatchChunk - 1) / kBatchChunk * kBatchChunk;
     const int64_t n_pad = (s.n_cells + kBatchTile - 1) / kBatchTile * kBatchTile;
-    const size_t packed = (size_t)m16 * Bp * split * sizeof(__bf16);
-    const size_t partial = ks > 1 ? (size_t)ks * n_pad * Bp * sizeof(float) : 0;   // 1: direct
-    return packed + partial;
+    const size_t packed = (size_t)m16 * Bp * split_parts(split) * sizeof(__bf16);
+    // one BF16 slice stores X directly; FP16 always goes through the reduce,
+    // which takes out the per-vector scales
+    const bool direct = ks == 1 && split != 4;
+    const size_t partial = direct ? 0 : (size_t)ks * n_pad * Bp * sizeof(float);
+    return packed + partial + kAmaxBytes;
 }
 
 int batched_adj_launch(const PmShape& s, int B, int ks, int split, const float* st,
-                       const float* R, double sy, double sz, double scale, void* ws, float* X,
-                       hipStream_t stream) {
+                       const float* R, double sy, double sz, double scale, double gmax, void* ws,
+                       float* X, hipStream_t stream) {
     if (B <= 0 || s.n_cells == 0) return ZTM_OK;
-    if (split != 2 && split != 3) return fail(ZTM_EINVAL, "split must be 2 or 3 (got %d)", split);
+    if (int rc = check_split(split, gmax)) return rc;
     const int nt = batched_nt(B);
     if (nt > 8) return fail(ZTM_EINVAL, "batched adjoint supports at most 256 vectors per call (got %d)", B);
     if (s.n_cells > (int64_t)INT32_MAX - 2 * kBatchTile)
@@ -762,50 +878,64 @@ int batched_adj_launch(const PmShape& s, int B, int ks, int split, const float* 
     const int64_t n_packed = n_chunks * Bp * kBatchChunk;
     const int64_t tiles = (s.n_cells + kBatchTile - 1) / kBatchTile;
     const int n_pad = (int)(tiles * kBatchTile);
+    const int np = split_parts(split);
     __bf16* parts = (__bf16*)ws;
-    float* part = (float*)(parts + split * n_packed);
-    float* direct = ks == 1 ? X : nullptr;
+    float* part = (float*)(parts + np * n_packed);
+    float* direct = ks == 1 && split != 4 ? X : nullptr;
     // The residuals are packed like the forward's models, stations in place
     // of cells; with one slice they carry the scale (kernel stores as is).
     const float mul = direct ? (float)scale : 1.f;
+    unsigned* amax = nullptr;
+    float gs = 1.f;
     const dim3 pgrid((unsigned)((s.m + kPackCells - 1) / kPackCells), (unsigned)(Bp / 32));
-    if (split == 3)
-        hipLaunchKernelGGL(batched_pack_kernel<3>, pgrid, dim3(256), 0, stream, R, B,
-                           (int64_t)s.m, Bp, n_chunks, n_packed, 0.f, mul, parts);
-    else
-        hipLaunchKernelGGL(batched_pack_kernel<2>, pgrid, dim3(256), 0, stream, R, B,
-                           (int64_t)s.m, Bp, n_chunks, n_packed, 0.f, mul, parts);
+    if (split == 4) {
+        amax = (unsigned*)(part + (size_t)ks * n_pad * Bp);
+        gs = g_scale(gmax);
+        hipLaunchKernelGGL(batched_clear_kernel, dim3(1), dim3(256), 0, stream, amax, Bp);
+        const int64_t blocks = std::min<int64_t>(64, ((int64_t)s.m + 1023) / 1024);
+        hipLaunchKernelGGL(batched_absmax_kernel, dim3((unsigned)blocks, (unsigned)B), dim3(256),
+                           0, stream, R, (int64_t)s.m, 0.f, amax);
+        hipLaunchKernelGGL((batched_pack_kernel<2, _Float16>), pgrid, dim3(256), 0, stream, R, B,
+                           (int64_t)s.m, Bp, n_chunks, n_packed, 0.f, 1.f,
+                           (const unsigned*)amax, (_Float16*)parts);
+    } else if (split == 3) {
+        hipLaunchKernelGGL((batched_pack_kernel<3, __bf16>), pgrid, dim3(256), 0, stream, R, B,
+                           (int64_t)s.m, Bp, n_chunks, n_packed, 0.f, mul, nullptr, parts);
+    } else {
+        hipLaunchKernelGGL((batched_pack_kernel<2, __bf16>), pgrid, dim3(256), 0, stream, R, B,
+                           (int64_t)s.m, Bp, n_chunks, n_packed, 0.f, mul, nullptr, parts);
+    }
     ZTM_HIP(hipGetLastError());
     const dim3 grid((unsigned)tiles, (unsigned)ks);
-    const int64_t stride = n_packed / 8;   // in bf16x8 units
+    const int64_t stride = n_packed / 8;   // in 16-byte units
     const int vec4 = s.n_cells % 4 == 0 && (uintptr_t)X % 16 == 0;
     {
         ProfScope prof("batched_adj", stream);
-#define ZTM_BATCHED_ADJ_ROW(NT, SP, ROW)                                                         \
-    hipLaunchKernelGGL((batched_adj_kernel<NT, SP, ROW>), grid, dim3(256), 0, stream,            \
-                       (const float4*)st, (const bf16x8*)parts, stride, s.nx, s.ny, 1.0 / s.nx,  \
-                       1.0 / s.ny, (float)sy, (float)sz, (int)n_chunks, per, n_pad, part, direct, \
-                       (int)s.n_cells, B, vec4)
-#define ZTM_BATCHED_ADJ(NT, SP)                                                                  \
+#define ZTM_BATCHED_ADJ_ROW(NT, SP, ROW, H)                                                      \
+    hipLaunchKernelGGL((batched_adj_kernel<NT, SP, ROW, H>), grid, dim3(256), 0, stream,         \
+                       (const float4*)st, (const typename Half<H>::v8*)parts, stride, s.nx,      \
+                       s.ny, 1.0 / s.nx, 1.0 / s.ny, (float)sy, (float)sz, gs, (int)n_chunks,    \
+                       per, n_pad, part, direct, (int)s.n_cells, B, vec4)
+#define ZTM_BATCHED_ADJ(NT, SP, H)                                                               \
     if (s.nx % 32 == 0)                                                                        \
-        ZTM_BATCHED_ADJ_ROW(NT, SP, 1);                                                        \
+        ZTM_BATCHED_ADJ_ROW(NT, SP, 1, H);                                                     \
     else                                                                                       \
-        ZTM_BATCHED_ADJ_ROW(NT, SP, 0)
-        if (split == 3) {
-            switch (nt) {
-                case 1: ZTM_BATCHED_ADJ(1, 3); break;
-                case 2: ZTM_BATCHED_ADJ(2, 3); break;
-                case 4: ZTM_BATCHED_ADJ(4, 3); break;
-                default: ZTM_BATCHED_ADJ(8, 3); break;
-            }
+        ZTM_BATCHED_ADJ_ROW(NT, SP, 0, H)
+#define ZTM_BATCHED_ADJ_NT(SP, H)                                                              \
+    switch (nt) {                                                                              \
+        case 1: ZTM_BATCHED_ADJ(1, SP, H); break;                                              \
+        case 2: ZTM_BATCHED_ADJ(2, SP, H); break;                                              \
+        case 4: ZTM_BATCHED_ADJ(4, SP, H); break;                                              \
+        default: ZTM_BATCHED_ADJ(8, SP, H); break;                                             \
+    }
+        if (split == 4) {
+            ZTM_BATCHED_ADJ_NT(2, _Float16)
+        } else if (split == 3) {
+            ZTM_BATCHED_ADJ_NT(3, __bf16)
         } else {
-            switch (nt) {
-                case 1: ZTM_BATCHED_ADJ(1, 2); break;
-                case 2: ZTM_BATCHED_ADJ(2, 2); break;
-                case 4: ZTM_BATCHED_ADJ(4, 2); break;
-                default: ZTM_BATCHED_ADJ(8, 2); break;
-            }
+            ZTM_BATCHED_ADJ_NT(2, __bf16)
         }
+#undef ZTM_BATCHED_ADJ_NT
 #undef ZTM_BATCHED_ADJ
 #undef ZTM_BATCHED_ADJ_ROW
         ZTM_HIP(hipGetLastError());
@@ -813,7 +943,8 @@ int batched_adj_launch(const PmShape& s, int B, int ks, int split, const float* 
     if (direct) return ZTM_OK;
     hipLaunchKernelGGL(batched_reduce_kernel<float>,
                        dim3((unsigned)((s.n_cells + 7) / 8), (B + 31) / 32), dim3(256), 0, stream,
-                       part, ks, (int)s.n_cells, n_pad, B, Bp, scale, X);
+                       part, ks, (int)s.n_cells, n_pad, B, Bp, scale, (const unsigned*)amax,
+                       (double)gs, X);
     ZTM_HIP(hipGetLastError());
     return ZTM_OK;
 }

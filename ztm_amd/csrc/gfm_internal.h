@@ -114,7 +114,7 @@ struct PmTuning {
     int cells_per_lane = 8;   // adjoint: cells per lane (4,8,16)
     int adj_splits = 0;       // adjoint: 0 = choose from the problem shape
     int batched_slices = 0;   // batched forward K-slices: 0 = choose
-    int batched_split = 3;    // batched forward BF16 split: 2 (hi/lo) or 3
+    int batched_split = 3;    // batched parts: 2 or 3 BF16, or 4 = FP16 hi/lo
 };
 
 struct PmShape {
@@ -291,16 +291,18 @@ int pm_fwd_fields_launch(const PmShape& s, int F, const float4* st, const float*
 // double [B][M]; ws must hold batched_workspace(s, B, ks) bytes.
 size_t batched_workspace(const PmShape& s, int B, int ks, int split);
 int batched_default_slices(const PmShape& s, int B);
+// split: 2 or 3 BF16 parts, or 4 = two FP16 parts scaled by powers of two,
+// which needs gmax, a bound on |G| over the station-cell pairs (else unused).
 int batched_fwd_launch(const PmShape& s, int B, int ks, int split, const float* st,
-                       const float* P, double thr, double sy, double sz, double scale, void* ws,
-                       double* Y, hipStream_t stream);
+                       const float* P, double thr, double sy, double sz, double scale,
+                       double gmax, void* ws, double* Y, hipStream_t stream);
 // Batched adjoint on the same scheme: R float [B][M] -> X float [B][N];
 // ws must hold batched_adj_workspace(s, B, ks, split) bytes.
 size_t batched_adj_workspace(const PmShape& s, int B, int ks, int split);
 int batched_adj_default_slices(const PmShape& s, int B);
 int batched_adj_launch(const PmShape& s, int B, int ks, int split, const float* st,
-                       const float* R, double sy, double sz, double scale, void* ws, float* X,
-                       hipStream_t stream);
+                       const float* R, double sy, double sz, double scale, double gmax, void* ws,
+                       float* X, hipStream_t stream);
 
 // out = L^T L rho for forward-difference gradients on an nx x ny x nz grid
 // (gfm_reg.hip).
@@ -347,6 +349,7 @@ struct ztm_plan_s {
     double* part = nullptr;      // split-K workspace
     size_t part_cap = 0;         // doubles
     float* G = nullptr;          // stored matrix [m][stored_ld], or null
+    double gmax = 0.0;           // bound on |G| (kernel units) over station-cell pairs
     void* bws = nullptr;         // batched forward workspace
     size_t bws_cap = 0;          // bytes
 };

@@ -410,8 +410,9 @@ class Plan:
         return out
 
     def set_batched_tuning(self, slices: int = 0, split: int = 0) -> None:
-        """Forward K-slices (0 = choose) and BF16 split (2 = hi/lo, 3 = three-way;
-        0 = keep); the split also applies to adjoint_batched."""
+        """Forward K-slices (0 = choose) and split (2 = BF16 hi/lo, 3 = BF16
+        three-way, 4 = FP16 hi/lo under power-of-two scales; 0 = keep); the
+        split also applies to adjoint_batched."""
         check(lib().ztm_plan_set_batched_tuning(self._h, slices, split))
 
     # stored-matrix variant
