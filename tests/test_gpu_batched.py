@@ -374,3 +374,28 @@ def test_batched_fp16_split_random_geometries(dev, orc):
             assert relerr(Y[b], want) < 1e-5, (case, nx, ny, nz, m, B, b)
             want = orc.pm_adjoint(g, W.KAPPA_SI, st, R[b].astype(np.float64))
             assert relerr(X[b], want) < 1e-5, (case, nx, ny, nz, m, B, b)
+
+
+def test_batched_fp16_split_adjoint_split_k(dev, orc):
+    """Split code 4 with more station chunks than one K-slice takes (13,000
+    stations over 4,096 cells): the adjoint's slice partials go through the
+    reduce with the per-vector scales, against the oracle and the BF16
+    three-way split."""
+    import torch
+    g = W.Grid(32, 16, 8, 0.0, 0.0, 0.0, 50.0, 50.0, 40.0)
+    c = g.centres()
+    rng = np.random.default_rng(77)
+    m = 13000
+    st = np.stack([rng.uniform(c[:, 0].min(), c[:, 0].max(), m),
+                   rng.uniform(c[:, 1].min(), c[:, 1].max(), m),
+                   rng.uniform(c[:, 2].max() + 25.0, c[:, 2].max() + 400.0, m)], axis=1)
+    R = (rng.normal(0.0, 1.0, (40, m)) * 10.0 ** rng.uniform(-3, 3, (40, 1))).astype(np.float32)
+    Rt = torch.from_numpy(R).to(dev)
+    with gfm.Plan(g, st, kappa=W.KAPPA_SI) as p:
+        X3 = p.adjoint_batched(Rt).cpu().numpy()
+        p.set_batched_tuning(split=4)
+        X = p.adjoint_batched(Rt).cpu().numpy()
+    for b in (0, 21, 39):
+        want = orc.pm_adjoint(g, W.KAPPA_SI, st, R[b].astype(np.float64))
+        assert relerr(X[b], want) < 1e-5, b
+        assert relerr(X3[b], want) < 1e-5, b

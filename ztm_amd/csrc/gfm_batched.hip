@@ -82,11 +82,26 @@ __global__ __launch_bounds__(256) void batched_absmax_kernel(const float* __rest
                                                              int64_t n, float thr,
                                                              unsigned* __restrict__ amax) {
     const int b = blockIdx.y;
+    const float* x = X + (size_t)b * n;
     float m = 0.f;
-    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
-        const float v = fabsf(X[(size_t)b * n + i]);
+    auto take = [&](float v) {
+        v = fabsf(v);
         if (v > thr) m = fmaxf(m, v);
+    };
+    int64_t i0 = 0;
+    if (((uintptr_t)x & 15) == 0) {   // 16-byte loads over the whole float4s of the row
+        const float4* x4 = reinterpret_cast<const float4*>(x);
+        const int64_t n4 = n >> 2;
+        for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n4;
+             i += (int64_t)gridDim.x * 256) {
+            const float4 v = x4[i];
+            take(v.x); take(v.y); take(v.z); take(v.w);
+        }
+        i0 = n4 << 2;
     }
+    for (int64_t i = i0 + (int64_t)blockIdx.x * 256 + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * 256)
+        take(x[i]);
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
     if ((threadIdx.x & 63) == 0) atomicMax(&amax[b], __float_as_uint(m));
@@ -175,13 +190,17 @@ __device__ __forceinline__ void divmod_pos(int n, int d, double inv, int& q, int
 }
 
 // One G value into element e of the SPLIT parts. FP16 parts take the value
-// times the power-of-two G scale gs, held to +-2^15: cells past the grid and
-// the padding station records, whose |G| the scale does not bound, stay
-// finite, so that their products with the zero-padded operand stay 0.
-template <int SPLIT, typename H>
+// times the power-of-two G scale gs (already applied when PRE), held to
+// +-2^15: cells past the grid and the padding station records, whose |G| the
+// scale does not bound, stay finite, so that their products with the
+// zero-padded operand stay 0.
+template <int SPLIT, typename H, bool PRE = false>
 __device__ __forceinline__ void store_split(float gv, float gs,
                                             typename Half<H>::v8 (&a)[SPLIT], int e) {
-    if constexpr (Half<H>::scaled) gv = __builtin_amdgcn_fmed3f(gv * gs, -32768.f, 32768.f);
+    if constexpr (Half<H>::scaled) {
+        if constexpr (!PRE) gv *= gs;
+        gv = __builtin_amdgcn_fmed3f(gv, -32768.f, 32768.f);
+    }
 #pragma unroll
     for (int q = 0; q < SPLIT; ++q) {
         const H h = (H)gv;
@@ -232,7 +251,9 @@ struct BatchedGen {
                 R1 = fmaf(dy1, dy1, fmaf(dz1, dz1, 1e-20f));
                 ew = nx_ - i0;
                 nxf = (float)nx_;
+                if constexpr (Half<H>::scaled) dz1 *= gs;
             }
+            if constexpr (Half<H>::scaled) dz0 *= gs;   // G scale folded into the row terms
         }
     }
 
@@ -275,7 +296,7 @@ struct BatchedGen {
             j = wj ? 0 : j;
             k += wj ? 1 : 0;
         }
-        store_split<SPLIT, H>(gv, gs, a, E);
+        store_split<SPLIT, H, MODE != 0>(gv, gs, a, E);
     }
 };
 
@@ -516,7 +537,7 @@ __global__ __launch_bounds__(256) void batched_adj_kernel(
             const float dy = p.y - cy;
             const float dz = p.z - cz;
             p.y = fmaf(dy, dy, fmaf(dz, dz, 1e-20f));
-            p.z = dz;
+            p.z = Half<H>::scaled ? dz * gs : dz;   // G scale folded into dz
         }
         static_for<0, 8>([&](auto ec) {
             constexpr int e = decltype(ec)::value;
@@ -531,7 +552,7 @@ __global__ __launch_bounds__(256) void batched_adj_kernel(
                 r2 = fmaf(x, x, fmaf(dy, dy, fmaf(dz, dz, 1e-20f)));
             }
             const float v = rsq_f(r2);
-            store_split<SPLIT, H>(dz * (v * v * v), gs, a, e);
+            store_split<SPLIT, H, ROW != 0>(dz * (v * v * v), gs, a, e);
         });
     };
     v8 a[SPLIT];
@@ -663,6 +684,43 @@ __global__ __launch_bounds__(256) void batched_reduce_kernel(const float* __rest
     }
 }
 
+// One K-slice: X[b][c] = f_b * part[c][b], f_b = scale, or with FP16 parts
+// scale / (gs * half_scale(amax[b])): a transpose through LDS in tiles of 64
+// cells x 32 models, so that each model's 64 cells leave as one 256-byte run
+// (the reduce's 8-entry runs halve the write rate at this size).
+// grid = (ceil(n / 64), ceil(B / 32)); block = 256.
+__global__ __launch_bounds__(256) void batched_transpose_kernel(const float* __restrict__ part,
+                                                                int n, int B, int Bp, double scale,
+                                                                const unsigned* __restrict__ amax,
+                                                                double gs, float* __restrict__ X) {
+    __shared__ float tile[32][65];
+    __shared__ double f[32];
+    const int nb = blockIdx.x * 64, bb = blockIdx.y * 32;
+    const int t = threadIdx.x;
+    if (t < 32) {
+        const int b = bb + t;
+        f[t] = amax && b < B ? scale / (gs * (double)half_scale(amax[b])) : scale;
+    }
+    {
+        const int tx = t & 31, ty = t >> 5;   // model, cell row
+        const int b = bb + tx;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const int c = nb + ty + 8 * i;
+            tile[tx][ty + 8 * i] = c < n && b < B ? part[(size_t)c * Bp + b] : 0.f;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        const int idx = i * 256 + t;
+        const int bl = idx >> 6, c = idx & 63;
+        const int b = bb + bl;
+        if (b < B && nb + c < n)
+            X[(size_t)b * n + nb + c] = (float)(f[bl] * (double)tile[bl][c]);
+    }
+}
+
 // --------------------------------------------------------------- launchers
 namespace {
 // model tiles of 32, padded to a power of two (1, 2, 4 or 8 for B <= 256)
@@ -775,7 +833,7 @@ int batched_fwd_launch(const PmShape& s, int B, int ks, int split, const float* 
         amax = (unsigned*)(part + (size_t)ks * s.m_pad * Bp);
         gs = g_scale(gmax);
         hipLaunchKernelGGL(batched_clear_kernel, dim3(1), dim3(256), 0, stream, amax, Bp);
-        const int64_t blocks = std::min<int64_t>(64, (s.n_cells + 1023) / 1024);
+        const int64_t blocks = std::min<int64_t>(64, (s.n_cells + 4095) / 4096);
         hipLaunchKernelGGL(batched_absmax_kernel, dim3((unsigned)blocks, (unsigned)B), dim3(256),
                            0, stream, P, s.n_cells, (float)thr, amax);
         hipLaunchKernelGGL((batched_pack_kernel<2, _Float16>), pgrid, dim3(256), 0, stream, P, B,
@@ -892,7 +950,7 @@ int batched_adj_launch(const PmShape& s, int B, int ks, int split, const float* 
         amax = (unsigned*)(part + (size_t)ks * n_pad * Bp);
         gs = g_scale(gmax);
         hipLaunchKernelGGL(batched_clear_kernel, dim3(1), dim3(256), 0, stream, amax, Bp);
-        const int64_t blocks = std::min<int64_t>(64, ((int64_t)s.m + 1023) / 1024);
+        const int64_t blocks = std::min<int64_t>(64, ((int64_t)s.m + 4095) / 4096);
         hipLaunchKernelGGL(batched_absmax_kernel, dim3((unsigned)blocks, (unsigned)B), dim3(256),
                            0, stream, R, (int64_t)s.m, 0.f, amax);
         hipLaunchKernelGGL((batched_pack_kernel<2, _Float16>), pgrid, dim3(256), 0, stream, R, B,
@@ -941,6 +999,14 @@ int batched_adj_launch(const PmShape& s, int B, int ks, int split, const float* 
         ZTM_HIP(hipGetLastError());
     }
     if (direct) return ZTM_OK;
+    if (ks == 1) {
+        hipLaunchKernelGGL(batched_transpose_kernel,
+                           dim3((unsigned)((s.n_cells + 63) / 64), (B + 31) / 32), dim3(256), 0,
+                           stream, part, (int)s.n_cells, B, Bp, scale, (const unsigned*)amax,
+                           (double)gs, X);
+        ZTM_HIP(hipGetLastError());
+        return ZTM_OK;
+    }
     hipLaunchKernelGGL(batched_reduce_kernel<float>,
                        dim3((unsigned)((s.n_cells + 7) / 8), (B + 31) / 32), dim3(256), 0, stream,
                        part, ks, (int)s.n_cells, n_pad, B, Bp, scale, (const unsigned*)amax,
