@@ -399,3 +399,26 @@ def test_batched_fp16_split_adjoint_split_k(dev, orc):
         want = orc.pm_adjoint(g, W.KAPPA_SI, st, R[b].astype(np.float64))
         assert relerr(X[b], want) < 1e-5, b
         assert relerr(X3[b], want) < 1e-5, b
+
+
+def test_batched_fp16_split_edge_plans(dev):
+    """Split code 4 where the bound on |G| is 0: no stations (the adjoint
+    writes zeros), and a one-layer grid with every station in the layer's
+    plane (G is 0 throughout, so are both directions)."""
+    import torch
+    g = W.Grid(20, 10, 1, 0.0, 0.0, 0.0, 50.0, 50.0, 40.0)
+    P = torch.ones(6, g.n_cells, device=dev)
+    with gfm.Plan(g, np.zeros((0, 3)), kappa=W.KAPPA_SI) as p:
+        p.set_batched_tuning(split=4)
+        X = p.adjoint_batched(torch.ones(6, 0, device=dev)).cpu().numpy()
+        assert X.shape == (6, g.n_cells) and not X.any()
+    c = g.centres()
+    st = np.stack([np.linspace(c[:, 0].min(), c[:, 0].max(), 7) + 3.0,
+                   np.linspace(c[:, 1].min(), c[:, 1].max(), 7) - 2.0,
+                   np.full(7, c[0, 2])], axis=1)
+    with gfm.Plan(g, st, kappa=W.KAPPA_SI) as p:
+        p.set_batched_tuning(split=4)
+        Y = p.forward_batched(P).cpu().numpy()
+        X = p.adjoint_batched(torch.ones(6, 7, device=dev)).cpu().numpy()
+    assert np.isfinite(Y).all() and not Y.any()
+    assert np.isfinite(X).all() and not X.any()

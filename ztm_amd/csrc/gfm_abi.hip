@@ -304,7 +304,7 @@ namespace {
 // layer, and |dz| (h^2 + dz^2)^-1.5 rises then falls in |dz| on either side
 // of the station, peaking at |dz| = h / sqrt(2); so the largest value lies
 // within a layer of Z -+ h / sqrt(2), and two layers either side of each are
-// tried. 0 without stations or cells.
+// tried. 0 without stations or cells (and where G is 0 throughout).
 double g_bound(const std::vector<float4>& hf, int64_t n, int nx, int ny, int nz, double sy,
                double sz) {
     double best = 0.0;
@@ -669,7 +669,7 @@ int ztm_plan_set_batched_tuning(ztm_plan plan, int slices, int split) {
     if (slices < 0) return fail(ZTM_EINVAL, "negative slice count");
     if (split != 0 && split != 2 && split != 3 && split != 4)
         return fail(ZTM_EINVAL, "split must be 2, 3 or 4");
-    if (split == 4 && !(plan->gmax > 0.0))
+    if (split == 4 && !(plan->gmax >= 0.0))
         return fail(ZTM_EINVAL, "the FP16 split (4) needs a plan made by ztm_plan_create");
     plan->tuning.batched_slices = slices;
     if (split) plan->tuning.batched_split = split;
@@ -910,7 +910,7 @@ int tfm_gravity_of_fields(ztm_plan p, int bfr, int di, int dj, int fh, int fv, i
     if (int rc = ensure_batched_workspace(p, need)) return rc;
     double* Y = (double*)((char*)p->bws + ws);
     if (int e = batched_fwd_launch(p->shape, B, ks, split, (const float*)p->st_f, drho_dev, thr,
-                                   p->sy, p->sz, pm_scale(p), 0.0, p->bws, Y, s))
+                                   p->sy, p->sz, pm_scale(p), -1.0, p->bws, Y, s))
         return e;
     return tfm_scatter_launch(Y, p->out_map, m, M, B, g_dev, s);
 }

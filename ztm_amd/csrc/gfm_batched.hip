@@ -104,7 +104,11 @@ __global__ __launch_bounds__(256) void batched_absmax_kernel(const float* __rest
         take(x[i]);
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
-    if ((threadIdx.x & 63) == 0) atomicMax(&amax[b], __float_as_uint(m));
+    __shared__ float wm[4];
+    if ((threadIdx.x & 63) == 0) wm[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0)   // one atomic per block
+        atomicMax(&amax[b], __float_as_uint(fmaxf(fmaxf(wm[0], wm[1]), fmaxf(wm[2], wm[3]))));
 }
 
 __global__ void batched_clear_kernel(unsigned* __restrict__ x, int n) {
@@ -738,8 +742,9 @@ int split_parts(int split) { return split == 4 ? 2 : split; }
 constexpr size_t kAmaxBytes = 256 * sizeof(unsigned);   // per-model absmax (FP16)
 
 // Power-of-two G scale of the FP16 parts: brings gmax (a bound on |G|) to at
-// most 2^kHalfTop.
+// most 2^kHalfTop; 1 when G is 0 throughout.
 float g_scale(double gmax) {
+    if (!(gmax > 0.0)) return 1.f;
     int e;
     std::frexp(gmax, &e);   // gmax < 2^e
     return (float)std::ldexp(1.0, std::max(-120, std::min(120, kHalfTop - e)));
@@ -748,7 +753,7 @@ float g_scale(double gmax) {
 int check_split(int split, double gmax) {
     if (split != 2 && split != 3 && split != 4)
         return fail(ZTM_EINVAL, "split must be 2, 3 or 4 (got %d)", split);
-    if (split == 4 && !(gmax > 0.0 && std::isfinite(gmax)))
+    if (split == 4 && !(gmax >= 0.0 && std::isfinite(gmax)))
         return fail(ZTM_EINVAL, "the FP16 split needs the plan's bound on |G| (none for this plan)");
     return ZTM_OK;
 }
@@ -833,7 +838,7 @@ int batched_fwd_launch(const PmShape& s, int B, int ks, int split, const float* 
         amax = (unsigned*)(part + (size_t)ks * s.m_pad * Bp);
         gs = g_scale(gmax);
         hipLaunchKernelGGL(batched_clear_kernel, dim3(1), dim3(256), 0, stream, amax, Bp);
-        const int64_t blocks = std::min<int64_t>(64, (s.n_cells + 4095) / 4096);
+        const int64_t blocks = std::min<int64_t>(16, (s.n_cells + 4095) / 4096);
         hipLaunchKernelGGL(batched_absmax_kernel, dim3((unsigned)blocks, (unsigned)B), dim3(256),
                            0, stream, P, s.n_cells, (float)thr, amax);
         hipLaunchKernelGGL((batched_pack_kernel<2, _Float16>), pgrid, dim3(256), 0, stream, P, B,
@@ -914,7 +919,8 @@ int batched_adj_launch(const PmShape& s, int B, int ks, int split, const float* 
                        const float* R, double sy, double sz, double scale, double gmax, void* ws,
                        float* X, hipStream_t stream) {
     if (B <= 0 || s.n_cells == 0) return ZTM_OK;
-    if (int rc = check_split(split, gmax)) return rc;
+    if (split != 4 || s.m > 0)
+        if (int rc = check_split(split, gmax)) return rc;
     const int nt = batched_nt(B);
     if (nt > 8) return fail(ZTM_EINVAL, "batched adjoint supports at most 256 vectors per call (got %d)", B);
     if (s.n_cells > (int64_t)INT32_MAX - 2 * kBatchTile)
@@ -950,7 +956,7 @@ int batched_adj_launch(const PmShape& s, int B, int ks, int split, const float* 
         amax = (unsigned*)(part + (size_t)ks * n_pad * Bp);
         gs = g_scale(gmax);
         hipLaunchKernelGGL(batched_clear_kernel, dim3(1), dim3(256), 0, stream, amax, Bp);
-        const int64_t blocks = std::min<int64_t>(64, ((int64_t)s.m + 4095) / 4096);
+        const int64_t blocks = std::min<int64_t>(16, ((int64_t)s.m + 4095) / 4096);
         hipLaunchKernelGGL(batched_absmax_kernel, dim3((unsigned)blocks, (unsigned)B), dim3(256),
                            0, stream, R, (int64_t)s.m, 0.f, amax);
         hipLaunchKernelGGL((batched_pack_kernel<2, _Float16>), pgrid, dim3(256), 0, stream, R, B,

@@ -292,7 +292,8 @@ int pm_fwd_fields_launch(const PmShape& s, int F, const float4* st, const float*
 size_t batched_workspace(const PmShape& s, int B, int ks, int split);
 int batched_default_slices(const PmShape& s, int B);
 // split: 2 or 3 BF16 parts, or 4 = two FP16 parts scaled by powers of two,
-// which needs gmax, a bound on |G| over the station-cell pairs (else unused).
+// which needs gmax >= 0, a bound on |G| over the station-cell pairs (else
+// unused; < 0 = no bound known).
 int batched_fwd_launch(const PmShape& s, int B, int ks, int split, const float* st,
                        const float* P, double thr, double sy, double sz, double scale,
                        double gmax, void* ws, double* Y, hipStream_t stream);
@@ -349,7 +350,7 @@ struct ztm_plan_s {
     double* part = nullptr;      // split-K workspace
     size_t part_cap = 0;         // doubles
     float* G = nullptr;          // stored matrix [m][stored_ld], or null
-    double gmax = 0.0;           // bound on |G| (kernel units) over station-cell pairs
+    double gmax = -1.0;          // bound on |G| (kernel units) over station-cell pairs; < 0: none
     void* bws = nullptr;         // batched forward workspace
     size_t bws_cap = 0;          // bytes
 };
