@@ -193,14 +193,14 @@ int ztm_plan_forward_batched(ztm_plan plan, int n_models, const float* rho_dev,
 int ztm_plan_adjoint_batched(ztm_plan plan, int n_vectors, const float* r_dev,
                              float* out_dev, void* stream);
 /* Batched forward tuning: K-slices (0 = choose from the problem shape) and
- * split (0 = keep): 2 or 3 BF16 parts, or 4 = two FP16 parts, G scaled by a
- * power of two from the plan's bound on |G| and each model / residual vector
+ * split (0 = keep): 2 or 3 BF16 parts, or 4 = two FP16 parts, G scaled by
+ * powers of two from the plan's bounds on |G| and each model / residual vector
  * by a power of two from its largest entry (22 significant bits in 3 MFMA
- * products, against 24 in the BF16 three-way split's 6). The G scale is one
- * per plan: stations whose largest |G| lies 2^10 or more below the plan's
- * keep fewer bits. The split applies to the batched adjoint too. ZTM_EINVAL
- * for other values. No reference counterpart (the reference has no batched
- * path). */
+ * products, against 24 in the BF16 three-way split's 6). The forward
+ * scales G per station; the adjoint takes one scale per plan, so there
+ * stations whose largest |G| lies 2^17 or more below the plan's keep fewer
+ * bits. The split applies to the batched adjoint too. ZTM_EINVAL for other
+ * values. No reference counterpart (the reference has no batched path). */
 int ztm_plan_set_batched_tuning(ztm_plan plan, int slices, int split);
 
 /* Smoothness operator of the regularised misfit 0.5||G rho-d||^2 +

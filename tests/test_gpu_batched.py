@@ -422,3 +422,34 @@ def test_batched_fp16_split_edge_plans(dev):
         X = p.adjoint_batched(torch.ones(6, 7, device=dev)).cpu().numpy()
     assert np.isfinite(Y).all() and not Y.any()
     assert np.isfinite(X).all() and not X.any()
+
+
+def test_batched_fp16_split_mixed_heights(dev, orc):
+    """Split code 4 with stations 0.05 cell above the grid's top face and
+    stations 2,000 cells above it (their largest |G| differ by ~2^23.6): the
+    forward's per-station G scales keep the far stations' results to 1e-5 of
+    their own largest value (an FP16 emulation with one plan-wide scale gives
+    1.3e-4 there, per-station scales 2.0e-6); the adjoint against the
+    oracle."""
+    import torch
+    g = W.Grid(40, 40, 10, 0.0, 0.0, 0.0, 50.0, 50.0, 50.0)
+    c = g.centres()
+    rng = np.random.default_rng(5)
+    m = 160
+    top = c[:, 2].max() + 25.0
+    z = np.where(np.arange(m) % 2 == 0, top + 2.5, top + 100000.0)
+    st = np.stack([rng.uniform(c[:, 0].min(), c[:, 0].max(), m),
+                   rng.uniform(c[:, 1].min(), c[:, 1].max(), m), z], axis=1)
+    far = np.arange(m) % 2 == 1
+    P = rng.uniform(-300, 300, (8, g.n_cells)).astype(np.float32)
+    Rv = rng.normal(0.0, 1.0, (8, m)).astype(np.float32)
+    with gfm.Plan(g, st, kappa=W.KAPPA_SI) as p:
+        p.set_batched_tuning(split=4)
+        Y = p.forward_batched(torch.from_numpy(P).to(dev)).cpu().numpy()
+        X = p.adjoint_batched(torch.from_numpy(Rv).to(dev)).cpu().numpy()
+    for b in (0, 7):
+        want = orc.pm_forward(g, W.KAPPA_SI, P[b].astype(np.float64), st)
+        assert relerr(Y[b], want) < 1e-5, b
+        assert relerr(Y[b][far], want[far]) < 1e-5, b
+        want = orc.pm_adjoint(g, W.KAPPA_SI, st, Rv[b].astype(np.float64))
+        assert relerr(X[b], want) < 1e-5, b

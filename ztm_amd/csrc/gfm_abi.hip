@@ -135,6 +135,7 @@ void plan_free(ztm_plan p) {
     if (p->st_f) (void)hipFree(p->st_f);
     if (p->st_d) (void)hipFree(p->st_d);
     if (p->st_c) (void)hipFree(p->st_c);
+    if (p->st_gs) (void)hipFree(p->st_gs);
     if (p->col_start) (void)hipFree(p->col_start);
     if (p->col_st) (void)hipFree(p->col_st);
     if (p->row_start) (void)hipFree(p->row_start);
@@ -304,10 +305,12 @@ namespace {
 // layer, and |dz| (h^2 + dz^2)^-1.5 rises then falls in |dz| on either side
 // of the station, peaking at |dz| = h / sqrt(2); so the largest value lies
 // within a layer of Z -+ h / sqrt(2), and two layers either side of each are
-// tried. 0 without stations or cells (and where G is 0 throughout).
+// tried. bound[m] is station m's largest value (0 where its G row is 0
+// throughout); the result is the largest over all stations.
 double g_bound(const std::vector<float4>& hf, int64_t n, int nx, int ny, int nz, double sy,
-               double sz) {
+               double sz, std::vector<double>& bound) {
     double best = 0.0;
+    bound.assign((size_t)n, 0.0);
     if (nx <= 0 || ny <= 0 || nz <= 0) return 0.0;
     auto nearest = [](double v, int cnt) {
         return std::max(0.0, std::min((double)(cnt - 1), std::nearbyint(v)));
@@ -321,9 +324,11 @@ double g_bound(const std::vector<float4>& hf, int64_t n, int nx, int ny, int nz,
             for (int d = -2; d <= 2; ++d) {
                 const double k = std::max(0.0, std::min((double)(nz - 1), k0 + d));
                 const double dz = Z - k * sz, r2 = h2 + dz * dz;
-                if (r2 > 0.0) best = std::max(best, std::fabs(dz) / (r2 * std::sqrt(r2)));
+                if (r2 > 0.0)
+                    bound[m] = std::max(bound[m], std::fabs(dz) / (r2 * std::sqrt(r2)));
             }
         }
+        best = std::max(best, bound[m]);
     }
     return best;
 }
@@ -371,7 +376,11 @@ int ztm_plan_create(ztm_plan* plan, const ztm_problem* problem, int64_t n_statio
         };
         hc[m] = make_int4(idx(X, g.nx), idx(Y / p->sy, g.ny), idx(Z / p->sz, g.nz), 0);
     }
-    p->gmax = g_bound(hf, n_stations, g.nx, g.ny, g.nz, p->sy, p->sz);
+    std::vector<double> gb;
+    p->gmax = g_bound(hf, n_stations, g.nx, g.ny, g.nz, p->sy, p->sz, gb);
+    // per-station FP16 G scales of the batched forward (padding records: 1)
+    std::vector<float> hgs(p->shape.m_pad, 1.f);
+    for (int64_t m = 0; m < n_stations; ++m) hgs[m] = half_g_scale(gb[m]);
     // Adjoint gather indexes (stable counting sorts by key):
     //  columns: stations by nearest (ci, cj) over the (cmx, cmy)-extended range;
     //  rows:    stations that may clamp some grid row, by nearest (cj, ck).
@@ -418,6 +427,7 @@ int ztm_plan_create(ztm_plan* plan, const ztm_problem* problem, int64_t n_statio
     hipError_t e = upload(&p->st_f, hf);
     if (e == hipSuccess) e = upload(&p->st_d, hd);
     if (e == hipSuccess) e = upload(&p->st_c, hc);
+    if (e == hipSuccess) e = upload(&p->st_gs, hgs);
     if (e == hipSuccess) e = upload(&p->col_start, col_start);
     if (e == hipSuccess) e = upload(&p->col_st, col_st);
     if (e == hipSuccess && any_row) e = upload(&p->row_start, row_start);
@@ -628,7 +638,7 @@ int ztm_plan_forward_batched(ztm_plan plan, int n_models, const float* rho_dev, 
     if (int rc = ensure_batched_workspace(plan, need)) return rc;
     return batched_fwd_launch(plan->shape, n_models, ks, split, (const float*)plan->st_f, rho_dev,
                               plan->prob.threshold, plan->sy, plan->sz, pm_scale(plan),
-                              plan->gmax, plan->bws, g_dev, (hipStream_t)stream);
+                              plan->st_gs, plan->bws, g_dev, (hipStream_t)stream);
 }
 
 int ztm_plan_adjoint_batched(ztm_plan plan, int n_vectors, const float* r_dev, float* out_dev,
@@ -669,7 +679,7 @@ int ztm_plan_set_batched_tuning(ztm_plan plan, int slices, int split) {
     if (slices < 0) return fail(ZTM_EINVAL, "negative slice count");
     if (split != 0 && split != 2 && split != 3 && split != 4)
         return fail(ZTM_EINVAL, "split must be 2, 3 or 4");
-    if (split == 4 && !(plan->gmax >= 0.0))
+    if (split == 4 && !(plan->gmax >= 0.0 && (plan->st_gs || plan->shape.m == 0)))
         return fail(ZTM_EINVAL, "the FP16 split (4) needs a plan made by ztm_plan_create");
     plan->tuning.batched_slices = slices;
     if (split) plan->tuning.batched_split = split;
@@ -910,7 +920,7 @@ int tfm_gravity_of_fields(ztm_plan p, int bfr, int di, int dj, int fh, int fv, i
     if (int rc = ensure_batched_workspace(p, need)) return rc;
     double* Y = (double*)((char*)p->bws + ws);
     if (int e = batched_fwd_launch(p->shape, B, ks, split, (const float*)p->st_f, drho_dev, thr,
-                                   p->sy, p->sz, pm_scale(p), -1.0, p->bws, Y, s))
+                                   p->sy, p->sz, pm_scale(p), nullptr, p->bws, Y, s))
         return e;
     return tfm_scatter_launch(Y, p->out_map, m, M, B, g_dev, s);
 }

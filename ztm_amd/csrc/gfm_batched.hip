@@ -341,12 +341,14 @@ __device__ __forceinline__ void gen_after_mfma(int m, F&& elem) {
 // c&1), the next chunk's P parts load from global memory straight into the
 // other LDS buffer and its G block is generated. The last iteration repeats
 // its own chunk's loads into the buffer that is not read again.
-// H: part type (BF16, or FP16 with the G scale gs applied to G).
+// H: part type (BF16, or FP16 with G's rows scaled by the stations' power-of-
+// two scales gsc[m_pad]).
 template <int NT, int SPLIT, int MODE, typename H>   // model tiles of 32 (B <= 32 * NT); 2- or 3-way split; batched_gen mode
 __global__ __launch_bounds__(256) void batched_fwd_kernel(
     const float4* __restrict__ st, const typename Half<H>::v8* __restrict__ parts,
     int64_t part_stride, int nx, int ny, double inv_nx, double inv_ny, float sy, float sz,
-    float gs, int n_chunks, int chunks_per_slice, int m_pad, float* __restrict__ part) {
+    const float* __restrict__ gsc, int n_chunks, int chunks_per_slice, int m_pad,
+    float* __restrict__ part) {
     using v8 = typename Half<H>::v8;
     constexpr int Bp = 32 * NT;
     constexpr int V = Bp * 2;                          // 16-byte slots per part per chunk
@@ -356,6 +358,7 @@ __global__ __launch_bounds__(256) void batched_fwd_kernel(
     const int row = lane & 31, kh = lane >> 5;
     const int mbase = blockIdx.x * kBatchTile + wave * 32;
     const float4 p = st[mbase + row];
+    const float gs = Half<H>::scaled ? gsc[mbase + row] : 1.f;
 
     f32x16 acc[NT];
 #pragma unroll
@@ -656,14 +659,16 @@ __global__ __launch_bounds__(256) void batched_adj_kernel(
 // (models) tile, one entry per thread (reads coalesced along the models), and
 // writes it transposed through LDS. The adjoint uses it with cells in place
 // of stations and a float output.
-// With FP16 parts (amax given) the sums carry the G scale gs and model b's
-// operand scale half_scale(amax[b]), which the FP64 scale takes out.
+// With FP16 parts (amax given) the sums carry model b's operand scale
+// half_scale(amax[b]) and, in the forward, station m's G scale gsc[m] (gsc
+// given), which the FP64 scale takes out.
 template <typename TO>
 __global__ __launch_bounds__(256) void batched_reduce_kernel(const float* __restrict__ part,
                                                              int ks, int m, int m_pad, int B,
                                                              int Bp, double scale,
                                                              const unsigned* __restrict__ amax,
-                                                             double gs, TO* __restrict__ Y) {
+                                                             const float* __restrict__ gsc,
+                                                             TO* __restrict__ Y) {
     __shared__ double tile[8][33];
     const int mb = blockIdx.x * 8, bb = blockIdx.y * 32;
     const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;   // 32 x 8
@@ -683,27 +688,28 @@ __global__ __launch_bounds__(256) void batched_reduce_kernel(const float* __rest
     const int t = threadIdx.x;
     const int b = bb + (t >> 3), mm = mb + (t & 7);
     if (b < B && mm < m) {
-        const double f = amax ? scale / (gs * (double)half_scale(amax[b])) : scale;
+        double f = amax ? scale / (double)half_scale(amax[b]) : scale;
+        if (gsc) f /= (double)gsc[mm];
         Y[(size_t)b * m + mm] = (TO)(f * tile[t & 7][t >> 3]);
     }
 }
 
 // One K-slice: X[b][c] = f_b * part[c][b], f_b = scale, or with FP16 parts
-// scale / (gs * half_scale(amax[b])): a transpose through LDS in tiles of 64
+// scale / half_scale(amax[b]): a transpose through LDS in tiles of 64
 // cells x 32 models, so that each model's 64 cells leave as one 256-byte run
 // (the reduce's 8-entry runs halve the write rate at this size).
 // grid = (ceil(n / 64), ceil(B / 32)); block = 256.
 __global__ __launch_bounds__(256) void batched_transpose_kernel(const float* __restrict__ part,
                                                                 int n, int B, int Bp, double scale,
                                                                 const unsigned* __restrict__ amax,
-                                                                double gs, float* __restrict__ X) {
+                                                                float* __restrict__ X) {
     __shared__ float tile[32][65];
     __shared__ double f[32];
     const int nb = blockIdx.x * 64, bb = blockIdx.y * 32;
     const int t = threadIdx.x;
     if (t < 32) {
         const int b = bb + t;
-        f[t] = amax && b < B ? scale / (gs * (double)half_scale(amax[b])) : scale;
+        f[t] = amax && b < B ? scale / (double)half_scale(amax[b]) : scale;
     }
     {
         const int tx = t & 31, ty = t >> 5;   // model, cell row
@@ -741,19 +747,12 @@ namespace {
 int split_parts(int split) { return split == 4 ? 2 : split; }
 constexpr size_t kAmaxBytes = 256 * sizeof(unsigned);   // per-model absmax (FP16)
 
-// Power-of-two G scale of the FP16 parts: brings gmax (a bound on |G|) to at
-// most 2^kHalfTop; 1 when G is 0 throughout.
-float g_scale(double gmax) {
-    if (!(gmax > 0.0)) return 1.f;
-    int e;
-    std::frexp(gmax, &e);   // gmax < 2^e
-    return (float)std::ldexp(1.0, std::max(-120, std::min(120, kHalfTop - e)));
-}
+static_assert(kHalfTop == 14, "half_g_scale brings G to 2^14");
 
-int check_split(int split, double gmax) {
+int check_split(int split, bool bound) {
     if (split != 2 && split != 3 && split != 4)
         return fail(ZTM_EINVAL, "split must be 2, 3 or 4 (got %d)", split);
-    if (split == 4 && !(gmax >= 0.0 && std::isfinite(gmax)))
+    if (split == 4 && !bound)
         return fail(ZTM_EINVAL, "the FP16 split needs the plan's bound on |G| (none for this plan)");
     return ZTM_OK;
 }
@@ -815,9 +814,9 @@ int batched_adj_default_slices(const PmShape& s, int B) {
 
 int batched_fwd_launch(const PmShape& s, int B, int ks, int split, const float* st,
                        const float* P, double thr, double sy, double sz, double scale,
-                       double gmax, void* ws, double* Y, hipStream_t stream) {
+                       const float* gsc, void* ws, double* Y, hipStream_t stream) {
     if (B <= 0 || s.m == 0) return ZTM_OK;
-    if (int rc = check_split(split, gmax)) return rc;
+    if (int rc = check_split(split, gsc != nullptr)) return rc;
     const int nt = batched_nt(B);
     if (nt > 8) return fail(ZTM_EINVAL, "batched forward supports at most 256 models per call (got %d)", B);
     const int Bp = 32 * nt;
@@ -832,11 +831,9 @@ int batched_fwd_launch(const PmShape& s, int B, int ks, int split, const float* 
     __bf16* parts = (__bf16*)ws;
     float* part = (float*)(parts + np * n_packed);
     unsigned* amax = nullptr;
-    float gs = 1.f;
     const dim3 pgrid((unsigned)((s.n_cells + kPackCells - 1) / kPackCells), (unsigned)(Bp / 32));
     if (split == 4) {
         amax = (unsigned*)(part + (size_t)ks * s.m_pad * Bp);
-        gs = g_scale(gmax);
         hipLaunchKernelGGL(batched_clear_kernel, dim3(1), dim3(256), 0, stream, amax, Bp);
         const int64_t blocks = std::min<int64_t>(16, (s.n_cells + 4095) / 4096);
         hipLaunchKernelGGL(batched_absmax_kernel, dim3((unsigned)blocks, (unsigned)B), dim3(256),
@@ -859,7 +856,7 @@ int batched_fwd_launch(const PmShape& s, int B, int ks, int split, const float* 
 #define ZTM_BATCHED_LAUNCH(NT, SP, MODE, H)                                                     \
     hipLaunchKernelGGL((batched_fwd_kernel<NT, SP, MODE, H>), grid, dim3(256), 0, stream,      \
                        (const float4*)st, (const typename Half<H>::v8*)parts, stride, s.nx,     \
-                       s.ny, 1.0 / s.nx, 1.0 / s.ny, (float)sy, (float)sz, gs, (int)n_chunks,   \
+                       s.ny, 1.0 / s.nx, 1.0 / s.ny, (float)sy, (float)sz, gsc, (int)n_chunks,  \
                        per, s.m_pad, part)
 #define ZTM_BATCHED_CASE(NT, SP, H)                                                            \
     if (s.nx % 8 == 0)                                                                         \
@@ -889,7 +886,7 @@ int batched_fwd_launch(const PmShape& s, int B, int ks, int split, const float* 
     }
     hipLaunchKernelGGL(batched_reduce_kernel<double>, dim3((s.m + 7) / 8, (B + 31) / 32), dim3(256), 0,
                        stream, part, ks, s.m, s.m_pad, B, Bp, scale, (const unsigned*)amax,
-                       (double)gs, Y);
+                       split == 4 ? gsc : nullptr, Y);
     ZTM_HIP(hipGetLastError());
     return ZTM_OK;
 }
@@ -920,7 +917,7 @@ int batched_adj_launch(const PmShape& s, int B, int ks, int split, const float* 
                        float* X, hipStream_t stream) {
     if (B <= 0 || s.n_cells == 0) return ZTM_OK;
     if (split != 4 || s.m > 0)
-        if (int rc = check_split(split, gmax)) return rc;
+        if (int rc = check_split(split, gmax >= 0.0 && std::isfinite(gmax))) return rc;
     const int nt = batched_nt(B);
     if (nt > 8) return fail(ZTM_EINVAL, "batched adjoint supports at most 256 vectors per call (got %d)", B);
     if (s.n_cells > (int64_t)INT32_MAX - 2 * kBatchTile)
@@ -954,7 +951,7 @@ int batched_adj_launch(const PmShape& s, int B, int ks, int split, const float* 
     const dim3 pgrid((unsigned)((s.m + kPackCells - 1) / kPackCells), (unsigned)(Bp / 32));
     if (split == 4) {
         amax = (unsigned*)(part + (size_t)ks * n_pad * Bp);
-        gs = g_scale(gmax);
+        gs = half_g_scale(gmax);
         hipLaunchKernelGGL(batched_clear_kernel, dim3(1), dim3(256), 0, stream, amax, Bp);
         const int64_t blocks = std::min<int64_t>(16, ((int64_t)s.m + 4095) / 4096);
         hipLaunchKernelGGL(batched_absmax_kernel, dim3((unsigned)blocks, (unsigned)B), dim3(256),
@@ -1008,15 +1005,15 @@ int batched_adj_launch(const PmShape& s, int B, int ks, int split, const float* 
     if (ks == 1) {
         hipLaunchKernelGGL(batched_transpose_kernel,
                            dim3((unsigned)((s.n_cells + 63) / 64), (B + 31) / 32), dim3(256), 0,
-                           stream, part, (int)s.n_cells, B, Bp, scale, (const unsigned*)amax,
-                           (double)gs, X);
+                           stream, part, (int)s.n_cells, B, Bp, scale / gs,
+                           (const unsigned*)amax, X);
         ZTM_HIP(hipGetLastError());
         return ZTM_OK;
     }
     hipLaunchKernelGGL(batched_reduce_kernel<float>,
                        dim3((unsigned)((s.n_cells + 7) / 8), (B + 31) / 32), dim3(256), 0, stream,
-                       part, ks, (int)s.n_cells, n_pad, B, Bp, scale, (const unsigned*)amax,
-                       (double)gs, X);
+                       part, ks, (int)s.n_cells, n_pad, B, Bp, scale / gs,
+                       (const unsigned*)amax, nullptr, X);
     ZTM_HIP(hipGetLastError());
     return ZTM_OK;
 }

@@ -2,6 +2,8 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <algorithm>
+#include <cmath>
 #include <cstdint>
 #include <string>
 
@@ -292,18 +294,29 @@ int pm_fwd_fields_launch(const PmShape& s, int F, const float4* st, const float*
 size_t batched_workspace(const PmShape& s, int B, int ks, int split);
 int batched_default_slices(const PmShape& s, int B);
 // split: 2 or 3 BF16 parts, or 4 = two FP16 parts scaled by powers of two,
-// which needs gmax >= 0, a bound on |G| over the station-cell pairs (else
-// unused; < 0 = no bound known).
+// which needs gsc [m_pad], the stations' power-of-two G scales
+// (half_g_scale of each station's bound on |G|; else unused, null = none).
 int batched_fwd_launch(const PmShape& s, int B, int ks, int split, const float* st,
                        const float* P, double thr, double sy, double sz, double scale,
-                       double gmax, void* ws, double* Y, hipStream_t stream);
+                       const float* gsc, void* ws, double* Y, hipStream_t stream);
 // Batched adjoint on the same scheme: R float [B][M] -> X float [B][N];
 // ws must hold batched_adj_workspace(s, B, ks, split) bytes.
 size_t batched_adj_workspace(const PmShape& s, int B, int ks, int split);
 int batched_adj_default_slices(const PmShape& s, int B);
+// The adjoint's split 4 takes one G scale, from gmax >= 0, the bound on |G|
+// over all station-cell pairs (< 0 = none).
 int batched_adj_launch(const PmShape& s, int B, int ks, int split, const float* st,
                        const float* R, double sy, double sz, double scale, double gmax, void* ws,
                        float* X, hipStream_t stream);
+
+// Power-of-two G scale of the FP16 parts for a bound gmax on |G|: brings it
+// to at most 2^14; 1 when gmax is 0.
+inline float half_g_scale(double gmax) {
+    if (!(gmax > 0.0)) return 1.f;
+    int e;
+    std::frexp(gmax, &e);   // gmax < 2^e
+    return (float)std::ldexp(1.0, std::max(-120, std::min(120, 14 - e)));
+}
 
 // out = L^T L rho for forward-difference gradients on an nx x ny x nz grid
 // (gfm_reg.hip).
@@ -351,6 +364,7 @@ struct ztm_plan_s {
     size_t part_cap = 0;         // doubles
     float* G = nullptr;          // stored matrix [m][stored_ld], or null
     double gmax = -1.0;          // bound on |G| (kernel units) over station-cell pairs; < 0: none
+    float* st_gs = nullptr;      // [m_pad] per-station FP16 G scales, or null
     void* bws = nullptr;         // batched forward workspace
     size_t bws_cap = 0;          // bytes
 };
