@@ -40,12 +40,18 @@ def test_plan_forward_adjoint_in_graph(dev, formula):
             torch.cuda.synchronize()
 
 
-def test_batched_forward_in_graph(dev):
+@pytest.mark.parametrize("split", [3, 4])
+def test_batched_forward_in_graph(dev, split):
+    """Batched forward (split 3: pack, MFMA kernel, reduce; split 4 adds the
+    scale clear and the absmax with its atomics) captured and replayed 3x
+    with a smaller eager call between replays, bitwise equal to the eager
+    result."""
     import torch
     w = W.pm_cpu_small()
     P = torch.from_numpy(np.stack([W.uniform(5, b, w.grid.n_cells, -300, 300)
                                    for b in range(40)]).astype(np.float32)).to(dev)
     with gfm.Plan(w.grid, w.stations, kappa=w.kappa) as p:
+        p.set_batched_tuning(split=split)
         Y = torch.empty(40, w.n_stations, dtype=torch.float64, device=dev)
         p.forward_batched(P, out=Y)
         torch.cuda.synchronize()
@@ -62,15 +68,18 @@ def test_batched_forward_in_graph(dev):
             torch.cuda.synchronize()
 
 
-def test_batched_adjoint_in_graph(dev):
-    """Batched adjoint (pack, MFMA kernel, reduce: no memset) captured and
-    replayed 3x with a smaller eager call between replays, bitwise equal to
-    the eager result."""
+@pytest.mark.parametrize("split", [3, 4])
+def test_batched_adjoint_in_graph(dev, split):
+    """Batched adjoint (pack, MFMA kernel, reduce: no memset; split 4 adds
+    the scale clear, the absmax and the transpose) captured and replayed 3x
+    with a smaller eager call between replays, bitwise equal to the eager
+    result."""
     import torch
     w = W.pm_cpu_small()
     R = torch.from_numpy(np.stack([W.normal(6, b, w.n_stations, 0.5)
                                    for b in range(40)]).astype(np.float32)).to(dev)
     with gfm.Plan(w.grid, w.stations, kappa=w.kappa) as p:
+        p.set_batched_tuning(split=split)
         X = torch.empty(40, w.grid.n_cells, dtype=torch.float32, device=dev)
         p.adjoint_batched(R, out=X)
         torch.cuda.synchronize()
