@@ -9,6 +9,12 @@
 // with FP32 accumulation:
 //   2-way (hi/lo, ~2^-16):  x0 y0 + x0 y1 + x1 y0                 (3 MFMAs)
 //   3-way (~2^-24):         + x0 y2 + x1 y1 + x2 y0               (6 MFMAs)
+// Split-FP16 (split code 4): hi/lo FP16 parts (~2^-22) on
+// v_mfma_f32_32x32x16_f16, 3 MFMAs. FP16 stops at 65504, so the operands
+// carry powers of two that bring their largest magnitudes to 2^14: each
+// model or residual vector its own (from a device absmax before the pack),
+// G in the forward one per station (the plan's per-station bound on |G|)
+// and in the adjoint one per plan. The reduce takes them out in FP64.
 // Each K-slice's FP32 accumulators are written out and the slices are summed
 // in FP64, which bounds the FP32 accumulation run to one slice (a few hundred
 // MFMA steps by default).
@@ -30,8 +36,10 @@
 //                 of one half-wave are consecutive 16-byte slots in LDS;
 //                 part [KS][m_pad][Bp] FP32;  Y [B][M] FP64 (model-major).
 //                 The adjoint packs R the same way over stations, and writes
-//                 X [B][N] FP32 directly (one K-slice) or through part
-//                 [KS][n_pad][Bp] and the reduce.
+//                 X [B][N] FP32 directly (one K-slice, BF16) or through part
+//                 [KS][n_pad][Bp] and the reduce (one K-slice with FP16:
+//                 the transpose). Split 4 keeps the models' absmax bits
+//                 [256] after the partials.
 #include "gfm_internal.h"
 #include "gfm_pair.h"
 
