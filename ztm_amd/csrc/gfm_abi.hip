@@ -305,8 +305,10 @@ namespace {
 // layer, and |dz| (h^2 + dz^2)^-1.5 rises then falls in |dz| on either side
 // of the station, peaking at |dz| = h / sqrt(2); so the largest value lies
 // within a layer of Z -+ h / sqrt(2), and two layers either side of each are
-// tried. bound[m] is station m's largest value (0 where its G row is 0
-// throughout); the result is the largest over all stations.
+// tried. The batched forward also generates the cells that pad the grid to
+// a multiple of 16 (raster indices N .. N16 - 1, past the last layer), so
+// they are included. bound[m] is station m's largest value (0 where its G
+// row is 0 throughout); the result is the largest over all stations.
 double g_bound(const std::vector<float4>& hf, int64_t n, int nx, int ny, int nz, double sy,
                double sz, std::vector<double>& bound) {
     double best = 0.0;
@@ -327,6 +329,14 @@ double g_bound(const std::vector<float4>& hf, int64_t n, int nx, int ny, int nz,
                 if (r2 > 0.0)
                     bound[m] = std::max(bound[m], std::fabs(dz) / (r2 * std::sqrt(r2)));
             }
+        }
+        const int64_t n_cells = (int64_t)nx * ny * nz;
+        for (int64_t c = n_cells; c < (n_cells + 15) / 16 * 16; ++c) {
+            const int64_t rw = c / nx;
+            const double cx = X - (double)(c - rw * nx), cy = Y - sy * (double)(rw % ny),
+                         cz = Z - sz * (double)(rw / ny);
+            const double r2 = cx * cx + cy * cy + cz * cz;
+            if (r2 > 0.0) bound[m] = std::max(bound[m], std::fabs(cz) / (r2 * std::sqrt(r2)));
         }
         best = std::max(best, bound[m]);
     }

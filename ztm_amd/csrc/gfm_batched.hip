@@ -203,15 +203,21 @@ __device__ __forceinline__ void divmod_pos(int n, int d, double inv, int& q, int
 
 // One G value into element e of the SPLIT parts. FP16 parts take the value
 // times the power-of-two G scale gs (already applied when PRE), held to
-// +-2^15: cells past the grid and the padding station records, whose |G| the
-// scale does not bound, stay finite, so that their products with the
-// zero-padded operand stay 0.
-template <int SPLIT, typename H, bool PRE = false>
+// +-2^15 when CLAMP: the adjoint's padding station records, whose |G| the
+// scale does not bound, then stay finite, so that their products with the
+// zero-padded residuals stay 0. The forward needs no clamp: its per-station
+// bounds cover the cells that pad the grid, and a padding station's row
+// only reaches its own (discarded) outputs. The empty asm makes the parts
+// come from the rounded FP32 value through v_cvt_f16_f32: left to itself the
+// compiler folds the product into v_fma_mix*_f16 conversions, which measured
+// ~2e-5 errors where these give ~1e-6.
+template <int SPLIT, typename H, bool PRE = false, bool CLAMP = true>
 __device__ __forceinline__ void store_split(float gv, float gs,
                                             typename Half<H>::v8 (&a)[SPLIT], int e) {
     if constexpr (Half<H>::scaled) {
         if constexpr (!PRE) gv *= gs;
-        gv = __builtin_amdgcn_fmed3f(gv, -32768.f, 32768.f);
+        if constexpr (CLAMP) gv = __builtin_amdgcn_fmed3f(gv, -32768.f, 32768.f);
+        asm volatile("" : "+v"(gv));
     }
 #pragma unroll
     for (int q = 0; q < SPLIT; ++q) {
@@ -308,7 +314,7 @@ struct BatchedGen {
             j = wj ? 0 : j;
             k += wj ? 1 : 0;
         }
-        store_split<SPLIT, H, MODE != 0>(gv, gs, a, E);
+        store_split<SPLIT, H, MODE != 0, false>(gv, gs, a, E);
     }
 };
 
