@@ -1,6 +1,6 @@
-"""Batched (ensemble) forward on split-BF16 MFMA against the single-model FP32/FP64
-kernels and the FP64 oracle. Tolerance: 1e-5 of max |g| per model (the product
-target); typical is ~1e-6."""
+"""Batched (ensemble) forward and adjoint on split-BF16 and split-FP16 MFMA
+against the single-model FP32/FP64 kernels and the FP64 oracle. Tolerance:
+1e-5 of max |g| per model (the product target); typical is ~1e-6."""
 import numpy as np
 import pytest
 
