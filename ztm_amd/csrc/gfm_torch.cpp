@@ -6,8 +6,9 @@
 //   torch.autograd. ZTM itself differentiates by finite differences, one full
 //   forward model per parameter (Jac, TFM.cpp:124-135).
 // GravityBatchedFn: the same for an ensemble, Y = G P for P float32
-//   [B, N] on split-BF16 MFMA (ztm_plan_forward_batched), backward
-//   G^T dY through the batched adjoint (ztm_plan_adjoint_batched).
+//   [B, N] on split-BF16 or split-FP16 MFMA, as the plan's batched tuning
+//   selects (ztm_plan_forward_batched), backward G^T dY through the batched
+//   adjoint (ztm_plan_adjoint_batched).
 // SmoothnessFn: s = 0.5 ||grad rho||^2 (forward differences), backward
 //   (L^T L rho) * ds, through ztm_plan_smoothness.
 //
@@ -142,8 +143,8 @@ PYBIND11_MODULE(_ztm_torch, m) {
     m.def("gravity", &gravity, "g = G rho (FP64 out); differentiable, backward = G^T grad",
           pybind11::arg("rho"), pybind11::arg("plan"));
     m.def("gravity_batched", &gravity_batched,
-          "Y = G P for P float32 [B, N] (FP64 out) on split-BF16 MFMA; differentiable, "
-          "backward = batched G^T grad",
+          "Y = G P for P float32 [B, N] (FP64 out) on split-BF16/FP16 MFMA (the plan's "
+          "batched tuning); differentiable, backward = batched G^T grad",
           pybind11::arg("rho"), pybind11::arg("plan"));
     m.def("smoothness", &smoothness, "0.5 ||grad rho||^2; differentiable",
           pybind11::arg("rho"), pybind11::arg("plan"));
