@@ -23,6 +23,11 @@
  *   oracle_pm_adjoint    its transpose, looped cells-outer (an independent
  *                        loop nest from the forward, so a transposed index
  *                        fails the dot-product test).
+ *   oracle_pm_forward_cells / oracle_pm_adjoint_cells
+ *                        the same sums restricted to a list of cells (bounded
+ *                        samples of large grids); pinned against the full
+ *                        forward on a density masked to the list, and against
+ *                        the full adjoint's entries, with partition additivity.
  *   oracle_tfm_drho      TFM.cpp:83-89 -- the producer of the gravity path:
  *                        the density perturbation on the fine raster from the
  *                        surface (sea), Moho (M) and velocity (v) grids and

@@ -167,6 +167,31 @@ def test_adjoint_is_transpose_on_anisotropic_grid(orc):
                        orc.pm_adjoint(g, 1.3, st, r)[cells], rtol=0, atol=1e-14)
 
 
+def test_forward_over_listed_cells_is_the_masked_forward(orc):
+    """oracle_pm_forward_cells against the pinned full forward: the sum over a
+    cell list equals the forward of rho masked to that list (a mis-decoded
+    cell or rho read at the list position instead of the cell fails), the full
+    list equals the forward itself, and the sums over the two halves of a
+    shuffled partition add up to the whole."""
+    g = W.Grid(7, 4, 3, 1.0, 2.0, -0.5, 1.5, -0.7, -2.0)
+    st = np.stack([W.uniform(16, a, 11, -3, 9) for a in range(3)], axis=1)
+    rho = W.uniform(16, 7, g.n_cells, -1, 1)
+    full = orc.pm_forward(g, 1.3, rho, st)
+    scale = np.abs(full).max()
+    allc = np.arange(g.n_cells)
+    assert np.abs(orc.pm_forward_cells(g, 1.3, rho, allc, st) - full).max() <= 1e-14 * scale
+    cells = np.array([3, 17, 40, 41, 66, g.n_cells - 1])
+    masked = np.zeros_like(rho)
+    masked[cells] = rho[cells]
+    sub = orc.pm_forward_cells(g, 1.3, rho, cells, st)
+    assert np.abs(sub - orc.pm_forward(g, 1.3, masked, st)).max() <= 1e-14 * scale
+    assert np.abs(sub).max() > 1e-3 * scale
+    perm = (W.uniform(16, 9, g.n_cells) * 1e6).argsort()
+    a, b = perm[: g.n_cells // 3], perm[g.n_cells // 3:]
+    parts = orc.pm_forward_cells(g, 1.3, rho, a, st) + orc.pm_forward_cells(g, 1.3, rho, b, st)
+    assert np.abs(parts - full).max() <= 1e-13 * scale
+
+
 def grad_laplacian_term(rho, shape):
     """Gradient of 0.5 * ||grad rho||^2 with forward differences on each axis."""
     f = rho.reshape(shape)
