@@ -128,9 +128,17 @@ int ztm_gfm_host(int bfr, int di, int dj, int dk, int fh, int fv,
  * time; calls on one plan from several host threads must be serialised. */
 int ztm_plan_create(ztm_plan* plan, const ztm_problem* problem,
                     int64_t n_stations, const double* stations_host);
+/* A plan is reference-counted: create returns it with one reference,
+ * ztm_plan_retain adds one, and ztm_plan_destroy drops one and frees the plan
+ * when none is left. Holders that outlive the creator's reference (the
+ * autograd nodes of the libtorch binding, which run the adjoint after the
+ * caller may have destroyed its plan) retain it. */
+int ztm_plan_retain(ztm_plan plan);
 int ztm_plan_destroy(ztm_plan plan);
 int64_t ztm_plan_num_cells(ztm_plan plan);
 int64_t ztm_plan_num_stations(ztm_plan plan);
+/* The problem the plan was created with (grid, formula, kappa, threshold). */
+int ztm_plan_get_problem(ztm_plan plan, ztm_problem* problem);
 
 /* Forward: g_dev[m] = kappa * sum_n rho[n] K(m,n)   (FP64 output)
  * precision ZTM_FP32: rho_dev is float[N];  ZTM_FP64: rho_dev is double[N]. */

@@ -63,3 +63,15 @@ def test_odd_byte_count(shm):
 def test_bad_geometry_env(shm):
     p = run(shm, ZTM_GFM_DI="abc")
     assert p.returncode == 2
+
+
+def test_no_device_fails_loudly(shm):
+    """Valid files but no usable GPU (none in this process's view): exit status
+    4 (ZTM_ENODEV), the reason on stderr, and no G.dat. TFM ignores the status
+    (TFM.cpp:93-94), so the missing file is what makes the failure visible."""
+    np.zeros(61 * 61 * 65, np.float32).tofile(shm / "R.dat")
+    np.zeros(61 * 61, np.float32).tofile(shm / "S.dat")
+    p = run(shm, HIP_VISIBLE_DEVICES="-1", ROCR_VISIBLE_DEVICES="-1")
+    assert p.returncode == 4, (p.returncode, p.stderr)
+    assert "GFM" in p.stderr
+    assert not (shm / "G.dat").exists()

@@ -143,3 +143,75 @@ def test_gravity_batched_backward_is_batched_adjoint(dev):
     assert np.array_equal(grad, Xb)
     for b in range(B):
         assert np.max(np.abs(grad[b] - a64[b])) < 1e-5 * np.max(np.abs(a64[b])), b
+
+
+def test_backward_outlives_the_plan_object(dev):
+    """The autograd nodes hold their own reference on the plan: backward after
+    the caller's Plan is gone (a temporary, or a `with` block that has exited)
+    runs on the live plan and equals the adjoint of a fresh plan."""
+    import gc
+    import torch
+    w = W.pm_cpu_small()
+    r = W.normal(31, 1, w.n_stations, 0.5)
+    rt = torch.from_numpy(r).to(dev)
+    x = torch.from_numpy(w.rho).to(dev).requires_grad_(True)
+    g = torch_ops.gravity(x, gfm.Plan(w.grid, w.stations, kappa=w.kappa))
+    gc.collect()
+    (g * rt).sum().backward()
+    xc = x.detach().clone().requires_grad_(True)
+    P = W.uniform(31, 2, 40 * w.grid.n_cells, -300, 300).reshape(40, -1)
+    xb = torch.from_numpy(P.astype(np.float32)).to(dev).requires_grad_(True)
+    with gfm.Plan(w.grid, w.stations, kappa=w.kappa) as p:
+        y = torch_ops.gravity(xc, p)
+        Y = torch_ops.gravity_batched(xb, p)
+    gc.collect()
+    (y * rt).sum().backward()
+    Wt = torch.from_numpy(np.stack([W.normal(31, 3 + b, w.n_stations, 0.5)
+                                    for b in range(40)])).to(dev)
+    (Y * Wt).sum().backward()
+    with gfm.Plan(w.grid, w.stations, kappa=w.kappa) as q:
+        want = q.adjoint(rt)
+        want_b = q.adjoint_batched(Wt.float().contiguous())
+    assert torch.equal(x.grad, want)
+    assert torch.equal(xc.grad, want)
+    assert torch.equal(xb.grad, want_b)
+
+
+def test_gradient_is_zero_on_thresholded_cells(dev):
+    """On a plan with threshold > 0 the forward skips |rho| <= threshold, so the
+    gradient there is 0 and G^T r elsewhere: torch.autograd.gradcheck (FP64
+    forward, central differences) on a small grid whose densities keep clear of
+    the threshold, and the masked adjoint on the FP32 path. NaN cells are
+    skipped by the forward and get a zero gradient at threshold 0 too."""
+    import torch
+    g = W.Grid(5, 4, 3, 0.0, 0.0, -600.0, 200.0, 200.0, 200.0)
+    st = np.stack([W.uniform(32, 0, 9, 0, 800), W.uniform(32, 1, 9, 0, 600),
+                   np.full(9, 10.0)], axis=1)
+    rho = W.uniform(32, 2, g.n_cells, -300, 300)
+    thr = 100.0
+    rho[np.abs(np.abs(rho) - thr) < 5.0] = 0.0           # keep clear of the jump
+    small = np.abs(rho) <= thr
+    assert 5 < small.sum() < g.n_cells - 5
+    with gfm.Plan(g, st, kappa=W.KAPPA_SI, threshold=thr) as p:
+        x = torch.from_numpy(rho).to(dev).requires_grad_(True)
+        assert torch.autograd.gradcheck(lambda t: torch_ops.gravity(t, p), (x,), eps=1e-3,
+                                        atol=1e-9, rtol=1e-6)
+        r = torch.from_numpy(W.normal(32, 3, 9, 0.5)).to(dev)
+        x32 = torch.from_numpy(rho.astype(np.float32)).to(dev).requires_grad_(True)
+        (torch_ops.gravity(x32, p) * r).sum().backward()
+        full = p.adjoint(r.float())
+        assert torch.equal(x32.grad[torch.from_numpy(small).to(dev)].abs().sum(),
+                           torch.zeros((), device=dev))
+        keep = torch.from_numpy(~small).to(dev)
+        assert torch.equal(x32.grad[keep], full[keep])
+    with gfm.Plan(g, st, kappa=W.KAPPA_SI) as p:
+        rn = rho.copy()
+        rn[[3, 17]] = np.nan
+        x = torch.from_numpy(rn).to(dev).requires_grad_(True)
+        gz = torch_ops.gravity(x, p)
+        rho0 = rho.copy()
+        rho0[[3, 17]] = 0.0
+        assert torch.equal(gz.detach(), p.forward(torch.from_numpy(rho0).to(dev)))
+        (gz * torch.from_numpy(W.normal(32, 4, 9, 0.5)).to(dev)).sum().backward()
+        gr = x.grad.cpu().numpy()
+        assert gr[3] == 0.0 and gr[17] == 0.0 and np.count_nonzero(gr) == g.n_cells - 2

@@ -94,3 +94,44 @@ def test_batched_adjoint_in_graph(dev, split):
             assert torch.equal(X, X0), rep
             p.adjoint_batched(R[:8].contiguous())
             torch.cuda.synchronize()
+
+
+def test_zero_station_adjoints_in_graph(dev):
+    """Adjoints over a plan with no stations clear their output with a kernel
+    (no memset), so they capture: single-vector FP32 and FP64, the batched
+    adjoint with 3 vectors (the per-vector FP32 path) and with 40, and the
+    stored adjoint; every replay rewrites the zeros over a poisoned output."""
+    import torch
+    g = W.Grid(9, 7, 5, 0.0, 0.0, -500.0, 100.0, 100.0, 100.0)
+    with gfm.Plan(g, np.zeros((0, 3))) as p:
+        r32 = torch.zeros(0, dtype=torch.float32, device=dev)
+        r64 = torch.zeros(0, dtype=torch.float64, device=dev)
+        R3 = torch.zeros(3, 0, dtype=torch.float32, device=dev)
+        R40 = torch.zeros(40, 0, dtype=torch.float32, device=dev)
+        a32 = torch.full((g.n_cells,), 7.0, dtype=torch.float32, device=dev)
+        a64 = torch.full((g.n_cells,), 7.0, dtype=torch.float64, device=dev)
+        X3 = torch.full((3, g.n_cells), 7.0, dtype=torch.float32, device=dev)
+        X40 = torch.full((40, g.n_cells), 7.0, dtype=torch.float32, device=dev)
+        s32 = torch.full((g.n_cells,), 7.0, dtype=torch.float32, device=dev)
+        p.store()
+        outs = (a32, a64, X3, X40, s32)
+
+        def calls():
+            p.adjoint(r32, out=a32)
+            p.adjoint(r64, out=a64)
+            p.adjoint_batched(R3, out=X3)
+            p.adjoint_batched(R40, out=X40)
+            p.adjoint_stored(r32, out=s32)
+
+        calls()
+        torch.cuda.synchronize()
+        assert all(not o.any() for o in outs)
+        graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(graph):
+            calls()
+        for rep in range(3):
+            for o in outs:
+                o.fill_(float(rep + 1))
+            graph.replay()
+            torch.cuda.synchronize()
+            assert all(not o.any() for o in outs), rep

@@ -52,6 +52,52 @@ def test_gf_matches_oracle(dev, orc, geom):
         assert relerr(got[interior] - g0[interior], want[interior] - g0[interior]) < TOL32
 
 
+def test_nan_cells_are_skipped(dev, orc):
+    """NaN densities: GF's test fabs(rho) > .001f is false for NaN
+    (GFM_GPU.cu:19), so the reference skips those cells. ztm_gf, the plan
+    forward (FP32 and FP64, threshold 0 and .001) and the batched forward
+    (MFMA split and the up-to-4-model pass) must give exactly the result with
+    those cells set to 0, and that result must match the oracle."""
+    import torch
+    di, dj, dk, fh, fv, bfr = 23, 17, 6, 1, 2, 3
+    rho, s = ref_case(seed=13, di=di, dj=dj, dk=dk, fh=fh, fv=fv)
+    nan_at = np.unique((W.uniform(13, 9, 40) * rho.size).astype(int))
+    rn = rho.copy()
+    rn[nan_at] = np.nan
+    r0 = rho.copy()
+    r0[nan_at] = 0.0
+    print(f"[nan] gf {di}x{dj}x{dk} nan cells {nan_at.size}", flush=True)
+    want = orc.gf(bfr, di, dj, dk, fh, fv, rn, s)
+    assert np.array_equal(want, orc.gf(bfr, di, dj, dk, fh, fv, r0, s))
+    outs = []
+    for field in (rn, r0):
+        g = torch.zeros(di * dj, dtype=torch.float32, device=dev)
+        gfm.GF(bfr, di, dj, dk, fh, fv, g, torch.from_numpy(field).to(dev),
+               torch.from_numpy(s).to(dev))
+        outs.append(g.cpu().numpy())
+    assert np.array_equal(outs[0], outs[1])
+    assert relerr(outs[0], want) < TOL32
+    w = W.pm_cpu_small()
+    rn = w.rho.copy()
+    cells = np.unique((W.uniform(13, 10, 64) * rn.size).astype(int))
+    rn[cells] = np.nan
+    r0 = w.rho.copy()
+    r0[cells] = 0.0
+    for thr in (0.0, 1e-3):
+        print(f"[nan] plan forward threshold {thr}", flush=True)
+        with gfm.Plan(w.grid, w.stations, kappa=w.kappa, threshold=thr) as p:
+            for dt in (torch.float32, torch.float64):
+                a = p.forward(_torch(rn, dev, dt)).cpu().numpy()
+                b = p.forward(_torch(r0, dev, dt)).cpu().numpy()
+                assert np.array_equal(a, b), (thr, dt)
+            assert relerr(a, orc.pm_forward(w.grid, w.kappa, r0, w.stations)) < TOL64
+            for B in (3, 40):
+                P = np.stack([rn] * B)
+                Y = p.forward_batched(_torch(P, dev, torch.float32)).cpu().numpy()
+                Y0 = p.forward_batched(_torch(np.stack([r0] * B), dev, torch.float32))
+                assert np.array_equal(Y, Y0.cpu().numpy()), (thr, B)
+
+
 def test_gfm_host_and_program(dev, orc):
     di, dj, dk, fh, fv, bfr = 61, 61, 33, 1, 2, 10
     rho, s = ref_case(seed=9)

@@ -98,6 +98,34 @@ using namespace ztm;
 
 namespace ztm {
 
+namespace {
+// Grid-stride clear: 16-byte stores over n16 uint4s at p16, 4-byte stores over
+// n4 words at p4.
+__global__ void zero_fill_kernel(uint4* __restrict__ p16, int64_t n16,
+                                 uint32_t* __restrict__ p4, int64_t n4) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (int64_t i = t; i < n16; i += stride) p16[i] = make_uint4(0u, 0u, 0u, 0u);
+    for (int64_t i = t; i < n4; i += stride) p4[i] = 0u;
+}
+}  // namespace
+
+// Clears `bytes` (a multiple of 4) at p with a kernel, not a memset, so the
+// call can sit inside a captured HIP graph.
+int zero_fill(void* p, size_t bytes, hipStream_t stream) {
+    if (bytes == 0) return ZTM_OK;
+    if (!p || (bytes & 3) || ((uintptr_t)p & 3))
+        return fail(ZTM_EINVAL, "zero_fill needs 4-byte aligned whole words");
+    const int64_t n16 = ((uintptr_t)p & 15) ? 0 : (int64_t)(bytes / 16);
+    const int64_t n4 = (int64_t)((bytes - (size_t)n16 * 16) / 4);
+    const int64_t need = (std::max(n16, n4) + 255) / 256;
+    const int64_t blocks = std::max<int64_t>(1, std::min<int64_t>(need, 1 << 16));
+    hipLaunchKernelGGL(zero_fill_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, (uint4*)p,
+                       n16, (uint32_t*)((char*)p + n16 * 16), n4);
+    ZTM_HIP(hipGetLastError());
+    return ZTM_OK;
+}
+
 int grid_check(const ztm_grid& g) {
     if (g.nx <= 0 || g.ny <= 0 || g.nz <= 0)
         return fail(ZTM_EINVAL, "grid dims must be positive (got %d x %d x %d)", g.nx, g.ny, g.nz);
@@ -450,13 +478,25 @@ int ztm_plan_create(ztm_plan* plan, const ztm_problem* problem, int64_t n_statio
     return ZTM_OK;
 }
 
+int ztm_plan_retain(ztm_plan plan) {
+    if (!plan) return fail(ZTM_EINVAL, "null plan");
+    plan->refs.fetch_add(1, std::memory_order_relaxed);
+    return ZTM_OK;
+}
+
 int ztm_plan_destroy(ztm_plan plan) {
-    plan_free(plan);
+    if (plan && plan->refs.fetch_sub(1, std::memory_order_acq_rel) == 1) plan_free(plan);
     return ZTM_OK;
 }
 
 int64_t ztm_plan_num_cells(ztm_plan plan) { return plan ? plan->shape.n_cells : -1; }
 int64_t ztm_plan_num_stations(ztm_plan plan) { return plan ? plan->shape.m : -1; }
+
+int ztm_plan_get_problem(ztm_plan plan, ztm_problem* problem) {
+    if (!plan || !problem) return fail(ZTM_EINVAL, "null plan or problem");
+    *problem = plan->prob;
+    return ZTM_OK;
+}
 
 int ztm_plan_set_tuning(ztm_plan plan, int st_per_lane, int rows_per_slice,
                         int cells_per_lane, int adj_splits) {
@@ -537,8 +577,7 @@ int ztm_plan_adjoint(ztm_plan plan, int precision, const void* r_dev, void* out_
     if (plan->shape.m == 0) {
         const size_t bytes = (size_t)plan->shape.n_cells *
                              (precision == ZTM_FP64 ? sizeof(double) : sizeof(float));
-        ZTM_HIP(hipMemsetAsync(out_dev, 0, bytes, s));
-        return ZTM_OK;
+        return zero_fill(out_dev, bytes, s);
     }
     if (plan->prob.formula == ZTM_PRISM && precision == ZTM_FP64) {
         if (int rc = ensure_workspace(plan, prism64_workspace(plan->shape, t, true))) return rc;

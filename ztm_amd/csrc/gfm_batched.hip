@@ -75,12 +75,14 @@ constexpr int kHalfTop = 14;
 
 // Power-of-two factor that brings a largest magnitude a (given by its float
 // bits) to at most 2^kHalfTop; 1 for a = 0.
+// The exponent is clamped to +-120, as half_g_scale clamps it, so tiny
+// maxima (below 2^-106) give a finite scale rather than 2^(14 - e) = inf.
 __device__ __forceinline__ float half_scale(unsigned abits) {
     const float a = __uint_as_float(abits);
     if (!(a > 0.f)) return 1.f;
     int e;
     frexpf(a, &e);   // a <= 2^e
-    return ldexpf(1.f, kHalfTop - e);
+    return ldexpf(1.f, max(-120, min(120, kHalfTop - e)));
 }
 
 // amax[b] |= bits of max over n of |X[b][n]| among entries with |X| > thr

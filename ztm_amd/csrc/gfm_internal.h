@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdint>
 #include <string>
@@ -367,10 +368,13 @@ struct ztm_plan_s {
     float* st_gs = nullptr;      // [m_pad] per-station FP16 G scales, or null
     void* bws = nullptr;         // batched forward workspace
     size_t bws_cap = 0;          // bytes
+    std::atomic<int> refs{1};    // ztm_plan_retain / ztm_plan_destroy
 };
 
 namespace ztm {
 // Plan helpers (gfm_abi.hip).
+// Clears bytes (a multiple of 4, 4-byte aligned) with a kernel: graph-safe.
+int zero_fill(void* p, size_t bytes, hipStream_t stream);
 int grid_check(const ztm_grid& g);
 int ensure_workspace(ztm_plan p, size_t doubles);
 void plan_free(ztm_plan p);

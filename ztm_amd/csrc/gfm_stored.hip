@@ -147,7 +147,7 @@ int stored_build_launch(const PmShape& s, bool prism, const double4* st, double 
     if (s.m == 0 || s.n_cells == 0) return ZTM_OK;
     if (s.m > 65535) return fail(ZTM_EINVAL, "stored matrix supports at most 65535 stations");
     const int64_t ld = stored_ld(s);
-    ZTM_HIP(hipMemsetAsync(G, 0, sizeof(float) * (size_t)s.m * ld, stream));
+    if (int rc = zero_fill(G, sizeof(float) * (size_t)s.m * ld, stream)) return rc;
     const dim3 grid((unsigned)((s.n_cells + 255) / 256), (unsigned)s.m);
     {
         ProfScope prof("stored_build", stream);
@@ -182,10 +182,7 @@ int stored_adj_splits(const PmShape& s) {
 int stored_adj_launch(const PmShape& s, const float* G, const float* r, double* part, float* out,
                       hipStream_t stream) {
     if (s.n_cells == 0) return ZTM_OK;
-    if (s.m == 0) {
-        ZTM_HIP(hipMemsetAsync(out, 0, sizeof(float) * (size_t)s.n_cells, stream));
-        return ZTM_OK;
-    }
+    if (s.m == 0) return zero_fill(out, sizeof(float) * (size_t)s.n_cells, stream);
     const int splits = stored_adj_splits(s);
     const int per = (s.m + splits - 1) / splits;
     {

@@ -14,7 +14,17 @@
 //
 // All run on the current HIP stream of rho's device, so they compose with
 // torch work and with stream capture the same way torch's own ops do.
+//
+// Lifetime: each node holds a reference on its plan (ztm_plan_retain, dropped
+// when autograd frees the node), so backward stays valid after the caller has
+// destroyed its own plan object, e.g. a temporary Plan or a `with` block that
+// has exited.
+// Threshold: the forward skips cells with |rho| <= threshold (and NaN cells),
+// so on a plan with threshold > 0 the gradient is 0 on those cells and G^T r
+// elsewhere: the derivative of the forward as computed. With threshold 0 the
+// forward is G rho for every finite rho, and only NaN cells get 0.
 #include <torch/extension.h>
+#include <torch/custom_class.h>
 #include <c10/hip/HIPStream.h>
 
 #include "../../include/ztm_gfm.h"
@@ -46,6 +56,33 @@ ztm_plan as_plan(int64_t h) {
     return reinterpret_cast<ztm_plan>(h);
 }
 
+// One reference on a plan, held by an autograd node's saved data.
+struct PlanRef : torch::CustomClassHolder {
+    ztm_plan plan;
+    double threshold = 0.0;
+    explicit PlanRef(ztm_plan p) : plan(p) {
+        check(ztm_plan_retain(p));
+        ztm_problem prob;
+        if (ztm_plan_get_problem(p, &prob) == ZTM_OK) threshold = prob.threshold;
+    }
+    ~PlanRef() override { ztm_plan_destroy(plan); }
+};
+
+const auto plan_ref_class = torch::class_<PlanRef>("ztm_amd", "PlanRef");
+
+c10::intrusive_ptr<PlanRef> saved_plan(AutogradContext* ctx) {
+    return ctx->saved_data["plan"].toCustomClass<PlanRef>();
+}
+
+// Zero the gradient on the cells the forward skipped (|rho| <= threshold,
+// compared in rho's precision as the kernels compare it, or NaN).
+torch::Tensor mask_skipped(torch::Tensor grad, const torch::Tensor& rho, double threshold) {
+    auto keep = threshold > 0.0
+                    ? rho.abs() > torch::scalar_tensor(threshold, rho.options())
+                    : rho.isnan().logical_not();
+    return torch::where(keep, grad, torch::zeros({}, grad.options()));
+}
+
 class GravityFn : public torch::autograd::Function<GravityFn> {
   public:
     static torch::Tensor forward(AutogradContext* ctx, torch::Tensor rho, int64_t plan) {
@@ -54,20 +91,21 @@ class GravityFn : public torch::autograd::Function<GravityFn> {
         const int prec = precision_of(rho, n, "rho");
         auto g = torch::empty({m}, rho.options().dtype(torch::kFloat64));
         check(ztm_plan_forward(p, prec, rho.data_ptr(), g.data_ptr<double>(), stream_of(rho)));
-        ctx->saved_data["plan"] = plan;
-        ctx->saved_data["dtype"] = static_cast<int64_t>(rho.scalar_type());
+        ctx->saved_data["plan"] = c10::make_intrusive<PlanRef>(p);
+        ctx->save_for_backward({rho});
         return g;
     }
 
     static variable_list backward(AutogradContext* ctx, variable_list grads) {
-        ztm_plan p = as_plan(ctx->saved_data["plan"].toInt());
-        const auto dtype = static_cast<torch::ScalarType>(ctx->saved_data["dtype"].toInt());
+        const auto ref = saved_plan(ctx);
+        ztm_plan p = ref->plan;
+        const auto rho = ctx->get_saved_variables()[0];
         const int64_t n = ztm_plan_num_cells(p), m = ztm_plan_num_stations(p);
-        auto r = grads[0].to(dtype).contiguous();
+        auto r = grads[0].to(rho.scalar_type()).contiguous();
         const int prec = precision_of(r, m, "grad_g");
         auto out = torch::empty({n}, r.options());
         check(ztm_plan_adjoint(p, prec, r.data_ptr(), out.data_ptr(), stream_of(r)));
-        return {out, torch::Tensor()};
+        return {mask_skipped(out, rho, ref->threshold), torch::Tensor()};
     }
 };
 
@@ -84,18 +122,21 @@ class GravityBatchedFn : public torch::autograd::Function<GravityBatchedFn> {
         auto g = torch::empty({B, m}, rho.options().dtype(torch::kFloat64));
         check(ztm_plan_forward_batched(p, (int)B, rho.data_ptr<float>(), g.data_ptr<double>(),
                                        stream_of(rho)));
-        ctx->saved_data["plan"] = plan;
+        ctx->saved_data["plan"] = c10::make_intrusive<PlanRef>(p);
+        ctx->save_for_backward({rho});
         return g;
     }
 
     static variable_list backward(AutogradContext* ctx, variable_list grads) {
-        ztm_plan p = as_plan(ctx->saved_data["plan"].toInt());
+        const auto ref = saved_plan(ctx);
+        ztm_plan p = ref->plan;
+        const auto rho = ctx->get_saved_variables()[0];
         const int64_t n = ztm_plan_num_cells(p);
         auto r = grads[0].to(torch::kFloat32).contiguous();
         auto out = torch::empty({r.size(0), n}, r.options());
         check(ztm_plan_adjoint_batched(p, (int)r.size(0), r.data_ptr<float>(),
                                        out.data_ptr<float>(), stream_of(r)));
-        return {out, torch::Tensor()};
+        return {mask_skipped(out, rho, ref->threshold), torch::Tensor()};
     }
 };
 

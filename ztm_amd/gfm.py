@@ -61,9 +61,11 @@ SIGNATURES = {
     "ztm_gf_async": (_I, [_I, _I, _I, _I, _I, _I, _P, _P, _P, _P]),
     "ztm_gfm_host": (_I, [_I, _I, _I, _I, _I, _I, _P, _P, _I64, _P]),
     "ztm_plan_create": (_I, [ctypes.POINTER(_P), ctypes.POINTER(CProblem), _I64, _P]),
+    "ztm_plan_retain": (_I, [_P]),
     "ztm_plan_destroy": (_I, [_P]),
     "ztm_plan_num_cells": (_I64, [_P]),
     "ztm_plan_num_stations": (_I64, [_P]),
+    "ztm_plan_get_problem": (_I, [_P, ctypes.POINTER(CProblem)]),
     "ztm_plan_forward": (_I, [_P, _I, _P, _P, _P]),
     "ztm_plan_adjoint": (_I, [_P, _I, _P, _P, _P]),
     "ztm_plan_smoothness": (_I, [_P, _I, _P, _P, _P]),
