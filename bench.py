@@ -673,15 +673,36 @@ def prism_leg(w, r, stream, args):
     from ztm_amd import gfm
     dev = torch.device("cuda:0")
     pairs = w.pairs
+    names = ("prism_near_fwd", "pm_fwd", "pm_fwd_reduce", "pm_adj", "pm_adj_reduce")
     with gfm.Plan(w.grid, w.stations, kappa=w.kappa, formula=gfm.PRISM) as p:
         rho32 = torch.from_numpy(w.rho.astype(np.float32)).to(dev)
         r32 = torch.from_numpy(r.astype(np.float32)).to(dev)
         g32 = torch.empty(w.n_stations, dtype=torch.float64, device=dev)
         a32 = torch.empty(w.grid.n_cells, dtype=torch.float32, device=dev)
+        reps = max(2, args.steps // 4)
+        gfm.profile_reset()
+        gfm.profile_enable(True)
         t32 = _timed(lambda: (p.forward(rho32, out=g32, stream=stream),
                               p.adjoint(r32, out=a32, stream=stream)),
-                     max(2, args.steps // 4), torch.cuda.synchronize)
-        out = {"value": 2.0 * pairs / t32, "unit": "pairs/s", "ms_per_step": t32 * 1e3}
+                     reps, torch.cuda.synchronize)
+        gfm.profile_enable(False)
+        kms = {}
+        for name in names:
+            tot, cnt = gfm.profile_read(name)
+            kms[name] = tot / max(cnt, 1)
+        gfm.profile_reset()
+        dom = "pm_adj" if kms["pm_adj"] >= kms["pm_fwd"] else "pm_fwd"
+        achieved = FLOP_PER_PAIR * pairs / (kms[dom] * 1e-3) / 1e12
+        out = {"value": 2.0 * pairs / t32, "unit": "pairs/s", "ms_per_step": t32 * 1e3,
+               "kernels_ms": kms,
+               "kernels_share_of_step": {k: v / (t32 * 1e3) for k, v in kms.items()},
+               "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
+                            "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS,
+                            "traffic": None, "kernel": dom, "flop_per_pair": FLOP_PER_PAIR,
+                            "pairs_per_launch": pairs,
+                            "note": "far-field kernel (point mass on cubic cells) over every "
+                                    "pair; the mid-zone and near-field corrections run in "
+                                    "prism_near_fwd and pm_adj_reduce"}}
         if args.fp64_steps > 0:
             rho64 = torch.from_numpy(w.rho).to(dev)
             r64 = torch.from_numpy(r).to(dev)

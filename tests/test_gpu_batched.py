@@ -213,11 +213,13 @@ def test_batched_adjoint_rejects_and_empty(dev):
         assert not torch.any(out).item()
 
 
-def test_batched_forward_adjoint_full_size(dev):
+def test_batched_forward_adjoint_full_size(dev, orc):
     """ensemble_256 at its full size (96x96x48 cells of 1 km, 9,216 stations,
-    256 models and 256 residual vectors), 3-way split: forward and adjoint
-    against the FP64 single-model kernels on models / vectors 0 and 255
-    (product bar 1e-5 of max |.|), and <G P, R> = <P, G^T R> for every model
+    256 models and 256 residual vectors), default split: forward and adjoint
+    of models / vectors 0, 85, 170 and 255 against the FP64 oracle on a seeded
+    sample (64 stations over all cells; 2,048 cells over all stations) and
+    against the FP64 single-model kernels on 0 and 255 everywhere (product bar
+    1e-5 of max |.|), and <G P, R> = <P, G^T R> for every model
     (size-independent; 1e-5 of sum |G P| |R|)."""
     import torch
     w = W.ensemble_256()
@@ -232,6 +234,13 @@ def test_batched_forward_adjoint_full_size(dev):
             a64 = p.adjoint(torch.from_numpy(R[b].astype(np.float64)).to(dev)).cpu().numpy()
             assert relerr(Y[b], g64) < 1e-5, b
             assert relerr(X[b], a64) < 1e-5, b
+    st = np.unique((W.uniform(15, 1, 64) * w.n_stations).astype(np.int64))
+    cells = np.unique((W.uniform(15, 2, 2048) * w.grid.n_cells).astype(np.int64))
+    for b in (0, 85, 170, 255):
+        g_or = orc.pm_forward(w.grid, w.kappa, w.rho[b].astype(np.float64), w.stations[st])
+        a_or = orc.pm_adjoint_cells(w.grid, w.kappa, w.stations, R[b].astype(np.float64), cells)
+        assert relerr(Y[b][st], g_or) < 1e-5, b
+        assert relerr(X[b][cells], a_or) < 1e-5, b
     R64 = R.astype(np.float64)
     lhs = np.einsum("bm,bm->b", Y, R64)
     rhs = np.einsum("bn,bn->b", w.rho.astype(np.float64), X.astype(np.float64))

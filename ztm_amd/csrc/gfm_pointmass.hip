@@ -998,10 +998,13 @@ int pm_fwd_launch(const PmShape& s, const PmTuning& t0, int precision,
         }
         ZTM_HIP(hipGetLastError());
     }
-    hipLaunchKernelGGL(pm_fwd_reduce_kernel, dim3((s.m + 3) / 4), dim3(256), 0,
-                       stream, part, n_slices, s.m_pad, s.m, scale, g, gref, out_map,
-                       (const float*)rho, na);
-    ZTM_HIP(hipGetLastError());
+    {
+        ProfScope prof("pm_fwd_reduce", stream);
+        hipLaunchKernelGGL(pm_fwd_reduce_kernel, dim3((s.m + 3) / 4), dim3(256), 0,
+                           stream, part, n_slices, s.m_pad, s.m, scale, g, gref, out_map,
+                           (const float*)rho, na);
+        ZTM_HIP(hipGetLastError());
+    }
     return ZTM_OK;
 }
 
@@ -1100,6 +1103,7 @@ int pm_adj_launch(const PmShape& s, const PmTuning& t, int precision,
         ZTM_HIP(hipGetLastError());
     }
     if (!direct) {
+        ProfScope prof("pm_adj_reduce", stream);
         int klo = 0, khi = -1;
         near_layers(na, s.nz, &klo, &khi);
         const int64_t plane = (int64_t)s.nx * s.ny;

@@ -66,6 +66,7 @@ int prism_near_fwd_launch(const PmShape& s, const NearArgs& near, const float* r
                           double* corr, hipStream_t stream) {
     if (s.m == 0) return ZTM_OK;
     const dim3 grid((s.m + 3) / 4);
+    ProfScope prof("prism_near_fwd", stream);
     if (near.quad)
         hipLaunchKernelGGL(prism_near_fwd_kernel<true>, grid, dim3(256), 0, stream, near, rho,
                            s.m, corr);
