@@ -261,9 +261,12 @@ def test_uneven_48_full_size(dev, orc):
       - one problem per size class bitwise equal to a single-problem plan
         with the same tiles;
       - one graph-captured 10-iteration CG solve on the blocks' noisy data cuts
-        every problem's normal-equation residual 100x, and the recovered
-        column mass (sum over depth) peaks over a buried block and
-        correlates with the true one."""
+        every problem's normal-equation residual 100x; the recovered column
+        mass (sum over depth) peaks over a buried block in at least 44 of the
+        48 problems (a smooth solution can peak between two close blocks),
+        correlates with the true column mass (median > 0.5), and every
+        problem recovers the blocks' total excess mass to within 50 %
+        (gravity fixes the total mass, Gauss's law; measured 0.55-1.32)."""
     import torch
     ps = W.uneven_48()
     assert len(ps) == 48
@@ -309,5 +312,6 @@ def test_uneven_48_full_size(dev, orc):
         stats.append((float(np.corrcoef(cm.ravel(), tm.ravel())[0, 1]), peak_on_block,
                       float(xs[i].sum() / p.rho.sum())))
     print("[uneven_48] column-mass corr, peak over a block, mass ratio:", stats, flush=True)
-    assert all(s[1] for s in stats), [i for i, s in enumerate(stats) if not s[1]]
-    assert np.median([s[0] for s in stats]) > 0.4
+    assert sum(s[1] for s in stats) >= 44, [i for i, s in enumerate(stats) if not s[1]]
+    assert np.median([s[0] for s in stats]) > 0.5
+    assert all(0.5 < s[2] < 1.5 for s in stats), [(i, s[2]) for i, s in enumerate(stats)]
