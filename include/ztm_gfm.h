@@ -111,6 +111,18 @@ int ztm_gf(int bfr, int di, int dj, int dk, int fh, int fv,
 int ztm_gf_async(int bfr, int di, int dj, int dk, int fh, int fv,
                  float* g_dev, const float* rho_dev, const float* s_dev,
                  void* stream);
+/* No reference counterpart: ztm_gf_async for callers that cannot allocate
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc). Same arguments, kernels and results (bit for bit), but no call
+ * allocates device or host memory: the station records and split-K partials
+ * live in a fixed 16 MiB device array per GPU, held until the process ends
+ * and looked up once per device (by ztm_init or the first call). ZTM_EINVAL
+ * when the geometry needs more than 16 MiB of it (about 2e5 interior
+ * stations). Calls that overlap in time on one GPU share that array and race:
+ * serialise them. */
+int ztm_gf_noalloc(int bfr, int di, int dj, int dk, int fh, int fv,
+                   float* g_dev, const float* rho_dev, const float* s_dev,
+                   void* stream);
 
 /* ---------------------------------------- Level 2: GFM program, host side */
 /* Replaces the body of the GFM program (GFM_GPU.cu:24-34): host float

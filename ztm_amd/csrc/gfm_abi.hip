@@ -43,9 +43,14 @@ struct ProfRec {
 };
 bool g_prof_on = false;
 std::vector<ProfRec> g_prof;
+thread_local int g_prof_off = 0;
 }  // namespace
 
+ProfTimingOff::ProfTimingOff() { ++g_prof_off; }
+ProfTimingOff::~ProfTimingOff() { --g_prof_off; }
+
 ProfScope::ProfScope(const char* name, hipStream_t s) : name_(name), stream_(s) {
+    if (g_prof_off) return;
     std::lock_guard<std::mutex> lk(g_mu);
     if (!g_prof_on) return;
     hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
@@ -324,7 +329,11 @@ const char* ztm_strerror(int code) {
 
 const char* ztm_last_error(void) { return g_last_error.c_str(); }
 
-int ztm_init(void) { return check_device(nullptr); }
+int ztm_init(void) {
+    int dev = 0;
+    if (int rc = check_device(&dev)) return rc;
+    return noalloc_setup(dev, nullptr);
+}
 
 namespace {
 // Largest |dz| r^-3 over all station-cell pairs, in the kernels' units and
@@ -747,10 +756,8 @@ int ztm_plan_smoothness(ztm_plan plan, int precision, const void* rho_dev, void*
 }
 
 // ------------------------------------------------ reference-signature ABI
-namespace {
-using RefKey = std::tuple<int, int, int, int, int, int, int>;   // dev,bfr,di,dj,dk,fh,fv
-std::map<RefKey, ztm_plan> g_ref_plans;
-
+extern "C++" {
+namespace ztm {
 int ref_check(int bfr, int di, int dj, int dk, int fh, int fv) {
     if (di <= 0 || dj <= 0 || dk <= 0 || fh <= 0 || fv <= 0 || bfr < 0)
         return fail(ZTM_EINVAL, "bad reference geometry bfr=%d di=%d dj=%d dk=%d fh=%d fv=%d",
@@ -760,6 +767,30 @@ int ref_check(int bfr, int di, int dj, int dk, int fh, int fv) {
     if (n > 2147483647.0) return fail(ZTM_EINVAL, "reference grid too large");
     return ZTM_OK;
 }
+
+// The reference geometry's problem and shape (index units, kappa 66.7, the
+// .001f skip, interior stations) in p, without allocating anything.
+int ref_plan_init(ztm_plan p, int dev, int bfr, int di, int dj, int dk, int fh, int fv) {
+    p->device = dev;
+    ztm_problem& pr = p->prob;
+    pr.grid.nx = (di - 1) * fh + 1;
+    pr.grid.ny = (dj - 1) * fh + 1;
+    pr.grid.nz = (dk - 1) * fv + 1;
+    pr.grid.x0 = pr.grid.y0 = pr.grid.z0 = 0.0;
+    pr.grid.dx = pr.grid.dy = 1.0 / fh;
+    pr.grid.dz = 1.0 / fv;
+    pr.formula = ZTM_POINT_MASS;
+    pr.kappa = ZTM_KAPPA_REF;
+    pr.threshold = 0.001f;   // GFM_GPU.cu:19 compares in float against .001f
+    const int ni = std::max(0, di - 2 * bfr), nj = std::max(0, dj - 2 * bfr);
+    return set_shape(p, (int64_t)ni * nj);
+}
+}  // namespace ztm
+}  // extern "C++"
+
+namespace {
+using RefKey = std::tuple<int, int, int, int, int, int, int>;   // dev,bfr,di,dj,dk,fh,fv
+std::map<RefKey, ztm_plan> g_ref_plans;
 
 // One cached plan per (device, geometry); station heights change per call and
 // are re-prepared on the device by prep_ref_stations().
@@ -774,19 +805,7 @@ int ref_plan(int bfr, int di, int dj, int dk, int fh, int fv, ztm_plan* out) {
         return ZTM_OK;
     }
     ztm_plan p = new ztm_plan_s;
-    p->device = dev;
-    ztm_problem& pr = p->prob;
-    pr.grid.nx = (di - 1) * fh + 1;
-    pr.grid.ny = (dj - 1) * fh + 1;
-    pr.grid.nz = (dk - 1) * fv + 1;
-    pr.grid.x0 = pr.grid.y0 = pr.grid.z0 = 0.0;
-    pr.grid.dx = pr.grid.dy = 1.0 / fh;
-    pr.grid.dz = 1.0 / fv;
-    pr.formula = ZTM_POINT_MASS;
-    pr.kappa = ZTM_KAPPA_REF;
-    pr.threshold = 0.001f;   // GFM_GPU.cu:19 compares in float against .001f
-    const int ni = std::max(0, di - 2 * bfr), nj = std::max(0, dj - 2 * bfr);
-    if (int rc = set_shape(p, (int64_t)ni * nj)) { plan_free(p); return rc; }
+    if (int rc = ref_plan_init(p, dev, bfr, di, dj, dk, fh, fv)) { plan_free(p); return rc; }
     hipError_t e = hipMalloc(&p->st_f, sizeof(float4) * p->shape.m_pad);
     if (e == hipSuccess) e = hipMalloc(&p->st_d, sizeof(double4) * p->shape.m_pad);
     if (e == hipSuccess) e = hipMalloc(&p->st_c, sizeof(int4) * p->shape.m_pad);

@@ -373,6 +373,19 @@ struct ztm_plan_s {
 
 namespace ztm {
 // Plan helpers (gfm_abi.hip).
+// Reference geometry (GFM_GPU.cu:16-24): argument check, and the problem and
+// shape of its plan set in p without allocating (no device buffers).
+int ref_check(int bfr, int di, int dj, int dk, int fh, int fv);
+int ref_plan_init(ztm_plan p, int dev, int bfr, int di, int dj, int dk, int fh, int fv);
+// Device address of this device's fixed scratch array (gfm_noalloc.hip),
+// looked up once per device with that device current.
+int noalloc_setup(int dev, char** out);
+// While a ProfTimingOff lives on a thread, ProfScope records no events there
+// (calls that must not create runtime objects).
+struct ProfTimingOff {
+    ProfTimingOff();
+    ~ProfTimingOff();
+};
 // Clears bytes (a multiple of 4, 4-byte aligned) with a kernel: graph-safe.
 int zero_fill(void* p, size_t bytes, hipStream_t stream);
 int grid_check(const ztm_grid& g);

@@ -59,6 +59,7 @@ SIGNATURES = {
     "ztm_init": (_I, []),
     "ztm_gf": (_I, [_I, _I, _I, _I, _I, _I, _P, _P, _P]),
     "ztm_gf_async": (_I, [_I, _I, _I, _I, _I, _I, _P, _P, _P, _P]),
+    "ztm_gf_noalloc": (_I, [_I, _I, _I, _I, _I, _I, _P, _P, _P, _P]),
     "ztm_gfm_host": (_I, [_I, _I, _I, _I, _I, _I, _P, _P, _I64, _P]),
     "ztm_plan_create": (_I, [ctypes.POINTER(_P), ctypes.POINTER(CProblem), _I64, _P]),
     "ztm_plan_retain": (_I, [_P]),
