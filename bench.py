@@ -637,6 +637,28 @@ def tfm_leg(stream, args, reps=20):
                                                t64(w.moho), vel, w.vm, w.ro, w.dr, s32, rB,
                                                ddrho=ddB, out=vjB, stream=stream), 5,
                    torch.cuda.synchronize)
+    # the Moho and velocity grids: the gradient of r . G over every node in one
+    # adjoint pass, and 256 node columns in one tangent call (TFM.cpp:134 runs
+    # one forward model per node)
+    n_nodes = M + M * w.dk
+    t_vjp_grids = _timed(lambda: gfm.tfm_gravity_vjp_grids(w.bfr, w.di, w.dj, w.dk, w.fh, w.fv,
+                                                           w.rc, sea, moho, vel, w.vm, w.ro,
+                                                           w.dr, s32, rB, stream=stream), 5,
+                         torch.cuda.synchronize)
+    dmo = torch.zeros(B, M, dtype=torch.float64, device=dev)
+    dve = torch.zeros(B, M * w.dk, dtype=torch.float64, device=dev)
+    for b in range(B):   # unit directions on 128 Moho and 128 velocity nodes
+        if b < 128:
+            dmo[b, (b * 29) % M] = 1.0
+        else:
+            dve[b, (b * 977) % (M * w.dk)] = 1.0
+    ddG = torch.empty(B, w.n_fine, dtype=torch.float32, device=dev)
+    dgG = torch.empty(B, M, dtype=torch.float64, device=dev)
+    t_jvp_grids = _timed(lambda: gfm.tfm_gravity_jvp_grids(w.bfr, w.di, w.dj, w.dk, w.fh, w.fv,
+                                                           w.rc, sea, moho, vel, w.vm, w.ro,
+                                                           w.dr, s32, dmoho=dmo, dvel=dve,
+                                                           ddrho=ddG, dg=dgG, stream=stream), 3,
+                         torch.cuda.synchronize)
     out = {
         "workload": "tfm_fm: TFM.cpp:83-94 at the reference grid, 241,865 fine cells x 1,681 "
                     "interior stations, producer + gravity per FM call",
@@ -651,6 +673,15 @@ def tfm_leg(stream, args, reps=20):
                                  "note": "exact dG/d(vm, ro, dr): derivative fields + forward"},
         "gravity_vjp_3_params": {"ms_per_call": t_vjp * 1e3,
                                  "note": "r . dG/d(vm, ro, dr): derivative fields + adjoint"},
+        "gravity_vjp_grids": {"ms_per_call": t_vjp_grids * 1e3, "nodes": n_nodes,
+                              "fd_equivalent_ms": n_nodes * t_dev * 1e3,
+                              "note": "d(r . G)/d(Moho, velocity) over every node: one adjoint "
+                                      "+ the transpose of the tangent map; the FD Jacobian "
+                                      "needs one FM call per node"},
+        "gravity_jvp_grids_256": {"ms_per_call": t_jvp_grids * 1e3,
+                                  "ms_per_column": t_jvp_grids * 1e3 / B,
+                                  "note": "256 exact node columns (128 Moho, 128 velocity): "
+                                          "tangent fields + split MFMA ensemble forward"},
         "ms_gfm_program": None if t_prog is None else t_prog * 1e3,
         "cpu_port": {"ms_producer": t_prod_cpu * 1e3,
                      "ms_gravity_thread_per_row": t_grav_rows * 1e3, "rows": nj,

@@ -316,6 +316,41 @@ int ztm_tfm_gravity_vjp(int bfr, int di, int dj, int dk, int fh, int fv, const d
                         double vm, double ro, double dr, const float* s_dev, const double* r_dev,
                         float* ddrho_dev, double* out_dev, void* stream);
 
+/* Exact gravity columns for tangent Moho and velocity grids. drho depends on
+ * the rendered Moho m1 = interp2(M) and velocity v = interp3(V) of each cell
+ * (TFM.cpp:86-88), piecewise linearly, and interp2/interp3 are linear in the
+ * grid values, so at one point (M, V, vm, ro, dr) the derivative along a
+ * tangent direction (dM, dV) is rendered directly: per kept cell (as
+ * ztm_tfm_drho_jvp decides them), d drho = dr dv below the band,
+ * -(dr (v - vm) + ro) dm1 / 2li + dr (1 + t1) dv in it (t1 = (k - m1 + li)
+ * / 2li, li = 1/fv) and 0 above it, with dm1, dv the tangent grids
+ * interpolated like M and V. For n_dirs <= 256 directions dmoho_dev
+ * [n_dirs][di*dj] and dvel_dev [n_dirs][di*dj*dk] (either may be null: no
+ * tangent on that grid): ddrho_dev [n_dirs][N] receives the tangent fields and
+ * dg_dev [n_dirs][di*dj] (FP64, 0 outside the interior) their gravity at the
+ * stations s_dev: one FP32 GF forward for up to 4 directions (FP64 near-field
+ * correction), the split MFMA ensemble forward for more. No reference
+ * counterpart: TFM.cpp:134 builds each node's column by a finite-difference
+ * forward model. Device pointers. */
+int ztm_tfm_gravity_jvp_grids(int bfr, int di, int dj, int dk, int fh, int fv, const double* rc,
+                              const double* sea_dev, const double* moho_dev,
+                              const double* vel_dev, double vm, double ro, double dr,
+                              const float* s_dev, int n_dirs, const double* dmoho_dev,
+                              const double* dvel_dev, float* ddrho_dev, double* dg_dev,
+                              void* stream);
+/* Its adjoint: for a station residual r_dev[di*dj] (FP64, exterior entries
+ * ignored), gmoho_dev[di*dj] = d(r . G)/dM and gvel_dev[di*dj*dk] =
+ * d(r . G)/dV (FP64; either may be null), from one FP32 adjoint G^T r and the
+ * transpose of the tangent-field map (per node, a fixed-order sum over the
+ * fine cells its interpolation stencil reaches). The gradient of the
+ * misfit over every Moho and velocity node in one pass, where TFM.cpp:134
+ * runs one forward model per node. */
+int ztm_tfm_gravity_vjp_grids(int bfr, int di, int dj, int dk, int fh, int fv, const double* rc,
+                              const double* sea_dev, const double* moho_dev,
+                              const double* vel_dev, double vm, double ro, double dr,
+                              const float* s_dev, const double* r_dev, double* gmoho_dev,
+                              double* gvel_dev, void* stream);
+
 /* ------------------------------------------------- uneven batches */
 /* A batch of independent point-mass problems (each its own grid, stations and
  * kappa; FP32 path with the FP64 near-field corrections). Every forward or

@@ -994,6 +994,42 @@ int tfm_gravity_of_fields(ztm_plan p, int bfr, int di, int dj, int fh, int fv, i
 }
 }  // namespace
 
+namespace {
+// Gravity at the interior stations of F already-masked derivative fields
+// fields[F][N] (no threshold): up to 4 fields through one FP32 GF forward
+// that forms each pair's r^-3 once (each field with its FP64 near-field
+// correction; one forward per field when F density tiles of a row do not
+// fit LDS), more on the split MFMA ensemble forward; dg [F][di*dj].
+int tfm_fields_gravity(ztm_plan p, int bfr, int di, int dj, int fh, int fv, int F,
+                       const float* fields, const float* s_dev, double* dg_dev, hipStream_t s) {
+    if (F > 4) return tfm_gravity_of_fields(p, bfr, di, dj, fh, fv, F, fields, 0.0, s_dev, dg_dev,
+                                            s);
+    const int M = di * dj, m = p->shape.m;
+    if (m == 0) return tfm_scatter_launch(nullptr, nullptr, 0, M, F, dg_dev, s);
+    if (int e = prep_ref_stations(s_dev, bfr, di, dj, fh, fv, p->st_f, p->st_d, p->st_c,
+                                  p->out_map, m, p->shape.m_pad, s))
+        return e;
+    if (int rc = ensure_batched_workspace(p, sizeof(double) * F * (size_t)m)) return rc;
+    double* Y = (double*)p->bws;
+    NearArgs na = near_args(p);
+    na.thr = 0.f;
+    const int64_t N = p->shape.n_cells;
+    if (pm_fwd_fields_supported(p->shape, F) && !na.quad) {
+        if (int rc = ensure_workspace(p, pm_fwd_fields_workspace(p->shape, F))) return rc;
+        if (int e = pm_fwd_fields_launch(p->shape, F, (const float4*)p->st_f, fields, N, p->sy,
+                                         p->sz, 0.0, pm_scale(p), p->part, Y, na, s))
+            return e;
+    } else {
+        for (int f = 0; f < F; ++f)
+            if (int e = pm_fwd_launch(p->shape, p->tuning, ZTM_FP32, p->st_f, fields + f * N,
+                                      p->sy, p->sz, 0.0, pm_scale(p), p->part,
+                                      Y + (size_t)f * m, nullptr, nullptr, na, s))
+                return e;
+    }
+    return tfm_scatter_launch(Y, p->out_map, m, M, F, dg_dev, s);
+}
+}  // namespace
+
 int ztm_tfm_gravity_batched(int bfr, int di, int dj, int dk, int fh, int fv, int n_models,
                             const double* rc, const double* sea_dev, const double* moho_dev,
                             int64_t moho_stride, const double* vel_dev, int64_t vel_stride,
@@ -1044,35 +1080,81 @@ int ztm_tfm_gravity_jvp(int bfr, int di, int dj, int dk, int fh, int fv, const d
     if (int e = tfm_drho_jvp_launch(di, dj, dk, fh, fv, rc, sea_dev, moho_dev, vel_dev, vm, ro,
                                     dr, ddrho_dev, s))
         return e;
-    // The three fields through one FP32 GF forward that forms each pair's
-    // r^-3 once (each field with its FP64 near-field correction), or one
-    // forward per field when three density tiles of a row do not fit LDS.
     // The fields are already masked by GF's skip at the parameter point, so
     // neither the main kernel nor the correction thresholds them again.
-    const int M = di * dj, m = p->shape.m;
-    if (m == 0) return tfm_scatter_launch(nullptr, nullptr, 0, M, 3, dg_dev, s);
+    return tfm_fields_gravity(p, bfr, di, dj, fh, fv, 3, ddrho_dev, s_dev, dg_dev, s);
+}
+
+int ztm_tfm_gravity_jvp_grids(int bfr, int di, int dj, int dk, int fh, int fv, const double* rc,
+                              const double* sea_dev, const double* moho_dev,
+                              const double* vel_dev, double vm, double ro, double dr,
+                              const float* s_dev, int n_dirs, const double* dmoho_dev,
+                              const double* dvel_dev, float* ddrho_dev, double* dg_dev,
+                              void* stream) {
+    if (int e = tfm_check(di, dj, dk, fh, fv, rc)) return e;
+    if (int e = ref_check(bfr, di, dj, dk, fh, fv)) return e;
+    if (!sea_dev || !moho_dev || !vel_dev || !s_dev || !ddrho_dev || !dg_dev)
+        return fail(ZTM_EINVAL, "null argument");
+    if (!dmoho_dev && !dvel_dev) return fail(ZTM_EINVAL, "no tangent grid (dmoho and dvel null)");
+    if (n_dirs < 1 || n_dirs > 256)
+        return fail(ZTM_EINVAL, "n_dirs must be in [1, 256] (got %d)", n_dirs);
+    if (!std::isfinite(vm) || !std::isfinite(ro) || !std::isfinite(dr))
+        return fail(ZTM_EINVAL, "vm, ro and dr must be finite");
+    ztm_plan p = nullptr;
+    if (int e = ref_plan(bfr, di, dj, dk, fh, fv, &p)) return e;
+    hipStream_t s = (hipStream_t)stream;
+    if (int e = tfm_drho_jvp_grids_launch(di, dj, dk, fh, fv, rc, sea_dev, moho_dev, vel_dev, vm,
+                                          ro, dr, n_dirs, dmoho_dev, dvel_dev, ddrho_dev, s))
+        return e;
+    return tfm_fields_gravity(p, bfr, di, dj, fh, fv, n_dirs, ddrho_dev, s_dev, dg_dev, s);
+}
+
+int ztm_tfm_gravity_vjp_grids(int bfr, int di, int dj, int dk, int fh, int fv, const double* rc,
+                              const double* sea_dev, const double* moho_dev,
+                              const double* vel_dev, double vm, double ro, double dr,
+                              const float* s_dev, const double* r_dev, double* gmoho_dev,
+                              double* gvel_dev, void* stream) {
+    if (int e = tfm_check(di, dj, dk, fh, fv, rc)) return e;
+    if (int e = ref_check(bfr, di, dj, dk, fh, fv)) return e;
+    if (!sea_dev || !moho_dev || !vel_dev || !s_dev || !r_dev)
+        return fail(ZTM_EINVAL, "null argument");
+    if (!gmoho_dev && !gvel_dev) return fail(ZTM_EINVAL, "no output (gmoho and gvel null)");
+    if (!std::isfinite(vm) || !std::isfinite(ro) || !std::isfinite(dr))
+        return fail(ZTM_EINVAL, "vm, ro and dr must be finite");
+    ztm_plan p = nullptr;
+    if (int e = ref_plan(bfr, di, dj, dk, fh, fv, &p)) return e;
+    hipStream_t s = (hipStream_t)stream;
+    const int64_t N = p->shape.n_cells;
+    const int m = p->shape.m;
+    const size_t M = (size_t)di * dj;
+    if (m == 0) {   // no interior stations: r . G = 0
+        if (gmoho_dev)
+            if (int e = zero_fill(gmoho_dev, sizeof(double) * M, s)) return e;
+        if (gvel_dev)
+            if (int e = zero_fill(gvel_dev, sizeof(double) * M * dk, s)) return e;
+        return ZTM_OK;
+    }
     if (int e = prep_ref_stations(s_dev, bfr, di, dj, fh, fv, p->st_f, p->st_d, p->st_c,
                                   p->out_map, m, p->shape.m_pad, s))
         return e;
-    const size_t need = sizeof(double) * 3 * (size_t)m;
+    // a = G^T r (FP32 adjoint with its near-field pass), then the transpose
+    // of the tangent-field map onto the grid nodes, in FP64
+    const size_t a_bytes = ((size_t)(N + m) * sizeof(float) + 255) / 256 * 256;
+    const size_t need = a_bytes + tfm_grid_vjp_workspace(di, dj, dk, fh, fv);
     if (int rc = ensure_batched_workspace(p, need)) return rc;
-    double* Y = (double*)p->bws;
-    NearArgs na = near_args(p);
-    na.thr = 0.f;
-    const int64_t N = p->shape.n_cells;
-    if (pm_fwd_fields_supported(p->shape, 3) && !na.quad) {
-        if (int rc = ensure_workspace(p, pm_fwd_fields_workspace(p->shape, 3))) return rc;
-        if (int e = pm_fwd_fields_launch(p->shape, 3, (const float4*)p->st_f, ddrho_dev, N, p->sy,
-                                         p->sz, 0.0, pm_scale(p), p->part, Y, na, s))
-            return e;
-    } else {
-        for (int f = 0; f < 3; ++f)
-            if (int e = pm_fwd_launch(p->shape, p->tuning, ZTM_FP32, p->st_f, ddrho_dev + f * N,
-                                      p->sy, p->sz, 0.0, pm_scale(p), p->part,
-                                      Y + (size_t)f * m, nullptr, nullptr, na, s))
-                return e;
-    }
-    return tfm_scatter_launch(Y, p->out_map, m, M, 3, dg_dev, s);
+    float* a = (float*)p->bws;
+    float* r_int = a + N;
+    double* ws = (double*)((char*)p->bws + a_bytes);
+    if (int e = ensure_workspace(p, std::max(pm_fwd_workspace(p->shape, p->tuning, ZTM_FP32, false),
+                                             pm_adj_workspace(p->shape, p->tuning, ZTM_FP32, true,
+                                                              false))))
+        return e;
+    if (int e = tfm_gather_launch(r_dev, p->out_map, m, r_int, s)) return e;
+    if (int e = pm_adj_launch(p->shape, p->tuning, ZTM_FP32, p->st_f, r_int, p->sy, p->sz,
+                              pm_scale(p), p->part, a, near_args(p), s))
+        return e;
+    return tfm_grid_vjp_launch(di, dj, dk, fh, fv, rc, sea_dev, moho_dev, vel_dev, vm, ro, dr, a,
+                               ws, gmoho_dev, gvel_dev, s);
 }
 
 int ztm_tfm_gravity_vjp(int bfr, int di, int dj, int dk, int fh, int fv, const double* rc,

@@ -236,6 +236,21 @@ int tfm_drho_batched_launch(int di, int dj, int dk, int fh, int fv, int B, const
 int tfm_drho_jvp_launch(int di, int dj, int dk, int fh, int fv, const double* rc,
                         const double* sea, const double* moho, const double* vel, double vm,
                         double ro, double dr, float* out, hipStream_t stream);
+// Tangent fields of nv directions on the Moho and velocity grids (dmoho
+// [nv][di*dj], dvel [nv][di*dj*dk], either may be null): out [nv][N] =
+// d drho / d(M, V) applied to them, at one parameter point (float).
+int tfm_drho_jvp_grids_launch(int di, int dj, int dk, int fh, int fv, const double* rc,
+                              const double* sea, const double* moho, const double* vel,
+                              double vm, double ro, double dr, int nv, const double* dmoho,
+                              const double* dvel, float* out, hipStream_t stream);
+// Its transpose for a = G^T r on the fine raster: gmoho [di*dj] and gvel
+// [di*dj*dk] (FP64; either may be null) = the gradients of r . G with
+// respect to the grids; ws holds tfm_grid_vjp_workspace(...) bytes.
+size_t tfm_grid_vjp_workspace(int di, int dj, int dk, int fh, int fv);
+int tfm_grid_vjp_launch(int di, int dj, int dk, int fh, int fv, const double* rc,
+                        const double* sea, const double* moho, const double* vel, double vm,
+                        double ro, double dr, const float* a, double* ws, double* gmoho,
+                        double* gvel, hipStream_t stream);
 // Residual gather to the reference plan's station order, and FP64 dot
 // products out[q] = sum_n a[n] f[q][n] (q < 3) in a fixed order;
 // tfm_vjp_workspace: bytes for a[N], r_int[m] and the dot partials.

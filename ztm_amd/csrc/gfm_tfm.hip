@@ -268,6 +268,196 @@ __global__ __launch_bounds__(256) void tfm_drho_jvp_kernel(
     out[2 * N + n] = (float)d_dr;
 }
 
+// Sensitivity of one cell's drho to its rendered Moho m1 and velocity v at
+// the parameter point: only r1 depends on them (TFM.cpp:86-88), so
+//   above the band (k > m1 + li):  0, 0;
+//   in the band, r1 = dr u + t1 (dr u + ro), t1 = (k - m1 + li) / 2li:
+//     d/dm1 = -(dr u + ro) / 2li,  d/dv = dr (1 + t1);
+//   below it:                      0, dr;
+// and 0 on the cells the producer or GF's skip drops at this point
+// (|float drho| <= .001, as tfm_drho_jvp_kernel decides them).
+struct DrhoSens {
+    double cm, cv;
+};
+
+__device__ __forceinline__ DrhoSens drho_sens(const DrhoCell& c, double m1, double v, double vm,
+                                              double ro, double dr) {
+#pragma clang fp contract(off)
+    DrhoSens d{0.0, 0.0};
+    if (!(fabsf(drho_from(c, m1, v, vm, ro, dr)) > .001f)) return d;
+    const double k = c.k, li = c.li;
+    if (k > m1 + li) return d;
+    if (fabs(k - m1) < li) {
+        const double t1 = (k - m1 + li) / (2 * li);
+        d.cm = -(dr * (v - vm) + ro) / (2 * li);
+        d.cv = dr * (1 + t1);
+    } else {
+        d.cv = dr;
+    }
+    return d;
+}
+
+// Tangent fields for directions b = kDrhoVariants * blockIdx.y onwards:
+// out[b][n] = d drho[n] / d(M, V) . (dmoho[b], dvel[b]), the tangent grids
+// interpolated with the cell's own weights (interp2 / interp3 are linear in
+// the grid values). dmoho [nv][di*dj] and dvel [nv][di*dj*dk] may be null
+// (no tangent on that grid).
+__global__ __launch_bounds__(256) void tfm_drho_jvp_grids_kernel(
+    int di, int dj, int dk, int fh, int fv, TfmFrame fr, const double* __restrict__ sea,
+    const double* __restrict__ moho, const double* __restrict__ vel, double vm, double ro,
+    double dr, int nv, const double* __restrict__ dmoho, const double* __restrict__ dvel,
+    float* __restrict__ out, int64_t N) {
+#pragma clang fp contract(off)
+    const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= N) return;
+    const DrhoCell c = drho_cell(n, di, dj, fh, fv, fr, sea);
+    const Interp2W w2 = interp2_weights(c.i, c.j, di, dj);
+    const Interp3W w3 = interp3_weights(c.i, c.j, c.k, di, dj, dk);
+    const DrhoSens d = drho_sens(c, interp2_eval(moho, w2), interp3_eval(vel, w3), vm, ro, dr);
+    const int64_t M = (int64_t)di * dj;
+    const int b0 = blockIdx.y * kDrhoVariants;
+    const int b1 = min(b0 + kDrhoVariants, nv);
+    for (int b = b0; b < b1; ++b) {
+        const double dm = (dmoho && d.cm != 0.0) ? interp2_eval(dmoho + b * M, w2) : 0.0;
+        const double dv = (dvel && d.cv != 0.0) ? interp3_eval(dvel + b * M * dk, w3) : 0.0;
+        out[(size_t)b * N + n] = (float)(d.cm * dm + d.cv * dv);
+    }
+}
+
+// Transpose, first step: per fine column (fi, fj), with a = G^T r on the
+// fine raster, wm[fi + ii fj] = sum over the column's cells of a cm (the
+// Moho enters per column) and wv[n] = a[n] cv per cell.
+__global__ __launch_bounds__(256) void tfm_grid_vjp_cells_kernel(
+    int di, int dj, int dk, int fh, int fv, TfmFrame fr, const double* __restrict__ sea,
+    const double* __restrict__ moho, const double* __restrict__ vel, double vm, double ro,
+    double dr, const float* __restrict__ a, double* __restrict__ wm, double* __restrict__ wv) {
+#pragma clang fp contract(off)
+    const int ii = (di - 1) * fh + 1, jj = (dj - 1) * fh + 1, kk = (dk - 1) * fv + 1;
+    const int64_t col = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (col >= (int64_t)ii * jj) return;
+    double sm = 0.0;
+    for (int k = 0; k < kk; ++k) {
+        const int64_t n = col + (int64_t)k * ii * jj;
+        const DrhoCell c = drho_cell(n, di, dj, fh, fv, fr, sea);
+        const DrhoSens d = drho_sens(c, interp2_unit(moho, c.i, c.j, di, dj),
+                                     interp3_unit(vel, c.i, c.j, c.k, di, dj, dk), vm, ro, dr);
+        const double an = (double)a[n];
+        sm += an * d.cm;
+        wv[n] = an * d.cv;
+    }
+    wm[col] = sm;
+}
+
+// Weight of coarse node c along one axis for a fine point whose stencil has
+// base corner b and fraction f (1 - f = r): r at b, f at b + 1, else 0.
+__device__ __forceinline__ double axis_weight(int c, int b, double f, double r, bool has_next) {
+    return c == b ? r : (c == b + 1 && has_next) ? f : 0.0;
+}
+
+// Transpose, second step, Moho: gm[c] = sum over the fine columns whose
+// bilinear stencil holds node c of wm * weight (fine points within one
+// coarse cell of the node), in a fixed order.
+__global__ __launch_bounds__(256) void tfm_grid_vjp_moho_kernel(int di, int dj, int fh,
+                                                                const double* __restrict__ wm,
+                                                                double* __restrict__ gm) {
+    const int ii = (di - 1) * fh + 1, jj = (dj - 1) * fh + 1;
+    const int node = blockIdx.x * blockDim.x + threadIdx.x;
+    if (node >= di * dj) return;
+    const int ci = node % di, cj = node / di;
+    const int fi0 = max((ci - 1) * fh, 0), fi1 = min((ci + 1) * fh, ii - 1);
+    const int fj0 = max((cj - 1) * fh, 0), fj1 = min((cj + 1) * fh, jj - 1);
+    double acc = 0.0;
+    for (int fj = fj0; fj <= fj1; ++fj)
+        for (int fi = fi0; fi <= fi1; ++fi) {
+            const Interp2W w = interp2_weights((double)fi / (1. * fh), (double)fj / (1. * fh), di,
+                                               dj);
+            const int bi = (int)(w.b % di), bj = (int)(w.b / di);
+            const double wx = axis_weight(ci, bi, w.si, w.ri, w.hi);
+            const double wy = axis_weight(cj, bj, w.sj, w.rj, w.hj);
+            if (wx != 0.0 && wy != 0.0) acc += wm[fi + (int64_t)ii * fj] * (wx * wy);
+        }
+    gm[node] = acc;
+}
+
+// Velocity: gv[c] = sum over the fine cells whose trilinear stencil holds
+// node c of wv * weight (the stencil's base is clamped to d - 2 per axis, so
+// the last node takes fraction 1 from the last fine points).
+__global__ __launch_bounds__(256) void tfm_grid_vjp_vel_kernel(int di, int dj, int dk, int fh,
+                                                               int fv,
+                                                               const double* __restrict__ wv,
+                                                               double* __restrict__ gv) {
+    const int ii = (di - 1) * fh + 1, jj = (dj - 1) * fh + 1, kk = (dk - 1) * fv + 1;
+    const int64_t node = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (node >= (int64_t)di * dj * dk) return;
+    const int ci = (int)(node % di), cj = (int)((node / di) % dj), ck = (int)(node / ((int64_t)di * dj));
+    const int fi0 = max((ci - 1) * fh, 0), fi1 = min((ci + 1) * fh, ii - 1);
+    const int fj0 = max((cj - 1) * fh, 0), fj1 = min((cj + 1) * fh, jj - 1);
+    const int fk0 = max((ck - 1) * fv, 0), fk1 = min((ck + 1) * fv, kk - 1);
+    double acc = 0.0;
+    for (int fk = fk0; fk <= fk1; ++fk)
+        for (int fj = fj0; fj <= fj1; ++fj)
+            for (int fi = fi0; fi <= fi1; ++fi) {
+                const Interp3W w = interp3_weights((double)fi / (1. * fh), (double)fj / (1. * fh),
+                                                   (double)fk / (1. * fv), di, dj, dk);
+                const int bi = (int)(w.b % di), bj = (int)((w.b / di) % dj);
+                const int bk = (int)(w.b / w.p);
+                const double wx = axis_weight(ci, bi, w.si, w.ri, true);
+                const double wy = axis_weight(cj, bj, w.sj, w.rj, true);
+                const double wz = axis_weight(ck, bk, w.sk, w.rk, true);
+                if (wx != 0.0 && wy != 0.0 && wz != 0.0)
+                    acc += wv[fi + (int64_t)ii * (fj + (int64_t)jj * fk)] * (wx * wy * wz);
+            }
+    gv[node] = acc;
+}
+
+int tfm_drho_jvp_grids_launch(int di, int dj, int dk, int fh, int fv, const double* rc,
+                              const double* sea, const double* moho, const double* vel,
+                              double vm, double ro, double dr, int nv, const double* dmoho,
+                              const double* dvel, float* out, hipStream_t stream) {
+    const int64_t N = ((int64_t)(di - 1) * fh + 1) * ((int64_t)(dj - 1) * fh + 1) *
+                      ((int64_t)(dk - 1) * fv + 1);
+    if (N == 0 || nv <= 0) return ZTM_OK;
+    const TfmFrame fr = tfm_frame(rc);
+    const unsigned gy = (unsigned)((nv + kDrhoVariants - 1) / kDrhoVariants);
+    hipLaunchKernelGGL(tfm_drho_jvp_grids_kernel, dim3((unsigned)((N + 255) / 256), gy),
+                       dim3(256), 0, stream, di, dj, dk, fh, fv, fr, sea, moho, vel, vm, ro, dr,
+                       nv, dmoho, dvel, out, N);
+    ZTM_HIP(hipGetLastError());
+    return ZTM_OK;
+}
+
+size_t tfm_grid_vjp_workspace(int di, int dj, int dk, int fh, int fv) {
+    const int64_t cols = ((int64_t)(di - 1) * fh + 1) * ((int64_t)(dj - 1) * fh + 1);
+    const int64_t N = cols * ((int64_t)(dk - 1) * fv + 1);
+    return sizeof(double) * (size_t)(cols + N);
+}
+
+int tfm_grid_vjp_launch(int di, int dj, int dk, int fh, int fv, const double* rc,
+                        const double* sea, const double* moho, const double* vel, double vm,
+                        double ro, double dr, const float* a, double* ws, double* gmoho,
+                        double* gvel, hipStream_t stream) {
+    const int64_t cols = ((int64_t)(di - 1) * fh + 1) * ((int64_t)(dj - 1) * fh + 1);
+    double* wm = ws;
+    double* wv = ws + cols;
+    const TfmFrame fr = tfm_frame(rc);
+    hipLaunchKernelGGL(tfm_grid_vjp_cells_kernel, dim3((unsigned)((cols + 255) / 256)),
+                       dim3(256), 0, stream, di, dj, dk, fh, fv, fr, sea, moho, vel, vm, ro, dr,
+                       a, wm, wv);
+    ZTM_HIP(hipGetLastError());
+    if (gmoho) {
+        hipLaunchKernelGGL(tfm_grid_vjp_moho_kernel, dim3((unsigned)((di * dj + 255) / 256)),
+                           dim3(256), 0, stream, di, dj, fh, wm, gmoho);
+        ZTM_HIP(hipGetLastError());
+    }
+    if (gvel) {
+        const int64_t nodes = (int64_t)di * dj * dk;
+        hipLaunchKernelGGL(tfm_grid_vjp_vel_kernel, dim3((unsigned)((nodes + 255) / 256)),
+                           dim3(256), 0, stream, di, dj, dk, fh, fv, wv, gvel);
+        ZTM_HIP(hipGetLastError());
+    }
+    return ZTM_OK;
+}
+
 int tfm_drho_jvp_launch(int di, int dj, int dk, int fh, int fv, const double* rc,
                         const double* sea, const double* moho, const double* vel, double vm,
                         double ro, double dr, float* out, hipStream_t stream) {

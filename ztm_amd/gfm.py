@@ -90,6 +90,10 @@ SIGNATURES = {
                                  _P, _P]),
     "ztm_tfm_gravity_vjp": (_I, [_I, _I, _I, _I, _I, _I, _P, _P, _P, _P, _D, _D, _D, _P, _P,
                                  _P, _P, _P]),
+    "ztm_tfm_gravity_jvp_grids": (_I, [_I, _I, _I, _I, _I, _I, _P, _P, _P, _P, _D, _D, _D, _P,
+                                       _I, _P, _P, _P, _P, _P]),
+    "ztm_tfm_gravity_vjp_grids": (_I, [_I, _I, _I, _I, _I, _I, _P, _P, _P, _P, _D, _D, _D, _P,
+                                       _P, _P, _P, _P]),
     "ztm_batch_create": (_I, [ctypes.POINTER(_P), _I, ctypes.POINTER(CProblem),
                               ctypes.POINTER(_I64), ctypes.POINTER(_P)]),
     "ztm_batch_destroy": (_I, [_P]),
@@ -310,6 +314,64 @@ def tfm_gravity_vjp(bfr, di, dj, dk, fh, fv, rc, sea, moho, vel, vm, ro, dr, s, 
                                     _ptr(moho), _ptr(vel), vm, ro, dr, _ptr(s), _ptr(r),
                                     _ptr(ddrho), _ptr(out), _stream(stream)))
     return out, ddrho
+
+
+def tfm_gravity_jvp_grids(bfr, di, dj, dk, fh, fv, rc, sea, moho, vel, vm, ro, dr, s,
+                          dmoho=None, dvel=None, ddrho=None, dg=None, stream=None):
+    """dG along n_dirs tangent directions of the Moho grid (dmoho float64
+    [n_dirs, di*dj]) and / or the velocity grid (dvel [n_dirs, di*dj*dk]) at
+    one point; sea/moho [di*dj], vel [di*dj*dk] float64, s float32 [di*dj],
+    all on the device. Returns (dg float64 [n_dirs, di*dj], ddrho float32
+    [n_dirs, N])."""
+    import torch
+    rc = _host_f64(rc, 7, "rc") if np.size(rc) == 7 else _host_f64(rc, 10, "rc")
+    n = ((di - 1) * fh + 1) * ((dj - 1) * fh + 1) * ((dk - 1) * fv + 1)
+    M = di * dj
+    for name, t, size in (("sea", sea, M), ("moho", moho, M), ("vel", vel, M * dk)):
+        _check_t(t, torch.float64, size, name)
+    _check_t(s, torch.float32, M, "s")
+    if dmoho is None and dvel is None:
+        raise ValueError("give dmoho and / or dvel")
+    nd = int((dmoho if dmoho is not None else dvel).numel() //
+             (M if dmoho is not None else M * dk))
+    if dmoho is not None:
+        _check_t(dmoho, torch.float64, nd * M, "dmoho")
+    if dvel is not None:
+        _check_t(dvel, torch.float64, nd * M * dk, "dvel")
+    dev = sea.device
+    if ddrho is None:
+        ddrho = torch.empty(nd, n, dtype=torch.float32, device=dev)
+    if dg is None:
+        dg = torch.empty(nd, M, dtype=torch.float64, device=dev)
+    _check_t(ddrho, torch.float32, nd * n, "ddrho")
+    _check_t(dg, torch.float64, nd * M, "dg")
+    check(lib().ztm_tfm_gravity_jvp_grids(
+        bfr, di, dj, dk, fh, fv, rc.ctypes.data, _ptr(sea), _ptr(moho), _ptr(vel), vm, ro, dr,
+        _ptr(s), nd, _ptr(dmoho) if dmoho is not None else None,
+        _ptr(dvel) if dvel is not None else None, _ptr(ddrho), _ptr(dg), _stream(stream)))
+    return dg, ddrho
+
+
+def tfm_gravity_vjp_grids(bfr, di, dj, dk, fh, fv, rc, sea, moho, vel, vm, ro, dr, s, r,
+                          moho_grad=True, vel_grad=True, stream=None):
+    """Gradients of r . G with respect to the Moho grid (float64 [di*dj]) and
+    the velocity grid (float64 [di*dj*dk]) for a station residual r (float64
+    [di*dj] on the device; exterior entries ignored). Returns (gmoho, gvel),
+    None where not asked for."""
+    import torch
+    rc = _host_f64(rc, 7, "rc") if np.size(rc) == 7 else _host_f64(rc, 10, "rc")
+    M = di * dj
+    for name, t, size in (("sea", sea, M), ("moho", moho, M), ("vel", vel, M * dk),
+                          ("r", r, M)):
+        _check_t(t, torch.float64, size, name)
+    _check_t(s, torch.float32, M, "s")
+    gm = torch.empty(M, dtype=torch.float64, device=sea.device) if moho_grad else None
+    gv = torch.empty(M * dk, dtype=torch.float64, device=sea.device) if vel_grad else None
+    check(lib().ztm_tfm_gravity_vjp_grids(
+        bfr, di, dj, dk, fh, fv, rc.ctypes.data, _ptr(sea), _ptr(moho), _ptr(vel), vm, ro, dr,
+        _ptr(s), _ptr(r), _ptr(gm) if gm is not None else None,
+        _ptr(gv) if gv is not None else None, _stream(stream)))
+    return gm, gv
 
 
 # ------------------------------------------------------------------ plans
