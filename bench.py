@@ -92,7 +92,8 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the stored-matrix and prism legs")
-    ap.add_argument("--leg", choices=["prism", "batched", "uneven", "tfm"], default=None,
+    ap.add_argument("--leg", choices=["prism", "batched", "uneven", "tfm", "accumulation"],
+                    default=None,
                     help="run only this extra leg and print its JSON object (for profiling)")
     ap.add_argument("--copy-calib-mib", type=int, default=0,
                     help="only run 5 device-to-device copies of this many MiB "
@@ -119,8 +120,9 @@ def main() -> int:
         torch.cuda.synchronize()
         print(json.dumps({"copy_calibration_bytes": 4 * n, "copies": 5}))
         return 0
-    if args.leg in ("batched", "uneven", "tfm"):
-        leg = {"batched": batched_leg, "uneven": uneven_leg, "tfm": tfm_leg}[args.leg]
+    if args.leg in ("batched", "uneven", "tfm", "accumulation"):
+        leg = {"batched": batched_leg, "uneven": uneven_leg, "tfm": tfm_leg,
+               "accumulation": accumulation_leg}[args.leg]
         print(json.dumps({args.leg: leg(torch.cuda.current_stream(), args)}))
         return 0
     w = W.pm_blobs_1m()
@@ -693,6 +695,33 @@ def tfm_leg(stream, args, reps=20):
     }
     if t_prog is not None:
         out["gfm_program_equals_device"] = bool(np.array_equal(G_prog.astype(np.float64), g_dev))
+    return out
+
+
+def accumulation_leg(stream, args):
+    """Error of the ensemble forward against the K-slice count: each slice is
+    one FP32 accumulation run on the MFMAs (N / 16 / slices chunk steps),
+    summed across slices in FP64. ensemble_256 at half size (48x48x24 cells,
+    2,304 stations), 64 models; error = max over models 0, 21, 63 of
+    max |Y - Y64| / max |Y64| against the FP64 single-model forward."""
+    import torch
+    from ztm_amd import gfm, workloads as W
+    dev = torch.device("cuda:0")
+    w = W.ensemble_256(n=48, nz=24)
+    Pt = torch.from_numpy(w.rho[:64].copy()).to(dev)
+    n_chunks = (w.grid.n_cells + 15) // 16
+    out = {"workload": "ensemble_256 half size: 48x48x24 cells, 2,304 stations, 64 models",
+           "chunks": n_chunks, "rows": []}
+    with gfm.Plan(w.grid, w.stations, kappa=w.kappa) as p:
+        ref = {b: p.forward(torch.from_numpy(w.rho[b].astype(np.float64)).to(dev),
+                            stream=stream).cpu().numpy() for b in (0, 21, 63)}
+        for ks in (1, 2, 4, 9, 27, 81, 243, n_chunks):
+            row = {"slices": ks, "chunks_per_run": -(-n_chunks // ks)}
+            for split, name in ((3, "split3"), (4, "fp16x2"), (2, "split2")):
+                p.set_batched_tuning(slices=ks, split=split)
+                Y = p.forward_batched(Pt, stream=stream).cpu().numpy()
+                row[name] = max(relerr(Y[b], r) for b, r in ref.items())
+            out["rows"].append(row)
     return out
 
 

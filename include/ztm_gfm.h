@@ -212,16 +212,27 @@ int ztm_plan_forward_batched(ztm_plan plan, int n_models, const float* rho_dev,
  * ztm_plan_adjoint (FP32) per vector: the transpose of that forward. */
 int ztm_plan_adjoint_batched(ztm_plan plan, int n_vectors, const float* r_dev,
                              float* out_dev, void* stream);
-/* Batched forward tuning: K-slices (0 = choose from the problem shape) and
+/* Batched tuning: K-slices (0 = choose from the problem shape), in both
+ * directions: each slice accumulates in FP32 on the MFMAs, whose accumulate
+ * loses a biased fraction of an ulp per 16-cell step, so the error grows
+ * with the run length (about 3e-9 of max |.| per step; the default keeps a
+ * run at most 384 steps), and the slices are summed in FP64; and
  * split (0 = keep): 2 or 3 BF16 parts, or 4 = two FP16 parts, G scaled by
  * powers of two from the plan's bounds on |G| and each model / residual vector
  * by a power of two from its largest entry (22 significant bits in 3 MFMA
- * products, against 24 in the BF16 three-way split's 6). The forward
- * scales G per station; the adjoint takes one scale per plan, so there
- * stations whose largest |G| lies 2^17 or more below the plan's keep fewer
- * bits. The split applies to the batched adjoint too. ZTM_EINVAL for other
- * values. No reference counterpart (the reference has no batched path). */
+ * products, against 24 in the BF16 three-way split's 6). Plans made by
+ * ztm_plan_create start at split 4; the reference-geometry plans of the
+ * ztm_tfm_* calls, whose stations change per call, at 3. The forward
+ * scales G per station. The adjoint takes one G scale per plan (stations are
+ * its MFMA K dimension), so it runs split 4 only where the per-station bounds
+ * on |G| lie within 2^19 / M of each other (stations at similar heights over
+ * the grid; the FP16 error then stays near 2^-20 of max |G^T r|), and the
+ * BF16 three-way split on other plans. ZTM_EINVAL for other values. No
+ * reference counterpart (the reference has no batched path). */
 int ztm_plan_set_batched_tuning(ztm_plan plan, int slices, int split);
+/* The batched tuning in force: K-slices (0 = automatic), the split of the
+ * forward, and the split the adjoint runs on this plan. */
+int ztm_plan_get_batched_tuning(ztm_plan plan, int* slices, int* split, int* adjoint_split);
 
 /* Smoothness operator of the regularised misfit 0.5||G rho-d||^2 +
  * 0.5 lambda ||grad rho||^2 (forward differences on each axis, none across

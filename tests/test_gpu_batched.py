@@ -48,14 +48,16 @@ def test_batched_ragged(dev, orc):
 
 
 def test_batched_256_half_size(dev):
-    """256 models, 48x48x24 cells of 1 km, 2,304 stations at z = +10 m: both
-    splits and a long-slice setting against the FP64 single-model kernel on 4
-    models; repeated runs are bitwise equal."""
+    """256 models, 48x48x24 cells of 1 km, 2,304 stations at z = +10 m: the
+    default (FP16 hi/lo), BF16 2-way and 3-way splits and a long-slice setting
+    against the FP64 single-model kernel on 4 models; repeated runs are
+    bitwise equal."""
     import torch
     w = W.ensemble_256(n=48, nz=24)
     Pt = torch.from_numpy(w.rho).to(dev)
     with gfm.Plan(w.grid, w.stations, kappa=w.kappa) as p:
-        Y = p.forward_batched(Pt).cpu().numpy()            # default: 3-way split
+        assert p.batched_tuning() == (0, 4, 4)             # stations at one height
+        Y = p.forward_batched(Pt).cpu().numpy()            # default: FP16 hi/lo split
         Y2 = p.forward_batched(Pt).cpu().numpy()
         p.set_batched_tuning(split=2)
         Yh = p.forward_batched(Pt).cpu().numpy()
@@ -397,12 +399,14 @@ def test_batched_fp16_split_adjoint_split_k(dev, orc):
     m = 13000
     st = np.stack([rng.uniform(c[:, 0].min(), c[:, 0].max(), m),
                    rng.uniform(c[:, 1].min(), c[:, 1].max(), m),
-                   rng.uniform(c[:, 2].max() + 25.0, c[:, 2].max() + 400.0, m)], axis=1)
+                   rng.uniform(c[:, 2].max() + 25.0, c[:, 2].max() + 30.0, m)], axis=1)
     R = (rng.normal(0.0, 1.0, (40, m)) * 10.0 ** rng.uniform(-3, 3, (40, 1))).astype(np.float32)
     Rt = torch.from_numpy(R).to(dev)
     with gfm.Plan(g, st, kappa=W.KAPPA_SI) as p:
+        p.set_batched_tuning(split=3)
         X3 = p.adjoint_batched(Rt).cpu().numpy()
         p.set_batched_tuning(split=4)
+        assert p.batched_tuning()[2] == 4
         X = p.adjoint_batched(Rt).cpu().numpy()
     for b in (0, 21, 39):
         want = orc.pm_adjoint(g, W.KAPPA_SI, st, R[b].astype(np.float64))
@@ -438,8 +442,9 @@ def test_batched_fp16_split_mixed_heights(dev, orc):
     stations 2,000 cells above it (their largest |G| differ by ~2^23.6): the
     forward's per-station G scales keep the far stations' results to 1e-5 of
     their own largest value (an FP16 emulation with one plan-wide scale gives
-    1.3e-4 there, per-station scales 2.0e-6); the adjoint against the
-    oracle."""
+    1.3e-4 there, per-station scales 2.0e-6); the adjoint, whose one G scale
+    cannot span that range, runs the BF16 three-way split on this plan, also
+    with residuals on the far stations only."""
     import torch
     g = W.Grid(40, 40, 10, 0.0, 0.0, 0.0, 50.0, 50.0, 50.0)
     c = g.centres()
@@ -452,13 +457,73 @@ def test_batched_fp16_split_mixed_heights(dev, orc):
     far = np.arange(m) % 2 == 1
     P = rng.uniform(-300, 300, (8, g.n_cells)).astype(np.float32)
     Rv = rng.normal(0.0, 1.0, (8, m)).astype(np.float32)
+    Rf = Rv * far
     with gfm.Plan(g, st, kappa=W.KAPPA_SI) as p:
         p.set_batched_tuning(split=4)
+        assert p.batched_tuning() == (0, 4, 3)
         Y = p.forward_batched(torch.from_numpy(P).to(dev)).cpu().numpy()
         X = p.adjoint_batched(torch.from_numpy(Rv).to(dev)).cpu().numpy()
+        Xf = p.adjoint_batched(torch.from_numpy(Rf).to(dev)).cpu().numpy()
     for b in (0, 7):
         want = orc.pm_forward(g, W.KAPPA_SI, P[b].astype(np.float64), st)
         assert relerr(Y[b], want) < 1e-5, b
         assert relerr(Y[b][far], want[far]) < 1e-5, b
         want = orc.pm_adjoint(g, W.KAPPA_SI, st, Rv[b].astype(np.float64))
         assert relerr(X[b], want) < 1e-5, b
+        want = orc.pm_adjoint(g, W.KAPPA_SI, st, Rf[b].astype(np.float64))
+        assert relerr(Xf[b], want) < 5e-6, b
+
+
+def test_batched_fp16_adjoint_far_residuals(dev, orc):
+    """Residuals on far stations only. Stations at one height over the grid
+    (bounds on |G| within a factor 3): the FP16 adjoint stays on and meets
+    5e-6 of max |G^T r| against the oracle with residuals only in one corner
+    of the station grid (the split itself adds ~1e-7 in an FP16 emulation;
+    the rest is the MFMA's FP32 accumulation). The same stations plus one
+    10^-3 cell from a cell centre (its bound ~10^5 times the others', outside
+    the FP16 adjoint's range): the adjoint runs the BF16 three-way split and
+    meets the oracle with the near station's residual 0."""
+    import torch
+    g = W.Grid(40, 40, 10, 0.0, 0.0, -25.0, 50.0, 50.0, -50.0)
+    c = g.centres()
+    xs = np.linspace(c[:, 0].min(), c[:, 0].max(), 30)
+    X0, Y0 = np.meshgrid(xs, xs)
+    st = np.stack([X0.ravel(), Y0.ravel(), np.full(X0.size, 10.0)], axis=1)
+    corner = (st[:, 0] > c[:, 0].max() - 300.0) & (st[:, 1] > c[:, 1].max() - 300.0)
+    rng = np.random.default_rng(12)
+    R = (rng.normal(0.0, 1.0, (40, st.shape[0])) * corner).astype(np.float32)
+    with gfm.Plan(g, st, kappa=W.KAPPA_SI) as p:
+        assert p.batched_tuning() == (0, 4, 4)
+        X = p.adjoint_batched(torch.from_numpy(R).to(dev)).cpu().numpy()
+    for b in (0, 39):
+        want = orc.pm_adjoint(g, W.KAPPA_SI, st, R[b].astype(np.float64))
+        assert relerr(X[b], want) < 5e-6, b
+    near = np.array([[c[1000, 0] + 0.05, c[1000, 1], c[1000, 2] + 0.02]])
+    st2 = np.concatenate([st, near])
+    R2 = np.concatenate([R, np.zeros((40, 1), np.float32)], axis=1)
+    with gfm.Plan(g, st2, kappa=W.KAPPA_SI) as p:
+        assert p.batched_tuning() == (0, 4, 3)
+        X2 = p.adjoint_batched(torch.from_numpy(R2).to(dev)).cpu().numpy()
+    for b in (0, 39):
+        want = orc.pm_adjoint(g, W.KAPPA_SI, st2, R2[b].astype(np.float64))
+        assert relerr(X2[b], want) < 5e-6, b
+
+
+def test_batched_fp16_tiny_models(dev, orc):
+    """Split code 4 on models whose largest entry is ~1e-36 (the power-of-two
+    model scale is held to 2^120, so the parts stay finite) beside ordinary
+    ones: every model against the oracle."""
+    import torch
+    w = W.pm_cpu_small()
+    P = models(19, 40, w.grid.n_cells).astype(np.float64)
+    P[3] *= 1e-38
+    P[17] *= 1e-30
+    P = P.astype(np.float32)
+    assert P[3].any()
+    with gfm.Plan(w.grid, w.stations, kappa=w.kappa) as p:
+        p.set_batched_tuning(split=4)
+        Y = p.forward_batched(torch.from_numpy(P).to(dev)).cpu().numpy()
+    assert np.isfinite(Y).all()
+    for b in (0, 3, 17):
+        want = orc.pm_forward(w.grid, w.kappa, P[b].astype(np.float64), w.stations)
+        assert relerr(Y[b], want) < 1e-5, b

@@ -293,6 +293,11 @@ NearArgs near_args(const ztm_plan p) {
     return a;
 }
 
+int batched_adjoint_split(const ztm_plan p) {
+    const int split = p->tuning.batched_split;
+    return split == 4 && !p->fp16_adjoint ? 3 : split;
+}
+
 double pm_scale(const ztm_plan p) {
     // kappa * V / L^2 turns  sum rho dz'/r'^3  (cell units) into g.
     const ztm_grid& g = p->prob.grid;
@@ -425,6 +430,25 @@ int ztm_plan_create(ztm_plan* plan, const ztm_problem* problem, int64_t n_statio
     }
     std::vector<double> gb;
     p->gmax = g_bound(hf, n_stations, g.nx, g.ny, g.nz, p->sy, p->sz, gb);
+    // The FP16 adjoint scales G by one power of two per plan (its stations are
+    // the MFMA's K dimension), 2^14 / gmax. A term below the FP16 normal range
+    // then keeps an absolute error of at most 2^-25 in scaled units, i.e.
+    // 2^-39 gmax in G units; over M stations that is M 2^-39 gmax per unit of
+    // residual, against |G^T r| ~ bmin max|r| at the cell under the largest
+    // residual (bmin = the smallest per-station bound). So the FP16 split stays
+    // within ~2^-20 of max|G^T r| while gmax / bmin <= 2^19 / M: stations at
+    // similar heights over the grid. Plans with a wider spread (stations at
+    // very different heights, or one almost on a cell centre) run the batched
+    // adjoint in the BF16 three-way split instead, whose parts keep the FP32
+    // exponent.
+    {
+        double bmin = 0.0;
+        for (double b : gb)
+            if (b > 0.0) bmin = bmin > 0.0 ? std::min(bmin, b) : b;
+        p->fp16_adjoint = n_stations == 0 || p->gmax <= 0.0 ||
+                          p->gmax <= bmin * std::ldexp(1.0, 19) / (double)n_stations;
+    }
+    p->tuning.batched_split = 4;
     // per-station FP16 G scales of the batched forward (padding records: 1)
     std::vector<float> hgs(p->shape.m_pad, 1.f);
     for (int64_t m = 0; m < n_stations; ++m) hgs[m] = half_g_scale(gb[m]);
@@ -723,13 +747,23 @@ int ztm_plan_adjoint_batched(ztm_plan plan, int n_vectors, const float* r_dev, f
                 return rc;
         return ZTM_OK;
     }
-    const int ks = batched_adj_default_slices(plan->shape, n_vectors);
-    const int split = plan->tuning.batched_split;
+    const int ks = plan->tuning.batched_slices > 0
+                       ? plan->tuning.batched_slices
+                       : batched_adj_default_slices(plan->shape, n_vectors);
+    const int split = batched_adjoint_split(plan);
     const size_t need = batched_adj_workspace(plan->shape, n_vectors, ks, split);
     if (int rc = ensure_batched_workspace(plan, need)) return rc;
     return batched_adj_launch(plan->shape, n_vectors, ks, split, (const float*)plan->st_f, r_dev,
                               plan->sy, plan->sz, pm_scale(plan), plan->gmax, plan->bws, out_dev,
                               (hipStream_t)stream);
+}
+
+int ztm_plan_get_batched_tuning(ztm_plan plan, int* slices, int* split, int* adjoint_split) {
+    if (!plan) return fail(ZTM_EINVAL, "null plan");
+    if (slices) *slices = plan->tuning.batched_slices;
+    if (split) *split = plan->tuning.batched_split;
+    if (adjoint_split) *adjoint_split = batched_adjoint_split(plan);
+    return ZTM_OK;
 }
 
 int ztm_plan_set_batched_tuning(ztm_plan plan, int slices, int split) {

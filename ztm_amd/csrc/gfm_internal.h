@@ -118,6 +118,7 @@ struct PmTuning {
     int adj_splits = 0;       // adjoint: 0 = choose from the problem shape
     int batched_slices = 0;   // batched forward K-slices: 0 = choose
     int batched_split = 3;    // batched parts: 2 or 3 BF16, or 4 = FP16 hi/lo
+                              // (plans made by ztm_plan_create start at 4)
 };
 
 struct PmShape {
@@ -380,6 +381,7 @@ struct ztm_plan_s {
     size_t part_cap = 0;         // doubles
     float* G = nullptr;          // stored matrix [m][stored_ld], or null
     double gmax = -1.0;          // bound on |G| (kernel units) over station-cell pairs; < 0: none
+    bool fp16_adjoint = false;   // the FP16 split is accurate for the batched adjoint here
     float* st_gs = nullptr;      // [m_pad] per-station FP16 G scales, or null
     void* bws = nullptr;         // batched forward workspace
     size_t bws_cap = 0;          // bytes
@@ -409,5 +411,8 @@ void plan_free(ztm_plan p);
 int set_shape(ztm_plan p, int64_t m);
 NearArgs near_args(const ztm_plan p);
 double pm_scale(const ztm_plan p);
+// The split the batched adjoint runs: the plan's, except the FP16 split on a
+// plan whose station bounds spread too far for it (then the BF16 three-way).
+int batched_adjoint_split(const ztm_plan p);
 PmTuning effective_tuning(ztm_plan plan, int precision);
 }  // namespace ztm
