@@ -230,9 +230,16 @@ int ztm_plan_adjoint_batched(ztm_plan plan, int n_vectors, const float* r_dev,
  * BF16 three-way split on other plans. ZTM_EINVAL for other values. No
  * reference counterpart (the reference has no batched path). */
 int ztm_plan_set_batched_tuning(ztm_plan plan, int slices, int split);
+/* Kernel variant of the batched forward and adjoint for the two-part splits
+ * (FP16 hi/lo and BF16 2-way): 0 or 1, two issue orders of the same MFMA
+ * products in the same order per accumulator, so the results are equal bit
+ * for bit. No reference counterpart. */
+int ztm_plan_set_batched_variant(ztm_plan plan, int variant);
 /* The batched tuning in force: K-slices (0 = automatic), the split of the
- * forward, and the split the adjoint runs on this plan. */
-int ztm_plan_get_batched_tuning(ztm_plan plan, int* slices, int* split, int* adjoint_split);
+ * forward, the split the adjoint runs on this plan, and the forward's kernel
+ * variant. */
+int ztm_plan_get_batched_tuning(ztm_plan plan, int* slices, int* split, int* adjoint_split,
+                                int* variant);
 
 /* Smoothness operator of the regularised misfit 0.5||G rho-d||^2 +
  * 0.5 lambda ||grad rho||^2 (forward differences on each axis, none across

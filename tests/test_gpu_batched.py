@@ -527,3 +527,43 @@ def test_batched_fp16_tiny_models(dev, orc):
     for b in (0, 3, 17):
         want = orc.pm_forward(w.grid, w.kappa, P[b].astype(np.float64), w.stations)
         assert relerr(Y[b], want) < 1e-5, b
+
+
+@pytest.mark.parametrize("split", [4, 2])
+def test_batched_kernel_variants_agree(dev, orc, split):
+    """Kernel variants 0 and 1 of the two-part splits issue the same MFMA
+    products in the same order per accumulator: the forward is bitwise
+    equal, on grids that take each cell-position mode (nx < 8, nx >= 8 not a
+    multiple of 8, a multiple of 8) and with one and several K-slices; the
+    adjoint is bitwise equal through the slice reduce and, with one slice,
+    within 1e-7 of max |X| where variant 1 applies FP16's per-vector factors in
+    its epilogue (float) instead of the transpose (double). Both against the
+    oracle."""
+    import torch
+    for nx, ny, nz, m in ((5, 7, 9, 40), (37, 5, 6, 130), (64, 8, 4, 300)):
+        g = W.Grid(nx, ny, nz, 10.0, 20.0, -15.0, 30.0, 35.0, -25.0)
+        c = g.centres()
+        st = np.stack([W.uniform(8, a, m, c[:, a].min() - 50, c[:, a].max() + 50)
+                       for a in range(2)] + [np.full(m, 40.0)], axis=1)
+        P = torch.from_numpy(models(9 + nx, 40, g.n_cells)).to(dev)
+        R = torch.from_numpy(residuals(10 + nx, 40, m)).to(dev)
+        with gfm.Plan(g, st, kappa=W.KAPPA_SI) as p:
+            for ks in (0, 3):
+                out = {}
+                for var in (0, 1):
+                    p.set_batched_tuning(slices=ks, split=split)
+                    p.set_batched_variant(var)
+                    assert p.batched_settings()[3] == var
+                    out[var] = (p.forward_batched(P).cpu().numpy(),
+                                p.adjoint_batched(R).cpu().numpy())
+                assert np.array_equal(out[0][0], out[1][0]), (nx, ks)
+                if ks == 3 or split == 2:
+                    assert np.array_equal(out[0][1], out[1][1]), (nx, ks)
+                else:
+                    assert relerr(out[1][1], out[0][1].astype(np.float64)) < 1e-7, (nx, ks)
+                tol = 1e-5 if split == 4 else 1e-4
+                for b in (0, 39):
+                    want = orc.pm_forward(g, W.KAPPA_SI, P[b].cpu().numpy().astype(np.float64), st)
+                    assert relerr(out[1][0][b], want) < tol, (nx, ks, b)
+                    want = orc.pm_adjoint(g, W.KAPPA_SI, st, R[b].cpu().numpy().astype(np.float64))
+                    assert relerr(out[1][1][b], want) < tol, (nx, ks, b)

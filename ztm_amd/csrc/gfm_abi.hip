@@ -720,7 +720,8 @@ int ztm_plan_forward_batched(ztm_plan plan, int n_models, const float* rho_dev, 
     if (int rc = ensure_batched_workspace(plan, need)) return rc;
     return batched_fwd_launch(plan->shape, n_models, ks, split, (const float*)plan->st_f, rho_dev,
                               plan->prob.threshold, plan->sy, plan->sz, pm_scale(plan),
-                              plan->st_gs, plan->bws, g_dev, (hipStream_t)stream);
+                              plan->st_gs, plan->bws, g_dev, (hipStream_t)stream,
+                              plan->tuning.batched_variant);
 }
 
 int ztm_plan_adjoint_batched(ztm_plan plan, int n_vectors, const float* r_dev, float* out_dev,
@@ -755,14 +756,23 @@ int ztm_plan_adjoint_batched(ztm_plan plan, int n_vectors, const float* r_dev, f
     if (int rc = ensure_batched_workspace(plan, need)) return rc;
     return batched_adj_launch(plan->shape, n_vectors, ks, split, (const float*)plan->st_f, r_dev,
                               plan->sy, plan->sz, pm_scale(plan), plan->gmax, plan->bws, out_dev,
-                              (hipStream_t)stream);
+                              (hipStream_t)stream, plan->tuning.batched_variant);
 }
 
-int ztm_plan_get_batched_tuning(ztm_plan plan, int* slices, int* split, int* adjoint_split) {
+int ztm_plan_get_batched_tuning(ztm_plan plan, int* slices, int* split, int* adjoint_split,
+                                int* variant) {
     if (!plan) return fail(ZTM_EINVAL, "null plan");
     if (slices) *slices = plan->tuning.batched_slices;
     if (split) *split = plan->tuning.batched_split;
     if (adjoint_split) *adjoint_split = batched_adjoint_split(plan);
+    if (variant) *variant = plan->tuning.batched_variant;
+    return ZTM_OK;
+}
+
+int ztm_plan_set_batched_variant(ztm_plan plan, int variant) {
+    if (!plan) return fail(ZTM_EINVAL, "null plan");
+    if (variant != 0 && variant != 1) return fail(ZTM_EINVAL, "variant must be 0 or 1");
+    plan->tuning.batched_variant = variant;
     return ZTM_OK;
 }
 

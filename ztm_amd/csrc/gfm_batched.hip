@@ -253,6 +253,12 @@ struct BatchedGen {
         int rw, i0, k0, j0;
         divmod_pos(cell0, nx_, inv_nx, rw, i0);
         divmod_pos(rw, ny_, inv_ny, k0, j0);
+        begin_at(p, i0, j0, k0, nx_, ny_, sy_, sz_);
+    }
+
+    // The same from the grid indices (i0, j0, k0) of the chunk's first cell.
+    __device__ __forceinline__ void begin_at(const float4 p, int i0, int j0, int k0, int nx_,
+                                             int ny_, float sy_, float sz_) {
         if constexpr (MODE == 0) {
             px = p.x; py = p.y; pz = p.z; sy = sy_; sz = sz_;
             i = i0; j = j0; k = k0; nx = nx_; ny = ny_;
@@ -469,6 +475,134 @@ __global__ __launch_bounds__(256) void batched_fwd_kernel(
         }
 }
 
+// Variant 1 of the forward for the two-part splits (FP16 hi/lo, BF16 2-way):
+// the same work and results as batched_fwd_kernel, another issue order. All
+// 2 NT B operands of a chunk are read from LDS at the top of the iteration
+// (the kernel may take up to 256 registers: 2 waves per SIMD), the MFMAs go
+// term by term over the model tiles (hi x hi on every tile, then hi x lo, then
+// lo x hi), so that consecutive MFMAs update different accumulators and none
+// waits on an LDS read, and scheduling groups place the next chunk's G
+// elements (VALU and the rsq, a transcendental) between the MFMAs.
+template <int NT, int MODE, typename H>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2)))
+void batched_fwd2_kernel(const float4* __restrict__ st, const typename Half<H>::v8* __restrict__ parts,
+                         int64_t part_stride, int nx, int ny, double inv_nx, double inv_ny,
+                         float sy, float sz, const float* __restrict__ gsc, int n_chunks,
+                         int chunks_per_slice, int m_pad, float* __restrict__ part) {
+    using v8 = typename Half<H>::v8;
+    constexpr int SPLIT = 2;
+    constexpr int Bp = 32 * NT;
+    constexpr int V = Bp * 2;
+    constexpr int LOADS = (V * SPLIT + 255) / 256;
+    __shared__ v8 lp[2][SPLIT][V];
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);   // uniform: SALU math
+    const int lane = threadIdx.x & 63;
+    const int row = lane & 31, kh = lane >> 5;
+    const int mbase = blockIdx.x * kBatchTile + wave * 32;
+    const float4 p = st[mbase + row];
+    const float gs = Half<H>::scaled ? gsc[mbase + row] : 1.f;
+
+    f32x16 acc[NT];
+#pragma unroll
+    for (int t = 0; t < NT; ++t)
+#pragma unroll
+        for (int i = 0; i < 16; ++i) acc[t][i] = 0.f;
+
+    const int c0 = blockIdx.y * chunks_per_slice;
+    const int c1 = min(c0 + chunks_per_slice, n_chunks);
+    // Per-thread source offsets of the chunk's P slots (fixed), and the
+    // wave's LDS destinations (uniform).
+    int64_t src[LOADS];
+#pragma unroll
+    for (int u = 0; u < LOADS; ++u) {
+        const int idx = 64 * wave + lane + 256 * u;
+        src[u] = (idx / V) * part_stride + (idx % V);
+    }
+    auto fetch = [&](int chunk, int buf) {
+        const typename Half<H>::v8* base = parts + (int64_t)chunk * V;
+#pragma unroll
+        for (int u = 0; u < LOADS; ++u) {
+            const int widx = 64 * wave + 256 * u;   // the wave's first slot
+            if ((V * SPLIT) % 256 == 0 || widx < V * SPLIT)
+                __builtin_amdgcn_global_load_lds(
+                    (const void*)(base + src[u]),
+                    (__attribute__((address_space(3))) void*)(&lp[buf][0][0] + widx), 16, 0, 0);
+        }
+    };
+    // Grid indices of this lane's first cell of the next chunk to generate,
+    // advanced by one chunk (16 cells) per iteration instead of a divide.
+    int gi = 0, gj = 0, gk = 0;
+    auto locate = [&](int cell) {
+        int rw;
+        divmod_pos(cell, nx, inv_nx, rw, gi);
+        divmod_pos(rw, ny, inv_ny, gk, gj);
+    };
+    auto advance = [&]() {   // nx >= 8: at most two row wraps per 16 cells
+        gi += kBatchChunk;
+#pragma unroll
+        for (int w = 0; w < 2; ++w) {
+            const bool wrap = gi >= nx;
+            gi = wrap ? gi - nx : gi;
+            gj += wrap ? 1 : 0;
+            const bool wk = gj == ny;
+            gj = wk ? 0 : gj;
+            gk += wk ? 1 : 0;
+        }
+    };
+    v8 a[SPLIT];
+    if (c0 < c1) {
+        fetch(c0, 0);
+        batched_gen<SPLIT, MODE, H>(p, c0 * kBatchChunk + 8 * kh, nx, ny, inv_nx, inv_ny, sy, sz,
+                                    gs, a);
+        if constexpr (MODE != 0) locate(min(c0 + 1, c1 - 1) * kBatchChunk + 8 * kh);
+    }
+    __builtin_amdgcn_s_waitcnt(kWaitVm0);
+    __syncthreads();
+    for (int c = c0; c < c1; ++c) {
+        const int buf = (c - c0) & 1;
+        const int cn = min(c + 1, c1 - 1);
+        fetch(cn, buf ^ 1);
+        v8 cur[SPLIT];
+#pragma unroll
+        for (int q = 0; q < SPLIT; ++q) cur[q] = a[q];
+        v8 b[SPLIT][NT];
+#pragma unroll
+        for (int q = 0; q < SPLIT; ++q)
+#pragma unroll
+            for (int t = 0; t < NT; ++t) b[q][t] = lp[buf][q][kh * Bp + 32 * t + row];
+        BatchedGen<SPLIT, MODE, H> g;
+        g.gs = gs;
+        if constexpr (MODE == 0) {
+            g.begin(p, cn * kBatchChunk + 8 * kh, nx, ny, inv_nx, inv_ny, sy, sz);
+        } else {
+            g.begin_at(p, gi, gj, gk, nx, ny, sy, sz);
+            if (cn + 1 < c1) advance();   // the chunk after cn (the last chunk repeats)
+        }
+        constexpr int TOTAL = 3 * NT;
+        static_for<0, 3>([&](auto pc) {
+            constexpr int pr = decltype(pc)::value;
+            constexpr int qa = pr == 2 ? 1 : 0, qb = pr == 1 ? 1 : 0;
+            static_for<0, NT>([&](auto tc) {
+                constexpr int t = decltype(tc)::value;
+                acc[t] = Half<H>::mfma(cur[qa], b[qb][t], acc[t]);
+                gen_after_mfma<TOTAL>(pr * NT + t,
+                                      [&](auto ec) { g.template elem<decltype(ec)::value>(a); });
+                __builtin_amdgcn_sched_barrier(0);   // keep this order
+            });
+        });
+        __builtin_amdgcn_s_waitcnt(kWaitVm0);
+        __syncthreads();
+    }
+    float* out = part + ((size_t)blockIdx.y * m_pad + mbase) * Bp;
+#pragma unroll
+    for (int t = 0; t < NT; ++t)
+#pragma unroll
+        for (int e = 0; e < 16; ++e) {
+            const int srow = 8 * (e >> 2) + 4 * kh + (e & 3);
+            out[(size_t)srow * Bp + 32 * t + row] = acc[t][e];
+        }
+}
+
 // --------------------------------------------------------------- adjoint
 // X = G^T R for R = [B residual vectors x M stations]: the same MFMA scheme
 // with the roles of stations and cells exchanged. A wave owns 32 cells (the
@@ -506,12 +640,19 @@ __device__ __forceinline__ float row_bcast(float v) {   // lane E of this lane's
         float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), 0x150 + E, 0xF, 0xF, false));
 }
 
-template <int NT, int SPLIT, int ROW, typename H>   // model tiles of 32 (B <= 32 * NT); 2- or 3-way split; part type
-__global__ __launch_bounds__(256) void batched_adj_kernel(
+// VAR 0: per tile pair, the pair's B operands read while the previous pair's
+// MFMAs run, scheduling-group hints for the G elements. VAR 1 (two-part
+// splits; batched_adj2_kernel, up to 256 registers): as batched_fwd2_kernel,
+// every B operand of the chunk read at the top, the MFMAs term by term over
+// the tiles, the G elements fixed between them. Same products in the same
+// order per accumulator: equal results.
+template <int NT, int SPLIT, int ROW, typename H, int VAR>
+__device__ __forceinline__ void batched_adj_body(
     const float4* __restrict__ st, const typename Half<H>::v8* __restrict__ parts,
     int64_t part_stride, int nx, int ny, double inv_nx, double inv_ny, float sy, float sz,
     float gs, int n_chunks, int chunks_per_slice, int n_pad, float* __restrict__ part,
-    float* __restrict__ X, int n_cells, int B, int vec4) {
+    float* __restrict__ X, int n_cells, int B, int vec4, const unsigned* __restrict__ amax,
+    double sdiv) {
     using v8 = typename Half<H>::v8;
     constexpr int Bp = 32 * NT;
     constexpr int V = Bp * 2;                          // 16-byte slots per part per chunk
@@ -555,28 +696,33 @@ __global__ __launch_bounds__(256) void batched_adj_kernel(
         const float* r = &st[(int64_t)chunk * kBatchChunk + 8 * kh + (lane & 7)].x;
         return make_float3(r[0], r[1], r[2]);
     };
-    auto gen = [&](float3 p, v8 (&a)[SPLIT]) {
+    auto prep = [&](float3 p) {
         if constexpr (ROW) {   // (x, R0, dz) of this lane's station for the wave's row
             const float dy = p.y - cy;
             const float dz = p.z - cz;
             p.y = fmaf(dy, dy, fmaf(dz, dz, 1e-20f));
             p.z = Half<H>::scaled ? dz * gs : dz;   // G scale folded into dz
         }
-        static_for<0, 8>([&](auto ec) {
-            constexpr int e = decltype(ec)::value;
-            float r2, dz;
-            const float x = row_bcast<e>(p.x) - cx;
-            if constexpr (ROW) {
-                r2 = fmaf(x, x, row_bcast<e>(p.y));
-                dz = row_bcast<e>(p.z);
-            } else {
-                const float dy = row_bcast<e>(p.y) - cy;
-                dz = row_bcast<e>(p.z) - cz;
-                r2 = fmaf(x, x, fmaf(dy, dy, fmaf(dz, dz, 1e-20f)));
-            }
-            const float v = rsq_f(r2);
-            store_split<SPLIT, H, ROW != 0>(dz * (v * v * v), gs, a, e);
-        });
+        return p;
+    };
+    auto elem = [&](auto ec, const float3& p, v8 (&a)[SPLIT]) {
+        constexpr int e = decltype(ec)::value;
+        float r2, dz;
+        const float x = row_bcast<e>(p.x) - cx;
+        if constexpr (ROW) {
+            r2 = fmaf(x, x, row_bcast<e>(p.y));
+            dz = row_bcast<e>(p.z);
+        } else {
+            const float dy = row_bcast<e>(p.y) - cy;
+            dz = row_bcast<e>(p.z) - cz;
+            r2 = fmaf(x, x, fmaf(dy, dy, fmaf(dz, dz, 1e-20f)));
+        }
+        const float v = rsq_f(r2);
+        store_split<SPLIT, H, ROW != 0>(dz * (v * v * v), gs, a, e);
+    };
+    auto gen = [&](float3 p, v8 (&a)[SPLIT]) {
+        p = prep(p);
+        static_for<0, 8>([&](auto ec) { elem(ec, p, a); });
     };
     v8 a[SPLIT];
     float3 rec = make_float3(0.f, 0.f, 0.f);
@@ -595,6 +741,29 @@ __global__ __launch_bounds__(256) void batched_adj_kernel(
         v8 cur[SPLIT];
 #pragma unroll
         for (int q = 0; q < SPLIT; ++q) cur[q] = a[q];
+        if constexpr (VAR == 1) {
+            static_assert(SPLIT == 2, "variant 1 is for the two-part splits");
+            v8 b[SPLIT][NT];
+#pragma unroll
+            for (int q = 0; q < SPLIT; ++q)
+#pragma unroll
+                for (int t = 0; t < NT; ++t) b[q][t] = lp[buf][q][kh * Bp + 32 * t + row];
+            const float3 pn = prep(now);
+            constexpr int TOTAL = 3 * NT;
+            static_for<0, 3>([&](auto pc) {
+                constexpr int pr = decltype(pc)::value;
+                constexpr int qa = pr == 2 ? 1 : 0, qb = pr == 1 ? 1 : 0;
+                static_for<0, NT>([&](auto tc) {
+                    constexpr int t = decltype(tc)::value;
+                    acc[t] = Half<H>::mfma(cur[qa], b[qb][t], acc[t]);
+                    gen_after_mfma<TOTAL>(pr * NT + t, [&](auto ec) { elem(ec, pn, a); });
+                    __builtin_amdgcn_sched_barrier(0);   // keep this order
+                });
+            });
+            __builtin_amdgcn_s_waitcnt(kWaitVm0);
+            __syncthreads();
+            continue;
+        }
         gen(now, a);
         constexpr int TP = NT >= 2 ? 2 : 1;
         v8 b[2][TP][SPLIT];
@@ -644,18 +813,25 @@ __global__ __launch_bounds__(256) void batched_adj_kernel(
         for (int t = 0; t < NT; ++t) {
             const int b = 32 * t + row;
             if (b >= B) continue;
+            // BF16: the scale was folded into the packed residuals. FP16 (here
+            // only in variant 1, whose register budget leaves room): the
+            // vector's factor sdiv / half_scale(amax[b]), formed in FP64.
+            float f = 1.f;
+            if constexpr (Half<H>::scaled) f = (float)(sdiv / (double)half_scale(amax[b]));
             float* xb = X + (size_t)b * n_cells;
 #pragma unroll
             for (int g = 0; g < 4; ++g) {
                 const int n = nbase + 8 * g + 4 * kh;
+                float v[4];
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+                    v[i] = Half<H>::scaled ? acc[t][4 * g + i] * f : acc[t][4 * g + i];
                 if (vec4 && n + 3 < n_cells) {
-                    *reinterpret_cast<float4*>(xb + n) =
-                        make_float4(acc[t][4 * g], acc[t][4 * g + 1], acc[t][4 * g + 2],
-                                    acc[t][4 * g + 3]);
+                    *reinterpret_cast<float4*>(xb + n) = make_float4(v[0], v[1], v[2], v[3]);
                 } else {
 #pragma unroll
                     for (int i = 0; i < 4; ++i)
-                        if (n + i < n_cells) xb[n + i] = acc[t][4 * g + i];
+                        if (n + i < n_cells) xb[n + i] = v[i];
                 }
             }
         }
@@ -669,6 +845,29 @@ __global__ __launch_bounds__(256) void batched_adj_kernel(
             const int crow = 8 * (e >> 2) + 4 * kh + (e & 3);
             out[(size_t)crow * Bp + 32 * t + row] = acc[t][e];
         }
+}
+
+template <int NT, int SPLIT, int ROW, typename H>   // model tiles of 32 (B <= 32 * NT); 2- or 3-way split; part type
+__global__ __launch_bounds__(256) void batched_adj_kernel(
+    const float4* __restrict__ st, const typename Half<H>::v8* __restrict__ parts,
+    int64_t part_stride, int nx, int ny, double inv_nx, double inv_ny, float sy, float sz,
+    float gs, int n_chunks, int chunks_per_slice, int n_pad, float* __restrict__ part,
+    float* __restrict__ X, int n_cells, int B, int vec4) {
+    batched_adj_body<NT, SPLIT, ROW, H, 0>(st, parts, part_stride, nx, ny, inv_nx, inv_ny, sy, sz,
+                                           gs, n_chunks, chunks_per_slice, n_pad, part, X,
+                                           n_cells, B, vec4, nullptr, 1.0);
+}
+
+template <int NT, int ROW, typename H>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2)))
+void batched_adj2_kernel(const float4* __restrict__ st, const typename Half<H>::v8* __restrict__ parts,
+                         int64_t part_stride, int nx, int ny, double inv_nx, double inv_ny,
+                         float sy, float sz, float gs, int n_chunks, int chunks_per_slice,
+                         int n_pad, float* __restrict__ part, float* __restrict__ X, int n_cells,
+                         int B, int vec4, const unsigned* __restrict__ amax, double sdiv) {
+    batched_adj_body<NT, 2, ROW, H, 1>(st, parts, part_stride, nx, ny, inv_nx, inv_ny, sy, sz, gs,
+                                       n_chunks, chunks_per_slice, n_pad, part, X, n_cells, B,
+                                       vec4, amax, sdiv);
 }
 
 // Y[b][m] = scale * sum_ks part[ks][m][b]. A block sums an 8 (stations) x 32
@@ -830,7 +1029,8 @@ int batched_adj_default_slices(const PmShape& s, int B) {
 
 int batched_fwd_launch(const PmShape& s, int B, int ks, int split, const float* st,
                        const float* P, double thr, double sy, double sz, double scale,
-                       const float* gsc, void* ws, double* Y, hipStream_t stream) {
+                       const float* gsc, void* ws, double* Y, hipStream_t stream,
+                       int variant) {
     if (B <= 0 || s.m == 0) return ZTM_OK;
     if (int rc = check_split(split, gsc != nullptr)) return rc;
     const int nt = batched_nt(B);
@@ -870,10 +1070,16 @@ int batched_fwd_launch(const PmShape& s, int B, int ks, int split, const float* 
     {
         ProfScope prof("batched_fwd", stream);
 #define ZTM_BATCHED_LAUNCH(NT, SP, MODE, H)                                                     \
-    hipLaunchKernelGGL((batched_fwd_kernel<NT, SP, MODE, H>), grid, dim3(256), 0, stream,      \
-                       (const float4*)st, (const typename Half<H>::v8*)parts, stride, s.nx,     \
-                       s.ny, 1.0 / s.nx, 1.0 / s.ny, (float)sy, (float)sz, gsc, (int)n_chunks,  \
-                       per, s.m_pad, part)
+    if (SP == 2 && variant == 1)                                                               \
+        hipLaunchKernelGGL((batched_fwd2_kernel<NT, MODE, H>), grid, dim3(256), 0, stream,     \
+                           (const float4*)st, (const typename Half<H>::v8*)parts, stride, s.nx, \
+                           s.ny, 1.0 / s.nx, 1.0 / s.ny, (float)sy, (float)sz, gsc,             \
+                           (int)n_chunks, per, s.m_pad, part);                                   \
+    else                                                                                       \
+        hipLaunchKernelGGL((batched_fwd_kernel<NT, SP, MODE, H>), grid, dim3(256), 0, stream,  \
+                           (const float4*)st, (const typename Half<H>::v8*)parts, stride, s.nx, \
+                           s.ny, 1.0 / s.nx, 1.0 / s.ny, (float)sy, (float)sz, gsc,             \
+                           (int)n_chunks, per, s.m_pad, part)
 #define ZTM_BATCHED_CASE(NT, SP, H)                                                            \
     if (s.nx % 8 == 0)                                                                         \
         ZTM_BATCHED_LAUNCH(NT, SP, 2, H);                                                      \
@@ -930,7 +1136,7 @@ size_t batched_adj_workspace(const PmShape& s, int B, int ks, int split) {
 
 int batched_adj_launch(const PmShape& s, int B, int ks, int split, const float* st,
                        const float* R, double sy, double sz, double scale, double gmax, void* ws,
-                       float* X, hipStream_t stream) {
+                       float* X, hipStream_t stream, int variant) {
     if (B <= 0 || s.n_cells == 0) return ZTM_OK;
     if (split != 4 || s.m > 0)
         if (int rc = check_split(split, gmax >= 0.0 && std::isfinite(gmax))) return rc;
@@ -958,10 +1164,13 @@ int batched_adj_launch(const PmShape& s, int B, int ks, int split, const float* 
     const int np = split_parts(split);
     __bf16* parts = (__bf16*)ws;
     float* part = (float*)(parts + np * n_packed);
-    float* direct = ks == 1 && split != 4 ? X : nullptr;
+    // one slice stores X from the accumulators: BF16 with the scale folded
+    // into the packed residuals; FP16 in variant 1 with each vector's factor
+    // applied in the epilogue (variant 0 goes through the transpose)
+    float* direct = ks == 1 && (split != 4 || variant == 1) ? X : nullptr;
     // The residuals are packed like the forward's models, stations in place
     // of cells; with one slice they carry the scale (kernel stores as is).
-    const float mul = direct ? (float)scale : 1.f;
+    const float mul = direct && split != 4 ? (float)scale : 1.f;
     unsigned* amax = nullptr;
     float gs = 1.f;
     const dim3 pgrid((unsigned)((s.m + kPackCells - 1) / kPackCells), (unsigned)(Bp / 32));
@@ -989,10 +1198,17 @@ int batched_adj_launch(const PmShape& s, int B, int ks, int split, const float* 
     {
         ProfScope prof("batched_adj", stream);
 #define ZTM_BATCHED_ADJ_ROW(NT, SP, ROW, H)                                                      \
-    hipLaunchKernelGGL((batched_adj_kernel<NT, SP, ROW, H>), grid, dim3(256), 0, stream,         \
-                       (const float4*)st, (const typename Half<H>::v8*)parts, stride, s.nx,      \
-                       s.ny, 1.0 / s.nx, 1.0 / s.ny, (float)sy, (float)sz, gs, (int)n_chunks,    \
-                       per, n_pad, part, direct, (int)s.n_cells, B, vec4)
+    if (SP == 2 && variant == 1)                                                               \
+        hipLaunchKernelGGL((batched_adj2_kernel<NT, ROW, H>), grid, dim3(256), 0, stream,        \
+                           (const float4*)st, (const typename Half<H>::v8*)parts, stride, s.nx,  \
+                           s.ny, 1.0 / s.nx, 1.0 / s.ny, (float)sy, (float)sz, gs,               \
+                           (int)n_chunks, per, n_pad, part, direct, (int)s.n_cells, B, vec4,     \
+                           (const unsigned*)amax, scale / gs);                                   \
+    else                                                                                       \
+        hipLaunchKernelGGL((batched_adj_kernel<NT, SP, ROW, H>), grid, dim3(256), 0, stream,     \
+                           (const float4*)st, (const typename Half<H>::v8*)parts, stride, s.nx,  \
+                           s.ny, 1.0 / s.nx, 1.0 / s.ny, (float)sy, (float)sz, gs,               \
+                           (int)n_chunks, per, n_pad, part, direct, (int)s.n_cells, B, vec4)
 #define ZTM_BATCHED_ADJ(NT, SP, H)                                                               \
     if (s.nx % 32 == 0)                                                                        \
         ZTM_BATCHED_ADJ_ROW(NT, SP, 1, H);                                                     \

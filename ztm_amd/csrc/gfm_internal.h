@@ -117,6 +117,7 @@ struct PmTuning {
     int cells_per_lane = 8;   // adjoint: cells per lane (4,8,16)
     int adj_splits = 0;       // adjoint: 0 = choose from the problem shape
     int batched_slices = 0;   // batched forward K-slices: 0 = choose
+    int batched_variant = 1;  // batched kernel issue order (two-part splits: 0 or 1)
     int batched_split = 3;    // batched parts: 2 or 3 BF16, or 4 = FP16 hi/lo
                               // (plans made by ztm_plan_create start at 4)
 };
@@ -313,9 +314,12 @@ int batched_default_slices(const PmShape& s, int B);
 // split: 2 or 3 BF16 parts, or 4 = two FP16 parts scaled by powers of two,
 // which needs gsc [m_pad], the stations' power-of-two G scales
 // (half_g_scale of each station's bound on |G|; else unused, null = none).
+// variant: the kernel's issue order for the two-part splits (0, or 1 =
+// batched_fwd2_kernel); same results.
 int batched_fwd_launch(const PmShape& s, int B, int ks, int split, const float* st,
                        const float* P, double thr, double sy, double sz, double scale,
-                       const float* gsc, void* ws, double* Y, hipStream_t stream);
+                       const float* gsc, void* ws, double* Y, hipStream_t stream,
+                       int variant = 0);
 // Batched adjoint on the same scheme: R float [B][M] -> X float [B][N];
 // ws must hold batched_adj_workspace(s, B, ks, split) bytes.
 size_t batched_adj_workspace(const PmShape& s, int B, int ks, int split);
@@ -324,7 +328,7 @@ int batched_adj_default_slices(const PmShape& s, int B);
 // over all station-cell pairs (< 0 = none).
 int batched_adj_launch(const PmShape& s, int B, int ks, int split, const float* st,
                        const float* R, double sy, double sz, double scale, double gmax, void* ws,
-                       float* X, hipStream_t stream);
+                       float* X, hipStream_t stream, int variant = 0);
 
 // Power-of-two G scale of the FP16 parts for a bound gmax on |G|: brings it
 // to at most 2^14; 1 when gmax is 0.

@@ -73,7 +73,8 @@ SIGNATURES = {
     "ztm_plan_forward_batched": (_I, [_P, _I, _P, _P, _P]),
     "ztm_plan_adjoint_batched": (_I, [_P, _I, _P, _P, _P]),
     "ztm_plan_set_batched_tuning": (_I, [_P, _I, _I]),
-    "ztm_plan_get_batched_tuning": (_I, [_P, _IP, _IP, _IP]),
+    "ztm_plan_get_batched_tuning": (_I, [_P, _IP, _IP, _IP, _IP]),
+    "ztm_plan_set_batched_variant": (_I, [_P, _I]),
     "ztm_plan_store_bytes": (_I64, [_P]),
     "ztm_plan_store": (_I, [_P, _P]),
     "ztm_plan_store_release": (_I, [_P]),
@@ -477,9 +478,17 @@ class Plan:
 
     def batched_tuning(self):
         """(slices, forward split, adjoint split) in force for the batched calls."""
-        v = [ctypes.c_int() for _ in range(3)]
+        return self.batched_settings()[:3]
+
+    def batched_settings(self):
+        """(slices, forward split, adjoint split, forward kernel variant)."""
+        v = [ctypes.c_int() for _ in range(4)]
         check(lib().ztm_plan_get_batched_tuning(self._h, *[ctypes.byref(x) for x in v]))
         return tuple(x.value for x in v)
+
+    def set_batched_variant(self, variant: int) -> None:
+        """Kernel variant (issue order) of the batched forward: 0 or 1."""
+        check(lib().ztm_plan_set_batched_variant(self._h, variant))
 
     def set_batched_tuning(self, slices: int = 0, split: int = 0) -> None:
         """Forward K-slices (0 = choose) and split (2 = BF16 hi/lo, 3 = BF16
