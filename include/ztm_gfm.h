@@ -213,7 +213,7 @@ int ztm_plan_forward_batched(ztm_plan plan, int n_models, const float* rho_dev,
 int ztm_plan_adjoint_batched(ztm_plan plan, int n_vectors, const float* r_dev,
                              float* out_dev, void* stream);
 /* Batched tuning: K-slices (0 = choose from the problem shape), in both
- * directions: each slice accumulates in FP32 on the MFMAs, whose accumulate
+ * directions and every class: each slice accumulates in FP32 on the MFMAs, whose accumulate
  * loses a biased fraction of an ulp per 16-cell step, so the error grows
  * with the run length (about 3e-9 of max |.| per step; the default keeps a
  * run at most 384 steps), and the slices are summed in FP64; and
@@ -231,13 +231,36 @@ int ztm_plan_adjoint_batched(ztm_plan plan, int n_vectors, const float* r_dev,
  * reference counterpart (the reference has no batched path). */
 int ztm_plan_set_batched_tuning(ztm_plan plan, int slices, int split);
 /* Kernel variant of the batched forward and adjoint for the two-part splits
- * (FP16 hi/lo and BF16 2-way): 0 or 1, two issue orders of the same MFMA
- * products in the same order per accumulator, so the results are equal bit
- * for bit. No reference counterpart. */
+ * (FP16 hi/lo and BF16 2-way), in every class: 0 or 1, two issue orders of
+ * the same MFMA products in the same order per accumulator, so the results
+ * are equal bit for bit (the FP16 adjoint with one K-slice to ~1e-7:
+ * variant 1 applies the per-vector factors in FP32 in its epilogue). No
+ * reference counterpart. */
 int ztm_plan_set_batched_variant(ztm_plan plan, int variant);
-/* The batched tuning in force: K-slices (0 = automatic), the split of the
- * forward, the split the adjoint runs on this plan, and the forward's kernel
- * variant. */
+/* One batched kernel setting, per direction (0 forward, 1 adjoint) and
+ * model-count class (the class of n_models: up to 32, 64, 128 or 256
+ * models): kernel variant (0, 1, or 2 = variant 1 with the forward's P
+ * loads two chunks ahead; the adjoint runs 2 as 1), waves per workgroup (4,
+ * or 8 with variants 1 and 2: 128 or 256 stations / cells per workgroup) and
+ * K-slices (0 = automatic). Variant and waves apply to the two-part splits;
+ * the BF16 three-way split always runs variant 0 with 4 waves. All variants
+ * give the same products in the same order. No reference counterpart. */
+int ztm_plan_set_batched_kernel(ztm_plan plan, int direction, int n_models, int variant,
+                                int waves, int slices);
+int ztm_plan_get_batched_kernel(ztm_plan plan, int direction, int n_models, int* variant,
+                                int* waves, int* slices);
+/* Autotuner of the batched (ensemble) kernels: times every candidate setting
+ * of the forward and of the adjoint for n_models models on this plan, reps
+ * calls each on `stream` (variant 0; variants 1 and, for the forward, 2 with
+ * 4 and 8 waves per workgroup; the automatic K-slice count and twice it; fewer slices would
+ * lengthen the FP32 accumulation runs), and keeps the fastest per direction
+ * for the class of n_models (ztm_plan_get_batched_kernel reports it). The
+ * split is not a candidate: tuning changes speed, not precision. Allocates
+ * and frees its own inputs (a set-up call; synchronises the stream). */
+int ztm_plan_autotune_batched(ztm_plan plan, int n_models, int reps, void* stream);
+/* The batched tuning in force for 256 models: K-slices (0 = automatic), the
+ * split of the forward, the split the adjoint runs on this plan, and the
+ * forward's kernel variant. */
 int ztm_plan_get_batched_tuning(ztm_plan plan, int* slices, int* split, int* adjoint_split,
                                 int* variant);
 

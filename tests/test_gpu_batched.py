@@ -531,7 +531,7 @@ def test_batched_fp16_tiny_models(dev, orc):
 
 @pytest.mark.parametrize("split", [4, 2])
 def test_batched_kernel_variants_agree(dev, orc, split):
-    """Kernel variants 0 and 1 of the two-part splits issue the same MFMA
+    """Kernel variants 0, 1 and 2 of the two-part splits issue the same MFMA
     products in the same order per accumulator: the forward is bitwise
     equal, on grids that take each cell-position mode (nx < 8, nx >= 8 not a
     multiple of 8, a multiple of 8) and with one and several K-slices; the
@@ -550,13 +550,15 @@ def test_batched_kernel_variants_agree(dev, orc, split):
         with gfm.Plan(g, st, kappa=W.KAPPA_SI) as p:
             for ks in (0, 3):
                 out = {}
-                for var in (0, 1):
+                for var in (0, 1, 2):
                     p.set_batched_tuning(slices=ks, split=split)
                     p.set_batched_variant(var)
                     assert p.batched_settings()[3] == var
                     out[var] = (p.forward_batched(P).cpu().numpy(),
                                 p.adjoint_batched(R).cpu().numpy())
                 assert np.array_equal(out[0][0], out[1][0]), (nx, ks)
+                assert np.array_equal(out[0][0], out[2][0]), (nx, ks)
+                assert np.array_equal(out[1][1], out[2][1]), (nx, ks)
                 if ks == 3 or split == 2:
                     assert np.array_equal(out[0][1], out[1][1]), (nx, ks)
                 else:
@@ -567,3 +569,53 @@ def test_batched_kernel_variants_agree(dev, orc, split):
                     assert relerr(out[1][0][b], want) < tol, (nx, ks, b)
                     want = orc.pm_adjoint(g, W.KAPPA_SI, st, R[b].cpu().numpy().astype(np.float64))
                     assert relerr(out[1][1][b], want) < tol, (nx, ks, b)
+
+
+@pytest.mark.parametrize("split", [4, 2])
+def test_batched_autotune_candidates_and_choice(dev, orc, split):
+    """Every kernel setting the autotuner chooses from (variant 0; variants 1
+    and 2 with 4 and 8 waves per workgroup, i.e. 128 or 256 stations / cells
+    per workgroup; automatic and explicit K-slices) gives the oracle's forward and
+    adjoint to 1e-5 (BF16 2-way: 1e-4) on a ragged grid with 40 models. After
+    autotune_batched each direction's setting is one of the candidates, and
+    the calls run it: their results equal, bit for bit, those of the same
+    setting set by hand on a fresh plan."""
+    import torch
+    g = W.Grid(37, 29, 11, 100.0, -50.0, 30.0, -70.0, 45.0, -25.0)
+    c = g.centres()
+    m = 700
+    st = np.stack([W.uniform(27, a, m, c[:, a].min() - 100, c[:, a].max() + 100)
+                   for a in range(2)] + [np.full(m, 90.0)], axis=1)
+    P = torch.from_numpy(models(28, 40, g.n_cells)).to(dev)
+    R = torch.from_numpy(residuals(29, 40, m)).to(dev)
+    tol = 1e-5 if split == 4 else 1e-4
+    want_f = [orc.pm_forward(g, W.KAPPA_SI, P[b].cpu().numpy().astype(np.float64), st)
+              for b in (0, 39)]
+    want_a = [orc.pm_adjoint(g, W.KAPPA_SI, st, R[b].cpu().numpy().astype(np.float64))
+              for b in (0, 39)]
+    with gfm.Plan(g, st, kappa=W.KAPPA_SI) as p:
+        p.set_batched_tuning(split=split)
+        for var, waves in ((0, 4), (1, 4), (1, 8), (2, 4), (2, 8)):
+            for ks in (0, 2, 7):
+                for d in (0, 1):
+                    p.set_batched_kernel(d, 40, var, waves, ks)
+                    assert p.batched_kernel(d, 40) == (var, waves, ks)
+                Y = p.forward_batched(P).cpu().numpy()
+                X = p.adjoint_batched(R).cpu().numpy()
+                for q, b in enumerate((0, 39)):
+                    assert relerr(Y[b], want_f[q]) < tol, (var, waves, ks, b)
+                    assert relerr(X[b], want_a[q]) < tol, (var, waves, ks, b)
+        p.autotune_batched(40, reps=2)
+        kf, ka = p.batched_kernel(0, 40), p.batched_kernel(1, 40)
+        assert (kf[0], kf[1]) in ((0, 4), (1, 4), (1, 8), (2, 4), (2, 8)) and kf[2] >= 1, kf
+        assert (ka[0], ka[1]) in ((0, 4), (1, 4), (1, 8)) and ka[2] >= 1, ka
+        Y = p.forward_batched(P).cpu().numpy()
+        X = p.adjoint_batched(R).cpu().numpy()
+    with gfm.Plan(g, st, kappa=W.KAPPA_SI) as q:
+        q.set_batched_tuning(split=split)
+        q.set_batched_kernel(0, 40, *kf)
+        q.set_batched_kernel(1, 40, *ka)
+        assert np.array_equal(q.forward_batched(P).cpu().numpy(), Y)
+        assert np.array_equal(q.adjoint_batched(R).cpu().numpy(), X)
+    for q_, b in enumerate((0, 39)):
+        assert relerr(Y[b], want_f[q_]) < tol and relerr(X[b], want_a[q_]) < tol

@@ -293,6 +293,14 @@ NearArgs near_args(const ztm_plan p) {
     return a;
 }
 
+BatchedKernel batched_kernel_in_force(const ztm_plan p, int direction, int n_models) {
+    BatchedKernel k = p->tuning.batched[direction][batched_class(n_models)];
+    const int split = direction == 0 ? p->tuning.batched_split : batched_adjoint_split(p);
+    if (split == 3) k.variant = 0;   // variant 1 is for the two-part splits
+    if (k.variant == 0) k.waves = 4;
+    return k;
+}
+
 int batched_adjoint_split(const ztm_plan p) {
     const int split = p->tuning.batched_split;
     return split == 4 && !p->fp16_adjoint ? 3 : split;
@@ -712,16 +720,15 @@ int ztm_plan_forward_batched(ztm_plan plan, int n_models, const float* rho_dev, 
                                     plan->prob.threshold, pm_scale(plan), plan->part, g_dev, na,
                                     (hipStream_t)stream);
     }
-    const int ks = plan->tuning.batched_slices > 0
-                       ? plan->tuning.batched_slices
-                       : batched_default_slices(plan->shape, n_models);
     const int split = plan->tuning.batched_split;
+    const BatchedKernel k = batched_kernel_in_force(plan, 0, n_models);
+    const int ks = k.slices > 0 ? k.slices : batched_default_slices(plan->shape, n_models);
     const size_t need = batched_workspace(plan->shape, n_models, ks, split);
     if (int rc = ensure_batched_workspace(plan, need)) return rc;
     return batched_fwd_launch(plan->shape, n_models, ks, split, (const float*)plan->st_f, rho_dev,
                               plan->prob.threshold, plan->sy, plan->sz, pm_scale(plan),
-                              plan->st_gs, plan->bws, g_dev, (hipStream_t)stream,
-                              plan->tuning.batched_variant);
+                              plan->st_gs, plan->bws, g_dev, (hipStream_t)stream, k.variant,
+                              k.waves);
 }
 
 int ztm_plan_adjoint_batched(ztm_plan plan, int n_vectors, const float* r_dev, float* out_dev,
@@ -748,32 +755,158 @@ int ztm_plan_adjoint_batched(ztm_plan plan, int n_vectors, const float* r_dev, f
                 return rc;
         return ZTM_OK;
     }
-    const int ks = plan->tuning.batched_slices > 0
-                       ? plan->tuning.batched_slices
-                       : batched_adj_default_slices(plan->shape, n_vectors);
     const int split = batched_adjoint_split(plan);
+    const BatchedKernel k = batched_kernel_in_force(plan, 1, n_vectors);
+    const int ks = k.slices > 0 ? k.slices : batched_adj_default_slices(plan->shape, n_vectors);
     const size_t need = batched_adj_workspace(plan->shape, n_vectors, ks, split);
     if (int rc = ensure_batched_workspace(plan, need)) return rc;
     return batched_adj_launch(plan->shape, n_vectors, ks, split, (const float*)plan->st_f, r_dev,
                               plan->sy, plan->sz, pm_scale(plan), plan->gmax, plan->bws, out_dev,
-                              (hipStream_t)stream, plan->tuning.batched_variant);
+                              (hipStream_t)stream, k.variant, k.waves);
 }
 
 int ztm_plan_get_batched_tuning(ztm_plan plan, int* slices, int* split, int* adjoint_split,
                                 int* variant) {
     if (!plan) return fail(ZTM_EINVAL, "null plan");
-    if (slices) *slices = plan->tuning.batched_slices;
+    const BatchedKernel& k = plan->tuning.batched[0][3];
+    if (slices) *slices = k.slices;
     if (split) *split = plan->tuning.batched_split;
     if (adjoint_split) *adjoint_split = batched_adjoint_split(plan);
-    if (variant) *variant = plan->tuning.batched_variant;
+    if (variant) *variant = k.variant;
     return ZTM_OK;
 }
 
 int ztm_plan_set_batched_variant(ztm_plan plan, int variant) {
     if (!plan) return fail(ZTM_EINVAL, "null plan");
-    if (variant != 0 && variant != 1) return fail(ZTM_EINVAL, "variant must be 0 or 1");
-    plan->tuning.batched_variant = variant;
+    if (variant < 0 || variant > 2) return fail(ZTM_EINVAL, "variant must be 0, 1 or 2");
+    for (auto& dir : plan->tuning.batched)
+        for (BatchedKernel& k : dir) {
+            k.variant = variant;
+            if (variant == 0) k.waves = 4;
+        }
     return ZTM_OK;
+}
+
+int ztm_plan_set_batched_kernel(ztm_plan plan, int direction, int n_models, int variant,
+                                int waves, int slices) {
+    if (!plan) return fail(ZTM_EINVAL, "null plan");
+    if (direction != 0 && direction != 1) return fail(ZTM_EINVAL, "direction must be 0 or 1");
+    if (n_models < 1 || n_models > 256) return fail(ZTM_EINVAL, "n_models must be in [1, 256]");
+    if (variant < 0 || variant > 2) return fail(ZTM_EINVAL, "variant must be 0, 1 or 2");
+    if (waves != 4 && waves != 8) return fail(ZTM_EINVAL, "waves must be 4 or 8");
+    if (waves == 8 && variant == 0) return fail(ZTM_EINVAL, "8 waves per workgroup need variant 1 or 2");
+    if (slices < 0) return fail(ZTM_EINVAL, "negative slice count");
+    plan->tuning.batched[direction][batched_class(n_models)] = BatchedKernel{variant, waves, slices};
+    return ZTM_OK;
+}
+
+int ztm_plan_get_batched_kernel(ztm_plan plan, int direction, int n_models, int* variant,
+                                int* waves, int* slices) {
+    if (!plan) return fail(ZTM_EINVAL, "null plan");
+    if (direction != 0 && direction != 1) return fail(ZTM_EINVAL, "direction must be 0 or 1");
+    if (n_models < 1 || n_models > 256) return fail(ZTM_EINVAL, "n_models must be in [1, 256]");
+    const BatchedKernel k = batched_kernel_in_force(plan, direction, n_models);
+    if (variant) *variant = k.variant;
+    if (waves) *waves = k.waves;
+    if (slices) *slices = k.slices;
+    return ZTM_OK;
+}
+
+namespace {
+__global__ void fill_ones_kernel(float* __restrict__ x, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        x[i] = (i & 1) ? 1.f : -0.5f;
+}
+}  // namespace
+
+// Times every kernel setting of the batched forward and adjoint for n_models
+// models on this plan (variant 0; variant 1 with 4 and 8 waves per
+// workgroup; for the forward also variant 2, loads two chunks ahead; the
+// automatic K-slice count and twice it, never fewer, which would lengthen the
+// FP32 accumulation runs) and keeps the fastest per direction for the
+// model-count class of n_models. The split is left as it
+// is: tuning changes speed, not precision.
+int ztm_plan_autotune_batched(ztm_plan plan, int n_models, int reps, void* stream) {
+    if (!plan) return fail(ZTM_EINVAL, "null plan");
+    if (n_models < 1 || n_models > 256) return fail(ZTM_EINVAL, "n_models must be in [1, 256]");
+    if (reps < 1) return fail(ZTM_EINVAL, "reps must be positive");
+    if (plan->prob.formula != ZTM_POINT_MASS)
+        return fail(ZTM_EINVAL, "the batched calls support the point-mass formula only");
+    const int64_t N = plan->shape.n_cells, M = plan->shape.m;
+    if (N == 0 || M == 0) return ZTM_OK;
+    if (n_models <= 4 && pm_fwd_fields_supported(plan->shape, n_models) && !near_args(plan).quad)
+        return ZTM_OK;   // the FP32 point-mass path: no MFMA kernel to tune
+    int dev = 0;
+    ZTM_HIP(hipGetDevice(&dev));
+    if (dev != plan->device)
+        return fail(ZTM_EINVAL, "plan belongs to device %d, current device is %d", plan->device, dev);
+    hipStream_t s = (hipStream_t)stream;
+    const int cls = batched_class(n_models);
+    float *P = nullptr, *R = nullptr, *X = nullptr;
+    double* Y = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t e = hipMalloc(&P, sizeof(float) * N * n_models);
+    if (e == hipSuccess) e = hipMalloc(&R, sizeof(float) * M * n_models);
+    if (e == hipSuccess) e = hipMalloc(&X, sizeof(float) * N * n_models);
+    if (e == hipSuccess) e = hipMalloc(&Y, sizeof(double) * M * n_models);
+    if (e == hipSuccess) e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    int rc = e == hipSuccess ? ZTM_OK : hip_fail(e, "autotune buffers");
+    if (rc == ZTM_OK) {
+        hipLaunchKernelGGL(fill_ones_kernel, dim3(4096), dim3(256), 0, s, P, N * n_models);
+        hipLaunchKernelGGL(fill_ones_kernel, dim3(1024), dim3(256), 0, s, R, M * n_models);
+        if (hipGetLastError() != hipSuccess) rc = fail(ZTM_EHIP, "autotune fill launch failed");
+    }
+    for (int dir = 0; dir < 2 && rc == ZTM_OK; ++dir) {
+        const int split = dir == 0 ? plan->tuning.batched_split : batched_adjoint_split(plan);
+        const int auto_ks = dir == 0 ? batched_default_slices(plan->shape, n_models)
+                                     : batched_adj_default_slices(plan->shape, n_models);
+        const int64_t kdim = dir == 0 ? (N + 15) / 16 : (M + 15) / 16;
+        std::vector<int> counts{auto_ks};
+        const int twice = (int)std::min<int64_t>(2 * (int64_t)auto_ks, std::min<int64_t>(kdim, 65535));
+        if (twice > auto_ks) counts.push_back(twice);
+        std::vector<BatchedKernel> cand;
+        for (int ks : counts) {
+            cand.push_back(BatchedKernel{0, 4, ks});
+            if (split != 3) {
+                cand.push_back(BatchedKernel{1, 4, ks});
+                cand.push_back(BatchedKernel{1, 8, ks});
+                if (dir == 0) {   // the forward's two-ahead prefetch
+                    cand.push_back(BatchedKernel{2, 4, ks});
+                    cand.push_back(BatchedKernel{2, 8, ks});
+                }
+            }
+        }
+        const BatchedKernel keep = plan->tuning.batched[dir][cls];
+        float best = INFINITY;
+        BatchedKernel won = keep;
+        auto call = [&]() {
+            return dir == 0 ? ztm_plan_forward_batched(plan, n_models, P, Y, stream)
+                            : ztm_plan_adjoint_batched(plan, n_models, R, X, stream);
+        };
+        for (const BatchedKernel& c : cand) {
+            plan->tuning.batched[dir][cls] = c;
+            if ((rc = call()) != ZTM_OK) break;   // warm-up (grows the workspace)
+            if ((e = hipEventRecord(e0, s)) != hipSuccess) { rc = hip_fail(e, "event"); break; }
+            for (int r = 0; r < reps && rc == ZTM_OK; ++r) rc = call();
+            if (rc != ZTM_OK) break;
+            if ((e = hipEventRecord(e1, s)) == hipSuccess) e = hipEventSynchronize(e1);
+            float ms = 0.f;
+            if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+            if (e != hipSuccess) { rc = hip_fail(e, "autotune timing"); break; }
+            if (ms < best) {
+                best = ms;
+                won = c;
+            }
+        }
+        plan->tuning.batched[dir][cls] = rc == ZTM_OK ? won : keep;
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    for (void* q : {(void*)P, (void*)R, (void*)X, (void*)Y})
+        if (q) (void)hipFree(q);
+    return rc;
 }
 
 int ztm_plan_set_batched_tuning(ztm_plan plan, int slices, int split) {
@@ -783,7 +916,8 @@ int ztm_plan_set_batched_tuning(ztm_plan plan, int slices, int split) {
         return fail(ZTM_EINVAL, "split must be 2, 3 or 4");
     if (split == 4 && !(plan->gmax >= 0.0 && (plan->st_gs || plan->shape.m == 0)))
         return fail(ZTM_EINVAL, "the FP16 split (4) needs a plan made by ztm_plan_create");
-    plan->tuning.batched_slices = slices;
+    for (auto& dir : plan->tuning.batched)
+        for (BatchedKernel& k : dir) k.slices = slices;
     if (split) plan->tuning.batched_split = split;
     return ZTM_OK;
 }
@@ -1024,15 +1158,16 @@ int tfm_gravity_of_fields(ztm_plan p, int bfr, int di, int dj, int fh, int fv, i
     if (int e = prep_ref_stations(s_dev, bfr, di, dj, fh, fv, p->st_f, p->st_d, p->st_c,
                                   p->out_map, m, p->shape.m_pad, s))
         return e;
-    const int ks = p->tuning.batched_slices > 0 ? p->tuning.batched_slices
-                                                : batched_default_slices(p->shape, B);
     const int split = p->tuning.batched_split;
+    const BatchedKernel k = batched_kernel_in_force(p, 0, B);
+    const int ks = k.slices > 0 ? k.slices : batched_default_slices(p->shape, B);
     const size_t ws = (batched_workspace(p->shape, B, ks, split) + 255) / 256 * 256;
     const size_t need = ws + sizeof(double) * (size_t)B * m;
     if (int rc = ensure_batched_workspace(p, need)) return rc;
     double* Y = (double*)((char*)p->bws + ws);
     if (int e = batched_fwd_launch(p->shape, B, ks, split, (const float*)p->st_f, drho_dev, thr,
-                                   p->sy, p->sz, pm_scale(p), nullptr, p->bws, Y, s))
+                                   p->sy, p->sz, pm_scale(p), nullptr, p->bws, Y, s, k.variant,
+                                   k.waves))
         return e;
     return tfm_scatter_launch(Y, p->out_map, m, M, B, g_dev, s);
 }

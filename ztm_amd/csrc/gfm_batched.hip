@@ -483,8 +483,11 @@ __global__ __launch_bounds__(256) void batched_fwd_kernel(
 // lo x hi), so that consecutive MFMAs update different accumulators and none
 // waits on an LDS read, and scheduling groups place the next chunk's G
 // elements (VALU and the rsq, a transcendental) between the MFMAs.
-template <int NT, int MODE, typename H>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2)))
+// PF: chunks of P in flight ahead of the one the MFMAs read (1: two LDS
+// buffers, the next chunk loads during this one; 2: three buffers, the loads
+// run two chunks ahead and the end-of-chunk wait covers only the older ones).
+template <int NT, int MODE, typename H, int WV, int PF = 1>   // WV waves (32 WV stations) per workgroup
+__global__ __launch_bounds__(64 * WV) __attribute__((amdgpu_waves_per_eu(2, 2)))
 void batched_fwd2_kernel(const float4* __restrict__ st, const typename Half<H>::v8* __restrict__ parts,
                          int64_t part_stride, int nx, int ny, double inv_nx, double inv_ny,
                          float sy, float sz, const float* __restrict__ gsc, int n_chunks,
@@ -493,12 +496,13 @@ void batched_fwd2_kernel(const float4* __restrict__ st, const typename Half<H>::
     constexpr int SPLIT = 2;
     constexpr int Bp = 32 * NT;
     constexpr int V = Bp * 2;
-    constexpr int LOADS = (V * SPLIT + 255) / 256;
-    __shared__ v8 lp[2][SPLIT][V];
+    constexpr int LOADS = (V * SPLIT + 64 * WV - 1) / (64 * WV);
+    constexpr int NBUF = PF + 1;
+    __shared__ v8 lp[NBUF][SPLIT][V];
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);   // uniform: SALU math
     const int lane = threadIdx.x & 63;
     const int row = lane & 31, kh = lane >> 5;
-    const int mbase = blockIdx.x * kBatchTile + wave * 32;
+    const int mbase = blockIdx.x * (32 * WV) + wave * 32;
     const float4 p = st[mbase + row];
     const float gs = Half<H>::scaled ? gsc[mbase + row] : 1.f;
 
@@ -515,15 +519,15 @@ void batched_fwd2_kernel(const float4* __restrict__ st, const typename Half<H>::
     int64_t src[LOADS];
 #pragma unroll
     for (int u = 0; u < LOADS; ++u) {
-        const int idx = 64 * wave + lane + 256 * u;
+        const int idx = 64 * wave + lane + 64 * WV * u;
         src[u] = (idx / V) * part_stride + (idx % V);
     }
     auto fetch = [&](int chunk, int buf) {
         const typename Half<H>::v8* base = parts + (int64_t)chunk * V;
 #pragma unroll
         for (int u = 0; u < LOADS; ++u) {
-            const int widx = 64 * wave + 256 * u;   // the wave's first slot
-            if ((V * SPLIT) % 256 == 0 || widx < V * SPLIT)
+            const int widx = 64 * wave + 64 * WV * u;   // the wave's first slot
+            if ((V * SPLIT) % (64 * WV) == 0 || widx < V * SPLIT)
                 __builtin_amdgcn_global_load_lds(
                     (const void*)(base + src[u]),
                     (__attribute__((address_space(3))) void*)(&lp[buf][0][0] + widx), 16, 0, 0);
@@ -552,16 +556,22 @@ void batched_fwd2_kernel(const float4* __restrict__ st, const typename Half<H>::
     v8 a[SPLIT];
     if (c0 < c1) {
         fetch(c0, 0);
+        if constexpr (PF == 2) fetch(min(c0 + 1, c1 - 1), 1);
         batched_gen<SPLIT, MODE, H>(p, c0 * kBatchChunk + 8 * kh, nx, ny, inv_nx, inv_ny, sy, sz,
                                     gs, a);
         if constexpr (MODE != 0) locate(min(c0 + 1, c1 - 1) * kBatchChunk + 8 * kh);
     }
-    __builtin_amdgcn_s_waitcnt(kWaitVm0);
+    if constexpr (PF == 2)
+        __builtin_amdgcn_s_waitcnt(kWaitVm0 | LOADS);   // chunk c0's loads have landed
+    else
+        __builtin_amdgcn_s_waitcnt(kWaitVm0);
     __syncthreads();
+    int buf = 0;   // LDS buffer of chunk c: (c - c0) % NBUF
     for (int c = c0; c < c1; ++c) {
-        const int buf = (c - c0) & 1;
         const int cn = min(c + 1, c1 - 1);
-        fetch(cn, buf ^ 1);
+        // buffer of chunk c + PF, last read in iteration c - 1 (before the barrier)
+        const int fb = buf + PF >= NBUF ? buf + PF - NBUF : buf + PF;
+        fetch(min(c + PF, c1 - 1), fb);
         v8 cur[SPLIT];
 #pragma unroll
         for (int q = 0; q < SPLIT; ++q) cur[q] = a[q];
@@ -590,9 +600,15 @@ void batched_fwd2_kernel(const float4* __restrict__ st, const typename Half<H>::
                 __builtin_amdgcn_sched_barrier(0);   // keep this order
             });
         });
-        __builtin_amdgcn_s_waitcnt(kWaitVm0);
+        if constexpr (PF == 2)
+            __builtin_amdgcn_s_waitcnt(kWaitVm0 | LOADS);   // all but chunk c + 2's loads
+        else
+            __builtin_amdgcn_s_waitcnt(kWaitVm0);
         __syncthreads();
+        buf = buf + 1 == NBUF ? 0 : buf + 1;
     }
+    // no load into this workgroup's LDS may still be in flight when it ends
+    if constexpr (PF == 2) __builtin_amdgcn_s_waitcnt(kWaitVm0);
     float* out = part + ((size_t)blockIdx.y * m_pad + mbase) * Bp;
 #pragma unroll
     for (int t = 0; t < NT; ++t)
@@ -646,7 +662,7 @@ __device__ __forceinline__ float row_bcast(float v) {   // lane E of this lane's
 // every B operand of the chunk read at the top, the MFMAs term by term over
 // the tiles, the G elements fixed between them. Same products in the same
 // order per accumulator: equal results.
-template <int NT, int SPLIT, int ROW, typename H, int VAR>
+template <int NT, int SPLIT, int ROW, typename H, int VAR, int WV = 4>   // WV waves per workgroup
 __device__ __forceinline__ void batched_adj_body(
     const float4* __restrict__ st, const typename Half<H>::v8* __restrict__ parts,
     int64_t part_stride, int nx, int ny, double inv_nx, double inv_ny, float sy, float sz,
@@ -656,11 +672,11 @@ __device__ __forceinline__ void batched_adj_body(
     using v8 = typename Half<H>::v8;
     constexpr int Bp = 32 * NT;
     constexpr int V = Bp * 2;                          // 16-byte slots per part per chunk
-    constexpr int LOADS = (V * SPLIT + 255) / 256;     // per thread per chunk
+    constexpr int LOADS = (V * SPLIT + 64 * WV - 1) / (64 * WV);   // per thread per chunk
     __shared__ v8 lp[2][SPLIT][V];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int row = lane & 31, kh = lane >> 5;
-    const int nbase = blockIdx.x * kBatchTile + wave * 32;
+    const int nbase = blockIdx.x * (32 * WV) + wave * 32;
     float cx, cy, cz;
     {
         int rw, i, j, k;
@@ -682,8 +698,8 @@ __device__ __forceinline__ void batched_adj_body(
     auto fetch = [&](int chunk, int buf) {
 #pragma unroll
         for (int u = 0; u < LOADS; ++u) {
-            const int idx = threadIdx.x + 256 * u;
-            if ((V * SPLIT) % 256 == 0 || idx < V * SPLIT)
+            const int idx = threadIdx.x + 64 * WV * u;
+            if ((V * SPLIT) % (64 * WV) == 0 || idx < V * SPLIT)
                 __builtin_amdgcn_global_load_lds(
                     (const void*)(parts + (idx / V) * part_stride + (int64_t)chunk * V + (idx % V)),
                     (__attribute__((address_space(3))) void*)(&lp[buf][0][0] + (idx - lane)),
@@ -858,14 +874,14 @@ __global__ __launch_bounds__(256) void batched_adj_kernel(
                                            n_cells, B, vec4, nullptr, 1.0);
 }
 
-template <int NT, int ROW, typename H>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2)))
+template <int NT, int ROW, typename H, int WV>
+__global__ __launch_bounds__(64 * WV) __attribute__((amdgpu_waves_per_eu(2, 2)))
 void batched_adj2_kernel(const float4* __restrict__ st, const typename Half<H>::v8* __restrict__ parts,
                          int64_t part_stride, int nx, int ny, double inv_nx, double inv_ny,
                          float sy, float sz, float gs, int n_chunks, int chunks_per_slice,
                          int n_pad, float* __restrict__ part, float* __restrict__ X, int n_cells,
                          int B, int vec4, const unsigned* __restrict__ amax, double sdiv) {
-    batched_adj_body<NT, 2, ROW, H, 1>(st, parts, part_stride, nx, ny, inv_nx, inv_ny, sy, sz, gs,
+    batched_adj_body<NT, 2, ROW, H, 1, WV>(st, parts, part_stride, nx, ny, inv_nx, inv_ny, sy, sz, gs,
                                        n_chunks, chunks_per_slice, n_pad, part, X, n_cells, B,
                                        vec4, amax, sdiv);
 }
@@ -1030,7 +1046,11 @@ int batched_adj_default_slices(const PmShape& s, int B) {
 int batched_fwd_launch(const PmShape& s, int B, int ks, int split, const float* st,
                        const float* P, double thr, double sy, double sz, double scale,
                        const float* gsc, void* ws, double* Y, hipStream_t stream,
-                       int variant) {
+                       int variant, int waves) {
+    if (waves != 4 && waves != 8) return fail(ZTM_EINVAL, "waves per workgroup must be 4 or 8");
+    if (variant < 0 || variant > 2) return fail(ZTM_EINVAL, "kernel variant must be 0, 1 or 2");
+    if (waves == 8 && (variant == 0 || split == 3))
+        return fail(ZTM_EINVAL, "8 waves per workgroup need variant 1 and a two-part split");
     if (B <= 0 || s.m == 0) return ZTM_OK;
     if (int rc = check_split(split, gsc != nullptr)) return rc;
     const int nt = batched_nt(B);
@@ -1065,16 +1085,25 @@ int batched_fwd_launch(const PmShape& s, int B, int ks, int split, const float* 
                            s.n_cells, Bp, n_chunks, n_packed, (float)thr, 1.f, nullptr, parts);
     }
     ZTM_HIP(hipGetLastError());
-    const dim3 grid((unsigned)((s.m + kBatchTile - 1) / kBatchTile), (unsigned)ks);
+    const int tile = 32 * waves;   // stations per workgroup (m_pad is a multiple of 512)
+    const dim3 grid((unsigned)((s.m + tile - 1) / tile), (unsigned)ks);
     const int64_t stride = n_packed / 8;   // in 16-byte units
     {
         ProfScope prof("batched_fwd", stream);
+#define ZTM_BATCHED_FWD2(NT, MODE, H, WV, PF)                                                   \
+    hipLaunchKernelGGL((batched_fwd2_kernel<NT, MODE, H, WV, PF>), grid, dim3(64 * WV), 0,      \
+                       stream, (const float4*)st, (const typename Half<H>::v8*)parts, stride,   \
+                       s.nx, s.ny, 1.0 / s.nx, 1.0 / s.ny, (float)sy, (float)sz, gsc,           \
+                       (int)n_chunks, per, s.m_pad, part)
 #define ZTM_BATCHED_LAUNCH(NT, SP, MODE, H)                                                     \
-    if (SP == 2 && variant == 1)                                                               \
-        hipLaunchKernelGGL((batched_fwd2_kernel<NT, MODE, H>), grid, dim3(256), 0, stream,     \
-                           (const float4*)st, (const typename Half<H>::v8*)parts, stride, s.nx, \
-                           s.ny, 1.0 / s.nx, 1.0 / s.ny, (float)sy, (float)sz, gsc,             \
-                           (int)n_chunks, per, s.m_pad, part);                                   \
+    if (SP == 2 && variant == 2 && waves == 8)                                                 \
+        ZTM_BATCHED_FWD2(NT, MODE, H, 8, 2);                                                   \
+    else if (SP == 2 && variant == 2)                                                          \
+        ZTM_BATCHED_FWD2(NT, MODE, H, 4, 2);                                                   \
+    else if (SP == 2 && variant == 1 && waves == 8)                                            \
+        ZTM_BATCHED_FWD2(NT, MODE, H, 8, 1);                                                   \
+    else if (SP == 2 && variant == 1)                                                          \
+        ZTM_BATCHED_FWD2(NT, MODE, H, 4, 1);                                                   \
     else                                                                                       \
         hipLaunchKernelGGL((batched_fwd_kernel<NT, SP, MODE, H>), grid, dim3(256), 0, stream,  \
                            (const float4*)st, (const typename Half<H>::v8*)parts, stride, s.nx, \
@@ -1104,6 +1133,7 @@ int batched_fwd_launch(const PmShape& s, int B, int ks, int split, const float* 
 #undef ZTM_BATCHED_NT
 #undef ZTM_BATCHED_CASE
 #undef ZTM_BATCHED_LAUNCH
+#undef ZTM_BATCHED_FWD2
         ZTM_HIP(hipGetLastError());
     }
     hipLaunchKernelGGL(batched_reduce_kernel<double>, dim3((s.m + 7) / 8, (B + 31) / 32), dim3(256), 0,
@@ -1125,7 +1155,7 @@ __global__ void batched_zero_kernel(float* __restrict__ x, int64_t n) {
 size_t batched_adj_workspace(const PmShape& s, int B, int ks, int split) {
     const int Bp = 32 * batched_nt(B);
     const int64_t m16 = ((int64_t)s.m + kBatchChunk - 1) / kBatchChunk * kBatchChunk;
-    const int64_t n_pad = (s.n_cells + kBatchTile - 1) / kBatchTile * kBatchTile;
+    const int64_t n_pad = (s.n_cells + 255) / 256 * 256;   // both workgroup sizes
     const size_t packed = (size_t)m16 * Bp * split_parts(split) * sizeof(__bf16);
     // one BF16 slice stores X directly; FP16 always goes through the reduce,
     // which takes out the per-vector scales
@@ -1136,7 +1166,11 @@ size_t batched_adj_workspace(const PmShape& s, int B, int ks, int split) {
 
 int batched_adj_launch(const PmShape& s, int B, int ks, int split, const float* st,
                        const float* R, double sy, double sz, double scale, double gmax, void* ws,
-                       float* X, hipStream_t stream, int variant) {
+                       float* X, hipStream_t stream, int variant, int waves) {
+    if (waves != 4 && waves != 8) return fail(ZTM_EINVAL, "waves per workgroup must be 4 or 8");
+    if (variant < 0 || variant > 2) return fail(ZTM_EINVAL, "kernel variant must be 0, 1 or 2");
+    if (waves == 8 && (variant == 0 || split == 3))
+        return fail(ZTM_EINVAL, "8 waves per workgroup need variant 1 and a two-part split");
     if (B <= 0 || s.n_cells == 0) return ZTM_OK;
     if (split != 4 || s.m > 0)
         if (int rc = check_split(split, gmax >= 0.0 && std::isfinite(gmax))) return rc;
@@ -1159,15 +1193,16 @@ int batched_adj_launch(const PmShape& s, int B, int ks, int split, const float* 
     const int per = (int)((n_chunks + ks - 1) / ks);
     ks = (int)((n_chunks + per - 1) / per);
     const int64_t n_packed = n_chunks * Bp * kBatchChunk;
-    const int64_t tiles = (s.n_cells + kBatchTile - 1) / kBatchTile;
-    const int n_pad = (int)(tiles * kBatchTile);
+    const int tile = 32 * waves;   // cells per workgroup
+    const int64_t tiles = (s.n_cells + tile - 1) / tile;
+    const int n_pad = (int)(tiles * tile);
     const int np = split_parts(split);
     __bf16* parts = (__bf16*)ws;
     float* part = (float*)(parts + np * n_packed);
     // one slice stores X from the accumulators: BF16 with the scale folded
     // into the packed residuals; FP16 in variant 1 with each vector's factor
     // applied in the epilogue (variant 0 goes through the transpose)
-    float* direct = ks == 1 && (split != 4 || variant == 1) ? X : nullptr;
+    float* direct = ks == 1 && (split != 4 || variant >= 1) ? X : nullptr;
     // The residuals are packed like the forward's models, stations in place
     // of cells; with one slice they carry the scale (kernel stores as is).
     const float mul = direct && split != 4 ? (float)scale : 1.f;
@@ -1197,13 +1232,17 @@ int batched_adj_launch(const PmShape& s, int B, int ks, int split, const float* 
     const int vec4 = s.n_cells % 4 == 0 && (uintptr_t)X % 16 == 0;
     {
         ProfScope prof("batched_adj", stream);
+#define ZTM_BATCHED_ADJ2(NT, ROW, H, WV)                                                         \
+    hipLaunchKernelGGL((batched_adj2_kernel<NT, ROW, H, WV>), grid, dim3(64 * WV), 0, stream,    \
+                       (const float4*)st, (const typename Half<H>::v8*)parts, stride, s.nx,      \
+                       s.ny, 1.0 / s.nx, 1.0 / s.ny, (float)sy, (float)sz, gs, (int)n_chunks,    \
+                       per, n_pad, part, direct, (int)s.n_cells, B, vec4,                        \
+                       (const unsigned*)amax, scale / gs)
 #define ZTM_BATCHED_ADJ_ROW(NT, SP, ROW, H)                                                      \
-    if (SP == 2 && variant == 1)                                                               \
-        hipLaunchKernelGGL((batched_adj2_kernel<NT, ROW, H>), grid, dim3(256), 0, stream,        \
-                           (const float4*)st, (const typename Half<H>::v8*)parts, stride, s.nx,  \
-                           s.ny, 1.0 / s.nx, 1.0 / s.ny, (float)sy, (float)sz, gs,               \
-                           (int)n_chunks, per, n_pad, part, direct, (int)s.n_cells, B, vec4,     \
-                           (const unsigned*)amax, scale / gs);                                   \
+    if (SP == 2 && variant >= 1 && waves == 8)                                                 \
+        ZTM_BATCHED_ADJ2(NT, ROW, H, 8);                                                       \
+    else if (SP == 2 && variant >= 1)                                                          \
+        ZTM_BATCHED_ADJ2(NT, ROW, H, 4);                                                       \
     else                                                                                       \
         hipLaunchKernelGGL((batched_adj_kernel<NT, SP, ROW, H>), grid, dim3(256), 0, stream,     \
                            (const float4*)st, (const typename Half<H>::v8*)parts, stride, s.nx,  \
@@ -1231,6 +1270,7 @@ int batched_adj_launch(const PmShape& s, int B, int ks, int split, const float* 
 #undef ZTM_BATCHED_ADJ_NT
 #undef ZTM_BATCHED_ADJ
 #undef ZTM_BATCHED_ADJ_ROW
+#undef ZTM_BATCHED_ADJ2
         ZTM_HIP(hipGetLastError());
     }
     if (direct) return ZTM_OK;

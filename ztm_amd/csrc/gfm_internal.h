@@ -111,13 +111,24 @@ inline void near_layers(const NearArgs& na, int nz, int* klo, int* khi) {
 }
 
 // Launch shapes of the point-mass kernels.
+// One batched kernel setting: issue-order variant (two-part splits: 0 or 1),
+// waves per workgroup (variant 1: 4 or 8, i.e. 128 or 256 stations / cells),
+// K-slices (0 = choose from the problem shape).
+struct BatchedKernel {
+    int variant = 1;
+    int waves = 4;
+    int slices = 0;
+};
+inline int batched_class(int B) { return B <= 32 ? 0 : B <= 64 ? 1 : B <= 128 ? 2 : 3; }
+
 struct PmTuning {
     int st_per_lane = 2;      // forward: stations per lane (1,2,4,8)
     int rows_per_slice = 0;   // forward: 0 = choose from the problem shape
     int cells_per_lane = 8;   // adjoint: cells per lane (4,8,16)
     int adj_splits = 0;       // adjoint: 0 = choose from the problem shape
-    int batched_slices = 0;   // batched forward K-slices: 0 = choose
-    int batched_variant = 1;  // batched kernel issue order (two-part splits: 0 or 1)
+    // batched kernels, per direction (0 forward, 1 adjoint) and model-count
+    // class (batched_class: up to 32, 64, 128, 256 models)
+    BatchedKernel batched[2][4];
     int batched_split = 3;    // batched parts: 2 or 3 BF16, or 4 = FP16 hi/lo
                               // (plans made by ztm_plan_create start at 4)
 };
@@ -319,7 +330,7 @@ int batched_default_slices(const PmShape& s, int B);
 int batched_fwd_launch(const PmShape& s, int B, int ks, int split, const float* st,
                        const float* P, double thr, double sy, double sz, double scale,
                        const float* gsc, void* ws, double* Y, hipStream_t stream,
-                       int variant = 0);
+                       int variant = 0, int waves = 4);
 // Batched adjoint on the same scheme: R float [B][M] -> X float [B][N];
 // ws must hold batched_adj_workspace(s, B, ks, split) bytes.
 size_t batched_adj_workspace(const PmShape& s, int B, int ks, int split);
@@ -328,7 +339,7 @@ int batched_adj_default_slices(const PmShape& s, int B);
 // over all station-cell pairs (< 0 = none).
 int batched_adj_launch(const PmShape& s, int B, int ks, int split, const float* st,
                        const float* R, double sy, double sz, double scale, double gmax, void* ws,
-                       float* X, hipStream_t stream, int variant = 0);
+                       float* X, hipStream_t stream, int variant = 0, int waves = 4);
 
 // Power-of-two G scale of the FP16 parts for a bound gmax on |G|: brings it
 // to at most 2^14; 1 when gmax is 0.
@@ -418,5 +429,9 @@ double pm_scale(const ztm_plan p);
 // The split the batched adjoint runs: the plan's, except the FP16 split on a
 // plan whose station bounds spread too far for it (then the BF16 three-way).
 int batched_adjoint_split(const ztm_plan p);
+// The kernel setting a batched call of n_models runs in `direction` (0
+// forward, 1 adjoint): the class entry, with variant 0 / 4 waves for the
+// BF16 three-way split.
+BatchedKernel batched_kernel_in_force(const ztm_plan p, int direction, int n_models);
 PmTuning effective_tuning(ztm_plan plan, int precision);
 }  // namespace ztm

@@ -75,6 +75,9 @@ SIGNATURES = {
     "ztm_plan_set_batched_tuning": (_I, [_P, _I, _I]),
     "ztm_plan_get_batched_tuning": (_I, [_P, _IP, _IP, _IP, _IP]),
     "ztm_plan_set_batched_variant": (_I, [_P, _I]),
+    "ztm_plan_set_batched_kernel": (_I, [_P, _I, _I, _I, _I, _I]),
+    "ztm_plan_get_batched_kernel": (_I, [_P, _I, _I, _IP, _IP, _IP]),
+    "ztm_plan_autotune_batched": (_I, [_P, _I, _I, _P]),
     "ztm_plan_store_bytes": (_I64, [_P]),
     "ztm_plan_store": (_I, [_P, _P]),
     "ztm_plan_store_release": (_I, [_P]),
@@ -485,6 +488,25 @@ class Plan:
         v = [ctypes.c_int() for _ in range(4)]
         check(lib().ztm_plan_get_batched_tuning(self._h, *[ctypes.byref(x) for x in v]))
         return tuple(x.value for x in v)
+
+    def set_batched_kernel(self, direction: int, n_models: int, variant: int, waves: int = 4,
+                           slices: int = 0) -> None:
+        """Kernel setting of the batched forward (direction 0) or adjoint (1)
+        for the model-count class of n_models."""
+        check(lib().ztm_plan_set_batched_kernel(self._h, direction, n_models, variant, waves,
+                                                slices))
+
+    def batched_kernel(self, direction: int, n_models: int):
+        """(variant, waves per workgroup, K-slices) a batched call of n_models runs."""
+        v = [ctypes.c_int() for _ in range(3)]
+        check(lib().ztm_plan_get_batched_kernel(self._h, direction, n_models,
+                                                *[ctypes.byref(x) for x in v]))
+        return tuple(x.value for x in v)
+
+    def autotune_batched(self, n_models: int, reps: int = 3, stream=None) -> None:
+        """Time the batched kernel settings for n_models models and keep the
+        fastest per direction (ztm_plan_autotune_batched)."""
+        check(lib().ztm_plan_autotune_batched(self._h, n_models, reps, _stream(stream)))
 
     def set_batched_variant(self, variant: int) -> None:
         """Kernel variant (issue order) of the batched forward: 0 or 1."""
