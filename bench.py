@@ -258,6 +258,8 @@ def summary(line):
     out = {"fwd_adj_vs_fp64_gpu": {
                "point_mass_fp32": line.get("speedup_vs_fp64_gpu"),
                "prism_fp32": get(line, "prism", "speedup_vs_fp64_gpu"),
+               "prism_fp32_vs_fp64_same_algorithm": get(line, "prism",
+                                                        "speedup_vs_fp64_same_algorithm"),
                "ensemble_split3": get(line, "batched", "split3", "forward_plus_adjoint",
                                       "speedup_vs_fp64_gpu"),
                "ensemble_split2": get(line, "batched", "split2", "forward_plus_adjoint",
@@ -281,6 +283,12 @@ def summary(line):
                                                        "rel_err_vs_fp64")) if v is not None),
                                       default=None)}}
     return out
+
+
+def _progress(msg):
+    """One line on stderr as each part of a leg starts (the JSON line stays on
+    stdout)."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
 def _timed(fn, reps, sync):
@@ -342,6 +350,7 @@ def batched_leg(stream, args):
     against the FP64 single-model
     kernels on models / vectors 0, 85, 170, 255 (extrapolated to 256 for the
     FP64 rates)."""
+    _progress("batched leg")
     import torch
     from ztm_amd import gfm, workloads as W
     dev = torch.device("cuda:0")
@@ -354,36 +363,36 @@ def batched_leg(stream, args):
     with gfm.Plan(w.grid, w.stations, kappa=w.kappa) as p:
         Y = torch.empty(B, w.n_stations, dtype=torch.float64, device=dev)
         check = (0, 85, 170, 255)
-        g64 = {}
-        t64 = 0.0
-        for b in check:
-            x = torch.from_numpy(w.rho[b].astype(np.float64)).to(dev)
-            p.forward(x)
-            torch.cuda.synchronize()
-            t0 = time.perf_counter()
-            g64[b] = p.forward(x).cpu().numpy()
-            t64 += time.perf_counter() - t0
+        # FP64 single-model forward and adjoint on 4 models / vectors: inputs and
+        # outputs resident, the calls timed back to back (no copies inside)
+        x64 = {b: torch.from_numpy(w.rho[b].astype(np.float64)).to(dev) for b in check}
+        y64 = {b: torch.empty(w.n_stations, dtype=torch.float64, device=dev) for b in check}
+        t64 = _timed(lambda: [p.forward(x64[b], out=y64[b], stream=stream) for b in check], 1,
+                     torch.cuda.synchronize)
+        g64 = {b: y64[b].cpu().numpy() for b in check}
         out["fp64_gpu"] = {"value": pm / (t64 / len(check) * B), "unit": "pair-models/s",
-                           "kind": "FP64 single-model kernel, timed on 4 models"}
+                           "kind": "FP64 single-model kernel, timed on 4 models (resident, "
+                                   "no copies in the timed region)"}
         R = np.stack([W.normal(W.SEED0 + 40, b, w.n_stations, 0.5) for b in range(B)]
                      ).astype(np.float32)
         Rt = torch.from_numpy(R).to(dev)
         X = torch.empty(B, w.grid.n_cells, dtype=torch.float32, device=dev)
-        a64 = {}
-        ta64 = 0.0
-        for b in check:
-            x = torch.from_numpy(R[b].astype(np.float64)).to(dev)
-            p.adjoint(x)
-            torch.cuda.synchronize()
-            t0 = time.perf_counter()
-            a64[b] = p.adjoint(x).cpu().numpy()
-            ta64 += time.perf_counter() - t0
+        r64 = {b: torch.from_numpy(R[b].astype(np.float64)).to(dev) for b in check}
+        z64 = {b: torch.empty(w.grid.n_cells, dtype=torch.float64, device=dev) for b in check}
+        ta64 = _timed(lambda: [p.adjoint(r64[b], out=z64[b], stream=stream) for b in check], 1,
+                      torch.cuda.synchronize)
+        a64 = {b: z64[b].cpu().numpy() for b in check}
         out["fp64_gpu_adjoint"] = {"value": pm / (ta64 / len(check) * B),
                                    "unit": "pair-models/s",
-                                   "kind": "FP64 single-vector adjoint, timed on 4 vectors"}
+                                   "kind": "FP64 single-vector adjoint, timed on 4 vectors "
+                                           "(resident, no copies in the timed region)"}
         fp64_fa = 2.0 * pm / ((t64 + ta64) / len(check) * B)
         for split in (3, 2, 4):
+            _progress(f"ensemble split {split}")
             p.set_batched_tuning(split=split)
+            t0 = time.perf_counter()
+            p.autotune_batched(B, reps=3, stream=stream)
+            t_tune = time.perf_counter() - t0
             products = 6 if split == 3 else 3
             gfm.profile_reset()
             gfm.profile_enable(True)
@@ -421,9 +430,15 @@ def batched_leg(stream, args):
                 "speedup_vs_fp64_gpu": (pm / ta) / out["fp64_gpu_adjoint"]["value"]}
             fa = 2.0 * pm / (t + ta)
             leg["forward_plus_adjoint"] = {"value": fa, "unit": "pair-models/s",
+                                           "ms_per_pair_of_calls": (t + ta) * 1e3,
                                            "speedup_vs_fp64_gpu": fa / fp64_fa}
+            leg["autotune"] = {"s": t_tune,
+                               "forward": dict(zip(("variant", "waves", "slices"),
+                                                   p.batched_kernel(0, B))),
+                               "adjoint": dict(zip(("variant", "waves", "slices"),
+                                                   p.batched_kernel(1, B)))}
             out["fp16x2" if split == 4 else f"split{split}"] = leg
-        p.set_batched_tuning(split=3)
+        out["default_split"] = "fp16x2"
     return out
 
 
@@ -434,6 +449,7 @@ def uneven_leg(stream, args, iters=10, solves=2):
     every pass one persistent launch over all 48, the whole solve one HIP
     graph, tiles autotuned per size class. Compared with the same passes run
     problem by problem (FP32 plans and FP64 plans, eager)."""
+    _progress("uneven leg")
     import torch
     from ztm_amd import gfm, workloads as W
     dev = torch.device("cuda:0")
@@ -546,6 +562,7 @@ def tfm_leg(stream, args, reps=20):
     start, files, PCIe), and the reference's CPU path restated (producer +
     TFM.cpp:95-99 gravity with one thread per station row, and on 16
     threads) on this host."""
+    _progress("tfm leg")
     import subprocess
     import tempfile
     import torch
@@ -704,6 +721,7 @@ def accumulation_leg(stream, args):
     summed across slices in FP64. ensemble_256 at half size (48x48x24 cells,
     2,304 stations), 64 models; error = max over models 0, 21, 63 of
     max |Y - Y64| / max |Y64| against the FP64 single-model forward."""
+    _progress("accumulation leg")
     import torch
     from ztm_amd import gfm, workloads as W
     dev = torch.device("cuda:0")
@@ -729,6 +747,7 @@ def prism_leg(w, r, stream, args):
     """Prism cell formula on the same workload: FP32 hybrid (point-mass far
     field, Gauss-Legendre mid zone, FP64 closed-form near field) against the
     FP64 closed form on every pair, forward + adjoint."""
+    _progress("prism leg")
     import torch
     from ztm_amd import gfm
     dev = torch.device("cuda:0")
@@ -776,6 +795,18 @@ def prism_leg(w, r, stream, args):
             out["speedup_vs_fp64_gpu"] = t64 / t32
             out["rel_err_vs_fp64"] = {"forward": relerr(g32.cpu().numpy(), g64.cpu().numpy()),
                                       "adjoint": relerr(a32.cpu().numpy(), a64.cpu().numpy())}
+            # the same zoned algorithm in FP64: what FP32 itself buys
+            gz = torch.empty(w.n_stations, dtype=torch.float64, device=dev)
+            az = torch.empty(w.grid.n_cells, dtype=torch.float64, device=dev)
+            tz = _timed(lambda: (p.forward(rho64, out=gz, stream=stream, zoned=True),
+                                 p.adjoint(r64, out=az, stream=stream, zoned=True)),
+                        reps, torch.cuda.synchronize)
+            out["fp64_gpu_same_algorithm"] = {
+                "value": 2.0 * pairs / tz, "unit": "pairs/s", "ms_per_step": tz * 1e3,
+                "kind": "FP64 with the FP32 path's zones (ZTM_FP64_ZONED)",
+                "rel_err_vs_fp64": {"forward": relerr(gz.cpu().numpy(), g64.cpu().numpy()),
+                                    "adjoint": relerr(az.cpu().numpy(), a64.cpu().numpy())}}
+            out["speedup_vs_fp64_same_algorithm"] = tz / t32
     return out
 
 

@@ -60,6 +60,13 @@ extern "C" {
 /* precisions */
 #define ZTM_FP32    0   /* FP32 bulk, FP64 per-row/per-tile and final sums     */
 #define ZTM_FP64    1   /* FP64 everywhere                                     */
+/* FP64 arithmetic with the FP32 path's zones: for prism plans, the point-mass
+ * far field (plus the quadrupole term of non-cubic cells) beyond the
+ * correction box, 3x3x3 Gauss-Legendre within it and the closed form in the
+ * near box, all in FP64 -- the FP32 path's algorithm at FP64 precision, for
+ * comparing the two. Point-mass plans: the same as ZTM_FP64. Arrays are
+ * double as for ZTM_FP64. */
+#define ZTM_FP64_ZONED 2
 
 /* cell formulas */
 #define ZTM_POINT_MASS 0
@@ -153,13 +160,14 @@ int64_t ztm_plan_num_stations(ztm_plan plan);
 int ztm_plan_get_problem(ztm_plan plan, ztm_problem* problem);
 
 /* Forward: g_dev[m] = kappa * sum_n rho[n] K(m,n)   (FP64 output)
- * precision ZTM_FP32: rho_dev is float[N];  ZTM_FP64: rho_dev is double[N]. */
+ * precision ZTM_FP32: rho_dev is float[N];
+ *           ZTM_FP64, ZTM_FP64_ZONED: rho_dev is double[N]. */
 int ztm_plan_forward(ztm_plan plan, int precision, const void* rho_dev,
                      double* g_dev, void* stream);
 
 /* Adjoint: out_dev[n] = kappa * sum_m K(m,n) r[m]   (threshold not applied)
  * precision ZTM_FP32: r_dev float[M], out_dev float[N];
- *           ZTM_FP64: r_dev double[M], out_dev double[N]. */
+ *           ZTM_FP64, ZTM_FP64_ZONED: r_dev double[M], out_dev double[N]. */
 int ztm_plan_adjoint(ztm_plan plan, int precision, const void* r_dev,
                      void* out_dev, void* stream);
 

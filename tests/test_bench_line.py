@@ -19,7 +19,7 @@ def test_summary_gathers_the_legs():
     b = load_bench()
     line = {"speedup_vs_fp64_gpu": 3.2, "speedup_vs_cpu": 4800.0,
             "rel_err_vs_fp64": {"forward": 6e-8, "adjoint": 3e-7},
-            "prism": {"speedup_vs_fp64_gpu": 220.0,
+            "prism": {"speedup_vs_fp64_gpu": 220.0, "speedup_vs_fp64_same_algorithm": 3.1,
                       "rel_err_vs_fp64": {"forward": 4e-7, "adjoint": 5e-7}},
             "batched": {"split3": {"rel_err_vs_fp64": 1.1e-6,
                                    "adjoint": {"rel_err_vs_fp64": 1.9e-6},
@@ -32,6 +32,7 @@ def test_summary_gathers_the_legs():
                                    "forward_plus_adjoint": {"speedup_vs_fp64_gpu": 94.0}}}}
     s = b.summary(line)
     assert s["fwd_adj_vs_fp64_gpu"] == {"point_mass_fp32": 3.2, "prism_fp32": 220.0,
+                                        "prism_fp32_vs_fp64_same_algorithm": 3.1,
                                         "ensemble_split3": 61.0, "ensemble_split2": 102.0,
                                         "ensemble_fp16x2": 94.0}
     assert s["point_mass_vs_cpu_baseline"] == 4800.0

@@ -293,6 +293,19 @@ NearArgs near_args(const ztm_plan p) {
     return a;
 }
 
+// The prism zones of the FP32 path for ZTM_FP64_ZONED: FP64 far field with
+// the quadrupole term, FP64 corrections inside the correction box, and no
+// clamped rows (FP64 row terms are never clamped).
+NearArgs near_args64(const ztm_plan p) {
+    NearArgs a = near_args(p);
+    a.mode = kCorrPrism64;
+    a.row_start = nullptr;
+    a.row_st = nullptr;
+    a.rkmin = 1;
+    a.rkmax = 0;
+    return a;
+}
+
 BatchedKernel batched_kernel_in_force(const ztm_plan p, int direction, int n_models) {
     BatchedKernel k = p->tuning.batched[direction][batched_class(n_models)];
     const int split = direction == 0 ? p->tuning.batched_split : batched_adjoint_split(p);
@@ -568,30 +581,39 @@ int ztm_plan_get_tuning(ztm_plan plan, int* st_per_lane, int* rows_per_slice,
 int ztm_plan_forward(ztm_plan plan, int precision, const void* rho_dev, double* g_dev,
                      void* stream) {
     if (!plan) return fail(ZTM_EINVAL, "null plan");
-    if (precision != ZTM_FP32 && precision != ZTM_FP64)
+    if (precision != ZTM_FP32 && precision != ZTM_FP64 && precision != ZTM_FP64_ZONED)
         return fail(ZTM_EINVAL, "unknown precision %d", precision);
     if (plan->shape.m > 0 && (!rho_dev || !g_dev)) return fail(ZTM_EINVAL, "null rho or g");
     int dev = 0;
     ZTM_HIP(hipGetDevice(&dev));
     if (dev != plan->device)
         return fail(ZTM_EINVAL, "plan belongs to device %d, current device is %d", plan->device, dev);
+    const bool zoned = precision == ZTM_FP64_ZONED;
+    if (zoned) precision = ZTM_FP64;   // FP64 arithmetic throughout
     const PmTuning t = effective_tuning(plan, precision);
     hipStream_t s = (hipStream_t)stream;
     if (plan->prob.formula == ZTM_PRISM) {
-        if (precision == ZTM_FP64) {
+        if (precision == ZTM_FP64 && !zoned) {
             if (int rc = ensure_workspace(plan, prism64_workspace(plan->shape, t, false))) return rc;
             return prism64_fwd_launch(plan->shape, t, plan->st_d, (const double*)rho_dev, plan->sy,
                                       plan->sz, plan->prob.threshold, plan->prob.kappa * plan->L,
                                       plan->part, g_dev, s);
         }
-        NearArgs na = near_args(plan);
+        NearArgs na = zoned ? near_args64(plan) : near_args(plan);
         const bool quad = na.quad != 0;
         const size_t ws = pm_fwd_workspace(plan->shape, t, precision, quad);
         if (int rc = ensure_workspace(plan, ws + plan->shape.m_pad)) return rc;
         na.corr = plan->part + ws;
-        if (int rc = prism_near_fwd_launch(plan->shape, na, (const float*)rho_dev, plan->part + ws, s))
+        if (zoned) {
+            if (int rc = prism_near_fwd64_launch(plan->shape, na, (const double*)rho_dev,
+                                                 plan->prob.threshold, plan->part + ws, s))
+                return rc;
+        } else if (int rc = prism_near_fwd_launch(plan->shape, na, (const float*)rho_dev,
+                                                  plan->part + ws, s)) {
             return rc;
-        return pm_fwd_launch(plan->shape, t, precision, plan->st_f, rho_dev, plan->sy, plan->sz,
+        }
+        const void* st = zoned ? (const void*)plan->st_d : (const void*)plan->st_f;
+        return pm_fwd_launch(plan->shape, t, precision, st, rho_dev, plan->sy, plan->sz,
                              plan->prob.threshold, pm_scale(plan), plan->part, g_dev, nullptr,
                              nullptr, na, s);
     }
@@ -606,13 +628,15 @@ int ztm_plan_forward(ztm_plan plan, int precision, const void* rho_dev, double* 
 int ztm_plan_adjoint(ztm_plan plan, int precision, const void* r_dev, void* out_dev,
                      void* stream) {
     if (!plan) return fail(ZTM_EINVAL, "null plan");
-    if (precision != ZTM_FP32 && precision != ZTM_FP64)
+    if (precision != ZTM_FP32 && precision != ZTM_FP64 && precision != ZTM_FP64_ZONED)
         return fail(ZTM_EINVAL, "unknown precision %d", precision);
     if (!out_dev || (plan->shape.m > 0 && !r_dev)) return fail(ZTM_EINVAL, "null r or out");
     int dev = 0;
     ZTM_HIP(hipGetDevice(&dev));
     if (dev != plan->device)
         return fail(ZTM_EINVAL, "plan belongs to device %d, current device is %d", plan->device, dev);
+    const bool zoned = precision == ZTM_FP64_ZONED;
+    if (zoned) precision = ZTM_FP64;
     const PmTuning t = effective_tuning(plan, precision);
     hipStream_t s = (hipStream_t)stream;
     if (plan->shape.m == 0) {
@@ -620,20 +644,23 @@ int ztm_plan_adjoint(ztm_plan plan, int precision, const void* r_dev, void* out_
                              (precision == ZTM_FP64 ? sizeof(double) : sizeof(float));
         return zero_fill(out_dev, bytes, s);
     }
-    if (plan->prob.formula == ZTM_PRISM && precision == ZTM_FP64) {
+    const bool prism = plan->prob.formula == ZTM_PRISM;
+    if (prism && precision == ZTM_FP64 && !zoned) {
         if (int rc = ensure_workspace(plan, prism64_workspace(plan->shape, t, true))) return rc;
         return prism64_adj_launch(plan->shape, t, plan->st_d, (const double*)r_dev, plan->sy,
                                   plan->sz, plan->prob.kappa * plan->L, plan->part,
                                   (double*)out_dev, s);
     }
+    const bool zoned_prism = zoned && prism;
+    const NearArgs na = zoned_prism ? near_args64(plan) : near_args(plan);
     if (int rc = ensure_workspace(
             plan, std::max<size_t>(1, pm_adj_workspace(plan->shape, t, precision,
-                                                       precision == ZTM_FP32,
-                                                       near_args(plan).quad != 0))))
+                                                       precision == ZTM_FP32 || zoned_prism,
+                                                       na.quad != 0))))
         return rc;
     const void* st = precision == ZTM_FP64 ? (const void*)plan->st_d : (const void*)plan->st_f;
     return pm_adj_launch(plan->shape, t, precision, st, r_dev, plan->sy, plan->sz,
-                         pm_scale(plan), plan->part, out_dev, near_args(plan), s);
+                         pm_scale(plan), plan->part, out_dev, na, s);
 }
 
 int64_t ztm_plan_store_bytes(ztm_plan plan) {

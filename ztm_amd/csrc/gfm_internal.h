@@ -57,7 +57,9 @@ struct QuadArgs {
 };
 
 // Correction applied by the reduce kernels of the FP32 path.
-enum CorrMode { kCorrNone = 0, kCorrPmNear = 1, kCorrPrism = 2 };
+// kCorrPrism64: the FP64 path with the FP32 path's prism zones (precision
+// ZTM_FP64_ZONED): mid and near zones corrected in FP64, no clamped rows.
+enum CorrMode { kCorrNone = 0, kCorrPmNear = 1, kCorrPrism = 2, kCorrPrism64 = 3 };
 
 // Everything the FP32 corrections need (mode kCorrNone disables them).
 // st_c[m] = (ci, cj, ck, 0): the station's nearest cell index per axis.
@@ -281,6 +283,9 @@ int tfm_scatter_launch(const double* Y, const int* out_map, int m, int M, int B,
 // runs pm_fwd_launch with near.mode = kCorrPrism and near.corr = corr.
 int prism_near_fwd_launch(const PmShape& s, const NearArgs& near, const float* rho,
                           double* corr, hipStream_t stream);
+// The same in FP64 for ZTM_FP64_ZONED (rho double, FP64 far-field kernel).
+int prism_near_fwd64_launch(const PmShape& s, const NearArgs& near, const double* rho,
+                            double thr, double* corr, hipStream_t stream);
 // FP64: exact closed form on every pair (the FP64 GPU path of the prism).
 size_t prism64_workspace(const PmShape& s, const PmTuning& t, bool adjoint);
 int prism64_fwd_launch(const PmShape& s, const PmTuning& t, const double4* st,
@@ -425,6 +430,7 @@ int ensure_workspace(ztm_plan p, size_t doubles);
 void plan_free(ztm_plan p);
 int set_shape(ztm_plan p, int64_t m);
 NearArgs near_args(const ztm_plan p);
+NearArgs near_args64(const ztm_plan p);   // ZTM_FP64_ZONED prism zones
 double pm_scale(const ztm_plan p);
 // The split the batched adjoint runs: the plan's, except the FP16 split on a
 // plan whose station bounds spread too far for it (then the BF16 three-way).

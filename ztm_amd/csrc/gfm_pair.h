@@ -100,6 +100,69 @@ __device__ __forceinline__ float gl3_cell(float X, float Y, float Z, int i, int 
     return acc;
 }
 
+// FP64 far-field kernel of one pair, as the FP64 main kernels evaluate it
+// (the same formula as pair_k32, in double, without the row clamp).
+template <bool QUAD>
+__device__ __forceinline__ double pair_k64(double a, double R, double qrow, const QuadArgs& q) {
+    const double v = rsqrt(fma(a, a, R));
+    const double v2 = v * v;
+    const double v3 = v2 * v;
+    if constexpr (QUAD) {
+        const double qq = fma(a * (double)q.q2x, a, qrow);
+        const double s = fma(qq, v2, -(double)q.qz);
+        return v3 * fma(s, v2, 1.0);
+    } else {
+        return v3;
+    }
+}
+
+// gl3_cell in FP64 (the mid zone of ZTM_FP64_ZONED).
+__device__ __forceinline__ double gl3_cell64(double X, double Y, double Z, int i, int j, int k,
+                                             double sy, double sz) {
+    const double n = 0.38729833462074170;   // sqrt(3/5)/2
+    const double wgt[3] = {5.0 / 18.0, 8.0 / 18.0, 5.0 / 18.0};
+    const double off[3] = {-n, 0.0, n};
+    const double ay = fabs(sy), az = fabs(sz);
+    double acc = 0.0;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const double dz = Z - ((double)k * sz + off[c] * az);
+#pragma unroll
+        for (int b = 0; b < 3; ++b) {
+            const double dy = Y - ((double)j * sy + off[b] * ay);
+            const double R = fma(dy, dy, dz * dz);
+            double row = 0.0;
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                const double dx = X - ((double)i + off[a]);
+                const double v = rsqrt(fma(dx, dx, R));
+                row = fma(wgt[a], v * v * v, row);
+            }
+            acc = fma(wgt[c] * wgt[b] * dz, row, acc);
+        }
+    }
+    return acc;
+}
+
+// Correction of one pair for ZTM_FP64_ZONED: w * (better - far field), all in
+// FP64; better = the FP64 prism closed form in the near box, FP64
+// Gauss-Legendre in the rest of the correction box (callers visit only
+// those cells).
+template <bool QUAD>
+__device__ __forceinline__ double corr_pair64(double w, const double4& pd, const int4& c, int i,
+                                              int j, int k, const NearArgs& a) {
+    const double ad = pd.x - (double)i;
+    const double dyd = pd.y - (double)j * a.syd;
+    const double dzd = pd.z - (double)k * a.szd;
+    const double R = fma(dyd, dyd, fma(dzd, dzd, kEpsD));
+    const double qrow = QUAD ? fma((double)a.q.q2y, dyd * dyd, (double)a.q.q2z * (dzd * dzd)) : 0.0;
+    const double far = dzd * pair_k64<QUAD>(ad, R, qrow, a.q);
+    const bool in_near = abs(i - c.x) <= a.nrx && abs(j - c.y) <= a.nry && abs(k - c.z) <= a.nrz;
+    const double better = in_near ? prism_cell(pd.x, pd.y, pd.z, i, j, k, a.syd, a.szd) / a.vol
+                                  : gl3_cell64(pd.x, pd.y, pd.z, i, j, k, a.syd, a.szd);
+    return w * (better - far);
+}
+
 // Correction of one station-cell pair for the FP32 path: the better value of
 // the pair minus its FP32 far-field term as the main kernels evaluate it (same
 // rounded coordinates, same row clamp, same v_rsq_f32), so adding it replaces

@@ -64,8 +64,7 @@ __device__ __forceinline__ T kterm(T a, T R, T qrow, const QuadArgs& q) {
     if constexpr (sizeof(T) == sizeof(float)) {
         return pair_k32<QUAD>(a, R, qrow, q);
     } else {
-        const T v = rsq(fma(a, a, R));
-        return v * v * v;
+        return pair_k64<QUAD>(a, R, qrow, q);
     }
 }
 
@@ -325,7 +324,7 @@ __device__ __forceinline__ double pm_fwd_reduce_wave(const double* __restrict__ 
     }
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
-    if (na.mode == kCorrPrism) s += na.corr[i];
+    if (na.mode == kCorrPrism || na.mode == kCorrPrism64) s += na.corr[i];
     return s * scale;
 }
 
@@ -610,6 +609,62 @@ __device__ __forceinline__ double pm_adj_reduce_wave(const double* __restrict__ 
 #pragma unroll
     for (int off = kNearLanes / 2; off > 0; off >>= 1) s += __shfl_xor(s, off);
     return s;
+}
+
+// ZTM_FP64_ZONED (kCorrPrism64): the same walk over the stations whose
+// correction box holds the cell, with FP64 residuals and the FP64
+// correction (no clamped rows in FP64).
+template <bool QUAD>
+__device__ __forceinline__ double pm_adj_reduce_wave64(const double* __restrict__ part,
+                                                       int n_splits, int64_t n_cells, int64_t n,
+                                                       const double* __restrict__ r_d,
+                                                       const NearArgs& na, int lane) {
+    double s = 0.0;
+    for (int k = lane; k < n_splits; k += kNearLanes) s += part[(size_t)k * n_cells + n];
+    const int ii = (int)(n % na.nx);
+    const int64_t rw = n / na.nx;
+    const int jj = (int)(rw % na.ny);
+    const int kk = (int)(rw / na.ny);
+    if (kk >= na.kmin - na.rz && kk <= na.kmax + na.rz) {
+        const int wcols = na.nx + 2 * na.cmx;
+        const int bw = 2 * na.rx + 1;
+        const int total = bw * (2 * na.ry + 1);
+        for (int t = lane; t < total; t += kNearLanes) {
+            const int ci = ii - na.rx + t % bw;
+            const int cj = jj - na.ry + t / bw;
+            if (cj < -na.cmy || cj >= na.ny + na.cmy) continue;
+            if (ci < -na.cmx || ci >= na.nx + na.cmx) continue;
+            const int col = (cj + na.cmy) * wcols + (ci + na.cmx);
+            for (int q = na.col_start[col]; q < na.col_start[col + 1]; ++q) {
+                const int m = na.col_st[q];
+                const int4 c = na.st_c[m];
+                if (c.z < kk - na.rz || c.z > kk + na.rz) continue;
+                if (abs(ii - c.x) > na.rx || abs(jj - c.y) > na.ry) continue;
+                s += corr_pair64<QUAD>(r_d[m], na.st_d[m], c, ii, jj, kk, na);
+            }
+        }
+    }
+#pragma unroll
+    for (int off = kNearLanes / 2; off > 0; off >>= 1) s += __shfl_xor(s, off);
+    return s;
+}
+
+template <bool QUAD>
+__global__ __launch_bounds__(256) void pm_adj_reduce64_kernel(const double* __restrict__ part,
+                                                              int n_splits, int64_t n_cells,
+                                                              int64_t n0, int64_t count,
+                                                              double scale,
+                                                              double* __restrict__ out,
+                                                              const double* __restrict__ r_d,
+                                                              NearArgs na) {
+    const int lane = threadIdx.x % kNearLanes;
+    constexpr int kPerBlock = 256 / kNearLanes;
+    for (int64_t t = (int64_t)blockIdx.x * kPerBlock + threadIdx.x / kNearLanes; t < count;
+         t += (int64_t)gridDim.x * kPerBlock) {   // uniform per half-wave
+        const double v = pm_adj_reduce_wave64<QUAD>(part, n_splits, n_cells, n0 + t, r_d, na,
+                                                    lane);
+        if (lane == 0) out[n0 + t] = scale * v;
+    }
 }
 
 // Cells outside the near-field layers [klo, khi]: the split-K sum only. The
@@ -910,7 +965,7 @@ void launch_fwd(const PmShape& s, int rps, int n_slices, const void* st,
     const size_t lds = std::max((size_t)((rps + G - 1) / G * G) * nxp * sizeof(T),
                                 (size_t)4 * S * 64 * sizeof(double));
     const dim3 grid(s.m_pad / (64 * S), n_slices);
-    if (quad && sizeof(T) == sizeof(float))
+    if (quad)
         hipLaunchKernelGGL((pm_fwd_kernel<T, S, true>), grid, dim3(256), lds, stream,
                            (const V4*)st, (const T*)rho, s.nx, s.ny, (int)s.n_rows, sy, sz,
                            thr, rps, part, s.m_pad, qa);
@@ -931,7 +986,7 @@ void launch_adj(const PmShape& s, int splits, int direct, const void* st, const 
     const int tiles = s.m_pad / 256;
     const int per_split = ((tiles + splits - 1) / splits) * 256;
     const size_t lds = adj_lds(C, sizeof(T), quad);
-    if (quad && sizeof(T) == sizeof(float))
+    if (quad)
         hipLaunchKernelGGL((pm_adj_kernel<T, C, true>), dim3(blocks_x, splits), dim3(256),
                            lds, stream, (const V4*)st, (const T*)r, s.m, s.nx, s.ny,
                            sy, sz, segs_per_row, n_segs, per_split, s.m_pad, part,
@@ -947,7 +1002,7 @@ void launch_adj(const PmShape& s, int splits, int direct, const void* st, const 
 
 size_t pm_fwd_workspace(const PmShape& s, const PmTuning& t, int precision, bool quad) {
     const size_t elem = precision == ZTM_FP64 ? sizeof(double) : sizeof(float);
-    const int rps = fwd_rows_per_slice(s, t, elem, quad && precision == ZTM_FP32);
+    const int rps = fwd_rows_per_slice(s, t, elem, quad);
     const int64_t n_slices = (s.n_rows + rps - 1) / rps;
     return (size_t)n_slices * (size_t)s.m_pad;
 }
@@ -956,7 +1011,7 @@ size_t pm_adj_workspace(const PmShape& s, const PmTuning& t, int precision, bool
                         bool quad) {
     // near: a correction will run, so the main kernel writes partials
     const size_t elem = precision == ZTM_FP64 ? sizeof(double) : sizeof(float);
-    const int sp = adj_splits(s, t, elem, quad && precision == ZTM_FP32);
+    const int sp = adj_splits(s, t, elem, quad);
     return (sp > 1 || near) ? (size_t)sp * (size_t)s.n_cells : 0;
 }
 
@@ -972,8 +1027,10 @@ int pm_fwd_launch(const PmShape& s, const PmTuning& t0, int precision,
                     (int)(65536 / elem) - 3);
     if (s.m == 0) return ZTM_OK;
     NearArgs na = near;
-    if (precision == ZTM_FP64) na.mode = kCorrNone;   // FP64 needs no correction
-    const bool quad = na.quad != 0 && precision == ZTM_FP32;
+    // FP64 needs no near-field correction; with the FP32 path's prism zones
+    // (kCorrPrism64) it adds the per-station FP64 corrections computed before
+    if (precision == ZTM_FP64 && na.mode != kCorrPrism64) na.mode = kCorrNone;
+    const bool quad = na.quad != 0 && (precision == ZTM_FP32 || na.mode == kCorrPrism64);
     const PmTuning t = fwd_adjust(s, t0, elem, quad);
     const int rps = fwd_rows_per_slice(s, t, elem, quad);
     const int n_slices = (int)((s.n_rows + rps - 1) / rps);
@@ -983,9 +1040,9 @@ int pm_fwd_launch(const PmShape& s, const PmTuning& t0, int precision,
         if (precision == ZTM_FP64) {
             const double a = sy, b = sz, c = thr;
             switch (t.st_per_lane) {
-                case 1: launch_fwd<double, 1>(s, rps, n_slices, st, rho, a, b, c, part, false, na.q, stream); break;
-                case 2: launch_fwd<double, 2>(s, rps, n_slices, st, rho, a, b, c, part, false, na.q, stream); break;
-                default: launch_fwd<double, 4>(s, rps, n_slices, st, rho, a, b, c, part, false, na.q, stream); break;
+                case 1: launch_fwd<double, 1>(s, rps, n_slices, st, rho, a, b, c, part, quad, na.q, stream); break;
+                case 2: launch_fwd<double, 2>(s, rps, n_slices, st, rho, a, b, c, part, quad, na.q, stream); break;
+                default: launch_fwd<double, 4>(s, rps, n_slices, st, rho, a, b, c, part, quad, na.q, stream); break;
             }
         } else {
             const float a = (float)sy, b = (float)sz, c = (float)thr;
@@ -1080,8 +1137,8 @@ int pm_adj_launch(const PmShape& s, const PmTuning& t, int precision,
                   hipStream_t stream) {
     if (s.n_cells == 0) return ZTM_OK;
     NearArgs na = near;
-    if (precision == ZTM_FP64) na.mode = kCorrNone;
-    const bool quad = na.quad != 0 && precision == ZTM_FP32;
+    if (precision == ZTM_FP64 && na.mode != kCorrPrism64) na.mode = kCorrNone;
+    const bool quad = na.quad != 0 && (precision == ZTM_FP32 || na.mode == kCorrPrism64);
     const size_t elem = precision == ZTM_FP64 ? sizeof(double) : sizeof(float);
     const int splits = adj_splits(s, t, elem, quad);
     const int direct = (splits == 1 && na.mode == kCorrNone) ? 1 : 0;
@@ -1089,8 +1146,8 @@ int pm_adj_launch(const PmShape& s, const PmTuning& t, int precision,
         ProfScope prof("pm_adj", stream);
         if (precision == ZTM_FP64) {
             switch (t.cells_per_lane) {
-                case 4: launch_adj<double, 4>(s, splits, direct, st, r, sy, sz, part, out, scale, false, na.q, stream); break;
-                default: launch_adj<double, 8>(s, splits, direct, st, r, sy, sz, part, out, scale, false, na.q, stream); break;
+                case 4: launch_adj<double, 4>(s, splits, direct, st, r, sy, sz, part, out, scale, quad, na.q, stream); break;
+                default: launch_adj<double, 8>(s, splits, direct, st, r, sy, sz, part, out, scale, quad, na.q, stream); break;
             }
         } else {
             const float a = (float)sy, b = (float)sz;
@@ -1115,7 +1172,19 @@ int pm_adj_launch(const PmShape& s, const PmTuning& t, int precision,
             hipLaunchKernelGGL((pm_adj_sum_kernel<float>), dim3(blocks), dim3(256), 0, stream,
                                part, splits, s.n_cells, plane, klo, khi, scale, (float*)out);
         ZTM_HIP(hipGetLastError());
-        if (klo <= khi && precision == ZTM_FP32) {
+        if (klo <= khi && na.mode == kCorrPrism64) {
+            const int64_t n0 = klo * plane, count = (int64_t)(khi - klo + 1) * plane;
+            const unsigned nb = (unsigned)std::min<int64_t>((count + 7) / 8, 1 << 20);
+            if (quad)
+                hipLaunchKernelGGL((pm_adj_reduce64_kernel<true>), dim3(nb), dim3(256), 0, stream,
+                                   part, splits, s.n_cells, n0, count, scale, (double*)out,
+                                   (const double*)r, na);
+            else
+                hipLaunchKernelGGL((pm_adj_reduce64_kernel<false>), dim3(nb), dim3(256), 0, stream,
+                                   part, splits, s.n_cells, n0, count, scale, (double*)out,
+                                   (const double*)r, na);
+            ZTM_HIP(hipGetLastError());
+        } else if (klo <= khi && precision == ZTM_FP32) {
             const int64_t n0 = klo * plane, count = (int64_t)(khi - klo + 1) * plane;
             const unsigned nb = (unsigned)std::min<int64_t>((count + 7) / 8, 1 << 20);
 #define ZTM_ADJ_NEAR(Q, M)                                                                 \

@@ -77,6 +77,54 @@ int prism_near_fwd_launch(const PmShape& s, const NearArgs& near, const float* r
     return ZTM_OK;
 }
 
+// ZTM_FP64_ZONED: the same walk over the correction box in FP64 (FP64
+// densities and threshold, corr_pair64: FP64 Gauss-Legendre mid zone, closed
+// form near zone); FP64 rows are never clamped, so there is no second part.
+template <bool QUAD>
+__global__ __launch_bounds__(256) void prism_near_fwd64_kernel(NearArgs na,
+                                                               const double* __restrict__ rho,
+                                                               double thr, int m,
+                                                               double* __restrict__ corr) {
+    const int lane = threadIdx.x & 63;
+    const int st = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (st >= m) return;   // uniform per wave
+    const double4 pd = na.st_d[st];
+    const int4 c = na.st_c[st];
+    const int i0 = max(c.x - na.rx, 0), i1 = min(c.x + na.rx, na.nx - 1);
+    const int j0 = max(c.y - na.ry, 0), j1 = min(c.y + na.ry, na.ny - 1);
+    const int k0 = max(c.z - na.rz, 0), k1 = min(c.z + na.rz, na.nz - 1);
+    const int bi = i1 - i0 + 1, bj = j1 - j0 + 1, bk = k1 - k0 + 1;
+    const int total = (bi > 0 && bj > 0 && bk > 0) ? bi * bj * bk : 0;
+    double acc = 0.0;
+    for (int t = lane; t < total; t += 64) {
+        const int ii = i0 + t % bi;
+        const int rest = t / bi;
+        const int jj = j0 + rest % bj;
+        const int kk = k0 + rest / bj;
+        const double w = rho[((size_t)kk * na.ny + jj) * na.nx + ii];
+        if (!(fabs(w) > thr)) continue;
+        acc += corr_pair64<QUAD>(w, pd, c, ii, jj, kk, na);
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
+    if (lane == 0) corr[st] = acc;
+}
+
+int prism_near_fwd64_launch(const PmShape& s, const NearArgs& near, const double* rho,
+                            double thr, double* corr, hipStream_t stream) {
+    if (s.m == 0) return ZTM_OK;
+    const dim3 grid((s.m + 3) / 4);
+    ProfScope prof("prism_near_fwd64", stream);
+    if (near.quad)
+        hipLaunchKernelGGL(prism_near_fwd64_kernel<true>, grid, dim3(256), 0, stream, near, rho,
+                           thr, s.m, corr);
+    else
+        hipLaunchKernelGGL(prism_near_fwd64_kernel<false>, grid, dim3(256), 0, stream, near, rho,
+                           thr, s.m, corr);
+    ZTM_HIP(hipGetLastError());
+    return ZTM_OK;
+}
+
 // ----------------------------------------------------- FP64 exact kernels
 // Four-corner sum over (y, z) at one x-corner plane, station-relative.
 __device__ __forceinline__ double corner_plane(double u, double v0, double v1, double w0,

@@ -25,7 +25,7 @@ import numpy as np
 _PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_PKG, "libztm_gfm.so")
 
-FP32, FP64 = 0, 1
+FP32, FP64, FP64_ZONED = 0, 1, 2
 POINT_MASS, PRISM = 0, 1
 KAPPA_SI = 6.674e-6
 KAPPA_REF = 66.7
@@ -441,11 +441,11 @@ class Plan:
         check(lib().ztm_plan_adjoint(self._h, precision, r_ptr, out_ptr, stream))
 
     # torch-tensor conveniences
-    def forward(self, rho, out=None, stream=None):
+    def forward(self, rho, out=None, stream=None, zoned=False):
         """g[M] (float64) = kappa * sum_n rho[n] K(m,n); rho float32 -> FP32 path,
-        float64 -> FP64 path."""
+        float64 -> FP64 path (zoned=True: FP64 with the FP32 path's prism zones)."""
         import torch
-        prec = _prec_of(rho, self.n_cells, "rho")
+        prec = _prec_of(rho, self.n_cells, "rho", zoned)
         if out is None:
             out = torch.empty(self.n_stations, dtype=torch.float64, device=rho.device)
         _check_t(out, torch.float64, self.n_stations, "g")
@@ -557,10 +557,11 @@ class Plan:
         check(lib().ztm_plan_smoothness(self._h, prec, _ptr(rho), _ptr(out), _stream(stream)))
         return out
 
-    def adjoint(self, r, out=None, stream=None):
-        """out[N] = kappa * sum_m K(m,n) r[m], in r's precision (float32/float64)."""
+    def adjoint(self, r, out=None, stream=None, zoned=False):
+        """out[N] = kappa * sum_m K(m,n) r[m], in r's precision (float32/float64;
+        zoned=True as for forward)."""
         import torch
-        prec = _prec_of(r, self.n_stations, "r")
+        prec = _prec_of(r, self.n_stations, "r", zoned)
         if out is None:
             out = torch.empty(self.n_cells, dtype=r.dtype, device=r.device)
         _check_t(out, r.dtype, self.n_cells, "out")
@@ -715,14 +716,16 @@ class Batch:
         check(lib().ztm_batch_autotune(self._h, reps, _stream(stream)))
 
 
-def _prec_of(t, n: int, name: str) -> int:
+def _prec_of(t, n: int, name: str, zoned: bool = False) -> int:
     import torch
     if not (isinstance(t, torch.Tensor) and t.is_cuda and t.is_contiguous() and t.numel() == n):
         raise ValueError(f"{name} must be a contiguous CUDA tensor of {n} values")
+    if zoned and t.dtype != torch.float64:
+        raise ValueError(f"zoned=True is an FP64 path: {name} must be float64, got {t.dtype}")
     if t.dtype == torch.float32:
         return FP32
     if t.dtype == torch.float64:
-        return FP64
+        return FP64_ZONED if zoned else FP64
     raise ValueError(f"{name} must be float32 or float64, got {t.dtype}")
 
 
