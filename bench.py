@@ -265,7 +265,10 @@ def summary(line):
                "ensemble_split2": get(line, "batched", "split2", "forward_plus_adjoint",
                                       "speedup_vs_fp64_gpu"),
                "ensemble_fp16x2": get(line, "batched", "fp16x2", "forward_plus_adjoint",
-                                      "speedup_vs_fp64_gpu")},
+                                      "speedup_vs_fp64_gpu"),
+               "ensemble_fp16x2_vs_fp64_same_algorithm": get(
+                   line, "batched", "fp16x2", "forward_plus_adjoint",
+                   "speedup_vs_fp64_same_algorithm")},
            "point_mass_vs_cpu_baseline": line.get("speedup_vs_cpu"),
            "max_rel_err_vs_fp64": {
                "point_mass_fp32": worst(line.get("rel_err_vs_fp64")),
@@ -341,6 +344,7 @@ def stored_leg(plan, rho32, r32, g64h, stream, args):
 
 
 PEAK_BF16_TFLOPS = 2500.0     # MI355X dense BF16 (and FP16) MFMA, published
+PEAK_FP64_TFLOPS = 78.6       # MI355X FP64 (matrix and vector), published
 
 
 def batched_leg(stream, args):
@@ -349,7 +353,7 @@ def batched_leg(stream, args):
     residual vectors N(0, 0.5)), BF16 split 3 and 2 and the FP16 hi/lo split,
     against the FP64 single-model
     kernels on models / vectors 0, 85, 170, 255 (extrapolated to 256 for the
-    FP64 rates)."""
+    FP64 rates) and against the FP64 ensemble (the same algorithm in FP64)."""
     _progress("batched leg")
     import torch
     from ztm_amd import gfm, workloads as W
@@ -387,6 +391,43 @@ def batched_leg(stream, args):
                                    "kind": "FP64 single-vector adjoint, timed on 4 vectors "
                                            "(resident, no copies in the timed region)"}
         fp64_fa = 2.0 * pm / ((t64 + ta64) / len(check) * B)
+        # the same algorithm in FP64: G once per tile in FP64, reused across the
+        # 256 models on FP64 MFMA (inputs are the same values in double)
+        _progress("ensemble FP64 (same algorithm)")
+        P64 = P.double()
+        R64 = Rt.double()
+        Y64 = torch.empty(B, w.n_stations, dtype=torch.float64, device=dev)
+        X64 = torch.empty(B, w.grid.n_cells, dtype=torch.float64, device=dev)
+        same = {}
+        for name, fn, kname in (
+                ("forward", lambda: p.forward_batched64(P64, out=Y64, stream=stream),
+                 "batched64_fwd"),
+                ("adjoint", lambda: p.adjoint_batched64(R64, out=X64, stream=stream),
+                 "batched64_adj")):
+            gfm.profile_reset()
+            gfm.profile_enable(True)
+            tt = _timed(fn, 2, torch.cuda.synchronize)
+            gfm.profile_enable(False)
+            ms, n = gfm.profile_read(kname)
+            gfm.profile_reset()
+            km = ms / max(n, 1)
+            flop = 2.0 * pm
+            ref = g64 if name == "forward" else a64
+            got = (Y64 if name == "forward" else X64).cpu().numpy()
+            same[name] = {"value": pm / tt, "ms_per_call": tt * 1e3, "kernel_ms": km,
+                          "rel_err_vs_fp64_single": max(relerr(got[b], ref[b]) for b in check),
+                          "roofline": {"bound": "mfma", "unit": "TFLOP/s",
+                                       "peak": PEAK_FP64_TFLOPS,
+                                       "achieved": flop / (km * 1e-3) / 1e12,
+                                       "frac": flop / (km * 1e-3) / 1e12 / PEAK_FP64_TFLOPS,
+                                       "flop": "2 x pair-models on v_mfma_f64_16x16x4_f64"}}
+        del P64, R64, Y64, X64
+        t_same = same["forward"]["ms_per_call"] + same["adjoint"]["ms_per_call"]
+        same["forward_plus_adjoint"] = {"value": 2.0 * pm / (t_same * 1e-3),
+                                        "ms_per_pair_of_calls": t_same}
+        same["kind"] = ("FP64 ensemble: G generated once per tile in FP64 and reused across "
+                        "the batch on FP64 MFMA (ztm_plan_forward/adjoint_batched64)")
+        out["fp64_gpu_same_algorithm"] = same
         for split in (3, 2, 4):
             _progress(f"ensemble split {split}")
             p.set_batched_tuning(split=split)
@@ -429,9 +470,11 @@ def batched_leg(stream, args):
                              "frac": mfma_flop / (kma * 1e-3) / 1e12 / PEAK_BF16_TFLOPS},
                 "speedup_vs_fp64_gpu": (pm / ta) / out["fp64_gpu_adjoint"]["value"]}
             fa = 2.0 * pm / (t + ta)
-            leg["forward_plus_adjoint"] = {"value": fa, "unit": "pair-models/s",
-                                           "ms_per_pair_of_calls": (t + ta) * 1e3,
-                                           "speedup_vs_fp64_gpu": fa / fp64_fa}
+            leg["forward_plus_adjoint"] = {
+                "value": fa, "unit": "pair-models/s", "ms_per_pair_of_calls": (t + ta) * 1e3,
+                "speedup_vs_fp64_gpu": fa / fp64_fa,
+                "speedup_vs_fp64_same_algorithm":
+                    fa / same["forward_plus_adjoint"]["value"]}
             leg["autotune"] = {"s": t_tune,
                                "forward": dict(zip(("variant", "waves", "slices"),
                                                    p.batched_kernel(0, B))),

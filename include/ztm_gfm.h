@@ -220,6 +220,19 @@ int ztm_plan_forward_batched(ztm_plan plan, int n_models, const float* rho_dev,
  * ztm_plan_adjoint (FP32) per vector: the transpose of that forward. */
 int ztm_plan_adjoint_batched(ztm_plan plan, int n_vectors, const float* r_dev,
                              float* out_dev, void* stream);
+/* FP64 ensemble (no reference counterpart; the FP64 form of the batched
+ * ensemble, for callers that need FP64 and as its same-algorithm FP64
+ * comparison): G generated once per tile in FP64, reused across the whole
+ * batch on FP64 MFMA, double inputs, accumulation and outputs. Point-mass
+ * plans; n <= 256 per call; no threshold on r (the transpose), the plan's
+ * threshold on rho. Forward: rho_dev double [n_models][N] -> g_dev double
+ * [n_models][M]. Adjoint: r_dev double [n_vectors][M] -> out_dev double
+ * [n_vectors][N]. G is the exact point mass dz/r^3 on every pair, as in the
+ * FP64 single-model calls, which these match to FP64 rounding. */
+int ztm_plan_forward_batched64(ztm_plan plan, int n_models, const double* rho_dev,
+                               double* g_dev, void* stream);
+int ztm_plan_adjoint_batched64(ztm_plan plan, int n_vectors, const double* r_dev,
+                               double* out_dev, void* stream);
 /* Batched tuning: K-slices (0 = choose from the problem shape), in both
  * directions and every class: each slice accumulates in FP32 on the MFMAs, whose accumulate
  * loses a biased fraction of an ulp per 16-cell step, so the error grows

@@ -29,12 +29,15 @@ def test_summary_gathers_the_legs():
                                    "forward_plus_adjoint": {"speedup_vs_fp64_gpu": 102.0}},
                         "fp16x2": {"rel_err_vs_fp64": 1.5e-6,
                                    "adjoint": {"rel_err_vs_fp64": 1.7e-6},
-                                   "forward_plus_adjoint": {"speedup_vs_fp64_gpu": 94.0}}}}
+                                   "forward_plus_adjoint": {
+                                       "speedup_vs_fp64_gpu": 94.0,
+                                       "speedup_vs_fp64_same_algorithm": 7.0}}}}
     s = b.summary(line)
     assert s["fwd_adj_vs_fp64_gpu"] == {"point_mass_fp32": 3.2, "prism_fp32": 220.0,
                                         "prism_fp32_vs_fp64_same_algorithm": 3.1,
                                         "ensemble_split3": 61.0, "ensemble_split2": 102.0,
-                                        "ensemble_fp16x2": 94.0}
+                                        "ensemble_fp16x2": 94.0,
+                                        "ensemble_fp16x2_vs_fp64_same_algorithm": 7.0}
     assert s["point_mass_vs_cpu_baseline"] == 4800.0
     assert s["max_rel_err_vs_fp64"] == {"point_mass_fp32": 3e-7, "prism_fp32": 5e-7,
                                         "ensemble_split3": 1.9e-6, "ensemble_split2": 6.1e-6,

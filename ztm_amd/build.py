@@ -22,7 +22,8 @@ OBJ = os.path.join(PKG, "_build")
 LIB = os.path.join(PKG, "libztm_gfm.so")
 GFM = os.path.join(PKG, "bin", "GFM")
 
-HIP_SOURCES = ["gfm_abi.hip", "gfm_batch.hip", "gfm_batched.hip", "gfm_noalloc.hip",
+HIP_SOURCES = ["gfm_abi.hip", "gfm_batch.hip", "gfm_batched.hip", "gfm_batched64.hip",
+               "gfm_noalloc.hip",
                "gfm_pointmass.hip", "gfm_prism.hip", "gfm_reg.hip", "gfm_stored.hip",
                "gfm_tfm.hip"]
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")

@@ -792,6 +792,51 @@ int ztm_plan_adjoint_batched(ztm_plan plan, int n_vectors, const float* r_dev, f
                               (hipStream_t)stream, k.variant, k.waves);
 }
 
+// Shared checks of the FP64 ensemble calls: ZTM_OK, an error, or -1 when
+// there is nothing to compute (no models, cells or stations).
+static int batched64_check(ztm_plan plan, int n, const void* in, const void* out,
+                           const char* what) {
+    if (!plan) return fail(ZTM_EINVAL, "null plan");
+    if (n < 0 || n > 256) return fail(ZTM_EINVAL, "%s must be in [0, 256] (got %d)", what, n);
+    if (plan->prob.formula != ZTM_POINT_MASS)
+        return fail(ZTM_EINVAL, "the FP64 ensemble supports the point-mass formula only");
+    if (n == 0 || plan->shape.n_cells == 0) return -1;
+    if (!out || (plan->shape.m > 0 && !in)) return fail(ZTM_EINVAL, "null input or output");
+    int dev = 0;
+    ZTM_HIP(hipGetDevice(&dev));
+    if (dev != plan->device)
+        return fail(ZTM_EINVAL, "plan belongs to device %d, current device is %d", plan->device, dev);
+    return plan->shape.m == 0 ? -1 : ZTM_OK;
+}
+
+int ztm_plan_forward_batched64(ztm_plan plan, int n_models, const double* rho_dev,
+                               double* g_dev, void* stream) {
+    const int rc = batched64_check(plan, n_models, rho_dev, g_dev, "n_models");
+    if (rc) return rc < 0 ? ZTM_OK : rc;
+    if (int e = ensure_batched_workspace(plan, batched64_fwd_workspace(plan->shape, n_models)))
+        return e;
+    return batched64_fwd_launch(plan->shape, n_models, plan->st_d, rho_dev, plan->prob.threshold,
+                                plan->sy, plan->sz, pm_scale(plan), (double*)plan->bws, g_dev,
+                                (hipStream_t)stream);
+}
+
+int ztm_plan_adjoint_batched64(ztm_plan plan, int n_vectors, const double* r_dev,
+                               double* out_dev, void* stream) {
+    const int rc = batched64_check(plan, n_vectors, r_dev, out_dev, "n_vectors");
+    if (rc > 0) return rc;
+    if (rc < 0) {   // no stations: G^T r = 0 (nothing else to do)
+        if (n_vectors == 0 || plan->shape.n_cells == 0) return ZTM_OK;
+        return zero_fill(out_dev, sizeof(double) * (size_t)n_vectors * plan->shape.n_cells,
+                         (hipStream_t)stream);
+    }
+    if (int e = ensure_batched_workspace(
+            plan, std::max<size_t>(1, batched64_adj_workspace(plan->shape, n_vectors))))
+        return e;
+    return batched64_adj_launch(plan->shape, n_vectors, plan->st_d, r_dev, plan->sy, plan->sz,
+                                pm_scale(plan), (double*)plan->bws, out_dev,
+                                (hipStream_t)stream);
+}
+
 int ztm_plan_get_batched_tuning(ztm_plan plan, int* slices, int* split, int* adjoint_split,
                                 int* variant) {
     if (!plan) return fail(ZTM_EINVAL, "null plan");

@@ -346,6 +346,16 @@ int batched_adj_launch(const PmShape& s, int B, int ks, int split, const float* 
                        const float* R, double sy, double sz, double scale, double gmax, void* ws,
                        float* X, hipStream_t stream, int variant = 0, int waves = 4);
 
+// FP64 ensemble on FP64 MFMA (gfm_batched64.hip): workspace bytes and the
+// K-slices it allows; the launchers (B in [1, 256], n_cells < 2^31).
+size_t batched64_fwd_workspace(const PmShape& s, int B, int* ks_out = nullptr);
+size_t batched64_adj_workspace(const PmShape& s, int B, int* ks_out = nullptr);
+int batched64_fwd_launch(const PmShape& s, int B, const double4* st, const double* P,
+                         double thr, double sy, double sz, double scale, double* ws, double* Y,
+                         hipStream_t stream);
+int batched64_adj_launch(const PmShape& s, int B, const double4* st, const double* R, double sy,
+                         double sz, double scale, double* ws, double* X, hipStream_t stream);
+
 // Power-of-two G scale of the FP16 parts for a bound gmax on |G|: brings it
 // to at most 2^14; 1 when gmax is 0.
 inline float half_g_scale(double gmax) {

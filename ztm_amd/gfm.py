@@ -72,6 +72,8 @@ SIGNATURES = {
     "ztm_plan_smoothness": (_I, [_P, _I, _P, _P, _P]),
     "ztm_plan_forward_batched": (_I, [_P, _I, _P, _P, _P]),
     "ztm_plan_adjoint_batched": (_I, [_P, _I, _P, _P, _P]),
+    "ztm_plan_forward_batched64": (_I, [_P, _I, _P, _P, _P]),
+    "ztm_plan_adjoint_batched64": (_I, [_P, _I, _P, _P, _P]),
     "ztm_plan_set_batched_tuning": (_I, [_P, _I, _I]),
     "ztm_plan_get_batched_tuning": (_I, [_P, _IP, _IP, _IP, _IP]),
     "ztm_plan_set_batched_variant": (_I, [_P, _I]),
@@ -477,6 +479,34 @@ class Plan:
             out = torch.empty(B, self.n_cells, dtype=torch.float32, device=r.device)
         _check_t(out, torch.float32, B * self.n_cells, "out")
         check(lib().ztm_plan_adjoint_batched(self._h, B, _ptr(r), _ptr(out), _stream(stream)))
+        return out
+
+    def forward_batched64(self, rho, out=None, stream=None):
+        """rho float64 [B, N] (B <= 256) -> g float64 [B, M]: the ensemble in FP64
+        (G generated once per tile, reused across the batch on FP64 MFMA)."""
+        import torch
+        if not (isinstance(rho, torch.Tensor) and rho.is_cuda and rho.dtype == torch.float64
+                and rho.is_contiguous() and rho.dim() == 2 and rho.shape[1] == self.n_cells):
+            raise ValueError(f"rho must be a contiguous float64 CUDA tensor [B, {self.n_cells}]")
+        B = rho.shape[0]
+        if out is None:
+            out = torch.empty(B, self.n_stations, dtype=torch.float64, device=rho.device)
+        _check_t(out, torch.float64, B * self.n_stations, "g")
+        check(lib().ztm_plan_forward_batched64(self._h, B, _ptr(rho), _ptr(out),
+                                               _stream(stream)))
+        return out
+
+    def adjoint_batched64(self, r, out=None, stream=None):
+        """r float64 [B, M] (B <= 256) -> G^T r float64 [B, N] on FP64 MFMA."""
+        import torch
+        if not (isinstance(r, torch.Tensor) and r.is_cuda and r.dtype == torch.float64
+                and r.is_contiguous() and r.dim() == 2 and r.shape[1] == self.n_stations):
+            raise ValueError(f"r must be a contiguous float64 CUDA tensor [B, {self.n_stations}]")
+        B = r.shape[0]
+        if out is None:
+            out = torch.empty(B, self.n_cells, dtype=torch.float64, device=r.device)
+        _check_t(out, torch.float64, B * self.n_cells, "out")
+        check(lib().ztm_plan_adjoint_batched64(self._h, B, _ptr(r), _ptr(out), _stream(stream)))
         return out
 
     def batched_tuning(self):
