@@ -885,10 +885,22 @@ int ztm_plan_get_batched_kernel(ztm_plan plan, int direction, int n_models, int*
 }
 
 namespace {
-__global__ void fill_ones_kernel(float* __restrict__ x, int64_t n) {
+// Pseudo-random values in [-1, 1) (an integer hash of the index): the MFMA
+// kernels' clock, and so their times, depend on how much their operands'
+// bits toggle -- on low-entropy data such as alternating 1 and -0.5 the same
+// kernels ran 20-25% faster than on real models -- so the timed inputs must
+// look like real ones.
+__global__ void fill_random_kernel(float* __restrict__ x, int64_t n, unsigned seed) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x)
-        x[i] = (i & 1) ? 1.f : -0.5f;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        unsigned h = (unsigned)i * 0x9E3779B1u ^ seed;
+        h ^= h >> 16;
+        h *= 0x85EBCA6Bu;
+        h ^= h >> 13;
+        h *= 0xC2B2AE35u;
+        h ^= h >> 16;
+        x[i] = (float)(h >> 8) * (1.f / 8388608.f) - 1.f;
+    }
 }
 }  // namespace
 
@@ -926,8 +938,8 @@ int ztm_plan_autotune_batched(ztm_plan plan, int n_models, int reps, void* strea
     if (e == hipSuccess) e = hipEventCreate(&e1);
     int rc = e == hipSuccess ? ZTM_OK : hip_fail(e, "autotune buffers");
     if (rc == ZTM_OK) {
-        hipLaunchKernelGGL(fill_ones_kernel, dim3(4096), dim3(256), 0, s, P, N * n_models);
-        hipLaunchKernelGGL(fill_ones_kernel, dim3(1024), dim3(256), 0, s, R, M * n_models);
+        hipLaunchKernelGGL(fill_random_kernel, dim3(4096), dim3(256), 0, s, P, N * n_models, 1u);
+        hipLaunchKernelGGL(fill_random_kernel, dim3(1024), dim3(256), 0, s, R, M * n_models, 2u);
         if (hipGetLastError() != hipSuccess) rc = fail(ZTM_EHIP, "autotune fill launch failed");
     }
     for (int dir = 0; dir < 2 && rc == ZTM_OK; ++dir) {
