@@ -65,8 +65,31 @@ def lib():
         L.oracle_tfm_drho.restype = i64
         L.oracle_prism_kernel.argtypes = [f64] * 9
         L.oracle_prism_kernel.restype = f64
+        L.oracle_mtm.argtypes = [i64, i32, P, P, i32]
+        L.oracle_jtr.argtypes = [i64, i32, P, P, P]
+        L.oracle_inv.argtypes = [i32, P, P]
+        L.oracle_inv.restype = i32
+        L.oracle_gn_steps.argtypes = [i32, i32, P, P, P, P, P, P, i32]
+        L.oracle_inv_diag.argtypes = [i32, P, f64, P]
         _lib = L
     return _lib
+
+
+_REF_PATH = os.path.join(_HERE, "_ref", "libzmath_ref.so")
+_ref = None
+
+
+def ref_lib():
+    """The reference's own zmath.cpp (MTM, inv) built by ``make -C oracle ref``
+    from the reference sources, or None where it was not built."""
+    global _ref
+    if _ref is None and os.path.exists(_REF_PATH):
+        L = ctypes.CDLL(_REF_PATH)
+        L.ref_mtm.argtypes = [ctypes.c_long, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
+        L.ref_inv.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
+        L.ref_inv.restype = ctypes.c_int
+        _ref = L
+    return _ref
 
 
 def _ptr(a: np.ndarray) -> ctypes.c_void_p:
@@ -201,3 +224,53 @@ def prism_adjoint(grid, kappa, xyz, r, threads=DEFAULT_THREADS) -> np.ndarray:
 def prism_kernel(xs, ys, zs, x1, x2, y1, y2, z1, z2) -> float:
     """Integral of (zs - z)/r^3 over [x1,x2]x[y1,y2]x[z1,z2]."""
     return lib().oracle_prism_kernel(xs, ys, zs, x1, x2, y1, y2, z1, z2)
+
+
+# ------------------------------------------------ dense LA of the GN step
+def mtm(J, threads=DEFAULT_THREADS) -> np.ndarray:
+    """J^T J (zmath.cpp:260-262) of J [rows, cols]."""
+    J = np.ascontiguousarray(J, dtype=np.float64)
+    out = np.empty((J.shape[1], J.shape[1]))
+    lib().oracle_mtm(J.shape[0], J.shape[1], _ptr(J), _ptr(out), threads)
+    return out
+
+
+def jtr(J, r) -> np.ndarray:
+    """J^T r (TFM.cpp:160)."""
+    J = np.ascontiguousarray(J, dtype=np.float64)
+    r = np.ascontiguousarray(r, dtype=np.float64)
+    out = np.empty(J.shape[1])
+    lib().oracle_jtr(J.shape[0], J.shape[1], _ptr(J), _ptr(r), _ptr(out))
+    return out
+
+
+def inv(a):
+    """(ok, inverse) by zmath.cpp:129-140's Gauss-Jordan; ok False and a zero
+    matrix on an exact zero pivot."""
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    out = np.empty_like(a)
+    ok = lib().oracle_inv(a.shape[0], _ptr(a), _ptr(out))
+    return bool(ok), out
+
+
+def gn_steps(JJ, jtr_, sig, dm, b, threads=DEFAULT_THREADS) -> np.ndarray:
+    """B[k] = b + inv(JJ + dm[k,0] diag(diag(JJ)) + dm[k,1] diag(1/sig^2)) jtr
+    (TFM.cpp:159-161)."""
+    JJ = np.ascontiguousarray(JJ, dtype=np.float64)
+    n = JJ.shape[0]
+    jtr_ = np.ascontiguousarray(jtr_, dtype=np.float64)
+    sig = np.ascontiguousarray(sig, dtype=np.float64)
+    dm = np.ascontiguousarray(dm, dtype=np.float64).reshape(-1, 2)
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    out = np.empty((dm.shape[0], n))
+    lib().oracle_gn_steps(n, dm.shape[0], _ptr(JJ), _ptr(jtr_), _ptr(sig), _ptr(dm), _ptr(b),
+                          _ptr(out), threads)
+    return out
+
+
+def inv_diag(JJ, d) -> np.ndarray:
+    """diag(inv(JJ + I d)) (TFM.cpp:150,173); zeros when singular."""
+    JJ = np.ascontiguousarray(JJ, dtype=np.float64)
+    out = np.empty(JJ.shape[0])
+    lib().oracle_inv_diag(JJ.shape[0], _ptr(JJ), float(d), _ptr(out))
+    return out

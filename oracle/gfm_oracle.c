@@ -43,6 +43,13 @@
  *   oracle_prism_*       right rectangular prism, exact volume integral of
  *                        (z_m - z)/r^3 (Plouff 1976; Nagy, Papp & Benedek
  *                        2000, eq. for g_z), no reference counterpart.
+ *   oracle_mtm           zmath.cpp:260-262, MTM(J) = J^T J.
+ *   oracle_jtr           TFM.cpp:160, T(J)*r.
+ *   oracle_inv           zmath.cpp:129-140, Gauss-Jordan inverse with row
+ *                        pivoting (zero matrix on an exact zero pivot).
+ *   oracle_gn_steps      TFM.cpp:159-161, the damped Gauss-Newton steps of
+ *                        LMA::fit over a grid of damping conditions.
+ *   oracle_inv_diag      TFM.cpp:150,173, diag(inv(J^T J + I d)).
  *
  * Deliberate deviations: (1) oracle_interp2 reads a neighbour outside the
  * grid (i = di-1 or j = dj-1, where its weight is exactly 0) as 0; the
@@ -59,6 +66,12 @@
  * finite differences for the adjoint; the prism against Gauss-Legendre
  * quadrature of the point-mass kernel, its far-field point-mass limit,
  * 8-subprism additivity and the infinite-slab limit 2*pi*G*rho*t.
+ * The dense LA functions (tests/test_oracle_gn.py): against numpy's LAPACK
+ * inverse, solve and products on random and reference-shaped systems, the
+ * identities inv(A) A = I and (A^T A) symmetric, a matrix that needs row
+ * swaps, exact-zero-pivot singularity, and -- where the reference's own
+ * zmath.cpp builds (oracle/_ref, zmath_ref.cpp) -- its MTM and inv bit for
+ * bit.
  */
 #include <math.h>
 #include <pthread.h>
@@ -480,4 +493,154 @@ void oracle_prism_adjoint(const og_grid* g, double kappa, int64_t m, const doubl
                           const double* r, double* out, int nthreads) {
     pm_ctx c = {g, kappa, 0.0, NULL, xyz, r, out, m};
     parallel_for((int64_t)g->nx * g->ny * g->nz, nthreads, prism_adj_range, &c);
+}
+
+/* ------------------------------------------- dense LA of the GN step */
+/* oracle_mtm: zmath.cpp:260-262, MTM(a) = a^T a for a [rows][cols] row-major
+ * (the reference's mat J[m][n]): out[i][j] = out[j][i] = sum_n a[n][i] a[n][j]
+ * for j >= i, the sum in row order n = 0, 1, ... (the reference transposes a
+ * first, then sums a^T[i][n] a^T[j][n] over n; one thread per column i there,
+ * a parallel_for over i here). */
+typedef struct { int64_t rows; int cols; const double* a; double* out; } mtm_ctx;
+
+static void mtm_range(void* p, int64_t lo, int64_t hi) {
+    const mtm_ctx* c = (const mtm_ctx*)p;
+    for (int64_t i = lo; i < hi; ++i)
+        for (int j = (int)i; j < c->cols; ++j) {
+            double s = 0.0;
+            for (int64_t n = 0; n < c->rows; ++n)
+                s += c->a[n * c->cols + i] * c->a[n * c->cols + j];
+            c->out[i * c->cols + j] = c->out[(int64_t)j * c->cols + i] = s;
+        }
+}
+
+void oracle_mtm(int64_t rows, int cols, const double* a, double* out, int nthreads) {
+    mtm_ctx c = {rows, cols, a, out};
+    parallel_for(cols, nthreads, mtm_range, &c);
+}
+
+/* oracle_jtr: J^T r of TFM.cpp:160 (T(J)*r), out[i] = sum_m J[m][i] r[m]. */
+void oracle_jtr(int64_t rows, int cols, const double* J, const double* r, double* out) {
+    for (int i = 0; i < cols; ++i) {
+        double s = 0.0;
+        for (int64_t m = 0; m < rows; ++m) s += J[m * cols + i] * r[m];
+        out[i] = s;
+    }
+}
+
+/* oracle_inv: zmath.cpp:129-140, Gauss-Jordan inversion of a [n][n]: for each
+ * column i the row with the largest |a[.][i]| at or below i (the first one on
+ * ties) is swapped into place in a and in m (which starts as I); an exact zero
+ * pivot returns the zero matrix (the reference's `return arr(l,l)`, here with
+ * return value 0, else 1); row i of a and m is divided by the pivot, then
+ * c = a[r][i] times row i is subtracted from every row r below; finally, for
+ * i = 1..n-1, c = a[r][i] times row i is subtracted from every row r above.
+ * The same operations in the same order. */
+int oracle_inv(int n, const double* a_in, double* m) {
+    const size_t nn = (size_t)n * n;
+    double* a = (double*)malloc(nn * sizeof(double));
+    double* tmp = (double*)malloc((size_t)n * sizeof(double));
+    memcpy(a, a_in, nn * sizeof(double));
+    memset(m, 0, nn * sizeof(double));
+    for (int i = 0; i < n; ++i) m[(size_t)i * n + i] = 1.0;
+    int ok = 1;
+    for (int i = 0; i < n && ok; ++i) {
+        int im = i;
+        for (int r = i; r < n; ++r)
+            if (fabs(a[(size_t)r * n + i]) > fabs(a[(size_t)im * n + i])) im = r;
+        if (im != i) {
+            memcpy(tmp, a + (size_t)i * n, n * sizeof(double));
+            memcpy(a + (size_t)i * n, a + (size_t)im * n, n * sizeof(double));
+            memcpy(a + (size_t)im * n, tmp, n * sizeof(double));
+            memcpy(tmp, m + (size_t)i * n, n * sizeof(double));
+            memcpy(m + (size_t)i * n, m + (size_t)im * n, n * sizeof(double));
+            memcpy(m + (size_t)im * n, tmp, n * sizeof(double));
+        }
+        if (a[(size_t)i * n + i] == 0.0) {
+            ok = 0;
+            break;
+        }
+        const double f = a[(size_t)i * n + i];
+        for (int j = 0; j < n; ++j) {
+            a[(size_t)i * n + j] /= f;
+            m[(size_t)i * n + j] /= f;
+        }
+        for (int r = i + 1; r < n; ++r) {
+            const double c = a[(size_t)r * n + i];
+            for (int j = 0; j < n; ++j) {
+                a[(size_t)r * n + j] -= c * a[(size_t)i * n + j];
+                m[(size_t)r * n + j] -= c * m[(size_t)i * n + j];
+            }
+        }
+    }
+    if (ok)
+        for (int i = 1; i < n; ++i)
+            for (int r = 0; r < i; ++r) {
+                const double c = a[(size_t)r * n + i];
+                for (int j = 0; j < n; ++j) {
+                    m[(size_t)r * n + j] -= c * m[(size_t)i * n + j];
+                    a[(size_t)r * n + j] -= c * a[(size_t)i * n + j];
+                }
+            }
+    else
+        memset(m, 0, nn * sizeof(double));
+    free(a);
+    free(tmp);
+    return ok;
+}
+
+/* oracle_gn_steps: TFM.cpp:159-161 (LMA::fit): with M = diag(diag(JJ)) and
+ * A = diag(1/sig^2), for each damping condition k, B[k] = b + inv(JJ +
+ * dm[k][0] M + dm[k][1] A) (J^T r); jtr is J^T r. The reference evaluates
+ * inv(...)*T(J)*r left to right, (inv * J^T) * r; here inv * (J^T r), the
+ * same product reassociated (FP64 rounding apart), which saves the n x rows
+ * intermediate. A singular system (exact zero pivot) gives B[k] = b, as the
+ * reference's zero inverse does. One damping condition per thread, as the
+ * reference runs them. */
+typedef struct {
+    int n; const double* JJ; const double* jtr; const double* sig; const double* dm;
+    const double* b; double* B;
+} gn_ctx;
+
+static void gn_range(void* p, int64_t lo, int64_t hi) {
+    const gn_ctx* c = (const gn_ctx*)p;
+    const int n = c->n;
+    const size_t nn = (size_t)n * n;
+    double* K = (double*)malloc(nn * sizeof(double));
+    double* Ki = (double*)malloc(nn * sizeof(double));
+    for (int64_t k = lo; k < hi; ++k) {
+        const double d0 = c->dm[2 * k], d1 = c->dm[2 * k + 1];
+        memcpy(K, c->JJ, nn * sizeof(double));
+        for (int i = 0; i < n; ++i)
+            K[(size_t)i * n + i] = c->JJ[(size_t)i * n + i] + d0 * c->JJ[(size_t)i * n + i] +
+                                   d1 * (1.0 / (c->sig[i] * c->sig[i]));
+        oracle_inv(n, K, Ki);
+        for (int i = 0; i < n; ++i) {
+            double s = 0.0;
+            for (int j = 0; j < n; ++j) s += Ki[(size_t)i * n + j] * c->jtr[j];
+            c->B[k * n + i] = c->b[i] + s;
+        }
+    }
+    free(K);
+    free(Ki);
+}
+
+void oracle_gn_steps(int n, int nd, const double* JJ, const double* jtr, const double* sig,
+                     const double* dm, const double* b, double* B, int nthreads) {
+    gn_ctx c = {n, JJ, jtr, sig, dm, b, B};
+    parallel_for(nd, nthreads, gn_range, &c);
+}
+
+/* oracle_inv_diag: TFM.cpp:150 (create) and :173 (annihilate), the diagonal
+ * of inv(JJ + I d): out[i] = inv(...)[i][i]; all 0 when singular. */
+void oracle_inv_diag(int n, const double* JJ, double d, double* out) {
+    const size_t nn = (size_t)n * n;
+    double* K = (double*)malloc(nn * sizeof(double));
+    double* Ki = (double*)malloc(nn * sizeof(double));
+    memcpy(K, JJ, nn * sizeof(double));
+    for (int i = 0; i < n; ++i) K[(size_t)i * n + i] += d;
+    oracle_inv(n, K, Ki);
+    for (int i = 0; i < n; ++i) out[i] = Ki[(size_t)i * n + i];
+    free(K);
+    free(Ki);
 }
