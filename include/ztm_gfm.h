@@ -220,6 +220,32 @@ int ztm_plan_forward_batched(ztm_plan plan, int n_models, const float* rho_dev,
  * ztm_plan_adjoint (FP32) per vector: the transpose of that forward. */
 int ztm_plan_adjoint_batched(ztm_plan plan, int n_vectors, const float* r_dev,
                              float* out_dev, void* stream);
+/* Dense LA of the Gauss-Newton step, FP64 on the GPU. The reference runs it
+ * on the CPU in zmath: MTM (zmath.cpp:260-262) and the Gauss-Jordan inv
+ * (zmath.cpp:129-140), called by LMA::fit (TFM.cpp:159-161), create
+ * (TFM.cpp:150) and annihilate (TFM.cpp:173). Row-major device arrays; work_dev
+ * is a caller-provided device workspace of ztm_gn_workspace_bytes(rows, n, nd)
+ * bytes (no allocation inside the calls). */
+int64_t ztm_gn_workspace_bytes(int64_t rows, int n, int nd);
+/* JJ = J^T J [n][n] (MTM(J), TFM.cpp:159) and, when r_dev is given,
+ * jtr = J^T r [n] (T(J)*r, TFM.cpp:160), for J [rows][n]. With rows = 0 both
+ * outputs are zero (jtr_dev, when given, is cleared even if r_dev is null). */
+int ztm_gn_normal(int64_t rows, int n, const double* J_dev, const double* r_dev,
+                  double* JJ_dev, double* jtr_dev, void* work_dev, void* stream);
+/* The damped Gauss-Newton steps of LMA::fit (TFM.cpp:160-161) for nd damping
+ * conditions dm [nd][2]: B[k] = b + inv(JJ + dm[k][0] diag(diag(JJ)) +
+ * dm[k][1] diag(1/sig^2)) jtr, B [nd][n]. Solved by Cholesky, the systems
+ * being symmetric positive definite when a damping term is positive; a system
+ * whose factorisation meets a pivot that is not positive gives B[k] = b, as
+ * the reference's zero inverse of a singular matrix does. n <= 7680. */
+int ztm_gn_damped_steps(int n, int nd, const double* JJ_dev, const double* jtr_dev,
+                        const double* sig_dev, const double* dm_dev, const double* b_dev,
+                        double* B_dev, void* work_dev, void* stream);
+/* out = diag(inv(JJ + I d)) (TFM.cpp:150, :173); all 0 when JJ + I d is not
+ * positive definite (the reference's loop then raises d). n <= 7680. */
+int ztm_gn_inverse_diag(int n, const double* JJ_dev, double d, double* out_dev,
+                        void* work_dev, void* stream);
+
 /* FP64 ensemble (no reference counterpart; the FP64 form of the batched
  * ensemble, for callers that need FP64 and as its same-algorithm FP64
  * comparison): G generated once per tile in FP64, reused across the whole

@@ -837,6 +837,42 @@ int ztm_plan_adjoint_batched64(ztm_plan plan, int n_vectors, const double* r_dev
                                 (hipStream_t)stream);
 }
 
+int64_t ztm_gn_workspace_bytes(int64_t rows, int n, int nd) {
+    if (rows < 0 || n < 1 || nd < 1) return -1;
+    return (int64_t)gn_workspace_bytes(rows, n, nd);
+}
+
+int ztm_gn_normal(int64_t rows, int n, const double* J_dev, const double* r_dev,
+                  double* JJ_dev, double* jtr_dev, void* work_dev, void* stream) {
+    if (rows < 0 || n < 1) return fail(ZTM_EINVAL, "need rows >= 0 and n >= 1");
+    if (rows >= ((int64_t)1 << 40)) return fail(ZTM_EINVAL, "too many rows");
+    if (!JJ_dev || (r_dev && !jtr_dev) || !work_dev || (rows > 0 && !J_dev))
+        return fail(ZTM_EINVAL, "null J, JJ, jtr (with r) or workspace");
+    hipStream_t s = (hipStream_t)stream;
+    if (rows == 0) {   // no data: J^T J = 0, J^T r = 0 (r may be null: nothing to read)
+        if (int rc = zero_fill(JJ_dev, sizeof(double) * (size_t)n * n, s)) return rc;
+        return jtr_dev ? zero_fill(jtr_dev, sizeof(double) * (size_t)n, s) : ZTM_OK;
+    }
+    return gn_normal_launch(rows, n, J_dev, r_dev, JJ_dev, jtr_dev, work_dev, s);
+}
+
+int ztm_gn_damped_steps(int n, int nd, const double* JJ_dev, const double* jtr_dev,
+                        const double* sig_dev, const double* dm_dev, const double* b_dev,
+                        double* B_dev, void* work_dev, void* stream) {
+    if (n < 1 || nd < 1) return fail(ZTM_EINVAL, "need n >= 1 and nd >= 1");
+    if (!JJ_dev || !jtr_dev || !sig_dev || !dm_dev || !b_dev || !B_dev || !work_dev)
+        return fail(ZTM_EINVAL, "null argument");
+    return gn_steps_launch(n, nd, JJ_dev, jtr_dev, sig_dev, dm_dev, b_dev, B_dev, work_dev,
+                           (hipStream_t)stream);
+}
+
+int ztm_gn_inverse_diag(int n, const double* JJ_dev, double d, double* out_dev, void* work_dev,
+                        void* stream) {
+    if (n < 1) return fail(ZTM_EINVAL, "need n >= 1");
+    if (!JJ_dev || !out_dev || !work_dev) return fail(ZTM_EINVAL, "null argument");
+    return gn_inverse_diag_launch(n, JJ_dev, d, out_dev, work_dev, (hipStream_t)stream);
+}
+
 int ztm_plan_get_batched_tuning(ztm_plan plan, int* slices, int* split, int* adjoint_split,
                                 int* variant) {
     if (!plan) return fail(ZTM_EINVAL, "null plan");

@@ -346,6 +346,16 @@ int batched_adj_launch(const PmShape& s, int B, int ks, int split, const float* 
                        const float* R, double sy, double sz, double scale, double gmax, void* ws,
                        float* X, hipStream_t stream, int variant = 0, int waves = 4);
 
+// Dense LA of the Gauss-Newton step (gfm_gn.hip).
+size_t gn_workspace_bytes(int64_t rows, int n, int nd);
+int gn_normal_launch(int64_t rows, int n, const double* J, const double* r, double* JJ,
+                     double* jtr, void* work, hipStream_t stream);
+int gn_steps_launch(int n, int nd, const double* JJ, const double* jtr, const double* sig,
+                    const double* dm, const double* b, double* B, void* work,
+                    hipStream_t stream);
+int gn_inverse_diag_launch(int n, const double* JJ, double d, double* out, void* work,
+                           hipStream_t stream);
+
 // FP64 ensemble on FP64 MFMA (gfm_batched64.hip): workspace bytes and the
 // K-slices it allows; the launchers (B in [1, 256], n_cells < 2^31).
 size_t batched64_fwd_workspace(const PmShape& s, int B, int* ks_out = nullptr);

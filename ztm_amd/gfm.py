@@ -73,6 +73,10 @@ SIGNATURES = {
     "ztm_plan_forward_batched": (_I, [_P, _I, _P, _P, _P]),
     "ztm_plan_adjoint_batched": (_I, [_P, _I, _P, _P, _P]),
     "ztm_plan_forward_batched64": (_I, [_P, _I, _P, _P, _P]),
+    "ztm_gn_workspace_bytes": (_I64, [_I64, _I, _I]),
+    "ztm_gn_normal": (_I, [_I64, _I, _P, _P, _P, _P, _P, _P]),
+    "ztm_gn_damped_steps": (_I, [_I, _I, _P, _P, _P, _P, _P, _P, _P, _P]),
+    "ztm_gn_inverse_diag": (_I, [_I, _P, _D, _P, _P, _P]),
     "ztm_plan_adjoint_batched64": (_I, [_P, _I, _P, _P, _P]),
     "ztm_plan_set_batched_tuning": (_I, [_P, _I, _I]),
     "ztm_plan_get_batched_tuning": (_I, [_P, _IP, _IP, _IP, _IP]),
@@ -744,6 +748,72 @@ class Batch:
 
     def autotune(self, reps: int = 3, stream=None) -> None:
         check(lib().ztm_batch_autotune(self._h, reps, _stream(stream)))
+
+
+# ------------------------------------------------ dense LA of the GN step
+def _f64_cuda(t, shape, name):
+    import torch
+    if not (isinstance(t, torch.Tensor) and t.is_cuda and t.dtype == torch.float64
+            and t.is_contiguous() and tuple(t.shape) == tuple(shape)):
+        raise ValueError(f"{name} must be a contiguous float64 CUDA tensor of shape {shape}")
+
+
+def _gn_work(rows, n, nd, device):
+    import torch
+    b = lib().ztm_gn_workspace_bytes(rows, n, nd)
+    if b < 0:
+        raise ValueError(f"invalid sizes rows={rows} n={n} nd={nd}")
+    return torch.empty(max(int(b), 1), dtype=torch.uint8, device=device)
+
+
+def gn_normal(J, r=None, stream=None):
+    """(J^T J, J^T r) of J float64 [rows, n] and r float64 [rows] on the GPU
+    (MTM, zmath.cpp:260-262; T(J)*r, TFM.cpp:160); J^T r is None without r."""
+    import torch
+    if not (isinstance(J, torch.Tensor) and J.dim() == 2):
+        raise ValueError("J must be a 2-D CUDA tensor")
+    rows, n = J.shape
+    _f64_cuda(J, (rows, n), "J")
+    if r is not None:
+        _f64_cuda(r, (rows,), "r")
+    JJ = torch.empty(n, n, dtype=torch.float64, device=J.device)
+    jtr = torch.empty(n, dtype=torch.float64, device=J.device) if r is not None else None
+    work = _gn_work(rows, n, 1, J.device)
+    check(lib().ztm_gn_normal(rows, n, _ptr(J), _ptr(r) if r is not None else None, _ptr(JJ),
+                              _ptr(jtr) if jtr is not None else None, _ptr(work),
+                              _stream(stream)))
+    return JJ, jtr
+
+
+def gn_damped_steps(JJ, jtr, sig, dm, b, stream=None):
+    """B[k] = b + inv(JJ + dm[k,0] diag(diag(JJ)) + dm[k,1] diag(1/sig^2)) jtr for
+    every damping condition dm [nd, 2] (LMA::fit, TFM.cpp:160-161); float64 CUDA
+    tensors, B [nd, n]. A system that is not positive definite gives B[k] = b."""
+    import torch
+    n = JJ.shape[0]
+    nd = dm.shape[0]
+    _f64_cuda(JJ, (n, n), "JJ")
+    _f64_cuda(jtr, (n,), "jtr")
+    _f64_cuda(sig, (n,), "sig")
+    _f64_cuda(dm, (nd, 2), "dm")
+    _f64_cuda(b, (n,), "b")
+    B = torch.empty(nd, n, dtype=torch.float64, device=JJ.device)
+    work = _gn_work(0, n, nd, JJ.device)
+    check(lib().ztm_gn_damped_steps(n, nd, _ptr(JJ), _ptr(jtr), _ptr(sig), _ptr(dm), _ptr(b),
+                                    _ptr(B), _ptr(work), _stream(stream)))
+    return B
+
+
+def gn_inverse_diag(JJ, d, stream=None):
+    """diag(inv(JJ + I d)) (TFM.cpp:150,173); zeros when not positive definite."""
+    import torch
+    n = JJ.shape[0]
+    _f64_cuda(JJ, (n, n), "JJ")
+    out = torch.empty(n, dtype=torch.float64, device=JJ.device)
+    work = _gn_work(0, n, 1, JJ.device)
+    check(lib().ztm_gn_inverse_diag(n, _ptr(JJ), float(d), _ptr(out), _ptr(work),
+                                    _stream(stream)))
+    return out
 
 
 def _prec_of(t, n: int, name: str, zoned: bool = False) -> int:
