@@ -34,6 +34,7 @@ METRIC = "station×cell pairs/s fwd+adjoint, FP32/BF16 vs FP64 on GPU at ≤1e-5
 FLOP_PER_PAIR = 13            # 3 sub, r^2 (1 mul + 2 fma), 2 mul r^-3, 1 mul rho*dz, 1 fma
 PEAK_FP32_TFLOPS = 157.3      # MI355X FP32 vector, published
 PEAK_HBM_GBS = 8000.0
+PEAK_FP64_MFMA_TFLOPS = 78.6  # MI355X dense FP64 matrix, published
 
 
 def cpu_model() -> str:
@@ -92,7 +93,7 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the stored-matrix and prism legs")
-    ap.add_argument("--leg", choices=["prism", "batched", "uneven", "tfm", "accumulation"],
+    ap.add_argument("--leg", choices=["prism", "batched", "uneven", "tfm", "accumulation", "gn"],
                     default=None,
                     help="run only this extra leg and print its JSON object (for profiling)")
     ap.add_argument("--copy-calib-mib", type=int, default=0,
@@ -120,9 +121,9 @@ def main() -> int:
         torch.cuda.synchronize()
         print(json.dumps({"copy_calibration_bytes": 4 * n, "copies": 5}))
         return 0
-    if args.leg in ("batched", "uneven", "tfm", "accumulation"):
+    if args.leg in ("batched", "uneven", "tfm", "accumulation", "gn"):
         leg = {"batched": batched_leg, "uneven": uneven_leg, "tfm": tfm_leg,
-               "accumulation": accumulation_leg}[args.leg]
+               "accumulation": accumulation_leg, "gn": gn_leg}[args.leg]
         print(json.dumps({args.leg: leg(torch.cuda.current_stream(), args)}))
         return 0
     w = W.pm_blobs_1m()
@@ -236,6 +237,7 @@ def main() -> int:
         line["batched"] = batched_leg(stream, args)
         line["uneven_batch"] = uneven_leg(stream, args)
         line["tfm_fm"] = tfm_leg(stream, args)
+        line["gn_step"] = gn_leg(stream, args)
     line["summary"] = summary(line)
     print(json.dumps(line))
     return 0
@@ -784,6 +786,101 @@ def accumulation_leg(stream, args):
                 row[name] = max(relerr(Y[b], r) for b, r in ref.items())
             out["rows"].append(row)
     return out
+
+
+def gn_leg(stream, args, reps=10, cpu_rows=2048):
+    """Dense LA of one damped Gauss-Newton step (LMA::fit, TFM.cpp:157-161) on
+    gn_step (J 32,768 x 1,024, the reference's 12 damping conditions), FP64 on
+    the device: J^T J with J^T r (one symmetric product on FP64 MFMA), the 12
+    damped steps (batched blocked Cholesky and substitutions), and
+    diag(inv(J^T J + I d)) (create / annihilate, TFM.cpp:150,173). Compared
+    with the oracle's FP64 restatement of zmath's MTM and Gauss-Jordan inv on
+    the host: MTM timed on the first cpu_rows rows and scaled to all rows
+    (its cost is linear in rows), the damped steps timed whole on the GPU's
+    J^T J, whose result also checks the device steps."""
+    _progress("gn leg")
+    import torch
+    import oracle
+    from ztm_amd import gfm, workloads as W
+    dev = torch.device("cuda:0")
+    w = W.gn_step()
+    rows, n = w.J.shape
+    nd = w.dm.shape[0]
+    t64 = lambda a: torch.from_numpy(np.ascontiguousarray(a, np.float64)).to(dev)
+    J, r, sig, b, dm = t64(w.J), t64(w.r), t64(w.sig), t64(w.b), t64(w.dm)
+    JJ = torch.empty(n, n, dtype=torch.float64, device=dev)
+    jtr = torch.empty(n, dtype=torch.float64, device=dev)
+    B = torch.empty(nd, n, dtype=torch.float64, device=dev)
+    V = torch.empty(n, dtype=torch.float64, device=dev)
+    work = torch.empty(gfm.gn_workspace_bytes(rows, n, nd), dtype=torch.uint8, device=dev)
+    sync = torch.cuda.synchronize
+
+    def normal():
+        gfm.gn_normal(J, r, stream=stream, out=(JJ, jtr), work=work)
+
+    def steps():
+        gfm.gn_damped_steps(JJ, jtr, sig, dm, b, stream=stream, out=B, work=work)
+
+    def invdiag():
+        gfm.gn_inverse_diag(JJ, 1e-3, stream=stream, out=V, work=work)
+
+    normal()
+    steps()
+    invdiag()
+    sync()
+    gfm.profile_reset()
+    gfm.profile_enable(True)
+    t_normal = _timed(normal, reps, sync)
+    t_steps = _timed(steps, reps, sync)
+    gfm.profile_enable(False)
+    t_inv = _timed(invdiag, reps, sync)
+    prof = {k: gfm.profile_read(k) for k in ("gn_syrk", "gn_cholesky", "gn_solve")}
+    gfm.profile_reset()
+    kms = {k: ms / max(cnt, 1) for k, (ms, cnt) in prof.items()}
+    n1 = n + 1
+    syrk_flop = float(rows) * n1 * (n1 + 1)          # the n1 (n1 + 1) / 2 entries of [J r]^T [J r]
+    chol_flop = nd * n ** 3 / 3.0
+    JJh, jtrh, Bh, Vh = JJ.cpu().numpy(), jtr.cpu().numpy(), B.cpu().numpy(), V.cpu().numpy()
+    err_normal = max(relerr(JJh, w.J.T @ w.J), relerr(jtrh, w.J.T @ w.r))
+    # CPU: the oracle's MTM on a row sample, scaled; the damped steps on the
+    # device's J^T J (Gauss-Jordan per condition, conditions on threads)
+    Js = np.ascontiguousarray(w.J[:cpu_rows])
+    c0 = time.perf_counter()
+    oracle.mtm(Js)
+    c1 = time.perf_counter()
+    Bc = oracle.gn_steps(JJh, jtrh, w.sig, w.dm, w.b)
+    c2 = time.perf_counter()
+    cpu_mtm = (c1 - c0) * rows / cpu_rows
+    cpu_ms = (cpu_mtm + (c2 - c1)) * 1e3
+    steps_err = max(float(np.max(np.abs(Bh[k] - Bc[k])) / np.max(np.abs(Bc[k] - w.b)))
+                    for k in range(nd))
+    inv_err = relerr(Vh, oracle.inv_diag(JJh, 1e-3))
+    ms_gpu = (t_normal + t_steps) * 1e3
+    return {
+        "workload": f"gn_step: J {rows:,} data x {n:,} parameters, {nd} damping conditions "
+                    "(TFM.cpp:157-161)",
+        "unit": "ms per GN step (J^T J, J^T r and the damped steps of every condition)",
+        "ms_per_step": ms_gpu,
+        "ms_normal": t_normal * 1e3, "ms_damped_steps": t_steps * 1e3,
+        "ms_inverse_diag": t_inv * 1e3,
+        "kernels_ms": kms,
+        "roofline": {"bound": "mfma", "unit": "TFLOP/s", "peak": PEAK_FP64_MFMA_TFLOPS,
+                     "kernel": "gn_syrk",
+                     "achieved": syrk_flop / (kms["gn_syrk"] * 1e-3) / 1e12,
+                     "frac": syrk_flop / (kms["gn_syrk"] * 1e-3) / 1e12 / PEAK_FP64_MFMA_TFLOPS,
+                     "flop": "rows x (n+1) x (n+2): the upper triangle of [J r]^T [J r]"},
+        "cholesky_tflops": chol_flop / (kms["gn_cholesky"] * 1e-3) / 1e12,
+        "rel_err": {"normal_vs_host_blas": err_normal, "steps_vs_oracle": steps_err,
+                    "inverse_diag_vs_oracle": inv_err},
+        "cpu_baseline": {"ms_per_step": cpu_ms, "ms_mtm": cpu_mtm * 1e3,
+                         "ms_damped_steps": (c2 - c1) * 1e3,
+                         "cores": oracle.DEFAULT_THREADS, "kind": "port",
+                         "sample": f"oracle MTM (zmath.cpp:260-262 loop order) on {cpu_rows} of "
+                                   f"{rows} rows scaled to all rows, + Gauss-Jordan damped steps "
+                                   f"(zmath.cpp:129-140) for {nd} conditions on "
+                                   f"{oracle.DEFAULT_THREADS} threads ({cpu_model()})"},
+        "speedup_vs_cpu": cpu_ms / ms_gpu,
+    }
 
 
 def prism_leg(w, r, stream, args):

@@ -766,9 +766,32 @@ def _gn_work(rows, n, nd, device):
     return torch.empty(max(int(b), 1), dtype=torch.uint8, device=device)
 
 
-def gn_normal(J, r=None, stream=None):
+def _gn_work_arg(work, rows, n, nd, device):
+    """The caller's workspace (a CUDA uint8 tensor of at least
+    gn_workspace_bytes(rows, n, nd) bytes) or a new one."""
+    import torch
+    if work is None:
+        return _gn_work(rows, n, nd, device)
+    need = lib().ztm_gn_workspace_bytes(rows, n, nd)
+    if not (isinstance(work, torch.Tensor) and work.is_cuda and work.dtype == torch.uint8
+            and work.is_contiguous() and work.numel() >= need):
+        raise ValueError(f"work must be a contiguous uint8 CUDA tensor of >= {need} bytes")
+    return work
+
+
+def gn_workspace_bytes(rows, n, nd=1):
+    """Bytes of device workspace the gn_* calls need for J [rows, n] and nd
+    damping conditions."""
+    b = lib().ztm_gn_workspace_bytes(rows, n, nd)
+    if b < 0:
+        raise ValueError(f"invalid sizes rows={rows} n={n} nd={nd}")
+    return int(b)
+
+
+def gn_normal(J, r=None, stream=None, out=None, work=None):
     """(J^T J, J^T r) of J float64 [rows, n] and r float64 [rows] on the GPU
-    (MTM, zmath.cpp:260-262; T(J)*r, TFM.cpp:160); J^T r is None without r."""
+    (MTM, zmath.cpp:260-262; T(J)*r, TFM.cpp:160); J^T r is None without r.
+    out=(JJ, jtr) and work reuse caller buffers (no allocation)."""
     import torch
     if not (isinstance(J, torch.Tensor) and J.dim() == 2):
         raise ValueError("J must be a 2-D CUDA tensor")
@@ -776,16 +799,22 @@ def gn_normal(J, r=None, stream=None):
     _f64_cuda(J, (rows, n), "J")
     if r is not None:
         _f64_cuda(r, (rows,), "r")
-    JJ = torch.empty(n, n, dtype=torch.float64, device=J.device)
-    jtr = torch.empty(n, dtype=torch.float64, device=J.device) if r is not None else None
-    work = _gn_work(rows, n, 1, J.device)
+    if out is None:
+        JJ = torch.empty(n, n, dtype=torch.float64, device=J.device)
+        jtr = torch.empty(n, dtype=torch.float64, device=J.device) if r is not None else None
+    else:
+        JJ, jtr = out
+        _f64_cuda(JJ, (n, n), "JJ")
+        if r is not None:
+            _f64_cuda(jtr, (n,), "jtr")
+    work = _gn_work_arg(work, rows, n, 1, J.device)
     check(lib().ztm_gn_normal(rows, n, _ptr(J), _ptr(r) if r is not None else None, _ptr(JJ),
                               _ptr(jtr) if jtr is not None else None, _ptr(work),
                               _stream(stream)))
     return JJ, jtr
 
 
-def gn_damped_steps(JJ, jtr, sig, dm, b, stream=None):
+def gn_damped_steps(JJ, jtr, sig, dm, b, stream=None, out=None, work=None):
     """B[k] = b + inv(JJ + dm[k,0] diag(diag(JJ)) + dm[k,1] diag(1/sig^2)) jtr for
     every damping condition dm [nd, 2] (LMA::fit, TFM.cpp:160-161); float64 CUDA
     tensors, B [nd, n]. A system that is not positive definite gives B[k] = b."""
@@ -797,20 +826,27 @@ def gn_damped_steps(JJ, jtr, sig, dm, b, stream=None):
     _f64_cuda(sig, (n,), "sig")
     _f64_cuda(dm, (nd, 2), "dm")
     _f64_cuda(b, (n,), "b")
-    B = torch.empty(nd, n, dtype=torch.float64, device=JJ.device)
-    work = _gn_work(0, n, nd, JJ.device)
+    if out is None:
+        B = torch.empty(nd, n, dtype=torch.float64, device=JJ.device)
+    else:
+        B = out
+        _f64_cuda(B, (nd, n), "out")
+    work = _gn_work_arg(work, 0, n, nd, JJ.device)
     check(lib().ztm_gn_damped_steps(n, nd, _ptr(JJ), _ptr(jtr), _ptr(sig), _ptr(dm), _ptr(b),
                                     _ptr(B), _ptr(work), _stream(stream)))
     return B
 
 
-def gn_inverse_diag(JJ, d, stream=None):
+def gn_inverse_diag(JJ, d, stream=None, out=None, work=None):
     """diag(inv(JJ + I d)) (TFM.cpp:150,173); zeros when not positive definite."""
     import torch
     n = JJ.shape[0]
     _f64_cuda(JJ, (n, n), "JJ")
-    out = torch.empty(n, dtype=torch.float64, device=JJ.device)
-    work = _gn_work(0, n, 1, JJ.device)
+    if out is None:
+        out = torch.empty(n, dtype=torch.float64, device=JJ.device)
+    else:
+        _f64_cuda(out, (n,), "out")
+    work = _gn_work_arg(work, 0, n, 1, JJ.device)
     check(lib().ztm_gn_inverse_diag(n, _ptr(JJ), float(d), _ptr(out), _ptr(work),
                                     _stream(stream)))
     return out

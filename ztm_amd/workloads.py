@@ -3,7 +3,8 @@
 Every random draw comes from a counter-based SplitMix64 stream, so any value
 can be regenerated from (seed, stream, index) alone and the host and any test
 see identical inputs. Seeds are 20190301 + workload id (pm_cpu_small 1,
-pm_blobs_1m 2, adjoint_small 3, ensemble_256 4, uneven_48 5, tfm_fm 6).
+pm_blobs_1m 2, adjoint_small 3, ensemble_256 4, uneven_48 5, tfm_fm 6,
+gn_step 7).
 
 Coordinates are metres with z up; cells of edge h fill x in [0, nx h],
 y in [0, ny h], z in [-nz h, 0] (top at z = 0); density contrast in kg/m^3;
@@ -26,6 +27,10 @@ the gravity output is in mGal (kappa = 1e5 * G).
                    log-U[1e3, 4e4] scattered over the footprint at z=+10 m,
                    cell edge log-U[250, 2000] m; 3 buried +250 kg/m^3 blocks
                    each; data noise N(0, 0.1 mGal)
+    gn_step        one damped Gauss-Newton step of LMA::fit (TFM.cpp:157-161):
+                   Jacobian 32,768 data x 1,024 parameters (column scales
+                   e^U[-3,3], 30 % zeros), residual N(0,1), a-priori sigma
+                   U[0.05, 4], the reference's 12 damping conditions
 """
 from __future__ import annotations
 
@@ -370,3 +375,28 @@ WORKLOADS = {
     "uneven_48": uneven_48,
     "tfm_fm": tfm_fm,
 }
+
+
+# the reference's damping grid dmp = {{0,.3,1,3,10,100},{1e4,1e5}} as the
+# (d0, d1) pairs LMA::fit iterates over (TFM.cpp:202,209)
+GN_DAMPING = np.array([[a, b] for a in (0.0, 0.3, 1.0, 3.0, 10.0, 100.0) for b in (1e4, 1e5)])
+
+
+@dataclass
+class GnStep:
+    J: np.ndarray          # [rows, n] float64, row-major (data x parameters)
+    r: np.ndarray          # [rows] residuals
+    sig: np.ndarray        # [n] a-priori parameter sigma
+    b: np.ndarray          # [n] current parameters
+    dm: np.ndarray         # [nd, 2] damping conditions (d0, d1)
+
+
+def gn_step(seed: int = SEED0 + 7, rows: int = 32768, n: int = 1024) -> GnStep:
+    """Inputs of one damped Gauss-Newton step (LMA::fit, TFM.cpp:157-161): a
+    Jacobian whose columns span e^[-3,3] in scale, as event-location,
+    velocity and Moho parameters do, with 30 % of its entries zero."""
+    scale = np.exp(uniform(seed, 1, n, -3.0, 3.0))
+    J = normal(seed, 2, rows * n).reshape(rows, n) * scale
+    J[uniform(seed, 3, rows * n).reshape(rows, n) < 0.3] = 0.0
+    return GnStep(J, normal(seed, 4, rows), uniform(seed, 5, n, 0.05, 4.0),
+                  normal(seed, 6, n), GN_DAMPING.copy())
