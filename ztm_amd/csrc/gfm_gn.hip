@@ -1,20 +1,22 @@
 // Dense LA of the Gauss-Newton step (LMA::fit, create, annihilate;
 // TFM.cpp:150,159-161,173), FP64 throughout:
-//   normal equations  C = [J | r]^T [J | r]: J^T J (MTM, zmath.cpp:260-262)
-//                     and J^T r in one symmetric product on FP64 MFMA;
+//   normal equations  J^T J (MTM, zmath.cpp:260-262) on FP64 MFMA and J^T r
+//                     in the same pass over J;
 //   damped steps      B[k] = b + (J^T J + d0 diag(J^T J) + d1 diag(1/sig^2))^-1
 //                     J^T r for the damping grid, all conditions at once;
 //   inverse diagonal  diag((J^T J + I d)^-1).
 // The reference inverts with Gauss-Jordan (zmath.cpp:129-140) and multiplies;
 // here the systems, symmetric positive definite whenever a damping term is
 // positive, are factored by a blocked Cholesky, batched over the damping
-// conditions, and solved by substitution. A system whose factorisation meets
-// a pivot that is not positive is reported singular and, as the reference's
-// zero inverse does, gives no step (and a zero inverse diagonal).
+// conditions, and solved by substitution with the inverses of the 32 x 32
+// diagonal factor blocks. A system whose factorisation meets a pivot that is
+// not positive is reported singular and, as the reference's zero inverse
+// does, gives no step (and a zero inverse diagonal).
 //
 // Layouts (device, row-major): J [rows][n], r [rows], J^T J [n][n] (full,
-// symmetric), J^T r [n]; factors [nd][n][n] (lower triangle used) in the
-// caller's workspace, flags [nd] after them.
+// symmetric), J^T r [n]. Workspace of the factorisations: factors K [nd][n][n]
+// (lower triangle used), the inverses of their diagonal blocks
+// Dinv [nd][n / 32][32][32] (lower triangle, zeros above), flags [nd].
 #include "gfm_internal.h"
 
 #include <algorithm>
@@ -31,43 +33,45 @@ __device__ __forceinline__ f64x4 mfma64(double a, double b, f64x4 c) {
 }
 
 // ------------------------------------------------------- normal equations
-// C = A^T A for A = [J | r] (rows x n1, n1 = n + 1 with r). A workgroup (4
-// waves) forms a 128 x 128 tile (bi, bj), bi <= bj, over one K-slice of rows;
-// wave w the 64 x 64 quadrant (w & 1, w >> 1): 4 x 4 MFMA tiles of 16 x 16.
-// The slice walks 16 rows at a time: both 16 x 128 row strips go to LDS
-// (stride 129: the lanes' 16 consecutive columns sit on distinct banks), the
-// next strip's loads issued unconditionally into registers right after the
-// barrier. MFMA f64 16x16x4: A[i][k] at lane i + 16k, B[k][j] at lane
-// j + 16k, D[i][j] in register i / 4 of lane j + 16 (i % 4).
-constexpr int kSyrkTile = 128;
+// C = J^T J by 64 x 64 tiles (bi, bj), bi <= bj, each over one K-slice of
+// rows: a workgroup (4 waves) forms one tile, wave w the 32 x 32 quadrant
+// (w & 1, w >> 1) as 2 x 2 MFMA tiles of 16 x 16. The slice walks 16 rows at a
+// time through two LDS buffers (one barrier per strip): while the MFMAs read
+// strip s, the loads of strip s + 1 are in flight into registers. MFMA f64
+// 16x16x4: A[i][k] at lane i + 16k, B[k][j] at lane j + 16k, D[i][j] in
+// register i / 4 of lane j + 16 (i % 4). The diagonal tiles also form their
+// 64 entries of J^T r on the VALU from the same strips.
+//
+// Workgroups go to the 8 XCDs round-robin. With the slice count a multiple
+// of 8, workgroup w runs slice (w % 8) + 8 (w / 8 / tiles), so every tile of
+// a slice runs on one XCD and the slice's rows are fetched into that XCD's L2
+// once for all its tiles.
+constexpr int kSyrkTile = 64;
 constexpr int kSyrkLd = kSyrkTile + 1;
 constexpr int kSyrkRows = 16;
-constexpr int kSyrkLoads = kSyrkRows * kSyrkTile / 256;   // per thread per strip
+constexpr int kSyrkLoads = kSyrkRows * kSyrkTile / 256;   // per thread per operand
+constexpr int kXcds = 8;
 
 struct SyrkStrip {
     double v[kSyrkLoads];
-    unsigned ok;
-    // strip rows k0 .. k0 + 15 (before k_end) of columns c0 .. c0 + 127 of A
-    __device__ __forceinline__ void load(const double* __restrict__ J,
-                                         const double* __restrict__ r, int n, int n1,
-                                         int64_t k0, int64_t k_end, int c0) {
-        ok = 0u;
+    // strip rows k0 .. k0 + 15 (before k_end) of columns c0 .. c0 + 63 of J
+    __device__ __forceinline__ void load(const double* __restrict__ J, int n, int64_t k0,
+                                         int64_t k_end, int c0) {
 #pragma unroll
         for (int u = 0; u < kSyrkLoads; ++u) {
             const int e = threadIdx.x + 256 * u;
             const int64_t k = k0 + e / kSyrkTile;
             const int c = c0 + e % kSyrkTile;
-            const bool in = k < k_end && c < n1;
-            ok |= in ? 1u << u : 0u;
-            const double* p = !in ? J : c < n ? J + k * n + c : r + k;
-            v[u] = *p;
+            const bool in = k < k_end && c < n;
+            const double x = *(in ? J + k * n + c : J);
+            v[u] = in ? x : 0.0;
         }
     }
     __device__ __forceinline__ void store(double* lds) const {
 #pragma unroll
         for (int u = 0; u < kSyrkLoads; ++u) {
             const int e = threadIdx.x + 256 * u;
-            lds[(e / kSyrkTile) * kSyrkLd + e % kSyrkTile] = (ok >> u) & 1u ? v[u] : 0.0;
+            lds[(e / kSyrkTile) * kSyrkLd + e % kSyrkTile] = v[u];
         }
     }
 };
@@ -84,87 +88,130 @@ __device__ __forceinline__ void upper_tile(int t, int nb, int& bi, int& bj) {
 
 __global__ __launch_bounds__(256) void gn_syrk_kernel(const double* __restrict__ J,
                                                       const double* __restrict__ r, int n,
-                                                      int n1, int64_t rows,
-                                                      int64_t rows_per_slice, int nb,
-                                                      double* __restrict__ part) {
-    __shared__ double li[kSyrkRows * kSyrkLd];
-    __shared__ double lj[kSyrkRows * kSyrkLd];
+                                                      int64_t rows, int64_t rows_per_slice,
+                                                      int nb, int tiles, int xcd_map,
+                                                      double* __restrict__ part,
+                                                      double* __restrict__ jpart) {
+    __shared__ double li[2][kSyrkRows * kSyrkLd];
+    __shared__ double lj[2][kSyrkRows * kSyrkLd];
+    __shared__ double lr[2][kSyrkRows];
+    int tile, slice;
+    if (xcd_map) {
+        const int x = blockIdx.x % kXcds, idx = blockIdx.x / kXcds;
+        slice = x + kXcds * (idx / tiles);
+        tile = idx % tiles;
+    } else {
+        tile = blockIdx.x % tiles;
+        slice = blockIdx.x / tiles;
+    }
     int bi, bj;
-    upper_tile(blockIdx.x, nb, bi, bj);
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    upper_tile(tile, nb, bi, bj);
+    const bool diag = bi == bj;
+    const int ci = bi * kSyrkTile, cj = bj * kSyrkTile;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int l16 = lane & 15, lk = lane >> 4;
-    const int wi = (wave & 1) * 64, wj = (wave >> 1) * 64;
-    const int64_t k_begin = (int64_t)blockIdx.y * rows_per_slice;
-    const int64_t k_end = min(rows, k_begin + rows_per_slice);
-    f64x4 acc[4][4];
+    const int wi = (wave & 1) * 32, wj = (wave >> 1) * 32;
+    const int64_t k_begin = (int64_t)slice * rows_per_slice;
+    const int64_t k_end = min(rows, k_begin + rows_per_slice);   // empty when k_begin >= rows
+    const bool with_r = diag && r != nullptr;
+    f64x4 acc[2][2];
 #pragma unroll
-    for (int a = 0; a < 4; ++a)
+    for (int a = 0; a < 2; ++a)
 #pragma unroll
-        for (int b = 0; b < 4; ++b) acc[a][b] = f64x4{0.0, 0.0, 0.0, 0.0};
+        for (int b = 0; b < 2; ++b) acc[a][b] = f64x4{0.0, 0.0, 0.0, 0.0};
+    double jr = 0.0;
     SyrkStrip si, sj;
-    si.load(J, r, n, n1, k_begin, k_end, bi * kSyrkTile);
-    sj.load(J, r, n, n1, k_begin, k_end, bj * kSyrkTile);
+    double rv = 0.0;
+    si.load(J, n, k_begin, k_end, ci);
+    if (!diag) sj.load(J, n, k_begin, k_end, cj);
+    if (with_r && tid < kSyrkRows && k_begin + tid < k_end) rv = r[k_begin + tid];
+    si.store(li[0]);
+    if (!diag) sj.store(lj[0]);
+    if (with_r && tid < kSyrkRows) lr[0][tid] = rv;
+    __syncthreads();
+    int buf = 0;
     for (int64_t k0 = k_begin; k0 < k_end; k0 += kSyrkRows) {
-        __syncthreads();   // every wave is done with the previous strip
-        si.store(li);
-        sj.store(lj);
-        __syncthreads();
-        if (k0 + kSyrkRows < k_end) {
-            si.load(J, r, n, n1, k0 + kSyrkRows, k_end, bi * kSyrkTile);
-            sj.load(J, r, n, n1, k0 + kSyrkRows, k_end, bj * kSyrkTile);
+        const bool more = k0 + kSyrkRows < k_end;
+        if (more) {
+            si.load(J, n, k0 + kSyrkRows, k_end, ci);
+            if (!diag) sj.load(J, n, k0 + kSyrkRows, k_end, cj);
+            rv = 0.0;
+            if (with_r && tid < kSyrkRows && k0 + kSyrkRows + tid < k_end)
+                rv = r[k0 + kSyrkRows + tid];
         }
+        const double* A = li[buf];
+        const double* B = diag ? li[buf] : lj[buf];
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
+        for (int q = 0; q < kSyrkRows / 4; ++q) {
             const int kq = 4 * q + lk;
-            double a[4], b[4];
+            double a[2], b[2];
 #pragma unroll
-            for (int t = 0; t < 4; ++t) {
-                a[t] = li[kq * kSyrkLd + wi + 16 * t + l16];
-                b[t] = lj[kq * kSyrkLd + wj + 16 * t + l16];
+            for (int t = 0; t < 2; ++t) {
+                a[t] = A[kq * kSyrkLd + wi + 16 * t + l16];
+                b[t] = B[kq * kSyrkLd + wj + 16 * t + l16];
             }
 #pragma unroll
-            for (int ta = 0; ta < 4; ++ta)
+            for (int ta = 0; ta < 2; ++ta)
 #pragma unroll
-                for (int tb = 0; tb < 4; ++tb) acc[ta][tb] = mfma64(a[ta], b[tb], acc[ta][tb]);
+                for (int tb = 0; tb < 2; ++tb) acc[ta][tb] = mfma64(a[ta], b[tb], acc[ta][tb]);
         }
+        if (with_r && tid < kSyrkTile) {
+#pragma unroll
+            for (int k = 0; k < kSyrkRows; ++k) jr += A[k * kSyrkLd + tid] * lr[buf][k];
+        }
+        if (more) {
+            si.store(li[buf ^ 1]);
+            if (!diag) sj.store(lj[buf ^ 1]);
+            if (with_r && tid < kSyrkRows) lr[buf ^ 1][tid] = rv;
+        }
+        __syncthreads();
+        buf ^= 1;
     }
-    // partial tile of this slice, full-matrix layout [n1][n1], upper tiles only
-    double* out = part + (size_t)blockIdx.y * n1 * n1;
+    // this slice's partial tile, full-matrix layout [n][n], upper tiles only
+    double* out = part + (size_t)slice * n * n;
 #pragma unroll
-    for (int ta = 0; ta < 4; ++ta)
+    for (int ta = 0; ta < 2; ++ta)
 #pragma unroll
-        for (int tb = 0; tb < 4; ++tb)
+        for (int tb = 0; tb < 2; ++tb)
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
-                const int i = bi * kSyrkTile + wi + 16 * ta + 4 * e + lk;
-                const int j = bj * kSyrkTile + wj + 16 * tb + l16;
-                if (i < n1 && j < n1) out[(size_t)i * n1 + j] = acc[ta][tb][e];
+                const int i = ci + wi + 16 * ta + 4 * e + lk;
+                const int j = cj + wj + 16 * tb + l16;
+                if (i < n && j < n) out[(size_t)i * n + j] = acc[ta][tb][e];
             }
+    if (with_r && tid < kSyrkTile && ci + tid < n) jpart[(size_t)slice * n + ci + tid] = jr;
 }
 
 // JJ[i][j] = JJ[j][i] = sum over slices of C[min(i,j)][max(i,j)] (an upper
-// tile's entry, so the result is exactly symmetric), Jtr[i] = C[i][n].
+// tile's entry, so the result is exactly symmetric); jtr[i] = sum over slices
+// of the diagonal tiles' J^T r entries. Slices summed in order.
 __global__ __launch_bounds__(256) void gn_syrk_reduce_kernel(const double* __restrict__ part,
-                                                             int ks, int n, int n1,
+                                                             const double* __restrict__ jpart,
+                                                             int ks, int n,
                                                              double* __restrict__ JJ,
                                                              double* __restrict__ jtr) {
-    const int64_t total = (int64_t)n * n1;
+    const int64_t nn = (int64_t)n * n;
+    const int64_t total = nn + (jtr ? n : 0);
     for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < total;
          e += (int64_t)gridDim.x * 256) {
-        const int i = (int)(e / n1), j = (int)(e % n1);
-        const size_t off = i <= j ? (size_t)i * n1 + j : (size_t)j * n1 + i;
         double s = 0.0;
-        for (int k = 0; k < ks; ++k) s += part[(size_t)k * n1 * n1 + off];
-        if (j < n)
-            JJ[(size_t)i * n + j] = s;
-        else if (jtr)
+        if (e < nn) {
+            const int i = (int)(e / n), j = (int)(e % n);
+            const size_t off = i <= j ? (size_t)i * n + j : (size_t)j * n + i;
+            for (int k = 0; k < ks; ++k) s += part[(size_t)k * nn + off];
+            JJ[e] = s;
+        } else {
+            const int i = (int)(e - nn);
+            for (int k = 0; k < ks; ++k) s += jpart[(size_t)k * n + i];
             jtr[i] = s;
+        }
     }
 }
 
 // ------------------------------------------------------ blocked Cholesky
 constexpr int kNb = 32;            // block size of the factorisation
 constexpr int kNbLd = kNb + 1;
+constexpr int kPanelRows = 64;     // panel rows per workgroup of the panel step
 
 // K_b = JJ with its diagonal raised: mode 0 (damped steps) adds
 // d0 JJ_ii + d1 / sig_i^2 in the reference's order (TFM.cpp:160,
@@ -196,77 +243,110 @@ __global__ __launch_bounds__(256) void gn_form_kernel(const double* __restrict__
     }
 }
 
-// (a) the diagonal block at p of every matrix, in LDS. grid = nd.
-__global__ __launch_bounds__(256) void gn_potrf_diag_kernel(double* __restrict__ K, int n, int p,
-                                                            int* __restrict__ flags) {
+// Panel step at column p of every matrix (grid = (panel workgroups, nd)).
+// Each workgroup factors the diagonal block A = K[p:p+w][p:p+w] in LDS by
+// symmetric elimination on [A | I] without pivoting: step j subtracts
+// a_ij / d_j times row j from every row i > j (the lower triangle of A and the
+// right half), one barrier per step. Then A = L' D L'^T with the pivots d_j
+// on the diagonal, the multipliers' numerators a_ij = L'_ij d_j below it and
+// L'^-1 in the right half, so L = L' D^1/2 and L^-1 = D^-1/2 L'^-1. A pivot
+// that is not positive (or NaN) flags the matrix singular and is taken as 1.
+// Workgroup x then solves its 64 rows of the panel below the block,
+// X = A_panel L^-T, from LDS; workgroup 0 also writes L^-1 and the flag. The
+// diagonal block of K keeps A: the other workgroups read it, and the
+// substitutions take the diagonal blocks from Dinv, so L is never stored.
+__global__ __launch_bounds__(256) void gn_panel_kernel(double* __restrict__ K, int n, int p,
+                                                       double* __restrict__ Dinv,
+                                                       int* __restrict__ flags) {
     __shared__ double a[kNb * kNbLd];
-    double* Kb = K + (size_t)blockIdx.x * n * n;
+    __shared__ double e[kNb * kNbLd];
+    __shared__ double x[kPanelRows * kNbLd];
+    __shared__ double rsq[kNb];
+    const int tid = threadIdx.x;
+    const int b = blockIdx.y;
+    double* Kb = K + (size_t)b * n * n;
     const int w = min(kNb, n - p);
-    for (int t = threadIdx.x; t < w * w; t += 256) {
-        const int i = t / w, j = t % w;
-        a[i * kNbLd + j] = Kb[(size_t)(p + i) * n + p + j];
+    const int r0 = p + w + blockIdx.x * kPanelRows;
+    const int prow = max(0, min(kPanelRows, n - r0));
+    // this workgroup's panel rows, loaded first so they arrive during the
+    // factorisation
+    constexpr int kPv = kPanelRows * kNb / 256;
+    double pv[kPv];
+#pragma unroll
+    for (int u = 0; u < kPv; ++u) {
+        const int t = tid + 256 * u;
+        const int i = t / kNb, j = t % kNb;
+        const bool in = i < prow && j < w;
+        const double v = *(in ? Kb + (size_t)(r0 + i) * n + p + j : Kb);
+        pv[u] = in ? v : 0.0;
+    }
+    for (int t = tid; t < kNb * kNb; t += 256) {
+        const int i = t / kNb, j = t % kNb;
+        const bool in = i < w && j <= i;
+        const double v = *(in ? Kb + (size_t)(p + i) * n + p + j : Kb);
+        a[i * kNbLd + j] = in ? v : 0.0;
+        e[i * kNbLd + j] = i == j ? 1.0 : 0.0;
     }
     __syncthreads();
+    bool singular = false;
     for (int j = 0; j < w; ++j) {
-        if (threadIdx.x == 0) {
-            double d = a[j * kNbLd + j];
-            if (!(d > 0.0)) {   // not positive definite (or NaN): singular
-                flags[blockIdx.x] = 1;
-                d = 1.0;
+        double d = a[j * kNbLd + j];
+        if (!(d > 0.0)) {
+            singular = true;
+            d = 1.0;
+        }
+        const double rd = 1.0 / d;
+        // thread tid owns the entries (i, k) = (t / 32, t % 32), t = tid + 256 u,
+        // of both halves: the lower trailing entries j < k <= i < w and the
+        // right half's j < i < w, k <= j
+#pragma unroll
+        for (int u = 0; u < kNb * kNb / 256; ++u) {
+            const int t = tid + 256 * u;
+            const int i = t / kNb, k = t % kNb;
+            if (i < w && j < i) {
+                const double f = a[i * kNbLd + j] * rd;
+                if (j < k && k <= i) a[i * kNbLd + k] -= f * a[k * kNbLd + j];
+                if (k <= j) e[i * kNbLd + k] -= f * e[j * kNbLd + k];
             }
-            a[j * kNbLd + j] = sqrt(d);
-        }
-        __syncthreads();
-        const double djj = a[j * kNbLd + j];
-        for (int i = j + 1 + threadIdx.x; i < w; i += 256) a[i * kNbLd + j] /= djj;
-        __syncthreads();
-        const int m = w - j - 1;   // trailing (i, k), j < k <= i < w
-        for (int t = threadIdx.x; t < m * m; t += 256) {
-            const int i = j + 1 + t / m, k = j + 1 + t % m;
-            if (k <= i) a[i * kNbLd + k] -= a[i * kNbLd + j] * a[k * kNbLd + j];
         }
         __syncthreads();
     }
-    for (int t = threadIdx.x; t < w * w; t += 256) {
-        const int i = t / w, j = t % w;
-        if (j <= i) Kb[(size_t)(p + i) * n + p + j] = a[i * kNbLd + j];
+    if (tid < kNb) {
+        double d = tid < w ? a[tid * kNbLd + tid] : 1.0;
+        if (!(d > 0.0)) d = 1.0;
+        rsq[tid] = 1.0 / sqrt(d);
+    }
+    __syncthreads();
+    // L^-1 = D^-1/2 L'^-1 in place (each entry read and written by one thread)
+    for (int t = tid; t < kNb * kNb; t += 256) {
+        const int i = t / kNb, c = t % kNb;
+        e[i * kNbLd + c] = i < w && c <= i ? e[i * kNbLd + c] * rsq[i] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < kPv; ++u) {
+        const int t = tid + 256 * u;
+        x[(t / kNb) * kNbLd + t % kNb] = pv[u];
+    }
+    __syncthreads();
+    if (blockIdx.x == 0) {
+        double* Db = Dinv + ((size_t)b * ((n + kNb - 1) / kNb) + p / kNb) * kNb * kNb;
+        for (int t = tid; t < kNb * kNb; t += 256) Db[t] = e[(t / kNb) * kNbLd + t % kNb];
+        if (tid == 0 && singular) flags[b] = 1;
+    }
+    // X[i][j] = sum_{k <= j} A[i][k] L^-1[j][k]
+#pragma unroll
+    for (int u = 0; u < kPv; ++u) {
+        const int t = tid + 256 * u;
+        const int i = t / kNb, j = t % kNb;
+        if (i < prow && j < w) {
+            double s = 0.0;
+            for (int k = 0; k <= j; ++k) s += x[i * kNbLd + k] * e[j * kNbLd + k];
+            Kb[(size_t)(r0 + i) * n + p + j] = s;
+        }
     }
 }
 
-// (b) the panel below the diagonal block: rows r >= p + w, columns p .. p + w,
-// X = A L^-T, one row per thread. grid = (ceil((n - p - w) / 64), nd); block 64.
-__global__ __launch_bounds__(64) void gn_trsm_kernel(double* __restrict__ K, int n, int p) {
-    __shared__ double l[kNb * kNbLd];
-    __shared__ double x[64 * kNbLd];
-    double* Kb = K + (size_t)blockIdx.y * n * n;
-    const int w = min(kNb, n - p);
-    const int r0 = p + w + blockIdx.x * 64;
-    const int rows = min(64, n - r0);
-    for (int t = threadIdx.x; t < w * w; t += 64) {
-        const int i = t / w, j = t % w;
-        l[i * kNbLd + j] = j <= i ? Kb[(size_t)(p + i) * n + p + j] : 0.0;
-    }
-    for (int t = threadIdx.x; t < rows * w; t += 64) {
-        const int i = t / w, j = t % w;
-        x[i * kNbLd + j] = Kb[(size_t)(r0 + i) * n + p + j];
-    }
-    __syncthreads();
-    if (threadIdx.x < rows) {
-        double* xr = x + threadIdx.x * kNbLd;
-        for (int j = 0; j < w; ++j) {
-            double s = xr[j];
-            for (int k = 0; k < j; ++k) s -= xr[k] * l[j * kNbLd + k];
-            xr[j] = s / l[j * kNbLd + j];
-        }
-    }
-    __syncthreads();
-    for (int t = threadIdx.x; t < rows * w; t += 64) {
-        const int i = t / w, j = t % w;
-        Kb[(size_t)(r0 + i) * n + p + j] = x[i * kNbLd + j];
-    }
-}
-
-// (c) the trailing lower triangle: K[i][j] -= sum_k L[i][p + k] L[j][p + k],
+// The trailing lower triangle: K[i][j] -= sum_k L[i][p + k] L[j][p + k],
 // i >= j >= p + kNb, 64 x 64 tiles (bi >= bj) on FP64 MFMA (each wave a
 // 32 x 32 quadrant, 2 x 2 tiles of 16, K = kNb in steps of 4).
 // grid = (tiles, nd). Only called with a full block (w = kNb).
@@ -325,158 +405,180 @@ __global__ __launch_bounds__(256) void gn_update_kernel(double* __restrict__ K, 
 }
 
 // ------------------------------------------------------------ substitution
+// One workgroup of 1,024 threads per system: the substitutions are a chain
+// of n / 32 dependent block steps, so each step issues all its global loads
+// at once (32 per thread at most, per 1,024 columns) and pays one memory
+// round trip and two barriers.
+constexpr int kSolveThreads = 1024;
+
+__device__ __forceinline__ double half_wave_sum(double v) {   // over 32 lanes
+#pragma unroll
+    for (int off = 16; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    return v;
+}
+
 // Forward substitution L y = rhs over rows q_begin .. n - 1 (earlier y are 0),
-// y kept in LDS; blocks of kNb rows: 8 threads per row form the dot product
-// with the solved part, then wave 0 solves the block's triangle.
-__device__ void forward_subst(const double* __restrict__ L, int n, int q_begin, double* y,
-                              double* s_blk) {
-    const int tid = threadIdx.x;
+// y in LDS, rhs in y on entry, by blocks of kNb rows. For block q, the 32
+// threads of row q + i (i = tid / 32) form its product with the solved part
+// over consecutive columns; then y_q = Dinv_q (rhs_q minus those products),
+// the 32 threads of row i over the block's columns. Dinv_q is loaded with
+// the row products.
+__device__ void forward_subst(const double* __restrict__ L, const double* __restrict__ Dv, int n,
+                              int q_begin, double* y, double* s_blk) {
+    const int tid = threadIdx.x, i = tid >> 5, c = tid & 31;
     for (int q = q_begin; q < n; q += kNb) {
         const int w = min(kNb, n - q);
-        {
-            const int rho = tid >> 3, part = tid & 7;
-            double s = 0.0;
-            if (rho < w) {
-                const double* row = L + (size_t)(q + rho) * n;
-                for (int k = q_begin + part; k < q; k += 8) s += row[k] * y[k];
+        const double dv0 = Dv[(size_t)(q / kNb) * kNb * kNb + (i < w ? i : 0) * kNb + c];
+        const double dv = i < w ? dv0 : 0.0;
+        double s = 0.0;
+        if (i < w) {
+            const double* row = L + (size_t)(q + i) * n;
+            for (int k0 = q_begin; k0 < q; k0 += 32 * kNb) {
+                double t[kNb];
+#pragma unroll
+                for (int m = 0; m < kNb; ++m) {
+                    const int k = k0 + c + 32 * m;
+                    const int kk = k < q ? k : q_begin;   // in range for every lane
+                    const double p = row[kk] * y[kk];
+                    t[m] = k < q ? p : 0.0;
+                }
+#pragma unroll
+                for (int m = 0; m < kNb; ++m) s += t[m];
             }
-            s += __shfl_xor(s, 1);
-            s += __shfl_xor(s, 2);
-            s += __shfl_xor(s, 4);
-            if (part == 0 && rho < w) s_blk[rho] = y[q + rho] - s;
         }
+        s = half_wave_sum(s);
+        if (c == 0 && i < w) s_blk[i] = y[q + i] - s;
         __syncthreads();
-        if (tid < 64) {
-            double v = tid < w ? s_blk[tid] : 0.0;
-            for (int j = 0; j < w; ++j) {
-                const double yj = __shfl(v, j) / L[(size_t)(q + j) * n + q + j];
-                if (tid == j) v = yj;
-                if (tid > j && tid < w) v -= L[(size_t)(q + tid) * n + q + j] * yj;
-            }
-            if (tid < w) y[q + tid] = v;
-        }
+        double v = i < w ? dv * s_blk[c] : 0.0;   // Dinv is zero beyond the block
+        v = half_wave_sum(v);
+        if (c == 0 && i < w) y[q + i] = v;
         __syncthreads();
     }
 }
 
 // Backward substitution L^T x = y in place (x overwrites y), blocks from the
-// last: the dot product with the solved part reads rows k of L (32 threads
-// per row segment), then wave 0 solves the block's triangle.
-__device__ void backward_subst(const double* __restrict__ L, int n, double* y, double* red) {
-    const int tid = threadIdx.x;
+// last, right-looking: x_q = Dinv_q^T y_q (the 32 threads of row j over the
+// block), then every earlier entry takes its share,
+// y[j] -= sum_i L[q + i][j] x_q[i] for j < q (one column per thread, the
+// block's rows of L read across threads).
+__device__ void backward_subst(const double* __restrict__ L, const double* __restrict__ Dv, int n,
+                               double* y, double* s_blk) {
+    const int tid = threadIdx.x, j = tid >> 5, c = tid & 31;
     const int nblk = (n + kNb - 1) / kNb;
     for (int bq = nblk - 1; bq >= 0; --bq) {
         const int q = bq * kNb;
         const int w = min(kNb, n - q);
-        {
-            const int rho = tid & 31, part = tid >> 5;   // 8 parts over k
-            double s = 0.0;
-            if (rho < w)
-                for (int k = q + w + part; k < n; k += 8) s += L[(size_t)k * n + q + rho] * y[k];
-            red[part * kNb + rho] = s;
-        }
+        const bool in = j < w && c < w;
+        const double p = Dv[(size_t)bq * kNb * kNb + (in ? c * kNb + j : 0)] * y[q + (in ? c : 0)];
+        double v = in ? p : 0.0;
+        v = half_wave_sum(v);
+        if (c == 0 && j < w) s_blk[j] = v;
         __syncthreads();
-        if (tid < 64) {
-            double v = 0.0;
-            if (tid < w) {
-                double s = 0.0;
-                for (int pp = 0; pp < 8; ++pp) s += red[pp * kNb + tid];
-                v = y[q + tid] - s;
+        if (tid < w) y[q + tid] = s_blk[tid];
+        for (int jj = tid; jj < q; jj += kSolveThreads) {
+            double t[kNb];
+#pragma unroll
+            for (int ii = 0; ii < kNb; ++ii) {
+                const double p = L[(size_t)(q + (ii < w ? ii : 0)) * n + jj] * s_blk[ii < w ? ii : 0];
+                t[ii] = ii < w ? p : 0.0;
             }
-            for (int j = w - 1; j >= 0; --j) {
-                const double xj = __shfl(v, j) / L[(size_t)(q + j) * n + q + j];
-                if (tid == j) v = xj;
-                if (tid < j) v -= L[(size_t)(q + j) * n + q + tid] * xj;
-            }
-            if (tid < w) y[q + tid] = v;
+            double acc = 0.0;
+#pragma unroll
+            for (int ii = 0; ii < kNb; ++ii) acc += t[ii];
+            y[jj] -= acc;
         }
         __syncthreads();
     }
 }
 
-// the substitutions keep the solution vector and 8 kNb partial sums in LDS,
-// within the 64 KB a workgroup may take
+// the substitutions keep the solution vector in LDS, within the 64 KB a
+// workgroup may take (with a kNb-entry scratch vector)
 constexpr int kSolveMaxN = 7680;
 
 // B[b] = bvec + x_b with K_b x_b = rhs, or bvec when K_b is singular. grid = nd.
-__global__ __launch_bounds__(256) void gn_solve_kernel(const double* __restrict__ K, int n,
+__global__ __launch_bounds__(kSolveThreads) void gn_solve_kernel(const double* __restrict__ K,
+                                                       const double* __restrict__ Dinv, int n,
                                                        const int* __restrict__ flags,
                                                        const double* __restrict__ rhs,
                                                        const double* __restrict__ bvec,
                                                        double* __restrict__ B) {
     extern __shared__ double sm[];
-    double* y = sm;                  // [n]
-    double* red = sm + n;            // [8 * kNb]
+    double* y = sm;                    // [n]
+    double* s_blk = sm + n;            // [kNb]
+    const int nblk = (n + kNb - 1) / kNb;
     const double* L = K + (size_t)blockIdx.x * n * n;
-    for (int i = threadIdx.x; i < n; i += 256) y[i] = rhs[i];
+    const double* Dv = Dinv + (size_t)blockIdx.x * nblk * kNb * kNb;
+    for (int i = threadIdx.x; i < n; i += kSolveThreads) y[i] = rhs[i];
     __syncthreads();
-    forward_subst(L, n, 0, y, red);
-    backward_subst(L, n, y, red);
+    forward_subst(L, Dv, n, 0, y, s_blk);
+    backward_subst(L, Dv, n, y, s_blk);
     const bool sing = flags[blockIdx.x] != 0;
-    for (int i = threadIdx.x; i < n; i += 256)
+    for (int i = threadIdx.x; i < n; i += kSolveThreads)
         B[(size_t)blockIdx.x * n + i] = bvec[i] + (sing ? 0.0 : y[i]);
 }
 
 // out[i] = (K^-1)_ii = |L^-1 e_i|^2, or 0 when K is singular. grid = n.
-__global__ __launch_bounds__(256) void gn_invdiag_kernel(const double* __restrict__ L, int n,
+__global__ __launch_bounds__(kSolveThreads) void gn_invdiag_kernel(const double* __restrict__ L,
+                                                         const double* __restrict__ Dinv, int n,
                                                          const int* __restrict__ flags,
                                                          double* __restrict__ out) {
     extern __shared__ double sm[];
     double* y = sm;
-    double* red = sm + n;
+    double* s_blk = sm + n;
     const int i = blockIdx.x;
     const int q0 = i / kNb * kNb;
-    for (int k = q0 + threadIdx.x; k < n; k += 256) y[k] = k == i ? 1.0 : 0.0;
+    for (int k = q0 + threadIdx.x; k < n; k += kSolveThreads) y[k] = k == i ? 1.0 : 0.0;
     __syncthreads();
-    forward_subst(L, n, q0, y, red);
+    forward_subst(L, Dinv, n, q0, y, s_blk);
     double s = 0.0;
-    for (int k = i + threadIdx.x; k < n; k += 256) s += y[k] * y[k];
+    for (int k = i + threadIdx.x; k < n; k += kSolveThreads) s += y[k] * y[k];
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
-    __shared__ double wsum[4];
+    __shared__ double wsum[kSolveThreads / 64];
     if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = s;
     __syncthreads();
-    if (threadIdx.x == 0)
-        out[i] = flags[0] ? 0.0 : (wsum[0] + wsum[1]) + (wsum[2] + wsum[3]);
-}
-
-// K-slices of the normal-equation product: whole rounds of 512 resident
-// workgroups to within 5%, at most 64, at least one 16-row strip each.
-int syrk_slices(int tiles, int64_t rows) {
-    const int64_t strips = (rows + kSyrkRows - 1) / kSyrkRows;
-    const int max_ks = (int)std::max<int64_t>(1, std::min<int64_t>(64, strips));
-    int best = 1;
-    double best_fill = 0.0;
-    for (int ks = 1; ks <= max_ks; ++ks) {
-        const double rounds = (double)tiles * ks / 512.0;
-        const double fill = rounds / std::ceil(rounds);
-        if (fill >= 0.95) return ks;
-        if (fill > best_fill + 1e-9) {
-            best_fill = fill;
-            best = ks;
-        }
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int v = 0; v < kSolveThreads / 64; ++v) t += wsum[v];
+        out[i] = flags[0] ? 0.0 : t;
     }
-    return best;
 }
 
 size_t align256(size_t b) { return (b + 255) / 256 * 256; }
 
-int syrk_tiles(int n1) {
-    const int nb = (n1 + kSyrkTile - 1) / kSyrkTile;
+int syrk_tiles(int n) {
+    const int nb = (n + kSyrkTile - 1) / kSyrkTile;
     return nb * (nb + 1) / 2;
 }
 
-// Factor nd matrices K [nd][n][n] in place (lower triangle).
-int cholesky_batched(double* K, int n, int nd, int* flags, hipStream_t stream) {
+// K-slices of the normal-equation product: a multiple of the 8 XCDs, enough
+// for about 1,024 workgroups (4 per CU) and at most 64; fewer (no XCD
+// mapping) when J has fewer 16-row strips than that.
+int syrk_slices(int tiles, int64_t rows) {
+    const int64_t strips = (rows + kSyrkRows - 1) / kSyrkRows;
+    const int per_xcd = std::max(1, std::min(8, (1024 + kXcds * tiles - 1) / (kXcds * tiles)));
+    const int ks = kXcds * per_xcd;
+    return (int)std::max<int64_t>(1, std::min<int64_t>(ks, strips));
+}
+
+size_t factor_bytes(int n, int nd) {
+    const size_t nblk = (size_t)(n + kNb - 1) / kNb;
+    return align256(sizeof(double) * (size_t)nd * n * n) +
+           align256(sizeof(double) * (size_t)nd * nblk * kNb * kNb) +
+           align256(sizeof(int) * (size_t)nd);
+}
+
+// Factor nd matrices K [nd][n][n] in place (lower triangle), the inverses of
+// the diagonal blocks into Dinv.
+int cholesky_batched(double* K, double* Dinv, int n, int nd, int* flags, hipStream_t stream) {
     for (int p = 0; p < n; p += kNb) {
         const int w = std::min(kNb, n - p);
-        hipLaunchKernelGGL(gn_potrf_diag_kernel, dim3(nd), dim3(256), 0, stream, K, n, p, flags);
-        ZTM_HIP(hipGetLastError());
         const int below = n - p - w;
-        if (below <= 0) break;
-        hipLaunchKernelGGL(gn_trsm_kernel, dim3((below + 63) / 64, nd), dim3(64), 0, stream, K,
-                           n, p);
+        const int pw = std::max(1, (below + kPanelRows - 1) / kPanelRows);
+        hipLaunchKernelGGL(gn_panel_kernel, dim3(pw, nd), dim3(256), 0, stream, K, n, p, Dinv,
+                           flags);
         ZTM_HIP(hipGetLastError());
+        if (below <= 0) break;
         const int nt = (below + 63) / 64;
         hipLaunchKernelGGL(gn_update_kernel, dim3(nt * (nt + 1) / 2, nd), dim3(256), 0, stream,
                            K, n, p, nt);
@@ -488,35 +590,32 @@ int cholesky_batched(double* K, int n, int nd, int* flags, hipStream_t stream) {
 }  // namespace
 
 size_t gn_workspace_bytes(int64_t rows, int n, int nd) {
-    const int n1 = n + 1;
-    const size_t syrk =
-        sizeof(double) * (size_t)syrk_slices(syrk_tiles(n1), rows) * n1 * n1;
-    const size_t chol = align256(sizeof(double) * (size_t)std::max(nd, 1) * n * n) +
-                        align256(sizeof(int) * (size_t)std::max(nd, 1));
-    return std::max(syrk, chol);
+    const size_t ks = (size_t)syrk_slices(syrk_tiles(n), rows);
+    const size_t syrk = align256(sizeof(double) * ks * n * n) + sizeof(double) * ks * n;
+    return std::max(syrk, factor_bytes(n, std::max(nd, 1)));
 }
 
 int gn_normal_launch(int64_t rows, int n, const double* J, const double* r, double* JJ,
                      double* jtr, void* work, hipStream_t stream) {
-    const int n1 = r ? n + 1 : n;
-    const int nb = (n1 + kSyrkTile - 1) / kSyrkTile;
+    const int nb = (n + kSyrkTile - 1) / kSyrkTile;
     const int tiles = nb * (nb + 1) / 2;
-    // the slice count the workspace was sized for (with the r column)
-    const int ks = syrk_slices(syrk_tiles(n + 1), rows);
+    const int ks = syrk_slices(tiles, rows);
     const int64_t strips = (rows + kSyrkRows - 1) / kSyrkRows;
     const int64_t rps = (strips + ks - 1) / ks * kSyrkRows;   // whole strips per slice
-    const int ks_used = (int)((rows + rps - 1) / rps);
     double* part = (double*)work;
+    double* jpart = (double*)((char*)work + align256(sizeof(double) * (size_t)ks * n * n));
+    if ((int64_t)tiles * ks >= ((int64_t)1 << 31))
+        return fail(ZTM_EINVAL, "normal equations: %d parameters need too many tiles", n);
     {
         ProfScope prof("gn_syrk", stream);
-        hipLaunchKernelGGL(gn_syrk_kernel, dim3(tiles, ks_used), dim3(256), 0, stream, J, r, n,
-                           n1, rows, rps, nb, part);
+        hipLaunchKernelGGL(gn_syrk_kernel, dim3((unsigned)(tiles * ks)), dim3(256), 0, stream, J,
+                           r, n, rows, rps, nb, tiles, ks % kXcds == 0 ? 1 : 0, part, jpart);
         ZTM_HIP(hipGetLastError());
     }
-    const int64_t total = (int64_t)n * n1;
+    const int64_t total = (int64_t)n * n + (r ? n : 0);
     const unsigned blocks = (unsigned)std::min<int64_t>((total + 255) / 256, 1 << 16);
-    hipLaunchKernelGGL(gn_syrk_reduce_kernel, dim3(blocks), dim3(256), 0, stream, part, ks_used,
-                       n, n1, JJ, r ? jtr : nullptr);
+    hipLaunchKernelGGL(gn_syrk_reduce_kernel, dim3(blocks), dim3(256), 0, stream, part, jpart, ks,
+                       n, JJ, r ? jtr : nullptr);
     ZTM_HIP(hipGetLastError());
     return ZTM_OK;
 }
@@ -527,8 +626,10 @@ int gn_steps_launch(int n, int nd, const double* JJ, const double* jtr, const do
     if (n > kSolveMaxN)
         return fail(ZTM_EINVAL, "the damped steps support up to %d parameters (got %d)",
                     kSolveMaxN, n);
+    const size_t nblk = (size_t)(n + kNb - 1) / kNb;
     double* K = (double*)work;
-    int* flags = (int*)((char*)work + align256(sizeof(double) * (size_t)nd * n * n));
+    double* Dinv = (double*)((char*)work + align256(sizeof(double) * (size_t)nd * n * n));
+    int* flags = (int*)((char*)Dinv + align256(sizeof(double) * (size_t)nd * nblk * kNb * kNb));
     const int64_t total = (int64_t)nd * n * n;
     hipLaunchKernelGGL(gn_form_kernel, dim3((unsigned)std::min<int64_t>((total + 255) / 256,
                                                                          1 << 16)),
@@ -536,11 +637,12 @@ int gn_steps_launch(int n, int nd, const double* JJ, const double* jtr, const do
     ZTM_HIP(hipGetLastError());
     {
         ProfScope prof("gn_cholesky", stream);
-        if (int rc = cholesky_batched(K, n, nd, flags, stream)) return rc;
+        if (int rc = cholesky_batched(K, Dinv, n, nd, flags, stream)) return rc;
     }
-    const size_t lds = sizeof(double) * ((size_t)n + 8 * kNb);
+    const size_t lds = sizeof(double) * ((size_t)n + kNb);
     ProfScope prof("gn_solve", stream);
-    hipLaunchKernelGGL(gn_solve_kernel, dim3(nd), dim3(256), lds, stream, K, n, flags, jtr, b, B);
+    hipLaunchKernelGGL(gn_solve_kernel, dim3(nd), dim3(kSolveThreads), lds, stream, K, Dinv, n, flags, jtr,
+                       b, B);
     ZTM_HIP(hipGetLastError());
     return ZTM_OK;
 }
@@ -550,16 +652,20 @@ int gn_inverse_diag_launch(int n, const double* JJ, double d, double* out, void*
     if (n > kSolveMaxN)
         return fail(ZTM_EINVAL, "the inverse diagonal supports up to %d parameters (got %d)",
                     kSolveMaxN, n);
+    const size_t nblk = (size_t)(n + kNb - 1) / kNb;
     double* K = (double*)work;
-    int* flags = (int*)((char*)work + align256(sizeof(double) * (size_t)n * n));
+    double* Dinv = (double*)((char*)work + align256(sizeof(double) * (size_t)n * n));
+    int* flags = (int*)((char*)Dinv + align256(sizeof(double) * nblk * kNb * kNb));
     const int64_t total = (int64_t)n * n;
     hipLaunchKernelGGL(gn_form_kernel, dim3((unsigned)std::min<int64_t>((total + 255) / 256,
                                                                          1 << 16)),
                        dim3(256), 0, stream, JJ, n, 1, 1, nullptr, nullptr, d, K, flags);
     ZTM_HIP(hipGetLastError());
-    if (int rc = cholesky_batched(K, n, 1, flags, stream)) return rc;
-    const size_t lds = sizeof(double) * ((size_t)n + 8 * kNb);
-    hipLaunchKernelGGL(gn_invdiag_kernel, dim3(n), dim3(256), lds, stream, K, n, flags, out);
+    if (int rc = cholesky_batched(K, Dinv, n, 1, flags, stream)) return rc;
+    const size_t lds = sizeof(double) * ((size_t)n + kNb);
+    ProfScope prof("gn_invdiag", stream);
+    hipLaunchKernelGGL(gn_invdiag_kernel, dim3(n), dim3(kSolveThreads), lds, stream, K, Dinv, n, flags,
+                       out);
     ZTM_HIP(hipGetLastError());
     return ZTM_OK;
 }
