@@ -837,8 +837,7 @@ def gn_leg(stream, args, reps=10, cpu_rows=2048):
     prof = {k: gfm.profile_read(k) for k in ("gn_syrk", "gn_cholesky", "gn_solve")}
     gfm.profile_reset()
     kms = {k: ms / max(cnt, 1) for k, (ms, cnt) in prof.items()}
-    n1 = n + 1
-    syrk_flop = float(rows) * n1 * (n1 + 1)          # the n1 (n1 + 1) / 2 entries of [J r]^T [J r]
+    syrk_flop = float(rows) * n * (n + 3)     # 2 per row for each of the n (n + 1) / 2 entries + J^T r
     chol_flop = nd * n ** 3 / 3.0
     JJh, jtrh, Bh, Vh = JJ.cpu().numpy(), jtr.cpu().numpy(), B.cpu().numpy(), V.cpu().numpy()
     err_normal = max(relerr(JJh, w.J.T @ w.J), relerr(jtrh, w.J.T @ w.r))
@@ -868,7 +867,7 @@ def gn_leg(stream, args, reps=10, cpu_rows=2048):
                      "kernel": "gn_syrk",
                      "achieved": syrk_flop / (kms["gn_syrk"] * 1e-3) / 1e12,
                      "frac": syrk_flop / (kms["gn_syrk"] * 1e-3) / 1e12 / PEAK_FP64_MFMA_TFLOPS,
-                     "flop": "rows x (n+1) x (n+2): the upper triangle of [J r]^T [J r]"},
+                     "flop": "rows x n x (n+3): the upper triangle of J^T J and J^T r"},
         "cholesky_tflops": chol_flop / (kms["gn_cholesky"] * 1e-3) / 1e12,
         "rel_err": {"normal_vs_host_blas": err_normal, "steps_vs_oracle": steps_err,
                     "inverse_diag_vs_oracle": inv_err},
