@@ -131,6 +131,14 @@ constexpr int kBatchChunk = 16;   // cells per MFMA K step
 // their maxima, i.e. wait only for this wave's vector memory operations.
 constexpr int kWaitVm0 = 0x0F70;
 constexpr int kBatchTile = 128;   // stations per workgroup
+
+// Workgroup barrier without __syncthreads()'s release fence: that fence makes
+// the compiler wait for every outstanding vector memory operation
+// (vmcnt(0)) before the barrier, which drains the loads a prefetch ring keeps
+// in flight for later chunks. The caller first waits (s_waitcnt) for the
+// loads the other waves need next; the memory clobber keeps the compiler from
+// moving memory accesses across the barrier.
+__device__ __forceinline__ void ring_barrier() { asm volatile("s_barrier" ::: "memory"); }
 // the adjoint's station chunks end inside the padded station records
 static_assert(kStationPad % 16 == 0, "station padding must hold whole chunks");
 
@@ -561,11 +569,13 @@ void batched_fwd2_kernel(const float4* __restrict__ st, const typename Half<H>::
                                     gs, a);
         if constexpr (MODE != 0) locate(min(c0 + 1, c1 - 1) * kBatchChunk + 8 * kh);
     }
-    if constexpr (PF == 2)
+    if constexpr (PF == 2) {
         __builtin_amdgcn_s_waitcnt(kWaitVm0 | LOADS);   // chunk c0's loads have landed
-    else
+        ring_barrier();
+    } else {
         __builtin_amdgcn_s_waitcnt(kWaitVm0);
-    __syncthreads();
+        __syncthreads();
+    }
     int buf = 0;   // LDS buffer of chunk c: (c - c0) % NBUF
     for (int c = c0; c < c1; ++c) {
         const int cn = min(c + 1, c1 - 1);
@@ -600,11 +610,13 @@ void batched_fwd2_kernel(const float4* __restrict__ st, const typename Half<H>::
                 __builtin_amdgcn_sched_barrier(0);   // keep this order
             });
         });
-        if constexpr (PF == 2)
+        if constexpr (PF == 2) {
             __builtin_amdgcn_s_waitcnt(kWaitVm0 | LOADS);   // all but chunk c + 2's loads
-        else
+            ring_barrier();
+        } else {
             __builtin_amdgcn_s_waitcnt(kWaitVm0);
-        __syncthreads();
+            __syncthreads();
+        }
         buf = buf + 1 == NBUF ? 0 : buf + 1;
     }
     // no load into this workgroup's LDS may still be in flight when it ends
@@ -685,6 +697,10 @@ __device__ __forceinline__ void batched_adj_body(
         cx = (float)i;
         cy = (float)j * sy;
         cz = (float)k * sz;
+        // rounded here: without the pin the compiler may fuse j * sy into a
+        // later subtraction (an FMA) in one variant's code and not in
+        // another's, and the variants' G entries would differ in the last bit
+        asm volatile("" : "+v"(cx), "+v"(cy), "+v"(cz));
     }
 
     f32x16 acc[NT];
@@ -886,13 +902,17 @@ void batched_adj2_kernel(const float4* __restrict__ st, const typename Half<H>::
                                        vec4, amax, sdiv);
 }
 
-// Y[b][m] = scale * sum_ks part[ks][m][b]. A block sums an 8 (stations) x 32
-// (models) tile, one entry per thread (reads coalesced along the models), and
-// writes it transposed through LDS. The adjoint uses it with cells in place
-// of stations and a float output.
+// Y[b][m] = scale * sum_ks part[ks][m][b]. A block sums a 16 (stations) x 64
+// (models) tile: thread (station t / 16, models 4 (t % 16) .. + 3) reads one
+// 16-byte run per slice, 256 contiguous bytes per station, and sums the four
+// entries in FP64, slice by slice in order; the sums leave transposed through
+// LDS, 4 consecutive stations of one model per thread. The adjoint uses it
+// with cells in place of stations and a float output.
 // With FP16 parts (amax given) the sums carry model b's operand scale
 // half_scale(amax[b]) and, in the forward, station m's G scale gsc[m] (gsc
 // given), which the FP64 scale takes out.
+// grid = (ceil(m / 16), ceil(B / 64)); Bp a multiple of 32, part 16-byte
+// aligned.
 template <typename TO>
 __global__ __launch_bounds__(256) void batched_reduce_kernel(const float* __restrict__ part,
                                                              int ks, int m, int m_pad, int B,
@@ -900,28 +920,43 @@ __global__ __launch_bounds__(256) void batched_reduce_kernel(const float* __rest
                                                              const unsigned* __restrict__ amax,
                                                              const float* __restrict__ gsc,
                                                              TO* __restrict__ Y) {
-    __shared__ double tile[8][33];
-    const int mb = blockIdx.x * 8, bb = blockIdx.y * 32;
-    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;   // 32 x 8
+    __shared__ double tile[16][65];
+    const int mb = blockIdx.x * 16, bb = blockIdx.y * 64;
+    const int t = threadIdx.x;
     {
-        const int mm = mb + ty, b = bb + tx;
-        double s = 0.0;
-        if (mm < m && b < B) {
-            const float* p = part + (size_t)mm * Bp + b;
-            const size_t step = (size_t)m_pad * Bp;
+        const int i = t >> 4, j = 4 * (t & 15);
+        const int mm = mb + i, b0 = bb + j;
+        double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+        if (mm < m && b0 < Bp) {   // b0 .. b0 + 3 < Bp: Bp is a multiple of 4
+            const float4* p = reinterpret_cast<const float4*>(part + (size_t)mm * Bp + b0);
+            const size_t step = (size_t)m_pad * Bp / 4;
 #pragma unroll 8
-            for (int q = 0; q < ks; ++q) s += (double)p[(size_t)q * step];
+            for (int q = 0; q < ks; ++q) {
+                const float4 v = p[(size_t)q * step];
+                s0 += (double)v.x;
+                s1 += (double)v.y;
+                s2 += (double)v.z;
+                s3 += (double)v.w;
+            }
         }
-        tile[ty][tx] = s;
+        tile[i][j] = s0;
+        tile[i][j + 1] = s1;
+        tile[i][j + 2] = s2;
+        tile[i][j + 3] = s3;
     }
     __syncthreads();
-    // 32 models x 8 stations out: thread (b = t / 8, station t % 8)
-    const int t = threadIdx.x;
-    const int b = bb + (t >> 3), mm = mb + (t & 7);
-    if (b < B && mm < m) {
-        double f = amax ? scale / (double)half_scale(amax[b]) : scale;
-        if (gsc) f /= (double)gsc[mm];
-        Y[(size_t)b * m + mm] = (TO)(f * tile[t & 7][t >> 3]);
+    // 64 models x 16 stations out: thread (model t / 4, stations 4 (t % 4) + 0..3)
+    const int bl = t >> 2, b = bb + bl;
+    if (b >= B) return;
+    const double f = amax ? scale / (double)half_scale(amax[b]) : scale;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        const int i = 4 * (t & 3) + u, mm = mb + i;
+        if (mm < m) {
+            double g = f;
+            if (gsc) g /= (double)gsc[mm];
+            Y[(size_t)b * m + mm] = (TO)(g * tile[i][bl]);
+        }
     }
 }
 
@@ -1136,7 +1171,7 @@ int batched_fwd_launch(const PmShape& s, int B, int ks, int split, const float* 
 #undef ZTM_BATCHED_FWD2
         ZTM_HIP(hipGetLastError());
     }
-    hipLaunchKernelGGL(batched_reduce_kernel<double>, dim3((s.m + 7) / 8, (B + 31) / 32), dim3(256), 0,
+    hipLaunchKernelGGL(batched_reduce_kernel<double>, dim3((s.m + 15) / 16, (B + 63) / 64), dim3(256), 0,
                        stream, part, ks, s.m, s.m_pad, B, Bp, scale, (const unsigned*)amax,
                        split == 4 ? gsc : nullptr, Y);
     ZTM_HIP(hipGetLastError());
@@ -1283,7 +1318,7 @@ int batched_adj_launch(const PmShape& s, int B, int ks, int split, const float* 
         return ZTM_OK;
     }
     hipLaunchKernelGGL(batched_reduce_kernel<float>,
-                       dim3((unsigned)((s.n_cells + 7) / 8), (B + 31) / 32), dim3(256), 0, stream,
+                       dim3((unsigned)((s.n_cells + 15) / 16), (B + 63) / 64), dim3(256), 0, stream,
                        part, ks, (int)s.n_cells, n_pad, B, Bp, scale / gs,
                        (const unsigned*)amax, nullptr, X);
     ZTM_HIP(hipGetLastError());
