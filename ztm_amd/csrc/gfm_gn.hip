@@ -54,24 +54,28 @@ constexpr int kXcds = 8;
 
 struct SyrkStrip {
     double v[kSyrkLoads];
-    // strip rows k0 .. k0 + 15 (before k_end) of columns c0 .. c0 + 63 of J
+    unsigned ok;
+    // strip rows k0 .. k0 + 15 (before k_end) of columns c0 .. c0 + 63 of J.
+    // Entries outside J load J[0] and are zeroed at the store, so that no
+    // instruction between the load and the store needs the loaded value.
     __device__ __forceinline__ void load(const double* __restrict__ J, int n, int64_t k0,
                                          int64_t k_end, int c0) {
+        ok = 0u;
 #pragma unroll
         for (int u = 0; u < kSyrkLoads; ++u) {
             const int e = threadIdx.x + 256 * u;
             const int64_t k = k0 + e / kSyrkTile;
             const int c = c0 + e % kSyrkTile;
             const bool in = k < k_end && c < n;
-            const double x = *(in ? J + k * n + c : J);
-            v[u] = in ? x : 0.0;
+            ok |= in ? 1u << u : 0u;
+            v[u] = *(in ? J + k * n + c : J);
         }
     }
     __device__ __forceinline__ void store(double* lds) const {
 #pragma unroll
         for (int u = 0; u < kSyrkLoads; ++u) {
             const int e = threadIdx.x + 256 * u;
-            lds[(e / kSyrkTile) * kSyrkLd + e % kSyrkTile] = v[u];
+            lds[(e / kSyrkTile) * kSyrkLd + e % kSyrkTile] = (ok >> u) & 1u ? v[u] : 0.0;
         }
     }
 };
