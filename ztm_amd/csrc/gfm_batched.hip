@@ -26,9 +26,9 @@
 // once, every wave generates its own 32 x 16 block of G in registers directly
 // in the MFMA A-operand layout (lane: station lane%32, cells 8*(lane/32)+0..7),
 // converts it to hi/lo and issues 3 MFMAs per model tile. Workgroups of one
-// K-slice differ only in stations; blockIdx.x runs over stations so that the
-// round-robin dispatch spreads one slice's tiles over the XCDs in sequence
-// and concurrently resident tiles of a slice share its P chunks in L2.
+// K-slice differ only in stations; with a slice count that is a multiple of
+// 8 all the station tiles of a slice run on one XCD (fwd_tile_slice), so
+// that its L2 fetches the slice's P chunks once for all of them.
 //
 // Layouts (HBM):  P parts [N16/16][2][Bp][8] bf16 each (chunk, cell half,
 //                 model, cell), zero padded (N16 = N up to a multiple of 16,
@@ -131,6 +131,27 @@ constexpr int kBatchChunk = 16;   // cells per MFMA K step
 // their maxima, i.e. wait only for this wave's vector memory operations.
 constexpr int kWaitVm0 = 0x0F70;
 constexpr int kBatchTile = 128;   // stations per workgroup
+
+// Station tile and K-slice of a forward workgroup, grid = (tiles, slices).
+// Workgroups are dispatched in grid order (x fastest) and go to the 8 XCDs
+// round-robin. With a slice count that is a multiple of 8, the workgroup at
+// dispatch position w takes slice (w % 8) + 8 (w / 8 / tiles) and tile
+// (w / 8) % tiles: every tile of a slice runs on one XCD, whose L2 then
+// fetches the slice's P parts once for all of them (otherwise each XCD
+// fetches every slice). Same work per workgroup either way.
+constexpr int kXcds = 8;
+__device__ __forceinline__ void fwd_tile_slice(int& tile, int& slice) {
+    const int tiles = gridDim.x;
+    if (gridDim.y % kXcds == 0) {
+        const int64_t w = blockIdx.x + (int64_t)tiles * blockIdx.y;
+        const int64_t idx = w / kXcds;
+        slice = (int)(w % kXcds + kXcds * (idx / tiles));
+        tile = (int)(idx % tiles);
+    } else {
+        tile = blockIdx.x;
+        slice = blockIdx.y;
+    }
+}
 
 // Workgroup barrier without __syncthreads()'s release fence: that fence makes
 // the compiler wait for every outstanding vector memory operation
@@ -386,7 +407,9 @@ __global__ __launch_bounds__(256) void batched_fwd_kernel(
     __shared__ v8 lp[2][SPLIT][V];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int row = lane & 31, kh = lane >> 5;
-    const int mbase = blockIdx.x * kBatchTile + wave * 32;
+    int tile, slice;
+    fwd_tile_slice(tile, slice);
+    const int mbase = tile * kBatchTile + wave * 32;
     const float4 p = st[mbase + row];
     const float gs = Half<H>::scaled ? gsc[mbase + row] : 1.f;
 
@@ -396,7 +419,7 @@ __global__ __launch_bounds__(256) void batched_fwd_kernel(
 #pragma unroll
         for (int i = 0; i < 16; ++i) acc[t][i] = 0.f;
 
-    const int c0 = blockIdx.y * chunks_per_slice;
+    const int c0 = slice * chunks_per_slice;
     const int c1 = min(c0 + chunks_per_slice, n_chunks);
     // P parts of a chunk go from global memory straight into LDS buffer buf
     // (global_load_lds_dwordx4: lane l of a wave writes 16 bytes at the wave's
@@ -473,7 +496,7 @@ __global__ __launch_bounds__(256) void batched_fwd_kernel(
         __syncthreads();
     }
     // D layout: lane holds model 32t + lane%32, stations 8*(i/4) + 4*(lane/32) + i%4
-    float* out = part + ((size_t)blockIdx.y * m_pad + mbase) * Bp;
+    float* out = part + ((size_t)slice * m_pad + mbase) * Bp;
 #pragma unroll
     for (int t = 0; t < NT; ++t)
 #pragma unroll
@@ -510,7 +533,9 @@ void batched_fwd2_kernel(const float4* __restrict__ st, const typename Half<H>::
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);   // uniform: SALU math
     const int lane = threadIdx.x & 63;
     const int row = lane & 31, kh = lane >> 5;
-    const int mbase = blockIdx.x * (32 * WV) + wave * 32;
+    int tile, slice;
+    fwd_tile_slice(tile, slice);
+    const int mbase = tile * (32 * WV) + wave * 32;
     const float4 p = st[mbase + row];
     const float gs = Half<H>::scaled ? gsc[mbase + row] : 1.f;
 
@@ -520,7 +545,7 @@ void batched_fwd2_kernel(const float4* __restrict__ st, const typename Half<H>::
 #pragma unroll
         for (int i = 0; i < 16; ++i) acc[t][i] = 0.f;
 
-    const int c0 = blockIdx.y * chunks_per_slice;
+    const int c0 = slice * chunks_per_slice;
     const int c1 = min(c0 + chunks_per_slice, n_chunks);
     // Per-thread source offsets of the chunk's P slots (fixed), and the
     // wave's LDS destinations (uniform).
@@ -621,7 +646,7 @@ void batched_fwd2_kernel(const float4* __restrict__ st, const typename Half<H>::
     }
     // no load into this workgroup's LDS may still be in flight when it ends
     if constexpr (PF == 2) __builtin_amdgcn_s_waitcnt(kWaitVm0);
-    float* out = part + ((size_t)blockIdx.y * m_pad + mbase) * Bp;
+    float* out = part + ((size_t)slice * m_pad + mbase) * Bp;
 #pragma unroll
     for (int t = 0; t < NT; ++t)
 #pragma unroll
