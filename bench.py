@@ -272,6 +272,8 @@ def summary(line):
                    line, "batched", "fp16x2", "forward_plus_adjoint",
                    "speedup_vs_fp64_same_algorithm")},
            "point_mass_vs_cpu_baseline": line.get("speedup_vs_cpu"),
+           "gn_step_vs_cpu_baseline": get(line, "gn_step", "speedup_vs_cpu"),
+           "gn_step_normal_equations_mfma_frac": get(line, "gn_step", "roofline", "frac"),
            "max_rel_err_vs_fp64": {
                "point_mass_fp32": worst(line.get("rel_err_vs_fp64")),
                "prism_fp32": worst(get(line, "prism", "rel_err_vs_fp64")),

@@ -39,6 +39,7 @@ def test_summary_gathers_the_legs():
                                         "ensemble_fp16x2": 94.0,
                                         "ensemble_fp16x2_vs_fp64_same_algorithm": 7.0}
     assert s["point_mass_vs_cpu_baseline"] == 4800.0
+    assert s["gn_step_vs_cpu_baseline"] is None
     assert s["max_rel_err_vs_fp64"] == {"point_mass_fp32": 3e-7, "prism_fp32": 5e-7,
                                         "ensemble_split3": 1.9e-6, "ensemble_split2": 6.1e-6,
                                         "ensemble_fp16x2": 1.7e-6}
@@ -51,6 +52,10 @@ def test_summary_without_extra_legs():
     assert s["max_rel_err_vs_fp64"]["ensemble_split2"] is None
     assert s["max_rel_err_vs_fp64"]["ensemble_fp16x2"] is None
     assert s["point_mass_vs_cpu_baseline"] is None
+    assert s["gn_step_normal_equations_mfma_frac"] is None
+    s = b.summary({"gn_step": {"speedup_vs_cpu": 4000.0, "roofline": {"frac": 0.5}}})
+    assert s["gn_step_vs_cpu_baseline"] == 4000.0
+    assert s["gn_step_normal_equations_mfma_frac"] == 0.5
 
 
 def test_more_than_one_gpu_is_refused():
