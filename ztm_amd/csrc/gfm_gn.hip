@@ -272,50 +272,101 @@ __global__ __launch_bounds__(256) void gn_form_kernel(const double* __restrict__
     }
 }
 
-// Panel step at column p of every matrix (grid = (panel workgroups, nd)).
-// Each workgroup factors the diagonal block A = K[p:p+w][p:p+w] in LDS by
-// symmetric elimination on [A | I] without pivoting: step j subtracts
-// a_ij / d_j times row j from every row i > j (the lower triangle of A and the
-// right half), one barrier per step. Then A = L' D L'^T with the pivots d_j
-// on the diagonal, the multipliers' numerators a_ij = L'_ij d_j below it and
-// L'^-1 in the right half, so L = L' D^1/2 and L^-1 = D^-1/2 L'^-1. A pivot
-// that is not positive (or NaN) flags the matrix singular and is taken as 1.
-// Workgroup x then solves its 64 rows of the panel below the block,
-// X = A_panel L^-T, from LDS; workgroup 0 also writes L^-1 and the flag. The
-// diagonal block of K keeps A: the other workgroups read it, and the
-// substitutions take the diagonal blocks from Dinv, so L is never stored.
-__global__ __launch_bounds__(256) void gn_panel_kernel(double* __restrict__ K, int n, int p,
-                                                       double* __restrict__ Dinv,
-                                                       int* __restrict__ flags) {
-    __shared__ double a[kNb * kNbLd];
-    __shared__ double e[kNb * kNbLd];
-    __shared__ double x[kPanelRows * kNbLd];
-    __shared__ double rsq[kNb];
+// One step of the blocked Cholesky, at column block q of every matrix
+// (grid = (panel workgroups + update tiles, nd)), with a look-ahead of one
+// block: the previous block's (p = q - 32) trailing update is split between
+// this step's two roles, which touch disjoint columns.
+//
+// Panel role (workgroups x < pw). First the previous block's update of the
+// columns q .. q + w of this workgroup's rows: the diagonal block and its 64
+// panel rows, each minus L[row][p..p+32) L[q+c][p..p+32)^T (the previous
+// panel's rows, in LDS). Then each workgroup factors the 32 x 32 diagonal
+// block A by symmetric elimination on [A | I] without pivoting: step j
+// subtracts a_ij / d_j times row j from every row i > j (the lower triangle
+// of A and the right half), one barrier per step. A = L' D L'^T with the
+// pivots d_j on the diagonal, the multipliers' numerators a_ij = L'_ij d_j
+// below it and L'^-1 in the right half, so L^-1 = D^-1/2 L'^-1. A pivot that
+// is not positive (or NaN) flags the matrix singular and is taken as 1. The
+// workgroup then solves its 64 panel rows, X = A_panel L^-T, from LDS;
+// workgroup 0 also writes L^-1 (Dinv) and the flag. The factorisation is
+// repeated in every panel workgroup: they run side by side, and the
+// diagonal block of K keeps its values for all of them to read. L's diagonal
+// blocks are never stored: the substitutions take them from Dinv.
+//
+// Update role (the other workgroups): the rest of the previous block's
+// update, K[i][j] -= sum_k L[i][p + k] L[j][p + k] for i >= j >= q + 32, in
+// 64 x 64 tiles (bi >= bj) on FP64 MFMA (each wave a 32 x 32 quadrant, 2 x 2
+// tiles of 16, K = 32 in steps of 4).
+//
+// So a step costs one launch instead of two, and the next block's panel,
+// which waits on only 32 columns of the update, no longer waits for all of it.
+constexpr int kStepLds = 2 * kPanelRows * kNbLd;   // doubles: both roles' tiles
+
+__device__ void chol_panel(double* __restrict__ Kb, int n, int q, int xw,
+                           double* __restrict__ Db, int* __restrict__ flag, double* sm) {
+    double* a = sm;                      // [kNb][kNbLd] the diagonal block (lower)
+    double* e = sm + kNb * kNbLd;        // [kNb][kNbLd] the previous panel's rows q.., then
+                                         // [A | I]'s right half, then L^-1
+    double* x = sm + 2 * kNb * kNbLd;    // [kPanelRows][kNbLd] this workgroup's panel rows
+    double* rsq = sm + kStepLds;         // [kNb]
     const int tid = threadIdx.x;
-    const int b = blockIdx.y;
-    double* Kb = K + (size_t)b * n * n;
-    const int w = min(kNb, n - p);
-    const int r0 = p + w + blockIdx.x * kPanelRows;
+    const int w = min(kNb, n - q);
+    const int r0 = q + w + xw * kPanelRows;
     const int prow = max(0, min(kPanelRows, n - r0));
-    // this workgroup's panel rows, loaded first so they arrive during the
-    // factorisation
-    constexpr int kPv = kPanelRows * kNb / 256;
-    double pv[kPv];
+    const bool prev = q > 0;
+    const int p = q - kNb;
+    // this thread's panel entries: row ri, columns c0 .. c0 + 7
+    const int ri = tid >> 2, c0 = 8 * (tid & 3);
+    const bool row_in = ri < prow;
+    double pv[8];
 #pragma unroll
-    for (int u = 0; u < kPv; ++u) {
-        const int t = tid + 256 * u;
-        const int i = t / kNb, j = t % kNb;
-        const bool in = i < prow && j < w;
-        const double v = *(in ? Kb + (size_t)(r0 + i) * n + p + j : Kb);
+    for (int u = 0; u < 8; ++u) {
+        const bool in = row_in && c0 + u < w;
+        const double v = *(in ? Kb + (size_t)(r0 + ri) * n + q + c0 + u : Kb);
         pv[u] = in ? v : 0.0;
+    }
+    double pr[kNb];   // the previous panel's L on this thread's row
+#pragma unroll
+    for (int k = 0; k < kNb; ++k) {
+        const bool in = prev && row_in;
+        const double v = *(in ? Kb + (size_t)(r0 + ri) * n + p + k : Kb);
+        pr[k] = in ? v : 0.0;
     }
     for (int t = tid; t < kNb * kNb; t += 256) {
         const int i = t / kNb, j = t % kNb;
         const bool in = i < w && j <= i;
-        const double v = *(in ? Kb + (size_t)(p + i) * n + p + j : Kb);
+        const double v = *(in ? Kb + (size_t)(q + i) * n + q + j : Kb);
         a[i * kNbLd + j] = in ? v : 0.0;
+        const bool inp = prev && i < w;
+        const double vp = *(inp ? Kb + (size_t)(q + i) * n + p + j : Kb);
+        e[i * kNbLd + j] = inp ? vp : 0.0;
+    }
+    __syncthreads();
+    if (prev) {   // the previous block's update of columns q .. q + w
+        for (int t = tid; t < kNb * kNb; t += 256) {
+            const int i = t / kNb, c = t % kNb;
+            if (i < w && c <= i) {
+                double s = 0.0;
+#pragma unroll 8
+                for (int k = 0; k < kNb; ++k) s += e[i * kNbLd + k] * e[c * kNbLd + k];
+                a[i * kNbLd + c] -= s;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            double s = 0.0;
+#pragma unroll
+            for (int k = 0; k < kNb; ++k) s += pr[k] * e[(c0 + u) * kNbLd + k];
+            pv[u] -= s;
+        }
+        __syncthreads();   // every read of the previous panel's rows is done
+    }
+    for (int t = tid; t < kNb * kNb; t += 256) {
+        const int i = t / kNb, j = t % kNb;
         e[i * kNbLd + j] = i == j ? 1.0 : 0.0;
     }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) x[ri * kNbLd + c0 + u] = pv[u];
     __syncthreads();
     bool singular = false;
     for (int j = 0; j < w; ++j) {
@@ -351,48 +402,34 @@ __global__ __launch_bounds__(256) void gn_panel_kernel(double* __restrict__ K, i
         const int i = t / kNb, c = t % kNb;
         e[i * kNbLd + c] = i < w && c <= i ? e[i * kNbLd + c] * rsq[i] : 0.0;
     }
-#pragma unroll
-    for (int u = 0; u < kPv; ++u) {
-        const int t = tid + 256 * u;
-        x[(t / kNb) * kNbLd + t % kNb] = pv[u];
-    }
     __syncthreads();
-    if (blockIdx.x == 0) {
-        double* Db = Dinv + ((size_t)b * ((n + kNb - 1) / kNb) + p / kNb) * kNb * kNb;
+    if (xw == 0) {
         for (int t = tid; t < kNb * kNb; t += 256) Db[t] = e[(t / kNb) * kNbLd + t % kNb];
-        if (tid == 0 && singular) flags[b] = 1;
+        if (tid == 0 && singular) *flag = 1;
     }
     // X[i][j] = sum_{k <= j} A[i][k] L^-1[j][k]
-#pragma unroll
-    for (int u = 0; u < kPv; ++u) {
-        const int t = tid + 256 * u;
+    for (int t = tid; t < kPanelRows * kNb; t += 256) {
         const int i = t / kNb, j = t % kNb;
         if (i < prow && j < w) {
             double s = 0.0;
             for (int k = 0; k <= j; ++k) s += x[i * kNbLd + k] * e[j * kNbLd + k];
-            Kb[(size_t)(r0 + i) * n + p + j] = s;
+            Kb[(size_t)(r0 + i) * n + q + j] = s;
         }
     }
 }
 
-// The trailing lower triangle: K[i][j] -= sum_k L[i][p + k] L[j][p + k],
-// i >= j >= p + kNb, 64 x 64 tiles (bi >= bj) on FP64 MFMA (each wave a
-// 32 x 32 quadrant, 2 x 2 tiles of 16, K = kNb in steps of 4).
-// grid = (tiles, nd). Only called with a full block (w = kNb).
-__global__ __launch_bounds__(256) void gn_update_kernel(double* __restrict__ K, int n, int p,
-                                                        int nt) {
-    __shared__ double li[64 * kNbLd];
-    __shared__ double lj[64 * kNbLd];
-    double* Kb = K + (size_t)blockIdx.y * n * n;
-    // blockIdx.x -> (bi, bj), bi >= bj, row by row: row bi holds bi + 1 tiles
-    int t = blockIdx.x, bi = 0;
+// Update tile t of the region i >= j >= u0 from the panel at columns p.
+__device__ void chol_update(double* __restrict__ Kb, int n, int p, int u0, int t, double* sm) {
+    double* li = sm;                     // [64][kNbLd]
+    double* lj = sm + 64 * kNbLd;        // [64][kNbLd]
+    // t -> (bi, bj), bi >= bj, row by row: row bi holds bi + 1 tiles
+    int bi = 0;
     while (t > bi) {
         t -= bi + 1;
         ++bi;
     }
     const int bj = t;
-    const int q0 = p + kNb;
-    const int i0 = q0 + 64 * bi, j0 = q0 + 64 * bj;
+    const int i0 = u0 + 64 * bi, j0 = u0 + 64 * bj;
     for (int e = threadIdx.x; e < 64 * kNb; e += 256) {
         const int rr = e / kNb, k = e % kNb;
         li[rr * kNbLd + k] = i0 + rr < n ? Kb[(size_t)(i0 + rr) * n + p + k] : 0.0;
@@ -431,6 +468,20 @@ __global__ __launch_bounds__(256) void gn_update_kernel(double* __restrict__ K, 
                 const int j = j0 + wj + 16 * tb + l16;
                 if (i < n && j <= i) Kb[(size_t)i * n + j] -= acc[ta][tb][e];
             }
+}
+
+__global__ __launch_bounds__(256) void gn_chol_step_kernel(double* __restrict__ K, int n, int q,
+                                                           int pw, double* __restrict__ Dinv,
+                                                           int* __restrict__ flags) {
+    __shared__ double sm[kStepLds + kNb];
+    const int b = blockIdx.y;
+    double* Kb = K + (size_t)b * n * n;
+    if ((int)blockIdx.x < pw) {
+        double* Db = Dinv + ((size_t)b * ((n + kNb - 1) / kNb) + q / kNb) * kNb * kNb;
+        chol_panel(Kb, n, q, blockIdx.x, Db, flags + b, sm);
+    } else {
+        chol_update(Kb, n, q - kNb, q + kNb, blockIdx.x - pw, sm);
+    }
 }
 
 // ------------------------------------------------------------ substitution
@@ -601,19 +652,21 @@ size_t factor_bytes(int n, int nd) {
 }
 
 // Factor nd matrices K [nd][n][n] in place (lower triangle), the inverses of
-// the diagonal blocks into Dinv.
+// the diagonal blocks into Dinv: one launch per 32-column block (the panel,
+// with the previous block's update of its columns, beside the rest of that
+// update).
 int cholesky_batched(double* K, double* Dinv, int n, int nd, int* flags, hipStream_t stream) {
-    for (int p = 0; p < n; p += kNb) {
-        const int w = std::min(kNb, n - p);
-        const int below = n - p - w;
+    for (int q = 0; q < n; q += kNb) {
+        const int w = std::min(kNb, n - q);
+        const int below = n - q - w;
         const int pw = std::max(1, (below + kPanelRows - 1) / kPanelRows);
-        hipLaunchKernelGGL(gn_panel_kernel, dim3(pw, nd), dim3(256), 0, stream, K, n, p, Dinv,
-                           flags);
-        ZTM_HIP(hipGetLastError());
-        if (below <= 0) break;
-        const int nt = (below + 63) / 64;
-        hipLaunchKernelGGL(gn_update_kernel, dim3(nt * (nt + 1) / 2, nd), dim3(256), 0, stream,
-                           K, n, p, nt);
+        int ut = 0;   // the previous block's update tiles, rows and columns >= q + 32
+        if (q > 0 && n > q + kNb) {
+            const int nt = (n - q - kNb + 63) / 64;
+            ut = nt * (nt + 1) / 2;
+        }
+        hipLaunchKernelGGL(gn_chol_step_kernel, dim3(pw + ut, nd), dim3(256), 0, stream, K, n, q,
+                           pw, Dinv, flags);
         ZTM_HIP(hipGetLastError());
     }
     return ZTM_OK;
