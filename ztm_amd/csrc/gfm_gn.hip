@@ -497,42 +497,66 @@ __device__ __forceinline__ double half_wave_sum(double v) {   // over 32 lanes
     return v;
 }
 
+// Workgroup barrier for the substitutions: orders the workgroup's LDS
+// accesses (lgkmcnt(0), then s_barrier) and leaves global loads in flight,
+// where __syncthreads()' release fence would wait for them too and so stall
+// on the next block's prefetched rows. The memory clobber keeps the compiler
+// from moving memory accesses across it.
+constexpr int kWaitLgkm0 = 0xC07F;   // s_waitcnt lgkmcnt(0), vmcnt and expcnt at their maxima
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_s_waitcnt(kWaitLgkm0);
+    asm volatile("s_barrier" ::: "memory");
+}
+
 // Forward substitution L y = rhs over rows q_begin .. n - 1 (earlier y are 0),
 // y in LDS, rhs in y on entry, by blocks of kNb rows. For block q, the 32
 // threads of row q + i (i = tid / 32) form its product with the solved part
 // over consecutive columns; then y_q = Dinv_q (rhs_q minus those products),
-// the 32 threads of row i over the block's columns. Dinv_q is loaded with
-// the row products.
+// the 32 threads of row i over the block's columns. A block's rows on the
+// first 1,024 columns and its Dinv entry are loaded while the block before
+// it is solved, so the step waits on no memory round trip below 1,024
+// columns.
 __device__ void forward_subst(const double* __restrict__ L, const double* __restrict__ Dv, int n,
                               int q_begin, double* y, double* s_blk) {
     const int tid = threadIdx.x, i = tid >> 5, c = tid & 31;
+    constexpr int kChunk = 32 * kNb;   // columns a row's 32 threads cover per pass
+    double lv[kNb], dv;
+    int lim = 0;   // the prefetched entries m with 32 m < lim are in the solved part
+    auto prefetch = [&](int qq) {   // block qq's rows on columns q_begin .. q_begin + kChunk
+        const int ww = min(kNb, n - qq);
+        const bool rin = i < ww;
+        const double* ptr = L + (size_t)(qq + (rin ? i : 0)) * n + q_begin + c;
+        lim = rin ? qq - q_begin - c : 0;
+#pragma unroll
+        for (int m = 0; m < kNb; ++m) {
+            lv[m] = 0.0;
+            if (32 * m < lim) lv[m] = ptr[32 * m];
+        }
+        const double d = Dv[(size_t)(qq / kNb) * kNb * kNb + (rin ? i : 0) * kNb + c];
+        dv = rin ? d : 0.0;
+    };
+    prefetch(q_begin);
     for (int q = q_begin; q < n; q += kNb) {
         const int w = min(kNb, n - q);
-        const double dv0 = Dv[(size_t)(q / kNb) * kNb * kNb + (i < w ? i : 0) * kNb + c];
-        const double dv = i < w ? dv0 : 0.0;
         double s = 0.0;
-        if (i < w) {
+        const double* yc = y + q_begin + c;
+#pragma unroll
+        for (int m = 0; m < kNb; ++m)
+            if (32 * m < lim) s += lv[m] * yc[32 * m];
+        if (i < w) {   // columns past the first kChunk (n > kChunk)
             const double* row = L + (size_t)(q + i) * n;
-            for (int k0 = q_begin; k0 < q; k0 += 32 * kNb) {
-                double t[kNb];
-#pragma unroll
-                for (int m = 0; m < kNb; ++m) {
-                    const int k = k0 + c + 32 * m;
-                    const int kk = k < q ? k : q_begin;   // in range for every lane
-                    const double p = row[kk] * y[kk];
-                    t[m] = k < q ? p : 0.0;
-                }
-#pragma unroll
-                for (int m = 0; m < kNb; ++m) s += t[m];
-            }
+#pragma unroll 8
+            for (int k = q_begin + kChunk + c; k < q; k += 32) s += row[k] * y[k];
         }
+        const double dq = dv;
+        if (q + kNb < n) prefetch(q + kNb);
         s = half_wave_sum(s);
         if (c == 0 && i < w) s_blk[i] = y[q + i] - s;
-        __syncthreads();
-        double v = i < w ? dv * s_blk[c] : 0.0;   // Dinv is zero beyond the block
+        lds_barrier();
+        double v = i < w ? dq * s_blk[c] : 0.0;   // Dinv is zero beyond the block
         v = half_wave_sum(v);
         if (c == 0 && i < w) y[q + i] = v;
-        __syncthreads();
+        lds_barrier();
     }
 }
 
@@ -540,34 +564,50 @@ __device__ void forward_subst(const double* __restrict__ L, const double* __rest
 // last, right-looking: x_q = Dinv_q^T y_q (the 32 threads of row j over the
 // block), then every earlier entry takes its share,
 // y[j] -= sum_i L[q + i][j] x_q[i] for j < q (one column per thread, the
-// block's rows of L read across threads).
+// block's rows of L read across threads). A block's rows on the thread's
+// first column and its Dinv entry are loaded at the end of the block after
+// it, so they arrive across the barrier and that block's Dinv step.
 __device__ void backward_subst(const double* __restrict__ L, const double* __restrict__ Dv, int n,
                                double* y, double* s_blk) {
     const int tid = threadIdx.x, j = tid >> 5, c = tid & 31;
     const int nblk = (n + kNb - 1) / kNb;
+    double lb[kNb], dv;
+    auto prefetch = [&](int bq) {   // block bq's rows on column tid, and its Dinv entry
+        const int q = bq * kNb, w = min(kNb, n - q);
+        const double* ptr = L + (size_t)q * n + tid;
+        const int lim = tid < q ? w : 0;   // rows ii < lim
+#pragma unroll
+        for (int ii = 0; ii < kNb; ++ii) {
+            lb[ii] = 0.0;
+            if (ii < lim) lb[ii] = ptr[(size_t)ii * n];
+        }
+        const bool din = j < w && c < w;
+        const double d = Dv[(size_t)bq * kNb * kNb + (din ? c * kNb + j : 0)];
+        dv = din ? d : 0.0;
+    };
+    prefetch(nblk - 1);
     for (int bq = nblk - 1; bq >= 0; --bq) {
         const int q = bq * kNb;
         const int w = min(kNb, n - q);
-        const bool in = j < w && c < w;
-        const double p = Dv[(size_t)bq * kNb * kNb + (in ? c * kNb + j : 0)] * y[q + (in ? c : 0)];
-        double v = in ? p : 0.0;
+        double v = dv * y[q + (j < w && c < w ? c : 0)];   // dv is 0 outside the block
         v = half_wave_sum(v);
         if (c == 0 && j < w) s_blk[j] = v;
-        __syncthreads();
+        lds_barrier();
         if (tid < w) y[q + tid] = s_blk[tid];
-        for (int jj = tid; jj < q; jj += kSolveThreads) {
-            double t[kNb];
-#pragma unroll
-            for (int ii = 0; ii < kNb; ++ii) {
-                const double p = L[(size_t)(q + (ii < w ? ii : 0)) * n + jj] * s_blk[ii < w ? ii : 0];
-                t[ii] = ii < w ? p : 0.0;
-            }
+        if (tid < q) {
             double acc = 0.0;
 #pragma unroll
-            for (int ii = 0; ii < kNb; ++ii) acc += t[ii];
+            for (int ii = 0; ii < kNb; ++ii) acc += lb[ii] * s_blk[ii < w ? ii : 0];
+            y[tid] -= acc;
+        }
+        for (int jj = tid + kSolveThreads; jj < q; jj += kSolveThreads) {
+            double acc = 0.0;
+#pragma unroll 8
+            for (int ii = 0; ii < w; ++ii) acc += L[(size_t)(q + ii) * n + jj] * s_blk[ii];
             y[jj] -= acc;
         }
-        __syncthreads();
+        if (bq > 0) prefetch(bq - 1);   // in flight across the barrier and the next Dinv step
+        lds_barrier();
     }
 }
 
