@@ -126,3 +126,23 @@ def test_gn_full_size_properties(dev):
         K = JJ + torch.diag(d0 * dJ + d1 / sig**2)
         res = torch.max(torch.abs(K @ B[k] - jtr)) / torch.max(torch.abs(jtr))
         assert res.item() < 1e-10, k
+
+
+def test_damped_steps_many_conditions(dev, orc):
+    """40 seeded damping conditions (d0 in [0, 100], d1 in [1e3, 1e5]) at
+    n = 97 (three full 32-column blocks and a 1-column one), 300 data: every
+    step against the oracle's Gauss-Jordan step to 1e-9 of its size."""
+    rng = np.random.default_rng(97)
+    n, rows, nd = 97, 300, 40
+    J = jacobian(98, rows, n)
+    r = rng.normal(size=rows)
+    sig = rng.uniform(0.05, 4.0, n)
+    b = rng.normal(size=n)
+    d1 = np.exp(rng.uniform(np.log(1e3), np.log(1e5), nd))
+    dm = np.stack([rng.uniform(0.0, 100.0, nd), d1], axis=1)
+    JJ, jtr = orc.mtm(J), orc.jtr(J, r)
+    want = orc.gn_steps(JJ, jtr, sig, dm, b)
+    B = gfm.gn_damped_steps(_t(JJ, dev), _t(jtr, dev), _t(sig, dev), _t(dm, dev),
+                            _t(b, dev)).cpu().numpy()
+    for k in range(nd):
+        assert np.max(np.abs(B[k] - want[k])) <= 1e-9 * np.max(np.abs(want[k] - b)), k

@@ -135,3 +135,48 @@ def test_zero_station_adjoints_in_graph(dev):
             graph.replay()
             torch.cuda.synchronize()
             assert all(not o.any() for o in outs), rep
+
+
+def test_gauss_newton_la_in_graph(dev):
+    """The dense LA of a Gauss-Newton step (normal equations, the 12 damped
+    steps, the inverse diagonal) on caller buffers, captured and replayed 3x
+    with eager calls on other buffers between replays: every replay equals
+    the eager result bit for bit (no call allocates or issues a memset)."""
+    import torch
+    rows, n = 2000, 100
+    J = torch.from_numpy(W.normal(11, 1, rows * n).reshape(rows, n)).to(dev)
+    r = torch.from_numpy(W.normal(11, 2, rows)).to(dev)
+    sig = torch.from_numpy(W.uniform(11, 3, n, 0.05, 4.0)).to(dev)
+    b = torch.from_numpy(W.normal(11, 4, n)).to(dev)
+    dm = torch.from_numpy(W.GN_DAMPING).to(dev)
+    nd = dm.shape[0]
+    f64 = dict(dtype=torch.float64, device=dev)
+    JJ, jtr, B, V = (torch.empty(n, n, **f64), torch.empty(n, **f64), torch.empty(nd, n, **f64),
+                     torch.empty(n, **f64))
+    work = torch.empty(gfm.gn_workspace_bytes(rows, n, nd), dtype=torch.uint8, device=dev)
+
+    def calls():
+        gfm.gn_normal(J, r, out=(JJ, jtr), work=work)
+        gfm.gn_damped_steps(JJ, jtr, sig, dm, b, out=B, work=work)
+        gfm.gn_inverse_diag(JJ, 1e-3, out=V, work=work)
+
+    calls()
+    torch.cuda.synchronize()
+    want = [t.clone() for t in (JJ, jtr, B, V)]
+    graph = torch.cuda.CUDAGraph()
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        with torch.cuda.graph(graph):
+            calls()
+    torch.cuda.current_stream().wait_stream(side)
+    for rep in range(3):
+        for t in (JJ, jtr, B, V):
+            t.fill_(-1.0)
+        graph.replay()
+        torch.cuda.synchronize()
+        for got, ref in zip((JJ, jtr, B, V), want):
+            assert torch.equal(got, ref), rep
+        JJ2, jtr2 = gfm.gn_normal(2.0 * J, r)          # eager work between replays
+        gfm.gn_damped_steps(JJ2, jtr2, sig, dm, b)
+        torch.cuda.synchronize()
