@@ -93,7 +93,8 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the stored-matrix and prism legs")
-    ap.add_argument("--leg", choices=["prism", "batched", "uneven", "tfm", "accumulation", "gn"],
+    ap.add_argument("--leg", choices=["prism", "batched", "uneven", "tfm", "accumulation", "gn",
+                                      "mfma"],
                     default=None,
                     help="run only this extra leg and print its JSON object (for profiling)")
     ap.add_argument("--copy-calib-mib", type=int, default=0,
@@ -121,9 +122,9 @@ def main() -> int:
         torch.cuda.synchronize()
         print(json.dumps({"copy_calibration_bytes": 4 * n, "copies": 5}))
         return 0
-    if args.leg in ("batched", "uneven", "tfm", "accumulation", "gn"):
+    if args.leg in ("batched", "uneven", "tfm", "accumulation", "gn", "mfma"):
         leg = {"batched": batched_leg, "uneven": uneven_leg, "tfm": tfm_leg,
-               "accumulation": accumulation_leg, "gn": gn_leg}[args.leg]
+               "accumulation": accumulation_leg, "gn": gn_leg, "mfma": mfma_leg}[args.leg]
         print(json.dumps({args.leg: leg(torch.cuda.current_stream(), args)}))
         return 0
     w = W.pm_blobs_1m()
@@ -787,6 +788,23 @@ def accumulation_leg(stream, args):
                 Y = p.forward_batched(Pt, stream=stream).cpu().numpy()
                 row[name] = max(relerr(Y[b], r) for b, r in ref.items())
             out["rows"].append(row)
+    return out
+
+
+def mfma_leg(stream, args):
+    """The MFMA pipe's sustained rate per shape (ztm_calib_mfma): every wave
+    streams MFMAs on register operands, nothing else in the loop, on random
+    and on low-entropy operands. Under the chip's power limit the rate with
+    random operands is the ceiling of a data-dependent MFMA kernel."""
+    _progress("mfma calibration leg")
+    from ztm_amd import gfm
+    out = {}
+    for op in gfm.CALIB_OPS:
+        peak = 2500.0   # dense FP16 / BF16, published
+        for low in (False, True):
+            v = gfm.calib_mfma(op, low_entropy=low, iters=2000, stream=stream)
+            out[f"{op}_{'low_entropy' if low else 'random'}"] = {
+                "tflops": v, "peak": peak, "frac": v / peak}
     return out
 
 

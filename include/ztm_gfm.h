@@ -246,6 +246,18 @@ int ztm_gn_damped_steps(int n, int nd, const double* JJ_dev, const double* jtr_d
 int ztm_gn_inverse_diag(int n, const double* JJ_dev, double d, double* out_dev,
                         void* work_dev, void* stream);
 
+/* MFMA pipe calibration (no reference counterpart; a measurement harness for
+ * the MFMA kernels' rooflines): the FLOP rate this chip sustains on one MFMA
+ * shape with nothing else in the loop -- every wave streams MFMAs on
+ * register operands into independent accumulators over 8 workgroups of 4
+ * waves per CU -- with operands of real entropy (low_entropy = 0: uniform
+ * random values) or of low entropy (1: values 1 and -0.5). op 0:
+ * v_mfma_f32_32x32x16_f16, 1: v_mfma_f32_32x32x16_bf16, 2:
+ * v_mfma_f32_16x16x32_f16. Under the power limit the sustained rate depends
+ * on the operands. Allocates and frees its own buffers and synchronises the
+ * stream (a measurement call). */
+int ztm_calib_mfma(int op, int low_entropy, int iters, double* tflops, void* stream);
+
 /* FP64 ensemble (no reference counterpart; the FP64 form of the batched
  * ensemble, for callers that need FP64 and as its same-algorithm FP64
  * comparison): G generated once per tile in FP64, reused across the whole

@@ -23,7 +23,7 @@ LIB = os.path.join(PKG, "libztm_gfm.so")
 GFM = os.path.join(PKG, "bin", "GFM")
 
 HIP_SOURCES = ["gfm_abi.hip", "gfm_batch.hip", "gfm_batched.hip", "gfm_batched64.hip",
-               "gfm_gn.hip", "gfm_noalloc.hip",
+               "gfm_calib.hip", "gfm_gn.hip", "gfm_noalloc.hip",
                "gfm_pointmass.hip", "gfm_prism.hip", "gfm_reg.hip", "gfm_stored.hip",
                "gfm_tfm.hip"]
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")

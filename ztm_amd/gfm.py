@@ -52,6 +52,7 @@ class CProblem(ctypes.Structure):
 # name -> (restype, argtypes); every symbol declared in include/ztm_gfm.h.
 _P, _I, _I64, _D = ctypes.c_void_p, ctypes.c_int, ctypes.c_int64, ctypes.c_double
 _IP = ctypes.POINTER(ctypes.c_int)
+_DP = ctypes.POINTER(ctypes.c_double)
 SIGNATURES = {
     "ztm_version": (_I, []),
     "ztm_strerror": (ctypes.c_char_p, [_I]),
@@ -73,6 +74,7 @@ SIGNATURES = {
     "ztm_plan_forward_batched": (_I, [_P, _I, _P, _P, _P]),
     "ztm_plan_adjoint_batched": (_I, [_P, _I, _P, _P, _P]),
     "ztm_plan_forward_batched64": (_I, [_P, _I, _P, _P, _P]),
+    "ztm_calib_mfma": (_I, [_I, _I, _I, _DP, _P]),
     "ztm_gn_workspace_bytes": (_I64, [_I64, _I, _I]),
     "ztm_gn_normal": (_I, [_I64, _I, _P, _P, _P, _P, _P, _P]),
     "ztm_gn_damped_steps": (_I, [_I, _I, _P, _P, _P, _P, _P, _P, _P, _P]),
@@ -875,6 +877,19 @@ def _check_t(t, dtype, n: int, name: str) -> None:
 # --------------------------------------------------------------- profiling
 def profile_enable(on: bool = True) -> None:
     check(lib().ztm_profile_enable(1 if on else 0))
+
+
+CALIB_OPS = {"32x32x16_f16": 0, "32x32x16_bf16": 1, "16x16x32_f16": 2}
+
+
+def calib_mfma(op: str, low_entropy: bool = False, iters: int = 2000, stream=None) -> float:
+    """TFLOP/s the chip sustains on one MFMA shape with nothing else in the
+    loop (ztm_calib_mfma): op is a key of CALIB_OPS; low_entropy takes
+    operands 1 and -0.5 instead of uniform random values."""
+    v = ctypes.c_double()
+    check(lib().ztm_calib_mfma(CALIB_OPS[op], 1 if low_entropy else 0, iters, ctypes.byref(v),
+                               _stream(stream)))
+    return v.value
 
 
 def profile_read(kernel: str) -> tuple[float, int]:
