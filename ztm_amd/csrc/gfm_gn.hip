@@ -273,155 +273,209 @@ __global__ __launch_bounds__(256) void gn_form_kernel(const double* __restrict__
 }
 
 // One step of the blocked Cholesky, at column block q of every matrix
-// (grid = (panel workgroups + update tiles, nd)), with a look-ahead of one
+// (grid = nd x (panel workgroups + update tiles)), with a look-ahead of one
 // block: the previous block's (p = q - 32) trailing update is split between
 // this step's two roles, which touch disjoint columns.
 //
 // Panel role (workgroups x < pw). First the previous block's update of the
 // columns q .. q + w of this workgroup's rows: the diagonal block and its 64
-// panel rows, each minus L[row][p..p+32) L[q+c][p..p+32)^T (the previous
-// panel's rows, in LDS). Then each workgroup factors the 32 x 32 diagonal
-// block A by symmetric elimination on [A | I] without pivoting: step j
-// subtracts a_ij / d_j times row j from every row i > j (the lower triangle
-// of A and the right half), one barrier per step. A = L' D L'^T with the
-// pivots d_j on the diagonal, the multipliers' numerators a_ij = L'_ij d_j
-// below it and L'^-1 in the right half, so L^-1 = D^-1/2 L'^-1. A pivot that
-// is not positive (or NaN) flags the matrix singular and is taken as 1. The
-// workgroup then solves its 64 panel rows, X = A_panel L^-T, from LDS;
-// workgroup 0 also writes L^-1 (Dinv) and the flag. The factorisation is
-// repeated in every panel workgroup: they run side by side, and the
-// diagonal block of K keeps its values for all of them to read. L's diagonal
-// blocks are never stored: the substitutions take them from Dinv.
+// panel rows, each minus L[row][p..p+32) L[q+c][p..p+32)^T, on FP64 MFMA with
+// the operands read from K into registers. Then wave 0 factors the 32 x 32
+// diagonal block A by symmetric elimination on [A | I] without pivoting,
+// lanes l < 32 holding row l of A and lanes 32 + c column c of I in
+// registers: step j publishes column j of A through LDS (a_jj is the pivot
+// d_j) and every lane subtracts a_kj (reg[j] / d_j) from its reg[k], k > j,
+// which updates the rows of A below j and the columns of the right half. No
+// workgroup barrier sits in this chain. A = L' D L'^T, with L'^-1 left in the
+// right half, each row j scaled by d_j^-1/2 once final, so that it holds
+// L^-1 = D^-1/2 L'^-1. A pivot that is not positive (or NaN) flags the matrix
+// singular and is taken as 1; the rows and columns past a last block's width
+// are an identity. The workgroup then solves its 64 panel rows,
+// X = A_panel L^-T, on FP64 MFMA; workgroup 0 also writes L^-1 (Dinv) and the
+// flag. The factorisation is repeated in every panel workgroup: they run side
+// by side, and the diagonal block of K keeps its values for all of them to
+// read. L's diagonal blocks are never stored: the substitutions take them
+// from Dinv.
 //
 // Update role (the other workgroups): the rest of the previous block's
 // update, K[i][j] -= sum_k L[i][p + k] L[j][p + k] for i >= j >= q + 32, in
 // 64 x 64 tiles (bi >= bj) on FP64 MFMA (each wave a 32 x 32 quadrant, 2 x 2
-// tiles of 16, K = 32 in steps of 4).
+// tiles of 16, K = 32 in steps of 4), operands and the tile's entries loaded
+// into registers at once.
 //
 // So a step costs one launch instead of two, and the next block's panel,
 // which waits on only 32 columns of the update, no longer waits for all of it.
-constexpr int kStepLds = 2 * kPanelRows * kNbLd;   // doubles: both roles' tiles
+//
+// MFMA f64 16x16x4: A[i][k] at lane i + 16k, B[k][j] at lane j + 16k, D[i][j]
+// in register i / 4 of lane j + 16 (i % 4); a lane (l16, lk) = (lane % 16,
+// lane / 16) supplies the operands' entries (l16, 4 s + lk) of k-step s.
+constexpr int kStepLds = 4 * kNb * kNbLd;   // doubles: the diagonal block, L^-1, 64 panel rows
+
+// Orders one wave's LDS accesses across its lanes: the wave's stores have
+// completed (lgkmcnt(0)) before any later access issues, and the fences keep
+// the compiler from moving accesses across this point.
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_s_waitcnt(0xC07F);   // lgkmcnt(0)
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 
 __device__ void chol_panel(double* __restrict__ Kb, int n, int q, int xw,
                            double* __restrict__ Db, int* __restrict__ flag, double* sm) {
     double* a = sm;                      // [kNb][kNbLd] the diagonal block (lower)
-    double* e = sm + kNb * kNbLd;        // [kNb][kNbLd] the previous panel's rows q.., then
-                                         // [A | I]'s right half, then L^-1
+    double* e = sm + kNb * kNbLd;        // [kNb][kNbLd] L^-1 of the block
     double* x = sm + 2 * kNb * kNbLd;    // [kPanelRows][kNbLd] this workgroup's panel rows
-    double* rsq = sm + kStepLds;         // [kNb]
-    const int tid = threadIdx.x;
+    double* col = sm + kStepLds;         // [2][kNb] the elimination's published column
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int l16 = lane & 15, lk = lane >> 4;
+    const int ct = wave & 1;             // column tile (16 of the block's 32) of both products
     const int w = min(kNb, n - q);
     const int r0 = q + w + xw * kPanelRows;
     const int prow = max(0, min(kPanelRows, n - r0));
     const bool prev = q > 0;
     const int p = q - kNb;
-    // this thread's panel entries: row ri, columns c0 .. c0 + 7
-    const int ri = tid >> 2, c0 = 8 * (tid & 3);
-    const bool row_in = ri < prow;
-    double pv[8];
+    // the previous block's update of columns q .. q + w: row tiles rt = 0, 1
+    // (the diagonal block) and 2 .. 5 (the panel rows), this wave's three
+    // (wave >> 1) + 2u, all against column tile ct; the loads go first
+    f64x4 acc[3];
 #pragma unroll
-    for (int u = 0; u < 8; ++u) {
-        const bool in = row_in && c0 + u < w;
-        const double v = *(in ? Kb + (size_t)(r0 + ri) * n + q + c0 + u : Kb);
-        pv[u] = in ? v : 0.0;
-    }
-    double pr[kNb];   // the previous panel's L on this thread's row
+    for (int u = 0; u < 3; ++u) acc[u] = f64x4{0.0, 0.0, 0.0, 0.0};
+    double av[3][kNb / 4], bv[kNb / 4];
+    if (prev) {
+        const int bj = 16 * ct + l16;
+        const bool bin = bj < w;
+        const double* bp = Kb + (size_t)(q + (bin ? bj : 0)) * n + p + lk;
 #pragma unroll
-    for (int k = 0; k < kNb; ++k) {
-        const bool in = prev && row_in;
-        const double v = *(in ? Kb + (size_t)(r0 + ri) * n + p + k : Kb);
-        pr[k] = in ? v : 0.0;
+        for (int s = 0; s < kNb / 4; ++s) {
+            const double v = bp[4 * s];
+            bv[s] = bin ? v : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < 3; ++u) {
+            const int rt = (wave >> 1) + 2 * u;
+            const int ri = rt < 2 ? 16 * rt + l16 : 16 * (rt - 2) + l16;
+            const bool in = rt < 2 ? ri < w : ri < prow;
+            const int row = in ? (rt < 2 ? q + ri : r0 + ri) : q;
+            const double* ap = Kb + (size_t)row * n + p + lk;
+#pragma unroll
+            for (int s = 0; s < kNb / 4; ++s) {
+                const double v = ap[4 * s];
+                av[u][s] = in ? v : 0.0;
+            }
+        }
     }
     for (int t = tid; t < kNb * kNb; t += 256) {
         const int i = t / kNb, j = t % kNb;
         const bool in = i < w && j <= i;
         const double v = *(in ? Kb + (size_t)(q + i) * n + q + j : Kb);
         a[i * kNbLd + j] = in ? v : 0.0;
-        const bool inp = prev && i < w;
-        const double vp = *(inp ? Kb + (size_t)(q + i) * n + p + j : Kb);
-        e[i * kNbLd + j] = inp ? vp : 0.0;
     }
-    __syncthreads();
-    if (prev) {   // the previous block's update of columns q .. q + w
-        for (int t = tid; t < kNb * kNb; t += 256) {
-            const int i = t / kNb, c = t % kNb;
-            if (i < w && c <= i) {
-                double s = 0.0;
-#pragma unroll 8
-                for (int k = 0; k < kNb; ++k) s += e[i * kNbLd + k] * e[c * kNbLd + k];
-                a[i * kNbLd + c] -= s;
-            }
-        }
-#pragma unroll
-        for (int u = 0; u < 8; ++u) {
-            double s = 0.0;
-#pragma unroll
-            for (int k = 0; k < kNb; ++k) s += pr[k] * e[(c0 + u) * kNbLd + k];
-            pv[u] -= s;
-        }
-        __syncthreads();   // every read of the previous panel's rows is done
-    }
-    for (int t = tid; t < kNb * kNb; t += 256) {
+    for (int t = tid; t < kPanelRows * kNb; t += 256) {
         const int i = t / kNb, j = t % kNb;
-        e[i * kNbLd + j] = i == j ? 1.0 : 0.0;
+        const bool in = i < prow && j < w;
+        const double v = *(in ? Kb + (size_t)(r0 + i) * n + q + j : Kb);
+        x[i * kNbLd + j] = in ? v : 0.0;
     }
+    if (prev) {
 #pragma unroll
-    for (int u = 0; u < 8; ++u) x[ri * kNbLd + c0 + u] = pv[u];
+        for (int u = 0; u < 3; ++u)
+#pragma unroll
+            for (int s = 0; s < kNb / 4; ++s) acc[u] = mfma64(av[u][s], bv[s], acc[u]);
+    }
     __syncthreads();
-    bool singular = false;
-    for (int j = 0; j < w; ++j) {
-        double d = a[j * kNbLd + j];
-        if (!(d > 0.0)) {
-            singular = true;
-            d = 1.0;
-        }
-        const double rd = 1.0 / d;
-        // thread tid owns the entries (i, k) = (t / 32, t % 32), t = tid + 256 u,
-        // of both halves: the lower trailing entries j < k <= i < w and the
-        // right half's j < i < w, k <= j
+    if (prev) {   // each entry belongs to one lane
 #pragma unroll
-        for (int u = 0; u < kNb * kNb / 256; ++u) {
-            const int t = tid + 256 * u;
-            const int i = t / kNb, k = t % kNb;
-            if (i < w && j < i) {
-                const double f = a[i * kNbLd + j] * rd;
-                if (j < k && k <= i) a[i * kNbLd + k] -= f * a[k * kNbLd + j];
-                if (k <= j) e[i * kNbLd + k] -= f * e[j * kNbLd + k];
+        for (int u = 0; u < 3; ++u) {
+            const int rt = (wave >> 1) + 2 * u;
+            const int j = 16 * ct + l16;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int i = 16 * rt + 4 * r + lk;
+                if (rt < 2) {
+                    if (j <= i) a[i * kNbLd + j] -= acc[u][r];
+                } else {
+                    x[(i - kNb) * kNbLd + j] -= acc[u][r];
+                }
             }
         }
         __syncthreads();
     }
-    if (tid < kNb) {
-        double d = tid < w ? a[tid * kNbLd + tid] : 1.0;
-        if (!(d > 0.0)) d = 1.0;
-        rsq[tid] = 1.0 / sqrt(d);
-    }
-    __syncthreads();
-    // L^-1 = D^-1/2 L'^-1 in place (each entry read and written by one thread)
-    for (int t = tid; t < kNb * kNb; t += 256) {
-        const int i = t / kNb, c = t % kNb;
-        e[i * kNbLd + c] = i < w && c <= i ? e[i * kNbLd + c] * rsq[i] : 0.0;
+    if (wave == 0) {
+        const int l = lane & 31;
+        const bool row_lane = lane < 32;
+        double reg[kNb];
+#pragma unroll
+        for (int k = 0; k < kNb; ++k) {
+            const double v = a[l * kNbLd + k];
+            reg[k] = row_lane && l < w ? v : (k == l ? 1.0 : 0.0);
+        }
+        bool singular = false;
+        double piv = 1.0;   // row lanes: the pivot of their row
+        if (row_lane) col[l] = reg[0];
+#pragma unroll
+        for (int j = 0; j < kNb; ++j) {
+            const double* cj = col + (j & 1) * kNb;   // column j, published
+            double* cn = col + ((j + 1) & 1) * kNb;   // column j + 1, published here
+            wave_lds_sync();
+            double d = cj[j];
+            if (!(d > 0.0)) {
+                singular = true;
+                d = 1.0;
+            }
+            double cv[kNb];   // the column below the pivot, all loads issued at once
+#pragma unroll
+            for (int k = j + 1; k < kNb; ++k) cv[k] = cj[k];
+            piv = l == j ? d : piv;
+            const double y = reg[j] * (1.0 / d);
+#pragma unroll
+            for (int k = j + 1; k < kNb; ++k) {
+                reg[k] -= cv[k] * y;
+                // done here: left free, the compiler defers each update to
+                // the step that reads it and keeps every loaded column live
+                asm volatile("" : "+v"(reg[k]));
+                // the next column goes out first, the rest of the step behind it
+                if (k == j + 1 && row_lane) cn[l] = reg[k];
+            }
+        }
+        // L^-1 = D^-1/2 L'^-1: row k scaled by d_k^-1/2, published by row lane k
+        if (row_lane) col[l] = 1.0 / sqrt(piv);
+        wave_lds_sync();
+        if (!row_lane) {
+#pragma unroll
+            for (int k = 0; k < kNb; ++k)
+                e[k * kNbLd + l] = k < w && l <= k ? reg[k] * col[k] : 0.0;
+        }
+        if (xw == 0 && lane == 0 && singular) *flag = 1;
     }
     __syncthreads();
     if (xw == 0) {
         for (int t = tid; t < kNb * kNb; t += 256) Db[t] = e[(t / kNb) * kNbLd + t % kNb];
-        if (tid == 0 && singular) *flag = 1;
     }
-    // X[i][j] = sum_{k <= j} A[i][k] L^-1[j][k]
-    for (int t = tid; t < kPanelRows * kNb; t += 256) {
-        const int i = t / kNb, j = t % kNb;
-        if (i < prow && j < w) {
-            double s = 0.0;
-            for (int k = 0; k <= j; ++k) s += x[i * kNbLd + k] * e[j * kNbLd + k];
-            Kb[(size_t)(r0 + i) * n + q + j] = s;
+    // X[i][j] = sum_k A[i][k] L^-1[j][k]: row tiles (wave >> 1) + 2u of the 4
+    if (prow > 0) {
+        double lb[kNb / 4];
+#pragma unroll
+        for (int s = 0; s < kNb / 4; ++s) lb[s] = e[(16 * ct + l16) * kNbLd + 4 * s + lk];
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            const int rt = (wave >> 1) + 2 * u;
+            f64x4 c = f64x4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int s = 0; s < kNb / 4; ++s)
+                c = mfma64(x[(16 * rt + l16) * kNbLd + 4 * s + lk], lb[s], c);
+            const int j = 16 * ct + l16;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int i = 16 * rt + 4 * r + lk;
+                if (i < prow && j < w) Kb[(size_t)(r0 + i) * n + q + j] = c[r];
+            }
         }
     }
 }
 
 // Update tile t of the region i >= j >= u0 from the panel at columns p.
-__device__ void chol_update(double* __restrict__ Kb, int n, int p, int u0, int t, double* sm) {
-    double* li = sm;                     // [64][kNbLd]
-    double* lj = sm + 64 * kNbLd;        // [64][kNbLd]
+__device__ void chol_update(double* __restrict__ Kb, int n, int p, int u0, int t) {
     // t -> (bi, bj), bi >= bj, row by row: row bi holds bi + 1 tiles
     int bi = 0;
     while (t > bi) {
@@ -429,58 +483,73 @@ __device__ void chol_update(double* __restrict__ Kb, int n, int p, int u0, int t
         ++bi;
     }
     const int bj = t;
-    const int i0 = u0 + 64 * bi, j0 = u0 + 64 * bj;
-    for (int e = threadIdx.x; e < 64 * kNb; e += 256) {
-        const int rr = e / kNb, k = e % kNb;
-        li[rr * kNbLd + k] = i0 + rr < n ? Kb[(size_t)(i0 + rr) * n + p + k] : 0.0;
-        lj[rr * kNbLd + k] = j0 + rr < n ? Kb[(size_t)(j0 + rr) * n + p + k] : 0.0;
-    }
-    __syncthreads();
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int l16 = lane & 15, lk = lane >> 4;
-    const int wi = (wave & 1) * 32, wj = (wave >> 1) * 32;
-    f64x4 acc[2][2];
-#pragma unroll
-    for (int a = 0; a < 2; ++a)
-#pragma unroll
-        for (int b = 0; b < 2; ++b) acc[a][b] = f64x4{0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-    for (int q = 0; q < kNb / 4; ++q) {
-        const int k = 4 * q + lk;
-        double a[2], b[2];
-#pragma unroll
-        for (int s = 0; s < 2; ++s) {
-            a[s] = li[(wi + 16 * s + l16) * kNbLd + k];
-            b[s] = lj[(wj + 16 * s + l16) * kNbLd + k];
-        }
-#pragma unroll
-        for (int ta = 0; ta < 2; ++ta)
-#pragma unroll
-            for (int tb = 0; tb < 2; ++tb) acc[ta][tb] = mfma64(a[ta], b[tb], acc[ta][tb]);
-    }
+    const int i0 = u0 + 64 * bi + (wave & 1) * 32, j0 = u0 + 64 * bj + (wave >> 1) * 32;
+    // this lane's 16 entries of the tile and its operands, all loads at once
+    double kv[2][2][4];
 #pragma unroll
     for (int ta = 0; ta < 2; ++ta)
 #pragma unroll
         for (int tb = 0; tb < 2; ++tb)
 #pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                const int i = i0 + wi + 16 * ta + 4 * e + lk;
-                const int j = j0 + wj + 16 * tb + l16;
-                if (i < n && j <= i) Kb[(size_t)i * n + j] -= acc[ta][tb][e];
+            for (int r = 0; r < 4; ++r) {
+                const int i = i0 + 16 * ta + 4 * r + lk, j = j0 + 16 * tb + l16;
+                const bool in = i < n && j <= i;
+                const double v = *(in ? Kb + (size_t)i * n + j : Kb);
+                kv[ta][tb][r] = in ? v : 0.0;
+            }
+    double av[2][kNb / 4], bv[2][kNb / 4];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int ra = i0 + 16 * h + l16, rb = j0 + 16 * h + l16;
+        const double* ap = Kb + (size_t)(ra < n ? ra : 0) * n + p + lk;
+        const double* bp = Kb + (size_t)(rb < n ? rb : 0) * n + p + lk;
+#pragma unroll
+        for (int s = 0; s < kNb / 4; ++s) {
+            const double va = ap[4 * s], vb = bp[4 * s];
+            av[h][s] = ra < n ? va : 0.0;
+            bv[h][s] = rb < n ? vb : 0.0;
+        }
+    }
+    f64x4 acc[2][2];
+#pragma unroll
+    for (int ta = 0; ta < 2; ++ta)
+#pragma unroll
+        for (int tb = 0; tb < 2; ++tb) acc[ta][tb] = f64x4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int s = 0; s < kNb / 4; ++s)
+#pragma unroll
+        for (int ta = 0; ta < 2; ++ta)
+#pragma unroll
+            for (int tb = 0; tb < 2; ++tb) acc[ta][tb] = mfma64(av[ta][s], bv[tb][s], acc[ta][tb]);
+#pragma unroll
+    for (int ta = 0; ta < 2; ++ta)
+#pragma unroll
+        for (int tb = 0; tb < 2; ++tb)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int i = i0 + 16 * ta + 4 * r + lk, j = j0 + 16 * tb + l16;
+                if (i < n && j <= i) Kb[(size_t)i * n + j] = kv[ta][tb][r] - acc[ta][tb][r];
             }
 }
 
+// grid = nd (pw + ut): the panel workgroups of every matrix first (they are
+// dispatched first, and the next step waits on them), then the update tiles.
 __global__ __launch_bounds__(256) void gn_chol_step_kernel(double* __restrict__ K, int n, int q,
-                                                           int pw, double* __restrict__ Dinv,
+                                                           int pw, int nd,
+                                                           double* __restrict__ Dinv,
                                                            int* __restrict__ flags) {
-    __shared__ double sm[kStepLds + kNb];
-    const int b = blockIdx.y;
-    double* Kb = K + (size_t)b * n * n;
-    if ((int)blockIdx.x < pw) {
+    __shared__ double sm[kStepLds + 2 * kNb];
+    const int x = blockIdx.x;
+    if (x < pw * nd) {
+        const int b = x / pw;
         double* Db = Dinv + ((size_t)b * ((n + kNb - 1) / kNb) + q / kNb) * kNb * kNb;
-        chol_panel(Kb, n, q, blockIdx.x, Db, flags + b, sm);
+        chol_panel(K + (size_t)b * n * n, n, q, x - b * pw, Db, flags + b, sm);
     } else {
-        chol_update(Kb, n, q - kNb, q + kNb, blockIdx.x - pw, sm);
+        const int ut = (gridDim.x - pw * nd) / nd;
+        const int b = (x - pw * nd) / ut;
+        chol_update(K + (size_t)b * n * n, n, q - kNb, q + kNb, x - pw * nd - b * ut);
     }
 }
 
@@ -705,8 +774,11 @@ int cholesky_batched(double* K, double* Dinv, int n, int nd, int* flags, hipStre
             const int nt = (n - q - kNb + 63) / 64;
             ut = nt * (nt + 1) / 2;
         }
-        hipLaunchKernelGGL(gn_chol_step_kernel, dim3(pw + ut, nd), dim3(256), 0, stream, K, n, q,
-                           pw, Dinv, flags);
+        const int64_t blocks = (int64_t)(pw + ut) * nd;
+        if (blocks >= ((int64_t)1 << 31))
+            return fail(ZTM_EINVAL, "Cholesky: %d matrices of %d need too many workgroups", nd, n);
+        hipLaunchKernelGGL(gn_chol_step_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, K, n,
+                           q, pw, nd, Dinv, flags);
         ZTM_HIP(hipGetLastError());
     }
     return ZTM_OK;
