@@ -486,8 +486,10 @@ __device__ void chol_update(double* __restrict__ Kb, int n, int p, int u0, int t
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int l16 = lane & 15, lk = lane >> 4;
     const int i0 = u0 + 64 * bi + (wave & 1) * 32, j0 = u0 + 64 * bj + (wave >> 1) * 32;
-    // this lane's 16 entries of the tile and its operands, all loads at once
-    double kv[2][2][4];
+    // this lane's 16 entries of the tile start the accumulators, and the
+    // products come in negated (an exact sign flip of one operand), so the
+    // MFMAs leave K[i][j] - sum_k directly; all loads are issued at once
+    f64x4 acc[2][2];
 #pragma unroll
     for (int ta = 0; ta < 2; ++ta)
 #pragma unroll
@@ -497,7 +499,7 @@ __device__ void chol_update(double* __restrict__ Kb, int n, int p, int u0, int t
                 const int i = i0 + 16 * ta + 4 * r + lk, j = j0 + 16 * tb + l16;
                 const bool in = i < n && j <= i;
                 const double v = *(in ? Kb + (size_t)i * n + j : Kb);
-                kv[ta][tb][r] = in ? v : 0.0;
+                acc[ta][tb][r] = in ? v : 0.0;
             }
     double av[2][kNb / 4], bv[2][kNb / 4];
 #pragma unroll
@@ -509,14 +511,9 @@ __device__ void chol_update(double* __restrict__ Kb, int n, int p, int u0, int t
         for (int s = 0; s < kNb / 4; ++s) {
             const double va = ap[4 * s], vb = bp[4 * s];
             av[h][s] = ra < n ? va : 0.0;
-            bv[h][s] = rb < n ? vb : 0.0;
+            bv[h][s] = rb < n ? -vb : 0.0;
         }
     }
-    f64x4 acc[2][2];
-#pragma unroll
-    for (int ta = 0; ta < 2; ++ta)
-#pragma unroll
-        for (int tb = 0; tb < 2; ++tb) acc[ta][tb] = f64x4{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
     for (int s = 0; s < kNb / 4; ++s)
 #pragma unroll
@@ -530,7 +527,7 @@ __device__ void chol_update(double* __restrict__ Kb, int n, int p, int u0, int t
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int i = i0 + 16 * ta + 4 * r + lk, j = j0 + 16 * tb + l16;
-                if (i < n && j <= i) Kb[(size_t)i * n + j] = kv[ta][tb][r] - acc[ta][tb][r];
+                if (i < n && j <= i) Kb[(size_t)i * n + j] = acc[ta][tb][r];
             }
 }
 
