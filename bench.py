@@ -94,7 +94,7 @@ def main() -> int:
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the stored-matrix and prism legs")
     ap.add_argument("--leg", choices=["prism", "batched", "uneven", "tfm", "accumulation", "gn",
-                                      "mfma"],
+                                      "gn_lib", "mfma"],
                     default=None,
                     help="run only this extra leg and print its JSON object (for profiling)")
     ap.add_argument("--copy-calib-mib", type=int, default=0,
@@ -122,9 +122,10 @@ def main() -> int:
         torch.cuda.synchronize()
         print(json.dumps({"copy_calibration_bytes": 4 * n, "copies": 5}))
         return 0
-    if args.leg in ("batched", "uneven", "tfm", "accumulation", "gn", "mfma"):
+    if args.leg in ("batched", "uneven", "tfm", "accumulation", "gn", "gn_lib", "mfma"):
         leg = {"batched": batched_leg, "uneven": uneven_leg, "tfm": tfm_leg,
-               "accumulation": accumulation_leg, "gn": gn_leg, "mfma": mfma_leg}[args.leg]
+               "accumulation": accumulation_leg, "gn": gn_leg, "gn_lib": gn_lib_leg,
+               "mfma": mfma_leg}[args.leg]
         print(json.dumps({args.leg: leg(torch.cuda.current_stream(), args)}))
         return 0
     w = W.pm_blobs_1m()
@@ -806,6 +807,35 @@ def mfma_leg(stream, args):
             out[f"{op}_{'low_entropy' if low else 'random'}"] = {
                 "tflops": v, "peak": peak, "frac": v / peak}
     return out
+
+
+def gn_lib_leg(stream, args, reps=10):
+    """The vendor libraries on the gn_step shapes, through torch (rocBLAS /
+    hipBLASLt for the product, rocSOLVER for the factorisation): J^T J as a
+    full FP64 product and the Cholesky factors of the 12 damped systems. A
+    comparison point for the gn leg, not part of it."""
+    _progress("gn library leg")
+    import torch
+    from ztm_amd import gfm, workloads as W
+    dev = torch.device("cuda:0")
+    w = W.gn_step()
+    rows, n = w.J.shape
+    t64 = lambda a: torch.from_numpy(np.ascontiguousarray(a, np.float64)).to(dev)
+    J, r, sig = t64(w.J), t64(w.r), t64(w.sig)
+    Jt = J.T.contiguous()
+    JJ, _ = gfm.gn_normal(J, r, stream=stream)
+    dJ = torch.diagonal(JJ)
+    K = torch.stack([JJ + torch.diag(d0 * dJ + d1 / sig ** 2) for d0, d1 in w.dm.tolist()])
+    sync = torch.cuda.synchronize
+    t_mm = _timed(lambda: torch.mm(Jt, J), reps, sync)
+    t_chol = _timed(lambda: torch.linalg.cholesky(K), reps, sync)
+    return {
+        "workload": f"gn_step: J {rows:,} x {n:,}, {K.shape[0]} damped systems",
+        "torch_mm_JtJ_ms": t_mm * 1e3,
+        "torch_mm_JtJ_tflops": 2.0 * rows * n * n / t_mm / 1e12,
+        "torch_linalg_cholesky_ms": t_chol * 1e3,
+        "kind": "torch.mm (FP64, full product) and torch.linalg.cholesky (batched FP64)",
+    }
 
 
 def gn_leg(stream, args, reps=10, cpu_rows=2048):
