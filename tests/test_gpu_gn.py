@@ -146,3 +146,25 @@ def test_damped_steps_many_conditions(dev, orc):
                             _t(b, dev)).cpu().numpy()
     for k in range(nd):
         assert np.max(np.abs(B[k] - want[k])) <= 1e-9 * np.max(np.abs(want[k] - b)), k
+
+
+def test_past_1024_parameters_vs_oracle(dev, orc):
+    """n = 1,100 (34 full 32-column blocks and a 12-column one): the
+    substitutions' columns past the first 1,024 (read from K, not prefetched)
+    and the update tiles of a ragged last block, against the oracle's
+    Gauss-Jordan step and inverse diagonal."""
+    n, rows = 1100, 2400
+    print(f"[gn steps] n={n} rows={rows}", flush=True)
+    rng = np.random.default_rng(1100)
+    J = jacobian(1101, rows, n)
+    r = rng.normal(size=rows)
+    sig = rng.uniform(0.05, 4.0, n)
+    b = rng.normal(size=n)
+    dm = np.array([[0.3, 1e4]])
+    JJ, jtr = orc.mtm(J), orc.jtr(J, r)
+    want = orc.gn_steps(JJ, jtr, sig, dm, b)
+    B = gfm.gn_damped_steps(_t(JJ, dev), _t(jtr, dev), _t(sig, dev), _t(dm, dev),
+                            _t(b, dev)).cpu().numpy()
+    assert np.max(np.abs(B[0] - want[0])) <= 1e-9 * np.max(np.abs(want[0] - b))
+    got = gfm.gn_inverse_diag(_t(JJ, dev), 1.0).cpu().numpy()
+    assert relerr(got, orc.inv_diag(JJ, 1.0)) < 1e-9
