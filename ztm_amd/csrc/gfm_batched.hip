@@ -165,12 +165,14 @@ static_assert(kStationPad % 16 == 0, "station padding must hold whole chunks");
 
 // ---------------------------------------------------------------- packing
 // A block packs 32 models x 64 cells (4 chunks): it reads the tile with
-// 256-byte row reads into LDS, then each thread converts the 8 cells of one
-// (chunk, half, model) slot and writes them as one 16-byte store per part,
-// contiguous across the models of a slot. grid = (ceil(n_chunks / 4), Bp / 32).
-// Element v (times mul, and times the model's half_scale(amax[model]) when
-// amax is given) is split as h_1 = H(v), h_2 = H(v - h_1), ...; entries
-// outside the model or cell range, and |v| <= thr, are 0.
+// 256-byte row reads into LDS (16-byte loads when vec4: n_cells a multiple of
+// 4 and P 16-byte aligned, so no load crosses a row's end), then each thread
+// converts the 8 cells of one (chunk, half, model) slot and writes them as one
+// 16-byte store per part, contiguous across the models of a slot.
+// grid = (ceil(n_chunks / 4), Bp / 32). Element v (times mul, and times the
+// model's half_scale(amax[model]) when amax is given, formed once per model
+// and block) is split as h_1 = H(v), h_2 = H(v - h_1), ...; entries outside
+// the model or cell range, and |v| <= thr, are 0.
 constexpr int kPackCells = 64;
 
 template <int SPLIT, typename H>
@@ -179,25 +181,50 @@ __global__ __launch_bounds__(256) void batched_pack_kernel(const float* __restri
                                                            int64_t n_chunks, int64_t n_packed,
                                                            float thr, float mul,
                                                            const unsigned* __restrict__ amax,
-                                                           H* __restrict__ parts) {
+                                                           int vec4, H* __restrict__ parts) {
     using v8 = typename Half<H>::v8;
     __shared__ float tile[32][kPackCells + 1];
+    __shared__ float fm[32];   // each model's factor
     const int t = threadIdx.x;
     const int64_t c0 = (int64_t)blockIdx.x * kPackCells;
     const int m0 = blockIdx.y * 32;
+    if (t < 32) fm[t] = amax && m0 + t < B ? mul * half_scale(amax[m0 + t]) : mul;
+    __syncthreads();
+    if (vec4) {
 #pragma unroll
-    for (int i = 0; i < 8; ++i) {
-        const int idx = i * 256 + t;
-        const int row = idx >> 6, col = idx & 63;
-        const int model = m0 + row;
-        const int64_t cell = c0 + col;
-        float v = 0.f;
-        if (model < B && cell < n_cells) {
-            v = P[(size_t)model * n_cells + cell];
-            const float f = amax ? mul * half_scale(amax[model]) : mul;
-            v = fabsf(v) > thr ? v * f : 0.f;
+        for (int i = 0; i < 2; ++i) {
+            const int idx = i * 256 + t;
+            const int row = idx >> 4, col = 4 * (idx & 15);
+            const int model = m0 + row;
+            const int64_t cell = c0 + col;
+            float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (model < B && cell < n_cells) {
+                v = *reinterpret_cast<const float4*>(P + (size_t)model * n_cells + cell);
+                const float f = fm[row];
+                v.x = fabsf(v.x) > thr ? v.x * f : 0.f;
+                v.y = fabsf(v.y) > thr ? v.y * f : 0.f;
+                v.z = fabsf(v.z) > thr ? v.z * f : 0.f;
+                v.w = fabsf(v.w) > thr ? v.w * f : 0.f;
+            }
+            tile[row][col] = v.x;
+            tile[row][col + 1] = v.y;
+            tile[row][col + 2] = v.z;
+            tile[row][col + 3] = v.w;
         }
-        tile[row][col] = v;
+    } else {
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const int idx = i * 256 + t;
+            const int row = idx >> 6, col = idx & 63;
+            const int model = m0 + row;
+            const int64_t cell = c0 + col;
+            float v = 0.f;
+            if (model < B && cell < n_cells) {
+                v = P[(size_t)model * n_cells + cell];
+                v = fabsf(v) > thr ? v * fm[row] : 0.f;
+            }
+            tile[row][col] = v;
+        }
     }
     __syncthreads();
     const int slot = t >> 5;          // 8 slots: chunk (c0/16 + slot/2), half slot%2
@@ -1128,6 +1155,7 @@ int batched_fwd_launch(const PmShape& s, int B, int ks, int split, const float* 
     float* part = (float*)(parts + np * n_packed);
     unsigned* amax = nullptr;
     const dim3 pgrid((unsigned)((s.n_cells + kPackCells - 1) / kPackCells), (unsigned)(Bp / 32));
+    const int pvec4 = s.n_cells % 4 == 0 && (uintptr_t)P % 16 == 0;
     if (split == 4) {
         amax = (unsigned*)(part + (size_t)ks * s.m_pad * Bp);
         hipLaunchKernelGGL(batched_clear_kernel, dim3(1), dim3(256), 0, stream, amax, Bp);
@@ -1136,13 +1164,13 @@ int batched_fwd_launch(const PmShape& s, int B, int ks, int split, const float* 
                            0, stream, P, s.n_cells, (float)thr, amax);
         hipLaunchKernelGGL((batched_pack_kernel<2, _Float16>), pgrid, dim3(256), 0, stream, P, B,
                            s.n_cells, Bp, n_chunks, n_packed, (float)thr, 1.f,
-                           (const unsigned*)amax, (_Float16*)parts);
+                           (const unsigned*)amax, pvec4, (_Float16*)parts);
     } else if (split == 3) {
         hipLaunchKernelGGL((batched_pack_kernel<3, __bf16>), pgrid, dim3(256), 0, stream, P, B,
-                           s.n_cells, Bp, n_chunks, n_packed, (float)thr, 1.f, nullptr, parts);
+                           s.n_cells, Bp, n_chunks, n_packed, (float)thr, 1.f, nullptr, pvec4, parts);
     } else {
         hipLaunchKernelGGL((batched_pack_kernel<2, __bf16>), pgrid, dim3(256), 0, stream, P, B,
-                           s.n_cells, Bp, n_chunks, n_packed, (float)thr, 1.f, nullptr, parts);
+                           s.n_cells, Bp, n_chunks, n_packed, (float)thr, 1.f, nullptr, pvec4, parts);
     }
     ZTM_HIP(hipGetLastError());
     const int tile = 32 * waves;   // stations per workgroup (m_pad is a multiple of 512)
@@ -1269,6 +1297,7 @@ int batched_adj_launch(const PmShape& s, int B, int ks, int split, const float* 
     unsigned* amax = nullptr;
     float gs = 1.f;
     const dim3 pgrid((unsigned)((s.m + kPackCells - 1) / kPackCells), (unsigned)(Bp / 32));
+    const int pvec4 = s.m % 4 == 0 && (uintptr_t)R % 16 == 0;
     if (split == 4) {
         amax = (unsigned*)(part + (size_t)ks * n_pad * Bp);
         gs = half_g_scale(gmax);
@@ -1278,13 +1307,13 @@ int batched_adj_launch(const PmShape& s, int B, int ks, int split, const float* 
                            0, stream, R, (int64_t)s.m, 0.f, amax);
         hipLaunchKernelGGL((batched_pack_kernel<2, _Float16>), pgrid, dim3(256), 0, stream, R, B,
                            (int64_t)s.m, Bp, n_chunks, n_packed, 0.f, 1.f,
-                           (const unsigned*)amax, (_Float16*)parts);
+                           (const unsigned*)amax, pvec4, (_Float16*)parts);
     } else if (split == 3) {
         hipLaunchKernelGGL((batched_pack_kernel<3, __bf16>), pgrid, dim3(256), 0, stream, R, B,
-                           (int64_t)s.m, Bp, n_chunks, n_packed, 0.f, mul, nullptr, parts);
+                           (int64_t)s.m, Bp, n_chunks, n_packed, 0.f, mul, nullptr, pvec4, parts);
     } else {
         hipLaunchKernelGGL((batched_pack_kernel<2, __bf16>), pgrid, dim3(256), 0, stream, R, B,
-                           (int64_t)s.m, Bp, n_chunks, n_packed, 0.f, mul, nullptr, parts);
+                           (int64_t)s.m, Bp, n_chunks, n_packed, 0.f, mul, nullptr, pvec4, parts);
     }
     ZTM_HIP(hipGetLastError());
     const dim3 grid((unsigned)tiles, (unsigned)ks);
