@@ -51,7 +51,7 @@ def test_normal_equations_no_rows(dev):
     assert not torch.any(JJ).item() and not torch.any(jtr).item()
 
 
-@pytest.mark.parametrize("n", [1, 31, 32, 100, 200, 257])
+@pytest.mark.parametrize("n", [1, 31, 32, 33, 64, 96, 100, 200, 257])
 def test_damped_steps_vs_oracle(dev, orc, n):
     print(f"[gn steps] n={n}", flush=True)
     rng = np.random.default_rng(n)
