@@ -168,3 +168,33 @@ def test_past_1024_parameters_vs_oracle(dev, orc):
     assert np.max(np.abs(B[0] - want[0])) <= 1e-9 * np.max(np.abs(want[0] - b))
     got = gfm.gn_inverse_diag(_t(JJ, dev), 1.0).cpu().numpy()
     assert relerr(got, orc.inv_diag(JJ, 1.0)) < 1e-9
+
+
+def test_random_shapes_vs_oracle(dev, orc):
+    """A seeded sweep of 8 shapes (n in [1, 300], data in [n, 3n], 1 to 20
+    damping conditions): the normal equations, every step and the inverse
+    diagonal against the oracle."""
+    rng = np.random.default_rng(2024)
+    for case in range(8):
+        n = int(rng.integers(1, 301))
+        rows = int(rng.integers(n, 3 * n + 1))
+        nd = int(rng.integers(1, 21))
+        print(f"[gn sweep] case {case}: n={n} rows={rows} nd={nd}", flush=True)
+        J = jacobian(case, rows, n)
+        r = rng.normal(size=rows)
+        sig = rng.uniform(0.05, 4.0, n)
+        b = rng.normal(size=n)
+        dm = np.stack([rng.uniform(0.0, 100.0, nd),
+                       np.exp(rng.uniform(np.log(1e3), np.log(1e5), nd))], axis=1)
+        JJd, jtrd = gfm.gn_normal(_t(J, dev), _t(r, dev))
+        JJ, jtr = orc.mtm(J), orc.jtr(J, r)
+        assert relerr(JJd.cpu().numpy(), JJ) < 1e-13, case
+        assert relerr(jtrd.cpu().numpy(), jtr) < 1e-13, case
+        want = orc.gn_steps(JJ, jtr, sig, dm, b)
+        B = gfm.gn_damped_steps(_t(JJ, dev), _t(jtr, dev), _t(sig, dev), _t(dm, dev),
+                                _t(b, dev)).cpu().numpy()
+        for k in range(nd):
+            assert np.max(np.abs(B[k] - want[k])) <= 1e-9 * np.max(np.abs(want[k] - b)), (case, k)
+        d = float(rng.uniform(0.01, 10.0))
+        got = gfm.gn_inverse_diag(_t(JJ, dev), d).cpu().numpy()
+        assert relerr(got, orc.inv_diag(JJ, d)) < 1e-9, case
