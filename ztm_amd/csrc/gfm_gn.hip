@@ -215,17 +215,31 @@ __global__ __launch_bounds__(256) void gn_syrk_reduce_kernel(const double* __res
     int bi, bj;
     upper_tile(blockIdx.x, nb, bi, bj);
     const int tx = tid & 31, ty = tid >> 5;
+    // the thread's 4 entries summed side by side, slice by slice in order,
+    // so that many loads are in flight (entries outside the matrix read
+    // entry 0 and are discarded)
+    size_t off[4];
+    bool in[4];
+    double s[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int i = bi * kRedTile + ty + 8 * r, j = bj * kRedTile + tx;
+        in[r] = i < n && j < n;
+        off[r] = !in[r] ? 0 : i <= j ? (size_t)i * n + j : (size_t)j * n + i;
+        s[r] = 0.0;
+    }
+#pragma unroll 8
+    for (int k = 0; k < ks; ++k) {
+        const double* pk = part + k * nn;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) s[r] += pk[off[r]];
+    }
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
         const int li = ty + 8 * r;
         const int i = bi * kRedTile + li, j = bj * kRedTile + tx;
-        double s = 0.0;
-        if (i < n && j < n) {
-            const size_t off = i <= j ? (size_t)i * n + j : (size_t)j * n + i;
-            for (int k = 0; k < ks; ++k) s += part[k * nn + off];
-            JJ[(size_t)i * n + j] = s;
-        }
-        t[li][tx] = s;
+        if (in[r]) JJ[(size_t)i * n + j] = s[r];
+        t[li][tx] = in[r] ? s[r] : 0.0;
     }
     if (bi == bj) return;
     __syncthreads();
