@@ -105,19 +105,29 @@ __device__ __forceinline__ void pm_fwd_block(
     const int nr_pad = (nr + G - 1) / G * G;   // whole row groups; extra rows are 0
     const size_t base = (size_t)row0 * (size_t)nx;
 
-    // Stage this slice's densities, thresholded and zero-padded to nxp.
+    // Stage this slice's densities, thresholded and zero-padded to nxp. A
+    // thread's element t = r nxp + i steps by 256 each pass; (r, i) follow it
+    // without a division per element.
     const int tile = nr_pad * nxp;   // one field's tile
-    for (int t = threadIdx.x; t < nr_pad * nxp; t += blockDim.x) {
-        const int r = t / nxp;
-        const int i = t - r * nxp;
+    {
+        const int dr = 256 / nxp, di = 256 - dr * nxp;
+        int r = threadIdx.x / nxp, i = threadIdx.x - r * nxp;
+        for (int t = threadIdx.x; t < tile; t += 256) {
 #pragma unroll
-        for (int f = 0; f < F; ++f) {
-            T v = T(0);
-            if (i < nx && r < nr) {
-                v = rho[(size_t)f * field_stride + base + (size_t)r * nx + i];
-                if (!(fabs(v) > thr)) v = T(0);   // NaN or |rho| <= thr: skipped
+            for (int f = 0; f < F; ++f) {
+                T v = T(0);
+                if (i < nx && r < nr) {
+                    v = rho[(size_t)f * field_stride + base + (size_t)r * nx + i];
+                    if (!(fabs(v) > thr)) v = T(0);   // NaN or |rho| <= thr: skipped
+                }
+                lrho[f * tile + t] = v;
             }
-            lrho[f * tile + t] = v;
+            r += dr;
+            i += di;
+            if (i >= nxp) {
+                i -= nxp;
+                ++r;
+            }
         }
     }
     __syncthreads();
