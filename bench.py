@@ -94,7 +94,7 @@ def main() -> int:
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the stored-matrix and prism legs")
     ap.add_argument("--leg", choices=["prism", "batched", "uneven", "tfm", "accumulation", "gn",
-                                      "gn_lib", "mfma"],
+                                      "gn_lib", "lm", "mfma"],
                     default=None,
                     help="run only this extra leg and print its JSON object (for profiling)")
     ap.add_argument("--copy-calib-mib", type=int, default=0,
@@ -122,10 +122,10 @@ def main() -> int:
         torch.cuda.synchronize()
         print(json.dumps({"copy_calibration_bytes": 4 * n, "copies": 5}))
         return 0
-    if args.leg in ("batched", "uneven", "tfm", "accumulation", "gn", "gn_lib", "mfma"):
+    if args.leg in ("batched", "uneven", "tfm", "accumulation", "gn", "gn_lib", "lm", "mfma"):
         leg = {"batched": batched_leg, "uneven": uneven_leg, "tfm": tfm_leg,
                "accumulation": accumulation_leg, "gn": gn_leg, "gn_lib": gn_lib_leg,
-               "mfma": mfma_leg}[args.leg]
+               "lm": lm_leg, "mfma": mfma_leg}[args.leg]
         print(json.dumps({args.leg: leg(torch.cuda.current_stream(), args)}))
         return 0
     w = W.pm_blobs_1m()
@@ -807,6 +807,54 @@ def mfma_leg(stream, args):
             out[f"{op}_{'low_entropy' if low else 'random'}"] = {
                 "tflops": v, "peak": peak, "frac": v / peak}
     return out
+
+
+def lm_leg(stream, args, reps=10):
+    """One Levenberg-Marquardt iteration of ZTM's gravity fit on the device
+    (ztm_amd.lm.GravityLM) at the reference grid (tfm_fm) with 9 parameters
+    (vm, ro, dr and 6 Moho basis coefficients) and the reference's 12
+    damping conditions: the forward model, the exact Jacobian, the normal
+    equations, the damped steps and the 12 candidates' forward models, with
+    inputs resident. Against the reference's CPU iteration restated: its
+    forward models, 2 + n + nd = 23 of them (LMA::fit's residual, Jac's one
+    per parameter, one per damping condition; TFM.cpp:124-135,157-161),
+    each timed as the oracle's producer + TFM.cpp:95-99 gravity on 16
+    threads (the dense LA of a 9-parameter step is negligible beside them)."""
+    _progress("lm leg")
+    import torch
+    import oracle
+    from ztm_amd import lm, workloads as W
+    dev = torch.device("cuda:0")
+    w = W.tfm_fm()
+    basis = W.tfm_moho_basis(w)
+    model = lm.GravityLM(w, basis, dev)
+    b0 = np.array([w.vm, w.ro, w.dr] + [0.0] * basis.shape[0])
+    d = model.forward(b0 + W.TFM_LM_STEP, stream)[0]
+    sync = torch.cuda.synchronize
+    out = {}
+
+    def it():
+        out["r"] = model.iteration(b0, d, W.TFM_LM_SIGMA, W.GN_DAMPING, stream)
+
+    t_dev = _timed(it, reps, sync)
+    _, misfits, best, misfit0 = out["r"]
+    n, nd = model.n, W.GN_DAMPING.shape[0]
+    c0 = time.perf_counter()
+    drho, nn = oracle.tfm_drho(w.di, w.dj, w.dk, w.fh, w.fv, w.rc, w.sea, w.moho, w.vel,
+                               w.vm, w.ro, w.dr)
+    oracle.tfm_gravity(w.bfr, w.di, w.dj, w.dk, w.fh, w.fv, drho, w.surf, nn, threads=16)
+    t_fm = time.perf_counter() - c0
+    fms = 2 + n + nd
+    return {
+        "workload": f"tfm_fm (61x61x33, fh=1, fv=2), {n} parameters, {nd} damping conditions",
+        "ms_per_iteration": t_dev * 1e3,
+        "misfit": {"start": float(misfit0), "best": float(misfits[best]), "condition": best},
+        "cpu_baseline": {"ms_per_iteration": t_fm * fms * 1e3, "forward_models": fms,
+                         "ms_per_forward_model": t_fm * 1e3, "cores": 16, "kind": "port",
+                         "sample": "one oracle forward model (producer + TFM.cpp:95-99 gravity) "
+                                   f"timed, times the iteration's {fms}"},
+        "speedup_vs_cpu": t_fm * fms / t_dev,
+    }
 
 
 def gn_lib_leg(stream, args, reps=10):

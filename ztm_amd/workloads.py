@@ -377,6 +377,23 @@ WORKLOADS = {
 }
 
 
+# A Moho basis for the gravity-fit iteration on tfm_fm: six Gaussian bumps
+# of the grid (k units, sigma 6 columns) centred in the interior; the step
+# from the start (vm, ro, dr, 0..0) to the synthetic truth; the parameters'
+# a-priori sigmas (LMA::fit's A = diag(1/sigma^2), TFM.cpp:160).
+TFM_MOHO_CENTRES = [(20, 20), (30, 20), (40, 20), (20, 40), (30, 40), (40, 40)]
+TFM_LM_STEP = np.array([0.05, 0.03, -0.03, 0.15, -0.10, 0.12, -0.08, 0.10, -0.12])
+TFM_LM_SIGMA = np.array([0.1, 0.1, 0.1] + [0.3] * 6)
+
+
+def tfm_moho_basis(w, centres=TFM_MOHO_CENTRES, sigma=6.0) -> np.ndarray:
+    """[len(centres), di * dj] Gaussian bumps of the Moho grid."""
+    j, i = np.meshgrid(np.arange(w.dj, dtype=np.float64), np.arange(w.di, dtype=np.float64),
+                       indexing="ij")
+    return np.stack([np.exp(-((i - ci) ** 2 + (j - cj) ** 2) / (2 * sigma ** 2)).ravel()
+                     for ci, cj in centres])
+
+
 # the reference's damping grid dmp = {{0,.3,1,3,10,100},{1e4,1e5}} as the
 # (d0, d1) pairs LMA::fit iterates over (TFM.cpp:202,209)
 GN_DAMPING = np.array([[a, b] for a in (0.0, 0.3, 1.0, 3.0, 10.0, 100.0) for b in (1e4, 1e5)])
