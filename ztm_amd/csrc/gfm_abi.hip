@@ -1267,12 +1267,20 @@ int ztm_tfm_gravity_host(int bfr, int di, int dj, int dk, int fh, int fv, const 
 }
 
 namespace {
+int tfm_fields_gravity(ztm_plan p, int bfr, int di, int dj, int fh, int fv, int F,
+                       const float* fields, double thr, const float* s_dev, double* dg_dev,
+                       hipStream_t s);
+
 // Gravity at the interior stations of B float fields drho_dev[B][N] on the
-// reference plan p (stations prepared from s_dev): the split-BF16 ensemble
-// forward with cell threshold thr, scattered to g_dev[B][di*dj] (0 outside).
+// reference plan p (stations prepared from s_dev) with cell threshold thr,
+// scattered to g_dev[B][di*dj] (0 outside): up to 4 fields through the FP32
+// multi-field forward (tfm_fields_gravity), more on the split-BF16 ensemble
+// forward, whose 32-model tiles would be mostly padding below that.
 int tfm_gravity_of_fields(ztm_plan p, int bfr, int di, int dj, int fh, int fv, int B,
                           const float* drho_dev, double thr, const float* s_dev, double* g_dev,
                           hipStream_t s) {
+    if (B <= 4 && pm_fwd_fields_supported(p->shape, B) && !near_args(p).quad)
+        return tfm_fields_gravity(p, bfr, di, dj, fh, fv, B, drho_dev, thr, s_dev, g_dev, s);
     const int M = di * dj, m = p->shape.m;
     if (m == 0) return tfm_scatter_launch(nullptr, nullptr, 0, M, B, g_dev, s);
     if (int e = prep_ref_stations(s_dev, bfr, di, dj, fh, fv, p->st_f, p->st_d, p->st_c,
@@ -1294,14 +1302,16 @@ int tfm_gravity_of_fields(ztm_plan p, int bfr, int di, int dj, int fh, int fv, i
 }  // namespace
 
 namespace {
-// Gravity at the interior stations of F already-masked derivative fields
-// fields[F][N] (no threshold): up to 4 fields through one FP32 GF forward
-// that forms each pair's r^-3 once (each field with its FP64 near-field
+// Gravity at the interior stations of F fields fields[F][N] with cell
+// threshold thr (0 for derivative fields, already masked by GF's skip at
+// the parameter point): up to 4 fields through one FP32 GF forward that
+// forms each pair's r^-3 once (each field with its FP64 near-field
 // correction; one forward per field when F density tiles of a row do not
 // fit LDS), more on the split MFMA ensemble forward; dg [F][di*dj].
 int tfm_fields_gravity(ztm_plan p, int bfr, int di, int dj, int fh, int fv, int F,
-                       const float* fields, const float* s_dev, double* dg_dev, hipStream_t s) {
-    if (F > 4) return tfm_gravity_of_fields(p, bfr, di, dj, fh, fv, F, fields, 0.0, s_dev, dg_dev,
+                       const float* fields, double thr, const float* s_dev, double* dg_dev,
+                       hipStream_t s) {
+    if (F > 4) return tfm_gravity_of_fields(p, bfr, di, dj, fh, fv, F, fields, thr, s_dev, dg_dev,
                                             s);
     const int M = di * dj, m = p->shape.m;
     if (m == 0) return tfm_scatter_launch(nullptr, nullptr, 0, M, F, dg_dev, s);
@@ -1311,17 +1321,17 @@ int tfm_fields_gravity(ztm_plan p, int bfr, int di, int dj, int fh, int fv, int 
     if (int rc = ensure_batched_workspace(p, sizeof(double) * F * (size_t)m)) return rc;
     double* Y = (double*)p->bws;
     NearArgs na = near_args(p);
-    na.thr = 0.f;
+    na.thr = (float)thr;
     const int64_t N = p->shape.n_cells;
     if (pm_fwd_fields_supported(p->shape, F) && !na.quad) {
         if (int rc = ensure_workspace(p, pm_fwd_fields_workspace(p->shape, F))) return rc;
         if (int e = pm_fwd_fields_launch(p->shape, F, (const float4*)p->st_f, fields, N, p->sy,
-                                         p->sz, 0.0, pm_scale(p), p->part, Y, na, s))
+                                         p->sz, thr, pm_scale(p), p->part, Y, na, s))
             return e;
     } else {
         for (int f = 0; f < F; ++f)
             if (int e = pm_fwd_launch(p->shape, p->tuning, ZTM_FP32, p->st_f, fields + f * N,
-                                      p->sy, p->sz, 0.0, pm_scale(p), p->part,
+                                      p->sy, p->sz, thr, pm_scale(p), p->part,
                                       Y + (size_t)f * m, nullptr, nullptr, na, s))
                 return e;
     }
@@ -1381,7 +1391,7 @@ int ztm_tfm_gravity_jvp(int bfr, int di, int dj, int dk, int fh, int fv, const d
         return e;
     // The fields are already masked by GF's skip at the parameter point, so
     // neither the main kernel nor the correction thresholds them again.
-    return tfm_fields_gravity(p, bfr, di, dj, fh, fv, 3, ddrho_dev, s_dev, dg_dev, s);
+    return tfm_fields_gravity(p, bfr, di, dj, fh, fv, 3, ddrho_dev, 0.0, s_dev, dg_dev, s);
 }
 
 int ztm_tfm_gravity_jvp_grids(int bfr, int di, int dj, int dk, int fh, int fv, const double* rc,
@@ -1405,7 +1415,7 @@ int ztm_tfm_gravity_jvp_grids(int bfr, int di, int dj, int dk, int fh, int fv, c
     if (int e = tfm_drho_jvp_grids_launch(di, dj, dk, fh, fv, rc, sea_dev, moho_dev, vel_dev, vm,
                                           ro, dr, n_dirs, dmoho_dev, dvel_dev, ddrho_dev, s))
         return e;
-    return tfm_fields_gravity(p, bfr, di, dj, fh, fv, n_dirs, ddrho_dev, s_dev, dg_dev, s);
+    return tfm_fields_gravity(p, bfr, di, dj, fh, fv, n_dirs, ddrho_dev, 0.0, s_dev, dg_dev, s);
 }
 
 int ztm_tfm_gravity_vjp_grids(int bfr, int di, int dj, int dk, int fh, int fv, const double* rc,
