@@ -57,3 +57,43 @@ def test_lm_iteration_on_the_device(dev, orc):
     assert tuple(J.shape) == (w.di * w.dj, 3 + basis.shape[0])
     with pytest.raises(ValueError):
         model.forward(np.zeros(4))
+
+
+def test_lm_fit_on_the_device(dev, orc):
+    """LMA::fit's loop (TFM.cpp:157-166) over the device iteration: the
+    misfit log never rises, the last logged misfit is the oracle forward
+    model's at the parameters kept, the fit lands near the truth, and from
+    the truth itself no candidate improves, so the loop stops after one
+    iteration with the parameters unchanged."""
+    w = W.tfm_fm()
+    basis = W.tfm_moho_basis(w)
+    model = lm.GravityLM(w, basis, dev)
+    b0 = np.array([w.vm, w.ro, w.dr] + [0.0] * basis.shape[0])
+    truth = b0 + W.TFM_LM_STEP
+    d = model.forward(truth)[0]
+    print("[lm] fit from the start point", flush=True)
+    b, Q, nb = model.fit(b0, d, W.TFM_LM_SIGMA, W.GN_DAMPING, iters=6)
+    print(f"[lm] misfit log {Q}, last condition {nb}", flush=True)
+    assert all(q1 <= q0 for q0, q1 in zip(Q, Q[1:])), Q
+    # the heavy damping (d1 >= 1e4 on the prior) cuts the misfit 3-10x per
+    # iteration after the first: 6 iterations take it below 1e-5 of the
+    # start, every parameter within a tenth of its distance from the start
+    assert len(Q) == 7 and Q[-1] < Q[0] * 1e-5, Q
+    assert np.max(np.abs(b - truth) / np.abs(W.TFM_LM_STEP)) < 0.1, (b, truth)
+
+    def fm_oracle(b):
+        moho = w.moho + basis.T @ b[3:]
+        drho, nn = orc.tfm_drho(w.di, w.dj, w.dk, w.fh, w.fv, w.rc, w.sea, moho, w.vel,
+                                b[0], b[1], b[2])
+        return orc.tfm_gravity(w.bfr, w.di, w.dj, w.dk, w.fh, w.fv, drho, w.surf, nn,
+                               threads=16)
+
+    d_cpu = fm_oracle(truth)
+    m_cpu = float(np.sum((d_cpu - fm_oracle(b)) ** 2))
+    assert abs(m_cpu - Q[-1]) <= 1e-2 * m_cpu + 1e-9 * Q[0], (m_cpu, Q[-1])
+    # at the truth the data are fitted exactly: nothing is accepted
+    b1, Q1, nb1 = model.fit(truth, d, W.TFM_LM_SIGMA, W.GN_DAMPING, iters=4)
+    assert len(Q1) == 2 and Q1[0] == Q1[1] and nb1 == W.GN_DAMPING.shape[0] - 1
+    assert np.array_equal(b1, truth)
+    with pytest.raises(ValueError):
+        model.fit(np.zeros(4), d, W.TFM_LM_SIGMA, W.GN_DAMPING)

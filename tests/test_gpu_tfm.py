@@ -106,6 +106,39 @@ def test_batched_variants_match_single_calls(dev, orc):
     assert relerr(g3[2], G0) < 5e-6
 
 
+def test_batched_field_counts_match_single_calls(dev, orc):
+    """5 to 16 FM variants per call, which run the multi-field forward with
+    5-8 fields per pass (two-row groups) and two passes past 8: every
+    variant's gravity equals its own one-variant call (one field per pass,
+    four-row groups) to 1e-12, since each row's FP32 sum is formed the same
+    way and only the FP64 grouping of rows into slices differs, and the
+    last variant of the largest call is within 5e-6 of the FP64 oracle."""
+    import torch
+    w = W.tfm_fm()
+    M = w.di * w.dj
+    Bmax = 16
+    params = np.array([[w.vm + 0.01 * b, w.ro - 0.004 * b, w.dr + 0.003 * b]
+                       for b in range(Bmax)])
+    bump = np.cos(np.arange(M) * 0.21)
+    moho = np.stack([w.moho + 0.05 * b * bump for b in range(Bmax)])
+    t64 = lambda a: torch.from_numpy(np.ascontiguousarray(a, np.float64)).to(dev)
+    s32 = torch.from_numpy(w.surf.astype(np.float32)).to(dev)
+    args = (w.bfr, w.di, w.dj, w.dk, w.fh, w.fv, w.rc, t64(w.sea))
+    single = [gfm.tfm_gravity_batched(*args, t64(moho[b:b + 1]), t64(w.vel),
+                                      t64(params[b:b + 1]), s32)[0].cpu().numpy()[0]
+              for b in range(Bmax)]
+    for B in (5, 6, 7, 8, 12, 16):
+        print(f"[tfm] {B} variants in one call", flush=True)
+        g, _ = gfm.tfm_gravity_batched(*args, t64(moho[:B]), t64(w.vel), t64(params[:B]), s32)
+        g = g.cpu().numpy()
+        for b in range(B):
+            assert relerr(g[b], single[b]) < 1e-12, (B, b)
+    want = gfm.tfm_gravity_host(w.bfr, w.di, w.dj, w.dk, w.fh, w.fv, w.rc, w.sea, moho[-1],
+                                w.vel, w.surf, *params[-1], want_drho=True)[1]
+    want = orc.gf(w.bfr, w.di, w.dj, w.dk, w.fh, w.fv, want, w.surf.astype(np.float32))
+    assert relerr(g[-1], want) < 5e-6
+
+
 def _t64(a, dev):
     import torch
     return torch.from_numpy(np.ascontiguousarray(a, np.float64)).to(dev)

@@ -1273,22 +1273,24 @@ int tfm_fields_gravity(ztm_plan p, int bfr, int di, int dj, int fh, int fv, int 
 
 // Gravity at the interior stations of B float fields drho_dev[B][N] on the
 // reference plan p (stations prepared from s_dev) with cell threshold thr,
-// scattered to g_dev[B][di*dj] (0 outside): up to 8 fields through the FP32
-// multi-field forward in groups of at most 4 (tfm_fields_gravity), more on
-// the split-BF16 ensemble forward. At the reference grid a group of 4 costs
-// ~0.19 ms and the ensemble call ~0.46 ms whatever B up to 32 (its 32-model
-// tile is then mostly padding).
-constexpr int kTfmFieldsMax = 8;
+// scattered to g_dev[B][di*dj] (0 outside): up to 16 fields through the
+// FP32 multi-field forward in groups of at most 8 (tfm_fields_gravity), more
+// on the split-BF16 ensemble forward. At the reference grid a group of 4
+// costs ~0.19 ms and the ensemble call ~0.46 ms whatever B up to 32 (its
+// 32-model tile is then mostly padding).
+constexpr int kTfmFieldsMax = 16;
+constexpr int kTfmFieldsGroup = 8;
 
 int tfm_gravity_of_fields(ztm_plan p, int bfr, int di, int dj, int fh, int fv, int B,
                           const float* drho_dev, double thr, const float* s_dev, double* g_dev,
                           hipStream_t s) {
-    if (B <= kTfmFieldsMax && pm_fwd_fields_supported(p->shape, std::min(B, 4)) &&
+    if (B <= kTfmFieldsMax && pm_fwd_fields_supported(p->shape, std::min(B, kTfmFieldsGroup)) &&
         !near_args(p).quad) {
         const int64_t N = p->shape.n_cells;
         const int M = di * dj;
-        for (int f0 = 0; f0 < B; f0 += 4)
-            if (int e = tfm_fields_gravity(p, bfr, di, dj, fh, fv, std::min(4, B - f0),
+        for (int f0 = 0; f0 < B; f0 += kTfmFieldsGroup)
+            if (int e = tfm_fields_gravity(p, bfr, di, dj, fh, fv,
+                                           std::min(kTfmFieldsGroup, B - f0),
                                            drho_dev + f0 * N, thr, s_dev,
                                            g_dev + (size_t)f0 * M, s))
                 return e;
@@ -1317,14 +1319,14 @@ int tfm_gravity_of_fields(ztm_plan p, int bfr, int di, int dj, int fh, int fv, i
 namespace {
 // Gravity at the interior stations of F fields fields[F][N] with cell
 // threshold thr (0 for derivative fields, already masked by GF's skip at
-// the parameter point): up to 4 fields through one FP32 GF forward that
+// the parameter point): up to 8 fields through one FP32 GF forward that
 // forms each pair's r^-3 once (each field with its FP64 near-field
 // correction; one forward per field when F density tiles of a row do not
-// fit LDS), more on the split MFMA ensemble forward; dg [F][di*dj].
+// fit LDS), more through tfm_gravity_of_fields; dg [F][di*dj].
 int tfm_fields_gravity(ztm_plan p, int bfr, int di, int dj, int fh, int fv, int F,
                        const float* fields, double thr, const float* s_dev, double* dg_dev,
                        hipStream_t s) {
-    if (F > 4) return tfm_gravity_of_fields(p, bfr, di, dj, fh, fv, F, fields, thr, s_dev, dg_dev,
+    if (F > kTfmFieldsGroup) return tfm_gravity_of_fields(p, bfr, di, dj, fh, fv, F, fields, thr, s_dev, dg_dev,
                                             s);
     const int M = di * dj, m = p->shape.m;
     if (m == 0) return tfm_scatter_launch(nullptr, nullptr, 0, M, F, dg_dev, s);

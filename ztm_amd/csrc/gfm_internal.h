@@ -312,7 +312,7 @@ int stored_adj_splits(const PmShape& s);
 int stored_adj_launch(const PmShape& s, const float* G, const float* r, double* part, float* out,
                       hipStream_t stream);
 
-// F <= 4 FP32 density fields (field f at rho + f * field_stride) through one
+// F <= 8 FP32 density fields (field f at rho + f * field_stride) through one
 // point-mass forward that forms every pair's r^-3 once (gfm_pointmass.hip);
 // g[f * m + i], each field with its FP64 near-field correction. |rho| <= thr
 // is skipped by the main pass, |rho| <= near.thr by the correction. part must

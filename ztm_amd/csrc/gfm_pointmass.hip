@@ -86,7 +86,12 @@ constexpr int kFwdGroup = QUAD ? 1
 
 // F > 1 (FP32 row groups only): F density fields, field f at rho + f *
 // field_stride, share every pair's r^-3; their partials go to
-// part[(by * F + f) * m_pad + m] and smem holds F density tiles.
+// part[(by * F + f) * m_pad + m] and smem holds F density tiles. Beyond 4
+// fields the row group is 2 rows, which bounds the registers the per-field
+// accumulators and density vectors take. Matches fields_group().
+template <int F>
+constexpr int kFieldsGroup = F > 4 ? 2 : 4;
+
 template <typename T, int S, bool QUAD, int F = 1>
 __device__ __forceinline__ void pm_fwd_block(
     const typename Vec4<T>::type* __restrict__ st, const T* __restrict__ rho,
@@ -94,9 +99,9 @@ __device__ __forceinline__ void pm_fwd_block(
     double* __restrict__ part, int m_pad, const QuadArgs& qa, int bx, int by,
     unsigned char* smem, int64_t field_stride = 0) {
     using V4 = typename Vec4<T>::type;
-    constexpr int G = kFwdGroup<T, S, QUAD>;
-    static_assert(F == 1 || (G > 1 && sizeof(T) == sizeof(float)),
+    static_assert(F == 1 || (S == 2 && !QUAD && sizeof(T) == sizeof(float)),
                   "several fields need the FP32 row-group path");
+    constexpr int G = F > 1 ? kFieldsGroup<F> : kFwdGroup<T, S, QUAD>;
     T* lrho = reinterpret_cast<T*>(smem);
 
     const int nxp = (nx + 3) & ~3;
@@ -274,8 +279,8 @@ __device__ __forceinline__ void pm_fwd_block(
     }
 }
 
-// F FP32 density fields through one pass (S = 2, row groups of 4).
-// grid = (m_pad / 128, n_slices); block = 256.
+// F FP32 density fields through one pass (S = 2, row groups of
+// kFieldsGroup<F> rows). grid = (m_pad / 128, n_slices); block = 256.
 template <int F>
 __global__ __launch_bounds__(256) void pm_fwd_fields_kernel(
     const float4* __restrict__ st, const float* __restrict__ rho, int64_t field_stride, int nx,
@@ -355,6 +360,23 @@ __global__ __launch_bounds__(256) void pm_fwd_reduce_kernel(const double* __rest
         gref[out_map[i]] += (float)v;
     else
         g[i] = v;
+}
+
+// The F-field forward's reduces in one launch: grid = (ceil(m / 4), F);
+// block = 256. Field f = blockIdx.y reads its partials at part + f * m_pad
+// (slice stride F * m_pad), its densities at rho_f + f * field_stride for the
+// near-field correction, and writes g[f * m + i]; per field the arithmetic of
+// pm_fwd_reduce_kernel.
+__global__ __launch_bounds__(256) void pm_fwd_fields_reduce_kernel(
+    const double* __restrict__ part, int n_slices, int stride, int m, int m_pad, double scale,
+    double* __restrict__ g, const float* __restrict__ rho_f, int64_t field_stride, NearArgs na) {
+    const int f = blockIdx.y;
+    const int lane = threadIdx.x & 63;
+    const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (i >= m) return;   // uniform per wave
+    const double v = pm_fwd_reduce_wave(part + (size_t)f * m_pad, n_slices, stride, i, scale,
+                                        rho_f + f * field_stride, na, lane);
+    if (lane == 0) g[(size_t)f * m + i] = v;
 }
 
 // ---------------------------------------------------------------- adjoint
@@ -1076,7 +1098,9 @@ int pm_fwd_launch(const PmShape& s, const PmTuning& t0, int precision,
 }
 
 namespace {
-constexpr int kFieldsMax = 4;
+constexpr int kFieldsMax = 8;
+// Row group of the F-field forward (kFieldsGroup on the device).
+int fields_group(int F) { return F > 4 ? 2 : fwd_group(2, sizeof(float), false); }
 // Rows per slice for the F-field forward: the single-field choice for S = 2,
 // held to F density tiles within the 32 KB LDS budget, in whole row groups.
 int fields_rows_per_slice(const PmShape& s, int F) {
@@ -1084,7 +1108,7 @@ int fields_rows_per_slice(const PmShape& s, int F) {
     t.st_per_lane = 2;
     t.rows_per_slice = 0;
     const int nxp = (s.nx + 3) & ~3;
-    const int G = fwd_group(2, sizeof(float), false);
+    const int G = fields_group(F);
     int rps = fwd_rows_per_slice(s, t, sizeof(float), false);
     rps = std::min(rps, std::max(G, 32768 / (nxp * (int)sizeof(float) * F) / G * G));
     return std::max(rps, 1);
@@ -1092,9 +1116,9 @@ int fields_rows_per_slice(const PmShape& s, int F) {
 }  // namespace
 
 bool pm_fwd_fields_supported(const PmShape& s, int F) {
+    if (F < 1 || F > kFieldsMax) return false;
     const int nxp = (s.nx + 3) & ~3;
-    const int G = fwd_group(2, sizeof(float), false);
-    return F >= 1 && F <= kFieldsMax && (size_t)nxp * sizeof(float) * G * F <= 32768;
+    return (size_t)nxp * sizeof(float) * fields_group(F) * F <= 32768;
 }
 
 size_t pm_fwd_fields_workspace(const PmShape& s, int F) {
@@ -1113,7 +1137,7 @@ int pm_fwd_fields_launch(const PmShape& s, int F, const float4* st, const float*
     const int n_slices = (int)((s.n_rows + rps - 1) / rps);
     if (n_slices > 65535) return fail(ZTM_EINVAL, "too many row slices (%d)", n_slices);
     const int nxp = (s.nx + 3) & ~3;
-    const int G = fwd_group(2, sizeof(float), false);
+    const int G = fields_group(F);
     const size_t lds = std::max((size_t)((rps + G - 1) / G * G) * nxp * sizeof(float) * F,
                                 (size_t)4 * 2 * 64 * sizeof(double) * F);
     const dim3 grid(s.m_pad / 128, n_slices);
@@ -1127,17 +1151,19 @@ int pm_fwd_fields_launch(const PmShape& s, int F, const float4* st, const float*
             case 1: ZTM_FIELDS(1); break;
             case 2: ZTM_FIELDS(2); break;
             case 3: ZTM_FIELDS(3); break;
-            default: ZTM_FIELDS(4); break;
+            case 4: ZTM_FIELDS(4); break;
+            case 5: ZTM_FIELDS(5); break;
+            case 6: ZTM_FIELDS(6); break;
+            case 7: ZTM_FIELDS(7); break;
+            default: ZTM_FIELDS(8); break;
         }
 #undef ZTM_FIELDS
         ZTM_HIP(hipGetLastError());
     }
-    for (int f = 0; f < F; ++f) {
-        hipLaunchKernelGGL(pm_fwd_reduce_kernel, dim3((s.m + 3) / 4), dim3(256), 0, stream,
-                           part + (size_t)f * s.m_pad, n_slices, F * s.m_pad, s.m, scale,
-                           g + (size_t)f * s.m, nullptr, nullptr, rho + f * field_stride, near);
-        ZTM_HIP(hipGetLastError());
-    }
+    hipLaunchKernelGGL(pm_fwd_fields_reduce_kernel, dim3((s.m + 3) / 4, F), dim3(256), 0, stream,
+                       part, n_slices, F * s.m_pad, s.m, s.m_pad, scale, g, rho, field_stride,
+                       near);
+    ZTM_HIP(hipGetLastError());
     return ZTM_OK;
 }
 

@@ -83,3 +83,27 @@ class GravityLM:
         g = self.forward(B, stream)
         misfits = torch.sum((d[None, :] - g) ** 2, dim=1)
         return B, misfits, int(torch.argmin(misfits)), torch.sum(r * r)
+
+    def fit(self, b, d, sig, dm, iters=4, g=0.999, stream=None):
+        """LMA::fit's loop (TFM.cpp:157-166): up to `iters` iterations from
+        b; each keeps the candidate of smallest misfit only if it is below
+        the current misfit (the first such condition on ties, as the
+        reference's in-order scan), logs the misfit, and the loop ends once
+        misfit / previous misfit exceeds g. Returns (b, the misfit log Q,
+        the index of the last accepted damping condition, or len(dm) - 1 if
+        none was)."""
+        b = np.asarray(b, np.float64).copy()
+        if b.shape != (self.n,):
+            raise ValueError(f"b needs {self.n} entries")
+        Q, nb = [], len(dm) - 1
+        for _ in range(iters):
+            B, misfits, best, q0 = self.iteration(b, d, sig, dm, stream)
+            if not Q:
+                Q.append(float(q0))
+            q, qt = Q[-1], float(misfits[best])
+            if qt < q:
+                q, b, nb = qt, B[best].cpu().numpy(), best
+            Q.append(q)
+            if Q[-2] == 0 or Q[-1] / Q[-2] > g:
+                break
+        return b, Q, nb
