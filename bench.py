@@ -552,11 +552,8 @@ def uneven_leg(stream, args, iters=10, solves=2):
         xh = xs
         tunings = [b.tuning(i) for i in range(len(ps))]
 
-    def colsum(v, grid):
-        return v.reshape(grid.nz, grid.ny, grid.nx).sum(axis=0).ravel()
-
-    corr = [float(np.corrcoef(colsum(xs[i], p.grid), colsum(p.rho, p.grid))[0, 1])
-            for i, p in enumerate(ps)]
+    rec = [W.recovery_stats(xs[i], p) for i, p in enumerate(ps)]
+    corr = [r["corr"] for r in rec]
     chi2 = [float(res[i] @ res[i] / (len(res[i]) * 0.1 ** 2)) for i in range(len(ps))]
     # problem by problem, eager: FP32 and FP64 plans, forward + adjoint of the
     # same vectors, and the FP32 batch result against FP64
@@ -594,7 +591,11 @@ def uneven_leg(stream, args, iters=10, solves=2):
                      "misfit_ratio_median": float(np.median(
                          [np.linalg.norm(res[i]) / np.linalg.norm(dn[i]) for i in range(len(ps))])),
                      "chi2_per_station_median": float(np.median(chi2)),
-                     "column_mass_corr_median": float(np.median(corr))},
+                     "column_mass_corr_median": float(np.median(corr)),
+                     "recovery": {k: {"median": float(np.median([r[k] for r in rec])),
+                                      "min": float(np.min([r[k] for r in rec])),
+                                      "max": float(np.max([r[k] for r in rec]))}
+                                  for k in ("mass", "contrast", "offset")}},
         "tiles": {"st_per_lane": sorted({t.st_per_lane for t in tunings}),
                   "cells_per_lane": sorted({t.cells_per_lane for t in tunings}),
                   "size_classes": sorted({t.size_class for t in tunings})},

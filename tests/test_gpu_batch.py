@@ -185,9 +185,14 @@ def test_batch_graph_replays_equal_eager_and_alone(dev):
 def test_batch_uneven_recovers_blocks(dev):
     """Twelve problems of the uneven workload at reduced size: 10 CG
     iterations cut the normal-equation residual by 100x, halve the data
-    misfit, and the recovered mass per column (sum over depth) correlates with
-    that of the buried blocks (gravity fixes the horizontal distribution; the
-    depth is left to the smoothness prior)."""
+    misfit, and find the buried blocks in the column masses (sums over
+    depth; gravity fixes the horizontal distribution of mass, the depth is
+    left to the smoothness prior): the recovered column mass correlates with
+    the true one, its total is 0.4-1.6 of the true excess mass (0.55-1.1
+    measured), it is at least 1.5x larger over the blocks' footprints than
+    the mean |c| outside them (1.8-5.7 measured; 2 or more in the median),
+    and its centroid lies within a quarter of the grid's width of the true
+    one (0.01-0.18 measured; under 0.12 in the median)."""
     import torch
     ps = small_set(12, cells=(2000, 80000), stations=(100, 3000), seed=W.SEED0 + 5)
     lam = np.array([p.lam for p in ps])
@@ -199,15 +204,15 @@ def test_batch_uneven_recovers_blocks(dev):
         res = b.unpack_stations((gx - d.double()).cpu().numpy())
         dn = b.unpack_stations(d.cpu().numpy())
         xs = b.unpack_cells(x.cpu().numpy())
-    def colsum(v, g):
-        return v.reshape(g.nz, g.ny, g.nx).sum(axis=0).ravel()
-
     stats = [(float(rr[-1, i] / rr[0, i]), float(np.linalg.norm(res[i]) / np.linalg.norm(dn[i])),
-              float(np.corrcoef(colsum(xs[i], p.grid), colsum(p.rho, p.grid))[0, 1]))
-             for i, p in enumerate(ps)]
-    print("rr ratio, misfit ratio, column-mass correlation:", stats)
-    for i, (q, f, c) in enumerate(stats):
-        assert q < 1e-2 and f < 0.5 and c > 0.2, (i, q, f, c)
+              W.recovery_stats(xs[i], p)) for i, p in enumerate(ps)]
+    for i, (q, f, r) in enumerate(stats):
+        print(f"[batch] problem {i}: rr ratio {q:.2e}, misfit ratio {f:.3f}, {r}", flush=True)
+    for i, (q, f, r) in enumerate(stats):
+        assert q < 1e-2 and f < 0.5 and r["corr"] > 0.2, (i, q, f, r)
+        assert 0.4 < r["mass"] < 1.6 and r["contrast"] > 1.5 and r["offset"] < 0.25, (i, r)
+    assert np.median([r["contrast"] for _, _, r in stats]) > 2.0
+    assert np.median([r["offset"] for _, _, r in stats]) < 0.12
 
 
 def test_batch_rejects(dev):

@@ -232,6 +232,34 @@ class Subproblem:
         return int(self.stations.shape[0]) * self.grid.n_cells
 
 
+def recovery_stats(x: np.ndarray, p: Subproblem) -> dict:
+    """How well a recovered model x [N] (cell order of p.grid) finds p's
+    buried blocks, from the column masses c = sum over depth (gravity fixes
+    the horizontal distribution of mass; its depth is left to the prior):
+    corr, the correlation of recovered and true column masses; mass, the
+    recovered total excess mass over the true one; contrast, the mean
+    recovered column mass over the blocks' footprints against that outside
+    them (mean |c| outside, so that a model of alternating sign is not
+    rewarded); offset, the distance between the recovered and the true
+    mass centroids (of c clipped at 0) in units of the grid's width."""
+    g = p.grid
+    c = x.reshape(g.nz, g.ny, g.nx).sum(axis=0)
+    t = p.rho.reshape(g.nz, g.ny, g.nx).sum(axis=0)
+    inside = t > 0
+    cin = float(c[inside].mean())
+    cout = float(np.abs(c[~inside]).mean()) if (~inside).any() else 0.0
+    jj, ii = np.meshgrid(np.arange(g.ny), np.arange(g.nx), indexing="ij")
+
+    def centroid(w):
+        w = np.clip(w, 0.0, None)
+        return np.array([(w * ii).sum(), (w * jj).sum()]) / w.sum()
+
+    return {"corr": float(np.corrcoef(c.ravel(), t.ravel())[0, 1]),
+            "mass": float(c.sum() / t.sum()),
+            "contrast": cin / cout if cout > 0 else float("inf"),
+            "offset": float(np.linalg.norm(centroid(c) - centroid(t)) / g.nx)}
+
+
 def _logu(seed: int, stream: int, lo: float, hi: float) -> float:
     return float(np.exp(uniform(seed, stream, 1, np.log(lo), np.log(hi))[0]))
 
