@@ -112,12 +112,22 @@ __device__ __forceinline__ void pm_fwd_block(
 
     // Stage this slice's densities, thresholded and zero-padded to nxp. A
     // thread's element t = r nxp + i steps by 256 each pass; (r, i) follow it
-    // without a division per element.
+    // without a division per element. With row groups (G > 1) the tile is
+    // laid out [row group][column quad][field][row in group][4]: the F x G
+    // quads a wave reads for one column quad are contiguous, so that one
+    // address and constant offsets reach them all. Without groups it is
+    // [field][row][nxp].
     const int tile = nr_pad * nxp;   // one field's tile
+    const int nx4 = nxp >> 2;
     {
         const int dr = 256 / nxp, di = 256 - dr * nxp;
         int r = threadIdx.x / nxp, i = threadIdx.x - r * nxp;
         for (int t = threadIdx.x; t < tile; t += 256) {
+            int o = t;   // float offset of field 0's element
+            if constexpr (G > 1) {
+                const int rgl = r / G, g = r - rgl * G;
+                o = (((rgl * nx4 + (i >> 2)) * F) * G + g) * 4 + (i & 3);
+            }
 #pragma unroll
             for (int f = 0; f < F; ++f) {
                 T v = T(0);
@@ -125,7 +135,7 @@ __device__ __forceinline__ void pm_fwd_block(
                     v = rho[(size_t)f * field_stride + base + (size_t)r * nx + i];
                     if (!(fabs(v) > thr)) v = T(0);   // NaN or |rho| <= thr: skipped
                 }
-                lrho[f * tile + t] = v;
+                lrho[G > 1 ? o + f * G * 4 : f * tile + t] = v;
             }
             r += dr;
             i += di;
@@ -175,15 +185,14 @@ __device__ __forceinline__ void pm_fwd_block(
             }
 #pragma unroll
             for (int s = 0; s < S; ++s) xm[s] = X[s];
-            const int nx4 = nxp >> 2;
-            const V4* lrow = reinterpret_cast<const V4*>(lrho + rg * nxp);
+            const V4* lgrp = reinterpret_cast<const V4*>(lrho) + (rg / G) * nx4 * F * G;
             for (int i4 = 0; i4 < nx4; ++i4) {
                 V4 q[F][G];
+                const V4* lq = lgrp + i4 * (F * G);
 #pragma unroll
                 for (int f = 0; f < F; ++f)
 #pragma unroll
-                    for (int g = 0; g < G; ++g)
-                        q[f][g] = lrow[f * (tile >> 2) + g * nx4 + i4];   // wave-uniform
+                    for (int g = 0; g < G; ++g) q[f][g] = lq[f * G + g];   // wave-uniform
 #pragma unroll
                 for (int s = 0; s < S; ++s) {
                     const T a0 = xm[s], a1 = xm[s] - T(1), a2 = xm[s] - T(2), a3 = xm[s] - T(3);
