@@ -97,3 +97,25 @@ def test_lm_fit_on_the_device(dev, orc):
     assert np.array_equal(b1, truth)
     with pytest.raises(ValueError):
         model.fit(np.zeros(4), d, W.TFM_LM_SIGMA, W.GN_DAMPING)
+
+
+def test_lm_fit_scalar_parameters_only(dev):
+    """With no Moho basis the parameters are (vm, ro, dr) alone and the
+    Jacobian is the three exact columns of ztm_tfm_gravity_jvp: the fit
+    from a step in the scalars alone recovers them (the forward model is
+    piecewise linear in them), and a basis of the wrong width is refused."""
+    w = W.tfm_fm()
+    M = w.di * w.dj
+    model = lm.GravityLM(w, np.zeros((0, M)), dev)
+    assert model.n == 3
+    b0 = np.array([w.vm, w.ro, w.dr])
+    truth = b0 + W.TFM_LM_STEP[:3]
+    d = model.forward(truth)[0]
+    assert tuple(model.jacobian(b0).shape) == (M, 3)
+    b, Q, nb = model.fit(b0, d, W.TFM_LM_SIGMA[:3], W.GN_DAMPING, iters=6)
+    print(f"[lm] scalars only: misfit log {Q}, b {b}", flush=True)
+    assert all(q1 <= q0 for q0, q1 in zip(Q, Q[1:])), Q
+    assert Q[-1] < Q[0] * 1e-4, Q
+    assert np.max(np.abs(b - truth) / np.abs(W.TFM_LM_STEP[:3])) < 0.1, (b, truth)
+    with pytest.raises(ValueError):
+        lm.GravityLM(w, np.zeros((2, M - 1)), dev)
