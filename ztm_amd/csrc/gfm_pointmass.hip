@@ -958,6 +958,12 @@ int fwd_rows_per_slice(const PmShape& s, const PmTuning& t0, size_t elem, bool q
         rps = std::max<int64_t>(rps, 4);
         rps = std::min<int64_t>(rps, lds_rows);
         rps = std::min<int64_t>(rps, s.n_rows);
+        // the same number of row groups for each of the 4 waves: the nearest
+        // multiple of 4 G rows that LDS holds (28 rows of 4-row groups were
+        // 2, 2, 2 and 1 groups per wave)
+        const int64_t q = 4 * G;
+        if (rps > q && lds_rows >= q && s.n_rows > q)
+            rps = std::min<int64_t>((rps + q / 2) / q * q, lds_rows / q * q);
     }
     if (rps > G) rps = rps / G * G;   // whole row groups per slice where possible
     return (int)std::max<int64_t>(rps, 1);
@@ -1119,7 +1125,9 @@ int fields_rows_per_slice(const PmShape& s, int F) {
     const int nxp = (s.nx + 3) & ~3;
     const int G = fields_group(F);
     int rps = fwd_rows_per_slice(s, t, sizeof(float), false);
-    rps = std::min(rps, std::max(G, 32768 / (nxp * (int)sizeof(float) * F) / G * G));
+    const int cap = 32768 / (nxp * (int)sizeof(float) * F);   // rows of F tiles in LDS
+    const int q = cap >= 4 * G ? 4 * G : G;   // as many row groups for each wave where LDS allows
+    rps = std::min(rps, std::max(G, cap / q * q));
     return std::max(rps, 1);
 }
 }  // namespace
